@@ -163,3 +163,149 @@ void oref_solve6(const double *A, const double *b, double *x) {
 }
 
 }  // extern "C"
+
+// ---- the GPU step's semantics (gpu3d.hpp), f32 or f64 ----------------------------------------
+#include "hull3d.hpp"
+namespace {
+struct ModelHandle { int prec; gpuo::Model<float> *f; gpuo::Model<double> *d; };
+template <class T> gpuo::Model<T> *as(void *h);
+template <> gpuo::Model<float> *as<float>(void *h) { return ((ModelHandle *)h)->f; }
+template <> gpuo::Model<double> *as<double>(void *h) { return ((ModelHandle *)h)->d; }
+#define DISPATCH(h, call) do { if (((ModelHandle *)(h))->prec == 32) { auto *M = as<float>(h); call; } else { auto *M = as<double>(h); call; } } while (0)
+
+template <class T> void setBodies(gpuo::Model<T> *M, const double *r, int n) {
+  M->bodies.resize(n);
+  for (int i = 0; i < n; i++) {
+    const double *o = r + 40 * i;
+    gpuo::Body<T> &b = M->bodies[i];
+    auto v = [&](int k) { gpuo::V3<T> x; x.x = (T)o[k]; x.y = (T)o[k + 1]; x.z = (T)o[k + 2]; return x; };
+    auto q = [&](int k) { gpuo::Q4<T> x; x.x = (T)o[k]; x.y = (T)o[k + 1]; x.z = (T)o[k + 2]; x.w = (T)o[k + 3]; return x; };
+    b.pos = v(0); b.friction = (T)o[3]; b.rot = q(4); b.ipos = v(8); b.lifecycle = (uint32_t)o[11]; b.irot = q(12);
+    b.size = v(16); b.mass = (T)o[19]; b.moment = v(20); b.radius = (T)o[23]; b.inpos = v(24); b.stamp = (uint32_t)o[27];
+    b.inrot = q(28); b.vel = v(32); b.prevUp = (T)o[35]; b.angvel = v(36);
+    double sh = o[39];
+    if (sh >= 3) { b.shape = gpuo::SHAPE_HULL; b.hull = (uint32_t)(sh - 3 + 0.5); } else { b.shape = (int)sh; b.hull = 0; }
+  }
+  M->filterGroups.assign(n, 0xffffffffu); M->filterMask.assign(n, 0xffffffffu);
+  M->refPos.resize(n); M->refRot.resize(n); M->refSet.assign(n, false);
+  M->colorA.assign(n, gpuo::NO_COLOR); M->colorB.assign(n, gpuo::NO_COLOR);
+}
+template <class T> void getBodies(gpuo::Model<T> *M, double *r) {
+  for (size_t i = 0; i < M->bodies.size(); i++) {
+    double *o = r + 40 * i;
+    const gpuo::Body<T> &b = M->bodies[i];
+    auto v = [&](int k, const gpuo::V3<T> &x) { o[k] = x.x; o[k + 1] = x.y; o[k + 2] = x.z; };
+    auto q = [&](int k, const gpuo::Q4<T> &x) { o[k] = x.x; o[k + 1] = x.y; o[k + 2] = x.z; o[k + 3] = x.w; };
+    v(0, b.pos); o[3] = b.friction; q(4, b.rot); v(8, b.ipos); o[11] = b.lifecycle; q(12, b.irot);
+    v(16, b.size); o[19] = b.mass; v(20, b.moment); o[23] = b.radius; v(24, b.inpos); o[27] = b.stamp;
+    q(28, b.inrot); v(32, b.vel); o[35] = b.prevUp; v(36, b.angvel);
+    o[39] = b.shape == gpuo::SHAPE_HULL ? 3.0 + b.hull : (double)b.shape;
+  }
+}
+template <class T> void setJoints(gpuo::Model<T> *M, const double *r, const int *info, int n) {
+  M->joints.resize(n);
+  M->noCollide.erase(std::remove_if(M->noCollide.begin(), M->noCollide.end(), [](const typename gpuo::Model<T>::NC &e) { return e.slot != gpuo::IGNORE_SLOT; }), M->noCollide.end());
+  for (int j = 0; j < n; j++) {
+    const double *o = r + 40 * j;
+    gpuo::Joint<T> &k = M->joints[j];
+    auto v = [&](int q) { gpuo::V3<T> x; x.x = (T)o[q]; x.y = (T)o[q + 1]; x.z = (T)o[q + 2]; return x; };
+    auto fin = [](double x) { return (T)(x > gpuo::BIGD ? gpuo::BIGD : (x < -gpuo::BIGD ? -gpuo::BIGD : x)); };
+    k.penLin = v(0); k.stiffLin = fin(o[3]); k.penAng = v(4); k.stiffAng = fin(o[7]); k.lamLin = v(8); k.fracture = fin(o[11]);
+    k.lamAng = v(12); k.torqueArm = (T)o[15]; k.c0Lin = v(16); k.pad0 = 0; k.c0Ang = v(20); k.yield = fin(o[23]);
+    k.rA = v(24); k.springRest = (T)o[27]; k.rB = v(28); k.breakBend = fin(o[31]);
+    k.rest.x = (T)o[32]; k.rest.y = (T)o[33]; k.rest.z = (T)o[34]; k.rest.w = (T)o[35];
+    k.restStart.x = (T)o[36]; k.restStart.y = (T)o[37]; k.restStart.z = (T)o[38]; k.restStart.w = (T)o[39];
+    k.type = info[4 * j]; k.a = info[4 * j + 1]; k.b = info[4 * j + 2];
+    if (k.a >= 0) M->noCollide.push_back({(uint32_t)std::max(k.a, k.b), (uint32_t)std::min(k.a, k.b), (uint32_t)j});
+  }
+  std::stable_sort(M->noCollide.begin(), M->noCollide.end(), [](const typename gpuo::Model<T>::NC &x, const typename gpuo::Model<T>::NC &y) { return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo); });
+}
+template <class T> void getJoints(gpuo::Model<T> *M, double *r) {
+  for (size_t j = 0; j < M->joints.size(); j++) {
+    double *o = r + 40 * j;
+    const gpuo::Joint<T> &k = M->joints[j];
+    auto v = [&](int q, const gpuo::V3<T> &x) { o[q] = x.x; o[q + 1] = x.y; o[q + 2] = x.z; };
+    v(0, k.penLin); o[3] = k.stiffLin; v(4, k.penAng); o[7] = k.stiffAng; v(8, k.lamLin); o[11] = k.fracture;
+    v(12, k.lamAng); o[15] = k.torqueArm; v(16, k.c0Lin); o[19] = 0; v(20, k.c0Ang); o[23] = k.yield;
+    v(24, k.rA); o[27] = k.springRest; v(28, k.rB); o[31] = k.breakBend;
+    o[32] = k.rest.x; o[33] = k.rest.y; o[34] = k.rest.z; o[35] = k.rest.w;
+    o[36] = k.restStart.x; o[37] = k.restStart.y; o[38] = k.restStart.z; o[39] = k.restStart.w;
+  }
+}
+template <class T> void setContacts(std::vector<gpuo::ManifoldT<T>> &ms, std::vector<gpuo::Contact<T>> &cs, const double *man, int nm, const double *con, int nc) {
+  ms.resize(nm); cs.resize(nc);
+  for (int m = 0; m < nm; m++) {
+    const double *o = man + 8 * m;
+    gpuo::ManifoldT<T> &x = ms[m];
+    x.a = (uint32_t)o[0]; x.b = (uint32_t)o[1]; x.first = (uint32_t)o[2]; x.countStep = (uint32_t)o[3];
+    x.n.x = (T)o[4]; x.n.y = (T)o[5]; x.n.z = (T)o[6]; x.friction = (T)o[7];
+  }
+  for (int c = 0; c < nc; c++) {
+    const double *o = con + 16 * c;
+    gpuo::Contact<T> &k = cs[c];
+    for (int r = 0; r < 3; r++) { k.rA[r] = (T)o[r]; k.rB[r] = (T)o[4 + r]; k.pen[r] = (T)o[8 + r]; k.lam[r] = (T)o[12 + r]; }
+    k.key = (uint32_t)o[3]; k.c0x = (T)o[7]; k.c0y = (T)o[11]; k.c0z = (T)o[15];
+  }
+}
+template <class T> void getContacts(const std::vector<gpuo::ManifoldT<T>> &ms, const std::vector<gpuo::Contact<T>> &cs, double *man, double *con) {
+  for (size_t m = 0; m < ms.size(); m++) {
+    double *o = man + 8 * m;
+    const gpuo::ManifoldT<T> &x = ms[m];
+    o[0] = x.a; o[1] = x.b; o[2] = x.first; o[3] = x.countStep; o[4] = x.n.x; o[5] = x.n.y; o[6] = x.n.z; o[7] = x.friction;
+  }
+  for (size_t c = 0; c < cs.size(); c++) {
+    double *o = con + 16 * c;
+    const gpuo::Contact<T> &k = cs[c];
+    for (int r = 0; r < 3; r++) { o[r] = k.rA[r]; o[4 + r] = k.rB[r]; o[8 + r] = k.pen[r]; o[12 + r] = k.lam[r]; }
+    o[3] = k.key; o[7] = k.c0x; o[11] = k.c0y; o[15] = k.c0z;
+  }
+}
+}  // namespace
+
+extern "C" {
+void *omodel_create(int prec) {
+  ModelHandle *h = new ModelHandle{prec, nullptr, nullptr};
+  if (prec == 32) h->f = new gpuo::Model<float>(); else h->d = new gpuo::Model<double>();
+  return h;
+}
+void omodel_destroy(void *h) { ModelHandle *m = (ModelHandle *)h; delete m->f; delete m->d; delete m; }
+void omodel_set_params(void *h, const gpuo::Params *p) { DISPATCH(h, M->p = *p); }
+void omodel_get_params(void *h, gpuo::Params *p) { DISPATCH(h, *p = M->p); }
+void omodel_set_bodies(void *h, const double *rec, int n) { DISPATCH(h, setBodies(M, rec, n)); }
+void omodel_get_bodies(void *h, double *rec) { DISPATCH(h, getBodies(M, rec)); }
+void omodel_set_joints(void *h, const double *rec, const int *info, int n) { DISPATCH(h, setJoints(M, rec, info, n)); }
+void omodel_get_joints(void *h, double *rec) { DISPATCH(h, getJoints(M, rec)); }
+void omodel_add_ignore(void *h, int a, int b) {
+  DISPATCH(h, {
+    M->noCollide.push_back({(uint32_t)std::max(a, b), (uint32_t)std::min(a, b), gpuo::IGNORE_SLOT});
+    std::stable_sort(M->noCollide.begin(), M->noCollide.end(), [](const auto &x, const auto &y) { return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo); });
+  });
+}
+void omodel_set_hulls(void *h, const uint32_t *words, int n) { DISPATCH(h, M->hulls.assign(words, words + n)); }
+void omodel_set_filters(void *h, const uint32_t *groups, const uint32_t *mask) {
+  DISPATCH(h, { for (size_t i = 0; i < M->bodies.size(); i++) { M->filterGroups[i] = groups[i]; M->filterMask[i] = mask[i]; } });
+}
+/** Last step's manifolds and contacts (the warm-start source), as seedFrom writes them. */
+void omodel_set_prev(void *h, const double *man, int nm, const double *con, int nc) { DISPATCH(h, setContacts(M->manifolds, M->contacts, man, nm, con, nc)); }
+void omodel_set_fixed_colors(void *h, const uint32_t *colors) {
+  DISPATCH(h, { M->fixedColors = colors != nullptr; if (colors) M->fixed.assign(colors, colors + M->bodies.size()); });
+}
+void omodel_set_color_state(void *h, const uint32_t *state) { DISPATCH(h, M->colorA.assign(state, state + M->bodies.size())); }
+void omodel_set_step_count(void *h, uint32_t s) { DISPATCH(h, M->stepCount = s); }
+void omodel_step(void *h, int n, int reuseManifolds) { DISPATCH(h, for (int i = 0; i < n; i++) M->step(reuseManifolds != 0)); }
+/** The collision phase of one step only (pairs, manifolds, contacts), then adjacency and colour. */
+void omodel_collide(void *h, int reuseManifolds) {
+  DISPATCH(h, { M->stepCount++; M->p.step = M->stepCount; M->collision(reuseManifolds != 0); M->adjacency(); M->colour(); });
+}
+void omodel_solve(void *h) { DISPATCH(h, M->solve()); }
+/** counts: pairs, manifolds, contacts, numColors, clashes, adjacency entries, colour cap used */
+void omodel_counts(void *h, uint32_t *out) {
+  DISPATCH(h, { out[0] = M->pairs.size(); out[1] = M->manifolds.size(); out[2] = M->contacts.size(); out[3] = M->numColors; out[4] = M->clashes; out[5] = M->adjList.size(); out[6] = M->colorCapUsed; });
+}
+void omodel_get_pairs(void *h, uint32_t *out) { DISPATCH(h, for (size_t i = 0; i < M->pairs.size(); i++) { out[2 * i] = M->pairs[i].first; out[2 * i + 1] = M->pairs[i].second; }); }
+void omodel_get_contacts(void *h, double *man, double *con) { DISPATCH(h, getContacts(M->manifolds, M->contacts, man, con)); }
+void omodel_get_colors(void *h, uint32_t *state) { DISPATCH(h, for (size_t i = 0; i < M->bodies.size(); i++) state[i] = M->colorA[i]); }
+void omodel_get_adjacency(void *h, uint32_t *start, uint32_t *list) {
+  DISPATCH(h, { for (size_t i = 0; i < M->adjStart.size(); i++) start[i] = M->adjStart[i]; for (size_t i = 0; i < M->adjList.size(); i++) list[i] = M->adjList[i]; });
+}
+}  // extern "C"

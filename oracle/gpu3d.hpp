@@ -1,1 +1,1015 @@
+// TEST INFRASTRUCTURE ONLY — the oracle's restatement of the WebGPU step's semantics.
+//
+// The reference's GPU step (GpuSolver3D.step, src/avbd3d/gpu/solver.ts:2112-2293) differs from
+// its CPU solver in documented ways (colour-parallel Gauss-Seidel, grid broadphase predicate,
+// sphere/hull shapes, pivot floor, warm-start heuristics). This file restates those semantics
+// from the WGSL (src/avbd3d/gpu/wgsl-collision.ts, wgsl-solve.ts, wgsl-hull.ts, layout.ts
+// PRELUDE_3D, src/avbd2d/gpu/wgsl-topology.ts) on the CPU, templated on the float type: in f32
+// it is the bit-exactness reference for pairs, contacts and colourings; in f64 it is the
+// finite-difference reference for the adjoint. Every operation is written in the order the
+// kernels evaluate it; build with -ffp-contract=off.
+//
+// What is deliberately canonical here and not in the reference: pairs and manifolds are kept
+// sorted by (a, b) (the reference's order depends on atomics; nothing numeric depends on it,
+// since adjacency lists are sorted before the solve).
 #pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <unordered_map>
+
+namespace gpuo {
+
+// layout.ts / avbd2d layout.ts constants
+static const uint32_t STICK_BIT = 0x80000000u, SPHERE_FEATURE = 3u << 24;
+static const uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
+                      FLAG_MASS_PENALTY = 16, FLAG_REUSE_MANIFOLDS = 64;
+static const int SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_HULL = 3;
+static const uint32_t NO_COLOR = 255, PENDING = 256, MAX_COLORS = 64;
+static const uint32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, IGNORE_SLOT = 0xffffffffu;
+static const double BIGD = 3.0e38, HARDD = 1e30;
+
+static inline uint32_t hash32(uint32_t x) {  // lowbias32 (avbd2d/gpu/layout.ts:107-115)
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return x;
+}
+
+template <class T> struct V3 { T x, y, z; T &operator[](int i) { return (&x)[i]; } T operator[](int i) const { return (&x)[i]; } };
+template <class T> struct Q4 { T x, y, z, w; };
+
+template <class T> struct Ops {
+  typedef V3<T> V; typedef Q4<T> Q;
+  static V v(T x, T y, T z) { V r; r.x = x; r.y = y; r.z = z; return r; }
+  static Q q(T x, T y, T z, T w) { Q r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }
+  static V add(V a, V b) { return v(a.x + b.x, a.y + b.y, a.z + b.z); }
+  static V sub(V a, V b) { return v(a.x - b.x, a.y - b.y, a.z - b.z); }
+  static V mul(V a, T s) { return v(a.x * s, a.y * s, a.z * s); }
+  static V mulv(V a, V b) { return v(a.x * b.x, a.y * b.y, a.z * b.z); }
+  static V divs(V a, T s) { return v(a.x / s, a.y / s, a.z / s); }
+  static V neg(V a) { return v(-a.x, -a.y, -a.z); }
+  static V vabs(V a) { return v(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
+  static T dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+  static T length(V a) { return std::sqrt(dot(a, a)); }
+  static V normalize(V a) { return divs(a, length(a)); }
+  static V cross(V a, V b) { return v(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+  static T mn(T a, T b) { return a < b ? a : b; }
+  static T mx(T a, T b) { return a > b ? a : b; }
+  static T clampt(T x, T lo, T hi) { return mn(mx(x, lo), hi); }
+  static T dot4(Q a, Q b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+  // PRELUDE_3D helpers (layout.ts:307-340)
+  static Q qmul(Q a, Q b) {
+    return q(a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+             a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+  }
+  static Q qconj(Q a) { return q(-a.x, -a.y, -a.z, a.w); }
+  static V qsub(Q a, Q b) {
+    T d = dot4(b, b);
+    Q inv = q(-b.x / d, -b.y / d, -b.z / d, b.w / d);
+    Q r = qmul(a, inv);
+    return v(r.x * T(2), r.y * T(2), r.z * T(2));
+  }
+  static Q qnormalize(Q a) { T l = std::sqrt(dot4(a, a)); return q(a.x / l, a.y / l, a.z / l, a.w / l); }
+  static Q qadd(Q a, V w) {
+    Q m = qmul(q(w.x, w.y, w.z, T(0)), a);
+    return qnormalize(q(a.x + m.x * T(0.5), a.y + m.y * T(0.5), a.z + m.z * T(0.5), a.w + m.w * T(0.5)));
+  }
+  static V qrotate(Q r, V a) {
+    V u = v(r.x, r.y, r.z);
+    V t = mul(cross(u, a), T(2));
+    return add(add(a, mul(t, r.w)), cross(u, t));
+  }
+  static void orthonormal(V n, V basis[3]) {
+    V t1 = std::fabs(n.x) > std::fabs(n.z) ? v(-n.y, n.x, T(0)) : v(T(0), -n.z, n.y);
+    t1 = normalize(t1);
+    basis[0] = n; basis[1] = t1; basis[2] = cross(n, t1);
+  }
+};
+
+/** Body record fields (layout.ts:219-230) in the model's float type, plus the bitcast words. */
+template <class T> struct Body {
+  V3<T> pos; T friction; Q4<T> rot; V3<T> ipos; uint32_t lifecycle; Q4<T> irot;
+  V3<T> size; T mass; V3<T> moment; T radius; V3<T> inpos; uint32_t stamp; Q4<T> inrot;
+  V3<T> vel; T prevUp; V3<T> angvel; int shape; uint32_t hull;  // hull: header offset (vec4 units) when shape == HULL
+};
+template <class T> struct Joint {  // layout.ts:232-243 (+ info)
+  V3<T> penLin; T stiffLin; V3<T> penAng; T stiffAng; V3<T> lamLin; T fracture; V3<T> lamAng; T torqueArm;
+  V3<T> c0Lin; T pad0; V3<T> c0Ang; T yield; V3<T> rA; T springRest; V3<T> rB; T breakBend; Q4<T> rest; Q4<T> restStart;
+  int type, a, b;
+};
+struct Manifold { uint32_t a, b, first, countStep; float nx, ny, nz, friction; };
+template <class T> struct ManifoldT { uint32_t a, b, first, countStep; V3<T> n; T friction; };
+template <class T> struct Contact { V3<T> rA; uint32_t key; V3<T> rB; T c0x; V3<T> pen; T c0y; V3<T> lam; T c0z; };
+
+struct Params {  // the subset of Params (layout.ts:265-305) the semantics depend on
+  double dt, gravity, betaLin, betaAng, gamma, alpha, contactMargin, up[3];
+  uint32_t flags, iterations, colorCap, colorRounds, step;
+  uint32_t lanesRule[3];  // colour sizes from which 1, 2, 4 lanes per body are used (else 8)
+};
+
+template <class T> struct Model {
+  typedef Ops<T> O; typedef V3<T> V; typedef Q4<T> Q;
+  Params p;
+  std::vector<Body<T>> bodies;
+  std::vector<Joint<T>> joints;
+  std::vector<uint32_t> filterGroups, filterMask;
+  std::vector<uint32_t> hulls;  // hull buffer words (vec4u units * 4), wgsl-hull.ts header format
+  struct NC { uint32_t hi, lo, slot; };
+  std::vector<NC> noCollide;  // sorted by (hi, lo)
+  std::vector<V> refPos; std::vector<Q> refRot; std::vector<bool> refSet;
+  std::vector<std::pair<uint32_t, uint32_t>> pairs;
+  std::vector<ManifoldT<T>> manifolds, prevManifolds;
+  std::vector<Contact<T>> contacts, prevContacts;
+  std::vector<uint32_t> adjStart, adjList;
+  std::vector<uint32_t> colorA, colorB;  // colour state ping-pong (wgsl-topology.ts)
+  std::vector<uint32_t> bucketStart, bucketBodies;  // colourStart[65], bodies by colour
+  uint32_t clashMask[2] = {0, 0};
+  uint32_t numColors = 0, clashes = 0, colorCapUsed = 0;
+  bool fixedColors = false;
+  std::vector<uint32_t> fixed;
+  uint32_t stepCount = 0;
+
+  T dtT() const { return T(p.dt); }
+  bool dynamic(int i) const { return i >= 0 && bodies[i].mass > T(0); }
+
+  // ---- broadphase (wgsl-collision.ts:150-234): the exact pair predicate ----
+  struct Shape { V ax[3]; V h; bool sphere; };
+  Shape shapeOf(uint32_t i) const {
+    const Body<T> &b = bodies[i];
+    Shape s;
+    s.h = O::mul(b.size, T(0.5));
+    s.sphere = b.shape == SHAPE_SPHERE;
+    s.ax[0] = O::qrotate(b.rot, O::v(T(1), T(0), T(0)));
+    s.ax[1] = O::qrotate(b.rot, O::v(T(0), T(1), T(0)));
+    s.ax[2] = O::qrotate(b.rot, O::v(T(0), T(0), T(1)));
+    return s;
+  }
+  static T extent(const Shape &s, V n) {
+    if (s.sphere) return s.h.x;
+    return s.h.x * std::fabs(O::dot(n, s.ax[0])) + s.h.y * std::fabs(O::dot(n, s.ax[1])) + s.h.z * std::fabs(O::dot(n, s.ax[2]));
+  }
+  static V aabbHalf(const Shape &s) {
+    if (s.sphere) return O::v(s.h.x, s.h.x, s.h.x);
+    return O::add(O::add(O::mul(O::vabs(s.ax[0]), s.h.x), O::mul(O::vabs(s.ax[1]), s.h.y)), O::mul(O::vabs(s.ax[2]), s.h.z));
+  }
+  static bool faceSeparated(const Shape &A, const Shape &B, V d) {
+    for (int k = 0; k < 3; k++) {
+      if (std::fabs(O::dot(d, A.ax[k])) - extent(A, A.ax[k]) - extent(B, A.ax[k]) > T(1e-3)) return true;
+      if (std::fabs(O::dot(d, B.ax[k])) - extent(A, B.ax[k]) - extent(B, B.ax[k]) > T(1e-3)) return true;
+    }
+    return false;
+  }
+  bool ignored(uint32_t hi, uint32_t lo) const {
+    auto it = std::lower_bound(noCollide.begin(), noCollide.end(), NC{hi, lo, 0},
+                               [](const NC &x, const NC &y) { return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo); });
+    for (; it != noCollide.end() && it->hi == hi && it->lo == lo; ++it) {
+      if (it->slot == IGNORE_SLOT) return true;
+      const Joint<T> &j = joints[it->slot];
+      if (j.stiffLin != T(0) || j.stiffAng != T(0)) return true;
+    }
+    return false;
+  }
+  /** testPair(P, j) with P the probe of body i. */
+  bool testPair(uint32_t i, uint32_t j) const {
+    if (!dynamic(i) && !dynamic(j)) return false;
+    if ((filterGroups[i] & filterMask[j]) == 0 || (filterGroups[j] & filterMask[i]) == 0) return false;
+    V d = O::sub(bodies[i].pos, bodies[j].pos);
+    T r = bodies[i].radius + bodies[j].radius;
+    if (O::dot(d, d) > r * r) return false;
+    Shape A = shapeOf(i), B = shapeOf(j);
+    V ha = aabbHalf(A), hb = aabbHalf(B);
+    V lim = O::add(O::add(ha, hb), O::v(T(1e-4), T(1e-4), T(1e-4)));
+    V ad = O::vabs(d);
+    if (ad.x > lim.x || ad.y > lim.y || ad.z > lim.z) return false;
+    if (faceSeparated(A, B, d)) return false;
+    if (ignored(std::max(i, j), std::min(i, j))) return false;
+    return true;
+  }
+  /** The pair set of the predicate over every (i, j), i > j, sorted by (a, b); a grid of
+   *  bounding-sphere candidates keeps it fast, and candidates only ever pass the exact test. */
+  void broadphase() {
+    pairs.clear();
+    const uint32_t n = (uint32_t)bodies.size();
+    if (n < 2) return;
+    std::vector<double> rs(n);
+    for (uint32_t i = 0; i < n; i++) rs[i] = (double)bodies[i].radius;
+    std::vector<double> sorted = rs;
+    std::sort(sorted.begin(), sorted.end());
+    double cut = sorted[std::min<size_t>(n - 1, (size_t)(0.99 * (n - 1)))] * 2.0;
+    std::vector<uint32_t> large;
+    double cell = 0;
+    for (uint32_t i = 0; i < n; i++) { if (rs[i] > cut) large.push_back(i); else cell = std::max(cell, 2 * rs[i]); }
+    cell = std::max(cell, 1e-3) * (1 + 1e-6);
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    auto key = [](int64_t x, int64_t y, int64_t z) { return (uint64_t)((x & 0x1fffff) | ((y & 0x1fffff) << 21) | ((z & 0x1fffff) << 42)); };
+    std::vector<int64_t> cx(n), cy(n), cz(n);
+    for (uint32_t i = 0; i < n; i++) {
+      if (rs[i] > cut) continue;
+      cx[i] = (int64_t)std::floor((double)bodies[i].pos.x / cell); cy[i] = (int64_t)std::floor((double)bodies[i].pos.y / cell); cz[i] = (int64_t)std::floor((double)bodies[i].pos.z / cell);
+      grid[key(cx[i], cy[i], cz[i])].push_back(i);
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> out;
+    for (uint32_t i = 0; i < n; i++) {
+      if (rs[i] > cut) continue;
+      for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
+        auto it = grid.find(key(cx[i] + dx, cy[i] + dy, cz[i] + dz));
+        if (it == grid.end()) continue;
+        for (uint32_t j : it->second) if (j < i && testPair(i, j)) out.push_back({i, j});
+      }
+    }
+    for (uint32_t l : large)
+      for (uint32_t j = 0; j < n; j++) {
+        if (j == l) continue;
+        bool jl = rs[j] > cut;
+        if (jl && j > l) continue;  // large-large once
+        uint32_t a = std::max(l, j), b = std::min(l, j);
+        if (testPair(a, b)) out.push_back({a, b});
+      }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    pairs = out;
+  }
+
+  // ---- narrowphase (wgsl-collision.ts:383-683) ----
+  struct Box { V c, h, ax[3]; };
+  struct Sat { uint32_t kind, ia, ib; T sep; V n; bool valid; };
+  struct Found { uint32_t feature[8]; V xA[8], xB[8]; uint32_t count; };
+  Box makeBox(uint32_t i) const {
+    Box b;
+    b.c = bodies[i].pos;
+    b.h = O::mul(bodies[i].size, T(0.5));
+    b.ax[0] = O::qrotate(bodies[i].rot, O::v(T(1), T(0), T(0)));
+    b.ax[1] = O::qrotate(bodies[i].rot, O::v(T(0), T(1), T(0)));
+    b.ax[2] = O::qrotate(bodies[i].rot, O::v(T(0), T(0), T(1)));
+    return b;
+  }
+  static T projRadius(const Box &b, V n) {
+    return b.h.x * std::fabs(O::dot(n, b.ax[0])) + b.h.y * std::fabs(O::dot(n, b.ax[1])) + b.h.z * std::fabs(O::dot(n, b.ax[2]));
+  }
+  static bool testAxis(const Box &A, const Box &B, V delta, V axis, uint32_t kind, uint32_t ia, uint32_t ib, Sat &best) {
+    T lenSq = O::dot(axis, axis);
+    if (lenSq < T(1.0e-6)) return true;
+    V n = O::mul(axis, T(1) / std::sqrt(lenSq));
+    if (O::dot(n, delta) < T(0)) n = O::neg(n);
+    T sep = std::fabs(O::dot(delta, n)) - (projRadius(A, n) + projRadius(B, n));
+    if (sep > T(0)) return false;
+    if (!best.valid || sep > best.sep) best = Sat{kind, ia, ib, sep, n, true};
+    return true;
+  }
+  static V support(const Box &b, V dir) {
+    T sx = O::dot(dir, b.ax[0]) >= T(0) ? T(1) : T(-1), sy = O::dot(dir, b.ax[1]) >= T(0) ? T(1) : T(-1), sz = O::dot(dir, b.ax[2]) >= T(0) ? T(1) : T(-1);
+    return O::add(O::add(O::add(b.c, O::mul(b.ax[0], b.h.x * sx)), O::mul(b.ax[1], b.h.y * sy)), O::mul(b.ax[2], b.h.z * sz));
+  }
+  static void addFound(Found &f, V xA, V xB, uint32_t feature) {
+    V mid = O::mul(O::add(xA, xB), T(0.5));
+    for (uint32_t i = 0; i < f.count; i++) {
+      V d = O::sub(mid, O::mul(O::add(f.xA[i], f.xB[i]), T(0.5)));
+      if (O::dot(d, d) < T(1.0e-6)) return;
+    }
+    if (f.count >= 8) return;
+    f.feature[f.count] = feature; f.xA[f.count] = xA; f.xB[f.count] = xB; f.count++;
+  }
+  static uint32_t clipPlane(const V *src, uint32_t n, V *dst, V pn, T offset) {
+    if (n == 0) return 0;
+    uint32_t count = 0;
+    V a = src[n - 1];
+    T da = O::dot(pn, a) - offset;
+    for (uint32_t i = 0; i < n; i++) {
+      V b = src[i];
+      T db = O::dot(pn, b) - offset;
+      bool aIn = da <= T(1.0e-5), bIn = db <= T(1.0e-5);
+      if (aIn != bIn) {
+        T t = T(0), denom = da - db;
+        if (std::fabs(denom) > T(1.0e-6)) t = O::clampt(da / denom, T(0), T(1));
+        if (count < 8) dst[count++] = O::add(a, O::mul(O::sub(b, a), t));
+      }
+      if (bIn && count < 8) dst[count++] = b;
+      a = b; da = db;
+    }
+    return count;
+  }
+  static uint32_t faceU(uint32_t k) { return k == 0 ? 1 : 0; }
+  static uint32_t faceV(uint32_t k) { return k != 2 ? 2 : 1; }
+  static Found faceManifold(const Box &A, const Box &B, bool refIsA, uint32_t refAxis, V nAB) {
+    Found found; found.count = 0;
+    const Box &R = refIsA ? A : B;
+    const Box &I = refIsA ? B : A;
+    V outward = refIsA ? nAB : O::neg(nAB);
+    T rs = O::dot(outward, R.ax[refAxis]) >= T(0) ? T(1) : T(-1);
+    V rn = O::mul(R.ax[refAxis], rs);
+    V rc = O::add(R.c, O::mul(rn, R.h[refAxis]));
+    V ru = R.ax[faceU(refAxis)], rv = R.ax[faceV(refAxis)];
+    T eu = R.h[faceU(refAxis)], ev = R.h[faceV(refAxis)];
+    uint32_t inc = 0; T bestDot = T(-1);
+    for (uint32_t k = 0; k < 3; k++) { T d = std::fabs(O::dot(I.ax[k], rn)); if (d > bestDot) { bestDot = d; inc = k; } }
+    T is = O::dot(I.ax[inc], rn) > T(0) ? T(-1) : T(1);
+    V ic = O::add(I.c, O::mul(I.ax[inc], is * I.h[inc]));
+    V iu = O::mul(I.ax[faceU(inc)], I.h[faceU(inc)]), iv = O::mul(I.ax[faceV(inc)], I.h[faceV(inc)]);
+    V poly[8], tmp[8];
+    poly[0] = O::add(O::add(ic, iu), iv);
+    poly[1] = O::add(O::sub(ic, iu), iv);
+    poly[2] = O::sub(O::sub(ic, iu), iv);
+    poly[3] = O::sub(O::add(ic, iu), iv);
+    uint32_t n = 4;
+    n = clipPlane(poly, n, tmp, ru, O::dot(ru, rc) + eu);
+    n = clipPlane(tmp, n, poly, O::neg(ru), O::dot(O::neg(ru), rc) + eu);
+    n = clipPlane(poly, n, tmp, rv, O::dot(rv, rc) + ev);
+    n = clipPlane(tmp, n, poly, O::neg(rv), O::dot(O::neg(rv), rc) + ev);
+    if (n == 0) return found;
+    uint32_t prefix = ((refIsA ? 0u : 1u) << 24) | (refAxis << 16) | (inc << 8);
+    for (uint32_t i = 0; i < n && found.count < 8; i++) {
+      V pt = poly[i];
+      T dist = O::dot(O::sub(pt, rc), rn);
+      if (dist > T(1.0e-5)) continue;
+      V onRef = O::sub(pt, O::mul(rn, dist));
+      addFound(found, refIsA ? onRef : pt, refIsA ? pt : onRef, prefix | i);
+    }
+    if (found.count == 0) addFound(found, support(A, nAB), support(B, O::neg(nAB)), prefix);
+    return found;
+  }
+  static void supportEdge(const Box &b, uint32_t k, V dir, V &e0, V &e1) {
+    uint32_t k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    T s1 = O::dot(dir, b.ax[k1]) >= T(0) ? T(1) : T(-1), s2 = O::dot(dir, b.ax[k2]) >= T(0) ? T(1) : T(-1);
+    V c = O::add(O::add(b.c, O::mul(b.ax[k1], b.h[k1] * s1)), O::mul(b.ax[k2], b.h[k2] * s2));
+    V e = O::mul(b.ax[k], b.h[k]);
+    e0 = O::sub(c, e); e1 = O::add(c, e);
+  }
+  static Found edgeContact(const Box &A, const Box &B, uint32_t ia, uint32_t ib, V nAB) {
+    Found found; found.count = 0;
+    V a0, a1, b0, b1;
+    supportEdge(A, ia, nAB, a0, a1);
+    supportEdge(B, ib, O::neg(nAB), b0, b1);
+    V p0 = a0, q0 = b0, d1 = O::sub(a1, p0), d2 = O::sub(b1, q0), r = O::sub(p0, q0);
+    T a = O::dot(d1, d1), e = O::dot(d2, d2), f = O::dot(d2, r), s = T(0), t = T(0);
+    const T EPS = T(1.0e-6);
+    if (a <= EPS && e <= EPS) {
+    } else if (a <= EPS) {
+      t = O::clampt(f / e, T(0), T(1));
+    } else {
+      T c = O::dot(d1, r);
+      if (e <= EPS) s = O::clampt(-c / a, T(0), T(1));
+      else {
+        T b = O::dot(d1, d2), denom = a * e - b * b;
+        if (std::fabs(denom) > EPS) s = O::clampt((b * f - c * e) / denom, T(0), T(1));
+        t = (b * s + f) / e;
+        if (t < T(0)) { t = T(0); s = O::clampt(-c / a, T(0), T(1)); }
+        else if (t > T(1)) { t = T(1); s = O::clampt((b - c) / a, T(0), T(1)); }
+      }
+    }
+    uint32_t key = (2u << 24) | (ia << 8) | ib;
+    addFound(found, O::add(p0, O::mul(d1, s)), O::add(q0, O::mul(d2, t)), key);
+    if (found.count == 0) addFound(found, support(A, nAB), support(B, O::neg(nAB)), key);
+    return found;
+  }
+  Found collideBoxes(const Box &A, const Box &B, Sat &sat) const {
+    Found none; none.count = 0;
+    V delta = O::sub(B.c, A.c);
+    Sat face{0, 0, 0, T(0), O::v(T(0), T(0), T(0)), false}, edge = face;
+    for (uint32_t i = 0; i < 3; i++) if (!testAxis(A, B, delta, A.ax[i], 0, i, 0, face)) return none;
+    for (uint32_t i = 0; i < 3; i++) if (!testAxis(A, B, delta, B.ax[i], 1, 0, i, face)) return none;
+    for (uint32_t i = 0; i < 3; i++)
+      for (uint32_t j = 0; j < 3; j++)
+        if (!testAxis(A, B, delta, O::cross(A.ax[i], B.ax[j]), 2, i, j, edge)) return none;
+    if (!face.valid) return none;
+    Sat best = face;
+    if (edge.valid) {
+      bool edgeWins = T(0.95) * edge.sep > face.sep + T(0.01);
+      if (p.flags & FLAG_FACE_BIAS) edgeWins = edge.sep > T(0.95) * face.sep + T(0.01);
+      if (edgeWins) best = edge;
+    }
+    sat = best;
+    if (best.kind == 2) return edgeContact(A, B, best.ia, best.ib, best.n);
+    return faceManifold(A, B, best.kind == 0, best.kind == 0 ? best.ia : best.ib, best.n);
+  }
+  static Found sphereSphere(const Box &A, const Box &B, Sat &sat) {
+    Found found; found.count = 0;
+    V d = O::sub(B.c, A.c);
+    T dist = O::length(d);
+    if (dist > A.h.x + B.h.x) return found;
+    V n = O::v(T(0), T(0), T(1));
+    if (dist > T(0)) n = O::divs(d, dist);
+    sat.n = n;
+    addFound(found, O::add(A.c, O::mul(n, A.h.x)), O::sub(B.c, O::mul(n, B.h.x)), SPHERE_FEATURE);
+    return found;
+  }
+  struct SphereHit { bool hit; V onSphere, onBox, n; };
+  static SphereHit sphereBox(const Box &S, const Box &X, Q qX) {
+    SphereHit out; out.hit = false;
+    V pl = O::qrotate(O::qconj(qX), O::sub(S.c, X.c));
+    V ql = O::v(O::clampt(pl.x, -X.h.x, X.h.x), O::clampt(pl.y, -X.h.y, X.h.y), O::clampt(pl.z, -X.h.z, X.h.z));
+    T r = S.h.x;
+    V nLocal, onBox = ql;
+    if (pl.x != ql.x || pl.y != ql.y || pl.z != ql.z) {
+      V d = O::sub(pl, ql);
+      T dist = O::length(d);
+      if (dist > r) return out;
+      nLocal = O::divs(d, dist);
+    } else {
+      V depth = O::sub(X.h, O::vabs(pl));
+      int k = 0;
+      if (depth.y < depth[k]) k = 1;
+      if (depth.z < depth[k]) k = 2;
+      T s = pl[k] >= T(0) ? T(1) : T(-1);
+      nLocal = O::v(T(0), T(0), T(0));
+      nLocal[k] = s;
+      onBox[k] = s * X.h[k];
+    }
+    V n = O::qrotate(qX, nLocal);
+    out.hit = true;
+    out.n = O::neg(n);
+    out.onBox = O::add(X.c, O::qrotate(qX, onBox));
+    out.onSphere = O::sub(S.c, O::mul(n, r));
+    return out;
+  }
+  // Hull shapes: the oracle's hull narrowphase (hull3d.hpp) when compiled in.
+  Found collideHull(uint32_t a, uint32_t b, bool sphereA, bool sphereB, Sat &sat) const;
+
+  T contactPenaltyMin(uint32_t a, uint32_t b) const {
+    if (!(p.flags & FLAG_MASS_PENALTY)) return T(1.0);
+    T mA = bodies[a].mass <= T(0) ? T(3.4e38) : bodies[a].mass;
+    T mB = bodies[b].mass <= T(0) ? T(3.4e38) : bodies[b].mass;
+    T dt = dtT();
+    return O::clampt(O::mn(mA, mB) / (dt * dt), T(1.0), T(1e10));
+  }
+  std::unordered_map<uint64_t, uint32_t> prevIndex;  // (a, b) -> previous manifold (hashInsert/hashFind)
+  static uint64_t pkey(uint32_t a, uint32_t b) { return ((uint64_t)a << 32) | b; }
+  int findPrev(uint32_t a, uint32_t b) const {
+    auto it = prevIndex.find(pkey(a, b));
+    return it == prevIndex.end() ? -1 : (int)it->second;
+  }
+  void copyContacts(const ManifoldT<T> &prev) {
+    uint32_t a = prev.a, b = prev.b, count = prev.countStep & 15u;
+    ManifoldT<T> m = prev;
+    m.first = (uint32_t)contacts.size();
+    V pA = bodies[a].pos, pB = bodies[b].pos;
+    Q qA = bodies[a].rot, qB = bodies[b].rot;
+    V basis[3];
+    O::orthonormal(prev.n, basis);
+    T alpha = T(p.alpha), gamma = T(p.gamma), margin = T(p.contactMargin), pmin = contactPenaltyMin(a, b);
+    for (uint32_t i = 0; i < count; i++) {
+      Contact<T> k = prevContacts[prev.first + i];
+      V d = O::sub(O::add(O::qrotate(qA, k.rA), pA), O::add(O::qrotate(qB, k.rB), pB));
+      k.c0x = O::dot(basis[0], d) + margin;
+      k.c0y = O::dot(basis[1], d);
+      k.c0z = O::dot(basis[2], d);
+      k.lam = O::mul(O::mul(k.lam, alpha), gamma);
+      for (int r = 0; r < 3; r++) k.pen[r] = O::clampt(k.pen[r] * gamma, T(1.0), T(1e10));
+      k.pen.x = O::mx(k.pen.x, pmin);
+      contacts.push_back(k);
+    }
+    manifolds.push_back(m);
+  }
+  bool reuseContacts(uint32_t a, uint32_t b) {
+    int pm = findPrev(a, b);
+    if (pm < 0) return false;
+    const ManifoldT<T> &prev = prevManifolds[pm];
+    uint32_t count = prev.countStep & 15u, generated = prev.countStep >> 4;
+    uint32_t moved = std::max(bodies[a].stamp, bodies[b].stamp);
+    if (count == 0 || generated < moved) return false;
+    copyContacts(prev);
+    return true;
+  }
+  void narrowphase(uint32_t a, uint32_t b) {
+    if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(a, b)) return;
+    Box A = makeBox(a), B = makeBox(b);
+    Sat sat; sat.n = O::v(T(0), T(0), T(0));
+    Found found; found.count = 0;
+    bool sphereA = bodies[a].shape == SHAPE_SPHERE, sphereB = bodies[b].shape == SHAPE_SPHERE;
+    bool hullA = bodies[a].shape == SHAPE_HULL, hullB = bodies[b].shape == SHAPE_HULL;
+    if (sphereA && sphereB) found = sphereSphere(A, B, sat);
+    else if (hullA || hullB) found = collideHull(a, b, sphereA, sphereB, sat);
+    else if (sphereA || sphereB) {
+      if (sphereA) {
+        SphereHit h = sphereBox(A, B, bodies[b].rot);
+        sat.n = h.n;
+        if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+      } else {
+        SphereHit h = sphereBox(B, A, bodies[a].rot);
+        sat.n = O::neg(h.n);
+        if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+      }
+    } else found = collideBoxes(A, B, sat);
+    if (found.count == 0) return;
+    ManifoldT<T> m;
+    m.a = a; m.b = b; m.first = (uint32_t)contacts.size(); m.countStep = found.count | (p.step << 4);
+    V n = O::neg(sat.n);
+    m.n = n;
+    m.friction = std::sqrt(bodies[a].friction * bodies[b].friction);
+    Q qA = bodies[a].rot, qB = bodies[b].rot;
+    V basis[3];
+    O::orthonormal(n, basis);
+    bool anySphere = sphereA || sphereB;
+    uint32_t prevFirst = 0, prevCount = 0;
+    int pm = findPrev(a, b);
+    if (pm >= 0) { prevFirst = prevManifolds[pm].first; prevCount = prevManifolds[pm].countStep & 15u; }
+    bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
+    T minSide = O::mn(O::mn(O::mn(A.h.x, A.h.y), A.h.z), O::mn(O::mn(B.h.x, B.h.y), B.h.z)) * T(2.0);
+    T penaltyMin = contactPenaltyMin(a, b);
+    T alpha = T(p.alpha), gamma = T(p.gamma), margin = T(p.contactMargin);
+    for (uint32_t i = 0; i < found.count; i++) {
+      V rA = O::qrotate(O::qconj(qA), O::sub(found.xA[i], A.c));
+      V rB = O::qrotate(O::qconj(qB), O::sub(found.xB[i], B.c));
+      V pen = O::v(T(0), T(0), T(0)), lam = pen;
+      uint32_t stick = 0;
+      int j = -1;
+      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+        if ((prevContacts[c].key & ~STICK_BIT) == found.feature[i]) { j = (int)c; break; }
+      if (j < 0 && matchNearest) {
+        T bestDist = T(0.05) * minSide;
+        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+          T dist = O::length(O::sub(prevContacts[c].rA, rA));
+          if (dist <= bestDist) { bestDist = dist; j = (int)c; }
+        }
+      }
+      if (j >= 0) {
+        const Contact<T> &pc = prevContacts[j];
+        pen = pc.pen; lam = pc.lam; stick = pc.key & STICK_BIT;
+        if (stick != 0 && !anySphere) { rA = pc.rA; rB = pc.rB; }
+      }
+      V d = O::sub(O::add(O::qrotate(qA, rA), A.c), O::add(O::qrotate(qB, rB), B.c));
+      Contact<T> k;
+      k.rA = rA; k.rB = rB;
+      k.key = found.feature[i] | stick;
+      k.c0x = O::dot(basis[0], d) + margin;
+      k.c0y = O::dot(basis[1], d);
+      k.c0z = O::dot(basis[2], d);
+      k.lam = O::mul(O::mul(lam, alpha), gamma);
+      for (int r = 0; r < 3; r++) k.pen[r] = O::clampt(pen[r] * gamma, T(1.0), T(1e10));
+      k.pen.x = O::mx(k.pen.x, penaltyMin);
+      contacts.push_back(k);
+    }
+    manifolds.push_back(m);
+  }
+
+  /** beginFrame + updateRefs + (hashInsert, broadphase, narrowphase) or reuseManifolds. */
+  void collision(bool reuseManifolds) {
+    prevManifolds.swap(manifolds); prevContacts.swap(contacts);
+    manifolds.clear(); contacts.clear();
+    if (p.flags & FLAG_REUSE_CONTACTS) {  // updateRefs (wgsl-collision.ts:34-48)
+      for (size_t i = 0; i < bodies.size(); i++) {
+        Body<T> &b = bodies[i];
+        bool moved = !refSet[i] || O::length(O::sub(b.pos, refPos[i])) > T(0.002) || O::length(O::qsub(b.rot, refRot[i])) > T(0.002);
+        if (moved) { refPos[i] = b.pos; refRot[i] = b.rot; refSet[i] = true; b.stamp = p.step; }
+      }
+    }
+    prevIndex.clear();
+    if (reuseManifolds) {
+      for (const ManifoldT<T> &m : prevManifolds) if ((m.countStep & 15u) > 0) copyContacts(m);
+      return;
+    }
+    for (size_t m = 0; m < prevManifolds.size(); m++) {  // hashInsert skips rewritten slots
+      const ManifoldT<T> &pm = prevManifolds[m];
+      if (bodies[pm.a].lifecycle != 0 || bodies[pm.b].lifecycle != 0) continue;
+      prevIndex[pkey(pm.a, pm.b)] = (uint32_t)m;
+    }
+    broadphase();
+    for (auto &pr : pairs) narrowphase(pr.first, pr.second);
+  }
+
+  // ---- adjacency (wgsl-topology.ts:63-153) ----
+  bool jointActive(uint32_t j) const {
+    const Joint<T> &k = joints[j];
+    return k.type != (int)T_NONE && (k.stiffLin != T(0) || k.stiffAng != T(0));
+  }
+  void endpoints(uint32_t id, int &e0, int &e1) const {
+    uint32_t J = (uint32_t)joints.size();
+    if (id < J) { e0 = joints[id].a; e1 = joints[id].b; }
+    else { e0 = (int)manifolds[id - J].a; e1 = (int)manifolds[id - J].b; }
+  }
+  void adjacency() {
+    const uint32_t n = (uint32_t)bodies.size(), J = (uint32_t)joints.size();
+    std::vector<std::vector<std::pair<std::pair<uint32_t, uint32_t>, uint32_t>>> lists(n);
+    auto addId = [&](uint32_t id) {
+      int e[2];
+      endpoints(id, e[0], e[1]);
+      for (int s = 0; s < 2; s++) {
+        int x = e[s];
+        if (!dynamic(x)) continue;
+        std::pair<uint32_t, uint32_t> key;
+        if (id < J) key = {0u, id};
+        else {
+          int other = e[0] == x ? e[1] : e[0];
+          const ManifoldT<T> &m = manifolds[id - J];
+          uint32_t sec = (m.countStep & 15u) == 0 ? 0u : (contacts[m.first].key & ~STICK_BIT);
+          key = {(uint32_t)(other + 2), sec};
+        }
+        lists[x].push_back({key, id});
+      }
+    };
+    for (uint32_t j = 0; j < J; j++) if (jointActive(j)) addId(j);
+    for (uint32_t m = 0; m < manifolds.size(); m++) addId(J + m);
+    adjStart.assign(n + 1, 0);
+    adjList.clear();
+    for (uint32_t b = 0; b < n; b++) {
+      std::stable_sort(lists[b].begin(), lists[b].end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+      adjStart[b] = (uint32_t)adjList.size();
+      for (auto &e : lists[b]) adjList.push_back(e.second);
+    }
+    adjStart[n] = (uint32_t)adjList.size();
+  }
+
+  // ---- colouring (wgsl-topology.ts:155-328) ----
+  static uint32_t priority(uint32_t i) { return hash32(i + 0x9e3779b9u); }
+  static bool beats(uint32_t i, uint32_t j) { uint32_t pi = priority(i), pj = priority(j); return pi > pj || (pi == pj && i > j); }
+  int neighbour(uint32_t b, uint32_t e) const {
+    int e0, e1;
+    endpoints(adjList[e], e0, e1);
+    int other = e0 == (int)b ? e1 : e0;
+    return dynamic(other) ? other : -1;
+  }
+  uint32_t smallestFree(uint32_t b, const std::vector<uint32_t> &src, bool skipPending) const {
+    uint32_t lo = 0, hi = 0;
+    for (uint32_t e = adjStart[b]; e < adjStart[b + 1]; e++) {
+      int j = neighbour(b, e);
+      if (j < 0) continue;
+      uint32_t s = src[j];
+      if (skipPending && (s & PENDING)) continue;
+      uint32_t col = s & 0xffu;
+      if (col == NO_COLOR) continue;
+      if (col < 32) lo |= 1u << col; else hi |= 1u << (col - 32);
+    }
+    if (~lo != 0) return (uint32_t)__builtin_ctz(~lo);
+    if (~hi != 0) return 32u + (uint32_t)__builtin_ctz(~hi);
+    return MAX_COLORS;
+  }
+  void colour() {
+    const uint32_t n = (uint32_t)bodies.size();
+    colorA.resize(n, NO_COLOR); colorB.resize(n, NO_COLOR);
+    uint32_t cap = p.colorCap;
+    if (fixedColors) {
+      for (uint32_t b = 0; b < n; b++) colorA[b] = fixed[b];
+      cap = 1;
+      for (uint32_t b = 0; b < n; b++) if (fixed[b] != NO_COLOR) cap = std::max(cap, fixed[b] + 1);
+    } else {
+      for (uint32_t b = 0; b < n; b++) {  // colorCompact: A -> B
+        if (!dynamic(b)) { colorB[b] = NO_COLOR; continue; }
+        uint32_t col = colorA[b] & 0xffu;
+        if (col != NO_COLOR) {
+          bool isMax = true;
+          for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && isMax; e++) { int j = neighbour(b, e); if (j >= 0 && !beats(b, j)) isMax = false; }
+          if (isMax) col = std::min(col, smallestFree(b, colorA, false));
+        }
+        colorB[b] = col;
+      }
+      for (uint32_t b = 0; b < n; b++) {  // colorMark: B -> A
+        uint32_t col = colorB[b] & 0xffu;
+        if (!dynamic(b)) { colorA[b] = NO_COLOR; continue; }
+        bool pending = col == NO_COLOR;
+        for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && !pending; e++) {
+          int j = neighbour(b, e);
+          if (j >= 0 && (colorB[j] & 0xffu) == col && beats(j, b)) pending = true;
+        }
+        colorA[b] = col | (pending ? PENDING : 0);
+      }
+      for (uint32_t r = 0; r < p.colorRounds; r++) {  // colorRoundAB / BA
+        std::vector<uint32_t> &src = (r % 2 == 0) ? colorA : colorB;
+        std::vector<uint32_t> &dst = (r % 2 == 0) ? colorB : colorA;
+        for (uint32_t b = 0; b < n; b++) {
+          uint32_t s = src[b];
+          if (!(s & PENDING)) { dst[b] = s; continue; }
+          bool isMax = true;
+          for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && isMax; e++) {
+            int j = neighbour(b, e);
+            if (j >= 0 && (src[j] & PENDING) && !beats(b, j)) isMax = false;
+          }
+          if (isMax) { uint32_t f = smallestFree(b, src, true); dst[b] = f >= cap ? ((cap - 1) | PENDING) : f; }
+          else dst[b] = s;
+        }
+      }
+    }
+    // colorCount / colorStarts / colorScatter: stable buckets, clash masks
+    colorCapUsed = cap;
+    numColors = 0; clashes = 0; clashMask[0] = clashMask[1] = 0;
+    std::vector<uint32_t> counts(MAX_COLORS, 0), colOf(n, NO_COLOR);
+    for (uint32_t b = 0; b < n; b++) {
+      if (!dynamic(b)) continue;
+      uint32_t s = colorA[b], col = s & 0xffu;
+      if (s & PENDING) clashes++;
+      if (col == NO_COLOR) col = cap - 1;
+      colorA[b] = col | (s & PENDING);
+      numColors = std::max(numColors, col + 1);
+      colOf[b] = col;
+      counts[col]++;
+      if (s & PENDING) clashMask[col >> 5] |= 1u << (col & 31);
+    }
+    bucketStart.assign(MAX_COLORS + 1, 0);
+    for (uint32_t c = 0; c < MAX_COLORS; c++) bucketStart[c + 1] = bucketStart[c] + counts[c];
+    bucketBodies.assign(bucketStart[MAX_COLORS], 0);
+    std::vector<uint32_t> fill(bucketStart.begin(), bucketStart.end() - 1);
+    for (uint32_t b = 0; b < n; b++) if (colOf[b] != NO_COLOR) bucketBodies[fill[colOf[b]]++] = b;
+  }
+
+  // ---- solve (wgsl-solve.ts) ----
+  struct Acc { T lin[6], ang[6], cr[9], rLin[3], rAng[3]; };  // lin/ang lower triangles: 00 10 11 20 21 22
+  static void zero(Acc &a) { std::memset(&a, 0, sizeof(a)); for (int i = 0; i < 6; i++) { a.lin[i] = T(0); a.ang[i] = T(0); } }
+  static void addRow(Acc &acc, V l, V a, T k, T f) {
+    static const int I[6] = {0, 1, 1, 2, 2, 2}, J[6] = {0, 0, 1, 0, 1, 2};
+    for (int e = 0; e < 6; e++) { acc.lin[e] += (l[I[e]] * l[J[e]]) * k; acc.ang[e] += (a[I[e]] * a[J[e]]) * k; }
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) acc.cr[i * 3 + j] += (a[i] * l[j]) * k;
+    for (int i = 0; i < 3; i++) { acc.rLin[i] += l[i] * f; acc.rAng[i] += a[i] * f; }
+  }
+  static void accAdd(Acc &a, const Acc &b) {
+    for (int i = 0; i < 6; i++) { a.lin[i] += b.lin[i]; a.ang[i] += b.ang[i]; }
+    for (int i = 0; i < 9; i++) a.cr[i] += b.cr[i];
+    for (int i = 0; i < 3; i++) { a.rLin[i] += b.rLin[i]; a.rAng[i] += b.rAng[i]; }
+  }
+  Q rotAOf(int a) const { return a < 0 ? O::q(T(0), T(0), T(0), T(1)) : bodies[a].rot; }
+  V anchorA(const Joint<T> &k, int a) const { return a < 0 ? k.rA : O::add(O::qrotate(bodies[a].rot, k.rA), bodies[a].pos); }
+  V jointLinC(const Joint<T> &k) const { return O::sub(anchorA(k, k.a), O::add(O::qrotate(bodies[k.b].rot, k.rB), bodies[k.b].pos)); }
+  V jointAngC(const Joint<T> &k) const {
+    Q qb = bodies[k.b].rot;
+    T dd = O::dot4(qb, qb);
+    Q inv = O::q(-qb.x / dd, -qb.y / dd, -qb.z / dd, qb.w / dd);
+    Q d = O::qmul(O::qmul(rotAOf(k.a), k.rest), inv);
+    if (d.w < T(0)) d = O::q(-d.x, -d.y, -d.z, -d.w);
+    return O::mul(O::mul(O::v(d.x, d.y, d.z), T(2.0)), k.torqueArm);
+  }
+  static bool hard(T s) { return s >= T(HARDD); }
+  V jointLinRows(const Joint<T> &k, T alpha) const { return O::sub(jointLinC(k), hard(k.stiffLin) ? O::mul(k.c0Lin, alpha) : O::v(T(0), T(0), T(0))); }
+  V jointAngRows(const Joint<T> &k, T alpha) const { return O::sub(jointAngC(k), hard(k.stiffAng) ? O::mul(k.c0Ang, alpha) : O::v(T(0), T(0), T(0))); }
+  void addJoint(Acc &acc, uint32_t jid, T alpha, uint32_t i) const {
+    const Joint<T> &k = joints[jid];
+    bool isA = (int)i == k.a;
+    T sg = isA ? T(1) : T(-1);
+    if (k.type == (int)T_SPRING) {
+      V rAW = O::qrotate(bodies[k.a].rot, k.rA), rBW = O::qrotate(bodies[k.b].rot, k.rB);
+      V d = O::sub(O::add(rAW, bodies[k.a].pos), O::add(rBW, bodies[k.b].pos));
+      T len = O::length(d);
+      if (len <= T(1.0e-6)) return;
+      V n = O::divs(d, len);
+      T st = k.stiffLin;
+      V r = isA ? rAW : rBW;
+      addRow(acc, O::mul(n, sg), O::mul(O::cross(r, n), sg), st, st * (len - k.springRest));
+      return;
+    }
+    if (O::dot(k.penLin, k.penLin) > T(0)) {
+      V F = O::add(O::mulv(k.penLin, jointLinRows(k, alpha)), k.lamLin);
+      V r = isA ? O::qrotate(bodies[k.a].rot, k.rA) : O::neg(O::qrotate(bodies[k.b].rot, k.rB));
+      V v = O::neg(r);
+      addRow(acc, O::v(sg, T(0), T(0)), O::v(T(0), -v.z, v.y), k.penLin.x, F.x);
+      addRow(acc, O::v(T(0), sg, T(0)), O::v(v.z, T(0), -v.x), k.penLin.y, F.y);
+      addRow(acc, O::v(T(0), T(0), sg), O::v(-v.y, v.x, T(0)), k.penLin.z, F.z);
+      T rf = O::dot(r, F);
+      T g0 = O::length(O::sub(O::mul(r, F.x), O::v(rf, T(0), T(0))));
+      T g1 = O::length(O::sub(O::mul(r, F.y), O::v(T(0), rf, T(0))));
+      T g2 = O::length(O::sub(O::mul(r, F.z), O::v(T(0), T(0), rf)));
+      acc.ang[0] += g0; acc.ang[1] += T(0); acc.ang[2] += g1; acc.ang[3] += T(0); acc.ang[4] += T(0); acc.ang[5] += g2;
+    }
+    if (O::dot(k.penAng, k.penAng) > T(0)) {
+      V F = O::add(O::mulv(k.penAng, jointAngRows(k, alpha)), k.lamAng);
+      T s = sg * k.torqueArm;
+      V z = O::v(T(0), T(0), T(0));
+      addRow(acc, z, O::v(s, T(0), T(0)), k.penAng.x, F.x);
+      addRow(acc, z, O::v(T(0), s, T(0)), k.penAng.y, F.y);
+      addRow(acc, z, O::v(T(0), T(0), s), k.penAng.z, F.z);
+    }
+  }
+  struct PairBody { uint32_t index; Q rot; V dLin, dAng; };
+  PairBody pairBody(uint32_t i) const { return PairBody{i, bodies[i].rot, O::sub(bodies[i].pos, bodies[i].ipos), O::qsub(bodies[i].rot, bodies[i].irot)}; }
+  struct Eval { V rAW, rBW, C, F; T frictionScale, bounds; };
+  Eval evalContact(const Contact<T> &k, const V basis[3], T friction, const PairBody &A, const PairBody &B, T alpha) const {
+    Q rotA = A.rot, rotB = B.rot;
+    if ((k.key & ~STICK_BIT) == SPHERE_FEATURE) { rotA = bodies[A.index].irot; rotB = bodies[B.index].irot; }
+    Eval e;
+    e.rAW = O::qrotate(rotA, k.rA);
+    e.rBW = O::qrotate(rotB, k.rB);
+    T keep = T(1.0) - alpha;
+    T c0[3] = {k.c0x, k.c0y, k.c0z};
+    for (int r = 0; r < 3; r++) {
+      V l = basis[r];
+      e.C[r] = c0[r] * keep + O::dot(l, A.dLin) - O::dot(l, B.dLin) + O::dot(O::cross(e.rAW, l), A.dAng) + O::dot(O::cross(e.rBW, O::neg(l)), B.dAng);
+    }
+    V F = O::add(O::mulv(k.pen, e.C), k.lam);
+    F.x = O::mn(F.x, T(0.0));
+    e.bounds = std::fabs(F.x) * friction;
+    e.frictionScale = std::sqrt(F.y * F.y + F.z * F.z);
+    if (e.frictionScale > e.bounds && e.frictionScale > T(0.0)) {
+      T s = e.bounds / e.frictionScale;
+      F.y = F.y * s; F.z = F.z * s;
+    }
+    e.F = F;
+    return e;
+  }
+  uint32_t lanesFor(uint32_t count) const {
+    if (count >= p.lanesRule[0]) return 1;
+    if (count >= p.lanesRule[1]) return 2;
+    if (count >= p.lanesRule[2]) return 4;
+    return 8;
+  }
+  /** accumulate(i, lane, lanes) (wgsl-solve.ts:436-501). */
+  Acc accumulate(uint32_t i, uint32_t lane, uint32_t lanes, T alpha) const {
+    Acc acc; zero(acc);
+    const Body<T> &bd = bodies[i];
+    T dt = dtT(), dt2 = dt * dt;
+    T m = bd.mass / dt2;
+    V I = O::divs(bd.moment, dt2);
+    if (lane == 0) {
+      acc.lin[0] = m; acc.lin[2] = m; acc.lin[5] = m;
+      acc.ang[0] = I.x; acc.ang[2] = I.y; acc.ang[5] = I.z;
+      V dx = O::sub(bd.pos, bd.inpos);
+      acc.rLin[0] = m * dx.x; acc.rLin[1] = m * dx.y; acc.rLin[2] = m * dx.z;
+      V dq = O::qsub(bd.rot, bd.inrot);
+      acc.rAng[0] = I.x * dq.x; acc.rAng[1] = I.y * dq.y; acc.rAng[2] = I.z * dq.z;
+    }
+    const uint32_t J = (uint32_t)joints.size();
+    for (uint32_t e = adjStart[i] + lane; e < adjStart[i + 1]; e += lanes) {
+      uint32_t id = adjList[e];
+      if (id < J) { addJoint(acc, id, alpha, i); continue; }
+      const ManifoldT<T> &mf = manifolds[id - J];
+      bool isA = i == mf.a;
+      PairBody A = pairBody(mf.a), B = pairBody(mf.b);
+      V basis[3];
+      uint32_t c = mf.first, cEnd = mf.first + (mf.countStep & 15u);
+      for (; c < cEnd; c++) {
+        const Contact<T> &k = contacts[c];
+        O::orthonormal(mf.n, basis);
+        Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
+        T sg = isA ? T(1) : T(-1);
+        V r = isA ? ev.rAW : ev.rBW;
+        V l0 = O::mul(basis[0], sg), l1 = O::mul(basis[1], sg), l2 = O::mul(basis[2], sg);
+        addRow(acc, l0, O::cross(r, l0), k.pen.x, ev.F.x);
+        addRow(acc, l1, O::cross(r, l1), k.pen.y, ev.F.y);
+        addRow(acc, l2, O::cross(r, l2), k.pen.z, ev.F.z);
+      }
+    }
+    return acc;
+  }
+  /** finishBody (wgsl-solve.ts:504-558): LDLᵀ with the 1e-5 pivot floor, then the update. */
+  void finishBody(uint32_t i, const Acc &acc) {
+    const T PF = T(1e-5);
+    T A11 = acc.lin[0], A21 = acc.lin[1], A22 = acc.lin[2], A31 = acc.lin[3], A32 = acc.lin[4], A33 = acc.lin[5];
+    T A41 = acc.cr[0], A42 = acc.cr[1], A43 = acc.cr[2], A44 = acc.ang[0];
+    T A51 = acc.cr[3], A52 = acc.cr[4], A53 = acc.cr[5], A54 = acc.ang[1], A55 = acc.ang[2];
+    T A61 = acc.cr[6], A62 = acc.cr[7], A63 = acc.cr[8], A64 = acc.ang[3], A65 = acc.ang[4], A66 = acc.ang[5];
+    T D1 = A11;
+    T L21 = A21 / D1, L31 = A31 / D1, L41 = A41 / D1, L51 = A51 / D1, L61 = A61 / D1;
+    T D2 = O::mx(A22 - L21 * L21 * D1, A22 * PF);
+    T L32 = (A32 - L21 * L31 * D1) / D2, L42 = (A42 - L21 * L41 * D1) / D2, L52 = (A52 - L21 * L51 * D1) / D2, L62 = (A62 - L21 * L61 * D1) / D2;
+    T D3 = O::mx(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33 * PF);
+    T L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3, L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3, L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+    T D4 = O::mx(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44 * PF);
+    T L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4, L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+    T D5 = O::mx(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55 * PF);
+    T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+    T D6 = O::mx(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66 * PF);
+    T y1 = acc.rLin[0], y2 = acc.rLin[1] - L21 * y1, y3 = acc.rLin[2] - L31 * y1 - L32 * y2;
+    T y4 = acc.rAng[0] - L41 * y1 - L42 * y2 - L43 * y3, y5 = acc.rAng[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
+    T y6 = acc.rAng[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
+    T w3 = y6 / D6, w2 = y5 / D5 - L65 * w3, w1 = y4 / D4 - L54 * w2 - L64 * w3;
+    T v3 = y3 / D3 - L43 * w1 - L53 * w2 - L63 * w3;
+    T v2 = y2 / D2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
+    T v1 = y1 / D1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
+    Body<T> &bd = bodies[i];
+    bd.pos = O::sub(bd.pos, O::v(v1, v2, v3));
+    bd.rot = O::qadd(bd.rot, O::neg(O::v(w1, w2, w3)));
+  }
+  /** One colour's primal: chunks of 64/lanes bodies; each chunk reads every pose it needs
+   *  before any body of it is written (the reference's workgroup barriers), so a clashing colour
+   *  is exact and a proper one is order-free. */
+  void primalColor(uint32_t c, T alpha) {
+    uint32_t start = bucketStart[c], count = bucketStart[c + 1] - start;
+    if (count == 0) return;
+    uint32_t lanes = lanesFor(count), per = 64 / lanes;
+    std::vector<Acc> accs(per);
+    for (uint32_t chunk = 0; chunk * per < count; chunk++) {
+      uint32_t k0 = chunk * per, k1 = std::min(count, k0 + per);
+      for (uint32_t k = k0; k < k1; k++) {
+        uint32_t i = bucketBodies[start + k];
+        Acc part[8];
+        for (uint32_t l = 0; l < lanes; l++) part[l] = accumulate(i, l, lanes, alpha);
+        for (uint32_t s = 4; s > 0; s >>= 1)
+          if (s < lanes) for (uint32_t l = 0; l < s; l++) accAdd(part[l], part[l + s]);
+        accs[k - k0] = part[0];
+      }
+      for (uint32_t k = k0; k < k1; k++) finishBody(bucketBodies[start + k], accs[k - k0]);
+    }
+  }
+  void dualJoint(uint32_t j, T alpha) {
+    Joint<T> &k = joints[j];
+    if (k.type != (int)T_JOINT) return;
+    Joint<T> k0 = k;
+    T BIG = T(BIGD);
+    V penLin = k0.penLin, penAng = k0.penAng, lamLin = k0.lamLin, lamAng = k0.lamAng;
+    T stLin = k0.stiffLin, stAng = k0.stiffAng;
+    if (O::dot(penLin, penLin) > T(0)) {
+      V C = jointLinRows(k0, alpha);
+      if (hard(stLin)) lamLin = O::add(O::mulv(penLin, C), lamLin);
+      T cap = O::mn(stLin, T(1e10));
+      V np = O::add(penLin, O::mul(O::vabs(C), T(p.betaLin)));
+      penLin = O::v(O::mn(np.x, cap), O::mn(np.y, cap), O::mn(np.z, cap));
+    }
+    T frac = k0.fracture, limit = std::fabs(frac);
+    if (O::dot(penAng, penAng) > T(0)) {
+      V C = jointAngRows(k0, alpha);
+      if (hard(stAng)) lamAng = O::add(O::mulv(penAng, C), lamAng);
+      T cap = O::mn(stAng, T(1e10));
+      V np = O::add(penAng, O::mul(O::vabs(C), T(p.betaAng)));
+      penAng = O::v(O::mn(np.x, cap), O::mn(np.y, cap), O::mn(np.z, cap));
+    }
+    bool linear = frac < T(0) && O::dot(lamLin, lamLin) > limit * limit;
+    if (limit < BIG && (O::dot(lamAng, lamAng) > limit * limit || linear)) {
+      penLin = O::v(T(0), T(0), T(0)); penAng = penLin; lamLin = penLin; lamAng = penLin;
+      stLin = T(0); stAng = T(0);
+    }
+    k.penLin = penLin; k.penAng = penAng; k.lamLin = lamLin; k.lamAng = lamAng; k.stiffLin = stLin; k.stiffAng = stAng;
+  }
+  void dualManifold(uint32_t m, T alpha) {
+    const ManifoldT<T> &mf = manifolds[m];
+    PairBody A = pairBody(mf.a), B = pairBody(mf.b);
+    V basis[3];
+    O::orthonormal(mf.n, basis);
+    uint32_t end = mf.first + (mf.countStep & 15u);
+    for (uint32_t c = mf.first; c < end; c++) {
+      Contact<T> &k = contacts[c];
+      Eval e = evalContact(k, basis, mf.friction, A, B, alpha);
+      k.lam = e.F;
+      V pen = k.pen;
+      T BL = T(p.betaLin), PMAX = T(1e10);
+      if (e.F.x < T(0)) pen.x = O::mn(pen.x + BL * std::fabs(e.C.x), PMAX);
+      if (e.frictionScale <= e.bounds) {
+        pen.y = O::mn(pen.y + BL * std::fabs(e.C.y), PMAX);
+        pen.z = O::mn(pen.z + BL * std::fabs(e.C.z), PMAX);
+        bool stick = std::sqrt(e.C.y * e.C.y + e.C.z * e.C.z) < T(0.00001);
+        k.key = (k.key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+      }
+      k.pen = pen;
+    }
+  }
+  void warmStartJoints() {
+    T alpha = T(p.alpha), gamma = T(p.gamma), decay = alpha * gamma;
+    for (Joint<T> &k : joints) {
+      if (k.type != (int)T_JOINT) continue;
+      k.c0Lin = jointLinC(k);
+      k.c0Ang = jointAngC(k);
+      k.lamLin = O::mul(k.lamLin, decay);
+      k.lamAng = O::mul(k.lamAng, decay);
+      for (int r = 0; r < 3; r++) {
+        k.penLin[r] = O::mn(O::clampt(k.penLin[r] * gamma, T(1.0), T(1e10)), k.stiffLin);
+        k.penAng[r] = O::mn(O::clampt(k.penAng[r] * gamma, T(1.0), T(1e10)), k.stiffAng);
+      }
+    }
+  }
+  void warmStartBodies() {
+    T dt = dtT(), g = T(p.gravity);
+    V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
+    for (Body<T> &b : bodies) {
+      bool dyn = b.mass > T(0);
+      V inpos = O::add(b.pos, O::mul(b.vel, dt));
+      if (dyn) inpos = O::add(inpos, O::mul(up, g * (dt * dt)));
+      b.inpos = inpos;
+      b.inrot = O::qadd(b.rot, O::mul(b.angvel, dt));
+      T w = T(0);
+      if (std::fabs(g) > T(0)) {
+        T sg = g > T(0) ? T(1) : (g < T(0) ? T(-1) : T(0));
+        w = O::clampt((O::dot(b.vel, up) - b.prevUp) / dt * sg / std::fabs(g), T(0), T(1));
+      }
+      bool slow = O::length(b.vel) + O::length(b.angvel) * b.radius < T(0.1);
+      bool atRest = (p.flags & FLAG_START_AT_REST) && slow;
+      b.ipos = b.pos; b.lifecycle = 0; b.irot = b.rot;
+      if (dyn && !atRest) {
+        b.pos = O::add(O::add(b.pos, O::mul(b.vel, dt)), O::mul(up, g * (w * dt * dt)));
+        b.rot = O::qadd(b.rot, O::mul(b.angvel, dt));
+      }
+      bool moving = b.vel.x != T(0) || b.vel.y != T(0) || b.vel.z != T(0) || b.angvel.x != T(0) || b.angvel.y != T(0) || b.angvel.z != T(0);
+      if (!dyn && moving) {
+        b.pos = O::add(b.pos, O::mul(b.vel, dt));
+        b.rot = O::qadd(b.rot, O::mul(b.angvel, dt));
+      }
+    }
+  }
+  void updateVelocities() {
+    T dt = dtT();
+    V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
+    for (Body<T> &b : bodies) {
+      T prevUp = O::dot(b.vel, up);
+      if (b.mass > T(0)) {
+        b.vel = O::divs(O::sub(b.pos, b.ipos), dt);
+        b.angvel = O::divs(O::qsub(b.rot, b.irot), dt);
+      }
+      b.prevUp = prevUp;
+    }
+  }
+  void solve() {
+    T alpha = T(p.alpha);
+    warmStartJoints();
+    warmStartBodies();
+    for (uint32_t it = 0; it < p.iterations; it++) {
+      for (uint32_t c = 0; c < colorCapUsed; c++) primalColor(c, alpha);
+      for (uint32_t j = 0; j < joints.size(); j++) dualJoint(j, alpha);
+      for (uint32_t m = 0; m < manifolds.size(); m++) dualManifold(m, alpha);
+    }
+    updateVelocities();
+  }
+  /** One step (GpuSolver3D.step). */
+  void step(bool reuseManifolds = false) {
+    stepCount++;
+    p.step = stepCount;
+    collision(reuseManifolds);
+    adjacency();
+    colour();
+    solve();
+  }
+};
+
+}  // namespace gpuo

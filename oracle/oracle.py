@@ -1,0 +1,299 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the oracle (oracle/liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this module, as the
+checker. Two restatements live behind it:
+
+* ``RefSolver`` -- the reference's f64 CPU solver (src/avbd3d/ref/*.ts), pinned by the golden
+  trajectories in tests/golden/oracle3d;
+* ``Model`` -- the WebGPU step's semantics (src/avbd3d/gpu/wgsl-*.ts), f32 or f64.
+
+Record layouts are the reference's (src/avbd3d/gpu/layout.ts): bodies 40 floats, joints 40
+floats + info (type, a, b, 0), manifolds 8 words, contacts 16 words. Here they travel as float64
+arrays with integer words (keys, ids, stamps) stored as plain numbers.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    """Compile the oracle (g++ -ffp-contract=off) into oracle/liboracle.so."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.oref_create.restype = vp
+        L.omodel_create.restype = vp
+        for name in dir(L):
+            pass
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Params(ctypes.Structure):
+    """Params of the GPU-semantics model (the fields of layout.ts Params it depends on)."""
+
+    _fields_ = [
+        ("dt", ctypes.c_double), ("gravity", ctypes.c_double), ("betaLin", ctypes.c_double),
+        ("betaAng", ctypes.c_double), ("gamma", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("contactMargin", ctypes.c_double), ("up", ctypes.c_double * 3),
+        ("flags", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("colorCap", ctypes.c_uint32),
+        ("colorRounds", ctypes.c_uint32), ("step", ctypes.c_uint32), ("lanesRule", ctypes.c_uint32 * 3),
+    ]
+
+
+FLAG_MATCH_NEAREST = 1
+FLAG_FACE_BIAS = 2
+FLAG_REUSE_CONTACTS = 4
+FLAG_START_AT_REST = 8
+FLAG_MASS_PENALTY = 16
+FLAG_DEFAULTS = FLAG_MATCH_NEAREST | FLAG_FACE_BIAS | FLAG_REUSE_CONTACTS | FLAG_START_AT_REST | FLAG_MASS_PENALTY
+NO_COLOR = 255
+
+
+def make_params(dt=1 / 60, gravity=-10.0, iterations=10, alpha=0.99, beta_lin=1e4, beta_ang=100.0, gamma=0.999,
+                flags=FLAG_DEFAULTS, color_cap=64, color_rounds=16, margin=0.01, up=(0, 0, 1), lanes=(0, 0, 0)):
+    p = Params()
+    p.dt, p.gravity, p.betaLin, p.betaAng, p.gamma, p.alpha, p.contactMargin = dt, gravity, beta_lin, beta_ang, gamma, alpha, margin
+    p.up[0], p.up[1], p.up[2] = up
+    p.flags, p.iterations, p.colorCap, p.colorRounds, p.step = flags, iterations, color_cap, color_rounds, 0
+    p.lanesRule[0], p.lanesRule[1], p.lanesRule[2] = lanes
+    return p
+
+
+class RefSolver:
+    """The reference's f64 CPU solver (src/avbd3d/ref/solver.ts)."""
+
+    def __init__(self, scene=None):
+        self.L = lib()
+        self.h = ctypes.c_void_p(self.L.oref_create())
+        if scene is not None:
+            self.L.oref_build_scene(self.h, scene)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oref_destroy(self.h)
+            self.h = None
+
+    def set_params(self, dt=1 / 60, gravity=-10.0, iterations=10, alpha=0.99, beta_lin=1e4, beta_ang=100.0, gamma=0.999):
+        self.L.oref_set_params(self.h, ctypes.c_double(dt), ctypes.c_double(gravity), iterations, ctypes.c_double(alpha),
+                               ctypes.c_double(beta_lin), ctypes.c_double(beta_ang), ctypes.c_double(gamma))
+
+    def add_body(self, size, density, friction, pos, rot=None, vel=None, angvel=None):
+        a = lambda v: None if v is None else _p(np.ascontiguousarray(v, dtype=np.float64))
+        keep = [np.ascontiguousarray(v, dtype=np.float64) for v in (size, pos) + tuple(x for x in (rot, vel, angvel) if x is not None)]
+        return self.L.oref_add_body(self.h, _p(keep[0]), ctypes.c_double(density), ctypes.c_double(friction), _p(keep[1]),
+                                    a(rot), a(vel), a(angvel))
+
+    def add_joint(self, a, b, rA, rB, stiff_lin=float("inf"), stiff_ang=0.0, fracture=float("inf")):
+        rA = np.ascontiguousarray(rA, dtype=np.float64)
+        rB = np.ascontiguousarray(rB, dtype=np.float64)
+        return self.L.oref_add_joint(self.h, a, b, _p(rA), _p(rB), ctypes.c_double(stiff_lin), ctypes.c_double(stiff_ang), ctypes.c_double(fracture))
+
+    def add_ignore(self, a, b):
+        return self.L.oref_add_ignore(self.h, a, b)
+
+    def step(self, n=1):
+        self.L.oref_step(self.h, n)
+
+    @property
+    def body_count(self):
+        return self.L.oref_body_count(self.h)
+
+    @property
+    def force_count(self):
+        return self.L.oref_force_count(self.h)
+
+    def poses(self):
+        out = np.zeros((self.body_count, 7))
+        self.L.oref_poses(self.h, _p(out))
+        return out
+
+    def body_records(self):
+        out = np.zeros((self.body_count, 40))
+        self.L.oref_body_records(self.h, _p(out))
+        return out
+
+    def forces(self):
+        out = np.zeros((self.force_count, 4), dtype=np.int32)
+        self.L.oref_forces(self.h, _p(out))
+        return out
+
+    def joint_records(self):
+        n = self.force_count
+        rec = np.zeros((max(n, 1), 40))
+        info = np.zeros((max(n, 1), 4), dtype=np.int32)
+        k = self.L.oref_joint_records(self.h, _p(rec), _p(info))
+        return rec[:k], info[:k]
+
+    def manifold_records(self):
+        forces = self.forces()
+        nm = int((forces[:, 0] == 0).sum()) if len(forces) else 0
+        nc = int(forces[forces[:, 0] == 0, 3].sum()) if nm else 0
+        man = np.zeros((max(nm, 1), 8))
+        con = np.zeros((max(nc, 1), 16))
+        counts = np.zeros(2, dtype=np.int32)
+        self.L.oref_manifold_records(self.h, _p(man), _p(con), _p(counts))
+        return man[:counts[0]], con[:counts[1]]
+
+    def collide(self, a, b):
+        basis = np.zeros(9)
+        out = np.zeros((8, 7))
+        n = self.L.oref_collide(self.h, a, b, _p(basis), _p(out))
+        return basis.reshape(3, 3), out[:n]
+
+
+def solve6(A, b):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(6)
+    lib().oref_solve6(_p(A), _p(b), _p(x))
+    return x
+
+
+class Model:
+    """The WebGPU step's semantics (oracle/gpu3d.hpp) in f32 (prec=32) or f64 (prec=64)."""
+
+    def __init__(self, prec=32):
+        self.L = lib()
+        self.h = ctypes.c_void_p(self.L.omodel_create(prec))
+        self.n = 0
+        self.nj = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.omodel_destroy(self.h)
+            self.h = None
+
+    def set_params(self, p):
+        self.L.omodel_set_params(self.h, ctypes.byref(p))
+
+    def set_bodies(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        self.n = len(rec)
+        self.L.omodel_set_bodies(self.h, _p(rec), self.n)
+
+    def bodies(self):
+        out = np.zeros((self.n, 40))
+        self.L.omodel_get_bodies(self.h, _p(out))
+        return out
+
+    def set_joints(self, rec, info):
+        rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 40)
+        info = np.ascontiguousarray(info, dtype=np.int32).reshape(-1, 4)
+        self.nj = len(rec)
+        self.L.omodel_set_joints(self.h, _p(rec), _p(info), self.nj)
+
+    def joints(self):
+        out = np.zeros((self.nj, 40))
+        self.L.omodel_get_joints(self.h, _p(out))
+        return out
+
+    def add_ignore(self, a, b):
+        self.L.omodel_add_ignore(self.h, a, b)
+
+    def set_hulls(self, words):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        self.L.omodel_set_hulls(self.h, _p(words), len(words))
+
+    def set_prev(self, man, con):
+        man = np.ascontiguousarray(man, dtype=np.float64).reshape(-1, 8)
+        con = np.ascontiguousarray(con, dtype=np.float64).reshape(-1, 16)
+        self.L.omodel_set_prev(self.h, _p(man), len(man), _p(con), len(con))
+
+    def set_fixed_colors(self, colors):
+        if colors is None:
+            self.L.omodel_set_fixed_colors(self.h, None)
+        else:
+            self._fixed = np.ascontiguousarray(colors, dtype=np.uint32)
+            self.L.omodel_set_fixed_colors(self.h, _p(self._fixed))
+
+    def set_color_state(self, state):
+        state = np.ascontiguousarray(state, dtype=np.uint32)
+        self.L.omodel_set_color_state(self.h, _p(state))
+
+    def set_step_count(self, s):
+        self.L.omodel_set_step_count(self.h, s)
+
+    def step(self, n=1, reuse_manifolds=False):
+        self.L.omodel_step(self.h, n, int(reuse_manifolds))
+
+    def collide(self, reuse_manifolds=False):
+        self.L.omodel_collide(self.h, int(reuse_manifolds))
+
+    def solve(self):
+        self.L.omodel_solve(self.h)
+
+    def counts(self):
+        out = np.zeros(7, dtype=np.uint32)
+        self.L.omodel_counts(self.h, _p(out))
+        keys = ("pairs", "manifolds", "contacts", "num_colors", "clashes", "adjacency", "color_cap")
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def pairs(self):
+        c = self.counts()
+        out = np.zeros((max(c["pairs"], 1), 2), dtype=np.uint32)
+        self.L.omodel_get_pairs(self.h, _p(out))
+        return out[:c["pairs"]]
+
+    def contacts(self):
+        c = self.counts()
+        man = np.zeros((max(c["manifolds"], 1), 8))
+        con = np.zeros((max(c["contacts"], 1), 16))
+        self.L.omodel_get_contacts(self.h, _p(man), _p(con))
+        return man[:c["manifolds"]], con[:c["contacts"]]
+
+    def colors(self):
+        out = np.zeros(self.n, dtype=np.uint32)
+        self.L.omodel_get_colors(self.h, _p(out))
+        return out
+
+    def adjacency(self):
+        c = self.counts()
+        start = np.zeros(self.n + 1, dtype=np.uint32)
+        lst = np.zeros(max(c["adjacency"], 1), dtype=np.uint32)
+        self.L.omodel_get_adjacency(self.h, _p(start), _p(lst))
+        return start, lst[:c["adjacency"]]
+
+
+def sequential_colors(bodies):
+    """The reference's Gauss-Seidel order as colours (solver.ts:1365-1372): newest dynamic first."""
+    n = len(bodies)
+    colors = np.full(n, NO_COLOR, dtype=np.uint32)
+    nxt = 0
+    for i in range(n - 1, -1, -1):
+        if bodies[i, 19] > 0:
+            colors[i] = nxt
+            nxt += 1
+    assert nxt <= 64
+    return colors
+
+
+def seed_model_from_ref(model, ref, params):
+    """seedFrom (solver.ts:1320-1359): bodies, joints with warm-start state, last contacts."""
+    model.set_params(params)
+    model.set_bodies(ref.body_records())
+    rec, info = ref.joint_records()
+    model.set_joints(rec, info)
+    for kind, a, b, _ in ref.forces():
+        if kind == 3:
+            model.add_ignore(int(a), int(b))
+    man, con = ref.manifold_records()
+    model.set_prev(man, con)
