@@ -1,0 +1,155 @@
+/*
+ * avbd.h — C ABI of the MI355X AVBD rigid-body step (libavbd_mi355x.so).
+ *
+ * Drop-in for the reference's WebGPU solver class GpuSolver3D
+ * (src/avbd3d/gpu/solver.ts:504), whose operational surface this mirrors. Plain C: pointers,
+ * sizes and status codes; no torch or HIP types. One host thread per solver; every call is
+ * ordered on the solver's own HIP stream; reads synchronise that stream.
+ *
+ * Record formats are the reference's (src/avbd3d/gpu/layout.ts) so callers keep their
+ * readback code:
+ *   body      40 f32  (B_POS 0 .. B_ANGVEL 36; layout.ts:49-58)
+ *   joint     40 f32  (J_PEN_LIN 0 .. J_REST_START 36; layout.ts:84-110) + info int32[4]
+ *             (type 1 joint / 2 spring / 0 none, body a (-1 world), body b, 0)
+ *   manifold  8 u32   (a, b, first contact, count | step << 4; normal xyz, friction as f32 bits)
+ *   contact   16 u32  (rA xyz, key | STICK; rB xyz, C0.n; pen xyz, C0.t1; lam xyz, C0.t2)
+ * Inside the library bodies live as SoA float4 streams; the 40-float record is the interchange
+ * format only.
+ *
+ * Status codes: 0 ok; negative on error (avbd_last_error gives the message). Invalid input is
+ * refused before anything changes (solver.ts:858-875). Capacity overflow during a step is not
+ * an error: it is counted in avbd_counters.overflow (bits 1 pairs, 2 contacts, 4 manifolds),
+ * as the reference counts it (wgsl-collision.ts:230, :727, :816).
+ */
+#ifndef AVBD_MI355X_H
+#define AVBD_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_OK 0
+#define AVBD_ERR_INVALID (-1)
+#define AVBD_ERR_CAPACITY (-2)
+#define AVBD_ERR_DEVICE (-3)
+#define AVBD_ERR_DESTROYED (-4)
+
+/* GpuParams3D.flags bits (layout.ts:156-166) */
+#define AVBD_FLAG_MATCH_NEAREST 1u
+#define AVBD_FLAG_FACE_BIAS 2u
+#define AVBD_FLAG_REUSE_CONTACTS 4u
+#define AVBD_FLAG_START_AT_REST 8u
+#define AVBD_FLAG_MASS_PENALTY 16u
+#define AVBD_FLAG_DEFAULTS 31u
+
+#define AVBD_BODY_FLOATS 40
+#define AVBD_JOINT_FLOATS 40
+#define AVBD_MANIFOLD_WORDS 8
+#define AVBD_CONTACT_WORDS 16
+#define AVBD_NO_COLOR 255u
+
+typedef struct avbd_solver avbd_solver;
+
+/* Replaces GpuParams3D + the parts of GpuSolverOptions a step reads (solver.ts:175-274). */
+typedef struct avbd_params {
+  float dt;             /* 1/60 */
+  float gravity;        /* along up; negative pulls down */
+  float beta_lin;       /* 1e4 */
+  float beta_ang;       /* 100 */
+  float gamma;          /* 0.999 */
+  float alpha;          /* 0.99 */
+  float contact_margin; /* 0.01 (COLLISION_MARGIN) */
+  float up[3];          /* unit up vector */
+  uint32_t flags;       /* AVBD_FLAG_* */
+  uint32_t iterations;  /* 10 */
+  uint32_t color_cap;   /* colours the colouring may use and the solve sweeps, <= 64 */
+  uint32_t color_rounds;/* Jones-Plassmann rounds per step, even */
+  uint32_t primal_lanes[3]; /* colour sizes from which 1, 2, 4 threads per body are used (else 8) */
+} avbd_params;
+
+/* Replaces GpuSolverOptions.bodyCapacity/capacity (solver.ts:233-274). 0 = default sizing. */
+typedef struct avbd_capacity {
+  uint32_t bodies, joints, pairs, manifolds, contacts, hull_words;
+} avbd_capacity;
+
+/* readCounters (solver.ts:2305-2308). */
+typedef struct avbd_counters {
+  uint32_t pairs, contacts, manifolds, overflow, clashes, colors, color_cap;
+} avbd_counters;
+
+/* StepProfile (avbd2d/gpu/layout.ts:289-293) in ms, from hipEvents on the solver's stream. */
+typedef struct avbd_profile {
+  float collision, adjacency, coloring, solve, total;
+} avbd_profile;
+
+/* Per-kernel timing over a profiled run (the roofline's launch durations). */
+typedef struct avbd_kernel_stats {
+  float primal_ms, dual_ms, narrowphase_ms, broadphase_ms, other_ms;
+  uint32_t primal_launches, dual_launches, narrowphase_launches, broadphase_launches;
+  uint64_t primal_bytes, dual_bytes; /* algorithmic bytes summed over those launches */
+} avbd_kernel_stats;
+
+typedef struct avbd_step_opts {
+  int reuse_manifolds; /* GpuStepOptions.reuseManifolds (solver.ts:43-50): substeps after the first */
+} avbd_step_opts;
+
+/* new GpuSolver3D(device, ref, options) (solver.ts:722): bodies and joints as records. */
+int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_count, const float *joints,
+                const int32_t *joint_info, uint32_t joint_count, const avbd_capacity *capacity, avbd_solver **out);
+/* GpuSolver3D.destroy (solver.ts:2439). */
+void avbd_destroy(avbd_solver *s);
+/* The last error's message (the reference throws Error / StorageCapacityError). */
+const char *avbd_last_error(const avbd_solver *s);
+/* Message of the last failed avbd_create (no solver exists to carry it). */
+const char *avbd_create_error(void);
+
+/* GpuSolver3D.step (solver.ts:2112-2293): one substep, enqueued on the solver's stream. */
+int avbd_step(avbd_solver *s, const avbd_step_opts *opts);
+/* Several steps; the 1st detects contacts, the rest reuse manifolds when substeps > 1. */
+int avbd_steps(avbd_solver *s, uint32_t steps, uint32_t substeps);
+/* params (mutable between steps). */
+int avbd_set_params(avbd_solver *s, const avbd_params *params);
+int avbd_get_params(const avbd_solver *s, avbd_params *params);
+
+/* readBodies / rewriteBodies (solver.ts:2297, :1774): records [first, first+count). */
+int avbd_read_bodies(avbd_solver *s, float *out, uint32_t first, uint32_t count);
+int avbd_write_bodies(avbd_solver *s, const float *in, uint32_t first, uint32_t count);
+/* readJoints (solver.ts:2300) */
+int avbd_read_joints(avbd_solver *s, float *out, uint32_t first, uint32_t count);
+/* readCounters (solver.ts:2305) */
+int avbd_read_counters(avbd_solver *s, avbd_counters *out);
+/* readContactList (solver.ts:2311): manifolds then contacts of the last step, raw records. */
+int avbd_read_contacts(avbd_solver *s, uint32_t *manifolds, uint32_t manifold_cap, uint32_t *contacts, uint32_t contact_cap);
+/* readPairs (solver.ts:2348): broadphase pairs (a > b) of the last detecting step. */
+int avbd_read_pairs(avbd_solver *s, uint32_t *out, uint32_t cap);
+/* Colour state per body (colour | PENDING<<8) and the colour buckets' starts (65 words). */
+int avbd_read_colors(avbd_solver *s, uint32_t *state, uint32_t count, uint32_t *starts);
+
+/* seedFrom (solver.ts:1320): bodies, joints with warm-start state, and last step's contacts. */
+int avbd_seed(avbd_solver *s, const float *bodies, uint32_t body_count, const float *joints, uint32_t joint_count,
+              const uint32_t *manifolds, uint32_t manifold_count, const uint32_t *contacts, uint32_t contact_count);
+/* fixedColors (solver.ts:560): colours per body, NO_COLOR for none; NULL returns to colouring. */
+int avbd_set_fixed_colors(avbd_solver *s, const uint32_t *colors, uint32_t count);
+/* Colour state the next colouring starts from (validation of the colouring on given state). */
+int avbd_set_color_state(avbd_solver *s, const uint32_t *state, uint32_t count);
+/* IgnoreCollision pairs (forces.ts:384; NoCollideTable.add with IGNORE_SLOT). */
+int avbd_add_ignore_pairs(avbd_solver *s, const uint32_t *pairs, uint32_t count);
+/* setFilters (solver.ts:1791): collision groups and the groups each collides with. */
+int avbd_set_filters(avbd_solver *s, const uint32_t *groups, const uint32_t *collides_with, uint32_t count);
+/* Hull buffer (wgsl-hull.ts header format, vec4u words); bodies reference it by shape code. */
+int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t word_count);
+
+/* profileNextStep (solver.ts:1952): phase times of the next step, synchronously. */
+int avbd_profile_step(avbd_solver *s, const avbd_step_opts *opts, avbd_profile *out);
+/* Runs `steps` x `substeps` with hipEvents around every launch of the hot kernels. */
+int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_kernel_stats *out);
+/* Times `steps` x `substeps` between two hipEvents on the solver's stream (no readback inside). */
+int avbd_time_steps(avbd_solver *s, uint32_t steps, uint32_t substeps, float *ms);
+/* Waits for the solver's stream. */
+int avbd_sync(avbd_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

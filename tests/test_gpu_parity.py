@@ -1,0 +1,213 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Bit-exact: pairs, contacts (anchors, keys, C0, warm-start state), colourings, and with one
+thread per body whole steps. Within the reference's own tolerances: poses vs the f64
+reference on the seeded protocol (tests-gpu/avbd3d-gpu.gpu.test.ts:35-99: 1e-5)."""
+import sys
+
+import numpy as np
+import pytest
+
+import avbd_amd
+import oracle
+from avbd_amd import records, scenes
+from helpers import PARITY_SCENES, brute_force_pairs, contact_match, random_box_pairs, random_broadphase_scene, sorted_contacts
+
+pytestmark = pytest.mark.gpu
+
+LANES1 = (0, 0, 0)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def model_for(bodies, params_kw, joints=None, info=None, prec=32):
+    params_kw = dict(params_kw)
+    params_kw.setdefault("gravity", -9.81)
+    m = oracle.Model(prec)
+    m.set_params(oracle.make_params(**params_kw))
+    m.set_bodies(records.bodies_to_oracle(bodies))
+    if joints is not None and len(joints):
+        m.set_joints(np.asarray(joints, np.float64), info)
+    return m
+
+
+def gpu_for(bodies, params_kw, joints=None, info=None, **kw):
+    pk = dict(params_kw)
+    pk.setdefault("gravity", -9.81)
+    lanes = pk.pop("lanes", LANES1)
+    pk["primal_lanes"] = lanes
+    return avbd_amd.GpuSolver3D(bodies, joints, info, params=avbd_amd.default_params(**pk), **kw)
+
+
+def assert_same_contacts(gpu, model):
+    gm, gc = gpu.read_contacts_raw()
+    om, oc = model.contacts()
+    om_u, oc_u = records.contacts_from_oracle(om, oc)
+    G = sorted_contacts(gm, gc)
+    O = sorted_contacts(om_u, oc_u)
+    assert len(G) == len(O)
+    for (ma, ca), (mb, cb) in zip(G, O):
+        assert (ma[:2] == mb[:2]).all() and (ma[3] & 15) == (mb[3] & 15), (ma, mb)
+        assert (ma[4:8] == mb[4:8]).all(), (ma, mb)
+        assert (ca == cb).all(), (ma[:2], ca, cb)
+
+
+def test_stack_two_steps_bit_exact(gpu):
+    say("case: Stack 60 frames then 2 GPU steps vs oracle f32")
+    ref = oracle.RefSolver(9)
+    ref.step(60)
+    bodies = records.bodies_from_oracle(ref.body_records())
+    kw = dict(gravity=-10.0, flags=0, color_cap=64, lanes=LANES1)
+    s = gpu_for(bodies, kw)
+    m = model_for(bodies, kw)
+    for _ in range(2):
+        s.step()
+        m.step(1)
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    assert_same_contacts(s, m)
+    s.destroy()
+
+
+def test_seeded_single_step_matches_reference(gpu):
+    tie_frames = 0
+    for name, sid in PARITY_SCENES.items():
+        ref = oracle.RefSolver(sid)
+        frame = 0
+        for at in (60, 120, 300):
+            say(f"case: seeded step {name} @{at}")
+            ref.step(at - frame)
+            frame = at
+            bodies64 = ref.body_records()
+            jrec, jinfo = ref.joint_records()
+            man, con = ref.manifold_records()
+            ign = [(int(a), int(b)) for k, a, b, _ in ref.forces() if k == 3]
+            bodies = records.bodies_from_oracle(bodies64)
+            kw = dict(gravity=-10.0, flags=0, color_cap=64, lanes=LANES1)
+            s = gpu_for(bodies, kw, records.joints_from_oracle(jrec), jinfo, ignore_pairs=ign or None)
+            mu, cu = records.contacts_from_oracle(man, con)
+            s.seed_from(bodies, records.joints_from_oracle(jrec), mu, cu)
+            s.set_fixed_colors(oracle.sequential_colors(bodies64))
+            s.step()
+            m = oracle.Model(32)
+            oracle.seed_model_from_ref(m, ref, oracle.make_params(gravity=-10.0, flags=0, color_cap=64, lanes=LANES1))
+            m.set_fixed_colors(oracle.sequential_colors(bodies64))
+            m.step(1)
+            ref.step(1)
+            frame += 1
+            c = s.read_counters()
+            assert c["overflow"] == 0
+            # bit-exact with the f32 restatement
+            got = records.bodies_to_oracle(s.read_bodies())
+            assert np.array_equal(got, m.bodies()), f"{name} @{at}: GPU != oracle f32"
+            assert_same_contacts(s, m)
+            # the reference's tolerances against the f64 reference
+            rm, rc = ref.manifold_records()
+            gm, gc = records.contacts_to_oracle(*s.read_contacts_raw())
+            expected, found, keymis, lam_ok = contact_match(rm, rc, gm, gc, bodies64, lam_tol=2e-3)
+            assert found == expected and len(gc) == expected, (name, at)
+            s.destroy()
+            if keymis:
+                tie_frames += 1
+                continue
+            assert lam_ok
+            p = ref.poses()
+            d = max(np.abs(got[:, 0:3] - p[:, 0:3]).max(), np.abs(got[:, 4:8] - p[:, 3:7]).max())
+            assert d < 1e-5, (name, at, d)
+    assert tie_frames <= 2
+
+
+def test_broadphase_pairs_exact(gpu):
+    say("case: broadphase 3000 random boxes")
+    ref = random_broadphase_scene(3000)
+    bodies64 = ref.body_records()
+    bodies = records.bodies_from_oracle(bodies64)
+    kw = dict(iterations=0, lanes=LANES1)
+    s = gpu_for(bodies, kw, ignore_pairs=[(2, 1)])
+    s.step()
+    got = {(int(a), int(b)) for a, b in s.read_pairs()}
+    m = model_for(bodies, kw)
+    m.add_ignore(2, 1)
+    m.collide()
+    want = {(int(a), int(b)) for a, b in m.pairs()}
+    assert got == want
+    assert got == brute_force_pairs(bodies64, ignore=[(2, 1)])
+    s.destroy()
+
+
+@pytest.mark.parametrize("face_bias", [False, True])
+def test_narrowphase_random_pairs_exact(gpu, face_bias):
+    say(f"case: narrowphase 300 random box pairs face_bias={face_bias}")
+    ref = random_box_pairs(300)
+    bodies64 = ref.body_records()
+    bodies = records.bodies_from_oracle(bodies64)
+    kw = dict(iterations=0, gravity=0.0, flags=(2 if face_bias else 0), lanes=LANES1)
+    s = gpu_for(bodies, kw)
+    s.step()
+    m = model_for(bodies, kw)
+    m.collide()
+    assert_same_contacts(s, m)
+    if not face_bias:
+        # and the reference's own check against its CPU collide (features, points within 1e-3)
+        gm, gc = records.contacts_to_oracle(*s.read_contacts_raw())
+        by = {(int(x[0]), int(x[1])): x for x in gm}
+        total = 0
+        for p in range(300):
+            basis, cpu = ref.collide(2 * p + 1, 2 * p)
+            g = by.get((2 * p + 1, 2 * p))
+            n = 0 if g is None else int(g[3]) & 15
+            assert n == len(cpu), p
+            total += n
+            for row in cpu:
+                ks = [k for k in range(int(g[2]), int(g[2]) + n)
+                      if np.abs(gc[k, 0:3] - row[1:4]).max() < 1e-3 and np.abs(gc[k, 4:7] - row[4:7]).max() < 1e-3]
+                assert ks, p
+                assert (int(gc[ks[0], 3]) & 0x7FFFFFFF) == int(row[0])
+    s.destroy()
+
+
+def test_pile_steps_bit_exact_one_lane(gpu):
+    say("case: pile 400 bodies 40 steps, heuristics on, lanes 1")
+    bodies, _, _ = scenes.random_pile(400, seed=3, bin_size=10.0)
+    kw = dict(lanes=LANES1, color_cap=64)
+    s = gpu_for(bodies, kw)
+    m = model_for(bodies, kw)
+    for k in range(40):
+        s.step()
+        m.step(1)
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    assert_same_contacts(s, m)
+    state, _ = s.read_colors()
+    assert np.array_equal(state, m.colors())
+    s.destroy()
+
+
+def test_pile_lane_split_matches_restatement(gpu):
+    say("case: pile 400 bodies 10 steps, 2/4/8 lanes per body")
+    bodies, _, _ = scenes.random_pile(400, seed=4, bin_size=10.0)
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=64)
+    s = gpu_for(bodies, kw)
+    m = model_for(bodies, kw)
+    for _ in range(10):
+        s.step()
+        m.step(1)
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    s.destroy()
+
+
+def test_determinism_twice(gpu):
+    say("case: determinism pile 1000 x 30 steps twice")
+    bodies, _, _ = scenes.random_pile(1000, seed=9, bin_size=14.0)
+    outs = []
+    for _ in range(2):
+        s = gpu_for(bodies, dict(color_cap=16, lanes=(2 ** 31, 2 ** 15, 2 ** 12)))
+        for _ in range(30):
+            s.step()
+        outs.append(s.read_bodies().copy())
+        s.destroy()
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))

@@ -1,0 +1,329 @@
+"""Host-side mirror of the reference's ``GpuSolver3D`` (src/avbd3d/gpu/solver.ts:504) over the
+C ABI of libavbd_mi355x.so (include/avbd.h).
+
+Same names, argument meaning and error behaviour as the reference's operational surface:
+``step``, ``params``, ``read_bodies``/``read_joints``/``read_counters``/``read_contact_list``/
+``read_pairs``, ``seed_from``, ``fixed_colors``/``sequential_colors``, ``profile_next_step``,
+``destroy``. Reads return numpy arrays in the reference's record formats (layout.ts). Invalid
+input raises ``ValueError`` (the reference's ``Error``), capacity refusals raise
+``StorageCapacityError``; overflow during a step is counted in ``read_counters()['overflow']``,
+not raised. The library must be built (``python -c "import __graft_entry__ as g; g.build()"``):
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libavbd_mi355x.so")
+
+FLAG_MATCH_NEAREST = 1
+FLAG_FACE_BIAS = 2
+FLAG_REUSE_CONTACTS = 4
+FLAG_START_AT_REST = 8
+FLAG_MASS_PENALTY = 16
+FLAG_DEFAULTS = 31
+NO_COLOR = 255
+STICK_BIT = 0x80000000
+BODY_FLOATS = 40
+JOINT_FLOATS = 40
+
+# record offsets (layout.ts:49-58, :84-110)
+B_POS, B_ROT, B_INITIAL_POS, B_INITIAL_ROT, B_SIZE, B_MOMENT, B_INERTIAL_POS, B_INERTIAL_ROT, B_VEL, B_ANGVEL = range(0, 40, 4)
+J_PEN_LIN, J_PEN_ANG, J_LAM_LIN, J_LAM_ANG, J_C0_LIN, J_C0_ANG, J_RA, J_RB, J_REST, J_REST_START = range(0, 40, 4)
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_SAIL, SHAPE_HULL = 0, 1, 2, 3
+T_NONE, T_JOINT, T_SPRING = 0, 1, 2
+BIG = 3.0e38
+
+
+class StorageCapacityError(RuntimeError):
+    """A capacity refused before anything changed (storage-budget.ts StorageCapacityError)."""
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_float), ("gravity", ctypes.c_float), ("beta_lin", ctypes.c_float), ("beta_ang", ctypes.c_float),
+        ("gamma", ctypes.c_float), ("alpha", ctypes.c_float), ("contact_margin", ctypes.c_float), ("up", ctypes.c_float * 3),
+        ("flags", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("color_cap", ctypes.c_uint32),
+        ("color_rounds", ctypes.c_uint32), ("primal_lanes", ctypes.c_uint32 * 3),
+    ]
+
+
+class Capacity(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in ("bodies", "joints", "pairs", "manifolds", "contacts", "hull_words")]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "color_cap")]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_float) for k in ("collision", "adjacency", "coloring", "solve", "total")]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [
+        ("primal_ms", ctypes.c_float), ("dual_ms", ctypes.c_float), ("narrowphase_ms", ctypes.c_float),
+        ("broadphase_ms", ctypes.c_float), ("other_ms", ctypes.c_float),
+        ("primal_launches", ctypes.c_uint32), ("dual_launches", ctypes.c_uint32),
+        ("narrowphase_launches", ctypes.c_uint32), ("broadphase_launches", ctypes.c_uint32),
+        ("primal_bytes", ctypes.c_uint64), ("dual_bytes", ctypes.c_uint64),
+    ]
+
+
+class StepOpts(ctypes.Structure):
+    _fields_ = [("reuse_manifolds", ctypes.c_int)]
+
+
+_lib = None
+
+EXPORTS = [
+    "avbd_create", "avbd_destroy", "avbd_last_error", "avbd_create_error", "avbd_step", "avbd_steps", "avbd_set_params",
+    "avbd_get_params", "avbd_read_bodies", "avbd_write_bodies", "avbd_read_joints", "avbd_read_counters",
+    "avbd_read_contacts", "avbd_read_pairs", "avbd_read_colors", "avbd_seed", "avbd_set_fixed_colors",
+    "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
+    "avbd_profile_kernels", "avbd_time_steps", "avbd_sync",
+]
+
+
+def lib():
+    """The native library; raises if it is not built (the path has no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        L.avbd_last_error.restype = ctypes.c_char_p
+        L.avbd_create_error.restype = ctypes.c_char_p
+        vp = ctypes.c_void_p
+        L.avbd_create.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp, ctypes.POINTER(vp)]
+        for name in EXPORTS[1:]:
+            fn = getattr(L, name)
+            if name not in ("avbd_last_error", "avbd_create_error"):
+                fn.restype = ctypes.c_int
+        L.avbd_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_params(dt=1 / 60, gravity=-9.81, iterations=10, alpha=0.99, beta_lin=1e4, beta_ang=100.0, gamma=0.999,
+                   flags=FLAG_DEFAULTS, color_cap=64, color_rounds=16, margin=0.01, up=(0.0, 0.0, 1.0),
+                   primal_lanes=(2 ** 31, 2 ** 15, 2 ** 12)):
+    """gpuParams3D (solver.ts:213-226) with z up, as a solver built from the reference's scenes."""
+    p = Params()
+    p.dt, p.gravity, p.beta_lin, p.beta_ang, p.gamma, p.alpha, p.contact_margin = dt, gravity, beta_lin, beta_ang, gamma, alpha, margin
+    p.up[0], p.up[1], p.up[2] = up
+    p.flags, p.iterations, p.color_cap, p.color_rounds = flags, iterations, color_cap, color_rounds
+    p.primal_lanes[0], p.primal_lanes[1], p.primal_lanes[2] = primal_lanes
+    return p
+
+
+class GpuSolver3D:
+    """The MI355X AVBD step behind the C ABI, with GpuSolver3D's method names."""
+
+    def __init__(self, bodies, joints=None, joint_info=None, params=None, capacity=None, hulls=None, ignore_pairs=None):
+        self.L = lib()
+        bodies = np.ascontiguousarray(bodies, dtype=np.float32).reshape(-1, BODY_FLOATS)
+        if joints is None or len(joints) == 0:
+            joints = np.zeros((0, JOINT_FLOATS), np.float32)
+            joint_info = np.zeros((0, 4), np.int32)
+        joints = np.ascontiguousarray(joints, dtype=np.float32).reshape(-1, JOINT_FLOATS)
+        joint_info = np.ascontiguousarray(joint_info, dtype=np.int32).reshape(-1, 4)
+        self._params = params if params is not None else default_params()
+        cap = Capacity()
+        for k, v in (capacity or {}).items():
+            setattr(cap, k, int(v))
+        if hulls is not None and cap.hull_words < len(hulls):
+            cap.hull_words = len(hulls)
+        h = ctypes.c_void_p()
+        rc = self.L.avbd_create(ctypes.byref(self._params), _p(bodies), len(bodies), _p(joints), _p(joint_info), len(joints),
+                                ctypes.byref(cap), ctypes.byref(h))
+        if rc != 0:
+            msg = self.L.avbd_create_error().decode()
+            raise (ValueError if rc == -1 else RuntimeError)(msg)
+        self.h = h
+        self.body_count = len(bodies)
+        self.joint_count = len(joints)
+        self.fixed_colors = None
+        self.destroyed = False
+        if hulls is not None and len(hulls):
+            self._check(self.L.avbd_set_hulls(self.h, _p(np.ascontiguousarray(hulls, np.uint32)), len(hulls)))
+        if ignore_pairs is not None and len(ignore_pairs):
+            pairs = np.ascontiguousarray(ignore_pairs, np.uint32).reshape(-1, 2)
+            self._check(self.L.avbd_add_ignore_pairs(self.h, _p(pairs), len(pairs)))
+
+    # --- lifecycle ---------------------------------------------------------------------------
+    def _check(self, rc):
+        if rc >= 0:
+            return rc
+        msg = self.L.avbd_last_error(self.h).decode()
+        if rc == -2:
+            raise StorageCapacityError(msg)
+        if rc == -1:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+    def destroy(self):
+        if not self.destroyed and getattr(self, "h", None):
+            self.L.avbd_destroy(self.h)
+        self.destroyed = True
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _live(self):
+        if self.destroyed:
+            raise RuntimeError("Solver is destroyed")
+
+    # --- params ------------------------------------------------------------------------------
+    @property
+    def params(self):
+        return self._params
+
+    def set_params(self, params):
+        self._live()
+        self._check(self.L.avbd_set_params(self.h, ctypes.byref(params)))
+        self._params = params
+
+    # --- stepping ----------------------------------------------------------------------------
+    def step(self, reuse_manifolds=False):
+        """GpuSolver3D.step (solver.ts:2112): one substep, enqueued on the solver's stream."""
+        self._live()
+        self._check(self.L.avbd_step(self.h, ctypes.byref(StepOpts(int(reuse_manifolds)))))
+
+    def steps(self, steps, substeps=1):
+        self._live()
+        self._check(self.L.avbd_steps(self.h, steps, substeps))
+
+    def sync(self):
+        self._check(self.L.avbd_sync(self.h))
+
+    def time_steps(self, steps, substeps=1):
+        ms = ctypes.c_float()
+        self._check(self.L.avbd_time_steps(self.h, steps, substeps, ctypes.byref(ms)))
+        return ms.value
+
+    def profile_next_step(self, reuse_manifolds=False):
+        out = Profile()
+        self._check(self.L.avbd_profile_step(self.h, ctypes.byref(StepOpts(int(reuse_manifolds))), ctypes.byref(out)))
+        return {k: getattr(out, k) for k, _ in Profile._fields_}
+
+    def profile_kernels(self, steps, substeps=1):
+        out = KernelStats()
+        self._check(self.L.avbd_profile_kernels(self.h, steps, substeps, ctypes.byref(out)))
+        return {k: getattr(out, k) for k, _ in KernelStats._fields_}
+
+    # --- reads -------------------------------------------------------------------------------
+    def read_bodies(self, first=0, count=None):
+        self._live()
+        count = self.body_count - first if count is None else count
+        out = np.zeros((count, BODY_FLOATS), np.float32)
+        self._check(self.L.avbd_read_bodies(self.h, _p(out), first, count))
+        return out
+
+    def write_bodies(self, records, first=0):
+        """rewriteBodies (solver.ts:1774)."""
+        self._live()
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, BODY_FLOATS)
+        self._check(self.L.avbd_write_bodies(self.h, _p(rec), first, len(rec)))
+
+    def read_joints(self, first=0, count=None):
+        self._live()
+        count = self.joint_count - first if count is None else count
+        out = np.zeros((count, JOINT_FLOATS), np.float32)
+        self._check(self.L.avbd_read_joints(self.h, _p(out), first, count))
+        return out
+
+    def read_counters(self):
+        self._live()
+        c = Counters()
+        self._check(self.L.avbd_read_counters(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+    def read_contacts_raw(self):
+        """Manifold records (M, 8) u32 and contact records (K, 16) u32 of the last step."""
+        c = self.read_counters()
+        man = np.zeros((max(c["manifolds"], 1), 8), np.uint32)
+        con = np.zeros((max(c["contacts"], 1), 16), np.uint32)
+        nm = self._check(self.L.avbd_read_contacts(self.h, _p(man), len(man), _p(con), len(con)))
+        return man[:nm], con[:c["contacts"]]
+
+    def read_contact_list(self):
+        """readContactList (solver.ts:2311-2346): one dict per contact point."""
+        man, con = self.read_contacts_raw()
+        conf = con.view(np.float32)
+        out = []
+        for m in man:
+            a, b, first, cs = (int(x) for x in m[:4])
+            for c in range(first, first + (cs & 15)):
+                out.append({
+                    "a": a, "b": b, "feature": int(con[c, 3]) & ~STICK_BIT, "stick": bool(int(con[c, 3]) & STICK_BIT),
+                    "rA": conf[c, 0:3].copy(), "rB": conf[c, 4:7].copy(), "pen": conf[c, 8:11].copy(),
+                    "lam": conf[c, 12:15].copy(), "c0": np.array([conf[c, 7], conf[c, 11], conf[c, 15]]),
+                    "normal": m[4:7].view(np.float32).copy(),
+                })
+        return out
+
+    def read_pairs(self):
+        c = self.read_counters()
+        out = np.zeros((max(c["pairs"], 1), 2), np.uint32)
+        n = self._check(self.L.avbd_read_pairs(self.h, _p(out), len(out)))
+        return out[:n]
+
+    def read_colors(self):
+        state = np.zeros(self.body_count, np.uint32)
+        starts = np.zeros(65, np.uint32)
+        self._check(self.L.avbd_read_colors(self.h, _p(state), self.body_count, _p(starts)))
+        return state, starts
+
+    # --- validation hooks --------------------------------------------------------------------
+    def seed_from(self, bodies, joints, manifolds, contacts):
+        """seedFrom (solver.ts:1320-1359): bodies, joints with warm-start state, last contacts."""
+        self._live()
+        b = np.ascontiguousarray(bodies, np.float32).reshape(-1, BODY_FLOATS)
+        j = np.ascontiguousarray(joints, np.float32).reshape(-1, JOINT_FLOATS) if joints is not None else np.zeros((0, 40), np.float32)
+        m = np.ascontiguousarray(manifolds, np.uint32).reshape(-1, 8)
+        c = np.ascontiguousarray(contacts, np.uint32).reshape(-1, 16)
+        self._check(self.L.avbd_seed(self.h, _p(b), len(b), _p(j), len(j), _p(m), len(m), _p(c), len(c)))
+
+    def set_fixed_colors(self, colors):
+        """fixedColors (solver.ts:560): colours per body; None returns to the colouring."""
+        self._live()
+        if colors is None:
+            self._check(self.L.avbd_set_fixed_colors(self.h, None, 0))
+            self.fixed_colors = None
+            return
+        c = np.ascontiguousarray(colors, np.uint32)
+        self._check(self.L.avbd_set_fixed_colors(self.h, _p(c), len(c)))
+        self.fixed_colors = c
+
+    def sequential_colors(self):
+        """sequentialColors (solver.ts:1365-1372): newest dynamic body first, one colour each."""
+        rec = self.read_bodies()
+        colors = np.full(self.body_count, NO_COLOR, np.uint32)
+        nxt = 0
+        for i in range(self.body_count - 1, -1, -1):
+            if rec[i, B_SIZE + 3] > 0:
+                colors[i] = nxt
+                nxt += 1
+        if nxt > 64:
+            raise ValueError(f"sequentialColors: {nxt} dynamic bodies, at most 64")
+        return colors
+
+    def set_color_state(self, state):
+        s = np.ascontiguousarray(state, np.uint32)
+        self._check(self.L.avbd_set_color_state(self.h, _p(s), len(s)))
+
+    def set_filters(self, groups, collides_with):
+        g = np.ascontiguousarray(groups, np.uint32)
+        w = np.ascontiguousarray(collides_with, np.uint32)
+        self._check(self.L.avbd_set_filters(self.h, _p(g), _p(w), len(g)))

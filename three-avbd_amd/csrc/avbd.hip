@@ -1,0 +1,688 @@
+// Host side of the MI355X AVBD step: device storage, the step's launch sequence and the C ABI
+// declared in include/avbd.h. Mirrors GpuSolver3D (src/avbd3d/gpu/solver.ts:504): construction
+// (:722-851), step (:2112-2293), readbacks (:2297-2352), seedFrom (:1320), fixedColors (:560),
+// uploadStatics (:1263) and its large-body rule (radii.ts:47).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/avbd.h"
+#include "avbd_kernels.h"
+
+using namespace avbd;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct KernelTimer {
+  bool on = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;  // kind, (start, stop)
+};
+
+enum Kind { K_PRIMAL = 0, K_DUAL = 1, K_NARROW = 2, K_BROAD = 3, K_OTHER = 4 };
+
+uint32_t pow2At(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return (uint32_t)p;
+}
+uint32_t blocksFor(uint64_t n, uint32_t threads, uint32_t cap = 16384) {
+  uint64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (uint32_t)b;
+}
+float finiteF(float x) { return x > 3.0e38f ? 3.0e38f : (x < -3.0e38f ? -3.0e38f : x); }
+
+/** The k-th smallest (Wirth's selection, radii.ts:21-43), reordering a. */
+double selectK(std::vector<double> &a, size_t k) {
+  size_t lo = 0, hi = a.size() - 1;
+  while (lo < hi) {
+    double pivot = a[k];
+    size_t i = lo, j = hi;
+    do {
+      while (a[i] < pivot) i++;
+      while (pivot < a[j]) j--;
+      if (i <= j) { std::swap(a[i], a[j]); i++; if (j == 0) break; j--; }
+    } while (i <= j);
+    if (j < k) lo = i;
+    if (k < i) hi = j;
+  }
+  return a[k];
+}
+/** largestSmall (radii.ts:47-55): the largest radius that still sizes the grid cells. */
+double largestSmall(std::vector<double> radii) {
+  size_t n = radii.size();
+  double median = n > 0 ? selectK(radii, n >> 1) : 1.0;
+  double maxSmall = 0;
+  for (double r : radii) if (r <= 2 * median) maxSmall = std::max(maxSmall, r);
+  if (n > 64) maxSmall = std::max(maxSmall, selectK(radii, n - 64 - 1));
+  return maxSmall;
+}
+
+}  // namespace
+
+struct avbd_solver {
+  hipStream_t stream = nullptr;
+  avbd_params hp{};
+  uint32_t n = 0, nj = 0, bodyCap = 0, jointCap = 0, pairCap = 0, manCap = 0, conCap = 0, hullCap = 0;
+  uint32_t tableSize = 0, hashSize = 0, colorGroups = 0, adjCap = 0;
+  float cellSize = 1, largeRadius = 1e30f;
+  std::vector<uint32_t> large;
+  struct NC { uint32_t hi, lo, slot; };
+  std::vector<NC> ignorePairs;  // IgnoreCollision entries
+  std::vector<int32_t> jinfoHost;
+  uint32_t noCollideCount = 0;
+  int parity = 0;
+  uint32_t stepCount = 0;
+  bool fixedColors = false;
+  uint32_t fixedCap = 1;
+  bool manifoldsValid = false;
+  bool anyDynamicLarge = false;
+  bool destroyed = false;
+  std::string err;
+  std::vector<DevBuf> bufs;
+  // device storage
+  Bodies b{};
+  float4 *refPos = nullptr, *refRot = nullptr;
+  uint2 *filters = nullptr;
+  float4 *joints = nullptr;
+  int4 *jinfo = nullptr;
+  uint32_t *statics = nullptr;
+  uint32_t staticsCap = 0;
+  uint4 *hulls = nullptr;
+  uint32_t *cellCount = nullptr, *cellStart = nullptr, *cellCursor = nullptr, *cellSorted = nullptr;
+  int4 *cellOf = nullptr;
+  float4 *sortedPos = nullptr;
+  uint2 *pairs = nullptr;
+  uint4 *mIds[2] = {nullptr, nullptr};
+  float4 *mGeo[2] = {nullptr, nullptr};
+  Contact *con[2] = {nullptr, nullptr};
+  uint32_t *table = nullptr;
+  uint32_t *adjCount = nullptr, *adjStart = nullptr, *adjFill = nullptr, *adjList = nullptr;
+  uint32_t *colA = nullptr, *colB = nullptr, *colorStart = nullptr, *colorBodies = nullptr, *colorRank = nullptr, *hist = nullptr;
+  uint32_t *fixedDev = nullptr;
+  uint32_t *counters = nullptr;
+  uint32_t *scanSums = nullptr;
+  float4 *staging = nullptr;
+  size_t stagingBytes = 0;
+  KernelTimer timer;
+
+  template <class T> T *alloc(size_t count) {
+    DevBuf d;
+    d.bytes = std::max<size_t>(count * sizeof(T), 256);
+    if (hipMalloc(&d.p, d.bytes) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    hipMemsetAsync(d.p, 0, d.bytes, stream);
+    bufs.push_back(d);
+    return (T *)d.p;
+  }
+  ~avbd_solver() {
+    for (auto &d : bufs) hipFree(d.p);
+    if (staging) hipFree(staging);
+    for (auto &m : timer.marks) { hipEventDestroy(m.second.first); hipEventDestroy(m.second.second); }
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  Params params() const {
+    Params p;
+    p.dt = hp.dt; p.gravity = hp.gravity; p.betaLin = hp.beta_lin; p.betaAng = hp.beta_ang; p.gamma = hp.gamma; p.alpha = hp.alpha;
+    p.margin = hp.contact_margin; p.upx = hp.up[0]; p.upy = hp.up[1]; p.upz = hp.up[2];
+    p.flags = hp.flags; p.iterations = hp.iterations; p.colorCap = fixedColors ? fixedCap : hp.color_cap; p.rounds = hp.color_rounds;
+    p.step = stepCount;
+    p.lanes1 = hp.primal_lanes[0]; p.lanes2 = hp.primal_lanes[1]; p.lanes4 = hp.primal_lanes[2];
+    return p;
+  }
+  Dev dev() const {
+    Dev d;
+    d.n = n; d.nj = nj; d.b = b; d.refPos = refPos; d.refRot = refRot; d.filters = filters; d.joints = joints; d.jinfo = jinfo;
+    d.statics = statics; d.largeCount = (uint32_t)large.size(); d.noCollideCount = noCollideCount; d.hulls = hulls;
+    d.cellSize = cellSize; d.largeRadius = largeRadius; d.tableMask = tableSize - 1;
+    d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos;
+    d.pairs = pairs; d.pairCap = pairCap;
+    d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
+    d.con = con[parity]; d.pcon = con[1 - parity]; d.manCap = manCap; d.conCap = conCap;
+    d.table = table; d.hashMask = hashSize - 1;
+    d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
+    d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
+    d.colorGroups = colorGroups; d.counters = counters; d.p = params();
+    return d;
+  }
+  hipEvent_t mark(int kind) {
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, stream);
+    timer.marks.push_back({kind, {e0, e1}});
+    return e1;
+  }
+  template <class F> void timed(int kind, F launch) {
+    if (!timer.on) { launch(); return; }
+    hipEvent_t e1 = mark(kind);
+    launch();
+    hipEventRecord(e1, stream);
+  }
+  void fill(uint32_t *p, uint32_t count, uint32_t v) { k_fill<<<blocksFor(count, 256), 256, 0, stream>>>(p, count, v); }
+  /** Exclusive scan of count words (in may equal out). */
+  void scan(const uint32_t *in, uint32_t *out, uint32_t count) {
+    uint32_t nb = (count + 1023) / 1024;
+    if (nb == 0) return;
+    k_scan_local<<<nb, 256, 0, stream>>>(in, out, scanSums, count);
+    k_scan_sums<<<1, 1024, 0, stream>>>(scanSums, nb);
+    k_scan_add<<<nb, 256, 0, stream>>>(out, scanSums, count);
+  }
+
+  /** uploadStatics (solver.ts:1263-1305): cell size, large list and sorted no-collide list. */
+  void uploadStatics(const std::vector<float> &radius, const std::vector<float> &mass) {
+    std::vector<double> r(radius.begin(), radius.end());
+    double maxSmall = n ? largestSmall(r) : 1.0;
+    double minLarge = INFINITY;
+    for (double x : r) if (x > maxSmall) minLarge = std::min(minLarge, x);
+    double threshold = std::isinf(minLarge) ? maxSmall * 1.5 : (maxSmall + minLarge) / 2;
+    largeRadius = (float)threshold;
+    cellSize = (float)std::max(2 * maxSmall * (1 + 1e-4), 1e-3);
+    large.clear();
+    anyDynamicLarge = false;
+    for (uint32_t i = 0; i < n; i++) if (radius[i] > largeRadius) { large.push_back(i); if (mass[i] > 0) anyDynamicLarge = true; }
+    std::vector<NC> nc = ignorePairs;
+    for (uint32_t j = 0; j < nj; j++) {
+      int a = jinfoHost[4 * j + 1], bb = jinfoHost[4 * j + 2];
+      if (jinfoHost[4 * j] != (int)T_NONE && a >= 0) nc.push_back({(uint32_t)std::max(a, bb), (uint32_t)std::min(a, bb), j});
+    }
+    std::stable_sort(nc.begin(), nc.end(), [](const NC &x, const NC &y) { return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo); });
+    noCollideCount = (uint32_t)nc.size();
+    std::vector<uint32_t> words(large.begin(), large.end());
+    for (auto &e : nc) { words.push_back(e.hi); words.push_back(e.lo); words.push_back(e.slot); }
+    if (words.size() > staticsCap) {
+      staticsCap = (uint32_t)words.size() * 2 + 16;
+      statics = alloc<uint32_t>(staticsCap);
+    }
+    if (!words.empty()) hipMemcpyAsync(statics, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream);
+  }
+
+  void collision(bool reuseManifolds) {
+    Dev d = dev();
+    k_begin<<<1, 1, 0, stream>>>(d, reuseManifolds ? 1 : 0);
+    if (hp.flags & FLAG_REUSE_CONTACTS) k_update_refs<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    if (reuseManifolds) {
+      timed(K_NARROW, [&] { k_reuse_manifolds<<<blocksFor(manCap, 256), 256, 0, stream>>>(d); });
+      return;
+    }
+    fill(table, hashSize, 0);
+    k_hash_insert<<<blocksFor(manCap, 256), 256, 0, stream>>>(d);
+    timed(K_BROAD, [&] {
+      fill(cellCount, tableSize + 1, 0);
+      fill(cellCursor, tableSize, 0);
+      k_grid_count<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+      scan(cellCount, cellStart, tableSize + 1);
+      k_grid_scatter<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+      k_find_pairs<<<blocksFor(n, 64), 64, 0, stream>>>(d);
+      if (anyDynamicLarge) k_find_fixed_large<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    });
+    timed(K_NARROW, [&] { k_narrowphase<<<blocksFor(pairCap, 64), 64, 0, stream>>>(d); });
+  }
+  void topology() {
+    Dev d = dev();
+    fill(adjCount, n + 1, 0);
+    fill(adjFill, n, 0);
+    k_degree<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d);
+    scan(adjCount, adjStart, n + 1);
+    k_fill_adj<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d);
+    k_sort_adj<<<blocksFor(n, 64), 64, 0, stream>>>(d);
+    if (fixedColors) {
+      hipMemcpyAsync(colA, fixedDev, n * 4, hipMemcpyDeviceToDevice, stream);
+    } else {
+      k_color_compact<<<blocksFor(n, 64), 64, 0, stream>>>(d);
+      k_color_mark<<<blocksFor(n, 64), 64, 0, stream>>>(d);
+      for (uint32_t r = 0; r < hp.color_rounds; r++) {
+        if (r % 2 == 0) k_color_round<<<blocksFor(n, 64), 64, 0, stream>>>(d, colA, colB);
+        else k_color_round<<<blocksFor(n, 64), 64, 0, stream>>>(d, colB, colA);
+      }
+    }
+    uint32_t groups = (n + 255) / 256;
+    if (groups == 0) groups = 1;
+    k_color_count<<<groups, 256, 0, stream>>>(d);
+    scan(hist, hist, 64 * colorGroups + 1);
+    k_color_starts<<<1, 64, 0, stream>>>(d);
+    k_color_scatter<<<groups, 256, 0, stream>>>(d);
+  }
+  void solve() {
+    Dev d = dev();
+    if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
+    k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
+    uint32_t pb = blocksFor(n, 64);
+    for (uint32_t it = 0; it < hp.iterations; it++) {
+      for (uint32_t c = 0; c < cap; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
+      timed(K_DUAL, [&] { k_dual<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d); });
+    }
+    k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+  }
+  void step(bool reuseManifolds) {
+    stepCount++;
+    reuseManifolds = reuseManifolds && manifoldsValid;
+    collision(reuseManifolds);
+    topology();
+    solve();
+    parity = 1 - parity;
+    manifoldsValid = true;
+  }
+};
+
+// Fix the size of a scan's temporary: blocks of 1024 words, at most 4096 of them.
+static const uint32_t MAX_SCAN = 4096u * 1024u;
+
+static int fail(avbd_solver *s, int code, const std::string &msg) {
+  if (s) s->err = msg;
+  return code;
+}
+static bool checkParams(const avbd_params *p, std::string &why) {
+  if (!p) { why = "params is null"; return false; }
+  if (!(p->dt > 0) || !std::isfinite(p->dt)) { why = "dt must be finite and > 0"; return false; }
+  if (!std::isfinite(p->contact_margin) || p->contact_margin < 0) { why = "Invalid contactMargin"; return false; }
+  if (p->color_cap < 1 || p->color_cap > 64) { why = "color_cap must be in [1, 64]"; return false; }
+  if (p->color_rounds % 2 != 0) { why = "color_rounds must be even"; return false; }
+  return true;
+}
+
+extern "C" {
+
+const char *avbd_create_error(void) { return g_create_error.c_str(); }
+const char *avbd_last_error(const avbd_solver *s) { return s ? s->err.c_str() : "null solver"; }
+
+int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_count, const float *jrec, const int32_t *jinfo,
+                uint32_t joint_count, const avbd_capacity *capacity, avbd_solver **out) {
+  g_create_error.clear();
+  if (!out) { g_create_error = "out is null"; return AVBD_ERR_INVALID; }
+  *out = nullptr;
+  std::string why;
+  if (!checkParams(params, why)) { g_create_error = why; return AVBD_ERR_INVALID; }
+  if (body_count > 0 && !bodies) { g_create_error = "bodies is null"; return AVBD_ERR_INVALID; }
+  if (joint_count > 0 && (!jrec || !jinfo)) { g_create_error = "joints or joint_info is null"; return AVBD_ERR_INVALID; }
+  for (uint32_t j = 0; j < joint_count; j++) {
+    int t = jinfo[4 * j], a = jinfo[4 * j + 1], b = jinfo[4 * j + 2];
+    if (t < 0 || t > 2 || b < 0 || (uint32_t)b >= body_count || a < -1 || a >= (int)body_count || (t == 2 && a < 0)) {
+      g_create_error = "joint " + std::to_string(j) + ": invalid type or bodies";
+      return AVBD_ERR_INVALID;
+    }
+  }
+  avbd_capacity cap = capacity ? *capacity : avbd_capacity{};
+  avbd_solver *s = new avbd_solver();
+  try {
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
+    s->hp = *params;
+    s->n = body_count;
+    s->nj = joint_count;
+    s->bodyCap = std::max({cap.bodies, body_count, 1u});
+    s->jointCap = std::max({cap.joints, joint_count, 1u});
+    s->pairCap = cap.pairs ? cap.pairs : std::max(4096u, 8u * s->bodyCap);
+    s->manCap = cap.manifolds ? cap.manifolds : s->pairCap;
+    s->conCap = cap.contacts ? cap.contacts : 4u * s->manCap;
+    s->hullCap = std::max(cap.hull_words, 4u);
+    s->tableSize = pow2At(2ull * s->bodyCap);
+    s->hashSize = pow2At(2ull * s->manCap);
+    s->colorGroups = (s->bodyCap + 255) / 256;
+    s->adjCap = 2u * (s->jointCap + s->manCap);
+    if (s->tableSize + 1 > MAX_SCAN || 64ull * s->colorGroups + 1 > MAX_SCAN) throw std::runtime_error("body capacity too large for one solver");
+    uint32_t N = s->bodyCap;
+    float4 **streams[10] = {&s->b.pos, &s->b.rot, &s->b.ipos, &s->b.irot, &s->b.size, &s->b.moment, &s->b.inpos, &s->b.inrot, &s->b.vel, &s->b.angvel};
+    for (auto p : streams) *p = s->alloc<float4>(N);
+    s->refPos = s->alloc<float4>(N);
+    s->refRot = s->alloc<float4>(N);
+    s->filters = s->alloc<uint2>(N);
+    s->joints = s->alloc<float4>(10ull * s->jointCap);
+    s->jinfo = s->alloc<int4>(s->jointCap);
+    s->hulls = s->alloc<uint4>(s->hullCap / 4 + 1);
+    s->cellCount = s->alloc<uint32_t>(s->tableSize + 1);
+    s->cellStart = s->alloc<uint32_t>(s->tableSize + 1);
+    s->cellCursor = s->alloc<uint32_t>(s->tableSize);
+    s->cellSorted = s->alloc<uint32_t>(N);
+    s->cellOf = s->alloc<int4>(N);
+    s->sortedPos = s->alloc<float4>(N);
+    s->pairs = s->alloc<uint2>(s->pairCap);
+    for (int k = 0; k < 2; k++) {
+      s->mIds[k] = s->alloc<uint4>(s->manCap);
+      s->mGeo[k] = s->alloc<float4>(s->manCap);
+      s->con[k] = s->alloc<Contact>(s->conCap);
+    }
+    s->table = s->alloc<uint32_t>(s->hashSize);
+    s->adjCount = s->alloc<uint32_t>(N + 1);
+    s->adjStart = s->alloc<uint32_t>(N + 1);
+    s->adjFill = s->alloc<uint32_t>(N);
+    s->adjList = s->alloc<uint32_t>(s->adjCap);
+    s->colA = s->alloc<uint32_t>(N);
+    s->colB = s->alloc<uint32_t>(N);
+    s->colorStart = s->alloc<uint32_t>(65);
+    s->colorBodies = s->alloc<uint32_t>(N);
+    s->colorRank = s->alloc<uint32_t>(N);
+    s->hist = s->alloc<uint32_t>(64ull * s->colorGroups + 1);
+    s->fixedDev = s->alloc<uint32_t>(N);
+    s->counters = s->alloc<uint32_t>(C_WORDS);
+    s->scanSums = s->alloc<uint32_t>(4096);
+    s->fill(s->colA, N, NO_COLOR);
+    s->fill(s->colB, N, NO_COLOR);
+    std::vector<uint32_t> f(2ull * N, 0xffffffffu);
+    hipMemcpyAsync(s->filters, f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream);
+    // bodies: records -> SoA streams; new bodies have no contact history (DIRTY, layout.ts:17)
+    std::vector<float> radius(body_count), mass(body_count);
+    if (body_count) {
+      std::vector<float> rec(bodies, bodies + 40ull * body_count);
+      for (uint32_t i = 0; i < body_count; i++) { radius[i] = rec[40ull * i + 23]; mass[i] = rec[40ull * i + 19]; }
+      int rc = avbd_write_bodies(s, rec.data(), 0, body_count);
+      if (rc) throw std::runtime_error(s->err);
+    }
+    if (joint_count) {
+      std::vector<float> jr(jrec, jrec + 40ull * joint_count);
+      for (auto &x : jr) x = finiteF(x);
+      hipMemcpyAsync(s->joints, jr.data(), jr.size() * 4, hipMemcpyHostToDevice, s->stream);
+      s->jinfoHost.assign(jinfo, jinfo + 4ull * joint_count);
+      hipMemcpyAsync(s->jinfo, s->jinfoHost.data(), s->jinfoHost.size() * 4, hipMemcpyHostToDevice, s->stream);
+    }
+    s->uploadStatics(radius, mass);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error during create");
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    delete s;
+    return AVBD_ERR_DEVICE;
+  }
+  *out = s;
+  return AVBD_OK;
+}
+
+void avbd_destroy(avbd_solver *s) {
+  if (!s) return;
+  hipStreamSynchronize(s->stream);
+  delete s;
+}
+
+int avbd_set_params(avbd_solver *s, const avbd_params *p) {
+  if (!s) return AVBD_ERR_INVALID;
+  std::string why;
+  if (!checkParams(p, why)) return fail(s, AVBD_ERR_INVALID, why);
+  s->hp = *p;
+  return AVBD_OK;
+}
+int avbd_get_params(const avbd_solver *s, avbd_params *p) {
+  if (!s || !p) return AVBD_ERR_INVALID;
+  *p = s->hp;
+  return AVBD_OK;
+}
+
+int avbd_step(avbd_solver *s, const avbd_step_opts *opts) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  s->step(opts && opts->reuse_manifolds);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e));
+  return AVBD_OK;
+}
+int avbd_steps(avbd_solver *s, uint32_t steps, uint32_t substeps) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (substeps == 0) return fail(s, AVBD_ERR_INVALID, "substeps must be >= 1");
+  for (uint32_t k = 0; k < steps; k++)
+    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e));
+  return AVBD_OK;
+}
+int avbd_sync(avbd_solver *s) {
+  if (!s) return AVBD_ERR_INVALID;
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e));
+  return AVBD_OK;
+}
+
+static int ensureStaging(avbd_solver *s, size_t bytes) {
+  if (s->stagingBytes >= bytes) return AVBD_OK;
+  if (s->staging) hipFree(s->staging);
+  s->staging = nullptr;
+  s->stagingBytes = 0;
+  if (hipMalloc(&s->staging, bytes) != hipSuccess) return fail(s, AVBD_ERR_DEVICE, "hipMalloc staging failed");
+  s->stagingBytes = bytes;
+  return AVBD_OK;
+}
+
+int avbd_read_bodies(avbd_solver *s, float *out, uint32_t first, uint32_t count) {
+  if (!s || (!out && count)) return AVBD_ERR_INVALID;
+  if ((uint64_t)first + count > s->n) return fail(s, AVBD_ERR_INVALID, "body range out of bounds");
+  if (!count) return AVBD_OK;
+  size_t bytes = 160ull * count;
+  if (int rc = ensureStaging(s, bytes)) return rc;
+  k_bodies_to_records<<<blocksFor(count, 256), 256, 0, s->stream>>>(s->dev(), s->staging, first, count);
+  hipMemcpyAsync(out, s->staging, bytes, hipMemcpyDeviceToHost, s->stream);
+  return avbd_sync(s);
+}
+int avbd_write_bodies(avbd_solver *s, const float *in, uint32_t first, uint32_t count) {
+  if (!s || (!in && count)) return AVBD_ERR_INVALID;
+  if ((uint64_t)first + count > s->n) return fail(s, AVBD_ERR_INVALID, "body range out of bounds");
+  for (uint64_t k = 0; k < 40ull * count; k++)
+    if (!std::isfinite(in[k]) && (k % 40) != 27) return fail(s, AVBD_ERR_INVALID, "body record holds a non-finite value");
+  if (!count) return AVBD_OK;
+  size_t bytes = 160ull * count;
+  if (int rc = ensureStaging(s, bytes)) return rc;
+  hipMemcpyAsync(s->staging, in, bytes, hipMemcpyHostToDevice, s->stream);
+  k_records_to_bodies<<<blocksFor(count, 256), 256, 0, s->stream>>>(s->dev(), s->staging, first, count);
+  return avbd_sync(s);
+}
+int avbd_read_joints(avbd_solver *s, float *out, uint32_t first, uint32_t count) {
+  if (!s || (!out && count)) return AVBD_ERR_INVALID;
+  if ((uint64_t)first + count > s->nj) return fail(s, AVBD_ERR_INVALID, "joint range out of bounds");
+  if (!count) return AVBD_OK;
+  hipMemcpyAsync(out, s->joints + 10ull * first, 160ull * count, hipMemcpyDeviceToHost, s->stream);
+  return avbd_sync(s);
+}
+int avbd_read_counters(avbd_solver *s, avbd_counters *out) {
+  if (!s || !out) return AVBD_ERR_INVALID;
+  uint32_t c[C_WORDS];
+  hipMemcpyAsync(c, s->counters, sizeof(c), hipMemcpyDeviceToHost, s->stream);
+  if (int rc = avbd_sync(s)) return rc;
+  out->pairs = c[C_PAIRS];
+  out->contacts = c[C_CONTACTS];
+  out->manifolds = c[C_MANIFOLDS];
+  out->overflow = c[C_OVERFLOW];
+  out->clashes = c[C_CLASHES];
+  out->colors = c[C_NUM_COLORS];
+  out->color_cap = s->fixedColors ? s->fixedCap : s->hp.color_cap;
+  return AVBD_OK;
+}
+int avbd_read_contacts(avbd_solver *s, uint32_t *man, uint32_t mcap, uint32_t *con, uint32_t ccap) {
+  if (!s) return AVBD_ERR_INVALID;
+  avbd_counters c;
+  if (int rc = avbd_read_counters(s, &c)) return rc;
+  uint32_t nm = std::min(c.manifolds, s->manCap), nc = std::min(c.contacts, s->conCap);
+  if (nm > mcap || nc > ccap) return fail(s, AVBD_ERR_CAPACITY, "output arrays too small");
+  int last = 1 - s->parity;  // the buffers the last step wrote
+  std::vector<uint4> ids(nm);
+  std::vector<float4> geo(nm);
+  if (nm) {
+    hipMemcpyAsync(ids.data(), s->mIds[last], nm * 16ull, hipMemcpyDeviceToHost, s->stream);
+    hipMemcpyAsync(geo.data(), s->mGeo[last], nm * 16ull, hipMemcpyDeviceToHost, s->stream);
+  }
+  if (nc && con) hipMemcpyAsync(con, s->con[last], nc * 64ull, hipMemcpyDeviceToHost, s->stream);
+  if (int rc = avbd_sync(s)) return rc;
+  for (uint32_t m = 0; m < nm && man; m++) {
+    memcpy(man + 8ull * m, &ids[m], 16);
+    memcpy(man + 8ull * m + 4, &geo[m], 16);
+  }
+  return (int)nm;
+}
+int avbd_read_pairs(avbd_solver *s, uint32_t *out, uint32_t cap) {
+  if (!s) return AVBD_ERR_INVALID;
+  avbd_counters c;
+  if (int rc = avbd_read_counters(s, &c)) return rc;
+  uint32_t np = std::min(c.pairs, s->pairCap);
+  if (np > cap) return fail(s, AVBD_ERR_CAPACITY, "output array too small");
+  if (np) hipMemcpyAsync(out, s->pairs, np * 8ull, hipMemcpyDeviceToHost, s->stream);
+  if (int rc = avbd_sync(s)) return rc;
+  return (int)np;
+}
+int avbd_read_colors(avbd_solver *s, uint32_t *state, uint32_t count, uint32_t *starts) {
+  if (!s || count > s->n) return AVBD_ERR_INVALID;
+  if (count && state) hipMemcpyAsync(state, s->colA, count * 4ull, hipMemcpyDeviceToHost, s->stream);
+  if (starts) hipMemcpyAsync(starts, s->colorStart, 65 * 4, hipMemcpyDeviceToHost, s->stream);
+  return avbd_sync(s);
+}
+
+int avbd_seed(avbd_solver *s, const float *bodies, uint32_t body_count, const float *jrec, uint32_t joint_count, const uint32_t *man,
+              uint32_t nm, const uint32_t *con, uint32_t nc) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (body_count != s->n) return fail(s, AVBD_ERR_INVALID, "seedFrom: bodies differ from the solver's");
+  if (joint_count != s->nj) return fail(s, AVBD_ERR_INVALID, "seedFrom: joints differ from the reference");
+  if (nm > s->manCap || nc > s->conCap) return fail(s, AVBD_ERR_CAPACITY, "seedFrom: too many contacts");
+  for (uint32_t m = 0; m < nm; m++) {
+    const uint32_t *o = man + 8ull * m;
+    if (o[0] >= s->n || o[1] >= s->n || (uint64_t)o[2] + (o[3] & 15u) > nc) return fail(s, AVBD_ERR_INVALID, "seedFrom: manifold out of range");
+  }
+  if (int rc = avbd_write_bodies(s, bodies, 0, body_count)) return rc;
+  if (joint_count) {
+    std::vector<float> jr(jrec, jrec + 40ull * joint_count);
+    for (auto &x : jr) x = finiteF(x);
+    hipMemcpyAsync(s->joints, jr.data(), jr.size() * 4, hipMemcpyHostToDevice, s->stream);
+  }
+  int prev = 1 - s->parity;  // the next step reads last step's contacts from these
+  std::vector<uint4> ids(nm);
+  std::vector<float4> geo(nm);
+  for (uint32_t m = 0; m < nm; m++) { memcpy(&ids[m], man + 8ull * m, 16); memcpy(&geo[m], man + 8ull * m + 4, 16); }
+  if (nm) {
+    hipMemcpyAsync(s->mIds[prev], ids.data(), nm * 16ull, hipMemcpyHostToDevice, s->stream);
+    hipMemcpyAsync(s->mGeo[prev], geo.data(), nm * 16ull, hipMemcpyHostToDevice, s->stream);
+  }
+  if (nc) hipMemcpyAsync(s->con[prev], con, nc * 64ull, hipMemcpyHostToDevice, s->stream);
+  uint32_t c[C_WORDS] = {0};
+  c[C_CONTACTS] = nc;
+  c[C_MANIFOLDS] = nm;
+  hipMemcpyAsync(s->counters, c, sizeof(c), hipMemcpyHostToDevice, s->stream);
+  s->manifoldsValid = false;
+  return avbd_sync(s);
+}
+int avbd_set_fixed_colors(avbd_solver *s, const uint32_t *colors, uint32_t count) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (!colors) { s->fixedColors = false; return AVBD_OK; }
+  if (count != s->n) return fail(s, AVBD_ERR_INVALID, "fixedColors: one colour per body");
+  uint32_t cap = 1;
+  for (uint32_t i = 0; i < count; i++) {
+    if (colors[i] != NO_COLOR && colors[i] >= MAX_COLORS) return fail(s, AVBD_ERR_INVALID, "fixedColors: colour >= 64");
+    if (colors[i] != NO_COLOR) cap = std::max(cap, colors[i] + 1);
+  }
+  hipMemcpyAsync(s->fixedDev, colors, count * 4ull, hipMemcpyHostToDevice, s->stream);
+  s->fixedColors = true;
+  s->fixedCap = cap;
+  return avbd_sync(s);
+}
+int avbd_set_color_state(avbd_solver *s, const uint32_t *state, uint32_t count) {
+  if (!s || !state || count != s->n) return AVBD_ERR_INVALID;
+  hipMemcpyAsync(s->colA, state, count * 4ull, hipMemcpyHostToDevice, s->stream);
+  return avbd_sync(s);
+}
+int avbd_add_ignore_pairs(avbd_solver *s, const uint32_t *pairs, uint32_t count) {
+  if (!s || (!pairs && count)) return AVBD_ERR_INVALID;
+  for (uint32_t k = 0; k < count; k++) {
+    uint32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    if (a >= s->n || b >= s->n || a == b) return fail(s, AVBD_ERR_INVALID, "IgnoreCollision: bodies out of range");
+  }
+  for (uint32_t k = 0; k < count; k++) {
+    uint32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+    s->ignorePairs.push_back({std::max(a, b), std::min(a, b), IGNORE_SLOT});
+  }
+  std::vector<float> rec(40ull * s->n);
+  if (int rc = avbd_read_bodies(s, rec.data(), 0, s->n)) return rc;
+  std::vector<float> radius(s->n), mass(s->n);
+  for (uint32_t i = 0; i < s->n; i++) { radius[i] = rec[40ull * i + 23]; mass[i] = rec[40ull * i + 19]; }
+  s->uploadStatics(radius, mass);
+  s->manifoldsValid = false;
+  return avbd_sync(s);
+}
+int avbd_set_filters(avbd_solver *s, const uint32_t *groups, const uint32_t *with, uint32_t count) {
+  if (!s || !groups || !with || count != s->n) return AVBD_ERR_INVALID;
+  std::vector<uint32_t> f(2ull * count);
+  for (uint32_t i = 0; i < count; i++) { f[2 * i] = groups[i]; f[2 * i + 1] = with[i]; }
+  hipMemcpyAsync(s->filters, f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream);
+  s->manifoldsValid = false;
+  return avbd_sync(s);
+}
+int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t count) {
+  if (!s || (!words && count)) return AVBD_ERR_INVALID;
+  if (count % 4 != 0) return fail(s, AVBD_ERR_INVALID, "hull buffer is whole vec4u words");
+  if (count > s->hullCap) return fail(s, AVBD_ERR_CAPACITY, "hull storage: " + std::to_string(count) + " words > capacity " + std::to_string(s->hullCap));
+  if (count) hipMemcpyAsync(s->hulls, words, count * 4ull, hipMemcpyHostToDevice, s->stream);
+  return avbd_sync(s);
+}
+
+int avbd_profile_step(avbd_solver *s, const avbd_step_opts *opts, avbd_profile *out) {
+  if (!s || !out) return AVBD_ERR_INVALID;
+  hipEvent_t e[5];
+  for (auto &x : e) hipEventCreate(&x);
+  s->stepCount++;
+  bool reuse = opts && opts->reuse_manifolds && s->manifoldsValid;
+  hipEventRecord(e[0], s->stream);
+  s->collision(reuse);
+  hipEventRecord(e[1], s->stream);
+  // adjacency and colouring share topology(); split them at the colouring launch
+  s->topology();
+  hipEventRecord(e[3], s->stream);
+  s->solve();
+  hipEventRecord(e[4], s->stream);
+  s->parity = 1 - s->parity;
+  s->manifoldsValid = true;
+  int rc = avbd_sync(s);
+  float a = 0, c = 0, d = 0;
+  hipEventElapsedTime(&a, e[0], e[1]);
+  hipEventElapsedTime(&c, e[1], e[3]);
+  hipEventElapsedTime(&d, e[3], e[4]);
+  out->collision = a;
+  out->adjacency = 0;
+  out->coloring = c;
+  out->solve = d;
+  out->total = a + c + d;
+  for (auto &x : e) hipEventDestroy(x);
+  return rc;
+}
+int avbd_time_steps(avbd_solver *s, uint32_t steps, uint32_t substeps, float *ms) {
+  if (!s || !ms || substeps == 0) return AVBD_ERR_INVALID;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, s->stream);
+  for (uint32_t k = 0; k < steps; k++)
+    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
+  hipEventRecord(e1, s->stream);
+  int rc = avbd_sync(s);
+  hipEventElapsedTime(ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_kernel_stats *out) {
+  if (!s || !out || substeps == 0) return AVBD_ERR_INVALID;
+  memset(out, 0, sizeof(*out));
+  s->timer.on = true;
+  for (uint32_t k = 0; k < steps; k++)
+    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
+  s->timer.on = false;
+  int rc = avbd_sync(s);
+  for (auto &m : s->timer.marks) {
+    float t = 0;
+    hipEventElapsedTime(&t, m.second.first, m.second.second);
+    switch (m.first) {
+      case K_PRIMAL: out->primal_ms += t; out->primal_launches++; break;
+      case K_DUAL: out->dual_ms += t; out->dual_launches++; break;
+      case K_NARROW: out->narrowphase_ms += t; out->narrowphase_launches++; break;
+      case K_BROAD: out->broadphase_ms += t; out->broadphase_launches++; break;
+      default: out->other_ms += t; break;
+    }
+    hipEventDestroy(m.second.first);
+    hipEventDestroy(m.second.second);
+  }
+  s->timer.marks.clear();
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,1027 @@
+// Kernels of one AVBD step on CDNA4 (gfx950). Each is a grid-stride loop over a count that
+// lives in device memory, so no step ever waits on the host and the whole step can be captured
+// in a hipGraph. Mirrors GpuSolver3D.step's phases (src/avbd3d/gpu/solver.ts:2174-2261):
+// collision (refs, hash of last step's pairs, uniform-grid broadphase, narrowphase with warm
+// start), adjacency (CSR + per-body sort), Jones-Plassmann colouring with stable buckets, and
+// the solve (warm starts, iterations x (per-colour primal, dual), velocities).
+#pragma once
+#include "avbd_device.h"
+#include "avbd_hull.h"
+
+namespace avbd {
+
+struct Dev {
+  uint32_t n, nj;
+  Bodies b;
+  float4 *refPos, *refRot;
+  uint2 *filters;
+  float4 *joints;  // 10 float4 per joint (layout.ts:232-243)
+  int4 *jinfo;     // type, a (-1 world), b, 0
+  uint32_t *statics;  // large bodies [largeCount] | no-collide (hi, lo, slot) triples sorted by (hi, lo)
+  uint32_t largeCount, noCollideCount;
+  const uint4 *hulls;
+  // uniform grid
+  float cellSize, largeRadius;
+  uint32_t tableMask;
+  uint32_t *cellCount, *cellStart, *cellCursor, *cellSorted;
+  int4 *cellOf;
+  float4 *sortedPos;  // pos.xyz, bounding radius of cellSorted[k]: the candidate scan's stream
+  // pairs, manifolds (current / previous ping-pong), contacts
+  uint2 *pairs;
+  uint32_t pairCap;
+  uint4 *mIds, *pmIds;
+  float4 *mGeo, *pmGeo;
+  Contact *con, *pcon;
+  uint32_t manCap, conCap;
+  uint32_t *table;
+  uint32_t hashMask;
+  // adjacency
+  uint32_t *adjCount, *adjStart, *adjFill, *adjList;
+  uint32_t adjCap;
+  // colouring
+  uint32_t *colA, *colB, *colorStart, *colorBodies, *colorRank, *hist;
+  uint32_t colorGroups;
+  uint32_t *counters;
+  Params p;
+};
+
+#define GRID_STRIDE(i, count) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (count); i += gridDim.x * blockDim.x)
+
+__device__ __forceinline__ bool isDynamic(const Dev &d, int i) { return i >= 0 && d.b.size[i].w > 0.0f; }
+__device__ __forceinline__ uint32_t manifoldCount(const Dev &d) { uint32_t m = d.counters[C_MANIFOLDS]; return m < d.manCap ? m : d.manCap; }
+__device__ __forceinline__ uint32_t prevManifoldCount(const Dev &d) { return d.counters[C_PREV_MANIFOLDS]; }
+
+// ---------------------------------------------------------------------------------------------
+// Utilities
+__global__ void k_fill(uint32_t *p, uint32_t n, uint32_t v) { GRID_STRIDE(i, n) p[i] = v; }
+
+/** beginFrame (wgsl-collision.ts:97-107): last step's counts become "previous". */
+__global__ void k_begin(Dev d, int reuseManifolds) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t *c = d.counters;
+  uint32_t m = c[C_MANIFOLDS], k = c[C_CONTACTS];
+  c[C_PREV_MANIFOLDS] = m < d.manCap ? m : d.manCap;
+  c[C_PREV_CONTACTS] = k < d.conCap ? k : d.conCap;
+  c[C_MANIFOLDS] = 0;
+  c[C_CONTACTS] = 0;
+  if (!reuseManifolds) c[C_PAIRS] = 0;
+  c[C_OVERFLOW] = 0;
+  c[C_CLASHES] = 0;
+  c[C_NUM_COLORS] = 0;
+}
+
+/** updateRefs (wgsl-collision.ts:34-48): reference poses for contact reuse. */
+__global__ void k_update_refs(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    V pos = xyz(d.b.pos[i]);
+    Q rot = q4(d.b.rot[i]);
+    float4 lp = d.refPos[i], lr = d.refRot[i];
+    Q last = q4(lr);
+    bool moved = dot4(last, last) == 0.0f || length(sub(pos, xyz(lp))) > 0.002f || length(qsub(rot, last)) > 0.002f;
+    if (moved) {
+      d.refPos[i] = f4(pos, 0.0f);
+      d.refRot[i] = f4q(rot);
+      float4 ip = d.b.inpos[i];
+      ip.w = __uint_as_float(d.p.step);
+      d.b.inpos[i] = ip;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Hash of last step's pairs (wgsl-collision.ts:344-381), compare-and-swap linear probing.
+__device__ __forceinline__ uint32_t pairHash(uint32_t a, uint32_t b) { return hash32((a * 0x9e3779b1u) ^ hash32(b)); }
+
+__global__ void k_hash_insert(Dev d) {
+  uint32_t count = prevManifoldCount(d);
+  GRID_STRIDE(m, count) {
+    uint4 ids = d.pmIds[m];
+    // Rewritten slots have no contact history
+    if (d.b.ipos[ids.x].w != 0.0f || d.b.ipos[ids.y].w != 0.0f) continue;
+    uint32_t h = pairHash(ids.x, ids.y) & d.hashMask;
+    for (uint32_t probe = 0; probe <= d.hashMask; probe++) {
+      if (atomicCAS(&d.table[h], 0u, m + 1) == 0u) break;
+      h = (h + 1) & d.hashMask;
+    }
+  }
+}
+__device__ __forceinline__ int hashFind(const Dev &d, uint32_t a, uint32_t b) {
+  uint32_t h = pairHash(a, b) & d.hashMask;
+  for (uint32_t probe = 0; probe <= d.hashMask; probe++) {
+    uint32_t v = d.table[h];
+    if (v == 0) return -1;
+    uint4 ids = d.pmIds[v - 1];
+    if (ids.x == a && ids.y == b) return (int)(v - 1);
+    h = (h + 1) & d.hashMask;
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Broadphase: uniform grid over bounding spheres (cells 2x the largest small radius), a list of
+// large bodies tested against every body, and the reference's exact pair predicate
+// (wgsl-collision.ts:150-234). The predicate decides; the grid only has to be conservative.
+__device__ __forceinline__ bool isLarge(const Dev &d, uint32_t i) { return d.b.moment[i].w > d.largeRadius; }
+__device__ __forceinline__ int4 cellAt(const Dev &d, V pos) {
+  return make_int4((int)floorf(pos.x / d.cellSize), (int)floorf(pos.y / d.cellSize), (int)floorf(pos.z / d.cellSize), 0);
+}
+__device__ __forceinline__ uint32_t cellHash(const Dev &d, int4 c) {
+  return (((uint32_t)c.x * 73856093u) ^ ((uint32_t)c.y * 19349663u) ^ ((uint32_t)c.z * 83492791u)) & d.tableMask;
+}
+__global__ void k_grid_count(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    if (isLarge(d, i)) continue;
+    int4 c = cellAt(d, xyz(d.b.pos[i]));
+    d.cellOf[i] = c;
+    atomicAdd(&d.cellCount[cellHash(d, c)], 1u);
+  }
+}
+__global__ void k_grid_scatter(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    if (isLarge(d, i)) continue;
+    uint32_t h = cellHash(d, d.cellOf[i]);
+    uint32_t slot = d.cellStart[h] + atomicAdd(&d.cellCursor[h], 1u);
+    d.cellSorted[slot] = i;
+    float4 p = d.b.pos[i];
+    d.sortedPos[slot] = make_float4(p.x, p.y, p.z, d.b.moment[i].w);
+  }
+}
+
+/** The broadphase's view of a body: world axes and half extents (a sphere: its radius). */
+struct Shape { V ax[3]; V h; bool sphere; };
+__device__ __forceinline__ Shape shapeOf(const Dev &d, uint32_t i) {
+  Q q = q4(d.b.rot[i]);
+  Shape s;
+  s.h = mul(xyz(d.b.size[i]), 0.5f);
+  s.sphere = d.b.angvel[i].w == SHAPE_SPHERE;
+  s.ax[0] = qrotate(q, vv(1.0f, 0.0f, 0.0f));
+  s.ax[1] = qrotate(q, vv(0.0f, 1.0f, 0.0f));
+  s.ax[2] = qrotate(q, vv(0.0f, 0.0f, 1.0f));
+  return s;
+}
+__device__ __forceinline__ float extent(const Shape &s, V n) {
+  if (s.sphere) return s.h.x;
+  return s.h.x * fabsf(dot(n, s.ax[0])) + s.h.y * fabsf(dot(n, s.ax[1])) + s.h.z * fabsf(dot(n, s.ax[2]));
+}
+__device__ __forceinline__ V aabbHalf(const Shape &s) {
+  if (s.sphere) return vv(s.h.x, s.h.x, s.h.x);
+  return add(add(mul(vabs(s.ax[0]), s.h.x), mul(vabs(s.ax[1]), s.h.y)), mul(vabs(s.ax[2]), s.h.z));
+}
+__device__ __forceinline__ bool faceSeparated(const Shape &A, const Shape &B, V dd) {
+  for (int k = 0; k < 3; k++) {
+    if (fabsf(dot(dd, A.ax[k])) - extent(A, A.ax[k]) - extent(B, A.ax[k]) > 1e-3f) return true;
+    if (fabsf(dot(dd, B.ax[k])) - extent(A, B.ax[k]) - extent(B, B.ax[k]) > 1e-3f) return true;
+  }
+  return false;
+}
+/** Binary search of the (hi, lo) no-collide list; an entry holds while its joint does. */
+__device__ __forceinline__ bool ignored(const Dev &d, uint32_t hi, uint32_t lo) {
+  const uint32_t *nc = d.statics + d.largeCount;
+  uint32_t first = 0, count = d.noCollideCount;
+  while (count > 0) {
+    uint32_t step = count / 2, mid = first + step;
+    uint32_t mh = nc[3 * mid], ml = nc[3 * mid + 1];
+    if (mh < hi || (mh == hi && ml < lo)) { first = mid + 1; count -= step + 1; }
+    else count = step;
+  }
+  for (uint32_t k = first; k < d.noCollideCount; k++) {
+    if (nc[3 * k] != hi || nc[3 * k + 1] != lo) break;
+    uint32_t j = nc[3 * k + 2];
+    if (j == IGNORE_SLOT) return true;
+    if (d.joints[10 * j].w != 0.0f || d.joints[10 * j + 1].w != 0.0f) return true;
+  }
+  return false;
+}
+struct Probe { uint32_t index; V pos; float radius; V aabb; bool dynamic; Shape shape; uint2 filter; };
+__device__ __forceinline__ Probe probeOf(const Dev &d, uint32_t i) {
+  Probe P;
+  P.index = i;
+  P.pos = xyz(d.b.pos[i]);
+  P.radius = d.b.moment[i].w;
+  P.shape = shapeOf(d, i);
+  P.aabb = aabbHalf(P.shape);
+  P.dynamic = d.b.size[i].w > 0.0f;
+  P.filter = d.filters[i];
+  return P;
+}
+__device__ __forceinline__ void emitPair(const Dev &d, uint32_t a, uint32_t b) {
+  uint32_t slot = atomicAdd(&d.counters[C_PAIRS], 1u);
+  if (slot >= d.pairCap) { atomicOr(&d.counters[C_OVERFLOW], 1u); return; }
+  d.pairs[slot] = make_uint2(a, b);
+}
+/** testPair (wgsl-collision.ts:208-234), given body j's position and radius already loaded. */
+__device__ __forceinline__ void testPair(const Dev &d, const Probe &P, uint32_t j, float4 pj) {
+  if (!P.dynamic && d.b.size[j].w <= 0.0f) return;
+  uint2 fj = d.filters[j];
+  if ((P.filter.x & fj.y) == 0u || (fj.x & P.filter.y) == 0u) return;
+  V dd = sub(P.pos, xyz(pj));
+  float r = P.radius + pj.w;
+  if (dot(dd, dd) > r * r) return;
+  Shape B = shapeOf(d, j);
+  V hb = aabbHalf(B);
+  V lim = add(add(P.aabb, hb), vv(1e-4f, 1e-4f, 1e-4f));
+  V ad = vabs(dd);
+  if (ad.x > lim.x || ad.y > lim.y || ad.z > lim.z) return;
+  if (faceSeparated(P.shape, B, dd)) return;
+  uint32_t a = P.index > j ? P.index : j, b = P.index > j ? j : P.index;
+  if (ignored(d, a, b)) return;
+  emitPair(d, a, b);
+}
+/** Dynamic bodies probe: small ones the 27 cells around them (a same-cell dynamic pair from
+ *  its higher index), then everyone the large list (a dynamic large-large pair once). */
+__global__ void k_find_pairs(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    if (d.b.size[i].w <= 0.0f) continue;
+    bool small = !isLarge(d, i);
+    Probe P = probeOf(d, i);
+    if (small) {
+      int4 c = d.cellOf[i];
+      for (int oz = -1; oz <= 1; oz++)
+        for (int oy = -1; oy <= 1; oy++)
+          for (int ox = -1; ox <= 1; ox++) {
+            int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, 0);
+            uint32_t h = cellHash(d, cell);
+            uint32_t end = d.cellStart[h + 1];
+            for (uint32_t k = d.cellStart[h]; k < end; k++) {
+              uint32_t j = d.cellSorted[k];
+              if (j == i) continue;
+              if (j > i && d.b.size[j].w > 0.0f) continue;
+              int4 cj = d.cellOf[j];
+              if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z) continue;
+              testPair(d, P, j, d.sortedPos[k]);
+            }
+          }
+    }
+    for (uint32_t k = 0; k < d.largeCount; k++) {
+      uint32_t l = d.statics[k];
+      if (l == i || (!small && l < i && d.b.size[l].w > 0.0f)) continue;
+      float4 pl = d.b.pos[l];
+      testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w));
+    }
+  }
+}
+/** Small fixed bodies against the large dynamic ones (findFixedLarge, wgsl-collision.ts:285). */
+__global__ void k_find_fixed_large(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    if (d.b.size[i].w > 0.0f || isLarge(d, i)) continue;
+    bool loaded = false;
+    Probe P;
+    for (uint32_t k = 0; k < d.largeCount; k++) {
+      uint32_t l = d.statics[k];
+      if (d.b.size[l].w <= 0.0f) continue;
+      if (!loaded) { P = probeOf(d, i); loaded = true; }
+      float4 pl = d.b.pos[l];
+      testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Narrowphase + warm start (wgsl-collision.ts:685-892)
+__device__ __forceinline__ float contactPenaltyMin(const Dev &d, uint32_t a, uint32_t b) {
+  if (!(d.p.flags & FLAG_MASS_PENALTY)) return PENALTY_MIN;
+  float ma = d.b.size[a].w, mb = d.b.size[b].w;
+  float mA = ma <= 0.0f ? 3.4e38f : ma, mB = mb <= 0.0f ? 3.4e38f : mb;
+  return clampf(mn(mA, mB) / (d.p.dt * d.p.dt), PENALTY_MIN, PENALTY_MAX);
+}
+__device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pgeo) {
+  uint32_t a = pids.x, b = pids.y, count = pids.w & 15u;
+  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
+  if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
+  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], count);
+  if (base + count > d.conCap) {
+    atomicOr(&d.counters[C_OVERFLOW], 2u);
+    d.mIds[m] = make_uint4(a, b, base, pids.w & ~15u);
+    d.mGeo[m] = pgeo;
+    return;
+  }
+  d.mIds[m] = make_uint4(a, b, base, pids.w);
+  d.mGeo[m] = pgeo;
+  V pA = xyz(d.b.pos[a]), pB = xyz(d.b.pos[b]);
+  Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
+  V basis[3];
+  orthonormal(xyz(pgeo), basis);
+  float pmin = contactPenaltyMin(d, a, b);
+  for (uint32_t i = 0; i < count; i++) {
+    Contact k = d.pcon[pids.z + i];
+    V dd = sub(add(qrotate(qA, xyz(k.rA)), pA), add(qrotate(qB, xyz(k.rB)), pB));
+    k.rB.w = dot(basis[0], dd) + d.p.margin;
+    k.pen.w = dot(basis[1], dd);
+    k.lam.w = dot(basis[2], dd);
+    V lam = mul(mul(xyz(k.lam), d.p.alpha), d.p.gamma);
+    V pen = xyz(k.pen);
+    pen = vv(clampf(pen.x * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.y * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.z * d.p.gamma, PENALTY_MIN, PENALTY_MAX));
+    pen.x = mx(pen.x, pmin);
+    k.lam = f4(lam, k.lam.w);
+    k.pen = f4(pen, k.pen.w);
+    d.con[base + i] = k;
+  }
+}
+/** reuseManifolds (wgsl-collision.ts:758): substeps after the first keep last substep's points. */
+__global__ void k_reuse_manifolds(Dev d) {
+  uint32_t count = prevManifoldCount(d);
+  GRID_STRIDE(m, count) {
+    uint4 ids = d.pmIds[m];
+    if ((ids.w & 15u) > 0) copyContacts(d, ids, d.pmGeo[m]);
+  }
+}
+__device__ __forceinline__ bool reuseContacts(const Dev &d, uint32_t a, uint32_t b) {
+  int pm = hashFind(d, a, b);
+  if (pm < 0) return false;
+  uint4 ids = d.pmIds[pm];
+  uint32_t count = ids.w & 15u, generated = ids.w >> 4;
+  uint32_t sa = __float_as_uint(d.b.inpos[a].w), sb = __float_as_uint(d.b.inpos[b].w);
+  uint32_t moved = sa > sb ? sa : sb;
+  if (count == 0 || generated < moved) return false;
+  copyContacts(d, ids, d.pmGeo[pm]);
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
+  uint32_t count = d.counters[C_PAIRS];
+  if (count > d.pairCap) count = d.pairCap;
+  GRID_STRIDE(p, count) {
+    uint2 pr = d.pairs[p];
+    uint32_t a = pr.x, b = pr.y;
+    if ((d.p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(d, a, b)) continue;
+    float4 posA = d.b.pos[a], posB = d.b.pos[b];
+    Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
+    float shA = d.b.angvel[a].w, shB = d.b.angvel[b].w;
+    Box A = makeBox(xyz(posA), qA, xyz(d.b.size[a])), B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
+    Sat sat;
+    sat.n = vv(0.0f, 0.0f, 0.0f);
+    Found found;
+    found.count = 0;
+    bool sphereA = shA == SHAPE_SPHERE, sphereB = shB == SHAPE_SPHERE;
+    bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
+    if (sphereA && sphereB) sphereSphere(A, B, sat, found);
+    else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, sat, found);
+    else if (sphereA || sphereB) {
+      if (sphereA) {
+        SphereHit h = sphereBox(A, B, qB);
+        sat.n = h.n;
+        if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+      } else {
+        SphereHit h = sphereBox(B, A, qA);
+        sat.n = neg(h.n);
+        if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+      }
+    } else collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
+    if (found.count == 0) continue;
+
+    uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
+    if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); continue; }
+    uint32_t base = atomicAdd(&d.counters[C_CONTACTS], found.count);
+    uint32_t cnt = found.count;
+    if (base + cnt > d.conCap) { atomicOr(&d.counters[C_OVERFLOW], 2u); cnt = 0; }
+    V n = neg(sat.n);
+    d.mIds[m] = make_uint4(a, b, base, cnt | (d.p.step << 4));
+    d.mGeo[m] = f4(n, sqrtf(posA.w * posB.w));
+    if (cnt == 0) continue;
+    V basis[3];
+    orthonormal(n, basis);
+    bool anySphere = sphereA || sphereB;
+    uint32_t prevFirst = 0, prevCount = 0;
+    int pm = hashFind(d, a, b);
+    if (pm >= 0) { uint4 pids = d.pmIds[pm]; prevFirst = pids.z; prevCount = pids.w & 15u; }
+    bool matchNearest = (d.p.flags & FLAG_MATCH_NEAREST) != 0;
+    float minSide = mn(mn(mn(A.h.x, A.h.y), A.h.z), mn(mn(B.h.x, B.h.y), B.h.z)) * 2.0f;
+    float penaltyMin = contactPenaltyMin(d, a, b);
+    for (uint32_t i = 0; i < cnt; i++) {
+      V rA = qrotate(qconj(qA), sub(found.xA[i], A.c));
+      V rB = qrotate(qconj(qB), sub(found.xB[i], B.c));
+      V pen = vv(0.0f, 0.0f, 0.0f), lam = pen;
+      uint32_t stick = 0;
+      int j = -1;
+      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+        if ((__float_as_uint(d.pcon[c].rA.w) & ~STICK_BIT) == found.feature[i]) { j = (int)c; break; }
+      if (j < 0 && matchNearest) {
+        float bestDist = 0.05f * minSide;
+        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+          float dist = length(sub(xyz(d.pcon[c].rA), rA));
+          if (dist <= bestDist) { bestDist = dist; j = (int)c; }
+        }
+      }
+      if (j >= 0) {
+        Contact pc = d.pcon[j];
+        pen = xyz(pc.pen); lam = xyz(pc.lam); stick = __float_as_uint(pc.rA.w) & STICK_BIT;
+        if (stick != 0 && !anySphere) { rA = xyz(pc.rA); rB = xyz(pc.rB); }
+      }
+      V dd = sub(add(qrotate(qA, rA), A.c), add(qrotate(qB, rB), B.c));
+      Contact k;
+      k.rA = f4(rA, __uint_as_float(found.feature[i] | stick));
+      k.rB = f4(rB, dot(basis[0], dd) + d.p.margin);
+      lam = mul(mul(lam, d.p.alpha), d.p.gamma);
+      pen = vv(clampf(pen.x * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.y * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.z * d.p.gamma, PENALTY_MIN, PENALTY_MAX));
+      pen.x = mx(pen.x, penaltyMin);
+      k.pen = f4(pen, dot(basis[1], dd));
+      k.lam = f4(lam, dot(basis[2], dd));
+      d.con[base + i] = k;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Adjacency (wgsl-topology.ts:63-153): constraint ids are joints 0..J-1, then manifolds.
+__device__ __forceinline__ bool jointActive(const Dev &d, uint32_t j) {
+  return d.jinfo[j].x != (int)T_NONE && (d.joints[10 * j].w != 0.0f || d.joints[10 * j + 1].w != 0.0f);
+}
+__device__ __forceinline__ int2 endpoints(const Dev &d, uint32_t id) {
+  if (id < d.nj) { int4 in = d.jinfo[id]; return make_int2(in.y, in.z); }
+  uint4 ids = d.mIds[id - d.nj];
+  return make_int2((int)ids.x, (int)ids.y);
+}
+__global__ void k_degree(Dev d) {
+  uint32_t total = d.nj + manifoldCount(d);
+  GRID_STRIDE(id, total) {
+    if (id < d.nj && !jointActive(d, id)) continue;
+    int2 e = endpoints(d, id);
+    if (isDynamic(d, e.x)) atomicAdd(&d.adjCount[e.x], 1u);
+    if (isDynamic(d, e.y)) atomicAdd(&d.adjCount[e.y], 1u);
+  }
+}
+__global__ void k_fill_adj(Dev d) {
+  uint32_t total = d.nj + manifoldCount(d);
+  GRID_STRIDE(id, total) {
+    if (id < d.nj && !jointActive(d, id)) continue;
+    int2 e = endpoints(d, id);
+    int ends[2] = {e.x, e.y};
+    for (int s = 0; s < 2; s++) {
+      int x = ends[s];
+      if (!isDynamic(d, x)) continue;
+      uint32_t slot = d.adjStart[x] + atomicAdd(&d.adjFill[x], 1u);
+      if (slot < d.adjCap) d.adjList[slot] = id;
+    }
+  }
+}
+/** Where id goes in body b's list: joints by id, then manifolds by (other + 2, first key). */
+__device__ __forceinline__ uint2 adjKey(const Dev &d, uint32_t b, uint32_t id) {
+  if (id < d.nj) return make_uint2(0u, id);
+  uint4 ids = d.mIds[id - d.nj];
+  uint32_t other = ids.x == b ? ids.y : ids.x;
+  uint32_t sec = (ids.w & 15u) == 0 ? 0u : (__float_as_uint(d.con[ids.z].rA.w) & ~STICK_BIT);
+  return make_uint2(other + 2, sec);
+}
+__device__ __forceinline__ bool keyLess(uint2 a, uint2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); }
+__global__ void k_sort_adj(Dev d) {
+  GRID_STRIDE(b, d.n) {
+    uint32_t lo = d.adjStart[b], hi = d.adjStart[b + 1];
+    for (uint32_t i = lo + 1; i < hi; i++) {
+      uint32_t id = d.adjList[i];
+      uint2 key = adjKey(d, b, id);
+      uint32_t j = i;
+      while (j > lo) {
+        uint32_t prev = d.adjList[j - 1];
+        if (!keyLess(key, adjKey(d, b, prev))) break;
+        d.adjList[j] = prev;
+        j--;
+      }
+      d.adjList[j] = id;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Colouring: incremental Jones-Plassmann with hashed priorities (wgsl-topology.ts:155-262).
+__device__ __forceinline__ uint32_t priority(uint32_t i) { return hash32(i + 0x9e3779b9u); }
+__device__ __forceinline__ bool beats(uint32_t i, uint32_t j) { uint32_t pi = priority(i), pj = priority(j); return pi > pj || (pi == pj && i > j); }
+__device__ __forceinline__ int neighbour(const Dev &d, uint32_t b, uint32_t e) {
+  int2 ends = endpoints(d, d.adjList[e]);
+  int other = ends.x == (int)b ? ends.y : ends.x;
+  return isDynamic(d, other) ? other : -1;
+}
+__device__ __forceinline__ uint32_t smallestFree(const Dev &d, uint32_t b, const uint32_t *src, bool skipPending) {
+  uint32_t lo = 0, hi = 0;
+  for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1]; e++) {
+    int j = neighbour(d, b, e);
+    if (j < 0) continue;
+    uint32_t s = src[j];
+    if (skipPending && (s & PENDING)) continue;
+    uint32_t col = s & 0xffu;
+    if (col == NO_COLOR) continue;
+    if (col < 32) lo |= 1u << col; else hi |= 1u << (col - 32);
+  }
+  if (~lo != 0u) return (uint32_t)__builtin_ctz(~lo);
+  if (~hi != 0u) return 32u + (uint32_t)__builtin_ctz(~hi);
+  return MAX_COLORS;
+}
+__global__ void k_color_compact(Dev d) {
+  GRID_STRIDE(b, d.n) {
+    if (!isDynamic(d, b)) { d.colB[b] = NO_COLOR; continue; }
+    uint32_t col = d.colA[b] & 0xffu;
+    if (col != NO_COLOR) {
+      bool isMax = true;
+      for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && isMax; e++) { int j = neighbour(d, b, e); if (j >= 0 && !beats(b, j)) isMax = false; }
+      if (isMax) { uint32_t f = smallestFree(d, b, d.colA, false); col = col < f ? col : f; }
+    }
+    d.colB[b] = col;
+  }
+}
+__global__ void k_color_mark(Dev d) {
+  GRID_STRIDE(b, d.n) {
+    uint32_t col = d.colB[b] & 0xffu;
+    if (!isDynamic(d, b)) { d.colA[b] = NO_COLOR; continue; }
+    bool pending = col == NO_COLOR;
+    for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && !pending; e++) {
+      int j = neighbour(d, b, e);
+      if (j >= 0 && (d.colB[j] & 0xffu) == col && beats(j, b)) pending = true;
+    }
+    d.colA[b] = col | (pending ? PENDING : 0u);
+  }
+}
+__global__ void k_color_round(Dev d, const uint32_t *src, uint32_t *dst) {
+  GRID_STRIDE(b, d.n) {
+    uint32_t s = src[b];
+    if (!(s & PENDING)) { dst[b] = s; continue; }
+    bool isMax = true;
+    for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && isMax; e++) {
+      int j = neighbour(d, b, e);
+      if (j >= 0 && (src[j] & PENDING) && !beats(b, j)) isMax = false;
+    }
+    if (isMax) {
+      uint32_t f = smallestFree(d, b, src, true);
+      dst[b] = f >= d.p.colorCap ? ((d.p.colorCap - 1) | PENDING) : f;
+    } else dst[b] = s;
+  }
+}
+/** colorCount (wgsl-topology.ts:276-304): per 256-body chunk, colour histogram and stable ranks. */
+__global__ void __launch_bounds__(256) k_color_count(Dev d) {
+  __shared__ uint32_t hist[64];
+  __shared__ uint32_t chunk[256];
+  uint32_t tid = threadIdx.x, b = blockIdx.x * 256 + tid;
+  if (tid < 64) hist[tid] = 0;
+  uint32_t col = NO_COLOR;
+  if (b < d.n && isDynamic(d, b)) {
+    uint32_t s = d.colA[b];
+    col = s & 0xffu;
+    if (s & PENDING) atomicAdd(&d.counters[C_CLASHES], 1u);
+    if (col == NO_COLOR) col = d.p.colorCap - 1;
+    d.colA[b] = col | (s & PENDING);
+    atomicMax(&d.counters[C_NUM_COLORS], col + 1);
+    if (s & PENDING) atomicOr(&d.counters[C_CLASHING + (col >> 5)], 1u << (col & 31u));
+  }
+  chunk[tid] = col;
+  __syncthreads();
+  if (col != NO_COLOR) {
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < tid; j++) rank += chunk[j] == col ? 1u : 0u;
+    d.colorRank[b] = rank;
+    atomicAdd(&hist[col], 1u);
+  }
+  __syncthreads();
+  if (tid < 64) d.hist[tid * d.colorGroups + blockIdx.x] = hist[tid];
+  if (b == 0) d.hist[64 * d.colorGroups] = 0;
+}
+__global__ void k_color_starts(Dev d) {
+  uint32_t c = threadIdx.x;
+  if (c < 64) d.colorStart[c] = d.hist[c * d.colorGroups];
+  if (c == 0) d.colorStart[64] = d.hist[64 * d.colorGroups];
+}
+__global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
+  uint32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= d.n || !isDynamic(d, b)) return;
+  uint32_t col = d.colA[b] & 0xffu;
+  uint32_t slot = d.hist[col * d.colorGroups + blockIdx.x] + d.colorRank[b];
+  d.colorBodies[slot] = b;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Solve (wgsl-solve.ts)
+/** One body's Newton system: lin/ang lower triangles (00 10 11 20 21 22), cross (row = angular). */
+struct Acc { float lin[6], ang[6], cr[9], rl[3], ra[3]; };
+__device__ __forceinline__ void accZero(Acc &a) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) { a.lin[i] = 0.0f; a.ang[i] = 0.0f; }
+#pragma unroll
+  for (int i = 0; i < 9; i++) a.cr[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 3; i++) { a.rl[i] = 0.0f; a.ra[i] = 0.0f; }
+}
+/** Fold one constraint row in: Jacobian (l, a), stiffness k, force f. */
+__device__ __forceinline__ void addRow(Acc &acc, V l, V a, float k, float f) {
+  float L[3] = {l.x, l.y, l.z}, A[3] = {a.x, a.y, a.z};
+  const int I[6] = {0, 1, 1, 2, 2, 2}, J[6] = {0, 0, 1, 0, 1, 2};
+#pragma unroll
+  for (int e = 0; e < 6; e++) { acc.lin[e] += (L[I[e]] * L[J[e]]) * k; acc.ang[e] += (A[I[e]] * A[J[e]]) * k; }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) acc.cr[i * 3 + j] += (A[i] * L[j]) * k;
+#pragma unroll
+  for (int i = 0; i < 3; i++) { acc.rl[i] += L[i] * f; acc.ra[i] += A[i] * f; }
+}
+__device__ __forceinline__ Q rotAOf(const Dev &d, int a) { return a < 0 ? qq(0.0f, 0.0f, 0.0f, 1.0f) : q4(d.b.rot[a]); }
+__device__ __forceinline__ V jointLinC(const Dev &d, const float4 *k, int a, int b) {
+  V anchor = a < 0 ? xyz(k[6]) : add(qrotate(q4(d.b.rot[a]), xyz(k[6])), xyz(d.b.pos[a]));
+  return sub(anchor, add(qrotate(q4(d.b.rot[b]), xyz(k[7])), xyz(d.b.pos[b])));
+}
+__device__ __forceinline__ V jointAngC(const Dev &d, const float4 *k, int a, int b) {
+  Q qb = q4(d.b.rot[b]);
+  float dd = dot4(qb, qb);
+  Q dq = qmul(qmul(rotAOf(d, a), q4(k[8])), qq(-qb.x / dd, -qb.y / dd, -qb.z / dd, qb.w / dd));
+  if (dq.w < 0.0f) dq = qq(-dq.x, -dq.y, -dq.z, -dq.w);
+  return mul(mul(vv(dq.x, dq.y, dq.z), 2.0f), k[3].w);
+}
+__device__ __forceinline__ V jointLinRows(const Dev &d, const float4 *k, int a, int b, float alpha) {
+  return sub(jointLinC(d, k, a, b), k[0].w >= HARD ? mul(xyz(k[4]), alpha) : vv(0.0f, 0.0f, 0.0f));
+}
+__device__ __forceinline__ V jointAngRows(const Dev &d, const float4 *k, int a, int b, float alpha) {
+  return sub(jointAngC(d, k, a, b), k[1].w >= HARD ? mul(xyz(k[5]), alpha) : vv(0.0f, 0.0f, 0.0f));
+}
+/** addJoint (wgsl-solve.ts:115-165): spring row, ball-socket rows with lumped geometric
+ *  stiffness, angle-lock rows. */
+__device__ __noinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, float alpha, uint32_t i) {
+  int4 in = d.jinfo[j];
+  int a = in.y, b = in.z;
+  const float4 *k = d.joints + 10 * j;
+  bool isA = (int)i == a;
+  float sg = isA ? 1.0f : -1.0f;
+  if (in.x == (int)T_SPRING) {
+    V rAW = qrotate(q4(d.b.rot[a]), xyz(k[6])), rBW = qrotate(q4(d.b.rot[b]), xyz(k[7]));
+    V dd = sub(add(rAW, xyz(d.b.pos[a])), add(rBW, xyz(d.b.pos[b])));
+    float len = length(dd);
+    if (len <= 1.0e-6f) return;
+    V n = divs(dd, len);
+    float st = k[0].w;
+    V r = isA ? rAW : rBW;
+    addRow(acc, mul(n, sg), mul(cross(r, n), sg), st, st * (len - k[6].w));
+    return;
+  }
+  V penLin = xyz(k[0]);
+  if (dot(penLin, penLin) > 0.0f) {
+    V F = add(mulv(penLin, jointLinRows(d, k, a, b, alpha)), xyz(k[2]));
+    V r = isA ? qrotate(q4(d.b.rot[a]), xyz(k[6])) : neg(qrotate(q4(d.b.rot[b]), xyz(k[7])));
+    V v = neg(r);
+    addRow(acc, vv(sg, 0.0f, 0.0f), vv(0.0f, -v.z, v.y), penLin.x, F.x);
+    addRow(acc, vv(0.0f, sg, 0.0f), vv(v.z, 0.0f, -v.x), penLin.y, F.y);
+    addRow(acc, vv(0.0f, 0.0f, sg), vv(-v.y, v.x, 0.0f), penLin.z, F.z);
+    float rf = dot(r, F);
+    float g0 = length(sub(mul(r, F.x), vv(rf, 0.0f, 0.0f)));
+    float g1 = length(sub(mul(r, F.y), vv(0.0f, rf, 0.0f)));
+    float g2 = length(sub(mul(r, F.z), vv(0.0f, 0.0f, rf)));
+    acc.ang[0] += g0; acc.ang[1] += 0.0f; acc.ang[2] += g1; acc.ang[3] += 0.0f; acc.ang[4] += 0.0f; acc.ang[5] += g2;
+  }
+  V penAng = xyz(k[1]);
+  if (dot(penAng, penAng) > 0.0f) {
+    V F = add(mulv(penAng, jointAngRows(d, k, a, b, alpha)), xyz(k[3]));
+    float s = sg * k[3].w;
+    V z = vv(0.0f, 0.0f, 0.0f);
+    addRow(acc, z, vv(s, 0.0f, 0.0f), penAng.x, F.x);
+    addRow(acc, z, vv(0.0f, s, 0.0f), penAng.y, F.y);
+    addRow(acc, z, vv(0.0f, 0.0f, s), penAng.z, F.z);
+  }
+}
+struct PairBody { Q rot; V dLin, dAng; };
+__device__ __forceinline__ PairBody pairBody(const Dev &d, uint32_t i) {
+  PairBody P;
+  P.rot = q4(d.b.rot[i]);
+  P.dLin = sub(xyz(d.b.pos[i]), xyz(d.b.ipos[i]));
+  P.dAng = qsub(P.rot, q4(d.b.irot[i]));
+  return P;
+}
+struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; };
+/** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. */
+__device__ __forceinline__ Eval evalContact(const Dev &d, const Contact &k, const V *basis, float friction, const PairBody &A, uint32_t ia,
+                                            const PairBody &B, uint32_t ib, float alpha) {
+  Q rotA = A.rot, rotB = B.rot;
+  if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE) { rotA = q4(d.b.irot[ia]); rotB = q4(d.b.irot[ib]); }
+  Eval e;
+  e.rAW = qrotate(rotA, xyz(k.rA));
+  e.rBW = qrotate(rotB, xyz(k.rB));
+  float keep = 1.0f - alpha;
+  float c0[3] = {k.rB.w, k.pen.w, k.lam.w};
+  float C[3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    V l = basis[r];
+    C[r] = c0[r] * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cross(e.rAW, l), A.dAng) + dot(cross(e.rBW, neg(l)), B.dAng);
+  }
+  e.C = vv(C[0], C[1], C[2]);
+  V F = add(mulv(xyz(k.pen), e.C), xyz(k.lam));
+  F.x = mn(F.x, 0.0f);
+  e.bounds = fabsf(F.x) * friction;
+  e.frictionScale = sqrtf(F.y * F.y + F.z * F.z);
+  if (e.frictionScale > e.bounds && e.frictionScale > 0.0f) {
+    float s = e.bounds / e.frictionScale;
+    F.y = F.y * s; F.z = F.z * s;
+  }
+  e.F = F;
+  return e;
+}
+/** accumulate (wgsl-solve.ts:436-501): body i's system from adjacency entries lane, lane+lanes, ... */
+__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t lane, uint32_t lanes, float alpha, Acc &acc) {
+  accZero(acc);
+  float dt = d.p.dt, dt2 = dt * dt;
+  if (lane == 0) {
+    float4 sz = d.b.size[i], mo = d.b.moment[i];
+    float m = sz.w / dt2;
+    V I = divs(xyz(mo), dt2);
+    acc.lin[0] = m; acc.lin[2] = m; acc.lin[5] = m;
+    acc.ang[0] = I.x; acc.ang[2] = I.y; acc.ang[5] = I.z;
+    V dx = sub(xyz(d.b.pos[i]), xyz(d.b.inpos[i]));
+    acc.rl[0] = m * dx.x; acc.rl[1] = m * dx.y; acc.rl[2] = m * dx.z;
+    V dq = qsub(q4(d.b.rot[i]), q4(d.b.inrot[i]));
+    acc.ra[0] = I.x * dq.x; acc.ra[1] = I.y * dq.y; acc.ra[2] = I.z * dq.z;
+  }
+  uint32_t end = d.adjStart[i + 1];
+  for (uint32_t e = d.adjStart[i] + lane; e < end; e += lanes) {
+    uint32_t id = d.adjList[e];
+    if (id < d.nj) { addJoint(d, acc, id, alpha, i); continue; }
+    uint4 ids = d.mIds[id - d.nj];
+    float4 geo = d.mGeo[id - d.nj];
+    bool isA = i == ids.x;
+    PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
+    V basis[3];
+    orthonormal(xyz(geo), basis);
+    float sg = isA ? 1.0f : -1.0f;
+    V l0 = mul(basis[0], sg), l1 = mul(basis[1], sg), l2 = mul(basis[2], sg);
+    uint32_t cEnd = ids.z + (ids.w & 15u);
+    for (uint32_t c = ids.z; c < cEnd; c++) {
+      Contact k = d.con[c];
+      Eval ev = evalContact(d, k, basis, geo.w, A, ids.x, B, ids.y, alpha);
+      V r = isA ? ev.rAW : ev.rBW;
+      addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+      addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+      addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+    }
+  }
+}
+/** finishBody (wgsl-solve.ts:504-558): LDLᵀ in registers, pivots floored at 1e-5 of the diagonal. */
+__device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &acc) {
+  const float PF = 1e-5f;
+  float A11 = acc.lin[0], A21 = acc.lin[1], A22 = acc.lin[2], A31 = acc.lin[3], A32 = acc.lin[4], A33 = acc.lin[5];
+  float A41 = acc.cr[0], A42 = acc.cr[1], A43 = acc.cr[2], A44 = acc.ang[0];
+  float A51 = acc.cr[3], A52 = acc.cr[4], A53 = acc.cr[5], A54 = acc.ang[1], A55 = acc.ang[2];
+  float A61 = acc.cr[6], A62 = acc.cr[7], A63 = acc.cr[8], A64 = acc.ang[3], A65 = acc.ang[4], A66 = acc.ang[5];
+  float D1 = A11;
+  float L21 = A21 / D1, L31 = A31 / D1, L41 = A41 / D1, L51 = A51 / D1, L61 = A61 / D1;
+  float D2 = mx(A22 - L21 * L21 * D1, A22 * PF);
+  float L32 = (A32 - L21 * L31 * D1) / D2, L42 = (A42 - L21 * L41 * D1) / D2, L52 = (A52 - L21 * L51 * D1) / D2, L62 = (A62 - L21 * L61 * D1) / D2;
+  float D3 = mx(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33 * PF);
+  float L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3, L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3, L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+  float D4 = mx(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44 * PF);
+  float L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4, L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+  float D5 = mx(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55 * PF);
+  float L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+  float D6 = mx(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66 * PF);
+  float y1 = acc.rl[0], y2 = acc.rl[1] - L21 * y1, y3 = acc.rl[2] - L31 * y1 - L32 * y2;
+  float y4 = acc.ra[0] - L41 * y1 - L42 * y2 - L43 * y3, y5 = acc.ra[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
+  float y6 = acc.ra[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
+  float w3 = y6 / D6, w2 = y5 / D5 - L65 * w3, w1 = y4 / D4 - L54 * w2 - L64 * w3;
+  float v3 = y3 / D3 - L43 * w1 - L53 * w2 - L63 * w3;
+  float v2 = y2 / D2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
+  float v1 = y1 / D1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
+  float4 pos = d.b.pos[i];
+  d.b.pos[i] = make_float4(pos.x - v1, pos.y - v2, pos.z - v3, pos.w);
+  d.b.rot[i] = f4q(qadd(q4(d.b.rot[i]), neg(vv(w1, w2, w3))));
+}
+__device__ __forceinline__ uint32_t lanesFor(const Params &p, uint32_t count) {
+  if (count >= p.lanes1) return 1;
+  if (count >= p.lanes2) return 2;
+  if (count >= p.lanes4) return 4;
+  return 8;
+}
+/** Reduce the partial systems of a body's lanes onto lane 0 (lanes s..2s-1 onto 0..s-1). */
+__device__ __forceinline__ void reduceLanes(Acc &acc, uint32_t lanes) {
+  for (uint32_t s = 4; s > 0; s >>= 1) {
+    if (s >= lanes) continue;
+#pragma unroll
+    for (int e = 0; e < 6; e++) { acc.lin[e] += __shfl_xor(acc.lin[e], (int)s); acc.ang[e] += __shfl_xor(acc.ang[e], (int)s); }
+#pragma unroll
+    for (int e = 0; e < 9; e++) acc.cr[e] += __shfl_xor(acc.cr[e], (int)s);
+#pragma unroll
+    for (int e = 0; e < 3; e++) { acc.rl[e] += __shfl_xor(acc.rl[e], (int)s); acc.ra[e] += __shfl_xor(acc.ra[e], (int)s); }
+  }
+}
+/** One colour's primal: lanesFor(size) threads per body in 64-wide wavefronts, lane partials
+ *  reduced by cross-lane shuffles, lane 0 solves. A clashing colour (bodies that share a
+ *  constraint) runs in one workgroup, chunk after chunk, reads before writes in each chunk. */
+__global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) {
+  uint32_t start = d.colorStart[c], count = d.colorStart[c + 1] - start;
+  if (count == 0) return;
+  uint32_t lanes = lanesFor(d.p, count), per = 64 / lanes;
+  uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes;
+  bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
+  if (serial && blockIdx.x != 0) return;
+  uint32_t stride = serial ? per : gridDim.x * per;
+  for (uint32_t base = serial ? 0 : blockIdx.x * per; base < count; base += stride) {
+    uint32_t k = base + slot;
+    bool live = k < count;
+    uint32_t i = live ? d.colorBodies[start + k] : 0;
+    Acc acc;
+    if (live) accumulate(d, i, lane, lanes, d.p.alpha, acc);
+    else accZero(acc);
+    reduceLanes(acc, lanes);
+    if (serial) __syncthreads();
+    if (live && lane == 0) finishBody(d, i, acc);
+    if (serial) { __threadfence_block(); __syncthreads(); }
+  }
+}
+/** dualJoint (wgsl-solve.ts:562-635) without the plastic (yield) extension. */
+__device__ __noinline__ void dualJoint(const Dev &d, uint32_t j, float alpha) {
+  int4 in = d.jinfo[j];
+  if (in.x != (int)T_JOINT) return;
+  int a = in.y, b = in.z;
+  float4 *k = d.joints + 10 * j;
+  float4 penLin = k[0], penAng = k[1], lamLin = k[2], lamAng = k[3];
+  if (dot(xyz(penLin), xyz(penLin)) > 0.0f) {
+    V C = jointLinRows(d, k, a, b, alpha);
+    if (penLin.w >= HARD) lamLin = f4(add(mulv(xyz(penLin), C), xyz(lamLin)), lamLin.w);
+    float cap = mn(penLin.w, PENALTY_MAX);
+    V np = add(xyz(penLin), mul(vabs(C), d.p.betaLin));
+    penLin = make_float4(mn(np.x, cap), mn(np.y, cap), mn(np.z, cap), penLin.w);
+  }
+  float frac = lamLin.w, limit = fabsf(frac);
+  if (dot(xyz(penAng), xyz(penAng)) > 0.0f) {
+    V C = jointAngRows(d, k, a, b, alpha);
+    if (penAng.w >= HARD) lamAng = f4(add(mulv(xyz(penAng), C), xyz(lamAng)), lamAng.w);
+    float cap = mn(penAng.w, PENALTY_MAX);
+    V np = add(xyz(penAng), mul(vabs(C), d.p.betaAng));
+    penAng = make_float4(mn(np.x, cap), mn(np.y, cap), mn(np.z, cap), penAng.w);
+  }
+  bool linear = frac < 0.0f && dot(xyz(lamLin), xyz(lamLin)) > limit * limit;
+  if (limit < BIG && (dot(xyz(lamAng), xyz(lamAng)) > limit * limit || linear)) {
+    penLin = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    penAng = penLin;
+    lamLin = make_float4(0.0f, 0.0f, 0.0f, frac);
+    lamAng = make_float4(0.0f, 0.0f, 0.0f, lamAng.w);
+  }
+  k[0] = penLin; k[1] = penAng; k[2] = lamLin; k[3] = lamAng;
+}
+/** dualManifold (wgsl-solve.ts:637-662): λ, penalty ramp, stick bit; writes only what changes. */
+__device__ __forceinline__ void dualManifold(const Dev &d, uint32_t m, float alpha) {
+  uint4 ids = d.mIds[m];
+  float4 geo = d.mGeo[m];
+  PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
+  V basis[3];
+  orthonormal(xyz(geo), basis);
+  uint32_t end = ids.z + (ids.w & 15u);
+  for (uint32_t c = ids.z; c < end; c++) {
+    Contact k = d.con[c];
+    Eval e = evalContact(d, k, basis, geo.w, A, ids.x, B, ids.y, alpha);
+    V pen = xyz(k.pen);
+    float BL = d.p.betaLin;
+    if (e.F.x < 0.0f) pen.x = mn(pen.x + BL * fabsf(e.C.x), PENALTY_MAX);
+    uint32_t key = __float_as_uint(k.rA.w);
+    if (e.frictionScale <= e.bounds) {
+      pen.y = mn(pen.y + BL * fabsf(e.C.y), PENALTY_MAX);
+      pen.z = mn(pen.z + BL * fabsf(e.C.z), PENALTY_MAX);
+      bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < 0.00001f;
+      key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+      d.con[c].rA.w = __uint_as_float(key);
+    }
+    d.con[c].pen = f4(pen, k.pen.w);
+    d.con[c].lam = f4(e.F, k.lam.w);
+  }
+}
+__global__ void __launch_bounds__(256) k_dual(Dev d) {
+  uint32_t total = d.nj + manifoldCount(d);
+  GRID_STRIDE(id, total) {
+    if (id < d.nj) dualJoint(d, id, d.p.alpha);
+    else dualManifold(d, id - d.nj, d.p.alpha);
+  }
+}
+/** warmStartJoints (wgsl-solve.ts:236-260). */
+__global__ void k_warm_joints(Dev d) {
+  GRID_STRIDE(j, d.nj) {
+    int4 in = d.jinfo[j];
+    if (in.x != (int)T_JOINT) continue;
+    float4 *k = d.joints + 10 * j;
+    k[4] = f4(jointLinC(d, k, in.y, in.z), 0.0f);
+    k[5] = f4(jointAngC(d, k, in.y, in.z), k[5].w);
+    float decay = d.p.alpha * d.p.gamma, g = d.p.gamma;
+    float4 pl = k[0], pa = k[1], ll = k[2], la = k[3];
+    k[2] = f4(mul(xyz(ll), decay), ll.w);
+    k[3] = f4(mul(xyz(la), decay), la.w);
+    k[0] = make_float4(mn(clampf(pl.x * g, PENALTY_MIN, PENALTY_MAX), pl.w), mn(clampf(pl.y * g, PENALTY_MIN, PENALTY_MAX), pl.w), mn(clampf(pl.z * g, PENALTY_MIN, PENALTY_MAX), pl.w), pl.w);
+    k[1] = make_float4(mn(clampf(pa.x * g, PENALTY_MIN, PENALTY_MAX), pa.w), mn(clampf(pa.y * g, PENALTY_MIN, PENALTY_MAX), pa.w), mn(clampf(pa.z * g, PENALTY_MIN, PENALTY_MAX), pa.w), pa.w);
+  }
+}
+/** warmStartBodies (wgsl-solve.ts:293-335): inertial target, adaptive warm start, x-. */
+__global__ void k_warm_bodies(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    float4 pos = d.b.pos[i], vel4 = d.b.vel[i], av4 = d.b.angvel[i];
+    Q rot = q4(d.b.rot[i]);
+    V vel = xyz(vel4), angVel = xyz(av4);
+    float dt = d.p.dt, g = d.p.gravity;
+    bool dyn = d.b.size[i].w > 0.0f;
+    V up = vv(d.p.upx, d.p.upy, d.p.upz);
+    V inpos = add(xyz(pos), mul(vel, dt));
+    if (dyn) inpos = add(inpos, mul(up, g * (dt * dt)));
+    d.b.inpos[i] = f4(inpos, d.b.inpos[i].w);
+    d.b.inrot[i] = f4q(qadd(rot, mul(angVel, dt)));
+    float w = 0.0f;
+    if (fabsf(g) > 0.0f) {
+      float sg = g > 0.0f ? 1.0f : (g < 0.0f ? -1.0f : 0.0f);
+      w = clampf((dot(vel, up) - vel4.w) / dt * sg / fabsf(g), 0.0f, 1.0f);
+    }
+    bool slow = length(vel) + length(angVel) * d.b.moment[i].w < 0.1f;
+    bool atRest = (d.p.flags & FLAG_START_AT_REST) && slow;
+    d.b.ipos[i] = f4(xyz(pos), 0.0f);
+    d.b.irot[i] = f4q(rot);
+    if (dyn && !atRest) {
+      d.b.pos[i] = f4(add(add(xyz(pos), mul(vel, dt)), mul(up, g * (w * dt * dt))), pos.w);
+      d.b.rot[i] = f4q(qadd(rot, mul(angVel, dt)));
+    }
+    bool moving = vel.x != 0.0f || vel.y != 0.0f || vel.z != 0.0f || angVel.x != 0.0f || angVel.y != 0.0f || angVel.z != 0.0f;
+    if (!dyn && moving) {
+      d.b.pos[i] = f4(add(xyz(pos), mul(vel, dt)), pos.w);
+      d.b.rot[i] = f4q(qadd(rot, mul(angVel, dt)));
+    }
+  }
+}
+/** updateVelocities (wgsl-solve.ts:672-684): BDF1. */
+__global__ void k_velocities(Dev d) {
+  GRID_STRIDE(i, d.n) {
+    V up = vv(d.p.upx, d.p.upy, d.p.upz);
+    float4 vel4 = d.b.vel[i];
+    float prevUp = dot(xyz(vel4), up);
+    if (d.b.size[i].w > 0.0f) {
+      d.b.vel[i] = f4(divs(sub(xyz(d.b.pos[i]), xyz(d.b.ipos[i])), d.p.dt), prevUp);
+      float4 av = d.b.angvel[i];
+      d.b.angvel[i] = f4(divs(qsub(q4(d.b.rot[i]), q4(d.b.irot[i])), d.p.dt), av.w);
+    } else {
+      vel4.w = prevUp;
+      d.b.vel[i] = vel4;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Exclusive scan of u32 (1024 elements per 256-thread block, then block sums, then add).
+__device__ __forceinline__ uint32_t waveInclusive(uint32_t v) {
+  uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t blockExclusive(uint32_t v, uint32_t *total) {
+  __shared__ uint32_t waves[17];  // per-wave offsets, then the block total in [16]
+  uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t inc = waveInclusive(v);
+  if (lane == 63) waves[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t x = lane < nw ? waves[lane] : 0u;
+    uint32_t xi = waveInclusive(x);
+    if (lane < nw) waves[lane] = xi - x;
+    if (lane == nw - 1) waves[16] = xi;
+  }
+  __syncthreads();
+  uint32_t r = inc - v + waves[w];
+  *total = waves[16];
+  __syncthreads();
+  return r;
+}
+__global__ void __launch_bounds__(256) k_scan_local(const uint32_t *in, uint32_t *out, uint32_t *sums, uint32_t n) {
+  uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = base + k < n ? in[base + k] : 0u;
+  uint32_t t = v[0] + v[1] + v[2] + v[3], total;
+  uint32_t ex = blockExclusive(t, &total);
+#pragma unroll
+  for (int k = 0; k < 4; k++) { if (base + k < n) out[base + k] = ex; ex += v[k]; }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t *sums, uint32_t nb) {
+  uint32_t base = threadIdx.x * 4;
+  uint32_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = base + k < nb ? sums[base + k] : 0u;
+  uint32_t t = v[0] + v[1] + v[2] + v[3], total;
+  uint32_t ex = blockExclusive(t, &total);
+#pragma unroll
+  for (int k = 0; k < 4; k++) { if (base + k < nb) sums[base + k] = ex; ex += v[k]; }
+}
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t *out, const uint32_t *sums, uint32_t n) {
+  uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t s = sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; k++) if (base + k < n) out[base + k] += s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Record interchange: SoA streams <-> 40-float body records (layout.ts:8-58).
+__global__ void k_bodies_to_records(Dev d, float4 *rec, uint32_t first, uint32_t count) {
+  GRID_STRIDE(k, count) {
+    uint32_t i = first + k;
+    float4 *o = rec + 10 * k;
+    o[0] = d.b.pos[i]; o[1] = d.b.rot[i]; o[2] = d.b.ipos[i]; o[3] = d.b.irot[i]; o[4] = d.b.size[i];
+    o[5] = d.b.moment[i]; o[6] = d.b.inpos[i]; o[7] = d.b.inrot[i]; o[8] = d.b.vel[i]; o[9] = d.b.angvel[i];
+  }
+}
+__global__ void k_records_to_bodies(Dev d, const float4 *rec, uint32_t first, uint32_t count) {
+  GRID_STRIDE(k, count) {
+    uint32_t i = first + k;
+    const float4 *o = rec + 10 * k;
+    d.b.pos[i] = o[0]; d.b.rot[i] = o[1]; d.b.ipos[i] = o[2]; d.b.irot[i] = o[3]; d.b.size[i] = o[4];
+    d.b.moment[i] = o[5]; d.b.inpos[i] = o[6]; d.b.inrot[i] = o[7]; d.b.vel[i] = o[8]; d.b.angvel[i] = o[9];
+    d.refRot[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+}  // namespace avbd
