@@ -211,3 +211,22 @@ def test_determinism_twice(gpu):
         outs.append(s.read_bodies().copy())
         s.destroy()
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_hulls_spheres_boxes_chains_bit_exact(gpu):
+    say("case: mixed hull/sphere/box pile + chains, 600 bodies, 20 steps, lanes 1")
+    bodies, joints, info, hw = scenes.large_scene(600, seed=7, chains=4, links=5, workers=1)
+    kw = dict(lanes=LANES1, color_cap=64)
+    s = gpu_for(bodies, kw, joints, info, hulls=hw)
+    m = model_for(bodies, kw, joints, info)
+    m.set_hulls(hw)
+    for k in range(20):
+        s.step()
+        m.step(1)
+    got = records.bodies_to_oracle(s.read_bodies())
+    want = m.bodies()
+    bad = np.nonzero((got != want).any(axis=1))[0]
+    assert len(bad) == 0, (bad[:10], np.abs(got - want).max())
+    assert_same_contacts(s, m)
+    assert np.array_equal(s.read_joints().astype(np.float64), m.joints().astype(np.float32).astype(np.float64))
+    s.destroy()

@@ -99,3 +99,111 @@ def joint_record(rA, rB, stiff_lin=3.0e38, stiff_ang=0.0, fracture=3.0e38, torqu
     r[35] = 1.0
     r[39] = 1.0
     return r
+
+
+def hull_record(pos, shape, offset, density, friction=0.5, rot=(0, 0, 0, 1), vel=(0, 0, 0), angvel=(0, 0, 0)):
+    """A hull body (shapes.ts:50-70): its principal frame at pos, size = the vertices' bounds,
+    mass = volume x density, principal moments, bounding radius; shape code SHAPE_HULL + offset."""
+    r = box_record(pos, shape["size"], density, friction, rot, vel, angvel)
+    r[19] = shape["volume"] * density
+    r[20:23] = np.asarray(shape["moments"]) * density
+    r[23] = shape["radius"]
+    r[39] = 3.0 + offset
+    return r
+
+
+def ellipsoid_points(rng, lo=0.3, hi=1.2, points=(16, 32)):
+    """16-32 points uniform on an ellipsoid with axes U[lo, hi] m."""
+    axes = [rng.uniform(lo, hi) for _ in range(3)]
+    k = min(points[0] + int(rng() * (points[1] - points[0] + 1)), points[1])
+    pts = []
+    for _ in range(k):
+        g = [rng.gauss() for _ in range(3)]
+        nrm = math.sqrt(sum(x * x for x in g)) or 1.0
+        pts.append([axes[c] * g[c] / nrm for c in range(3)])
+    return pts
+
+
+def ellipsoid_hull(rng, lo=0.3, hi=1.2, points=(16, 32)):
+    """The convex hull of ellipsoid_points (drawn again when they span no volume)."""
+    from .hull import convex_hull
+    while True:
+        shape = convex_hull(ellipsoid_points(rng, lo, hi, points))
+        if shape is not None:
+            return shape
+
+
+def _hull_or_none(pts):
+    from .hull import convex_hull
+    return convex_hull(pts)
+
+
+def build_hulls(point_sets, workers=8):
+    """convex_hull of each point set; a pool of forked workers for many (call it before the GPU is used)."""
+    if len(point_sets) < 2000 or workers <= 1:
+        return [_hull_or_none(p) for p in point_sets]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_hull_or_none, point_sets, chunksize=256)
+
+
+def large_scene(n=100000, seed=2026, chains=200, links=20, pitch=2.5, friction=0.5, workers=8):
+    """Large scene: n bodies, boxes (half-extents U[0.3, 1.2] m), spheres (radius U[0.3, 1.2] m)
+    and convex hulls (16-32 points on ellipsoids with axes U[0.3, 1.2] m) 1:1:1, density 1, on a
+    jittered grid over a square walled bin; plus `chains` chains of `links` unit boxes joined by
+    ball joints (linear rows hard, angular free) hanging from static anchors beside the bin.
+    Returns (bodies, joints, joint_info, hull_words)."""
+    from .hull import HullBuffer
+    rng = Lcg(seed)
+    per_layer = max(1, int(math.sqrt(n / 16.0)))
+    bin_size = per_layer * pitch
+    recs = walled_bin(bin_size, height=30.0, friction=friction)
+    # one ground under the bin and the chains beside it
+    recs[0] = ground(-0.5, (bin_size + 80.0, bin_size + 80.0, 1.0), friction)
+    hulls = HullBuffer()
+    origin = -bin_size / 2.0 + pitch / 2.0
+    draws = []
+    for k in range(n):
+        ix, iy, iz = k % per_layer, (k // per_layer) % per_layer, k // (per_layer * per_layer)
+        pos = (origin + ix * pitch + rng.uniform(-0.1, 0.1), origin + iy * pitch + rng.uniform(-0.1, 0.1),
+               pitch / 2.0 + iz * pitch + rng.uniform(-0.1, 0.1))
+        q = rng.quat()
+        kind = k % 3
+        if kind == 0:
+            draws.append((0, pos, q, [2 * rng.uniform(0.3, 1.2) for _ in range(3)]))
+        elif kind == 1:
+            draws.append((1, pos, q, rng.uniform(0.3, 1.2)))
+        else:
+            draws.append((2, pos, q, ellipsoid_points(rng)))
+    shapes = iter(build_hulls([d[3] for d in draws if d[0] == 2], workers))
+    for kind, pos, q, arg in draws:
+        if kind == 0:
+            recs.append(box_record(pos, arg, 1.0, friction, rot=q))
+        elif kind == 1:
+            recs.append(sphere_record(pos, arg, 1.0, friction, rot=q))
+        else:
+            shape = next(shapes)
+            if shape is None:  # degenerate draw (no volume): a box of the same extents instead
+                ext = np.ptp(np.asarray(arg), axis=0)
+                recs.append(box_record(pos, np.maximum(ext, 0.1), 1.0, friction, rot=q))
+                continue
+            at = hulls.add(shape)
+            recs.append(hull_record(pos, shape, at, 1.0, friction, rot=q))
+    joints, info = [], []
+    side = bin_size / 2.0 + 10.0
+    top = links + 3.0
+    for c in range(chains):
+        x = side + 4.0 * (c % 8)
+        y = -bin_size / 2.0 + (c // 8) * (bin_size / max(1, (chains + 7) // 8))
+        anchor = len(recs)
+        recs.append(box_record((x, y, top), (1.0, 1.0, 1.0), 0.0, friction))
+        prev = anchor
+        for l in range(links):
+            cur = len(recs)
+            recs.append(box_record((x, y, top - 1.0 - l), (1.0, 1.0, 1.0), 1.0, friction))
+            arm = (1.0 + 1.0) ** 2 * 3  # torqueArm = |size_a + size_b|^2 (forces.ts:146)
+            joints.append(joint_record((0.0, 0.0, -0.5), (0.0, 0.0, 0.5), 3.0e38, 0.0, 3.0e38, arm))
+            info.append((T_JOINT, prev, cur, 0))
+            prev = cur
+    return (np.array(recs, np.float32), np.array(joints, np.float32).reshape(-1, 40), np.array(info, np.int32).reshape(-1, 4),
+            hulls.words())

@@ -1,11 +1,349 @@
-// Convex hull narrowphase (wgsl-hull.ts): hull-hull, hull-box and sphere-hull contacts.
+// Convex hull narrowphase (src/avbd3d/gpu/wgsl-hull.ts): pairs with a hull on either side.
+// SAT after Gregorius: face axes of both shapes by support points, then only the edge pairs
+// whose Gauss-map arcs cross; face contact clips the incident face (the most anti-parallel one
+// holding the deepest vertex) to the reference face's side planes and keeps up to 8 points,
+// deepest first, then spread; edge contact is the closest points of the two edges.
+// Hull buffer: at header h, (vertex start, count, face start, count), (edge start, count, index
+// start, 0); vertices xyz; faces (normal, offset), (first index, count); indices 4 per vec4;
+// edges (a, b, face left, face right). Offsets in vec4 units, f32 stored as bits.
 #pragma once
 #include "avbd_device.h"
 
 namespace avbd {
-/** Pairs with a hull on either side; sat.n points from A towards B. */
-__device__ __noinline__ void collideHullPair(const uint4 *hulls, const Box &A, Q qA, float shA, const Box &B, Q qB, float shB,
-                                             bool sphereA, bool sphereB, Sat &sat, Found &found) {
-  found.count = 0;
+
+static const uint32_t NO_HULL = 0xffffffffu, MAX_CLIP = 64;
+__constant__ const uint32_t BOX_FACE_VERTS[24] = {1, 3, 7, 5, 0, 2, 6, 4, 2, 3, 7, 6, 0, 1, 5, 4, 4, 5, 7, 6, 0, 1, 3, 2};
+__constant__ const uint4 BOX_EDGES[12] = {{0, 1, 3, 5}, {2, 3, 2, 5}, {4, 5, 3, 4}, {6, 7, 2, 4}, {0, 2, 1, 5}, {1, 3, 0, 5},
+                                           {4, 6, 1, 4}, {5, 7, 0, 4}, {0, 4, 1, 3}, {1, 5, 0, 3}, {2, 6, 1, 2}, {3, 7, 0, 2}};
+
+/** A convex shape as the hull code sees it: a box (hull == NO_HULL, half extents h) or a hull. */
+struct Poly { V c; Q q; V h; uint32_t hull, vs, nv, fs, nf, es, ne, is; };
+
+__device__ __forceinline__ Poly polyOf(const uint4 *hulls, V c, Q q, V h, float shape) {
+  Poly P;
+  P.c = c; P.q = q; P.h = h;
+  P.hull = NO_HULL; P.nv = 8; P.nf = 6; P.ne = 12; P.vs = P.fs = P.es = P.is = 0;
+  if (shape >= SHAPE_HULL) {
+    uint32_t hh = (uint32_t)(shape - SHAPE_HULL + 0.5f);
+    uint4 h0 = hulls[hh], h1 = hulls[hh + 1];
+    P.hull = hh;
+    P.vs = h0.x; P.nv = h0.y; P.fs = h0.z; P.nf = h0.w;
+    P.es = h1.x; P.ne = h1.y; P.is = h1.z;
+  }
+  return P;
 }
+__device__ __forceinline__ V vertLocal(const uint4 *hulls, const Poly &P, uint32_t v) {
+  if (P.hull == NO_HULL) return vv((v & 1) ? P.h.x : -P.h.x, (v & 2) ? P.h.y : -P.h.y, (v & 4) ? P.h.z : -P.h.z);
+  uint4 w = hulls[P.vs + v];
+  return vv(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z));
+}
+__device__ __forceinline__ V vert(const uint4 *hulls, const Poly &P, uint32_t v) { return add(P.c, qrotate(P.q, vertLocal(hulls, P, v))); }
+__device__ __forceinline__ V faceNormalLocal(const uint4 *hulls, const Poly &P, uint32_t f) {
+  if (P.hull == NO_HULL) { V n = vv(0.0f, 0.0f, 0.0f); setComp(n, (int)(f / 2), (f & 1) ? -1.0f : 1.0f); return n; }
+  uint4 w = hulls[P.fs + 2 * f];
+  return vv(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z));
+}
+/** Face f's world plane: outward normal n and offset. */
+__device__ __forceinline__ void facePlane(const uint4 *hulls, const Poly &P, uint32_t f, V &n, float &off) {
+  V nl;
+  float d;
+  if (P.hull == NO_HULL) {
+    uint32_t k = f / 2;
+    nl = vv(0.0f, 0.0f, 0.0f);
+    setComp(nl, (int)k, (f & 1) ? -1.0f : 1.0f);
+    d = comp(P.h, (int)k);
+  } else {
+    uint4 w = hulls[P.fs + 2 * f];
+    nl = vv(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z));
+    d = __uint_as_float(w.w);
+  }
+  n = qrotate(P.q, nl);
+  off = d + dot(n, P.c);
+}
+__device__ __forceinline__ uint32_t faceVertCount(const uint4 *hulls, const Poly &P, uint32_t f) {
+  return P.hull == NO_HULL ? 4u : hulls[P.fs + 2 * f + 1].y;
+}
+__device__ __forceinline__ uint32_t faceVert(const uint4 *hulls, const Poly &P, uint32_t f, uint32_t j) {
+  if (P.hull == NO_HULL) return BOX_FACE_VERTS[f * 4 + j];
+  uint32_t k = hulls[P.fs + 2 * f + 1].x + j;
+  uint4 w = hulls[P.is + k / 4];
+  uint32_t r = k % 4;
+  return r == 0 ? w.x : (r == 1 ? w.y : (r == 2 ? w.z : w.w));
+}
+__device__ __forceinline__ uint4 edgeOf(const uint4 *hulls, const Poly &P, uint32_t e) { return P.hull == NO_HULL ? BOX_EDGES[e] : hulls[P.es + e]; }
+__device__ __forceinline__ V polySupport(const uint4 *hulls, const Poly &P, V dir) {
+  V l = qrotate(qconj(P.q), dir);
+  if (P.hull == NO_HULL) return add(P.c, qrotate(P.q, vv(l.x >= 0.0f ? P.h.x : -P.h.x, l.y >= 0.0f ? P.h.y : -P.h.y, l.z >= 0.0f ? P.h.z : -P.h.z)));
+  float best = -3.4e38f;
+  V at = vv(0.0f, 0.0f, 0.0f);
+  for (uint32_t v = 0; v < P.nv; v++) {
+    V p = vertLocal(hulls, P, v);
+    float s = dot(p, l);
+    if (s > best) { best = s; at = p; }
+  }
+  return add(P.c, qrotate(P.q, at));
+}
+__device__ __forceinline__ void queryFaces(const uint4 *hulls, const Poly &A, const Poly &B, float &sep, uint32_t &face) {
+  sep = -3.4e38f;
+  face = 0;
+  for (uint32_t f = 0; f < A.nf; f++) {
+    V n;
+    float d;
+    facePlane(hulls, A, f, n, d);
+    float s = dot(n, polySupport(hulls, B, neg(n))) - d;
+    if (s > sep) { sep = s; face = f; }
+  }
+}
+__device__ __forceinline__ bool minkowskiFace(V a, V b, V c, V d) {
+  V bxa = cross(b, a), dxc = cross(d, c);
+  float cba = dot(c, bxa), dba = dot(d, bxa), adc = dot(a, dxc), bdc = dot(b, dxc);
+  return cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f;
+}
+struct EdgeQuery { float sep; uint32_t ea, eb; V n; bool valid; };
+__device__ __noinline__ EdgeQuery queryEdges(const uint4 *hulls, const Poly &A, const Poly &B) {
+  EdgeQuery out;
+  out.sep = -3.4e38f; out.ea = out.eb = 0; out.n = vv(0.0f, 0.0f, 0.0f); out.valid = false;
+  Q toB = qmul(qconj(B.q), A.q);
+  for (uint32_t i = 0; i < A.ne; i++) {
+    uint4 ea = edgeOf(hulls, A, i);
+    V a = qrotate(toB, faceNormalLocal(hulls, A, ea.z)), b = qrotate(toB, faceNormalLocal(hulls, A, ea.w));
+    for (uint32_t j = 0; j < B.ne; j++) {
+      uint4 eb = edgeOf(hulls, B, j);
+      if (!minkowskiFace(a, b, neg(faceNormalLocal(hulls, B, eb.z)), neg(faceNormalLocal(hulls, B, eb.w)))) continue;
+      V pa = vert(hulls, A, ea.x), qa = vert(hulls, A, ea.y), pb = vert(hulls, B, eb.x), qb = vert(hulls, B, eb.y);
+      V n = cross(sub(qa, pa), sub(qb, pb));
+      float l = length(n);
+      if (l < 1e-5f * length(sub(qa, pa)) * length(sub(qb, pb))) continue;
+      n = divs(n, l);
+      if (dot(n, sub(pa, A.c)) < 0.0f) n = neg(n);
+      float s = dot(n, sub(pb, pa));
+      if (s > out.sep) { out.sep = s; out.ea = i; out.eb = j; out.n = n; out.valid = true; }
+    }
+  }
+  return out;
+}
+__device__ __forceinline__ uint32_t clipHull(const V *src, uint32_t n, V *dst, V pn, float offset) {
+  if (n == 0) return 0;
+  uint32_t count = 0;
+  V a = src[n - 1];
+  float da = dot(pn, a) - offset;
+  for (uint32_t i = 0; i < n; i++) {
+    V b = src[i];
+    float db = dot(pn, b) - offset;
+    if ((da <= 1.0e-5f) != (db <= 1.0e-5f)) {
+      float t = 0.0f;
+      if (fabsf(da - db) > 1.0e-6f) t = clampf(da / (da - db), 0.0f, 1.0f);
+      if (count < MAX_CLIP) dst[count++] = add(a, mul(sub(b, a), t));
+    }
+    if (db <= 1.0e-5f && count < MAX_CLIP) dst[count++] = b;
+    a = b;
+    da = db;
+  }
+  return count;
+}
+__device__ __noinline__ void hullFaceContact(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, Found &found) {
+  found.count = 0;
+  const Poly &R = refIsA ? A : B;
+  const Poly &I = refIsA ? B : A;
+  V rn;
+  float pw;
+  facePlane(hulls, R, refFace, rn, pw);
+  uint32_t deepest = 0;
+  float low = 3.4e38f;
+  for (uint32_t v = 0; v < I.nv; v++) { float d = dot(vert(hulls, I, v), rn); if (d < low) { low = d; deepest = v; } }
+  uint32_t inc = 0;
+  float least = 3.4e38f;
+  for (uint32_t f = 0; f < I.nf; f++) {
+    bool touches = false;
+    uint32_t cnt = faceVertCount(hulls, I, f);
+    for (uint32_t j = 0; j < cnt; j++) if (faceVert(hulls, I, f, j) == deepest) { touches = true; break; }
+    if (!touches) continue;
+    V fn;
+    float fd;
+    facePlane(hulls, I, f, fn, fd);
+    float d = dot(fn, rn);
+    if (d < least) { least = d; inc = f; }
+  }
+  V poly[64], tmp[64];
+  uint32_t n = faceVertCount(hulls, I, inc);
+  if (n > MAX_CLIP) n = MAX_CLIP;
+  for (uint32_t j = 0; j < n; j++) poly[j] = vert(hulls, I, faceVert(hulls, I, inc, j));
+  uint32_t k = faceVertCount(hulls, R, refFace);
+  V centre = vv(0.0f, 0.0f, 0.0f);
+  for (uint32_t j = 0; j < k; j++) centre = add(centre, vert(hulls, R, faceVert(hulls, R, refFace, j)));
+  centre = divs(centre, (float)k);
+  for (uint32_t j = 0; j < k && n > 0; j++) {
+    V v0 = vert(hulls, R, faceVert(hulls, R, refFace, j)), v1 = vert(hulls, R, faceVert(hulls, R, refFace, (j + 1) % k));
+    V s = cross(sub(v1, v0), rn);
+    float l = length(s);
+    if (l < 1.0e-6f) continue;
+    s = divs(s, l);
+    if (dot(s, sub(centre, v0)) > 0.0f) s = neg(s);
+    n = clipHull(poly, n, tmp, s, dot(s, v0));
+    for (uint32_t m = 0; m < n; m++) poly[m] = tmp[m];
+  }
+  float depth[64];
+  uint32_t index[64];
+  uint32_t keep = 0;
+  for (uint32_t m = 0; m < n; m++) {
+    float dist = dot(rn, poly[m]) - pw;
+    if (dist <= 1.0e-5f) { tmp[keep] = poly[m]; depth[keep] = dist; index[keep] = m; keep++; }
+  }
+  uint32_t prefix = ((refIsA ? 0u : 1u) << 24) | ((refFace < 255u ? refFace : 255u) << 16) | ((inc < 255u ? inc : 255u) << 8);
+  uint64_t chosen = 0;
+  uint32_t picks = keep < 8u ? keep : 8u;
+  for (uint32_t c = 0; c < picks; c++) {
+    uint32_t pick = 0;
+    float bestScore = -3.4e38f;
+    for (uint32_t m = 0; m < keep; m++) {
+      if ((chosen >> m) & 1ull) continue;
+      float score = -depth[m];
+      if (c > 0) {
+        score = 3.4e38f;
+        for (uint32_t o = 0; o < keep; o++)
+          if ((chosen >> o) & 1ull) { V dd = sub(tmp[m], tmp[o]); score = mn(score, dot(dd, dd)); }
+      }
+      if (score > bestScore) { bestScore = score; pick = m; }
+    }
+    chosen |= 1ull << pick;
+    V pt = tmp[pick];
+    V onRef = sub(pt, mul(rn, depth[pick]));
+    uint32_t idx = index[pick] < 255u ? index[pick] : 255u;
+    addFound(found, refIsA ? onRef : pt, refIsA ? pt : onRef, prefix | idx);
+  }
+  if (found.count == 0) {
+    V nAB = refIsA ? rn : neg(rn);
+    addFound(found, polySupport(hulls, A, nAB), polySupport(hulls, B, neg(nAB)), prefix);
+  }
+}
+__device__ __forceinline__ void hullEdgeContact(const uint4 *hulls, const Poly &A, const Poly &B, const EdgeQuery &q, Found &found) {
+  found.count = 0;
+  uint4 ea = edgeOf(hulls, A, q.ea), eb = edgeOf(hulls, B, q.eb);
+  V p0 = vert(hulls, A, ea.x), q0 = vert(hulls, B, eb.x);
+  V d1 = sub(vert(hulls, A, ea.y), p0), d2 = sub(vert(hulls, B, eb.y), q0), r = sub(p0, q0);
+  float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r), c = dot(d1, r), b = dot(d1, d2);
+  float denom = a * e - b * b;
+  const float EPS = 1.0e-6f;
+  float s = 0.0f;
+  if (fabsf(denom) > EPS * a * e) s = clampf((b * f - c * e) / denom, 0.0f, 1.0f);
+  float t = (b * s + f) / mx(e, EPS);
+  if (t < 0.0f) { t = 0.0f; s = clampf(-c / mx(a, EPS), 0.0f, 1.0f); }
+  else if (t > 1.0f) { t = 1.0f; s = clampf((b - c) / mx(a, EPS), 0.0f, 1.0f); }
+  uint32_t ia = q.ea < 255u ? q.ea : 255u, ib = q.eb < 255u ? q.eb : 255u;
+  addFound(found, add(p0, mul(d1, s)), add(q0, mul(d2, t)), (2u << 24) | (ia << 8) | ib);
+}
+/** Two convex shapes, at least one a hull; sat.n from A towards B. */
+__device__ __noinline__ void collidePoly(const uint4 *hulls, const Poly &A, const Poly &B, bool faceBias, Sat &sat, Found &found) {
+  found.count = 0;
+  float faSep, fbSep;
+  uint32_t faFace, fbFace;
+  queryFaces(hulls, A, B, faSep, faFace);
+  if (faSep > 0.0f) return;
+  queryFaces(hulls, B, A, fbSep, fbFace);
+  if (fbSep > 0.0f) return;
+  EdgeQuery eq = queryEdges(hulls, A, B);
+  if (eq.valid && eq.sep > 0.0f) return;
+  float faceSep = mx(faSep, fbSep);
+  bool edgeWins = eq.valid && eq.sep > 0.95f * faceSep + 0.01f;
+  if (!faceBias) edgeWins = eq.valid && 0.95f * eq.sep > faceSep + 0.01f;
+  if (edgeWins) { sat.n = eq.n; hullEdgeContact(hulls, A, B, eq, found); return; }
+  V n;
+  float d;
+  if (fbSep > faSep) {
+    facePlane(hulls, B, fbFace, n, d);
+    sat.n = neg(n);
+    hullFaceContact(hulls, A, B, false, fbFace, found);
+    return;
+  }
+  facePlane(hulls, A, faFace, n, d);
+  sat.n = n;
+  hullFaceContact(hulls, A, B, true, faFace, found);
+}
+__device__ __forceinline__ V segmentPoint(V a, V b, V p) {
+  V d = sub(b, a);
+  return add(a, mul(d, clampf(dot(sub(p, a), d) / mx(dot(d, d), 1e-20f), 0.0f, 1.0f)));
+}
+/** Sphere (centre c, radius r) against hull X: n from the sphere towards X. */
+__device__ __noinline__ SphereHit sphereHull(const uint4 *hulls, V c, float r, const Poly &X) {
+  SphereHit out;
+  out.hit = false;
+  float deepest = -3.4e38f;
+  uint32_t deepFace = 0;
+  for (uint32_t f = 0; f < X.nf; f++) {
+    V n;
+    float d;
+    facePlane(hulls, X, f, n, d);
+    float s = dot(n, c) - d;
+    if (s > deepest) { deepest = s; deepFace = f; }
+  }
+  if (deepest > r) return out;
+  if (deepest <= 0.0f) {
+    V n;
+    float d;
+    facePlane(hulls, X, deepFace, n, d);
+    out.hit = true;
+    out.n = neg(n);
+    out.onBox = sub(c, mul(n, deepest));
+    out.onSphere = sub(c, mul(n, r));
+    return out;
+  }
+  V best = vv(0.0f, 0.0f, 0.0f);
+  float bestDist = 3.4e38f;
+  for (uint32_t f = 0; f < X.nf; f++) {
+    V pn;
+    float pd;
+    facePlane(hulls, X, f, pn, pd);
+    float s = dot(pn, c) - pd;
+    if (s <= 0.0f) continue;
+    V p = sub(c, mul(pn, s));
+    uint32_t k = faceVertCount(hulls, X, f);
+    V centre = vv(0.0f, 0.0f, 0.0f);
+    for (uint32_t j = 0; j < k; j++) centre = add(centre, vert(hulls, X, faceVert(hulls, X, f, j)));
+    centre = divs(centre, (float)k);
+    bool inside = true;
+    V edgeBest = vv(0.0f, 0.0f, 0.0f);
+    float edgeDist = 3.4e38f;
+    for (uint32_t j = 0; j < k; j++) {
+      V v0 = vert(hulls, X, faceVert(hulls, X, f, j)), v1 = vert(hulls, X, faceVert(hulls, X, f, (j + 1) % k));
+      V side = cross(sub(v1, v0), pn);
+      if (dot(side, sub(centre, v0)) > 0.0f) side = neg(side);
+      if (dot(side, sub(p, v0)) > 0.0f) inside = false;
+      V q = segmentPoint(v0, v1, c);
+      V dq3 = sub(q, c);
+      float dq = dot(dq3, dq3);
+      if (dq < edgeDist) { edgeDist = dq; edgeBest = q; }
+    }
+    if (!inside) p = edgeBest;
+    float dist = length(sub(p, c));
+    if (dist < bestDist) { bestDist = dist; best = p; }
+  }
+  if (bestDist > r) return out;
+  V n = vv(0.0f, 0.0f, 1.0f);
+  if (bestDist > 0.0f) n = divs(sub(c, best), bestDist);
+  out.hit = true;
+  out.n = neg(n);
+  out.onBox = best;
+  out.onSphere = sub(c, mul(n, r));
+  return out;
+}
+
+/** The hull branch of the narrowphase (wgsl-collision.ts:781-795); sat.n from A towards B. */
+__device__ __noinline__ void collideHullPair(const uint4 *hulls, const Box &A, Q qA, float shA, const Box &B, Q qB, float shB,
+                                             bool sphereA, bool sphereB, bool faceBias, Sat &sat, Found &found) {
+  found.count = 0;
+  if (sphereA || sphereB) {
+    if (sphereA) {
+      SphereHit h = sphereHull(hulls, A.c, A.h.x, polyOf(hulls, B.c, qB, B.h, shB));
+      sat.n = h.n;
+      if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+    } else {
+      SphereHit h = sphereHull(hulls, B.c, B.h.x, polyOf(hulls, A.c, qA, A.h, shA));
+      sat.n = neg(h.n);
+      if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+    }
+    return;
+  }
+  collidePoly(hulls, polyOf(hulls, A.c, qA, A.h, shA), polyOf(hulls, B.c, qB, B.h, shB), faceBias, sat, found);
+}
+
 }  // namespace avbd

@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
     bool sphereA = shA == SHAPE_SPHERE, sphereB = shB == SHAPE_SPHERE;
     bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
     if (sphereA && sphereB) sphereSphere(A, B, sat, found);
-    else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, sat, found);
+    else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
     else if (sphereA || sphereB) {
       if (sphereA) {
         SphereHit h = sphereBox(A, B, qB);
