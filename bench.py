@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X AVBD step on one GPU: body-substeps/s.
+
+Default workload: the large scene, 100,000 bodies (boxes, spheres, convex hulls 1:1:1) in a
+walled bin plus 200 chains of 20 ball-jointed boxes hanging from static anchors, 4 substeps x
+10 iterations per step, f32. A lead-in of `--warmup` steps lets the pile fall and settle; then
+exactly `--steps` steps (each `substeps` substeps: the first detects contacts, the rest reuse
+manifolds) are timed between device synchronisations, bodies resident in HBM, no readback.
+
+Also printed on the JSON line:
+- roofline: the dominant kernel (the cooperative solve kernel: warm starts, iterations x
+  (per-colour primal, dual), velocities) -- algorithmic bytes per launch (the unique-touch model
+  below, from the counters read back after the timed run) / its average launch duration, timed
+  with hipEvents on the solver's stream, against the MI355X HBM peak of 8 TB/s;
+- cpu_baseline: the oracle's f32 restatement of the same step (single thread), seeded with the
+  GPU's state after the timed run and stepped for a bounded sample of substeps on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "three-avbd_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def solve_bytes(n, m, k, j, iters):
+    """Algorithmic bytes of one solve launch (unique-touch model of the record fields each pass
+    reads or writes; N bodies, M manifolds, K contact points, J joints):
+      primal sweep  160N + 4(2N + 2(M+J)) + 4N + 32M + 64K + 176J
+      dual          64N + 32M + 92K + 240J
+      warm starts + velocities  368N + 304J"""
+    primal = 160 * n + 4 * (2 * n + 2 * (m + j)) + 4 * n + 32 * m + 64 * k + 176 * j
+    dual = 64 * n + 32 * m + 92 * k + 240 * j
+    return iters * (primal + dual) + 368 * n + 304 * j
+
+
+def build_workload(name, workers):
+    from avbd_amd import scenes
+    if name == "large-100k":
+        bodies, joints, info, hulls = scenes.large_scene(100000, seed=2026, workers=workers)
+        return bodies, joints, info, hulls, 4, 10
+    if name == "pile-10k":
+        bodies, joints, info = scenes.random_pile(10000, seed=12345)
+        return bodies, joints, info, None, 1, 10
+    if name == "pyramid-55":
+        bodies, joints, info = scenes.pyramid(10)
+        return bodies, joints, info, None, 1, 10
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(solver, hulls, params, substeps, budget_s=15.0, max_substeps=8):
+    """The oracle's f32 step on one host thread from the GPU's current state; stops after the
+    first whole step (all `substeps`) that takes the elapsed time past budget_s."""
+    import numpy as np
+    import oracle
+    from avbd_amd import records
+    m = oracle.Model(32)
+    p = solver.params
+    m.set_params(oracle.make_params(dt=p.dt, gravity=p.gravity, iterations=p.iterations, alpha=p.alpha, beta_lin=p.beta_lin,
+                                    beta_ang=p.beta_ang, gamma=p.gamma, flags=p.flags, color_cap=p.color_cap,
+                                    color_rounds=p.color_rounds, margin=p.contact_margin, up=tuple(p.up),
+                                    lanes=tuple(p.primal_lanes)))
+    m.set_bodies(records.bodies_to_oracle(solver.read_bodies()))
+    if solver.joint_count:
+        m.set_joints(solver.read_joints().astype(np.float64), solver._joint_info)
+    if hulls is not None:
+        m.set_hulls(hulls)
+    man, con = solver.read_contacts_raw()
+    m.set_prev(*records.contacts_to_oracle(man, con))
+    m.set_color_state(solver.read_colors()[0])
+    m.set_step_count(solver.step_count)
+    done = 0
+    t0 = time.perf_counter()
+    while done < max_substeps:
+        for u in range(substeps):
+            m.step(1, reuse_manifolds=u > 0)
+            done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return done, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55"])
+    ap.add_argument("--workers", type=int, default=8, help="host processes building the scene's hulls")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (worlds would shard across GPUs "
+              "with no per-step exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    bodies, joints, info, hulls, substeps, iters = build_workload(args.workload, args.workers)
+    print(f"# scene {args.workload}: {len(bodies)} bodies, {len(joints)} joints, built in {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+
+    import torch
+    import avbd_amd
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    params = avbd_amd.default_params(dt=1.0 / (60.0 * substeps), iterations=iters, color_cap=32, color_rounds=16)
+    n = len(bodies)
+    solver = avbd_amd.GpuSolver3D(bodies, joints, info, params=params, hulls=hulls,
+                                  capacity={"pairs": 12 * n, "manifolds": 12 * n, "contacts": 40 * n})
+    t = time.perf_counter()
+    solver.steps(args.warmup, substeps)
+    solver.sync()
+    print(f"# lead-in {args.warmup} steps x {substeps} substeps in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.steps(args.steps, substeps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    c = solver.read_counters()
+    ev_ms = solver.time_steps(2, substeps)  # hipEvent-bracketed cross-check of the wall clock
+
+    # roofline of the dominant kernel: HIP events around every launch of it, then bytes from counters
+    ks = solver.profile_kernels(2, substeps)
+    c2 = solver.read_counters()
+    if ks["solve_launches"]:
+        avg_ms = ks["solve_ms"] / ks["solve_launches"]
+        kernel = "k_solve (cooperative: warm starts, iterations x (primal per colour, dual), velocities)"
+        nbytes = solve_bytes(n, c2["manifolds"], c2["contacts"], len(joints), iters)
+    else:
+        avg_ms = ks["primal_ms"] / max(1, ks["primal_launches"])
+        kernel = "k_primal (one colour)"
+        nbytes = solve_bytes(n, c2["manifolds"], c2["contacts"], len(joints), 1) / max(1, c2["colors"])
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+
+    value = n * args.steps * substeps / wall
+    out = {
+        "metric": "body-substeps/s @100k bodies",
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload, "bodies": n, "joints": int(len(joints)), "substeps": substeps, "iterations": iters,
+            "contacts": c["contacts"], "manifolds": c["manifolds"], "pairs": c["pairs"], "colors": c["colors"],
+            "clashes": c["clashes"], "overflow": c["overflow"], "parallelism": "single-gpu",
+            "event_ms_per_step": ev_ms / 2,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
+            "launches": ks["solve_launches"] or ks["primal_launches"], "solve_blocks": ks["solve_blocks"],
+        },
+    }
+    if not args.no_cpu_baseline:
+        done, dt = cpu_baseline(solver, hulls, params, substeps)
+        out["cpu_baseline"] = {
+            "value": n * done / dt, "unit": "body-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{done} substeps of the same {n}-body state (after the timed run) through the oracle's f32 "
+                      f"restatement of the step on one host thread, {dt:.1f} s",
+        }
+    solver.destroy()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

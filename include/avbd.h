@@ -85,9 +85,9 @@ typedef struct avbd_profile {
 
 /* Per-kernel timing over a profiled run (the roofline's launch durations). */
 typedef struct avbd_kernel_stats {
-  float primal_ms, dual_ms, narrowphase_ms, broadphase_ms, other_ms;
-  uint32_t primal_launches, dual_launches, narrowphase_launches, broadphase_launches;
-  uint64_t primal_bytes, dual_bytes; /* algorithmic bytes summed over those launches */
+  float primal_ms, dual_ms, narrowphase_ms, broadphase_ms, solve_ms, other_ms;
+  uint32_t primal_launches, dual_launches, narrowphase_launches, broadphase_launches, solve_launches;
+  uint32_t solve_blocks; /* grid of the cooperative solve kernel (0: per-colour launches) */
 } avbd_kernel_stats;
 
 typedef struct avbd_step_opts {

@@ -230,3 +230,37 @@ def test_hulls_spheres_boxes_chains_bit_exact(gpu):
     assert_same_contacts(s, m)
     assert np.array_equal(s.read_joints().astype(np.float64), m.joints().astype(np.float32).astype(np.float64))
     s.destroy()
+
+
+def test_per_colour_launch_path_matches(gpu, monkeypatch):
+    say("case: per-colour launches (no cooperative solve) vs oracle, pile 400 x 10 steps")
+    monkeypatch.setenv("AVBD_SOLVE", "colors")
+    bodies, _, _ = scenes.random_pile(400, seed=6, bin_size=10.0)
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=24)
+    s = gpu_for(bodies, kw)
+    m = model_for(bodies, kw)
+    for _ in range(10):
+        s.step()
+        m.step(1)
+    assert s.profile_kernels(1)["solve_launches"] == 0
+    m.step(1)
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    s.destroy()
+
+
+def test_substeps_reuse_manifolds_match(gpu):
+    say("case: 4 substeps (reuse manifolds) hull/box/sphere scene, 5 steps")
+    bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32, dt=1 / 240)
+    s = gpu_for(bodies, kw, joints, info, hulls=hw)
+    m = model_for(bodies, kw, joints, info)
+    m.set_hulls(hw)
+    for _ in range(5):
+        s.steps(1, 4)
+        for u in range(4):
+            m.step(1, reuse_manifolds=u > 0)
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    assert_same_contacts(s, m)
+    s.destroy()

@@ -65,10 +65,10 @@ class Profile(ctypes.Structure):
 class KernelStats(ctypes.Structure):
     _fields_ = [
         ("primal_ms", ctypes.c_float), ("dual_ms", ctypes.c_float), ("narrowphase_ms", ctypes.c_float),
-        ("broadphase_ms", ctypes.c_float), ("other_ms", ctypes.c_float),
+        ("broadphase_ms", ctypes.c_float), ("solve_ms", ctypes.c_float), ("other_ms", ctypes.c_float),
         ("primal_launches", ctypes.c_uint32), ("dual_launches", ctypes.c_uint32),
         ("narrowphase_launches", ctypes.c_uint32), ("broadphase_launches", ctypes.c_uint32),
-        ("primal_bytes", ctypes.c_uint64), ("dual_bytes", ctypes.c_uint64),
+        ("solve_launches", ctypes.c_uint32), ("solve_blocks", ctypes.c_uint32),
     ]
 
 
@@ -151,6 +151,8 @@ class GpuSolver3D:
         self.joint_count = len(joints)
         self.fixed_colors = None
         self.destroyed = False
+        self.step_count = 0
+        self._joint_info = joint_info
         if hulls is not None and len(hulls):
             self._check(self.L.avbd_set_hulls(self.h, _p(np.ascontiguousarray(hulls, np.uint32)), len(hulls)))
         if ignore_pairs is not None and len(ignore_pairs):
@@ -199,10 +201,12 @@ class GpuSolver3D:
         """GpuSolver3D.step (solver.ts:2112): one substep, enqueued on the solver's stream."""
         self._live()
         self._check(self.L.avbd_step(self.h, ctypes.byref(StepOpts(int(reuse_manifolds)))))
+        self.step_count += 1
 
     def steps(self, steps, substeps=1):
         self._live()
         self._check(self.L.avbd_steps(self.h, steps, substeps))
+        self.step_count += steps * substeps
 
     def sync(self):
         self._check(self.L.avbd_sync(self.h))
@@ -210,16 +214,19 @@ class GpuSolver3D:
     def time_steps(self, steps, substeps=1):
         ms = ctypes.c_float()
         self._check(self.L.avbd_time_steps(self.h, steps, substeps, ctypes.byref(ms)))
+        self.step_count += steps * substeps
         return ms.value
 
     def profile_next_step(self, reuse_manifolds=False):
         out = Profile()
         self._check(self.L.avbd_profile_step(self.h, ctypes.byref(StepOpts(int(reuse_manifolds))), ctypes.byref(out)))
+        self.step_count += 1
         return {k: getattr(out, k) for k, _ in Profile._fields_}
 
     def profile_kernels(self, steps, substeps=1):
         out = KernelStats()
         self._check(self.L.avbd_profile_kernels(self.h, steps, substeps, ctypes.byref(out)))
+        self.step_count += steps * substeps
         return {k: getattr(out, k) for k, _ in KernelStats._fields_}
 
     # --- reads -------------------------------------------------------------------------------

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -28,7 +29,7 @@ struct KernelTimer {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;  // kind, (start, stop)
 };
 
-enum Kind { K_PRIMAL = 0, K_DUAL = 1, K_NARROW = 2, K_BROAD = 3, K_OTHER = 4 };
+enum Kind { K_PRIMAL = 0, K_DUAL = 1, K_NARROW = 2, K_BROAD = 3, K_SOLVE = 4, K_OTHER = 5 };
 
 uint32_t pow2At(uint64_t x) {
   uint64_t p = 1;
@@ -89,6 +90,7 @@ struct avbd_solver {
   bool manifoldsValid = false;
   bool anyDynamicLarge = false;
   bool destroyed = false;
+  uint32_t coopBlocks = 0;  // grid of the cooperative solve kernel; 0: per-colour launches
   std::string err;
   std::vector<DevBuf> bufs;
   // device storage
@@ -256,6 +258,14 @@ struct avbd_solver {
   }
   void solve() {
     Dev d = dev();
+    if (coopBlocks) {
+      void *args[] = {&d};
+      hipError_t e;
+      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(64), args, 0, stream); });
+      if (e == hipSuccess) return;
+      (void)hipGetLastError();
+      coopBlocks = 0;  // not co-resident here: per-colour launches from now on
+    }
     if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
     k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
@@ -319,6 +329,19 @@ int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_co
   try {
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
     s->hp = *params;
+    {
+      int dev = 0, coop = 0, cus = 0, perCU = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_solve, 64, 0);
+      const char *mode = getenv("AVBD_SOLVE");
+      int want = 8;
+      if (const char *b = getenv("AVBD_SOLVE_BLOCKS_PER_CU")) want = atoi(b);
+      if (want < 1) want = 1;
+      if (perCU > want) perCU = want;
+      if (coop && cus > 0 && perCU > 0 && !(mode && std::string(mode) == "colors")) s->coopBlocks = (uint32_t)(perCU * cus);
+    }
     s->n = body_count;
     s->nj = joint_count;
     s->bodyCap = std::max({cap.bodies, body_count, 1u});
@@ -668,6 +691,7 @@ int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd
     for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
   s->timer.on = false;
   int rc = avbd_sync(s);
+  out->solve_blocks = s->coopBlocks;
   for (auto &m : s->timer.marks) {
     float t = 0;
     hipEventElapsedTime(&t, m.second.first, m.second.second);
@@ -676,6 +700,7 @@ int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd
       case K_DUAL: out->dual_ms += t; out->dual_launches++; break;
       case K_NARROW: out->narrowphase_ms += t; out->narrowphase_launches++; break;
       case K_BROAD: out->broadphase_ms += t; out->broadphase_launches++; break;
+      case K_SOLVE: out->solve_ms += t; out->solve_launches++; break;
       default: out->other_ms += t; break;
     }
     hipEventDestroy(m.second.first);

@@ -7,6 +7,9 @@
 #pragma once
 #include "avbd_device.h"
 #include "avbd_hull.h"
+#include <hip/hip_cooperative_groups.h>
+
+namespace cg = cooperative_groups;
 
 namespace avbd {
 
@@ -793,18 +796,19 @@ __device__ __forceinline__ void reduceLanes(Acc &acc, uint32_t lanes) {
     for (int e = 0; e < 3; e++) { acc.rl[e] += __shfl_xor(acc.rl[e], (int)s); acc.ra[e] += __shfl_xor(acc.ra[e], (int)s); }
   }
 }
-/** One colour's primal: lanesFor(size) threads per body in 64-wide wavefronts, lane partials
- *  reduced by cross-lane shuffles, lane 0 solves. A clashing colour (bodies that share a
- *  constraint) runs in one workgroup, chunk after chunk, reads before writes in each chunk. */
-__global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) {
+/** One colour's primal over blocks [block, block + nblocks·k): lanesFor(size) threads per body in
+ *  64-wide wavefronts, lane partials reduced by cross-lane shuffles, lane 0 solves. A clashing
+ *  colour (bodies that share a constraint) runs in block 0 alone, chunk after chunk, every read
+ *  of a chunk before any of its writes (the reference's serial workgroup, wgsl-solve.ts:363-374). */
+__device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t block, uint32_t nblocks) {
   uint32_t start = d.colorStart[c], count = d.colorStart[c + 1] - start;
   if (count == 0) return;
   uint32_t lanes = lanesFor(d.p, count), per = 64 / lanes;
   uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes;
   bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
-  if (serial && blockIdx.x != 0) return;
-  uint32_t stride = serial ? per : gridDim.x * per;
-  for (uint32_t base = serial ? 0 : blockIdx.x * per; base < count; base += stride) {
+  if (serial && block != 0) return;
+  uint32_t stride = serial ? per : nblocks * per;
+  for (uint32_t base = serial ? 0 : block * per; base < count; base += stride) {
     uint32_t k = base + slot;
     bool live = k < count;
     uint32_t i = live ? d.colorBodies[start + k] : 0;
@@ -817,6 +821,7 @@ __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) {
     if (serial) { __threadfence_block(); __syncthreads(); }
   }
 }
+__global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
 /** dualJoint (wgsl-solve.ts:562-635) without the plastic (yield) extension. */
 __device__ __noinline__ void dualJoint(const Dev &d, uint32_t j, float alpha) {
   int4 in = d.jinfo[j];
@@ -874,16 +879,17 @@ __device__ __forceinline__ void dualManifold(const Dev &d, uint32_t m, float alp
     d.con[c].lam = f4(e.F, k.lam.w);
   }
 }
-__global__ void __launch_bounds__(256) k_dual(Dev d) {
+__device__ __forceinline__ void dualAll(const Dev &d, uint32_t t0, uint32_t nt) {
   uint32_t total = d.nj + manifoldCount(d);
-  GRID_STRIDE(id, total) {
+  for (uint32_t id = t0; id < total; id += nt) {
     if (id < d.nj) dualJoint(d, id, d.p.alpha);
     else dualManifold(d, id - d.nj, d.p.alpha);
   }
 }
+__global__ void __launch_bounds__(256) k_dual(Dev d) { dualAll(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
 /** warmStartJoints (wgsl-solve.ts:236-260). */
-__global__ void k_warm_joints(Dev d) {
-  GRID_STRIDE(j, d.nj) {
+__device__ __forceinline__ void warmJoints(const Dev &d, uint32_t t0, uint32_t nt) {
+  for (uint32_t j = t0; j < d.nj; j += nt) {
     int4 in = d.jinfo[j];
     if (in.x != (int)T_JOINT) continue;
     float4 *k = d.joints + 10 * j;
@@ -897,9 +903,10 @@ __global__ void k_warm_joints(Dev d) {
     k[1] = make_float4(mn(clampf(pa.x * g, PENALTY_MIN, PENALTY_MAX), pa.w), mn(clampf(pa.y * g, PENALTY_MIN, PENALTY_MAX), pa.w), mn(clampf(pa.z * g, PENALTY_MIN, PENALTY_MAX), pa.w), pa.w);
   }
 }
+__global__ void k_warm_joints(Dev d) { warmJoints(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
 /** warmStartBodies (wgsl-solve.ts:293-335): inertial target, adaptive warm start, x-. */
-__global__ void k_warm_bodies(Dev d) {
-  GRID_STRIDE(i, d.n) {
+__device__ __forceinline__ void warmBodies(const Dev &d, uint32_t t0, uint32_t nt) {
+  for (uint32_t i = t0; i < d.n; i += nt) {
     float4 pos = d.b.pos[i], vel4 = d.b.vel[i], av4 = d.b.angvel[i];
     Q rot = q4(d.b.rot[i]);
     V vel = xyz(vel4), angVel = xyz(av4);
@@ -930,9 +937,10 @@ __global__ void k_warm_bodies(Dev d) {
     }
   }
 }
+__global__ void k_warm_bodies(Dev d) { warmBodies(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
 /** updateVelocities (wgsl-solve.ts:672-684): BDF1. */
-__global__ void k_velocities(Dev d) {
-  GRID_STRIDE(i, d.n) {
+__device__ __forceinline__ void velocities(const Dev &d, uint32_t t0, uint32_t nt) {
+  for (uint32_t i = t0; i < d.n; i += nt) {
     V up = vv(d.p.upx, d.p.upy, d.p.upz);
     float4 vel4 = d.b.vel[i];
     float prevUp = dot(xyz(vel4), up);
@@ -945,6 +953,31 @@ __global__ void k_velocities(Dev d) {
       d.b.vel[i] = vel4;
     }
   }
+}
+__global__ void k_velocities(Dev d) { velocities(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
+
+/** The whole solve of one substep in one cooperative launch (GpuSolver3D.step's solve pass,
+ *  solver.ts:2226-2261): warm starts, iterations x (primal of each colour in use, dual), BDF1
+ *  velocities, with a grid-wide barrier wherever the reference ends a dispatch. Sweeps only the
+ *  colours the colouring used (the device's colour count), not the whole cap. */
+__global__ void __launch_bounds__(64) k_solve(Dev d) {
+  cg::grid_group grid = cg::this_grid();
+  uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  warmJoints(d, t0, nt);
+  grid.sync();
+  warmBodies(d, t0, nt);
+  grid.sync();
+  uint32_t colors = d.counters[C_NUM_COLORS];
+  if (colors > d.p.colorCap) colors = d.p.colorCap;
+  for (uint32_t it = 0; it < d.p.iterations; it++) {
+    for (uint32_t c = 0; c < colors; c++) {
+      primalColor(d, c, blockIdx.x, gridDim.x);
+      grid.sync();
+    }
+    dualAll(d, t0, nt);
+    grid.sync();
+  }
+  velocities(d, t0, nt);
 }
 
 // ---------------------------------------------------------------------------------------------
