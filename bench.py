@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55"])
     ap.add_argument("--workers", type=int, default=8, help="host processes building the scene's hulls")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--color-cap", type=int, default=32)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (the HBM counters' calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (worlds would shard across GPUs "
@@ -108,7 +111,16 @@ def main():
     import torch
     import avbd_amd
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    params = avbd_amd.default_params(dt=1.0 / (60.0 * substeps), iterations=iters, color_cap=32, color_rounds=16)
+    if args.copy_calibration_mb:
+        src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: 3 x {args.copy_calibration_mb} MiB read + {args.copy_calibration_mb} MiB written",
+              file=sys.stderr, flush=True)
+        del src, dst
+    params = avbd_amd.default_params(dt=1.0 / (60.0 * substeps), iterations=iters, color_cap=args.color_cap, color_rounds=16)
     n = len(bodies)
     solver = avbd_amd.GpuSolver3D(bodies, joints, info, params=params, hulls=hulls,
                                   capacity={"pairs": 12 * n, "manifolds": 12 * n, "contacts": 40 * n})

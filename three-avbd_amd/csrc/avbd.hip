@@ -91,6 +91,7 @@ struct avbd_solver {
   bool anyDynamicLarge = false;
   bool destroyed = false;
   uint32_t coopBlocks = 0;  // grid of the cooperative solve kernel; 0: per-colour launches
+  uint32_t *barrier = nullptr;  // the solve kernel's grid barrier: arrivals, generation
   std::string err;
   std::vector<DevBuf> bufs;
   // device storage
@@ -259,9 +260,9 @@ struct avbd_solver {
   void solve() {
     Dev d = dev();
     if (coopBlocks) {
-      void *args[] = {&d};
+      void *args[] = {&d, &barrier};
       hipError_t e;
-      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(64), args, 0, stream); });
+      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(256), args, 0, stream); });
       if (e == hipSuccess) return;
       (void)hipGetLastError();
       coopBlocks = 0;  // not co-resident here: per-colour launches from now on
@@ -334,9 +335,9 @@ int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_co
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_solve, 64, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_solve, 256, 0);
       const char *mode = getenv("AVBD_SOLVE");
-      int want = 8;
+      int want = 1;
       if (const char *b = getenv("AVBD_SOLVE_BLOCKS_PER_CU")) want = atoi(b);
       if (want < 1) want = 1;
       if (perCU > want) perCU = want;
@@ -390,6 +391,7 @@ int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_co
     s->fixedDev = s->alloc<uint32_t>(N);
     s->counters = s->alloc<uint32_t>(C_WORDS);
     s->scanSums = s->alloc<uint32_t>(4096);
+    s->barrier = s->alloc<uint32_t>(64);
     s->fill(s->colA, N, NO_COLOR);
     s->fill(s->colB, N, NO_COLOR);
     std::vector<uint32_t> f(2ull * N, 0xffffffffu);

@@ -7,9 +7,6 @@
 #pragma once
 #include "avbd_device.h"
 #include "avbd_hull.h"
-#include <hip/hip_cooperative_groups.h>
-
-namespace cg = cooperative_groups;
 
 namespace avbd {
 
@@ -803,12 +800,30 @@ __device__ __forceinline__ void reduceLanes(Acc &acc, uint32_t lanes) {
 __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t block, uint32_t nblocks) {
   uint32_t start = d.colorStart[c], count = d.colorStart[c + 1] - start;
   if (count == 0) return;
-  uint32_t lanes = lanesFor(d.p, count), per = 64 / lanes;
+  uint32_t lanes = lanesFor(d.p, count);
   uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes;
   bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
-  if (serial && block != 0) return;
-  uint32_t stride = serial ? per : nblocks * per;
-  for (uint32_t base = serial ? 0 : block * per; base < count; base += stride) {
+  if (serial) {
+    // chunks of one 64-wide workgroup's bodies, in order, on block 0's first wavefront
+    if (block != 0) return;
+    uint32_t per = 64 / lanes;
+    for (uint32_t base = 0; base < count; base += per) {
+      uint32_t k = base + slot;
+      bool live = threadIdx.x < 64 && k < count;
+      uint32_t i = live ? d.colorBodies[start + k] : 0;
+      Acc acc;
+      if (live) accumulate(d, i, lane, lanes, d.p.alpha, acc);
+      else accZero(acc);
+      reduceLanes(acc, lanes);
+      __syncthreads();
+      if (live && lane == 0) finishBody(d, i, acc);
+      __threadfence_block();
+      __syncthreads();
+    }
+    return;
+  }
+  uint32_t per = blockDim.x / lanes;
+  for (uint32_t base = block * per; base < count; base += nblocks * per) {
     uint32_t k = base + slot;
     bool live = k < count;
     uint32_t i = live ? d.colorBodies[start + k] : 0;
@@ -816,9 +831,7 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     if (live) accumulate(d, i, lane, lanes, d.p.alpha, acc);
     else accZero(acc);
     reduceLanes(acc, lanes);
-    if (serial) __syncthreads();
     if (live && lane == 0) finishBody(d, i, acc);
-    if (serial) { __threadfence_block(); __syncthreads(); }
   }
 }
 __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
@@ -956,26 +969,56 @@ __device__ __forceinline__ void velocities(const Dev &d, uint32_t t0, uint32_t n
 }
 __global__ void k_velocities(Dev d) { velocities(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
 
+/** Grid-wide barrier for a cooperative (co-resident) grid: one atomic arrival per workgroup on
+ *  bar[0]; the last to arrive resets it and bumps the generation bar[32] (its own 128-byte line),
+ *  the others wait for the bump. The wait is bounded: past 2^22 polls it records bit 8 in the overflow counter and lets
+ *  the workgroup go on, so a barrier can give a wrong result but never hang the device. */
+__device__ __forceinline__ void gridBarrier(uint32_t *bar, uint32_t *counters, uint32_t &gen) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    uint32_t arrived = atomicAdd(&bar[0], 1u);
+    if (arrived == gridDim.x - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicAdd(&bar[32], 1u);
+    } else {
+      // relaxed polls (no cache invalidation per poll), one acquire fence once released
+      uint32_t polls = 0;
+      while (__hip_atomic_load(&bar[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++polls > (1u << 22)) { atomicOr(&counters[C_OVERFLOW], 256u); break; }
+      }
+    }
+    __threadfence();
+  }
+  gen++;
+  __syncthreads();
+}
+
 /** The whole solve of one substep in one cooperative launch (GpuSolver3D.step's solve pass,
  *  solver.ts:2226-2261): warm starts, iterations x (primal of each colour in use, dual), BDF1
  *  velocities, with a grid-wide barrier wherever the reference ends a dispatch. Sweeps only the
  *  colours the colouring used (the device's colour count), not the whole cap. */
-__global__ void __launch_bounds__(64) k_solve(Dev d) {
-  cg::grid_group grid = cg::this_grid();
+__global__ void __launch_bounds__(256) k_solve(Dev d, uint32_t *bar) {
+  __shared__ uint32_t genShared;
+  if (threadIdx.x == 0) genShared = __hip_atomic_load(&bar[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  uint32_t gen = genShared;
   uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   warmJoints(d, t0, nt);
-  grid.sync();
+  gridBarrier(bar, d.counters, gen);
   warmBodies(d, t0, nt);
-  grid.sync();
+  gridBarrier(bar, d.counters, gen);
   uint32_t colors = d.counters[C_NUM_COLORS];
   if (colors > d.p.colorCap) colors = d.p.colorCap;
   for (uint32_t it = 0; it < d.p.iterations; it++) {
     for (uint32_t c = 0; c < colors; c++) {
       primalColor(d, c, blockIdx.x, gridDim.x);
-      grid.sync();
+      gridBarrier(bar, d.counters, gen);
     }
     dualAll(d, t0, nt);
-    grid.sync();
+    gridBarrier(bar, d.counters, gen);
   }
   velocities(d, t0, nt);
 }
