@@ -42,6 +42,10 @@ extern "C" {
 #define AVBD_FLAG_START_AT_REST 8u
 #define AVBD_FLAG_MASS_PENALTY 16u
 #define AVBD_FLAG_DEFAULTS 31u
+/* Host-side: every 10 steps read the counters back and adapt the colour cap and colouring rounds
+ * (GpuSolver3D.adaptColors, solver.ts:2379-2407); the read is synchronous, so runs stay
+ * deterministic. Not in the reference's flag word (World.step drives its adapt, world.ts:1740). */
+#define AVBD_FLAG_ADAPT_COLORS 128u
 
 #define AVBD_BODY_FLOATS 40
 #define AVBD_JOINT_FLOATS 40
@@ -76,6 +80,7 @@ typedef struct avbd_capacity {
 /* readCounters (solver.ts:2305-2308). */
 typedef struct avbd_counters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, color_cap;
+  uint32_t step; /* steps since creation (the contact stamps' clock) */
 } avbd_counters;
 
 /* StepProfile (avbd2d/gpu/layout.ts:289-293) in ms, from hipEvents on the solver's stream. */
@@ -139,6 +144,22 @@ int avbd_add_ignore_pairs(avbd_solver *s, const uint32_t *pairs, uint32_t count)
 int avbd_set_filters(avbd_solver *s, const uint32_t *groups, const uint32_t *collides_with, uint32_t count);
 /* Hull buffer (wgsl-hull.ts header format, vec4u words); bodies reference it by shape code. */
 int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t word_count);
+
+/* adapt's colouring half (solver.ts:2379-2407): colour cap = colours in use + 3 spare, grown at
+ * once on clashes or a spent spare, shrunk after three quiet reads; rounds 4..32. */
+int avbd_adapt(avbd_solver *s, const avbd_counters *counters);
+
+/* How the solve pass is scheduled (results are identical whichever runs). */
+typedef struct avbd_schedule {
+  uint32_t cooperative;      /* 1: one cooperative launch per substep with grid barriers; 0: per-colour launches */
+  uint32_t threads;          /* workgroup size of the cooperative kernel: 64..1024 */
+  uint32_t blocks_per_cu;    /* workgroups per CU of the cooperative kernel */
+} avbd_schedule;
+int avbd_set_schedule(avbd_solver *s, const avbd_schedule *schedule);
+int avbd_get_schedule(const avbd_solver *s, avbd_schedule *schedule);
+/* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
+ * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`. */
+int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);
 
 /* profileNextStep (solver.ts:1952): phase times of the next step, synchronously. */
 int avbd_profile_step(avbd_solver *s, const avbd_step_opts *opts, avbd_profile *out);

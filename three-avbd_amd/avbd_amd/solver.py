@@ -24,6 +24,7 @@ FLAG_REUSE_CONTACTS = 4
 FLAG_START_AT_REST = 8
 FLAG_MASS_PENALTY = 16
 FLAG_DEFAULTS = 31
+FLAG_ADAPT_COLORS = 128
 NO_COLOR = 255
 STICK_BIT = 0x80000000
 BODY_FLOATS = 40
@@ -55,7 +56,7 @@ class Capacity(ctypes.Structure):
 
 
 class Counters(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_uint32) for k in ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "color_cap")]
+    _fields_ = [(k, ctypes.c_uint32) for k in ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "color_cap", "step")]
 
 
 class Profile(ctypes.Structure):
@@ -72,6 +73,10 @@ class KernelStats(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("cooperative", ctypes.c_uint32), ("threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32)]
+
+
 class StepOpts(ctypes.Structure):
     _fields_ = [("reuse_manifolds", ctypes.c_int)]
 
@@ -83,7 +88,8 @@ EXPORTS = [
     "avbd_get_params", "avbd_read_bodies", "avbd_write_bodies", "avbd_read_joints", "avbd_read_counters",
     "avbd_read_contacts", "avbd_read_pairs", "avbd_read_colors", "avbd_seed", "avbd_set_fixed_colors",
     "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
-    "avbd_profile_kernels", "avbd_time_steps", "avbd_sync",
+    "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
+    "avbd_autotune",
 ]
 
 
@@ -228,6 +234,33 @@ class GpuSolver3D:
         self._check(self.L.avbd_profile_kernels(self.h, steps, substeps, ctypes.byref(out)))
         self.step_count += steps * substeps
         return {k: getattr(out, k) for k, _ in KernelStats._fields_}
+
+    def adapt(self, counters=None):
+        """adaptColors (solver.ts:2379): the colour cap and rounds from a counters readback."""
+        c = Counters()
+        for k, v in (counters or self.read_counters()).items():
+            setattr(c, k, int(v))
+        self._check(self.L.avbd_adapt(self.h, ctypes.byref(c)))
+        p = Params()
+        self._check(self.L.avbd_get_params(self.h, ctypes.byref(p)))
+        self._params = p
+
+    @property
+    def schedule(self):
+        s = Schedule()
+        self._check(self.L.avbd_get_schedule(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Schedule._fields_}
+
+    def set_schedule(self, cooperative=1, threads=512, blocks_per_cu=1):
+        self._check(self.L.avbd_set_schedule(self.h, ctypes.byref(Schedule(cooperative, threads, blocks_per_cu))))
+
+    def autotune(self, steps=4, substeps=1):
+        """Times the candidate solve schedules (the simulation advances) and keeps the fastest."""
+        best = Schedule()
+        ms = ctypes.c_float()
+        self._check(self.L.avbd_autotune(self.h, steps, substeps, ctypes.byref(best), ctypes.byref(ms)))
+        self.step_count += 12 * steps * substeps
+        return {k: getattr(best, k) for k, _ in Schedule._fields_}, ms.value
 
     # --- reads -------------------------------------------------------------------------------
     def read_bodies(self, first=0, count=None):

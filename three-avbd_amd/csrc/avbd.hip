@@ -92,6 +92,13 @@ struct avbd_solver {
   bool destroyed = false;
   uint32_t coopBlocks = 0;  // grid of the cooperative solve kernel; 0: per-colour launches
   uint32_t *barrier = nullptr;  // the solve kernel's grid barrier: arrivals, generation
+  uint32_t coopThreads = 512;   // threads per workgroup of the solve kernel
+  uint32_t coopPerCU = 1;       // its workgroups per CU
+  uint32_t coopMaxBlocks = 0;   // co-resident limit for coopThreads (0: no cooperative launch here)
+  uint32_t numCUs = 0;
+  bool coopSupported = false;
+  uint32_t shrinkVotes = 0;
+  uint32_t stepsSinceAdapt = 0;
   std::string err;
   std::vector<DevBuf> bufs;
   // device storage
@@ -133,6 +140,43 @@ struct avbd_solver {
     if (staging) hipFree(staging);
     for (auto &m : timer.marks) { hipEventDestroy(m.second.first); hipEventDestroy(m.second.second); }
     if (stream) hipStreamDestroy(stream);
+  }
+
+  /** Sets the solve schedule; a cooperative one needs its grid co-resident on this device. */
+  int applySchedule(const avbd_schedule &sch) {
+    uint32_t t = sch.threads;
+    if (sch.cooperative) {
+      if (!(t == 64 || t == 128 || t == 256 || t == 512 || t == 1024) || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
+      int perCU = 0;
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_solve, (int)t, 0);
+      if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
+      coopThreads = t;
+      coopPerCU = sch.blocks_per_cu;
+      coopBlocks = coopPerCU * numCUs;
+    } else {
+      coopBlocks = 0;
+    }
+    return AVBD_OK;
+  }
+  avbd_schedule schedule() const { return avbd_schedule{coopBlocks ? 1u : 0u, coopThreads, coopPerCU}; }
+  /** adaptColors (solver.ts:2379-2407). */
+  void adapt(const avbd_counters &c) {
+    if (fixedColors) return;
+    const uint32_t SPARE = 3;
+    uint32_t used = c.colors;
+    if (c.clashes > 0) hp.color_rounds = std::min(32u, hp.color_rounds * 2);
+    if (c.clashes > 0 || hp.color_cap < used + 2) {
+      hp.color_cap = std::min(MAX_COLORS, std::max(used + SPARE + 2, hp.color_cap + 4));
+      shrinkVotes = 0;
+    } else if (used + SPARE + 2 < hp.color_cap || hp.color_rounds > 4) {
+      if (++shrinkVotes >= 3) {
+        hp.color_cap = std::min(MAX_COLORS, std::max(1u, used + SPARE));
+        hp.color_rounds = std::max(4u, hp.color_rounds - 4);
+        shrinkVotes = 0;
+      }
+    } else {
+      shrinkVotes = 0;
+    }
   }
 
   Params params() const {
@@ -262,7 +306,7 @@ struct avbd_solver {
     if (coopBlocks) {
       void *args[] = {&d, &barrier};
       hipError_t e;
-      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(256), args, 0, stream); });
+      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(coopThreads), args, 0, stream); });
       if (e == hipSuccess) return;
       (void)hipGetLastError();
       coopBlocks = 0;  // not co-resident here: per-colour launches from now on
@@ -285,6 +329,16 @@ struct avbd_solver {
     solve();
     parity = 1 - parity;
     manifoldsValid = true;
+    if ((hp.flags & AVBD_FLAG_ADAPT_COLORS) && ++stepsSinceAdapt >= 10) {
+      stepsSinceAdapt = 0;
+      uint32_t c[C_WORDS];
+      if (hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, stream) == hipSuccess && hipStreamSynchronize(stream) == hipSuccess) {
+        avbd_counters k{};
+        k.colors = c[C_NUM_COLORS];
+        k.clashes = c[C_CLASHES];
+        adapt(k);
+      }
+    }
   }
 };
 
@@ -331,17 +385,17 @@ int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_co
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
     s->hp = *params;
     {
-      int dev = 0, coop = 0, cus = 0, perCU = 0;
+      int dev = 0, coop = 0, cus = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_solve, 256, 0);
-      const char *mode = getenv("AVBD_SOLVE");
-      int want = 1;
-      if (const char *b = getenv("AVBD_SOLVE_BLOCKS_PER_CU")) want = atoi(b);
-      if (want < 1) want = 1;
-      if (perCU > want) perCU = want;
-      if (coop && cus > 0 && perCU > 0 && !(mode && std::string(mode) == "colors")) s->coopBlocks = (uint32_t)(perCU * cus);
+      s->numCUs = (uint32_t)(cus > 0 ? cus : 0);
+      s->coopSupported = coop != 0 && cus > 0;
+      avbd_schedule sch{1u, 512u, 1u};
+      if (const char *t = getenv("AVBD_SOLVE_THREADS")) sch.threads = (uint32_t)atoi(t);
+      if (const char *b = getenv("AVBD_SOLVE_BLOCKS_PER_CU")) sch.blocks_per_cu = (uint32_t)atoi(b);
+      if (const char *mode = getenv("AVBD_SOLVE")) sch.cooperative = std::string(mode) == "colors" ? 0u : 1u;
+      if (s->applySchedule(sch) != AVBD_OK) s->applySchedule(avbd_schedule{0u, 512u, 1u});
     }
     s->n = body_count;
     s->nj = joint_count;
@@ -516,6 +570,7 @@ int avbd_read_counters(avbd_solver *s, avbd_counters *out) {
   out->clashes = c[C_CLASHES];
   out->colors = c[C_NUM_COLORS];
   out->color_cap = s->fixedColors ? s->fixedCap : s->hp.color_cap;
+  out->step = s->stepCount;
   return AVBD_OK;
 }
 int avbd_read_contacts(avbd_solver *s, uint32_t *man, uint32_t mcap, uint32_t *con, uint32_t ccap) {
@@ -639,6 +694,44 @@ int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t count) {
   if (count > s->hullCap) return fail(s, AVBD_ERR_CAPACITY, "hull storage: " + std::to_string(count) + " words > capacity " + std::to_string(s->hullCap));
   if (count) hipMemcpyAsync(s->hulls, words, count * 4ull, hipMemcpyHostToDevice, s->stream);
   return avbd_sync(s);
+}
+
+int avbd_adapt(avbd_solver *s, const avbd_counters *c) {
+  if (!s || !c) return AVBD_ERR_INVALID;
+  s->adapt(*c);
+  return AVBD_OK;
+}
+int avbd_set_schedule(avbd_solver *s, const avbd_schedule *sch) {
+  if (!s || !sch) return AVBD_ERR_INVALID;
+  if (s->applySchedule(*sch) != AVBD_OK) return fail(s, AVBD_ERR_INVALID, "schedule: not a co-resident cooperative grid on this device");
+  return AVBD_OK;
+}
+int avbd_get_schedule(const avbd_solver *s, avbd_schedule *sch) {
+  if (!s || !sch) return AVBD_ERR_INVALID;
+  *sch = s->schedule();
+  return AVBD_OK;
+}
+int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms) {
+  if (!s || substeps == 0 || steps == 0) return AVBD_ERR_INVALID;
+  const avbd_schedule cands[] = {{0, 512, 1}, {1, 256, 1}, {1, 512, 1}, {1, 1024, 1}, {1, 256, 2}, {1, 512, 2}};
+  const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
+  float bestT[8];
+  for (int k = 0; k < nc; k++) bestT[k] = INFINITY;
+  avbd_schedule keep = s->schedule();
+  for (int pass = 0; pass < 2; pass++)
+    for (int k = 0; k < nc; k++) {
+      if (s->applySchedule(cands[k]) != AVBD_OK) continue;
+      float ms = 0;
+      if (int rc = avbd_time_steps(s, steps, substeps, &ms)) return rc;
+      bestT[k] = std::min(bestT[k], ms / steps);
+    }
+  int pick = -1;
+  for (int k = 0; k < nc; k++) if (bestT[k] < INFINITY && (pick < 0 || bestT[k] < bestT[pick])) pick = k;
+  if (pick < 0) { s->applySchedule(keep); return fail(s, AVBD_ERR_DEVICE, "autotune: no schedule ran"); }
+  s->applySchedule(cands[pick]);
+  if (best) *best = cands[pick];
+  if (best_ms) *best_ms = bestT[pick];
+  return AVBD_OK;
 }
 
 int avbd_profile_step(avbd_solver *s, const avbd_step_opts *opts, avbd_profile *out) {

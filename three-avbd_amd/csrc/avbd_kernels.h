@@ -1000,7 +1000,7 @@ __device__ __forceinline__ void gridBarrier(uint32_t *bar, uint32_t *counters, u
  *  solver.ts:2226-2261): warm starts, iterations x (primal of each colour in use, dual), BDF1
  *  velocities, with a grid-wide barrier wherever the reference ends a dispatch. Sweeps only the
  *  colours the colouring used (the device's colour count), not the whole cap. */
-__global__ void __launch_bounds__(256) k_solve(Dev d, uint32_t *bar) {
+__global__ void __launch_bounds__(1024) k_solve(Dev d, uint32_t *bar) {
   __shared__ uint32_t genShared;
   if (threadIdx.x == 0) genShared = __hip_atomic_load(&bar[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
