@@ -630,7 +630,7 @@ __device__ __forceinline__ V jointAngRows(const Dev &d, const float4 *k, int a, 
 }
 /** addJoint (wgsl-solve.ts:115-165): spring row, ball-socket rows with lumped geometric
  *  stiffness, angle-lock rows. */
-__device__ __noinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, float alpha, uint32_t i) {
+__device__ __forceinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, float alpha, uint32_t i) {
   int4 in = d.jinfo[j];
   int a = in.y, b = in.z;
   const float4 *k = d.joints + 10 * j;
@@ -836,7 +836,7 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
 }
 __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
 /** dualJoint (wgsl-solve.ts:562-635) without the plastic (yield) extension. */
-__device__ __noinline__ void dualJoint(const Dev &d, uint32_t j, float alpha) {
+__device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha) {
   int4 in = d.jinfo[j];
   if (in.x != (int)T_JOINT) return;
   int a = in.y, b = in.z;
