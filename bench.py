@@ -91,12 +91,72 @@ def cpu_baseline(solver, hulls, params, substeps, budget_s=15.0, max_substeps=8)
     return done, dt
 
 
+def bench_batched(args, worlds=4096, loop_sample=256):
+    """worlds x steps per second of 4096 independent worlds stepped by one batched solver (every
+    launch covers all worlds), against looping the plain step (one solver per world) over a
+    bounded sample of the same worlds, same parameters and iterations."""
+    import numpy as np
+    import torch
+    import avbd_amd
+    from avbd_amd import scenes
+    t = time.perf_counter()
+    bodies, starts = scenes.batched_worlds(worlds)
+    print(f"# {worlds} worlds, {len(bodies)} bodies, built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    params = avbd_amd.default_params(color_cap=24, color_rounds=16, flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS)
+    n = len(bodies)
+    batch = avbd_amd.GpuSolver3D(bodies, params=params, world_start=starts,
+                                 capacity={"pairs": 8 * n, "manifolds": 8 * n, "contacts": 24 * n})
+    lead = max(0, args.warmup - 24)
+    batch.steps(lead)
+    schedule, _ = batch.autotune(2)
+    batch.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.steps(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c = batch.read_counters()
+    batched_rate = worlds * args.steps / wall
+    batch.destroy()
+    # the plain step looped over a sample of the same worlds (one solver each), same lead-in
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(worlds, size=min(loop_sample, worlds), replace=False))
+    singles = []
+    for w in sample:
+        s = avbd_amd.GpuSolver3D(bodies[starts[w]:starts[w + 1]], params=avbd_amd.default_params(
+            color_cap=24, color_rounds=16, flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS))
+        s.steps(args.warmup)
+        singles.append(s)
+    loop_steps = max(2, args.steps // 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for s in singles:
+            s.step()
+    torch.cuda.synchronize()
+    loop_wall = time.perf_counter() - t0
+    loop_rate = len(singles) * loop_steps / loop_wall
+    for s in singles:
+        s.destroy()
+    out = {
+        "metric": "batched worlds*steps/s (4096 worlds)", "value": batched_rate, "unit": "worlds*steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched-4096", "worlds": worlds, "bodies": n, "substeps": 1, "iterations": 10,
+                   "contacts": c["contacts"], "colors": c["colors"], "clashes": c["clashes"], "overflow": c["overflow"],
+                   "schedule": schedule, "color_cap": c["color_cap"]},
+        "looped_plain_step": {"value": loop_rate, "unit": "worlds*steps/s", "sample_worlds": len(singles),
+                              "steps": loop_steps, "speedup": batched_rate / loop_rate},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=240)
-    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55"])
+    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55", "batched-4096"])
     ap.add_argument("--workers", type=int, default=8, help="host processes building the scene's hulls")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--color-cap", type=int, default=32)
@@ -108,6 +168,8 @@ def main():
               "with no per-step exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batched-4096":
+        return bench_batched(args)
     t = time.perf_counter()
     bodies, joints, info, hulls, substeps, iters = build_workload(args.workload, args.workers)
     print(f"# scene {args.workload}: {len(bodies)} bodies, {len(joints)} joints, built in {time.perf_counter() - t:.1f} s",

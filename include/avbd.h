@@ -102,6 +102,14 @@ typedef struct avbd_step_opts {
 /* new GpuSolver3D(device, ref, options) (solver.ts:722): bodies and joints as records. */
 int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_count, const float *joints,
                 const int32_t *joint_info, uint32_t joint_count, const avbd_capacity *capacity, avbd_solver **out);
+/* Batched worlds (no counterpart in the reference, which has one world per solver): `worlds`
+ * independent worlds packed back to back, world w holding bodies [world_start[w],
+ * world_start[w + 1]). Bodies of different worlds never collide; colouring priorities use each
+ * body's index within its world, so every world steps exactly as it would in a solver of its
+ * own, while each kernel launch covers all worlds. Joints index bodies globally. */
+int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t body_count, const float *joints,
+                      const int32_t *joint_info, uint32_t joint_count, const uint32_t *world_start, uint32_t worlds,
+                      const avbd_capacity *capacity, avbd_solver **out);
 /* GpuSolver3D.destroy (solver.ts:2439). */
 void avbd_destroy(avbd_solver *s);
 /* The last error's message (the reference throws Error / StorageCapacityError). */

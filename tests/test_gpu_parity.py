@@ -264,3 +264,35 @@ def test_substeps_reuse_manifolds_match(gpu):
     assert np.array_equal(got, m.bodies())
     assert_same_contacts(s, m)
     s.destroy()
+
+
+def test_batched_worlds_step_like_their_own_solvers(gpu):
+    say("case: 24 batched worlds vs each world in its own solver, 15 steps")
+    bodies, starts = scenes.batched_worlds(24, lo=8, hi=160)
+    kw = dict(lanes=LANES1, color_cap=64)
+    batch = gpu_for(bodies, kw, world_start=starts)
+    singles = [gpu_for(bodies[starts[w]:starts[w + 1]], kw) for w in range(24)]
+    for _ in range(15):
+        batch.step()
+        for s in singles:
+            s.step()
+    got = batch.read_bodies()
+    assert batch.read_counters()["clashes"] == 0
+    gm, gc = batch.read_contacts_raw()
+    Gb = sorted_contacts(gm, gc)
+    per = {}
+    for mm, cc in Gb:
+        per.setdefault(int(np.searchsorted(starts, mm[0], side="right") - 1), []).append((mm, cc))
+    for w, s in enumerate(singles):
+        want = s.read_bodies()
+        assert np.array_equal(got[starts[w]:starts[w + 1]].view(np.uint32), want.view(np.uint32)), w
+        sm, sc = s.read_contacts_raw()
+        S = sorted_contacts(sm, sc)
+        B = per.get(w, [])
+        assert len(S) == len(B), w
+        for (ma, ca), (mb, cb) in zip(B, S):
+            assert ma[0] - starts[w] == mb[0] and ma[1] - starts[w] == mb[1]
+            assert (ma[3] & 15) == (mb[3] & 15) and (ma[4:8] == mb[4:8]).all()
+            assert (ca == cb).all()
+        s.destroy()
+    batch.destroy()

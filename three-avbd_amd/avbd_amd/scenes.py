@@ -207,3 +207,41 @@ def large_scene(n=100000, seed=2026, chains=200, links=20, pitch=2.5, friction=0
             prev = cur
     return (np.array(recs, np.float32), np.array(joints, np.float32).reshape(-1, 40), np.array(info, np.int32).reshape(-1, 4),
             hulls.words())
+
+
+def batched_worlds(worlds=4096, lo=8, hi=512, seed_xor=0x9E3779B9):
+    """Batched: `worlds` independent worlds, world w seeded with w ^ 0x9e3779b9. Each holds a
+    static ground and N_w = round(exp(U[ln lo, ln hi])) bodies, half boxes (half-extents U[0.2, 0.5]
+    m) and half spheres (radius U[0.2, 0.5] m), densities log-uniform in [1, 100] (mass ratios up
+    to 100:1), friction U[0.2, 1.0] for the whole world, arranged as a pyramid, a random pile or
+    stacks (chosen uniformly). Returns (bodies, world_start) with world w's bodies at
+    [world_start[w], world_start[w + 1]), its ground first."""
+    recs = []
+    starts = [0]
+    for w in range(worlds):
+        rng = Lcg((w ^ seed_xor) & 0xFFFFFFFF)
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        mu = rng.uniform(0.2, 1.0)
+        kind = min(2, int(rng() * 3))
+        recs.append(ground(-0.5, (60.0, 60.0, 1.0), mu))
+        if kind == 0:  # pyramid: rows of unit-pitch slots, bottom row wide enough for n
+            rows = 1
+            while rows * (rows + 1) // 2 < n:
+                rows += 1
+            slots = [(x * 1.05 + y * 0.525 - rows / 2.0, 0.0, y * 1.0 + 0.5) for y in range(rows) for x in range(rows - y)][:n]
+        elif kind == 1:  # pile: jittered grid, 1.3 m pitch
+            side = max(1, int(math.ceil(n ** (1.0 / 3.0))))
+            slots = [((k % side) * 1.3 - side * 0.65 + rng.uniform(-0.1, 0.1), ((k // side) % side) * 1.3 - side * 0.65 + rng.uniform(-0.1, 0.1),
+                      0.65 + (k // (side * side)) * 1.3 + rng.uniform(-0.1, 0.1)) for k in range(n)]
+        else:  # stacks: columns of at most 16
+            cols = max(1, int(math.ceil(n / 16.0)))
+            slots = [((k % cols) * 1.2 - cols * 0.6, 0.0, 0.5 + (k // cols) * 1.02) for k in range(n)]
+        for k, pos in enumerate(slots):
+            density = math.exp(rng.uniform(0.0, math.log(100.0)))
+            if (k + w) % 2 == 0:
+                he = [rng.uniform(0.2, 0.5) for _ in range(3)] if kind != 0 else [0.5, 0.5, 0.5]
+                recs.append(box_record(pos, [2 * h for h in he], density, mu))
+            else:
+                recs.append(sphere_record(pos, rng.uniform(0.2, 0.5) if kind != 0 else 0.5, density, mu))
+        starts.append(len(recs))
+    return np.array(recs, np.float32), np.array(starts, np.uint32)

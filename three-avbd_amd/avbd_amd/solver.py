@@ -89,7 +89,7 @@ EXPORTS = [
     "avbd_read_contacts", "avbd_read_pairs", "avbd_read_colors", "avbd_seed", "avbd_set_fixed_colors",
     "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
-    "avbd_autotune",
+    "avbd_autotune", "avbd_create_batch",
 ]
 
 
@@ -104,6 +104,7 @@ def lib():
         L.avbd_create_error.restype = ctypes.c_char_p
         vp = ctypes.c_void_p
         L.avbd_create.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp, ctypes.POINTER(vp)]
+        L.avbd_create_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.POINTER(vp)]
         for name in EXPORTS[1:]:
             fn = getattr(L, name)
             if name not in ("avbd_last_error", "avbd_create_error"):
@@ -132,7 +133,8 @@ def default_params(dt=1 / 60, gravity=-9.81, iterations=10, alpha=0.99, beta_lin
 class GpuSolver3D:
     """The MI355X AVBD step behind the C ABI, with GpuSolver3D's method names."""
 
-    def __init__(self, bodies, joints=None, joint_info=None, params=None, capacity=None, hulls=None, ignore_pairs=None):
+    def __init__(self, bodies, joints=None, joint_info=None, params=None, capacity=None, hulls=None, ignore_pairs=None,
+                 world_start=None):
         self.L = lib()
         bodies = np.ascontiguousarray(bodies, dtype=np.float32).reshape(-1, BODY_FLOATS)
         if joints is None or len(joints) == 0:
@@ -147,8 +149,12 @@ class GpuSolver3D:
         if hulls is not None and cap.hull_words < len(hulls):
             cap.hull_words = len(hulls)
         h = ctypes.c_void_p()
-        rc = self.L.avbd_create(ctypes.byref(self._params), _p(bodies), len(bodies), _p(joints), _p(joint_info), len(joints),
-                                ctypes.byref(cap), ctypes.byref(h))
+        if world_start is None:
+            world_start = np.array([0, len(bodies)], np.uint32)
+        self.world_start = np.ascontiguousarray(world_start, np.uint32)
+        rc = self.L.avbd_create_batch(ctypes.byref(self._params), _p(bodies), len(bodies), _p(joints), _p(joint_info),
+                                      len(joints), _p(self.world_start), len(self.world_start) - 1, ctypes.byref(cap),
+                                      ctypes.byref(h))
         if rc != 0:
             msg = self.L.avbd_create_error().decode()
             raise (ValueError if rc == -1 else RuntimeError)(msg)

@@ -77,8 +77,11 @@ struct avbd_solver {
   avbd_params hp{};
   uint32_t n = 0, nj = 0, bodyCap = 0, jointCap = 0, pairCap = 0, manCap = 0, conCap = 0, hullCap = 0;
   uint32_t tableSize = 0, hashSize = 0, colorGroups = 0, adjCap = 0;
-  float cellSize = 1, largeRadius = 1e30f;
-  std::vector<uint32_t> large;
+  float cellSize = 1;
+  std::vector<uint32_t> large;          // large bodies, grouped by world
+  std::vector<uint32_t> worldStartHost;  // W + 1 body offsets
+  uint32_t *worldOf = nullptr, *worldStart = nullptr, *largeStart = nullptr;
+  float *largeRadiusW = nullptr;
   struct NC { uint32_t hi, lo, slot; };
   std::vector<NC> ignorePairs;  // IgnoreCollision entries
   std::vector<int32_t> jinfoHost;
@@ -192,7 +195,9 @@ struct avbd_solver {
     Dev d;
     d.n = n; d.nj = nj; d.b = b; d.refPos = refPos; d.refRot = refRot; d.filters = filters; d.joints = joints; d.jinfo = jinfo;
     d.statics = statics; d.largeCount = (uint32_t)large.size(); d.noCollideCount = noCollideCount; d.hulls = hulls;
-    d.cellSize = cellSize; d.largeRadius = largeRadius; d.tableMask = tableSize - 1;
+    d.worldOf = worldOf; d.worldStart = worldStart; d.largeStart = largeStart; d.largeRadius = largeRadiusW;
+    d.worlds = (uint32_t)worldStartHost.size() - 1;
+    d.cellSize = cellSize; d.tableMask = tableSize - 1;
     d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos;
     d.pairs = pairs; d.pairCap = pairCap;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
@@ -227,18 +232,33 @@ struct avbd_solver {
     k_scan_add<<<nb, 256, 0, stream>>>(out, scanSums, count);
   }
 
-  /** uploadStatics (solver.ts:1263-1305): cell size, large list and sorted no-collide list. */
+  /** uploadStatics (solver.ts:1263-1305): cell size, large list and sorted no-collide list. Each
+   *  world classifies its own bodies (largestSmall, radii.ts:47) and lists its large ones. */
   void uploadStatics(const std::vector<float> &radius, const std::vector<float> &mass) {
-    std::vector<double> r(radius.begin(), radius.end());
-    double maxSmall = n ? largestSmall(r) : 1.0;
-    double minLarge = INFINITY;
-    for (double x : r) if (x > maxSmall) minLarge = std::min(minLarge, x);
-    double threshold = std::isinf(minLarge) ? maxSmall * 1.5 : (maxSmall + minLarge) / 2;
-    largeRadius = (float)threshold;
-    cellSize = (float)std::max(2 * maxSmall * (1 + 1e-4), 1e-3);
+    uint32_t W = (uint32_t)worldStartHost.size() - 1;
+    std::vector<float> thr(W, 1e30f);
+    std::vector<uint32_t> lstart(W + 1, 0);
+    double maxCell = 0;
     large.clear();
     anyDynamicLarge = false;
-    for (uint32_t i = 0; i < n; i++) if (radius[i] > largeRadius) { large.push_back(i); if (mass[i] > 0) anyDynamicLarge = true; }
+    for (uint32_t w = 0; w < W; w++) {
+      uint32_t b0 = worldStartHost[w], b1 = worldStartHost[w + 1];
+      lstart[w] = (uint32_t)large.size();
+      if (b1 <= b0) continue;
+      std::vector<double> r(radius.begin() + b0, radius.begin() + b1);
+      double maxSmall = largestSmall(r);
+      double minLarge = INFINITY;
+      for (double x : r) if (x > maxSmall) minLarge = std::min(minLarge, x);
+      double threshold = std::isinf(minLarge) ? maxSmall * 1.5 : (maxSmall + minLarge) / 2;
+      thr[w] = (float)threshold;
+      maxCell = std::max(maxCell, 2 * maxSmall * (1 + 1e-4));
+      for (uint32_t i = b0; i < b1; i++)
+        if (radius[i] > thr[w]) { large.push_back(i); if (mass[i] > 0) anyDynamicLarge = true; }
+    }
+    lstart[W] = (uint32_t)large.size();
+    cellSize = (float)std::max(maxCell, 1e-3);
+    hipMemcpyAsync(largeRadiusW, thr.data(), W * 4ull, hipMemcpyHostToDevice, stream);
+    hipMemcpyAsync(largeStart, lstart.data(), (W + 1) * 4ull, hipMemcpyHostToDevice, stream);
     std::vector<NC> nc = ignorePairs;
     for (uint32_t j = 0; j < nj; j++) {
       int a = jinfoHost[4 * j + 1], bb = jinfoHost[4 * j + 2];
@@ -365,6 +385,13 @@ const char *avbd_last_error(const avbd_solver *s) { return s ? s->err.c_str() : 
 
 int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_count, const float *jrec, const int32_t *jinfo,
                 uint32_t joint_count, const avbd_capacity *capacity, avbd_solver **out) {
+  uint32_t ws[2] = {0, body_count};
+  return avbd_create_batch(params, bodies, body_count, jrec, jinfo, joint_count, ws, 1, capacity, out);
+}
+
+int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t body_count, const float *jrec, const int32_t *jinfo,
+                      uint32_t joint_count, const uint32_t *world_start, uint32_t worlds, const avbd_capacity *capacity,
+                      avbd_solver **out) {
   g_create_error.clear();
   if (!out) { g_create_error = "out is null"; return AVBD_ERR_INVALID; }
   *out = nullptr;
@@ -372,6 +399,12 @@ int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_co
   if (!checkParams(params, why)) { g_create_error = why; return AVBD_ERR_INVALID; }
   if (body_count > 0 && !bodies) { g_create_error = "bodies is null"; return AVBD_ERR_INVALID; }
   if (joint_count > 0 && (!jrec || !jinfo)) { g_create_error = "joints or joint_info is null"; return AVBD_ERR_INVALID; }
+  if (worlds < 1 || !world_start || world_start[0] != 0 || world_start[worlds] != body_count) {
+    g_create_error = "world_start must run from 0 to body_count over worlds + 1 entries";
+    return AVBD_ERR_INVALID;
+  }
+  for (uint32_t w = 0; w < worlds; w++)
+    if (world_start[w + 1] < world_start[w]) { g_create_error = "world_start must not decrease"; return AVBD_ERR_INVALID; }
   for (uint32_t j = 0; j < joint_count; j++) {
     int t = jinfo[4 * j], a = jinfo[4 * j + 1], b = jinfo[4 * j + 2];
     if (t < 0 || t > 2 || b < 0 || (uint32_t)b >= body_count || a < -1 || a >= (int)body_count || (t == 2 && a < 0)) {
@@ -444,6 +477,18 @@ int avbd_create(const avbd_params *params, const float *bodies, uint32_t body_co
     s->hist = s->alloc<uint32_t>(64ull * s->colorGroups + 1);
     s->fixedDev = s->alloc<uint32_t>(N);
     s->counters = s->alloc<uint32_t>(C_WORDS);
+    s->worldStartHost.assign(world_start, world_start + worlds + 1);
+    s->worldOf = s->alloc<uint32_t>(N);
+    s->worldStart = s->alloc<uint32_t>(worlds + 1);
+    s->largeStart = s->alloc<uint32_t>(worlds + 1);
+    s->largeRadiusW = s->alloc<float>(worlds);
+    {
+      std::vector<uint32_t> wo(N, 0);
+      for (uint32_t w = 0; w < worlds; w++) for (uint32_t i = world_start[w]; i < world_start[w + 1]; i++) wo[i] = w;
+      hipMemcpyAsync(s->worldOf, wo.data(), N * 4ull, hipMemcpyHostToDevice, s->stream);
+      hipMemcpyAsync(s->worldStart, world_start, (worlds + 1) * 4ull, hipMemcpyHostToDevice, s->stream);
+      if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error during create");
+    }
     s->scanSums = s->alloc<uint32_t>(4096);
     s->barrier = s->alloc<uint32_t>(64);
     s->fill(s->colA, N, NO_COLOR);

@@ -19,9 +19,14 @@ struct Dev {
   int4 *jinfo;     // type, a (-1 world), b, 0
   uint32_t *statics;  // large bodies [largeCount] | no-collide (hi, lo, slot) triples sorted by (hi, lo)
   uint32_t largeCount, noCollideCount;
+  // worlds: bodies [worldStart[w], worldStart[w+1]) never touch another world's; large bodies are
+  // listed by world (statics[largeStart[w] .. largeStart[w+1]))
+  uint32_t *worldOf, *worldStart, *largeStart;
+  float *largeRadius;  // per world: bodies with a larger bounding radius are tested against all of theirs
+  uint32_t worlds;
   const uint4 *hulls;
   // uniform grid
-  float cellSize, largeRadius;
+  float cellSize;
   uint32_t tableMask;
   uint32_t *cellCount, *cellStart, *cellCursor, *cellSorted;
   int4 *cellOf;
@@ -121,17 +126,18 @@ __device__ __forceinline__ int hashFind(const Dev &d, uint32_t a, uint32_t b) {
 // Broadphase: uniform grid over bounding spheres (cells 2x the largest small radius), a list of
 // large bodies tested against every body, and the reference's exact pair predicate
 // (wgsl-collision.ts:150-234). The predicate decides; the grid only has to be conservative.
-__device__ __forceinline__ bool isLarge(const Dev &d, uint32_t i) { return d.b.moment[i].w > d.largeRadius; }
+__device__ __forceinline__ bool isLarge(const Dev &d, uint32_t i) { return d.b.moment[i].w > d.largeRadius[d.worldOf[i]]; }
 __device__ __forceinline__ int4 cellAt(const Dev &d, V pos) {
   return make_int4((int)floorf(pos.x / d.cellSize), (int)floorf(pos.y / d.cellSize), (int)floorf(pos.z / d.cellSize), 0);
 }
 __device__ __forceinline__ uint32_t cellHash(const Dev &d, int4 c) {
-  return (((uint32_t)c.x * 73856093u) ^ ((uint32_t)c.y * 19349663u) ^ ((uint32_t)c.z * 83492791u)) & d.tableMask;
+  return (((uint32_t)c.x * 73856093u) ^ ((uint32_t)c.y * 19349663u) ^ ((uint32_t)c.z * 83492791u) ^ ((uint32_t)c.w * 2654435769u)) & d.tableMask;
 }
 __global__ void k_grid_count(Dev d) {
   GRID_STRIDE(i, d.n) {
     if (isLarge(d, i)) continue;
     int4 c = cellAt(d, xyz(d.b.pos[i]));
+    c.w = (int)d.worldOf[i];
     d.cellOf[i] = c;
     atomicAdd(&d.cellCount[cellHash(d, c)], 1u);
   }
@@ -234,12 +240,13 @@ __global__ void k_find_pairs(Dev d) {
     if (d.b.size[i].w <= 0.0f) continue;
     bool small = !isLarge(d, i);
     Probe P = probeOf(d, i);
+    uint32_t w = d.worldOf[i];
     if (small) {
       int4 c = d.cellOf[i];
       for (int oz = -1; oz <= 1; oz++)
         for (int oy = -1; oy <= 1; oy++)
           for (int ox = -1; ox <= 1; ox++) {
-            int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, 0);
+            int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
             uint32_t h = cellHash(d, cell);
             uint32_t end = d.cellStart[h + 1];
             for (uint32_t k = d.cellStart[h]; k < end; k++) {
@@ -247,12 +254,12 @@ __global__ void k_find_pairs(Dev d) {
               if (j == i) continue;
               if (j > i && d.b.size[j].w > 0.0f) continue;
               int4 cj = d.cellOf[j];
-              if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z) continue;
+              if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z || cj.w != cell.w) continue;
               testPair(d, P, j, d.sortedPos[k]);
             }
           }
     }
-    for (uint32_t k = 0; k < d.largeCount; k++) {
+    for (uint32_t k = d.largeStart[w]; k < d.largeStart[w + 1]; k++) {
       uint32_t l = d.statics[k];
       if (l == i || (!small && l < i && d.b.size[l].w > 0.0f)) continue;
       float4 pl = d.b.pos[l];
@@ -266,7 +273,8 @@ __global__ void k_find_fixed_large(Dev d) {
     if (d.b.size[i].w > 0.0f || isLarge(d, i)) continue;
     bool loaded = false;
     Probe P;
-    for (uint32_t k = 0; k < d.largeCount; k++) {
+    uint32_t w = d.worldOf[i];
+    for (uint32_t k = d.largeStart[w]; k < d.largeStart[w + 1]; k++) {
       uint32_t l = d.statics[k];
       if (d.b.size[l].w <= 0.0f) continue;
       if (!loaded) { P = probeOf(d, i); loaded = true; }
@@ -483,8 +491,13 @@ __global__ void k_sort_adj(Dev d) {
 
 // ---------------------------------------------------------------------------------------------
 // Colouring: incremental Jones-Plassmann with hashed priorities (wgsl-topology.ts:155-262).
-__device__ __forceinline__ uint32_t priority(uint32_t i) { return hash32(i + 0x9e3779b9u); }
-__device__ __forceinline__ bool beats(uint32_t i, uint32_t j) { uint32_t pi = priority(i), pj = priority(j); return pi > pj || (pi == pj && i > j); }
+/** lowbias32 priority of a body's index within its world (its global index in a single world). */
+__device__ __forceinline__ uint32_t priority(const Dev &d, uint32_t i) { return hash32(i - d.worldStart[d.worldOf[i]] + 0x9e3779b9u); }
+/** i beats j (always the same world: ties by index there, which the global index orders alike). */
+__device__ __forceinline__ bool beats(const Dev &d, uint32_t i, uint32_t j) {
+  uint32_t pi = priority(d, i), pj = priority(d, j);
+  return pi > pj || (pi == pj && i > j);
+}
 __device__ __forceinline__ int neighbour(const Dev &d, uint32_t b, uint32_t e) {
   int2 ends = endpoints(d, d.adjList[e]);
   int other = ends.x == (int)b ? ends.y : ends.x;
@@ -511,7 +524,7 @@ __global__ void k_color_compact(Dev d) {
     uint32_t col = d.colA[b] & 0xffu;
     if (col != NO_COLOR) {
       bool isMax = true;
-      for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && isMax; e++) { int j = neighbour(d, b, e); if (j >= 0 && !beats(b, j)) isMax = false; }
+      for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && isMax; e++) { int j = neighbour(d, b, e); if (j >= 0 && !beats(d, b, j)) isMax = false; }
       if (isMax) { uint32_t f = smallestFree(d, b, d.colA, false); col = col < f ? col : f; }
     }
     d.colB[b] = col;
@@ -524,7 +537,7 @@ __global__ void k_color_mark(Dev d) {
     bool pending = col == NO_COLOR;
     for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && !pending; e++) {
       int j = neighbour(d, b, e);
-      if (j >= 0 && (d.colB[j] & 0xffu) == col && beats(j, b)) pending = true;
+      if (j >= 0 && (d.colB[j] & 0xffu) == col && beats(d, j, b)) pending = true;
     }
     d.colA[b] = col | (pending ? PENDING : 0u);
   }
@@ -536,7 +549,7 @@ __global__ void k_color_round(Dev d, const uint32_t *src, uint32_t *dst) {
     bool isMax = true;
     for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && isMax; e++) {
       int j = neighbour(d, b, e);
-      if (j >= 0 && (src[j] & PENDING) && !beats(b, j)) isMax = false;
+      if (j >= 0 && (src[j] & PENDING) && !beats(d, b, j)) isMax = false;
     }
     if (isMax) {
       uint32_t f = smallestFree(d, b, src, true);
