@@ -178,6 +178,21 @@ int avbd_time_steps(avbd_solver *s, uint32_t steps, uint32_t substeps, float *ms
 /* Waits for the solver's stream. */
 int avbd_sync(avbd_solver *s);
 
+/* Adjoint (no counterpart in the reference, which has no differentiation): the step with its
+ * active set frozen (contact set, anchors, normals, warm-start sources, colouring, per-iteration
+ * penalties, clamp/cone and warm-start branches, pivot floor), differentiated in f64 at the f32
+ * forward's own states. Contact-only scenes (joints and springs are refused).
+ * avbd_forward runs `steps` detecting steps (as avbd_step) keeping a checkpoint every
+ * `checkpoint_every` steps (0: record every step's tape during the rollout, no re-run later);
+ * a capacity overflow or a clashing colouring during it makes the next avbd_backward fail. */
+#define AVBD_ADJ_FLOATS 16 /* pos xyz, previous up-velocity, rot xyzw, vel xyz, 0, angvel xyz, 0 */
+int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every);
+/* Reverse sweep of the last avbd_forward (re-running each segment from its checkpoint): from the
+ * adjoint of the final state (AVBD_ADJ_FLOATS doubles per body) to the adjoint of the initial
+ * state (same layout) and of each body's friction (NULL to skip). The solver ends where the
+ * rollout ended. */
+int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, double *d_friction);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@
 //     by the known-answer checks listed in each test.
 // Build: make -C oracle (g++ -O2 -ffp-contract=off, no fast-math).
 #include <cstring>
+#include <string>
 #include "ref3d.hpp"
 #include "scenes.hpp"
 #include "gpu3d.hpp"
@@ -307,5 +308,92 @@ void omodel_get_contacts(void *h, double *man, double *con) { DISPATCH(h, getCon
 void omodel_get_colors(void *h, uint32_t *state) { DISPATCH(h, for (size_t i = 0; i < M->bodies.size(); i++) state[i] = M->colorA[i]); }
 void omodel_get_adjacency(void *h, uint32_t *start, uint32_t *list) {
   DISPATCH(h, { for (size_t i = 0; i < M->adjStart.size(); i++) start[i] = M->adjStart[i]; for (size_t i = 0; i < M->adjList.size(); i++) list[i] = M->adjList[i]; });
+}
+}  // extern "C"
+
+// ---- adjoint (adjoint3d.hpp) ----
+#include "adjoint3d.hpp"
+namespace {
+struct TapeOwner { gpuo::Tape tape; };
+std::string g_adjError;
+}
+extern "C" {
+/** Starts recording a tape on the model (replacing any earlier one); stop with on = 0. */
+void omodel_tape(void *h, int on) {
+  ModelHandle *m = (ModelHandle *)h;
+  static std::unordered_map<void *, TapeOwner *> owners;
+  TapeOwner *&o = owners[h];
+  if (on) { delete o; o = new TapeOwner(); }
+  gpuo::Tape *t = on ? &o->tape : nullptr;
+  if (m->prec == 32) m->f->tape = t; else m->d->tape = t;
+  if (!on) return;
+}
+/** Frozen replay over the model's recorded tape, from the body records the tape started at. */
+void *ofrozen_create(void *h, const double *records, int n) {
+  ModelHandle *m = (ModelHandle *)h;
+  const gpuo::Tape *t = m->prec == 32 ? m->f->tape : m->d->tape;
+  const gpuo::Params &p = m->prec == 32 ? m->f->p : m->d->p;
+  uint32_t joints = m->prec == 32 ? (uint32_t)m->f->joints.size() : (uint32_t)m->d->joints.size();
+  if (!t) { g_adjError = "no tape recorded"; return nullptr; }
+  try {
+    return new gpuo::Frozen(p, *t, records, (size_t)n, joints);
+  } catch (const std::exception &e) { g_adjError = e.what(); return nullptr; }
+}
+const char *ofrozen_error(void) { return g_adjError.c_str(); }
+void ofrozen_destroy(void *f) { delete (gpuo::Frozen *)f; }
+int ofrozen_steps(void *f) { return (int)((gpuo::Frozen *)f)->tape.steps.size(); }
+/** run: state (16 per body) and friction (per body, or NULL) -> final state (16 per body). */
+void ofrozen_run(void *f, const double *state, const double *mu, double *out) { ((gpuo::Frozen *)f)->run(state, mu, out); }
+/** Linearise at the recorded model's own states (friction per body, or NULL). */
+int ofrozen_load_recorded(void *f, const double *mu) {
+  try { ((gpuo::Frozen *)f)->loadRecorded(mu); return 0; } catch (const std::exception &e) { g_adjError = e.what(); return -1; }
+}
+/** Reverse over the last run: adjoint of the final state -> adjoints of the initial state and friction. */
+void ofrozen_backward(void *f, const double *dFinal, double *dInit, double *dMu) { ((gpuo::Frozen *)f)->backward(dFinal, dInit, dMu); }
+/** Tape statistics: steps, contacts in total, floored pivots, sliding / clamped evaluations. */
+void ofrozen_stats(void *f, double *out) {
+  const gpuo::Tape &t = ((gpuo::Frozen *)f)->tape;
+  double contacts = 0, floors = 0, slide = 0, clamp = 0;
+  for (const gpuo::TapeStep &s : t.steps) {
+    contacts += s.contacts.size();
+    for (const auto &d : s.delta) for (double x : d) floors += x != 0;
+    for (const auto &b : s.branch) for (uint8_t x : b) { slide += (x & 2) != 0; clamp += (x & 1) != 0; }
+  }
+  out[0] = t.steps.size(); out[1] = contacts; out[2] = floors; out[3] = slide; out[4] = clamp;
+}
+/** Central differences (h = 1e-6) of the quaternion helpers against their reverses at the given
+ *  point; returns the largest absolute difference. */
+double ofrozen_check_helpers(const double *q, const double *b, const double *w, const double *g) {
+  typedef gpuo::Frozen F; typedef gpuo::Ops<double> O;
+  gpuo::Q4<double> Q = O::q(q[0], q[1], q[2], q[3]), B = O::q(b[0], b[1], b[2], b[3]);
+  gpuo::V3<double> W = O::v(w[0], w[1], w[2]), G3 = O::v(g[0], g[1], g[2]);
+  gpuo::Q4<double> G4 = O::q(g[0], g[1], g[2], g[3]);
+  double h = 1e-6, worst = 0;
+  auto upd = [&](double a, double fd) { worst = std::max(worst, std::fabs(a - fd)); };
+  // qadd
+  { gpuo::Q4<double> qb = O::q(0, 0, 0, 0); gpuo::V3<double> wb = O::v(0, 0, 0); F::qaddRev(Q, W, G4, qb, wb);
+    for (int k = 0; k < 4; k++) {
+      gpuo::Q4<double> qp = Q, qm = Q; (&qp.x)[k] += h; (&qm.x)[k] -= h;
+      upd((&qb.x)[k], (O::dot4(O::qadd(qp, W), G4) - O::dot4(O::qadd(qm, W), G4)) / (2 * h));
+    }
+    for (int k = 0; k < 3; k++) {
+      gpuo::V3<double> wp = W, wm = W; wp[k] += h; wm[k] -= h;
+      upd(wb[k], (O::dot4(O::qadd(Q, wp), G4) - O::dot4(O::qadd(Q, wm), G4)) / (2 * h));
+    } }
+  // qsub
+  { gpuo::Q4<double> ab = O::q(0, 0, 0, 0), bb = ab; F::qsubRev(Q, B, G3, ab, bb);
+    for (int k = 0; k < 4; k++) {
+      gpuo::Q4<double> p1 = Q, m1 = Q; (&p1.x)[k] += h; (&m1.x)[k] -= h;
+      upd((&ab.x)[k], (O::dot(O::qsub(p1, B), G3) - O::dot(O::qsub(m1, B), G3)) / (2 * h));
+      gpuo::Q4<double> p2 = B, m2 = B; (&p2.x)[k] += h; (&m2.x)[k] -= h;
+      upd((&bb.x)[k], (O::dot(O::qsub(Q, p2), G3) - O::dot(O::qsub(Q, m2), G3)) / (2 * h));
+    } }
+  // qrotate
+  { gpuo::Q4<double> qb = O::q(0, 0, 0, 0); F::qrotateRev(Q, W, G3, qb);
+    for (int k = 0; k < 4; k++) {
+      gpuo::Q4<double> p1 = Q, m1 = Q; (&p1.x)[k] += h; (&m1.x)[k] -= h;
+      upd((&qb.x)[k], (O::dot(O::qrotate(p1, W), G3) - O::dot(O::qrotate(m1, W), G3)) / (2 * h));
+    } }
+  return worst;
 }
 }  // extern "C"

@@ -108,8 +108,30 @@ struct Params {  // the subset of Params (layout.ts:265-305) the semantics depen
   uint32_t lanesRule[3];  // colour sizes from which 1, 2, 4 lanes per body are used (else 8)
 };
 
+/** One step's frozen decisions, recorded in f64 whatever the model's float type (the adjoint's
+ *  tape): the contact set as collision produced it, the warm-start source of every contact, the
+ *  colouring and adjacency, and per iteration the penalties, the clamp/cone branches of every
+ *  contact evaluation and the pivot-floor shifts of every body update. */
+struct TapeContact { uint32_t a, b, key; double rA[3], rB[3], n[3], friction, pen0[3], lam0[3], c0[3]; int src; };
+struct TapeStep {
+  std::vector<TapeContact> contacts;
+  std::vector<uint32_t> manifoldFirst;              // contacts of manifold m: [first[m], first[m+1])
+  std::vector<uint32_t> adjStart, adjList, bucketStart, bucketBodies;
+  uint32_t colorCapUsed = 0, clashes = 0;
+  std::vector<std::vector<double>> pen;             // [iteration][contact*3 + row]
+  std::vector<std::vector<uint8_t>> branch;         // [iteration][contact*3 + role]: bit0 normal clamped, bit1 sliding
+  std::vector<std::vector<double>> delta;           // [iteration][body*6 + pivot]: pivot-floor shift
+  std::vector<uint8_t> atRest, wMode;               // per body: initial guess held; 0 w interior, 1 w = 0, 2 w = 1
+  // the forward's own states (xyz or xyzw per body / contact), for a reverse sweep at them
+  std::vector<double> p0, q0, v0, w0, u0, inpos, inrot;  // at the step's start (after collision)
+  std::vector<std::vector<double>> P, R, lam;            // P/R: [0] initial guess, [it+1] after sweep it; lam: [it], [iters] final
+};
+struct Tape { std::vector<TapeStep> steps; };
+
 template <class T> struct Model {
   typedef Ops<T> O; typedef V3<T> V; typedef Q4<T> Q;
+  Tape *tape = nullptr;  // when set, step() appends its frozen decisions
+  int curIt = -1;
   Params p;
   std::vector<Body<T>> bodies;
   std::vector<Joint<T>> joints;
@@ -121,6 +143,7 @@ template <class T> struct Model {
   std::vector<std::pair<uint32_t, uint32_t>> pairs;
   std::vector<ManifoldT<T>> manifolds, prevManifolds;
   std::vector<Contact<T>> contacts, prevContacts;
+  std::vector<int> contactSrc;  // per contact: the previous step's contact it was warm-started from, or -1
   std::vector<uint32_t> adjStart, adjList;
   std::vector<uint32_t> colorA, colorB;  // colour state ping-pong (wgsl-topology.ts)
   std::vector<uint32_t> bucketStart, bucketBodies;  // colourStart[65], bodies by colour
@@ -457,6 +480,7 @@ template <class T> struct Model {
       for (int r = 0; r < 3; r++) k.pen[r] = O::clampt(k.pen[r] * gamma, T(1.0), T(1e10));
       k.pen.x = O::mx(k.pen.x, pmin);
       contacts.push_back(k);
+      contactSrc.push_back((int)(prev.first + i));
     }
     manifolds.push_back(m);
   }
@@ -538,6 +562,7 @@ template <class T> struct Model {
       for (int r = 0; r < 3; r++) k.pen[r] = O::clampt(pen[r] * gamma, T(1.0), T(1e10));
       k.pen.x = O::mx(k.pen.x, penaltyMin);
       contacts.push_back(k);
+      contactSrc.push_back(j);
     }
     manifolds.push_back(m);
   }
@@ -545,7 +570,7 @@ template <class T> struct Model {
   /** beginFrame + updateRefs + (hashInsert, broadphase, narrowphase) or reuseManifolds. */
   void collision(bool reuseManifolds) {
     prevManifolds.swap(manifolds); prevContacts.swap(contacts);
-    manifolds.clear(); contacts.clear();
+    manifolds.clear(); contacts.clear(); contactSrc.clear();
     if (p.flags & FLAG_REUSE_CONTACTS) {  // updateRefs (wgsl-collision.ts:34-48)
       for (size_t i = 0; i < bodies.size(); i++) {
         Body<T> &b = bodies[i];
@@ -767,7 +792,7 @@ template <class T> struct Model {
   }
   struct PairBody { uint32_t index; Q rot; V dLin, dAng; };
   PairBody pairBody(uint32_t i) const { return PairBody{i, bodies[i].rot, O::sub(bodies[i].pos, bodies[i].ipos), O::qsub(bodies[i].rot, bodies[i].irot)}; }
-  struct Eval { V rAW, rBW, C, F; T frictionScale, bounds; };
+  struct Eval { V rAW, rBW, C, F; T frictionScale, bounds; uint8_t bits; };
   Eval evalContact(const Contact<T> &k, const V basis[3], T friction, const PairBody &A, const PairBody &B, T alpha) const {
     Q rotA = A.rot, rotB = B.rot;
     if ((k.key & ~STICK_BIT) == SPHERE_FEATURE) { rotA = bodies[A.index].irot; rotB = bodies[B.index].irot; }
@@ -781,12 +806,14 @@ template <class T> struct Model {
       e.C[r] = c0[r] * keep + O::dot(l, A.dLin) - O::dot(l, B.dLin) + O::dot(O::cross(e.rAW, l), A.dAng) + O::dot(O::cross(e.rBW, O::neg(l)), B.dAng);
     }
     V F = O::add(O::mulv(k.pen, e.C), k.lam);
+    e.bits = F.x < T(0.0) ? 0 : 1;
     F.x = O::mn(F.x, T(0.0));
     e.bounds = std::fabs(F.x) * friction;
     e.frictionScale = std::sqrt(F.y * F.y + F.z * F.z);
     if (e.frictionScale > e.bounds && e.frictionScale > T(0.0)) {
       T s = e.bounds / e.frictionScale;
       F.y = F.y * s; F.z = F.z * s;
+      e.bits |= 2;
     }
     e.F = F;
     return e;
@@ -825,6 +852,7 @@ template <class T> struct Model {
         const Contact<T> &k = contacts[c];
         O::orthonormal(mf.n, basis);
         Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
+        if (tape && curIt >= 0) tape->steps.back().branch[curIt][c * 3 + (isA ? 0 : 1)] = ev.bits;
         T sg = isA ? T(1) : T(-1);
         V r = isA ? ev.rAW : ev.rBW;
         V l0 = O::mul(basis[0], sg), l1 = O::mul(basis[1], sg), l2 = O::mul(basis[2], sg);
@@ -844,15 +872,20 @@ template <class T> struct Model {
     T A61 = acc.cr[6], A62 = acc.cr[7], A63 = acc.cr[8], A64 = acc.ang[3], A65 = acc.ang[4], A66 = acc.ang[5];
     T D1 = A11;
     T L21 = A21 / D1, L31 = A31 / D1, L41 = A41 / D1, L51 = A51 / D1, L61 = A61 / D1;
-    T D2 = O::mx(A22 - L21 * L21 * D1, A22 * PF);
+    T S2 = A22 - L21 * L21 * D1, D2 = O::mx(S2, A22 * PF);
     T L32 = (A32 - L21 * L31 * D1) / D2, L42 = (A42 - L21 * L41 * D1) / D2, L52 = (A52 - L21 * L51 * D1) / D2, L62 = (A62 - L21 * L61 * D1) / D2;
-    T D3 = O::mx(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33 * PF);
+    T S3 = A33 - (L31 * L31 * D1 + L32 * L32 * D2), D3 = O::mx(S3, A33 * PF);
     T L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3, L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3, L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
-    T D4 = O::mx(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44 * PF);
+    T S4 = A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), D4 = O::mx(S4, A44 * PF);
     T L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4, L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
-    T D5 = O::mx(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55 * PF);
+    T S5 = A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), D5 = O::mx(S5, A55 * PF);
     T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
-    T D6 = O::mx(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66 * PF);
+    T S6 = A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), D6 = O::mx(S6, A66 * PF);
+    if (tape && curIt >= 0) {
+      double *dl = &tape->steps.back().delta[curIt][(size_t)i * 6];
+      dl[1] = (double)D2 - (double)S2; dl[2] = (double)D3 - (double)S3; dl[3] = (double)D4 - (double)S4;
+      dl[4] = (double)D5 - (double)S5; dl[5] = (double)D6 - (double)S6;
+    }
     T y1 = acc.rLin[0], y2 = acc.rLin[1] - L21 * y1, y3 = acc.rLin[2] - L31 * y1 - L32 * y2;
     T y4 = acc.rAng[0] - L41 * y1 - L42 * y2 - L43 * y3, y5 = acc.rAng[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
     T y6 = acc.rAng[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
@@ -923,6 +956,7 @@ template <class T> struct Model {
     for (uint32_t c = mf.first; c < end; c++) {
       Contact<T> &k = contacts[c];
       Eval e = evalContact(k, basis, mf.friction, A, B, alpha);
+      if (tape && curIt >= 0) tape->steps.back().branch[curIt][c * 3 + 2] = e.bits;
       k.lam = e.F;
       V pen = k.pen;
       T BL = T(p.betaLin), PMAX = T(1e10);
@@ -960,12 +994,16 @@ template <class T> struct Model {
       b.inpos = inpos;
       b.inrot = O::qadd(b.rot, O::mul(b.angvel, dt));
       T w = T(0);
+      uint8_t wMode = 1;
       if (std::fabs(g) > T(0)) {
         T sg = g > T(0) ? T(1) : (g < T(0) ? T(-1) : T(0));
-        w = O::clampt((O::dot(b.vel, up) - b.prevUp) / dt * sg / std::fabs(g), T(0), T(1));
+        T raw = (O::dot(b.vel, up) - b.prevUp) / dt * sg / std::fabs(g);
+        w = O::clampt(raw, T(0), T(1));
+        wMode = raw > T(0) ? (raw < T(1) ? 0 : 2) : 1;
       }
       bool slow = O::length(b.vel) + O::length(b.angvel) * b.radius < T(0.1);
       bool atRest = (p.flags & FLAG_START_AT_REST) && slow;
+      if (tape) { tape->steps.back().atRest.push_back(atRest ? 1 : 0); tape->steps.back().wMode.push_back(wMode); }
       b.ipos = b.pos; b.lifecycle = 0; b.irot = b.rot;
       if (dyn && !atRest) {
         b.pos = O::add(O::add(b.pos, O::mul(b.vel, dt)), O::mul(up, g * (w * dt * dt)));
@@ -994,20 +1032,77 @@ template <class T> struct Model {
     T alpha = T(p.alpha);
     warmStartJoints();
     warmStartBodies();
+    TapeStep *ts = tape ? &tape->steps.back() : nullptr;
+    if (ts) {
+      for (const Body<T> &b : bodies) {
+        for (int k = 0; k < 3; k++) {
+          ts->p0.push_back((double)b.ipos[k]); ts->inpos.push_back((double)b.inpos[k]);
+          ts->v0.push_back((double)b.vel[k]); ts->w0.push_back((double)b.angvel[k]);
+        }
+        const T *q = &b.irot.x, *iq = &b.inrot.x;
+        for (int k = 0; k < 4; k++) { ts->q0.push_back((double)q[k]); ts->inrot.push_back((double)iq[k]); }
+        ts->u0.push_back((double)b.prevUp);
+      }
+      recordPoses(*ts);
+    }
     for (uint32_t it = 0; it < p.iterations; it++) {
+      curIt = (int)it;
+      if (ts) {
+        ts->lam.emplace_back();
+        for (const Contact<T> &k : contacts) for (int r = 0; r < 3; r++) ts->lam.back().push_back((double)k.lam[r]);
+        ts->pen.emplace_back(); ts->branch.emplace_back(contacts.size() * 3, 0); ts->delta.emplace_back(bodies.size() * 6, 0.0);
+        for (const Contact<T> &k : contacts) for (int r = 0; r < 3; r++) ts->pen.back().push_back((double)k.pen[r]);
+      }
       for (uint32_t c = 0; c < colorCapUsed; c++) primalColor(c, alpha);
+      if (ts) recordPoses(*ts);
       for (uint32_t j = 0; j < joints.size(); j++) dualJoint(j, alpha);
       for (uint32_t m = 0; m < manifolds.size(); m++) dualManifold(m, alpha);
     }
+    curIt = -1;
+    if (ts) {
+      ts->lam.emplace_back();
+      for (const Contact<T> &k : contacts) for (int r = 0; r < 3; r++) ts->lam.back().push_back((double)k.lam[r]);
+    }
     updateVelocities();
+  }
+  void recordPoses(TapeStep &ts) const {
+    ts.P.emplace_back(); ts.R.emplace_back();
+    for (const Body<T> &b : bodies) {
+      for (int k = 0; k < 3; k++) ts.P.back().push_back((double)b.pos[k]);
+      const T *q = &b.rot.x;
+      for (int k = 0; k < 4; k++) ts.R.back().push_back((double)q[k]);
+    }
+  }
+  void recordTopology(TapeStep &ts) const {
+    for (size_t m = 0; m < manifolds.size(); m++) {
+      const ManifoldT<T> &mf = manifolds[m];
+      ts.manifoldFirst.push_back((uint32_t)ts.contacts.size());
+      for (uint32_t c = mf.first; c < mf.first + (mf.countStep & 15u); c++) {
+        const Contact<T> &k = contacts[c];
+        TapeContact t;
+        t.a = mf.a; t.b = mf.b; t.key = k.key; t.src = contactSrc[c];
+        t.friction = (double)mf.friction;
+        for (int r = 0; r < 3; r++) {
+          t.rA[r] = (double)k.rA[r]; t.rB[r] = (double)k.rB[r]; t.n[r] = (double)mf.n[r];
+          t.pen0[r] = (double)k.pen[r]; t.lam0[r] = (double)k.lam[r];
+        }
+        t.c0[0] = (double)k.c0x; t.c0[1] = (double)k.c0y; t.c0[2] = (double)k.c0z;
+        ts.contacts.push_back(t);
+      }
+    }
+    ts.manifoldFirst.push_back((uint32_t)ts.contacts.size());
+    ts.adjStart = adjStart; ts.adjList = adjList; ts.bucketStart = bucketStart; ts.bucketBodies = bucketBodies;
+    ts.colorCapUsed = colorCapUsed; ts.clashes = clashes;
   }
   /** One step (GpuSolver3D.step). */
   void step(bool reuseManifolds = false) {
     stepCount++;
     p.step = stepCount;
+    if (tape) tape->steps.emplace_back();
     collision(reuseManifolds);
     adjacency();
     colour();
+    if (tape) recordTopology(tape->steps.back());
     solve();
   }
 };

@@ -37,8 +37,9 @@ def lib():
         vp = ctypes.c_void_p
         L.oref_create.restype = vp
         L.omodel_create.restype = vp
-        for name in dir(L):
-            pass
+        L.ofrozen_create.restype = vp
+        L.ofrozen_error.restype = ctypes.c_char_p
+        L.ofrozen_check_helpers.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -265,12 +266,88 @@ class Model:
         self.L.omodel_get_colors(self.h, _p(out))
         return out
 
+    def record_tape(self, on=True):
+        """Record the adjoint's tape (the frozen decisions of every following step)."""
+        self.L.omodel_tape(self.h, int(on))
+
     def adjacency(self):
         c = self.counts()
         start = np.zeros(self.n + 1, dtype=np.uint32)
         lst = np.zeros(max(c["adjacency"], 1), dtype=np.uint32)
         self.L.omodel_get_adjacency(self.h, _p(start), _p(lst))
         return start, lst[:c["adjacency"]]
+
+
+ADJ_FLOATS = 16  # pos xyz + previous up-velocity, rot xyzw, vel xyz + 0, angvel xyz + 0
+
+
+def state_from_records(rec):
+    """Adjoint state slots (16 per body) from 40-float body records."""
+    rec = np.asarray(rec, dtype=np.float64)
+    st = np.zeros((len(rec), ADJ_FLOATS))
+    st[:, 0:3] = rec[:, 0:3]
+    st[:, 3] = rec[:, 35]
+    st[:, 4:8] = rec[:, 4:8]
+    st[:, 8:11] = rec[:, 32:35]
+    st[:, 12:15] = rec[:, 36:39]
+    return st
+
+
+class Frozen:
+    """The step with its active set frozen (oracle/adjoint3d.hpp), replayed in f64 over the tape a
+    Model recorded, and its reverse sweep. ``records`` are the body records the tape started at."""
+
+    def __init__(self, model, records):
+        self.L = lib()
+        rec = np.ascontiguousarray(records, dtype=np.float64)
+        self.n = len(rec)
+        self.records = rec
+        h = self.L.ofrozen_create(model.h, _p(rec), self.n)
+        if not h:
+            raise RuntimeError(self.L.ofrozen_error().decode())
+        self.h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ofrozen_destroy(self.h)
+            self.h = None
+
+    @property
+    def steps(self):
+        return self.L.ofrozen_steps(self.h)
+
+    def run(self, state=None, friction=None):
+        """Final state (n, 16) from the initial state (n, 16) and per-body friction."""
+        st = np.ascontiguousarray(state_from_records(self.records) if state is None else state, dtype=np.float64)
+        out = np.zeros((self.n, ADJ_FLOATS))
+        mu = None if friction is None else np.ascontiguousarray(friction, dtype=np.float64)
+        self.L.ofrozen_run(self.h, _p(st), None if mu is None else _p(mu), _p(out))
+        return out
+
+    def load_recorded(self, friction=None):
+        """Linearise at the recording model's own states (its float type's trajectory)."""
+        mu = None if friction is None else np.ascontiguousarray(friction, dtype=np.float64)
+        if self.L.ofrozen_load_recorded(self.h, None if mu is None else _p(mu)) != 0:
+            raise RuntimeError(self.L.ofrozen_error().decode())
+
+    def backward(self, d_final):
+        """Adjoints of the initial state (n, 16) and friction (n,) for the last run()."""
+        d = np.ascontiguousarray(d_final, dtype=np.float64).reshape(self.n, ADJ_FLOATS)
+        d_init = np.zeros((self.n, ADJ_FLOATS))
+        d_mu = np.zeros(self.n)
+        self.L.ofrozen_backward(self.h, _p(d), _p(d_init), _p(d_mu))
+        return d_init, d_mu
+
+    def stats(self):
+        out = np.zeros(5)
+        self.L.ofrozen_stats(self.h, _p(out))
+        return dict(zip(("steps", "contacts", "floored_pivots", "sliding", "clamped"), out.tolist()))
+
+
+def check_quaternion_reverses(q, b, w, g):
+    """Largest |reverse - central difference| of the quaternion helpers' reverses at a point."""
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (q, b, w, g)]
+    return lib().ofrozen_check_helpers(*(_p(a) for a in arr))
 
 
 def sequential_colors(bodies):

@@ -245,3 +245,34 @@ def batched_worlds(worlds=4096, lo=8, hi=512, seed_xor=0x9E3779B9):
                 recs.append(sphere_record(pos, rng.uniform(0.2, 0.5) if kind != 0 else 0.5, density, mu))
         starts.append(len(recs))
     return np.array(recs, np.float32), np.array(starts, np.uint32)
+
+
+def thrown_at_stack(worlds=256, seed=5, stack=4, thrown=64, steps_hint=120):
+    """Boxes thrown at a box stack, one copy per world: a stack of stack^3 boxes (half-extent
+    0.5 m, density 1, 0.01 m gaps) on a static ground, and `thrown` boxes of the same size on a
+    lattice above it (pitch 1.6 m from z = 6 m) with linear velocity U[-5, 5]^3 m/s and angular
+    velocity U[-2, 2]^3 rad/s; each world's friction (all its bodies) is U[0.3, 0.8].
+    Returns (bodies, world_start, friction per world)."""
+    rng = Lcg(seed)
+    recs, starts, mus = [], [], []
+    side = int(round(thrown ** (1.0 / 3.0)))
+    for w in range(worlds):
+        starts.append(len(recs))
+        mu = rng.uniform(0.3, 0.8)
+        mus.append(mu)
+        recs.append(ground(-0.5, (40.0, 40.0, 1.0), mu))
+        off = (stack - 1) / 2.0
+        for k in range(stack):
+            for j in range(stack):
+                for i in range(stack):
+                    recs.append(box_record(((i - off) * 1.01, (j - off) * 1.01, 0.5 + k * 1.0), (1.0, 1.0, 1.0), 1.0, mu))
+        toff = (side - 1) / 2.0
+        for t in range(thrown):
+            i, j, k = t % side, (t // side) % side, t // (side * side)
+            v = (rng.uniform(-5, 5), rng.uniform(-5, 5), rng.uniform(-5, 5))
+            om = (rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(-2, 2))
+            recs.append(box_record(((i - toff) * 1.6, (j - toff) * 1.6, 6.0 + k * 1.6), (1.0, 1.0, 1.0), 1.0, mu,
+                                   rng.quat(), v, om))
+    starts.append(len(recs))
+    return np.array(recs, np.float32), np.array(starts, np.uint32), np.array(mus)
+

@@ -89,8 +89,9 @@ EXPORTS = [
     "avbd_read_contacts", "avbd_read_pairs", "avbd_read_colors", "avbd_seed", "avbd_set_fixed_colors",
     "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
-    "avbd_autotune", "avbd_create_batch",
+    "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward",
 ]
+ADJ_FLOATS = 16  # adjoint state slots per body: pos xyz, previous up-velocity, rot xyzw, vel xyz, 0, angvel xyz, 0
 
 
 def lib():
@@ -240,6 +241,25 @@ class GpuSolver3D:
         self._check(self.L.avbd_profile_kernels(self.h, steps, substeps, ctypes.byref(out)))
         self.step_count += steps * substeps
         return {k: getattr(out, k) for k, _ in KernelStats._fields_}
+
+    # --- adjoint ------------------------------------------------------------------------------
+    def forward(self, steps, checkpoint_every=10):
+        """Differentiable rollout of `steps` detecting steps, a checkpoint every `checkpoint_every`
+        (0: record every step's tape now, no re-run in backward)."""
+        self._live()
+        self._check(self.L.avbd_forward(self.h, steps, checkpoint_every))
+        self.step_count += steps
+
+    def backward(self, d_final):
+        """Reverse sweep of the last forward(): the adjoint of the final state ((n, 16) float64,
+        ADJ_FLOATS slots per body) to the adjoints of the initial state ((n, 16)) and of each
+        body's friction ((n,))."""
+        self._live()
+        d = np.ascontiguousarray(d_final, dtype=np.float64).reshape(self.body_count, ADJ_FLOATS)
+        d_init = np.zeros((self.body_count, ADJ_FLOATS))
+        d_mu = np.zeros(self.body_count)
+        self._check(self.L.avbd_backward(self.h, _p(d), _p(d_init), _p(d_mu)))
+        return d_init, d_mu
 
     def adapt(self, counters=None):
         """adaptColors (solver.ts:2379): the colour cap and rounds from a counters readback."""

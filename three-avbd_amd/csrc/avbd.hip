@@ -10,8 +10,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <stdexcept>
 #include "../../include/avbd.h"
-#include "avbd_kernels.h"
+#include "avbd_adjoint.h"
 
 using namespace avbd;
 
@@ -129,6 +130,31 @@ struct avbd_solver {
   float4 *staging = nullptr;
   size_t stagingBytes = 0;
   KernelTimer timer;
+  // ---- adjoint: checkpoints of the forward rollout, the recorded segment, f64 adjoints ----
+  struct Checkpoint {
+    float4 *streams = nullptr;  // 12 x bodyCap: the 10 body streams, refPos, refRot
+    uint32_t *colA = nullptr, *counters = nullptr;
+    uint4 *mIds = nullptr;
+    float4 *mGeo = nullptr;
+    Contact *con = nullptr;
+    uint32_t stepCount = 0, shrinkVotes = 0, stepsSinceAdapt = 0;
+    int parity = 0;
+    bool manifoldsValid = false;
+    avbd_params hp{};
+  };
+  std::vector<Checkpoint> ckpts;   // segment starts, then the rollout's end
+  std::vector<void *> adjBufs;     // every adjoint allocation (freed together)
+  int *src = nullptr;              // per contact warm-start source while recording
+  Tape tape{};
+  uint32_t tapeSlots = 0;
+  Tape *rec = nullptr;             // when set, steps record into slot recSlot
+  uint32_t recSlot = 0;
+  std::vector<uint32_t> slotCap;   // colour cap each recorded step swept
+  AdjBuf adj{};
+  uint32_t adjConCap = 0;
+  uint32_t fwdSteps = 0, fwdEvery = 0, fwdEndStep = 0, fwdMaxContacts = 0, fwdMaxManifolds = 0;
+  bool fwdValid = false;
+  std::string fwdWhy = "no forward rollout recorded";
 
   template <class T> T *alloc(size_t count) {
     DevBuf d;
@@ -139,6 +165,7 @@ struct avbd_solver {
     return (T *)d.p;
   }
   ~avbd_solver() {
+    freeAdjoint();
     for (auto &d : bufs) hipFree(d.p);
     if (staging) hipFree(staging);
     for (auto &m : timer.marks) { hipEventDestroy(m.second.first); hipEventDestroy(m.second.second); }
@@ -206,6 +233,7 @@ struct avbd_solver {
     d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
+    d.src = rec ? src : nullptr;
     return d;
   }
   hipEvent_t mark(int kind) {
@@ -320,10 +348,16 @@ struct avbd_solver {
     scan(hist, hist, 64 * colorGroups + 1);
     k_color_starts<<<1, 64, 0, stream>>>(d);
     k_color_scatter<<<groups, 256, 0, stream>>>(d);
+    if (rec) {
+      uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
+      k_tape_topology<<<blocksFor(std::max({conCap, manCap, adjCap, n + 1}), 256), 256, 0, stream>>>(d, *rec, recSlot);
+      k_tape_colors<<<blocksFor(n, 256), 256, 0, stream>>>(d, *rec, recSlot, cap);
+      slotCap[recSlot] = cap;
+    }
   }
   void solve() {
     Dev d = dev();
-    if (coopBlocks) {
+    if (coopBlocks && !rec) {
       void *args[] = {&d, &barrier};
       hipError_t e;
       timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(coopThreads), args, 0, stream); });
@@ -334,13 +368,128 @@ struct avbd_solver {
     if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
     k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
-    uint32_t pb = blocksFor(n, 64);
+    uint32_t pb = blocksFor(n, 64), snapBlocks = blocksFor(std::max(n, conCap), 256);
+    if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, *rec, recSlot, 0);
     for (uint32_t it = 0; it < hp.iterations; it++) {
       for (uint32_t c = 0; c < cap; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
       timed(K_DUAL, [&] { k_dual<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d); });
+      if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, *rec, recSlot, it + 1);
     }
     k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
   }
+  // ---- adjoint (avbd_forward / avbd_backward) ----
+  template <class T> T *adjAlloc(size_t count) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256)) != hipSuccess) throw std::runtime_error("hipMalloc failed (adjoint)");
+    adjBufs.push_back(p);
+    return (T *)p;
+  }
+  void freeAdjoint() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void *p : adjBufs) hipFree(p);
+    adjBufs.clear();
+    ckpts.clear();
+    src = nullptr; tape = Tape{}; tapeSlots = 0; adj = AdjBuf{}; adjConCap = 0; rec = nullptr;
+  }
+  float4 *const *streamPtrs(float4 *out[12]) {
+    float4 *list[12] = {b.pos, b.rot, b.ipos, b.irot, b.size, b.moment, b.inpos, b.inrot, b.vel, b.angvel, refPos, refRot};
+    for (int k = 0; k < 12; k++) out[k] = list[k];
+    return out;
+  }
+  void allocCheckpoint(Checkpoint &c) {
+    c.streams = adjAlloc<float4>(12ull * bodyCap);
+    c.colA = adjAlloc<uint32_t>(bodyCap);
+    c.counters = adjAlloc<uint32_t>(C_WORDS);
+    c.mIds = adjAlloc<uint4>(manCap);
+    c.mGeo = adjAlloc<float4>(manCap);
+    c.con = adjAlloc<Contact>(conCap);
+  }
+  /** Everything the next step reads from earlier steps, on the stream. */
+  void saveCheckpoint(Checkpoint &c) {
+    float4 *ps[12];
+    streamPtrs(ps);
+    for (int k = 0; k < 12; k++) hipMemcpyAsync(c.streams + (size_t)k * bodyCap, ps[k], 16ull * n, hipMemcpyDeviceToDevice, stream);
+    hipMemcpyAsync(c.colA, colA, 4ull * n, hipMemcpyDeviceToDevice, stream);
+    hipMemcpyAsync(c.counters, counters, 4ull * C_WORDS, hipMemcpyDeviceToDevice, stream);
+    int last = 1 - parity;  // the buffers the last step wrote: the next step's previous ones
+    k_copy_manifolds<<<blocksFor(manCap, 256), 256, 0, stream>>>(mIds[last], mGeo[last], c.mIds, c.mGeo, counters, C_MANIFOLDS, manCap);
+    k_copy_contacts<<<blocksFor(conCap, 256), 256, 0, stream>>>(con[last], c.con, counters, C_CONTACTS, conCap);
+    c.stepCount = stepCount; c.shrinkVotes = shrinkVotes; c.stepsSinceAdapt = stepsSinceAdapt;
+    c.parity = parity; c.manifoldsValid = manifoldsValid; c.hp = hp;
+  }
+  void restoreCheckpoint(const Checkpoint &c) {
+    stepCount = c.stepCount; shrinkVotes = c.shrinkVotes; stepsSinceAdapt = c.stepsSinceAdapt;
+    parity = c.parity; manifoldsValid = c.manifoldsValid; hp = c.hp;
+    float4 *ps[12];
+    streamPtrs(ps);
+    for (int k = 0; k < 12; k++) hipMemcpyAsync(ps[k], c.streams + (size_t)k * bodyCap, 16ull * n, hipMemcpyDeviceToDevice, stream);
+    hipMemcpyAsync(colA, c.colA, 4ull * n, hipMemcpyDeviceToDevice, stream);
+    hipMemcpyAsync(counters, c.counters, 4ull * C_WORDS, hipMemcpyDeviceToDevice, stream);
+    int last = 1 - parity;
+    k_copy_manifolds<<<blocksFor(manCap, 256), 256, 0, stream>>>(c.mIds, c.mGeo, mIds[last], mGeo[last], c.counters, C_MANIFOLDS, manCap);
+    k_copy_contacts<<<blocksFor(conCap, 256), 256, 0, stream>>>(c.con, con[last], c.counters, C_CONTACTS, conCap);
+  }
+  void allocTape(uint32_t slots, uint32_t cCap, uint32_t mCap) {
+    Tape t{};
+    t.n = n; t.iters = hp.iterations; t.conCap = std::max(cCap, 1u); t.manCap = std::max(mCap, 1u); t.adjCap = 2 * t.manCap + 2 * nj;
+    size_t S = slots, K = hp.iterations + 1;
+    float4 **bodyArrays[6] = {&t.p0, &t.q0, &t.inpos, &t.inrot, &t.v0, &t.w0};
+    for (auto a : bodyArrays) *a = adjAlloc<float4>(S * n);
+    t.P = adjAlloc<float4>(S * K * n);
+    t.R = adjAlloc<float4>(S * K * n);
+    t.con = adjAlloc<Contact>(S * t.conCap);
+    t.src = adjAlloc<int>(S * t.conCap);
+    t.lam = adjAlloc<float4>(S * K * t.conCap);
+    t.pen = adjAlloc<float4>(S * K * t.conCap);
+    t.mIds = adjAlloc<uint4>(S * t.manCap);
+    t.mGeo = adjAlloc<float4>(S * t.manCap);
+    t.adjStart = adjAlloc<uint32_t>(S * (n + 1));
+    t.adjList = adjAlloc<uint32_t>(S * t.adjCap);
+    t.colorStart = adjAlloc<uint32_t>(S * 65);
+    t.colorBodies = adjAlloc<uint32_t>(S * n);
+    t.col = adjAlloc<uint32_t>(S * n);
+    t.counts = adjAlloc<uint32_t>(S * 4);
+    tape = t;
+    tapeSlots = slots;
+    slotCap.assign(slots, 0);
+    if (!src) src = adjAlloc<int>(conCap);
+  }
+  void allocAdjoint(uint32_t cCap) {
+    AdjBuf a{};
+    a.state = adjAlloc<double>(16ull * n);
+    double **bodies4[5] = {&a.ip, &a.ir, &a.np, &a.nr, &a.vs};
+    for (auto p : bodies4) *p = adjAlloc<double>(4ull * n);
+    a.mu = adjAlloc<double>(n);
+    a.lamb = adjAlloc<double>(4ull * cCap);
+    a.lambPrev = adjAlloc<double>(4ull * cCap);
+    a.c0b = adjAlloc<double>(4ull * cCap);
+    a.mub = adjAlloc<double>(cCap);
+    adj = a;
+    adjConCap = cCap;
+  }
+  void zeroF64(double *p, size_t count) { k_fill_f64<<<blocksFor(count, 256), 256, 0, stream>>>(p, count, 0.0); }
+  /** Reverse of the recorded slots [0, len) (global steps first + slot). */
+  void reverseSlots(uint32_t len, uint32_t first) {
+    uint32_t manBlocks = blocksFor(tape.manCap, 64), bodyBlocks = blocksFor(n, 64), nb = blocksFor(n, 256);
+    for (uint32_t t = len; t-- > 0;) {
+      Dev d = dev();
+      d.b.irot = tape.q0 + (size_t)t * n;  // x- of step t: the sphere contacts' rotations in evalContact
+      k_adj_velocities<<<nb, 256, 0, stream>>>(d, tape, adj, t);
+      for (uint32_t it = hp.iterations; it-- > 0;) {
+        k_adj_dual<<<manBlocks, 64, 0, stream>>>(d, tape, adj, t, it);
+        for (uint32_t c = slotCap[t]; c-- > 0;) k_adj_primal<<<bodyBlocks, 64, 0, stream>>>(d, tape, adj, t, it, c);
+      }
+      k_adj_contacts<<<manBlocks, 64, 0, stream>>>(d, tape, adj, t, first + t > 0 ? 1 : 0);
+      k_adj_bodies<<<nb, 256, 0, stream>>>(d, tape, adj, t);
+      std::swap(adj.lamb, adj.lambPrev);
+    }
+  }
+  void recordSteps(uint32_t count) {
+    rec = &tape;
+    for (uint32_t t = 0; t < count; t++) { recSlot = t; step(false); }
+    rec = nullptr;
+  }
+
   void step(bool reuseManifolds) {
     stepCount++;
     reuseManifolds = reuseManifolds && manifoldsValid;
@@ -848,6 +997,100 @@ int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd
   }
   s->timer.marks.clear();
   return rc;
+}
+
+/* Differentiable rollout: `steps` detecting steps with checkpoints every `checkpoint_every`
+ * steps (0: record every step's tape during the rollout itself, no re-run in the backward). */
+int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
+  for (uint32_t j = 0; j < s->nj; j++)
+    if (s->jinfoHost[4 * j] != (int)T_NONE) return fail(s, AVBD_ERR_INVALID, "the adjoint covers contact-only scenes (joint " + std::to_string(j) + ")");
+  if (steps == 0) return fail(s, AVBD_ERR_INVALID, "steps must be > 0");
+  try {
+    s->fwdValid = false;
+    s->fwdWhy = "forward rollout incomplete";
+    s->freeAdjoint();
+    uint32_t K = checkpoint_every;
+    uint32_t segments = K ? (steps + K - 1) / K : 0;
+    s->ckpts.resize(segments + 1);
+    for (auto &c : s->ckpts) s->allocCheckpoint(c);
+    k_fill<<<1, 64, 0, s->stream>>>(s->counters + C_STICKY, 3, 0u);
+    if (K == 0) {
+      s->allocTape(steps, s->conCap, s->manCap);
+      s->recordSteps(steps);
+    } else {
+      for (uint32_t t = 0; t < steps; t++) {
+        if (t % K == 0) s->saveCheckpoint(s->ckpts[t / K]);
+        s->step(false);
+      }
+    }
+    s->saveCheckpoint(s->ckpts.back());
+    uint32_t c[C_WORDS];
+    hipMemcpyAsync(c, s->counters, sizeof(c), hipMemcpyDeviceToHost, s->stream);
+    if (int rc = avbd_sync(s)) return rc;
+    uint32_t sticky = c[C_STICKY] | c[C_OVERFLOW] | (c[C_CLASHES] ? 0x10000u : 0u);
+    s->fwdMaxContacts = std::max(c[C_MAX_CONTACTS], c[C_CONTACTS]);
+    s->fwdMaxManifolds = std::max(c[C_MAX_MANIFOLDS], c[C_MANIFOLDS]);
+    s->fwdSteps = steps; s->fwdEvery = K; s->fwdEndStep = s->stepCount;
+    if (sticky & 0xffffu) { s->fwdWhy = "capacity overflow during the rollout: raise the pair/manifold/contact capacities"; return AVBD_OK; }
+    if (sticky & 0x10000u) { s->fwdWhy = "clashing colours during the rollout: the adjoint needs a proper colouring (raise color_cap)"; return AVBD_OK; }
+    s->fwdValid = true;
+    s->fwdWhy.clear();
+    return AVBD_OK;
+  } catch (const std::exception &e) {
+    s->freeAdjoint();
+    return fail(s, AVBD_ERR_DEVICE, e.what());
+  }
+}
+/* Reverse sweep over the last avbd_forward: adjoint of the final state (16 doubles per body:
+ * pos xyz, previous up-velocity, rot xyzw, vel xyz, 0, angvel xyz, 0) -> adjoint of the initial
+ * state (same layout) and of every body's friction. Leaves the solver at the rollout's end. */
+int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, double *d_friction) {
+  if (!s || !d_final || !d_initial) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
+  if (!s->fwdValid) return fail(s, AVBD_ERR_INVALID, s->fwdWhy);
+  if (s->stepCount != s->fwdEndStep) return fail(s, AVBD_ERR_INVALID, "the solver stepped after avbd_forward");
+  try {
+    uint32_t K = s->fwdEvery, steps = s->fwdSteps, n = s->n;
+    if (!s->adj.state) s->allocAdjoint(K ? std::max(s->fwdMaxContacts, 1u) : s->conCap);
+    if (K && s->tapeSlots < K) s->allocTape(K, std::max(s->fwdMaxContacts, 1u), std::max(s->fwdMaxManifolds, 1u));
+    AdjBuf &a = s->adj;
+    hipMemcpyAsync(a.state, d_final, 128ull * n, hipMemcpyHostToDevice, s->stream);
+    double *zero[4] = {a.lamb, a.lambPrev, a.c0b, a.mub};
+    size_t sizes[4] = {4ull * s->adjConCap, 4ull * s->adjConCap, 4ull * s->adjConCap, (size_t)s->adjConCap};
+    for (int k = 0; k < 4; k++) s->zeroF64(zero[k], sizes[k]);
+    s->zeroF64(a.mu, n);
+    if (K == 0) {
+      s->reverseSlots(steps, 0);
+    } else {
+      uint32_t segments = (steps + K - 1) / K;
+      for (uint32_t g = segments; g-- > 0;) {
+        uint32_t first = g * K, len = std::min(K, steps - first);
+        s->restoreCheckpoint(s->ckpts[g]);
+        s->recordSteps(len);
+        if (g + 1 < segments) {
+          // λ adjoint of this segment's last step arrives in the checkpoint's contact order
+          const avbd_solver::Checkpoint &c = s->ckpts[g + 1];
+          s->fill(s->table, s->hashSize, 0);
+          uint32_t mb = blocksFor(s->manCap, 256);
+          k_adj_remap_insert_counted<<<mb, 256, 0, s->stream>>>(c.mIds, c.counters + C_MANIFOLDS, s->manCap, s->table, s->hashSize - 1);
+          s->zeroF64(a.lambPrev, 4ull * s->adjConCap);
+          k_adj_remap_counted<<<blocksFor(s->tape.manCap, 256), 256, 0, s->stream>>>(c.mIds, s->tape.mIds + (size_t)(len - 1) * s->tape.manCap,
+                                                                                     s->tape.counts + 4 * (len - 1), s->table, s->hashSize - 1, a.lamb, a.lambPrev);
+          std::swap(a.lamb, a.lambPrev);
+          s->zeroF64(a.lambPrev, 4ull * s->adjConCap);
+        }
+        s->reverseSlots(len, first);
+      }
+      s->restoreCheckpoint(s->ckpts.back());
+    }
+    hipMemcpyAsync(d_initial, a.state, 128ull * n, hipMemcpyDeviceToHost, s->stream);
+    if (d_friction) hipMemcpyAsync(d_friction, a.mu, 8ull * n, hipMemcpyDeviceToHost, s->stream);
+    return avbd_sync(s);
+  } catch (const std::exception &e) {
+    return fail(s, AVBD_ERR_DEVICE, e.what());
+  }
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@ struct Dev {
   uint4 *mIds, *pmIds;
   float4 *mGeo, *pmGeo;
   Contact *con, *pcon;
+  int *src;  // when recording: per contact, the previous step's contact it warm-started from (-1: none)
   uint32_t manCap, conCap;
   uint32_t *table;
   uint32_t hashMask;
@@ -65,6 +66,9 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t *c = d.counters;
   uint32_t m = c[C_MANIFOLDS], k = c[C_CONTACTS];
+  c[C_STICKY] |= c[C_OVERFLOW] | (c[C_CLASHES] ? 0x10000u : 0u);
+  if (m > c[C_MAX_MANIFOLDS]) c[C_MAX_MANIFOLDS] = m;
+  if (k > c[C_MAX_CONTACTS]) c[C_MAX_CONTACTS] = k;
   c[C_PREV_MANIFOLDS] = m < d.manCap ? m : d.manCap;
   c[C_PREV_CONTACTS] = k < d.conCap ? k : d.conCap;
   c[C_MANIFOLDS] = 0;
@@ -323,6 +327,7 @@ __device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pg
     k.lam = f4(lam, k.lam.w);
     k.pen = f4(pen, k.pen.w);
     d.con[base + i] = k;
+    if (d.src) d.src[base + i] = (int)(pids.z + i);
   }
 }
 /** reuseManifolds (wgsl-collision.ts:758): substeps after the first keep last substep's points. */
@@ -425,6 +430,7 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
       k.pen = f4(pen, dot(basis[1], dd));
       k.lam = f4(lam, dot(basis[2], dd));
       d.con[base + i] = k;
+      if (d.src) d.src[base + i] = j;
     }
   }
 }
