@@ -151,15 +151,166 @@ def bench_batched(args, worlds=4096, loop_sample=256):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
+    """Differentiation: 256 worlds, each 64 boxes thrown at a 4x4x4 box stack, 120 steps at dt
+    1/60. Forward: the plain step with a checkpoint every 10 steps (avbd_forward); backward: the
+    reverse sweep with the active set frozen, re-running each 10-step segment from its checkpoint
+    (avbd_backward), for the gradient of sum over worlds of (1,1,1) . (final mean centroid) with
+    respect to every body's initial state and friction. The value is backward time / forward
+    time. Gradient checks on sample worlds: the oracle's f64 reverse at the same f32 states, and
+    f64 central differences of the oracle's frozen replay."""
+    import numpy as np
+    import torch
+    import avbd_amd
+    import oracle
+    from avbd_amd import records, scenes
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t = time.perf_counter()
+    bodies, ws, mus = scenes.thrown_at_stack(worlds, seed=seed)
+    n = len(bodies)
+    print(f"# {worlds} worlds, {n} bodies, built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    params = avbd_amd.default_params(dt=1 / 60, gravity=-9.81, color_cap=args.color_cap_adjoint)
+    d_final = np.zeros((n, 16))
+    for w in range(worlds):
+        idx = np.arange(ws[w], ws[w + 1])
+        dyn = idx[bodies[idx, 19] > 0]
+        d_final[dyn, 0:3] = 1.0 / len(dyn)
+
+    s = avbd_amd.GpuSolver3D(bodies, params=params, world_start=ws)  # load every kernel once, pick the schedule
+    s.forward(12, 10)
+    s.backward(d_final)
+    schedule, _ = s.autotune(2)
+    s.destroy()
+
+    def fresh():
+        solver = avbd_amd.GpuSolver3D(bodies, params=params, world_start=ws)
+        solver.set_schedule(**schedule)
+        return solver
+
+    s = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.steps(steps)
+    s.sync()
+    plain = time.perf_counter() - t0
+    s.destroy()
+    timings = {}
+    for mode in (0, every):
+        s = fresh()
+        s.sync()
+        t0 = time.perf_counter()
+        s.forward(steps, mode)
+        t1 = time.perf_counter()
+        d_init, d_mu = s.backward(d_final)
+        t2 = time.perf_counter()
+        timings[mode] = (t1 - t0, t2 - t1)
+        print(f"# checkpoint_every={mode}: forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t2 - t1):.1f} ms", file=sys.stderr, flush=True)
+        s.destroy()
+    fwd, bwd = timings[every]
+
+    # the oracle at the same f32 states (world 0 and 1): same linear map, f64
+    F32 = lambda x: float(np.float32(x))
+    op = oracle.make_params(dt=F32(params.dt), gravity=F32(params.gravity), alpha=F32(params.alpha), gamma=F32(params.gamma),
+                            margin=F32(params.contact_margin), beta_lin=F32(params.beta_lin), beta_ang=F32(params.beta_ang),
+                            lanes=tuple(params.primal_lanes))
+    vs_oracle, fd_rows = 0.0, []
+    t_cpu = 0.0
+    for w in (0, 1):
+        sl = slice(int(ws[w]), int(ws[w + 1]))
+        # the world on a solver of its own (its colour sizes pick the lane split, as the oracle's)
+        one = avbd_amd.GpuSolver3D(bodies[sl], params=params)
+        one.forward(steps, every)
+        final1 = one.read_bodies()
+        gi, gm = one.backward(d_final[sl])
+        one.destroy()
+        d_init = np.zeros((n, 16)); d_mu = np.zeros(n); final = np.zeros((n, 40), np.float32)
+        d_init[sl], d_mu[sl], final[sl] = gi, gm, final1
+        rec = records.bodies_to_oracle(bodies[sl])
+        tc = time.perf_counter()
+        m = oracle.Model(32)
+        m.set_params(op)
+        m.set_bodies(rec)
+        m.record_tape(True)
+        m.step(steps)
+        fz = oracle.Frozen(m, rec)
+        fz.load_recorded()
+        g, gmu = fz.backward(d_final[sl])
+        t_cpu += time.perf_counter() - tc
+        assert np.array_equal(m.bodies()[:, 0:3].astype(np.float32), final[sl, 0:3]), "forward differs from the oracle"
+        dyn = bodies[sl, 19] > 0
+        for cols in (slice(0, 3), slice(4, 8), slice(8, 11), slice(12, 15)):
+            a, b = d_init[sl][dyn][:, cols], g[dyn][:, cols]
+            vs_oracle = max(vs_oracle, float(np.abs(a - b).max() / np.abs(b).max()))
+        vs_oracle = max(vs_oracle, abs(d_mu[sl].sum() - gmu.sum()) / abs(gmu.sum()))
+        if w == 0:
+            # f64 central differences of the frozen replay of this world's tape
+            st0 = oracle.state_from_records(rec)
+            out = fz.run(st0)
+            gr, gmur = fz.backward(d_final[sl])  # the replay's own adjoint (f64 trajectory)
+            loss = lambda o: float((o * d_final[sl]).sum())
+            order = np.argsort(-np.abs(gr[:, 8:15]).ravel())
+            picks = [(int(k // 7), 8 + int(k % 7)) for k in order if (k % 7) != 3][:6]
+            for i, slot in picks + [(-1, -1)]:
+                row = {"input": "friction" if i < 0 else f"body {i} slot {slot}"}
+                for h in (1e-4, 1e-7):
+                    if i < 0:
+                        mu0 = rec[:, 3].copy()
+                        fd = (loss(fz.run(st0, mu0 + h)) - loss(fz.run(st0, mu0 - h))) / (2 * h)
+                    else:
+                        e = np.zeros_like(st0)
+                        e[i, slot] = h
+                        fd = (loss(fz.run(st0 + e)) - loss(fz.run(st0 - e))) / (2 * h)
+                    row[f"fd_h{h:.0e}"] = fd
+                an = gmur.sum() if i < 0 else gr[i, slot]
+                gp = d_mu[sl].sum() if i < 0 else d_init[sl][i, slot]
+                row.update({"oracle_replay": an, "gpu": gp,
+                            "oracle_vs_fd1e-4": abs(an - row["fd_h1e-04"]) / abs(an),
+                            "oracle_vs_fd1e-7": abs(an - row["fd_h1e-07"]) / abs(an),
+                            "gpu_vs_fd1e-4": abs(gp - row["fd_h1e-04"]) / abs(row["fd_h1e-04"])})
+                fd_rows.append(row)
+    out = {
+        "metric": f"adjoint/forward time ({worlds} worlds, {steps} steps)",
+        "value": bwd / fwd,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * fwd / steps,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 forward, f64 reverse",
+        "data": "synthetic",
+        "config": {"workload": "adjoint-256", "worlds": worlds, "bodies": n, "steps": steps, "checkpoint_every": every,
+                   "color_cap": params.color_cap, "schedule": schedule,
+                   "loss": "sum over worlds of (1,1,1) . mean final position of the world's dynamic bodies"},
+        "adjoint": {
+            "forward_ms": 1e3 * fwd, "backward_ms": 1e3 * bwd, "plain_forward_ms": 1e3 * plain,
+            "forward_plus_backward_over_plain": (fwd + bwd) / plain,
+            "record_all": {"forward_ms": 1e3 * timings[0][0], "backward_ms": 1e3 * timings[0][1],
+                           "backward_over_forward": timings[0][1] / timings[0][0]},
+            "grad_rel_err_vs_oracle_same_states": vs_oracle,
+            "fd_world0": fd_rows,
+        },
+        "cpu_baseline": {"value": 2 * steps / t_cpu, "unit": "worlds*steps/s (forward + backward)", "cores": 1, "kind": "port",
+                         "sample": f"2 worlds: the oracle's f32 step recording its tape, then its f64 reverse at those states, {t_cpu:.1f} s"},
+        "gpu_worlds_steps_per_s": worlds * steps / (fwd + bwd),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=240)
-    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55", "batched-4096"])
+    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55", "batched-4096", "adjoint-256"])
     ap.add_argument("--workers", type=int, default=8, help="host processes building the scene's hulls")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--color-cap", type=int, default=32)
+    ap.add_argument("--color-cap-adjoint", type=int, default=16)
+    ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
+    ap.add_argument("--adjoint-steps", type=int, default=120, help="adjoint-256 workload: steps")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration kernel)")
     args = ap.parse_args()
@@ -170,6 +321,8 @@ def main():
 
     if args.workload == "batched-4096":
         return bench_batched(args)
+    if args.workload == "adjoint-256":
+        return bench_adjoint(args, worlds=args.adjoint_worlds, steps=args.adjoint_steps)
     t = time.perf_counter()
     bodies, joints, info, hulls, substeps, iters = build_workload(args.workload, args.workers)
     print(f"# scene {args.workload}: {len(bodies)} bodies, {len(joints)} joints, built in {time.perf_counter() - t:.1f} s",

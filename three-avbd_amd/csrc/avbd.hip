@@ -145,11 +145,13 @@ struct avbd_solver {
   std::vector<Checkpoint> ckpts;   // segment starts, then the rollout's end
   std::vector<void *> adjBufs;     // every adjoint allocation (freed together)
   int *src = nullptr;              // per contact warm-start source while recording
-  Tape tape{};
-  uint32_t tapeSlots = 0;
-  Tape *rec = nullptr;             // when set, steps record into slot recSlot
+  struct TapeBuf { Tape t{}; uint32_t slots = 0; std::vector<uint32_t> cap; };  // cap: colours each recorded step swept
+  TapeBuf tapes[2];                // a segment recorded while the previous one is reversed
+  TapeBuf *rec = nullptr;          // when set, steps record into slot recSlot
   uint32_t recSlot = 0;
-  std::vector<uint32_t> slotCap;   // colour cap each recorded step swept
+  hipStream_t revStream = nullptr; // the reverse sweep's stream (the re-runs use the solver's)
+  hipEvent_t recorded[2] = {nullptr, nullptr}, reversed = nullptr;
+  uint32_t *remapTable = nullptr;
   AdjBuf adj{};
   uint32_t adjConCap = 0;
   uint32_t fwdSteps = 0, fwdEvery = 0, fwdEndStep = 0, fwdMaxContacts = 0, fwdMaxManifolds = 0;
@@ -166,6 +168,9 @@ struct avbd_solver {
   }
   ~avbd_solver() {
     freeAdjoint();
+    if (revStream) hipStreamDestroy(revStream);
+    for (auto e : recorded) if (e) hipEventDestroy(e);
+    if (reversed) hipEventDestroy(reversed);
     for (auto &d : bufs) hipFree(d.p);
     if (staging) hipFree(staging);
     for (auto &m : timer.marks) { hipEventDestroy(m.second.first); hipEventDestroy(m.second.second); }
@@ -234,6 +239,7 @@ struct avbd_solver {
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
     d.src = rec ? src : nullptr;
+    d.bits = nullptr;
     return d;
   }
   hipEvent_t mark(int kind) {
@@ -350,9 +356,9 @@ struct avbd_solver {
     k_color_scatter<<<groups, 256, 0, stream>>>(d);
     if (rec) {
       uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
-      k_tape_topology<<<blocksFor(std::max({conCap, manCap, adjCap, n + 1}), 256), 256, 0, stream>>>(d, *rec, recSlot);
-      k_tape_colors<<<blocksFor(n, 256), 256, 0, stream>>>(d, *rec, recSlot, cap);
-      slotCap[recSlot] = cap;
+      k_tape_topology<<<blocksFor(std::max({conCap, manCap, adjCap, n + 1}), 256), 256, 0, stream>>>(d, rec->t, recSlot);
+      k_tape_colors<<<blocksFor(n, 256), 256, 0, stream>>>(d, rec->t, recSlot, cap);
+      rec->cap[recSlot] = cap;
     }
   }
   void solve() {
@@ -369,11 +375,12 @@ struct avbd_solver {
     k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
     uint32_t pb = blocksFor(n, 64), snapBlocks = blocksFor(std::max(n, conCap), 256);
-    if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, *rec, recSlot, 0);
+    if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, 0);
     for (uint32_t it = 0; it < hp.iterations; it++) {
+      if (rec) d.bits = rec->t.bits + ((size_t)recSlot * rec->t.iters + it) * rec->t.conCap;
       for (uint32_t c = 0; c < cap; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
       timed(K_DUAL, [&] { k_dual<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d); });
-      if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, *rec, recSlot, it + 1);
+      if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, it + 1);
     }
     k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
   }
@@ -386,10 +393,18 @@ struct avbd_solver {
   }
   void freeAdjoint() {
     if (stream) hipStreamSynchronize(stream);
+    if (revStream) hipStreamSynchronize(revStream);
     for (void *p : adjBufs) hipFree(p);
     adjBufs.clear();
     ckpts.clear();
-    src = nullptr; tape = Tape{}; tapeSlots = 0; adj = AdjBuf{}; adjConCap = 0; rec = nullptr;
+    src = nullptr; adj = AdjBuf{}; adjConCap = 0; rec = nullptr; remapTable = nullptr;
+    for (auto &t : tapes) t = TapeBuf{};
+  }
+  void ensureReverseStream() {
+    if (revStream) return;
+    if (hipStreamCreateWithFlags(&revStream, hipStreamNonBlocking) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
+    for (auto &e : recorded) hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    hipEventCreateWithFlags(&reversed, hipEventDisableTiming);
   }
   float4 *const *streamPtrs(float4 *out[12]) {
     float4 *list[12] = {b.pos, b.rot, b.ipos, b.irot, b.size, b.moment, b.inpos, b.inrot, b.vel, b.angvel, refPos, refRot};
@@ -429,7 +444,7 @@ struct avbd_solver {
     k_copy_manifolds<<<blocksFor(manCap, 256), 256, 0, stream>>>(c.mIds, c.mGeo, mIds[last], mGeo[last], c.counters, C_MANIFOLDS, manCap);
     k_copy_contacts<<<blocksFor(conCap, 256), 256, 0, stream>>>(c.con, con[last], c.counters, C_CONTACTS, conCap);
   }
-  void allocTape(uint32_t slots, uint32_t cCap, uint32_t mCap) {
+  void allocTape(TapeBuf &tb, uint32_t slots, uint32_t cCap, uint32_t mCap) {
     Tape t{};
     t.n = n; t.iters = hp.iterations; t.conCap = std::max(cCap, 1u); t.manCap = std::max(mCap, 1u); t.adjCap = 2 * t.manCap + 2 * nj;
     size_t S = slots, K = hp.iterations + 1;
@@ -449,9 +464,11 @@ struct avbd_solver {
     t.colorBodies = adjAlloc<uint32_t>(S * n);
     t.col = adjAlloc<uint32_t>(S * n);
     t.counts = adjAlloc<uint32_t>(S * 4);
-    tape = t;
-    tapeSlots = slots;
-    slotCap.assign(slots, 0);
+    t.cman = adjAlloc<uint32_t>(S * t.conCap);
+    t.bits = adjAlloc<uchar4>(S * hp.iterations * t.conCap);
+    tb.t = t;
+    tb.slots = slots;
+    tb.cap.assign(slots, 0);
     if (!src) src = adjAlloc<int>(conCap);
   }
   void allocAdjoint(uint32_t cCap) {
@@ -466,26 +483,40 @@ struct avbd_solver {
     a.mub = adjAlloc<double>(cCap);
     adj = a;
     adjConCap = cCap;
+    remapTable = adjAlloc<uint32_t>(hashSize);
   }
-  void zeroF64(double *p, size_t count) { k_fill_f64<<<blocksFor(count, 256), 256, 0, stream>>>(p, count, 0.0); }
-  /** Reverse of the recorded slots [0, len) (global steps first + slot). */
-  void reverseSlots(uint32_t len, uint32_t first) {
-    uint32_t manBlocks = blocksFor(tape.manCap, 64), bodyBlocks = blocksFor(n, 64), nb = blocksFor(n, 256);
+  void zeroF64(double *p, size_t count, hipStream_t st) { k_fill_f64<<<blocksFor(count, 256), 256, 0, st>>>(p, count, 0.0); }
+  /** Reverse of the recorded slots [0, len) (global steps first + slot). Reads the recorded
+   *  colour buckets and counts back first (one synchronisation) to size each launch. */
+  void reverseSlots(const TapeBuf &tb, uint32_t len, uint32_t first, hipStream_t st) {
+    const Tape &tape = tb.t;
+    std::vector<uint32_t> starts(65ull * len), counts(4ull * len);
+    if (hipMemcpy(starts.data(), tape.colorStart, 4 * starts.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(counts.data(), tape.counts, 4 * counts.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("reading the tape's colour buckets failed");
+    uint32_t manBlocks = blocksFor(tape.manCap, 64), nb = blocksFor(n, 256);
     for (uint32_t t = len; t-- > 0;) {
       Dev d = dev();
       d.b.irot = tape.q0 + (size_t)t * n;  // x- of step t: the sphere contacts' rotations in evalContact
-      k_adj_velocities<<<nb, 256, 0, stream>>>(d, tape, adj, t);
+      uint32_t conBlocks = blocksFor(std::max(counts[4 * t + 1], 1u), 64);
+      k_adj_velocities<<<nb, 256, 0, st>>>(d, tape, adj, t);
       for (uint32_t it = hp.iterations; it-- > 0;) {
-        k_adj_dual<<<manBlocks, 64, 0, stream>>>(d, tape, adj, t, it);
-        for (uint32_t c = slotCap[t]; c-- > 0;) k_adj_primal<<<bodyBlocks, 64, 0, stream>>>(d, tape, adj, t, it, c);
+        k_adj_dual<<<conBlocks, 64, 0, st>>>(d, tape, adj, t, it);
+        for (uint32_t c = tb.cap[t]; c-- > 0;) {
+          uint32_t size = starts[65 * t + c + 1] - starts[65 * t + c];
+          if (size == 0) continue;
+          uint32_t lanes = 16;  // about 64k threads per launch: one wave per SIMD of the chip
+          while (lanes > 1 && (uint64_t)size * lanes > 65536) lanes >>= 1;
+          k_adj_primal<<<blocksFor((uint64_t)size * lanes, 64), 64, 0, st>>>(d, tape, adj, t, it, c, lanes);
+        }
       }
-      k_adj_contacts<<<manBlocks, 64, 0, stream>>>(d, tape, adj, t, first + t > 0 ? 1 : 0);
-      k_adj_bodies<<<nb, 256, 0, stream>>>(d, tape, adj, t);
+      k_adj_contacts<<<manBlocks, 64, 0, st>>>(d, tape, adj, t, first + t > 0 ? 1 : 0);
+      k_adj_bodies<<<nb, 256, 0, st>>>(d, tape, adj, t);
       std::swap(adj.lamb, adj.lambPrev);
     }
   }
-  void recordSteps(uint32_t count) {
-    rec = &tape;
+  void recordSteps(TapeBuf &tb, uint32_t count) {
+    rec = &tb;
     for (uint32_t t = 0; t < count; t++) { recSlot = t; step(false); }
     rec = nullptr;
   }
@@ -1017,8 +1048,8 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
     for (auto &c : s->ckpts) s->allocCheckpoint(c);
     k_fill<<<1, 64, 0, s->stream>>>(s->counters + C_STICKY, 3, 0u);
     if (K == 0) {
-      s->allocTape(steps, s->conCap, s->manCap);
-      s->recordSteps(steps);
+      s->allocTape(s->tapes[0], steps, s->conCap, s->manCap);
+      s->recordSteps(s->tapes[0], steps);
     } else {
       for (uint32_t t = 0; t < steps; t++) {
         if (t % K == 0) s->saveCheckpoint(s->ckpts[t / K]);
@@ -1045,7 +1076,9 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
 }
 /* Reverse sweep over the last avbd_forward: adjoint of the final state (16 doubles per body:
  * pos xyz, previous up-velocity, rot xyzw, vel xyz, 0, angvel xyz, 0) -> adjoint of the initial
- * state (same layout) and of every body's friction. Leaves the solver at the rollout's end. */
+ * state (same layout) and of every body's friction. Leaves the solver at the rollout's end.
+ * Segments are re-run from their checkpoints on the solver's stream while the reverse sweep of
+ * the segment after runs on a second stream (two tape buffers). */
 int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, double *d_friction) {
   if (!s || !d_final || !d_initial) return AVBD_ERR_INVALID;
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
@@ -1053,41 +1086,60 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
   if (s->stepCount != s->fwdEndStep) return fail(s, AVBD_ERR_INVALID, "the solver stepped after avbd_forward");
   try {
     uint32_t K = s->fwdEvery, steps = s->fwdSteps, n = s->n;
+    s->ensureReverseStream();
+    hipStream_t R = s->revStream, S = s->stream;
     if (!s->adj.state) s->allocAdjoint(K ? std::max(s->fwdMaxContacts, 1u) : s->conCap);
-    if (K && s->tapeSlots < K) s->allocTape(K, std::max(s->fwdMaxContacts, 1u), std::max(s->fwdMaxManifolds, 1u));
+    for (int k = 0; K && k < 2; k++)
+      if (s->tapes[k].slots < K) s->allocTape(s->tapes[k], K, std::max(s->fwdMaxContacts, 1u), std::max(s->fwdMaxManifolds, 1u));
     AdjBuf &a = s->adj;
-    hipMemcpyAsync(a.state, d_final, 128ull * n, hipMemcpyHostToDevice, s->stream);
+    if (hipStreamSynchronize(S) != hipSuccess) throw std::runtime_error("device error before the backward");
+    hipMemcpyAsync(a.state, d_final, 128ull * n, hipMemcpyHostToDevice, R);
     double *zero[4] = {a.lamb, a.lambPrev, a.c0b, a.mub};
     size_t sizes[4] = {4ull * s->adjConCap, 4ull * s->adjConCap, 4ull * s->adjConCap, (size_t)s->adjConCap};
-    for (int k = 0; k < 4; k++) s->zeroF64(zero[k], sizes[k]);
-    s->zeroF64(a.mu, n);
+    for (int k = 0; k < 4; k++) s->zeroF64(zero[k], sizes[k], R);
+    s->zeroF64(a.mu, n, R);
     if (K == 0) {
-      s->reverseSlots(steps, 0);
+      s->reverseSlots(s->tapes[0], steps, 0, R);
     } else {
       uint32_t segments = (steps + K - 1) / K;
+      auto runSegment = [&](uint32_t g) {
+        s->restoreCheckpoint(s->ckpts[g]);
+        s->recordSteps(s->tapes[g % 2], std::min(K, steps - g * K));
+        hipEventRecord(s->recorded[g % 2], S);
+      };
+      runSegment(segments - 1);
       for (uint32_t g = segments; g-- > 0;) {
         uint32_t first = g * K, len = std::min(K, steps - first);
-        s->restoreCheckpoint(s->ckpts[g]);
-        s->recordSteps(len);
+        avbd_solver::TapeBuf &tb = s->tapes[g % 2];
+        if (hipEventSynchronize(s->recorded[g % 2]) != hipSuccess) throw std::runtime_error("re-running a segment failed");
+        if (g > 0) {
+          // the next segment re-runs into the other buffer once the reverse that read it is done
+          hipStreamWaitEvent(S, s->reversed, 0);
+          runSegment(g - 1);
+        }
         if (g + 1 < segments) {
           // λ adjoint of this segment's last step arrives in the checkpoint's contact order
           const avbd_solver::Checkpoint &c = s->ckpts[g + 1];
-          s->fill(s->table, s->hashSize, 0);
+          k_fill<<<blocksFor(s->hashSize, 256), 256, 0, R>>>(s->remapTable, s->hashSize, 0u);
           uint32_t mb = blocksFor(s->manCap, 256);
-          k_adj_remap_insert_counted<<<mb, 256, 0, s->stream>>>(c.mIds, c.counters + C_MANIFOLDS, s->manCap, s->table, s->hashSize - 1);
-          s->zeroF64(a.lambPrev, 4ull * s->adjConCap);
-          k_adj_remap_counted<<<blocksFor(s->tape.manCap, 256), 256, 0, s->stream>>>(c.mIds, s->tape.mIds + (size_t)(len - 1) * s->tape.manCap,
-                                                                                     s->tape.counts + 4 * (len - 1), s->table, s->hashSize - 1, a.lamb, a.lambPrev);
+          k_adj_remap_insert_counted<<<mb, 256, 0, R>>>(c.mIds, c.counters + C_MANIFOLDS, s->manCap, s->remapTable, s->hashSize - 1);
+          s->zeroF64(a.lambPrev, 4ull * s->adjConCap, R);
+          k_adj_remap_counted<<<blocksFor(tb.t.manCap, 256), 256, 0, R>>>(c.mIds, tb.t.mIds + (size_t)(len - 1) * tb.t.manCap,
+                                                                         tb.t.counts + 4 * (len - 1), s->remapTable, s->hashSize - 1, a.lamb, a.lambPrev);
           std::swap(a.lamb, a.lambPrev);
-          s->zeroF64(a.lambPrev, 4ull * s->adjConCap);
+          s->zeroF64(a.lambPrev, 4ull * s->adjConCap, R);
         }
-        s->reverseSlots(len, first);
+        s->reverseSlots(tb, len, first, R);
+        hipEventRecord(s->reversed, R);
       }
+      hipStreamWaitEvent(S, s->reversed, 0);
       s->restoreCheckpoint(s->ckpts.back());
     }
-    hipMemcpyAsync(d_initial, a.state, 128ull * n, hipMemcpyDeviceToHost, s->stream);
-    if (d_friction) hipMemcpyAsync(d_friction, a.mu, 8ull * n, hipMemcpyDeviceToHost, s->stream);
-    return avbd_sync(s);
+    hipMemcpyAsync(d_initial, a.state, 128ull * n, hipMemcpyDeviceToHost, R);
+    if (d_friction) hipMemcpyAsync(d_friction, a.mu, 8ull * n, hipMemcpyDeviceToHost, R);
+    hipError_t e1 = hipStreamSynchronize(R), e2 = hipStreamSynchronize(S);
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    return AVBD_OK;
   } catch (const std::exception &e) {
     return fail(s, AVBD_ERR_DEVICE, e.what());
   }

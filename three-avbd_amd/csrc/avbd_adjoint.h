@@ -43,15 +43,15 @@ __device__ __forceinline__ DQ dqmul(DQ a, DQ b) {
             a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
 }
 __device__ __forceinline__ DV dqsub(DQ a, DQ b) {
-  double d = ddot4(b, b);
-  DQ r = dqmul(a, dq(-b.x / d, -b.y / d, -b.z / d, b.w / d));
+  double id = 1.0 / ddot4(b, b);
+  DQ r = dqmul(a, dq(-b.x * id, -b.y * id, -b.z * id, b.w * id));
   return dv(r.x * 2.0, r.y * 2.0, r.z * 2.0);
 }
 __device__ __forceinline__ DQ dqadd(DQ a, DV w) {
   DQ m = dqmul(dq(w.x, w.y, w.z, 0.0), a);
   DQ s = dq(a.x + m.x * 0.5, a.y + m.y * 0.5, a.z + m.z * 0.5, a.w + m.w * 0.5);
-  double l = sqrt(ddot4(s, s));
-  return dq(s.x / l, s.y / l, s.z / l, s.w / l);
+  double il = 1.0 / sqrt(ddot4(s, s));
+  return dq(s.x * il, s.y * il, s.z * il, s.w * il);
 }
 __device__ __forceinline__ DV dqrotate(DQ r, DV a) {
   DV u = dv(r.x, r.y, r.z);
@@ -67,10 +67,10 @@ __device__ __forceinline__ void dorthonormal(DV n, DV *b) {
 __device__ __forceinline__ void qaddRev(DQ q, DV w, DQ g, DQ &qb, DV &wb) {
   DQ m = dqmul(dq(w.x, w.y, w.z, 0.0), q);
   DQ s = dq(q.x + m.x * 0.5, q.y + m.y * 0.5, q.z + m.z * 0.5, q.w + m.w * 0.5);
-  double len = sqrt(ddot4(s, s));
-  DQ o = dq(s.x / len, s.y / len, s.z / len, s.w / len);
+  double il = 1.0 / sqrt(ddot4(s, s));
+  DQ o = dq(s.x * il, s.y * il, s.z * il, s.w * il);
   double og = ddot4(o, g);
-  DQ mb = dq((g.x - o.x * og) / len, (g.y - o.y * og) / len, (g.z - o.z * og) / len, (g.w - o.w * og) / len);
+  DQ mb = dq((g.x - o.x * og) * il, (g.y - o.y * og) * il, (g.z - o.z * og) * il, (g.w - o.w * og) * il);
   qb.x += mb.x + 0.5 * (w.z * mb.y - w.y * mb.z - w.x * mb.w);
   qb.y += mb.y + 0.5 * (-w.z * mb.x + w.x * mb.z - w.y * mb.w);
   qb.z += mb.z + 0.5 * (w.y * mb.x - w.x * mb.y - w.z * mb.w);
@@ -81,19 +81,19 @@ __device__ __forceinline__ void qaddRev(DQ q, DV w, DQ g, DQ &qb, DV &wb) {
 }
 /** Reverse of dqsub(a, b) = 2 vec(a b⁻¹). */
 __device__ __forceinline__ void qsubRev(DQ a, DQ b, DV g, DQ &ab, DQ &bb) {
-  double d = ddot4(b, b);
-  DQ c = dq(-b.x / d, -b.y / d, -b.z / d, b.w / d);
+  double id = 1.0 / ddot4(b, b);
+  DQ c = dq(-b.x * id, -b.y * id, -b.z * id, b.w * id);
   double gx = 2 * g.x, gy = 2 * g.y, gz = 2 * g.z;
   ab.x += gx * c.w - gy * c.z + gz * c.y;
   ab.y += gx * c.z + gy * c.w - gz * c.x;
   ab.z += -gx * c.y + gy * c.x + gz * c.w;
   ab.w += gx * c.x + gy * c.y + gz * c.z;
   DQ cb = dq(gx * a.w + gy * a.z - gz * a.y, -gx * a.z + gy * a.w + gz * a.x, gx * a.y - gy * a.x + gz * a.w, gx * a.x + gy * a.y + gz * a.z);
-  double cc = ddot4(cb, c);
-  bb.x += -cb.x / d - 2 * b.x * cc / d;
-  bb.y += -cb.y / d - 2 * b.y * cc / d;
-  bb.z += -cb.z / d - 2 * b.z * cc / d;
-  bb.w += cb.w / d - 2 * b.w * cc / d;
+  double cc2 = 2 * ddot4(cb, c);
+  bb.x += (-cb.x - b.x * cc2) * id;
+  bb.y += (-cb.y - b.y * cc2) * id;
+  bb.z += (-cb.z - b.z * cc2) * id;
+  bb.w += (cb.w - b.w * cc2) * id;
 }
 /** Reverse of dqrotate(q, v) with respect to q. */
 __device__ __forceinline__ void qrotateRev(DQ q, DV v, DV g, DQ &qb) {
@@ -121,6 +121,8 @@ struct Tape {
   uint32_t *colorStart;                       // [s*65 + c]
   uint32_t *colorBodies, *col;                // [s*n + k], colour of body i (255: not swept)
   uint32_t *counts;                           // [s*4]: manifolds, contacts, adjacency entries
+  uint32_t *cman;                             // [s*conCap + c]: the manifold contact c belongs to
+  uchar4 *bits;                               // [(s*iters + it)*conCap + c]: the forward's branch bits
   __device__ __forceinline__ size_t snap(uint32_t s, uint32_t k) const { return (size_t)s * (iters + 1) + k; }
 };
 /** f64 adjoints. state: 16 per body (pos xyz, prev-up, rot xyzw, vel xyz, 0, angvel xyz, 0): the
@@ -133,8 +135,11 @@ struct AdjBuf {
   double *c0b, *mub;               // 4 / 1 per contact of this step
 };
 
-__device__ __forceinline__ void atomicAddV(double *p, DV v) { atomicAdd(p, v.x); atomicAdd(p + 1, v.y); atomicAdd(p + 2, v.z); }
-__device__ __forceinline__ void atomicAddQ(double *p, DQ v) { atomicAdd(p, v.x); atomicAdd(p + 1, v.y); atomicAdd(p + 2, v.z); atomicAdd(p + 3, v.w); }
+// f64 adds into the adjoint buffers (device memory from hipMalloc, coarse-grained): the hardware
+// float atomics (global_atomic_add_f64), not the compare-and-swap loop atomicAdd compiles to.
+__device__ __forceinline__ void addD(double *p, double v) { (void)unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomicAddV(double *p, DV v) { addD(p, v.x); addD(p + 1, v.y); addD(p + 2, v.z); }
+__device__ __forceinline__ void atomicAddQ(double *p, DQ v) { addD(p, v.x); addD(p + 1, v.y); addD(p + 2, v.z); addD(p + 3, v.w); }
 __device__ __forceinline__ DV ldV(const double *p) { return dv(p[0], p[1], p[2]); }
 __device__ __forceinline__ DQ ldQ(const double *p) { return dq(p[0], p[1], p[2], p[3]); }
 __device__ __forceinline__ void stV(double *p, DV v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
@@ -149,7 +154,11 @@ __global__ void k_tape_topology(Dev d, Tape T, uint32_t s) {
   uint32_t total = max(max(max(M, C), max(E, n + 1)), MAX_COLORS + 1);
   GRID_STRIDE(i, total) {
     if (i < C) { T.con[(size_t)s * T.conCap + i] = d.con[i]; T.src[(size_t)s * T.conCap + i] = d.src[i]; }
-    if (i < M) { T.mIds[(size_t)s * T.manCap + i] = d.mIds[i]; T.mGeo[(size_t)s * T.manCap + i] = d.mGeo[i]; }
+    if (i < M) {
+      uint4 ids = d.mIds[i];
+      T.mIds[(size_t)s * T.manCap + i] = ids; T.mGeo[(size_t)s * T.manCap + i] = d.mGeo[i];
+      for (uint32_t k = 0; k < (ids.w & 15u); k++) T.cman[(size_t)s * T.conCap + ids.z + k] = i;
+    }
     if (i < E) T.adjList[(size_t)s * T.adjCap + i] = d.adjList[i];
     if (i <= n) T.adjStart[(size_t)s * (n + 1) + i] = d.adjStart[i];
     if (i < n) T.colorBodies[(size_t)s * n + i] = d.colorBodies[i];
@@ -199,17 +208,6 @@ __global__ void k_fill_f64(double *p, size_t n, double v) { for (size_t i = bloc
 // ---- one contact evaluation in f64 with the forward's branches --------------------------------
 struct PoseView { DV p; DQ q; DV ip; DQ ir; };
 struct DEv { DV rAW, rBW, dLinA, dLinB, dAngA, dAngB, C, G, F; double fs, bounds, s; };
-/** The forward's branch bits of one evaluation (bit0: normal force clamped to 0, bit1: sliding),
- *  from its own f32 evalContact on the recorded inputs. */
-__device__ __forceinline__ uint32_t branchBits(const Dev &d, const Contact &k, const V *basis, float friction, float4 pA, float4 rA,
-                                               float4 ipA, float4 irA, uint32_t ia, float4 pB, float4 rB, float4 ipB, float4 irB,
-                                               uint32_t ib) {
-  PairBody A, B;
-  A.rot = q4(rA); A.dLin = sub(xyz(pA), xyz(ipA)); A.dAng = qsub(A.rot, q4(irA));
-  B.rot = q4(rB); B.dLin = sub(xyz(pB), xyz(ipB)); B.dAng = qsub(B.rot, q4(irB));
-  Eval e = evalContact(d, k, basis, friction, A, ia, B, ib, d.p.alpha);
-  return (e.F.x < 0.0f ? 0u : 1u) | ((e.frictionScale > e.bounds && e.frictionScale > 0.0f) ? 2u : 0u);
-}
 __device__ __forceinline__ DEv evalD(const Contact &k, DV c0, const DV *l, DV pen, DV lam, uint32_t bits, double mu, double alpha,
                                      const PoseView &A, const PoseView &B) {
   DEv e;
@@ -232,7 +230,7 @@ __device__ __forceinline__ DEv evalD(const Contact &k, DV c0, const DV *l, DV pe
   e.bounds = fabs(e.F.x) * mu;
   e.fs = sqrt(e.G.y * e.G.y + e.G.z * e.G.z);
   e.s = 1.0;
-  if (bits & 2u) { e.s = e.fs > 0 ? e.bounds / e.fs : 0.0; e.F.y = e.G.y * e.s; e.F.z = e.G.z * e.s; }
+  if (bits & 2u) { e.s = e.fs > 0 ? e.bounds * (1.0 / e.fs) : 0.0; e.F.y = e.G.y * e.s; e.F.z = e.G.z * e.s; }
   return e;
 }
 /** Adjoints of one evaluation's inputs for output adjoint Fb and world-anchor adjoints rAb, rBb. */
@@ -243,10 +241,10 @@ __device__ __forceinline__ void evalRev(const DEv &e, const Contact &k, const DV
   double Fxb = Fb.x;
   if (bits & 2u) {
     if (e.fs > 0) {
-      double sb = Fb.y * e.G.y + Fb.z * e.G.z;
+      double sb = Fb.y * e.G.y + Fb.z * e.G.z, ifs = 1.0 / e.fs;
       Gb.y = e.s * Fb.y; Gb.z = e.s * Fb.z;
-      double boundsb = sb / e.fs, fsb = -sb * e.s / e.fs;
-      Gb.y += fsb * e.G.y / e.fs; Gb.z += fsb * e.G.z / e.fs;
+      double boundsb = sb * ifs, fsb = -sb * e.s * ifs;
+      Gb.y += fsb * e.G.y * ifs; Gb.z += fsb * e.G.z * ifs;
       o.mu += fabs(e.F.x) * boundsb;
       Fxb += (e.F.x < 0 ? -mu : mu) * boundsb;
     }
@@ -310,47 +308,40 @@ __global__ void k_adj_velocities(Dev d, Tape T, AdjBuf A, uint32_t s) {
     st[3] = 0; for (int k = 8; k < 16; k++) st[k] = 0;
   }
 }
-/** The dual of iteration `it` of step s in reverse, one manifold per thread. */
+/** The dual of iteration `it` of step s in reverse, one contact per thread. */
 __global__ void __launch_bounds__(64) k_adj_dual(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it) {
-  uint32_t M = T.counts[s * 4], n = d.n;
+  uint32_t C = T.counts[s * 4 + 1], n = d.n;
   const float4 *P = T.P + T.snap(s, it + 1) * n, *R = T.R + T.snap(s, it + 1) * n;
   const float4 *p0 = T.p0 + (size_t)s * n, *q0 = T.q0 + (size_t)s * n;
   const float4 *lam = T.lam + T.snap(s, it) * T.conCap, *pen = T.pen + T.snap(s, it) * T.conCap;
   double alpha = d.p.alpha;
-  GRID_STRIDE(m, M) {
+  GRID_STRIDE(c, C) {
+    size_t cc = (size_t)s * T.conCap + c;
+    uint32_t m = T.cman[cc];
+    if (m >= T.counts[s * 4]) continue;
     uint4 ids = T.mIds[(size_t)s * T.manCap + m];
     float4 geo = T.mGeo[(size_t)s * T.manCap + m];
     uint32_t a = ids.x, b = ids.y;
-    V basis[3];
-    orthonormal(xyz(geo), basis);
     DV l[3];
     dorthonormal(dv3(geo), l);
     double mu = sqrt((double)d.b.pos[a].w * (double)d.b.pos[b].w);
     PoseView PA{dv3(P[a]), dq4(R[a]), dv3(p0[a]), dq4(q0[a])}, PB{dv3(P[b]), dq4(R[b]), dv3(p0[b]), dq4(q0[b])};
-    EvBar sum;
-    zeroBar(sum);
-    uint32_t end = ids.z + (ids.w & 15u);
-    for (uint32_t c = ids.z; c < end; c++) {
-      size_t cc = (size_t)s * T.conCap + c;
-      Contact k = T.con[cc];
-      k.pen = pen[c];
-      k.lam = lam[c];
-      uint32_t bits = branchBits(d, k, basis, geo.w, P[a], R[a], p0[a], q0[a], a, P[b], R[b], p0[b], q0[b], b);
-      DEv e = evalD(k, dv(k.rB.w, k.pen.w, k.lam.w), l, dv3(k.pen), dv3(k.lam), bits, mu, alpha, PA, PB);
-      double *lb = A.lamb + 4 * (size_t)c;
-      DV Fb = ldV(lb);
-      EvBar o;
-      zeroBar(o);
-      evalRev(e, k, l, dv3(k.pen), bits, mu, alpha, PA, PB, Fb, dv(0, 0, 0), dv(0, 0, 0), o);
-      stV(lb, o.lam);
-      double *cb = A.c0b + 4 * (size_t)c;
-      stV(cb, ldV(cb) + o.c0);
-      A.mub[c] += o.mu;
-      sum.pA = sum.pA + o.pA; sum.qA = sum.qA + o.qA; sum.ipA = sum.ipA + o.ipA; sum.irA = sum.irA + o.irA;
-      sum.pB = sum.pB + o.pB; sum.qB = sum.qB + o.qB; sum.ipB = sum.ipB + o.ipB; sum.irB = sum.irB + o.irB;
-    }
-    scatterBody(d, A, a, sum.pA, sum.qA, sum.ipA, sum.irA);
-    scatterBody(d, A, b, sum.pB, sum.qB, sum.ipB, sum.irB);
+    Contact k = T.con[cc];
+    k.pen = pen[c];
+    k.lam = lam[c];
+    uint32_t bits = T.bits[((size_t)s * T.iters + it) * T.conCap + c].z;
+    DEv e = evalD(k, dv(k.rB.w, k.pen.w, k.lam.w), l, dv3(k.pen), dv3(k.lam), bits, mu, alpha, PA, PB);
+    double *lb = A.lamb + 4 * (size_t)c;
+    DV Fb = ldV(lb);
+    EvBar o;
+    zeroBar(o);
+    evalRev(e, k, l, dv3(k.pen), bits, mu, alpha, PA, PB, Fb, dv(0, 0, 0), dv(0, 0, 0), o);
+    stV(lb, o.lam);
+    double *cb = A.c0b + 4 * (size_t)c;
+    stV(cb, ldV(cb) + o.c0);
+    A.mub[c] += o.mu;
+    scatterBody(d, A, a, o.pA, o.qA, o.ipA, o.irA);
+    scatterBody(d, A, b, o.pB, o.qB, o.ipB, o.irB);
   }
 }
 
@@ -370,7 +361,7 @@ __device__ __forceinline__ void addRowD(DSys &S, DV jl, DV ja, double k, double 
   for (int i = 0; i < 3; i++) { S.r[i] += L[i] * f; S.r[3 + i] += G[i] * f; }
 }
 /** LDLᵀ of the system with the forward's pivot floor (1e-5 of the diagonal), in f64. */
-struct DFac { double L[15], D[6]; };
+struct DFac { double L[15], D[6], iD[6]; };
 __device__ __forceinline__ double sysAt(const DSys &S, int i, int j) {  // i >= j
   const int tri[3][3] = {{0, -1, -1}, {1, 2, -1}, {3, 4, 5}};
   if (i < 3) return S.lin[tri[i][j]];
@@ -386,12 +377,13 @@ __device__ __forceinline__ void factor(const DSys &S, DFac &F) {
     for (int j = 0; j < k; j++) d -= F.L[lidx(k, j)] * F.L[lidx(k, j)] * F.D[j];
     double floor = sysAt(S, k, k) * 1e-5;
     F.D[k] = k == 0 ? d : (d > floor ? d : floor);
+    F.iD[k] = 1.0 / F.D[k];
 #pragma unroll
     for (int i = k + 1; i < 6; i++) {
       double v = sysAt(S, i, k);
 #pragma unroll
       for (int j = 0; j < k; j++) v -= F.L[lidx(i, j)] * F.L[lidx(k, j)] * F.D[j];
-      F.L[lidx(i, k)] = v / F.D[k];
+      F.L[lidx(i, k)] = v * F.iD[k];
     }
   }
 }
@@ -405,159 +397,196 @@ __device__ __forceinline__ void solveF(const DFac &F, const double *b, double *x
   }
 #pragma unroll
   for (int i = 5; i >= 0; i--) {
-    x[i] = y[i] / F.D[i];
+    x[i] = y[i] * F.iD[i];
 #pragma unroll
     for (int j = i + 1; j < 6; j++) x[i] -= F.L[lidx(j, i)] * x[j];
   }
 }
-/** The colour `col` of iteration `it` of step s in reverse, one body per thread. */
-__global__ void __launch_bounds__(64) k_adj_primal(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it, uint32_t col) {
+__device__ __forceinline__ double shx(double v, int m) { return __shfl_xor(v, m); }
+__device__ __forceinline__ DV shxV(DV v, int m) { return dv(shx(v.x, m), shx(v.y, m), shx(v.z, m)); }
+__device__ __forceinline__ DQ shxQ(DQ v, int m) { return dq(shx(v.x, m), shx(v.y, m), shx(v.z, m), shx(v.w, m)); }
+/** Inputs of one adjacency entry (a manifold) of body i at the versions its update read. */
+struct EntryView {
+  uint4 ids; float4 geo; bool isA; uint32_t j; DV l[3]; double mu; PoseView PA, PB;
+};
+__device__ __forceinline__ void entryView(const Dev &d, const Tape &T, uint32_t s, uint32_t i, uint4 ids, uint32_t mid, uint32_t col,
+                                          const float4 *Pn, const float4 *Rn, const float4 *Po, const float4 *Ro, const float4 *p0,
+                                          const float4 *q0, EntryView &v) {
+  v.ids = ids;
+  v.geo = T.mGeo[(size_t)s * T.manCap + mid];
+  v.isA = v.ids.x == i;
+  v.j = v.isA ? v.ids.y : v.ids.x;
+  bool swept = T.col[(size_t)s * d.n + v.j] < col;  // already updated in this iteration
+  float4 pj = swept ? Pn[v.j] : Po[v.j], rj = swept ? Rn[v.j] : Ro[v.j];
+  float4 pa = v.isA ? Po[i] : pj, ra = v.isA ? Ro[i] : rj, pb = v.isA ? pj : Po[i], rb = v.isA ? rj : Ro[i];
+  dorthonormal(dv3(v.geo), v.l);
+  v.mu = sqrt((double)d.b.pos[v.ids.x].w * (double)d.b.pos[v.ids.y].w);
+  v.PA = PoseView{dv3(pa), dq4(ra), dv3(p0[v.ids.x]), dq4(q0[v.ids.x])};
+  v.PB = PoseView{dv3(pb), dq4(rb), dv3(p0[v.ids.y]), dq4(q0[v.ids.y])};
+}
+/** The colour `col` of iteration `it` of step s in reverse: `lanes` threads per body share its
+ *  contact points (the k-th point over its adjacency entries goes to lane k mod lanes); the
+ *  system and the body's own adjoints are reduced across them with shuffles (every lane then
+ *  holds the sums and solves the same 6x6 systems). */
+__global__ void __launch_bounds__(64) k_adj_primal(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it, uint32_t col, uint32_t lanes) {
   uint32_t n = d.n;
   uint32_t start = T.colorStart[s * 65 + col], count = T.colorStart[s * 65 + col + 1] - start;
   const float4 *Pn = T.P + T.snap(s, it + 1) * n, *Rn = T.R + T.snap(s, it + 1) * n;
   const float4 *Po = T.P + T.snap(s, it) * n, *Ro = T.R + T.snap(s, it) * n;
   const float4 *p0 = T.p0 + (size_t)s * n, *q0 = T.q0 + (size_t)s * n;
   const float4 *lam = T.lam + T.snap(s, it) * T.conCap, *pen = T.pen + T.snap(s, it) * T.conCap;
-  const uint32_t *adjS = T.adjStart + (size_t)s * (n + 1), *adjL = T.adjList + (size_t)s * T.adjCap, *colOf = T.col + (size_t)s * n;
-  const uint4 *mIds = T.mIds + (size_t)s * T.manCap;
-  const float4 *mGeo = T.mGeo + (size_t)s * T.manCap;
+  const uint32_t *adjS = T.adjStart + (size_t)s * (n + 1), *adjL = T.adjList + (size_t)s * T.adjCap;
+  const uchar4 *bitsIt = T.bits + ((size_t)s * T.iters + it) * T.conCap;
   double dt = d.p.dt, dt2 = dt * dt, alpha = d.p.alpha;
-  GRID_STRIDE(k, count) {
-    uint32_t i = T.colorBodies[(size_t)s * n + start + k];
-    double mass = d.b.size[i].w;
-    float4 mo = d.b.moment[i];
-    double m = mass / dt2;
-    double I[3] = {mo.x / dt2, mo.y / dt2, mo.z / dt2};
+  uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes, per = blockDim.x / lanes;
+  for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
+    uint32_t k = base + slot;
+    bool live = k < count;
+    uint32_t i = live ? T.colorBodies[(size_t)s * n + start + k] : 0;
+    double m = 0, I[3] = {0, 0, 0};
     DSys S;
 #pragma unroll
     for (int e = 0; e < 6; e++) { S.lin[e] = 0; S.ang[e] = 0; S.r[e] = 0; }
 #pragma unroll
     for (int e = 0; e < 9; e++) S.cr[e] = 0;
-    S.lin[0] = m; S.lin[2] = m; S.lin[5] = m;
-    S.ang[0] = I[0]; S.ang[2] = I[1]; S.ang[5] = I[2];
-    DV pi = dv3(Po[i]);
-    DQ qi = dq4(Ro[i]);
-    DV dx = pi - dv3(T.inpos[(size_t)s * n + i]);
-    DV dqv = dqsub(qi, dq4(T.inrot[(size_t)s * n + i]));
-    S.r[0] = m * dx.x; S.r[1] = m * dx.y; S.r[2] = m * dx.z;
-    S.r[3] = I[0] * dqv.x; S.r[4] = I[1] * dqv.y; S.r[5] = I[2] * dqv.z;
-    uint32_t e0 = adjS[i], e1 = adjS[i + 1];
-    // pass 1: the system at the pre-update state
-    for (uint32_t e = e0; e < e1; e++) {
-      uint32_t mid = adjL[e] - d.nj;
-      uint4 ids = mIds[mid];
-      float4 geo = mGeo[mid];
-      bool isA = ids.x == i;
-      uint32_t j = isA ? ids.y : ids.x;
-      bool swept = colOf[j] < col;
-      float4 pj = swept ? Pn[j] : Po[j], rj = swept ? Rn[j] : Ro[j];
-      float4 pa = isA ? Po[i] : pj, ra = isA ? Ro[i] : rj, pb = isA ? pj : Po[i], rb = isA ? rj : Ro[i];
-      V basis[3];
-      orthonormal(xyz(geo), basis);
-      DV l[3];
-      dorthonormal(dv3(geo), l);
-      double mu = sqrt((double)d.b.pos[ids.x].w * (double)d.b.pos[ids.y].w);
-      PoseView PA{dv3(pa), dq4(ra), dv3(p0[ids.x]), dq4(q0[ids.x])}, PB{dv3(pb), dq4(rb), dv3(p0[ids.y]), dq4(q0[ids.y])};
-      double sg = isA ? 1.0 : -1.0;
-      uint32_t cEnd = ids.z + (ids.w & 15u);
-      for (uint32_t c = ids.z; c < cEnd; c++) {
-        Contact kc = T.con[(size_t)s * T.conCap + c];
-        kc.pen = pen[c];
-        kc.lam = lam[c];
-        uint32_t bits = branchBits(d, kc, basis, geo.w, pa, ra, p0[ids.x], q0[ids.x], ids.x, pb, rb, p0[ids.y], q0[ids.y], ids.y);
-        DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), l, dv3(kc.pen), dv3(kc.lam), bits, mu, alpha, PA, PB);
-        DV r = isA ? ev.rAW : ev.rBW;
-        for (int row = 0; row < 3; row++) {
-          DV jl = l[row] * sg;
-          addRowD(S, jl, dcross(r, jl), row == 0 ? kc.pen.x : (row == 1 ? kc.pen.y : kc.pen.z), dcomp(ev.F, row));
+    DV pi = dv(0, 0, 0);
+    DQ qi = dq(0, 0, 0, 1);
+    uint32_t e0 = 0, e1 = 0;
+    if (live) {
+      float4 mo = d.b.moment[i];
+      m = (double)d.b.size[i].w / dt2;
+      I[0] = mo.x / dt2; I[1] = mo.y / dt2; I[2] = mo.z / dt2;
+      pi = dv3(Po[i]);
+      qi = dq4(Ro[i]);
+      e0 = adjS[i]; e1 = adjS[i + 1];
+      if (lane == 0) {
+        S.lin[0] = m; S.lin[2] = m; S.lin[5] = m;
+        S.ang[0] = I[0]; S.ang[2] = I[1]; S.ang[5] = I[2];
+        DV dx = pi - dv3(T.inpos[(size_t)s * n + i]);
+        DV dqv = dqsub(qi, dq4(T.inrot[(size_t)s * n + i]));
+        S.r[0] = m * dx.x; S.r[1] = m * dx.y; S.r[2] = m * dx.z;
+        S.r[3] = I[0] * dqv.x; S.r[4] = I[1] * dqv.y; S.r[5] = I[2] * dqv.z;
+      }
+      // pass 1: this lane's share of the system at the pre-update state
+      uint32_t run = 0;
+      for (uint32_t e = e0; e < e1; e++) {
+        uint32_t mid = adjL[e] - d.nj;
+        uint4 ids = T.mIds[(size_t)s * T.manCap + mid];
+        uint32_t cnt = ids.w & 15u, k0 = (lane + lanes - run % lanes) % lanes;
+        run += cnt;
+        if (k0 >= cnt) continue;
+        EntryView v;
+        entryView(d, T, s, i, ids, mid, col, Pn, Rn, Po, Ro, p0, q0, v);
+        double sg = v.isA ? 1.0 : -1.0;
+        for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
+          Contact kc = T.con[(size_t)s * T.conCap + c];
+          kc.pen = pen[c];
+          kc.lam = lam[c];
+          uchar4 bb = bitsIt[c];
+          uint32_t bits = v.isA ? bb.x : bb.y;
+          DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), v.l, dv3(kc.pen), dv3(kc.lam), bits, v.mu, alpha, v.PA, v.PB);
+          DV r = v.isA ? ev.rAW : ev.rBW;
+          for (int row = 0; row < 3; row++) {
+            DV jl = v.l[row] * sg;
+            addRowD(S, jl, dcross(r, jl), row == 0 ? kc.pen.x : (row == 1 ? kc.pen.y : kc.pen.z), dcomp(ev.F, row));
+          }
         }
       }
     }
-    DFac F;
-    factor(S, F);
-    double x[6];
-    solveF(F, S.r, x);
-    // reverse of p+ = p - Δlin, q+ = qadd(q, -Δang)
-    double *st = A.state + 16 * (size_t)i;
-    DV pbar = ldV(st);
-    DQ qbar = dq(0, 0, 0, 0);
-    DV wb = dv(0, 0, 0);
-    qaddRev(qi, dv(-x[3], -x[4], -x[5]), ldQ(st + 4), qbar, wb);
-    double xb[6] = {-pbar.x, -pbar.y, -pbar.z, -wb.x, -wb.y, -wb.z};
-    double rb[6];
-    solveF(F, xb, rb);
-    DV rl = dv(m * rb[0], m * rb[1], m * rb[2]);
-    pbar = pbar + rl;
-    DV npb = -rl;
-    DQ nrb = dq(0, 0, 0, 0);
-    qsubRev(qi, dq4(T.inrot[(size_t)s * n + i]), dv(I[0] * rb[3], I[1] * rb[4], I[2] * rb[5]), qbar, nrb);
-    DV ipb = dv(0, 0, 0);
-    DQ irb = dq(0, 0, 0, 0);
-    // pass 2: the rows' adjoints
-    for (uint32_t e = e0; e < e1; e++) {
-      uint32_t mid = adjL[e] - d.nj;
-      uint4 ids = mIds[mid];
-      float4 geo = mGeo[mid];
-      bool isA = ids.x == i;
-      uint32_t j = isA ? ids.y : ids.x;
-      bool swept = colOf[j] < col;
-      float4 pj = swept ? Pn[j] : Po[j], rj = swept ? Rn[j] : Ro[j];
-      float4 pa = isA ? Po[i] : pj, ra = isA ? Ro[i] : rj, pb = isA ? pj : Po[i], rbq = isA ? rj : Ro[i];
-      V basis[3];
-      orthonormal(xyz(geo), basis);
-      DV l[3];
-      dorthonormal(dv3(geo), l);
-      double mu = sqrt((double)d.b.pos[ids.x].w * (double)d.b.pos[ids.y].w);
-      PoseView PA{dv3(pa), dq4(ra), dv3(p0[ids.x]), dq4(q0[ids.x])}, PB{dv3(pb), dq4(rbq), dv3(p0[ids.y]), dq4(q0[ids.y])};
-      double sg = isA ? 1.0 : -1.0;
-      DV nbP = dv(0, 0, 0), nbIp = nbP;
-      DQ nbQ = dq(0, 0, 0, 0), nbIr = nbQ;
-      uint32_t cEnd = ids.z + (ids.w & 15u);
-      for (uint32_t c = ids.z; c < cEnd; c++) {
-        Contact kc = T.con[(size_t)s * T.conCap + c];
-        kc.pen = pen[c];
-        kc.lam = lam[c];
-        uint32_t bits = branchBits(d, kc, basis, geo.w, pa, ra, p0[ids.x], q0[ids.x], ids.x, pb, rbq, p0[ids.y], q0[ids.y], ids.y);
-        DV penD = dv3(kc.pen);
-        DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), l, penD, dv3(kc.lam), bits, mu, alpha, PA, PB);
-        DV r = isA ? ev.rAW : ev.rBW;
-        DV Fb = dv(0, 0, 0), rW = dv(0, 0, 0);
-        for (int row = 0; row < 3; row++) {
-          DV jl = l[row] * sg, ja = dcross(r, jl);
-          double J[6] = {jl.x, jl.y, jl.z, ja.x, ja.y, ja.z};
-          double kk = dcomp(penD, row), Jx = 0, rJ = 0;
+    for (uint32_t w = 1; w < lanes; w <<= 1) {
 #pragma unroll
-          for (int q = 0; q < 6; q++) { Jx += J[q] * x[q]; rJ += rb[q] * J[q]; }
-          if (row == 0) Fb.x = rJ; else if (row == 1) Fb.y = rJ; else Fb.z = rJ;
-          double f = dcomp(ev.F, row);
-          DV jab = dv(f * rb[3] - kk * (Jx * rb[3] + rJ * x[3]), f * rb[4] - kk * (Jx * rb[4] + rJ * x[4]), f * rb[5] - kk * (Jx * rb[5] + rJ * x[5]));
-          rW = rW + dcross(jl, jab);
-        }
-        EvBar o;
-        zeroBar(o);
-        evalRev(ev, kc, l, penD, bits, mu, alpha, PA, PB, Fb, isA ? rW : dv(0, 0, 0), isA ? dv(0, 0, 0) : rW, o);
-        double *lb = A.lamb + 4 * (size_t)c;
-        stV(lb, ldV(lb) + o.lam);
-        double *cb = A.c0b + 4 * (size_t)c;
-        stV(cb, ldV(cb) + o.c0);
-        A.mub[c] += o.mu;
-        if (isA) {
-          pbar = pbar + o.pA; qbar = qbar + o.qA; ipb = ipb + o.ipA; irb = irb + o.irA;
-          nbP = nbP + o.pB; nbQ = nbQ + o.qB; nbIp = nbIp + o.ipB; nbIr = nbIr + o.irB;
-        } else {
-          pbar = pbar + o.pB; qbar = qbar + o.qB; ipb = ipb + o.ipB; irb = irb + o.irB;
-          nbP = nbP + o.pA; nbQ = nbQ + o.qA; nbIp = nbIp + o.ipA; nbIr = nbIr + o.irA;
-        }
-      }
-      scatterBody(d, A, j, nbP, nbQ, nbIp, nbIr);
+      for (int e = 0; e < 6; e++) { S.lin[e] += shx(S.lin[e], w); S.ang[e] += shx(S.ang[e], w); S.r[e] += shx(S.r[e], w); }
+#pragma unroll
+      for (int e = 0; e < 9; e++) S.cr[e] += shx(S.cr[e], w);
     }
-    stV(st, pbar);
-    stQ(st + 4, qbar);
-    double *ip = A.ip + 4 * (size_t)i, *ir = A.ir + 4 * (size_t)i, *np = A.np + 4 * (size_t)i, *nr = A.nr + 4 * (size_t)i;
-    stV(ip, ldV(ip) + ipb);
-    stQ(ir, ldQ(ir) + irb);
-    stV(np, ldV(np) + npb);
-    stQ(nr, ldQ(nr) + nrb);
+    double x[6] = {0, 0, 0, 0, 0, 0}, rb[6] = {0, 0, 0, 0, 0, 0};
+    DV pbar = dv(0, 0, 0), ipb = pbar, npb = pbar;
+    DQ qbar = dq(0, 0, 0, 0), irb = qbar, nrb = qbar;
+    double *st = A.state + 16 * (size_t)i;
+    if (live) {
+      DFac F;
+      factor(S, F);
+      solveF(F, S.r, x);
+      // reverse of p+ = p - Δlin, q+ = qadd(q, -Δang)
+      DV P = ldV(st);
+      DQ qb0 = dq(0, 0, 0, 0);
+      DV wb = dv(0, 0, 0);
+      qaddRev(qi, dv(-x[3], -x[4], -x[5]), ldQ(st + 4), qb0, wb);
+      double xb[6] = {-P.x, -P.y, -P.z, -wb.x, -wb.y, -wb.z};
+      solveF(F, xb, rb);
+      if (lane == 0) {
+        DV rl = dv(m * rb[0], m * rb[1], m * rb[2]);
+        pbar = P + rl;
+        npb = -rl;
+        qbar = qb0;
+        qsubRev(qi, dq4(T.inrot[(size_t)s * n + i]), dv(I[0] * rb[3], I[1] * rb[4], I[2] * rb[5]), qbar, nrb);
+      }
+      // pass 2: the rows' adjoints, this lane's points
+      uint32_t run2 = 0;
+      for (uint32_t e = e0; e < e1; e++) {
+        uint32_t mid = adjL[e] - d.nj;
+        uint4 ids = T.mIds[(size_t)s * T.manCap + mid];
+        uint32_t cnt = ids.w & 15u, k0 = (lane + lanes - run2 % lanes) % lanes;
+        run2 += cnt;
+        if (k0 >= cnt) continue;
+        EntryView v;
+        entryView(d, T, s, i, ids, mid, col, Pn, Rn, Po, Ro, p0, q0, v);
+        double sg = v.isA ? 1.0 : -1.0;
+        DV nbP = dv(0, 0, 0), nbIp = nbP;
+        DQ nbQ = dq(0, 0, 0, 0), nbIr = nbQ;
+        for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
+          Contact kc = T.con[(size_t)s * T.conCap + c];
+          kc.pen = pen[c];
+          kc.lam = lam[c];
+          uchar4 bb = bitsIt[c];
+          uint32_t bits = v.isA ? bb.x : bb.y;
+          DV penD = dv3(kc.pen);
+          DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), v.l, penD, dv3(kc.lam), bits, v.mu, alpha, v.PA, v.PB);
+          DV r = v.isA ? ev.rAW : ev.rBW;
+          DV Fb = dv(0, 0, 0), rW = dv(0, 0, 0);
+          for (int row = 0; row < 3; row++) {
+            DV jl = v.l[row] * sg, ja = dcross(r, jl);
+            double J[6] = {jl.x, jl.y, jl.z, ja.x, ja.y, ja.z};
+            double kk = dcomp(penD, row), Jx = 0, rJ = 0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) { Jx += J[q] * x[q]; rJ += rb[q] * J[q]; }
+            if (row == 0) Fb.x = rJ; else if (row == 1) Fb.y = rJ; else Fb.z = rJ;
+            double f = dcomp(ev.F, row);
+            DV jab = dv(f * rb[3] - kk * (Jx * rb[3] + rJ * x[3]), f * rb[4] - kk * (Jx * rb[4] + rJ * x[4]), f * rb[5] - kk * (Jx * rb[5] + rJ * x[5]));
+            rW = rW + dcross(jl, jab);
+          }
+          EvBar o;
+          zeroBar(o);
+          evalRev(ev, kc, v.l, penD, bits, v.mu, alpha, v.PA, v.PB, Fb, v.isA ? rW : dv(0, 0, 0), v.isA ? dv(0, 0, 0) : rW, o);
+          double *lb = A.lamb + 4 * (size_t)c;
+          stV(lb, ldV(lb) + o.lam);
+          double *cb = A.c0b + 4 * (size_t)c;
+          stV(cb, ldV(cb) + o.c0);
+          A.mub[c] += o.mu;
+          if (v.isA) {
+            pbar = pbar + o.pA; qbar = qbar + o.qA; ipb = ipb + o.ipA; irb = irb + o.irA;
+            nbP = nbP + o.pB; nbQ = nbQ + o.qB; nbIp = nbIp + o.ipB; nbIr = nbIr + o.irB;
+          } else {
+            pbar = pbar + o.pB; qbar = qbar + o.qB; ipb = ipb + o.ipB; irb = irb + o.irB;
+            nbP = nbP + o.pA; nbQ = nbQ + o.qA; nbIp = nbIp + o.ipA; nbIr = nbIr + o.irA;
+          }
+        }
+        scatterBody(d, A, v.j, nbP, nbQ, nbIp, nbIr);
+      }
+    }
+    for (uint32_t w = 1; w < lanes; w <<= 1) {
+      pbar = pbar + shxV(pbar, w); ipb = ipb + shxV(ipb, w);
+      qbar = qbar + shxQ(qbar, w); irb = irb + shxQ(irb, w);
+    }
+    if (live && lane == 0) {
+      stV(st, pbar);
+      stQ(st + 4, qbar);
+      double *ip = A.ip + 4 * (size_t)i, *ir = A.ir + 4 * (size_t)i, *np = A.np + 4 * (size_t)i, *nr = A.nr + 4 * (size_t)i;
+      stV(ip, ldV(ip) + ipb);
+      stQ(ir, ldQ(ir) + irb);
+      stV(np, ldV(np) + npb);
+      stQ(nr, ldQ(nr) + nrb);
+    }
   }
 }
 /** Warm start of λ (to the previous step's contacts), c0 and friction of step s in reverse; clears
@@ -595,8 +624,8 @@ __global__ void __launch_bounds__(64) k_adj_contacts(Dev d, Tape T, AdjBuf A, ui
     if (dynamicBody(d, a)) { atomicAddV(A.ip + 4 * (size_t)a, ipA); atomicAddQ(A.ir + 4 * (size_t)a, irA); }
     if (dynamicBody(d, b)) { atomicAddV(A.ip + 4 * (size_t)b, ipB); atomicAddQ(A.ir + 4 * (size_t)b, irB); }
     if (mu > 0) {
-      atomicAdd(A.mu + a, mub * fb / (2 * mu));
-      atomicAdd(A.mu + b, mub * fa / (2 * mu));
+      addD(A.mu + a, mub * fb / (2 * mu));
+      addD(A.mu + b, mub * fa / (2 * mu));
     }
   }
 }

@@ -38,6 +38,7 @@ struct Dev {
   float4 *mGeo, *pmGeo;
   Contact *con, *pcon;
   int *src;  // when recording: per contact, the previous step's contact it warm-started from (-1: none)
+  uchar4 *bits;  // when recording: this iteration's branch bits per contact (x: body a's update, y: b's, z: dual)
   uint32_t manCap, conCap;
   uint32_t *table;
   uint32_t hashMask;
@@ -699,6 +700,10 @@ __device__ __forceinline__ PairBody pairBody(const Dev &d, uint32_t i) {
   return P;
 }
 struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; };
+/** Branch bits of an evaluation: bit0 normal force clamped to 0, bit1 friction on the cone. */
+__device__ __forceinline__ unsigned char evalBits(const Eval &e) {
+  return (unsigned char)((e.F.x < 0.0f ? 0u : 1u) | ((e.frictionScale > e.bounds && e.frictionScale > 0.0f) ? 2u : 0u));
+}
 /** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. */
 __device__ __forceinline__ Eval evalContact(const Dev &d, const Contact &k, const V *basis, float friction, const PairBody &A, uint32_t ia,
                                             const PairBody &B, uint32_t ib, float alpha) {
@@ -758,6 +763,7 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t la
     for (uint32_t c = ids.z; c < cEnd; c++) {
       Contact k = d.con[c];
       Eval ev = evalContact(d, k, basis, geo.w, A, ids.x, B, ids.y, alpha);
+      if (d.bits) { if (isA) d.bits[c].x = evalBits(ev); else d.bits[c].y = evalBits(ev); }
       V r = isA ? ev.rAW : ev.rBW;
       addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
       addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
@@ -896,6 +902,7 @@ __device__ __forceinline__ void dualManifold(const Dev &d, uint32_t m, float alp
   for (uint32_t c = ids.z; c < end; c++) {
     Contact k = d.con[c];
     Eval e = evalContact(d, k, basis, geo.w, A, ids.x, B, ids.y, alpha);
+    if (d.bits) d.bits[c].z = evalBits(e);
     V pen = xyz(k.pen);
     float BL = d.p.betaLin;
     if (e.F.x < 0.0f) pen.x = mn(pen.x + BL * fabsf(e.C.x), PENALTY_MAX);
