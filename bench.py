@@ -268,6 +268,41 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
                             "oracle_vs_fd1e-7": abs(an - row["fd_h1e-07"]) / abs(an),
                             "gpu_vs_fd1e-4": abs(gp - row["fd_h1e-04"]) / abs(row["fd_h1e-04"])})
                 fd_rows.append(row)
+    # the gradient against central differences as the horizon grows (world 0 on its own solver;
+    # differences of the oracle's f64 frozen replay of that run's tape at h = 1e-6, where they
+    # have converged; the six largest velocity components)
+    horizon = []
+    sl = slice(int(ws[0]), int(ws[1]))
+    rec0 = records.bodies_to_oracle(bodies[sl])
+    for hs in (10, 20, 40, 60, steps):
+        one = avbd_amd.GpuSolver3D(bodies[sl], params=params)
+        one.forward(hs, every)
+        gi, _ = one.backward(d_final[sl])
+        one.destroy()
+        m = oracle.Model(32)
+        m.set_params(op)
+        m.set_bodies(rec0)
+        m.record_tape(True)
+        m.step(hs)
+        fz = oracle.Frozen(m, rec0)
+        st0 = oracle.state_from_records(rec0)
+        fz.run(st0)
+        gr, _ = fz.backward(d_final[sl])
+        loss = lambda o: float((o * d_final[sl]).sum())
+        order = np.argsort(-np.abs(gr[:, 8:15]).ravel())
+        picks = [(int(k // 7), 8 + int(k % 7)) for k in order if (k % 7) != 3][:6]
+        fd = []
+        for i, slot in picks:
+            e = np.zeros_like(st0)
+            e[i, slot] = 1e-6
+            fd.append((loss(fz.run(st0 + e)) - loss(fz.run(st0 - e))) / 2e-6)
+        fd = np.array(fd)
+        g_gpu = np.array([gi[i, slot] for i, slot in picks])
+        g_or = np.array([gr[i, slot] for i, slot in picks])
+        horizon.append({"steps": hs, "gpu_vs_fd": float(np.abs(g_gpu - fd).max() / np.abs(fd).max()),
+                        "oracle_f64_vs_fd": float(np.abs(g_or - fd).max() / np.abs(fd).max())})
+        print(f"# horizon {hs}: gpu vs fd {horizon[-1]['gpu_vs_fd']:.2e}, oracle f64 vs fd {horizon[-1]['oracle_f64_vs_fd']:.2e}",
+              file=sys.stderr, flush=True)
     out = {
         "metric": f"adjoint/forward time ({worlds} worlds, {steps} steps)",
         "value": bwd / fwd,
@@ -290,6 +325,7 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
             "record_all": {"forward_ms": 1e3 * timings[0][0], "backward_ms": 1e3 * timings[0][1],
                            "backward_over_forward": timings[0][1] / timings[0][0]},
             "grad_rel_err_vs_oracle_same_states": vs_oracle,
+            "grad_rel_err_vs_fd_by_horizon": horizon,
             "fd_world0": fd_rows,
         },
         "cpu_baseline": {"value": 2 * steps / t_cpu, "unit": "worlds*steps/s (forward + backward)", "cores": 1, "kind": "port",
@@ -348,7 +384,10 @@ def main():
     t = time.perf_counter()
     tune_steps = 2
     lead = max(0, args.warmup - 12 * tune_steps)
-    solver.steps(lead, substeps)
+    for done in range(0, lead, 40):  # in chunks, with a progress line each (profiled runs are slow)
+        solver.steps(min(40, lead - done), substeps)
+        solver.sync()
+        print(f"# lead-in {min(lead, done + 40)}/{lead} steps", file=sys.stderr, flush=True)
     schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
     solver.sync()
     print(f"# lead-in {args.warmup} steps x {substeps} substeps (autotune: {schedule}, {tuned_ms:.2f} ms/step) "
