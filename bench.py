@@ -44,6 +44,19 @@ def solve_bytes(n, m, k, j, iters):
     return iters * (primal + dual) + 368 * n + 304 * j
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3 PMC summary
+    (profiles/<workload>_hbm_traffic.txt: FETCH_SIZE and WRITE_SIZE runs scaled by a copy
+    kernel's reading), or None when that workload has none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def build_workload(name, workers):
     from avbd_amd import scenes
     if name == "large-100k":
@@ -345,6 +358,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--color-cap", type=int, default=32)
     ap.add_argument("--color-cap-adjoint", type=int, default=16)
+    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop"],
+                    help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
     ap.add_argument("--adjoint-steps", type=int, default=120, help="adjoint-256 workload: steps")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -388,7 +403,11 @@ def main():
         solver.steps(min(40, lead - done), substeps)
         solver.sync()
         print(f"# lead-in {min(lead, done + 40)}/{lead} steps", file=sys.stderr, flush=True)
-    schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
+    if args.schedule == "auto":
+        schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
+    else:  # counter-collection runs distort the autotuner's timings: fix the schedule instead
+        solver.set_schedule(cooperative=int(args.schedule == "coop"), threads=512, blocks_per_cu=1)
+        schedule, tuned_ms = solver.schedule, solver.time_steps(12 * tune_steps, substeps) / (12 * tune_steps)
     solver.sync()
     print(f"# lead-in {args.warmup} steps x {substeps} substeps (autotune: {schedule}, {tuned_ms:.2f} ms/step) "
           f"in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
@@ -438,7 +457,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+            "traffic_source": f"profiles/{args.workload}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)",
+            "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
             "launches": ks["solve_launches"] or ks["primal_launches"], "solve_blocks": ks["solve_blocks"],
         },
     }
