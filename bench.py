@@ -358,6 +358,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--color-cap", type=int, default=32)
     ap.add_argument("--color-cap-adjoint", type=int, default=16)
+    ap.add_argument("--primal-lanes", type=int, nargs=3, default=[2 ** 31, 2 ** 15, 2 ** 12],
+                    help="colour sizes from which 1, 2, 4 threads per body are used in the primal (else 8)")
     ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop"],
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
@@ -392,7 +394,8 @@ def main():
               file=sys.stderr, flush=True)
         del src, dst
     params = avbd_amd.default_params(dt=1.0 / (60.0 * substeps), iterations=iters, color_cap=args.color_cap, color_rounds=16,
-                                     flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS)
+                                     flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS,
+                                     primal_lanes=tuple(args.primal_lanes))
     n = len(bodies)
     solver = avbd_amd.GpuSolver3D(bodies, joints, info, params=params, hulls=hulls,
                                   capacity={"pairs": 12 * n, "manifolds": 12 * n, "contacts": 40 * n})

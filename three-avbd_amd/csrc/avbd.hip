@@ -406,13 +406,15 @@ struct avbd_solver {
     for (auto &e : recorded) hipEventCreateWithFlags(&e, hipEventDisableTiming);
     hipEventCreateWithFlags(&reversed, hipEventDisableTiming);
   }
-  float4 *const *streamPtrs(float4 *out[12]) {
-    float4 *list[12] = {b.pos, b.rot, b.ipos, b.irot, b.size, b.moment, b.inpos, b.inrot, b.vel, b.angvel, refPos, refRot};
-    for (int k = 0; k < 12; k++) out[k] = list[k];
-    return out;
+  /** The body state as (pointer, float4s per body): the interleaved pose block, the other six
+   *  streams, the reference poses. */
+  int streamPtrs(float4 *out[9], uint32_t width[9]) {
+    float4 *list[9] = {b.pos.p, b.size, b.moment, b.inpos, b.inrot, b.vel, b.angvel, refPos, refRot};
+    for (int k = 0; k < 9; k++) { out[k] = list[k]; width[k] = k == 0 ? 4 : 1; }
+    return 9;
   }
   void allocCheckpoint(Checkpoint &c) {
-    c.streams = adjAlloc<float4>(12ull * bodyCap);
+    c.streams = adjAlloc<float4>(12ull * bodyCap);  // 4 (pose block) + 8 streams
     c.colA = adjAlloc<uint32_t>(bodyCap);
     c.counters = adjAlloc<uint32_t>(C_WORDS);
     c.mIds = adjAlloc<uint4>(manCap);
@@ -421,9 +423,13 @@ struct avbd_solver {
   }
   /** Everything the next step reads from earlier steps, on the stream. */
   void saveCheckpoint(Checkpoint &c) {
-    float4 *ps[12];
-    streamPtrs(ps);
-    for (int k = 0; k < 12; k++) hipMemcpyAsync(c.streams + (size_t)k * bodyCap, ps[k], 16ull * n, hipMemcpyDeviceToDevice, stream);
+    float4 *ps[9];
+    uint32_t wd[9];
+    size_t off = 0;
+    for (int k = 0, m = streamPtrs(ps, wd); k < m; k++) {
+      hipMemcpyAsync(c.streams + off, ps[k], 16ull * wd[k] * n, hipMemcpyDeviceToDevice, stream);
+      off += (size_t)wd[k] * bodyCap;
+    }
     hipMemcpyAsync(c.colA, colA, 4ull * n, hipMemcpyDeviceToDevice, stream);
     hipMemcpyAsync(c.counters, counters, 4ull * C_WORDS, hipMemcpyDeviceToDevice, stream);
     int last = 1 - parity;  // the buffers the last step wrote: the next step's previous ones
@@ -435,9 +441,13 @@ struct avbd_solver {
   void restoreCheckpoint(const Checkpoint &c) {
     stepCount = c.stepCount; shrinkVotes = c.shrinkVotes; stepsSinceAdapt = c.stepsSinceAdapt;
     parity = c.parity; manifoldsValid = c.manifoldsValid; hp = c.hp;
-    float4 *ps[12];
-    streamPtrs(ps);
-    for (int k = 0; k < 12; k++) hipMemcpyAsync(ps[k], c.streams + (size_t)k * bodyCap, 16ull * n, hipMemcpyDeviceToDevice, stream);
+    float4 *ps[9];
+    uint32_t wd[9];
+    size_t off = 0;
+    for (int k = 0, m = streamPtrs(ps, wd); k < m; k++) {
+      hipMemcpyAsync(ps[k], c.streams + off, 16ull * wd[k] * n, hipMemcpyDeviceToDevice, stream);
+      off += (size_t)wd[k] * bodyCap;
+    }
     hipMemcpyAsync(colA, c.colA, 4ull * n, hipMemcpyDeviceToDevice, stream);
     hipMemcpyAsync(counters, c.counters, 4ull * C_WORDS, hipMemcpyDeviceToDevice, stream);
     int last = 1 - parity;
@@ -497,7 +507,7 @@ struct avbd_solver {
     uint32_t manBlocks = blocksFor(tape.manCap, 64), nb = blocksFor(n, 256);
     for (uint32_t t = len; t-- > 0;) {
       Dev d = dev();
-      d.b.irot = tape.q0 + (size_t)t * n;  // x- of step t: the sphere contacts' rotations in evalContact
+      d.b.irot = Strided4{tape.q0 + (size_t)t * n, 1};  // x- of step t: the sphere contacts' rotations in evalContact
       uint32_t conBlocks = blocksFor(std::max(counts[4 * t + 1], 1u), 64);
       k_adj_velocities<<<nb, 256, 0, st>>>(d, tape, adj, t);
       for (uint32_t it = hp.iterations; it-- > 0;) {
@@ -624,7 +634,9 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->adjCap = 2u * (s->jointCap + s->manCap);
     if (s->tableSize + 1 > MAX_SCAN || 64ull * s->colorGroups + 1 > MAX_SCAN) throw std::runtime_error("body capacity too large for one solver");
     uint32_t N = s->bodyCap;
-    float4 **streams[10] = {&s->b.pos, &s->b.rot, &s->b.ipos, &s->b.irot, &s->b.size, &s->b.moment, &s->b.inpos, &s->b.inrot, &s->b.vel, &s->b.angvel};
+    float4 *pose = s->alloc<float4>(4ull * N);  // pos, rot, x-, q- interleaved: 64 bytes per body
+    s->b.pos = Strided4{pose, 4}; s->b.rot = Strided4{pose + 1, 4}; s->b.ipos = Strided4{pose + 2, 4}; s->b.irot = Strided4{pose + 3, 4};
+    float4 **streams[6] = {&s->b.size, &s->b.moment, &s->b.inpos, &s->b.inrot, &s->b.vel, &s->b.angvel};
     for (auto p : streams) *p = s->alloc<float4>(N);
     s->refPos = s->alloc<float4>(N);
     s->refRot = s->alloc<float4>(N);

@@ -95,12 +95,20 @@ struct Params {
   uint32_t lanes1, lanes2, lanes4;
 };
 
-/** SoA body streams: one float4 array per record vec4 (layout.ts:8-15, 49-58). */
+/** A body stream whose elements are `stride` float4 apart: the four pose streams share one
+ *  64-byte block per body, so a partner's pose is one cache line, not four. */
+struct Strided4 {
+  float4 *p;
+  uint32_t stride;
+  __host__ __device__ __forceinline__ float4 &operator[](uint32_t i) const { return p[i * stride]; }
+};
+/** Body streams, one per record vec4 (layout.ts:8-15, 49-58): the pose four interleaved
+ *  (pos, rot, x-, q- per body), the rest one float4 array each. */
 struct Bodies {
-  float4 *pos;     // xyz, friction
-  float4 *rot;     // quaternion xyzw
-  float4 *ipos;    // x- xyz, lifecycle flags (1 = rewritten slot: no contact history)
-  float4 *irot;
+  Strided4 pos;    // xyz, friction
+  Strided4 rot;    // quaternion xyzw
+  Strided4 ipos;   // x- xyz, lifecycle flags (1 = rewritten slot: no contact history)
+  Strided4 irot;
   float4 *size;    // full widths, mass (0: static)
   float4 *moment;  // principal moments, bounding radius
   float4 *inpos;   // inertial target, step it last moved from its reference pose (u32 bits)

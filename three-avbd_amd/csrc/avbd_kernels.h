@@ -848,6 +848,13 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     return;
   }
   uint32_t per = blockDim.x / lanes;
+  // XCD-aware: workgroup b runs on XCD b mod 8, so give each XCD a contiguous range of the bucket
+  // (bodies in index order, spatially coherent): its L2 then holds the range's neighbours.
+  uint32_t active = (count + per - 1) / per;
+  if (nblocks >= active && block < active) {
+    uint32_t x = block & 7u, q = active >> 3, r = active & 7u;
+    block = x * q + (x < r ? x : r) + (block >> 3);
+  }
   for (uint32_t base = block * per; base < count; base += nblocks * per) {
     uint32_t k = base + slot;
     bool live = k < count;
