@@ -232,6 +232,27 @@ def test_hulls_spheres_boxes_chains_bit_exact(gpu):
     s.destroy()
 
 
+def test_hull_wavefront_narrowphase_matches_inline(gpu, monkeypatch):
+    say("case: hull pairs one wavefront each vs one thread each, 1500-body mixed pile, 15 steps")
+    bodies, joints, info, hw = scenes.large_scene(1500, seed=12, chains=2, links=4, workers=1)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32)
+    outs = []
+    for inline in (False, True):
+        if inline:
+            monkeypatch.setenv("AVBD_HULL_INLINE", "1")
+        s = gpu_for(bodies, kw, joints, info, hulls=hw)
+        for _ in range(15):
+            s.step()
+        man, con = s.read_contacts_raw()
+        outs.append((s.read_bodies().copy(), sorted_contacts(man, con)))
+        s.destroy()
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert len(outs[0][1]) == len(outs[1][1]) > 0
+    for (ma, ca), (mb, cb) in zip(outs[0][1], outs[1][1]):
+        assert (ma[:2] == mb[:2]).all() and (ma[3] & 15) == (mb[3] & 15) and (ma[4:8] == mb[4:8]).all(), (ma, mb)
+        assert (ca == cb).all(), (ma[:2], ca, cb)
+
+
 def test_per_colour_launch_path_matches(gpu, monkeypatch):
     say("case: per-colour launches (no cooperative solve) vs oracle, pile 400 x 10 steps")
     monkeypatch.setenv("AVBD_SOLVE", "colors")

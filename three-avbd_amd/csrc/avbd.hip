@@ -117,7 +117,9 @@ struct avbd_solver {
   uint32_t *cellCount = nullptr, *cellStart = nullptr, *cellCursor = nullptr, *cellSorted = nullptr;
   int4 *cellOf = nullptr;
   float4 *sortedPos = nullptr;
+  int4 *sortedKey = nullptr;
   uint2 *pairs = nullptr;
+  uint2 *hullPairs = nullptr;  // the narrowphase's deferred hull pairs
   uint4 *mIds[2] = {nullptr, nullptr};
   float4 *mGeo[2] = {nullptr, nullptr};
   Contact *con[2] = {nullptr, nullptr};
@@ -230,8 +232,8 @@ struct avbd_solver {
     d.worldOf = worldOf; d.worldStart = worldStart; d.largeStart = largeStart; d.largeRadius = largeRadiusW;
     d.worlds = (uint32_t)worldStartHost.size() - 1;
     d.cellSize = cellSize; d.tableMask = tableSize - 1;
-    d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos;
-    d.pairs = pairs; d.pairCap = pairCap;
+    d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
+    d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
     d.con = con[parity]; d.pcon = con[1 - parity]; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
@@ -328,7 +330,10 @@ struct avbd_solver {
       k_find_pairs<<<blocksFor(n, 64), 64, 0, stream>>>(d);
       if (anyDynamicLarge) k_find_fixed_large<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
-    timed(K_NARROW, [&] { k_narrowphase<<<blocksFor(pairCap, 64), 64, 0, stream>>>(d); });
+    timed(K_NARROW, [&] {
+      k_narrowphase<<<blocksFor(pairCap, 64), 64, 0, stream>>>(d);
+      if (hullPairs) k_narrow_hull<<<blocksFor(64ull * pairCap, 256, 4 * std::max(numCUs, 1u)), 256, 0, stream>>>(d);
+    });
   }
   void topology() {
     Dev d = dev();
@@ -650,7 +655,9 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->cellSorted = s->alloc<uint32_t>(N);
     s->cellOf = s->alloc<int4>(N);
     s->sortedPos = s->alloc<float4>(N);
+    s->sortedKey = s->alloc<int4>(N);
     s->pairs = s->alloc<uint2>(s->pairCap);
+    if (getenv("AVBD_HULL_INLINE") == nullptr) s->hullPairs = s->alloc<uint2>(s->pairCap);
     for (int k = 0; k < 2; k++) {
       s->mIds[k] = s->alloc<uint4>(s->manCap);
       s->mGeo[k] = s->alloc<float4>(s->manCap);

@@ -346,4 +346,189 @@ __device__ __noinline__ void collideHullPair(const uint4 *hulls, const Box &A, Q
   collidePoly(hulls, polyOf(hulls, A.c, qA, A.h, shA), polyOf(hulls, B.c, qB, B.h, shB), faceBias, sat, found);
 }
 
+// ---- one wavefront per pair --------------------------------------------------------------------
+// The same queries with the work items (faces, edge pairs) split over the 64 lanes. Each lane scans
+// its items in increasing order keeping the first strict extremum, and the lanes are combined by
+// (value, index) with the lower index winning ties: the result is the serial loop's, bit for bit.
+struct ArgBest { float v; uint32_t i; };
+__device__ __forceinline__ ArgBest waveArgMax(ArgBest a) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    float ov = __shfl_xor(a.v, m);
+    uint32_t oi = __shfl_xor(a.i, m);
+    if (ov > a.v || (ov == a.v && oi < a.i)) { a.v = ov; a.i = oi; }
+  }
+  return a;
+}
+__device__ __forceinline__ ArgBest waveArgMin(ArgBest a) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    float ov = __shfl_xor(a.v, m);
+    uint32_t oi = __shfl_xor(a.i, m);
+    if (ov < a.v || (ov == a.v && oi < a.i)) { a.v = ov; a.i = oi; }
+  }
+  return a;
+}
+__device__ __forceinline__ V shflV(V v, int src) { return vv(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)); }
+/** queryFaces with lane l taking faces l, l + 64, ... */
+__device__ __forceinline__ void queryFacesWave(const uint4 *hulls, const Poly &A, const Poly &B, uint32_t lane, float &sep, uint32_t &face) {
+  ArgBest best{-3.4e38f, 0xffffffffu};
+  for (uint32_t f = lane; f < A.nf; f += 64) {
+    V n;
+    float d;
+    facePlane(hulls, A, f, n, d);
+    float s = dot(n, polySupport(hulls, B, neg(n))) - d;
+    if (s > best.v) { best.v = s; best.i = f; }
+  }
+  best = waveArgMax(best);
+  sep = best.v;
+  face = best.i == 0xffffffffu ? 0u : best.i;
+}
+/** queryEdges with lane l taking edge pairs k = i·B.ne + j = l, l + 64, ... */
+__device__ __forceinline__ EdgeQuery queryEdgesWave(const uint4 *hulls, const Poly &A, const Poly &B, uint32_t lane) {
+  Q toB = qmul(qconj(B.q), A.q);
+  ArgBest best{-3.4e38f, 0xffffffffu};
+  V bestN = vv(0.0f, 0.0f, 0.0f);
+  uint32_t total = A.ne * B.ne;
+  for (uint32_t k = lane; k < total; k += 64) {
+    uint32_t i = k / B.ne, j = k - i * B.ne;
+    uint4 ea = edgeOf(hulls, A, i);
+    V a = qrotate(toB, faceNormalLocal(hulls, A, ea.z)), b = qrotate(toB, faceNormalLocal(hulls, A, ea.w));
+    uint4 eb = edgeOf(hulls, B, j);
+    if (!minkowskiFace(a, b, neg(faceNormalLocal(hulls, B, eb.z)), neg(faceNormalLocal(hulls, B, eb.w)))) continue;
+    V pa = vert(hulls, A, ea.x), qa = vert(hulls, A, ea.y), pb = vert(hulls, B, eb.x), qb = vert(hulls, B, eb.y);
+    V n = cross(sub(qa, pa), sub(qb, pb));
+    float l = length(n);
+    if (l < 1e-5f * length(sub(qa, pa)) * length(sub(qb, pb))) continue;
+    n = divs(n, l);
+    if (dot(n, sub(pa, A.c)) < 0.0f) n = neg(n);
+    float s = dot(n, sub(pb, pa));
+    if (s > best.v) { best.v = s; best.i = k; bestN = n; }
+  }
+  ArgBest w = waveArgMax(best);
+  EdgeQuery out;
+  out.valid = w.i != 0xffffffffu;
+  out.sep = w.v;
+  out.ea = out.valid ? w.i / B.ne : 0u;
+  out.eb = out.valid ? w.i - out.ea * B.ne : 0u;
+  out.n = out.valid ? shflV(bestN, (int)(w.i & 63u)) : vv(0.0f, 0.0f, 0.0f);
+  if (!out.valid) out.sep = -3.4e38f;
+  return out;
+}
+/** collidePoly with the SAT queries spread over the wavefront; lane 0 builds the contact. */
+__device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &A, const Poly &B, bool faceBias, uint32_t lane, Sat &sat, Found &found) {
+  found.count = 0;
+  float faSep, fbSep;
+  uint32_t faFace, fbFace;
+  queryFacesWave(hulls, A, B, lane, faSep, faFace);
+  if (faSep > 0.0f) return;
+  queryFacesWave(hulls, B, A, lane, fbSep, fbFace);
+  if (fbSep > 0.0f) return;
+  EdgeQuery eq = queryEdgesWave(hulls, A, B, lane);
+  if (eq.valid && eq.sep > 0.0f) return;
+  if (lane != 0) return;
+  float faceSep = mx(faSep, fbSep);
+  bool edgeWins = eq.valid && eq.sep > 0.95f * faceSep + 0.01f;
+  if (!faceBias) edgeWins = eq.valid && 0.95f * eq.sep > faceSep + 0.01f;
+  if (edgeWins) { sat.n = eq.n; hullEdgeContact(hulls, A, B, eq, found); return; }
+  V n;
+  float d;
+  if (fbSep > faSep) {
+    facePlane(hulls, B, fbFace, n, d);
+    sat.n = neg(n);
+    hullFaceContact(hulls, A, B, false, fbFace, found);
+    return;
+  }
+  facePlane(hulls, A, faFace, n, d);
+  sat.n = n;
+  hullFaceContact(hulls, A, B, true, faFace, found);
+}
+/** sphereHull with the faces spread over the wavefront (the result is valid in every lane). */
+__device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, float r, const Poly &X, uint32_t lane) {
+  SphereHit out;
+  out.hit = false;
+  ArgBest deep{-3.4e38f, 0xffffffffu};
+  for (uint32_t f = lane; f < X.nf; f += 64) {
+    V n;
+    float d;
+    facePlane(hulls, X, f, n, d);
+    float s = dot(n, c) - d;
+    if (s > deep.v) { deep.v = s; deep.i = f; }
+  }
+  deep = waveArgMax(deep);
+  float deepest = deep.v;
+  uint32_t deepFace = deep.i == 0xffffffffu ? 0u : deep.i;
+  if (deepest > r) return out;
+  if (deepest <= 0.0f) {
+    V n;
+    float d;
+    facePlane(hulls, X, deepFace, n, d);
+    out.hit = true;
+    out.n = neg(n);
+    out.onBox = sub(c, mul(n, deepest));
+    out.onSphere = sub(c, mul(n, r));
+    return out;
+  }
+  ArgBest near{3.4e38f, 0xffffffffu};
+  V best = vv(0.0f, 0.0f, 0.0f);
+  for (uint32_t f = lane; f < X.nf; f += 64) {
+    V pn;
+    float pd;
+    facePlane(hulls, X, f, pn, pd);
+    float s = dot(pn, c) - pd;
+    if (s <= 0.0f) continue;
+    V p = sub(c, mul(pn, s));
+    uint32_t k = faceVertCount(hulls, X, f);
+    V centre = vv(0.0f, 0.0f, 0.0f);
+    for (uint32_t j = 0; j < k; j++) centre = add(centre, vert(hulls, X, faceVert(hulls, X, f, j)));
+    centre = divs(centre, (float)k);
+    bool inside = true;
+    V edgeBest = vv(0.0f, 0.0f, 0.0f);
+    float edgeDist = 3.4e38f;
+    for (uint32_t j = 0; j < k; j++) {
+      V v0 = vert(hulls, X, faceVert(hulls, X, f, j)), v1 = vert(hulls, X, faceVert(hulls, X, f, (j + 1) % k));
+      V side = cross(sub(v1, v0), pn);
+      if (dot(side, sub(centre, v0)) > 0.0f) side = neg(side);
+      if (dot(side, sub(p, v0)) > 0.0f) inside = false;
+      V q = segmentPoint(v0, v1, c);
+      V dq3 = sub(q, c);
+      float dq = dot(dq3, dq3);
+      if (dq < edgeDist) { edgeDist = dq; edgeBest = q; }
+    }
+    if (!inside) p = edgeBest;
+    float dist = length(sub(p, c));
+    if (dist < near.v) { near.v = dist; near.i = f; best = p; }
+  }
+  ArgBest w = waveArgMin(near);
+  float bestDist = w.v;
+  if (w.i != 0xffffffffu) best = shflV(best, (int)(w.i & 63u));
+  else best = vv(0.0f, 0.0f, 0.0f);
+  if (bestDist > r) return out;
+  V n = vv(0.0f, 0.0f, 1.0f);
+  if (bestDist > 0.0f) n = divs(sub(c, best), bestDist);
+  out.hit = true;
+  out.n = neg(n);
+  out.onBox = best;
+  out.onSphere = sub(c, mul(n, r));
+  return out;
+}
+/** collideHullPair for one wavefront; the contact is valid in lane 0. */
+__device__ __forceinline__ void collideHullPairWave(const uint4 *hulls, const Box &A, Q qA, float shA, const Box &B, Q qB, float shB,
+                                                    bool sphereA, bool sphereB, bool faceBias, uint32_t lane, Sat &sat, Found &found) {
+  found.count = 0;
+  if (sphereA || sphereB) {
+    if (sphereA) {
+      SphereHit h = sphereHullWave(hulls, A.c, A.h.x, polyOf(hulls, B.c, qB, B.h, shB), lane);
+      sat.n = h.n;
+      if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+    } else {
+      SphereHit h = sphereHullWave(hulls, B.c, B.h.x, polyOf(hulls, A.c, qA, A.h, shA), lane);
+      sat.n = neg(h.n);
+      if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+    }
+    return;
+  }
+  collidePolyWave(hulls, polyOf(hulls, A.c, qA, A.h, shA), polyOf(hulls, B.c, qB, B.h, shB), faceBias, lane, sat, found);
+}
+
 }  // namespace avbd

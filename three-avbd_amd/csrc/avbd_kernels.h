@@ -31,6 +31,7 @@ struct Dev {
   uint32_t *cellCount, *cellStart, *cellCursor, *cellSorted;
   int4 *cellOf;
   float4 *sortedPos;  // pos.xyz, bounding radius of cellSorted[k]: the candidate scan's stream
+  int4 *sortedKey;    // cell xyz, world | 0x80000000 when dynamic, of cellSorted[k]
   // pairs, manifolds (current / previous ping-pong), contacts
   uint2 *pairs;
   uint32_t pairCap;
@@ -38,6 +39,7 @@ struct Dev {
   float4 *mGeo, *pmGeo;
   Contact *con, *pcon;
   int *src;  // when recording: per contact, the previous step's contact it warm-started from (-1: none)
+  uint2 *hullPairs;  // pairs with a hull on either side, for the wavefront-per-pair narrowphase (null: inline)
   uchar4 *bits;  // when recording: this iteration's branch bits per contact (x: body a's update, y: b's, z: dual)
   uint32_t manCap, conCap;
   uint32_t *table;
@@ -76,6 +78,7 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   c[C_CONTACTS] = 0;
   if (!reuseManifolds) c[C_PAIRS] = 0;
   c[C_OVERFLOW] = 0;
+  c[C_HULL_PAIRS] = 0;
   c[C_CLASHES] = 0;
   c[C_NUM_COLORS] = 0;
 }
@@ -155,6 +158,8 @@ __global__ void k_grid_scatter(Dev d) {
     d.cellSorted[slot] = i;
     float4 p = d.b.pos[i];
     d.sortedPos[slot] = make_float4(p.x, p.y, p.z, d.b.moment[i].w);
+    int4 c = d.cellOf[i];
+    d.sortedKey[slot] = make_int4(c.x, c.y, c.z, (int)((uint32_t)c.w | (d.b.size[i].w > 0.0f ? 0x80000000u : 0u)));
   }
 }
 
@@ -255,11 +260,11 @@ __global__ void k_find_pairs(Dev d) {
             uint32_t h = cellHash(d, cell);
             uint32_t end = d.cellStart[h + 1];
             for (uint32_t k = d.cellStart[h]; k < end; k++) {
+              int4 key = d.sortedKey[k];  // the cell check and the ordering rule from the sorted stream
+              if (key.x != cell.x || key.y != cell.y || key.z != cell.z || (key.w & 0x7fffffff) != cell.w) continue;
               uint32_t j = d.cellSorted[k];
               if (j == i) continue;
-              if (j > i && d.b.size[j].w > 0.0f) continue;
-              int4 cj = d.cellOf[j];
-              if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z || cj.w != cell.w) continue;
+              if (j > i && key.w < 0) continue;
               testPair(d, P, j, d.sortedPos[k]);
             }
           }
@@ -351,6 +356,64 @@ __device__ __forceinline__ bool reuseContacts(const Dev &d, uint32_t a, uint32_t
   return true;
 }
 
+/** The manifold of pair (a, b) from its contact points (wgsl-collision.ts:685-757): slot, normal
+ *  basis, per point the body-local anchors, warm start from last step's contact with the same
+ *  feature (or the nearest anchor), C0 and the decayed λ/penalty. */
+__device__ __forceinline__ void emitManifold(const Dev &d, uint32_t a, uint32_t b, const Box &A, const Box &B, Q qA, Q qB, float4 posA,
+                                             float4 posB, bool anySphere, const Sat &sat, const Found &found) {
+  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
+  if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
+  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], found.count);
+  uint32_t cnt = found.count;
+  if (base + cnt > d.conCap) { atomicOr(&d.counters[C_OVERFLOW], 2u); cnt = 0; }
+  V n = neg(sat.n);
+  d.mIds[m] = make_uint4(a, b, base, cnt | (d.p.step << 4));
+  d.mGeo[m] = f4(n, sqrtf(posA.w * posB.w));
+  if (cnt == 0) return;
+  V basis[3];
+  orthonormal(n, basis);
+  uint32_t prevFirst = 0, prevCount = 0;
+  int pm = hashFind(d, a, b);
+  if (pm >= 0) { uint4 pids = d.pmIds[pm]; prevFirst = pids.z; prevCount = pids.w & 15u; }
+  bool matchNearest = (d.p.flags & FLAG_MATCH_NEAREST) != 0;
+  float minSide = mn(mn(mn(A.h.x, A.h.y), A.h.z), mn(mn(B.h.x, B.h.y), B.h.z)) * 2.0f;
+  float penaltyMin = contactPenaltyMin(d, a, b);
+  for (uint32_t i = 0; i < cnt; i++) {
+    V rA = qrotate(qconj(qA), sub(found.xA[i], A.c));
+    V rB = qrotate(qconj(qB), sub(found.xB[i], B.c));
+    V pen = vv(0.0f, 0.0f, 0.0f), lam = pen;
+    uint32_t stick = 0;
+    int j = -1;
+    for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+      if ((__float_as_uint(d.pcon[c].rA.w) & ~STICK_BIT) == found.feature[i]) { j = (int)c; break; }
+    if (j < 0 && matchNearest) {
+      float bestDist = 0.05f * minSide;
+      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+        float dist = length(sub(xyz(d.pcon[c].rA), rA));
+        if (dist <= bestDist) { bestDist = dist; j = (int)c; }
+      }
+    }
+    if (j >= 0) {
+      Contact pc = d.pcon[j];
+      pen = xyz(pc.pen); lam = xyz(pc.lam); stick = __float_as_uint(pc.rA.w) & STICK_BIT;
+      if (stick != 0 && !anySphere) { rA = xyz(pc.rA); rB = xyz(pc.rB); }
+    }
+    V dd = sub(add(qrotate(qA, rA), A.c), add(qrotate(qB, rB), B.c));
+    Contact k;
+    k.rA = f4(rA, __uint_as_float(found.feature[i] | stick));
+    k.rB = f4(rB, dot(basis[0], dd) + d.p.margin);
+    lam = mul(mul(lam, d.p.alpha), d.p.gamma);
+    pen = vv(clampf(pen.x * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.y * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.z * d.p.gamma, PENALTY_MIN, PENALTY_MAX));
+    pen.x = mx(pen.x, penaltyMin);
+    k.pen = f4(pen, dot(basis[1], dd));
+    k.lam = f4(lam, dot(basis[2], dd));
+    d.con[base + i] = k;
+    if (d.src) d.src[base + i] = j;
+  }
+}
+
+/** Narrowphase of every broadphase pair, one thread per pair; pairs with a hull on either side
+ *  go to the hull list for k_narrow_hull (a wavefront per pair) when it is given. */
 __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
   uint32_t count = d.counters[C_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
@@ -359,15 +422,19 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
     uint32_t a = pr.x, b = pr.y;
     if ((d.p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(d, a, b)) continue;
     float4 posA = d.b.pos[a], posB = d.b.pos[b];
-    Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
     float shA = d.b.angvel[a].w, shB = d.b.angvel[b].w;
+    bool sphereA = shA == SHAPE_SPHERE, sphereB = shB == SHAPE_SPHERE;
+    bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
+    if ((hullA || hullB) && d.hullPairs) {
+      d.hullPairs[atomicAdd(&d.counters[C_HULL_PAIRS], 1u)] = pr;
+      continue;
+    }
+    Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
     Box A = makeBox(xyz(posA), qA, xyz(d.b.size[a])), B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
     Sat sat;
     sat.n = vv(0.0f, 0.0f, 0.0f);
     Found found;
     found.count = 0;
-    bool sphereA = shA == SHAPE_SPHERE, sphereB = shB == SHAPE_SPHERE;
-    bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
     if (sphereA && sphereB) sphereSphere(A, B, sat, found);
     else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
     else if (sphereA || sphereB) {
@@ -382,57 +449,31 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
       }
     } else collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
     if (found.count == 0) continue;
+    emitManifold(d, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
+  }
+}
 
-    uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
-    if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); continue; }
-    uint32_t base = atomicAdd(&d.counters[C_CONTACTS], found.count);
-    uint32_t cnt = found.count;
-    if (base + cnt > d.conCap) { atomicOr(&d.counters[C_OVERFLOW], 2u); cnt = 0; }
-    V n = neg(sat.n);
-    d.mIds[m] = make_uint4(a, b, base, cnt | (d.p.step << 4));
-    d.mGeo[m] = f4(n, sqrtf(posA.w * posB.w));
-    if (cnt == 0) continue;
-    V basis[3];
-    orthonormal(n, basis);
-    bool anySphere = sphereA || sphereB;
-    uint32_t prevFirst = 0, prevCount = 0;
-    int pm = hashFind(d, a, b);
-    if (pm >= 0) { uint4 pids = d.pmIds[pm]; prevFirst = pids.z; prevCount = pids.w & 15u; }
-    bool matchNearest = (d.p.flags & FLAG_MATCH_NEAREST) != 0;
-    float minSide = mn(mn(mn(A.h.x, A.h.y), A.h.z), mn(mn(B.h.x, B.h.y), B.h.z)) * 2.0f;
-    float penaltyMin = contactPenaltyMin(d, a, b);
-    for (uint32_t i = 0; i < cnt; i++) {
-      V rA = qrotate(qconj(qA), sub(found.xA[i], A.c));
-      V rB = qrotate(qconj(qB), sub(found.xB[i], B.c));
-      V pen = vv(0.0f, 0.0f, 0.0f), lam = pen;
-      uint32_t stick = 0;
-      int j = -1;
-      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
-        if ((__float_as_uint(d.pcon[c].rA.w) & ~STICK_BIT) == found.feature[i]) { j = (int)c; break; }
-      if (j < 0 && matchNearest) {
-        float bestDist = 0.05f * minSide;
-        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
-          float dist = length(sub(xyz(d.pcon[c].rA), rA));
-          if (dist <= bestDist) { bestDist = dist; j = (int)c; }
-        }
-      }
-      if (j >= 0) {
-        Contact pc = d.pcon[j];
-        pen = xyz(pc.pen); lam = xyz(pc.lam); stick = __float_as_uint(pc.rA.w) & STICK_BIT;
-        if (stick != 0 && !anySphere) { rA = xyz(pc.rA); rB = xyz(pc.rB); }
-      }
-      V dd = sub(add(qrotate(qA, rA), A.c), add(qrotate(qB, rB), B.c));
-      Contact k;
-      k.rA = f4(rA, __uint_as_float(found.feature[i] | stick));
-      k.rB = f4(rB, dot(basis[0], dd) + d.p.margin);
-      lam = mul(mul(lam, d.p.alpha), d.p.gamma);
-      pen = vv(clampf(pen.x * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.y * d.p.gamma, PENALTY_MIN, PENALTY_MAX), clampf(pen.z * d.p.gamma, PENALTY_MIN, PENALTY_MAX));
-      pen.x = mx(pen.x, penaltyMin);
-      k.pen = f4(pen, dot(basis[1], dd));
-      k.lam = f4(lam, dot(basis[2], dd));
-      d.con[base + i] = k;
-      if (d.src) d.src[base + i] = j;
-    }
+/** The pairs the narrowphase deferred (a hull on either side), one 64-wide wavefront per pair:
+ *  the SAT's face and edge-pair queries are spread over the lanes, lane 0 clips and emits. */
+__global__ void __launch_bounds__(256) k_narrow_hull(Dev d) {
+  uint32_t count = d.counters[C_HULL_PAIRS];
+  if (count > d.pairCap) count = d.pairCap;
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t p = wave; p < count; p += waves) {
+    uint2 pr = d.hullPairs[p];
+    uint32_t a = pr.x, b = pr.y;
+    float4 posA = d.b.pos[a], posB = d.b.pos[b];
+    Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
+    float shA = d.b.angvel[a].w, shB = d.b.angvel[b].w;
+    Box A = makeBox(xyz(posA), qA, xyz(d.b.size[a])), B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
+    bool sphereA = shA == SHAPE_SPHERE, sphereB = shB == SHAPE_SPHERE;
+    Sat sat;
+    sat.n = vv(0.0f, 0.0f, 0.0f);
+    Found found;
+    found.count = 0;
+    collideHullPairWave(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, lane, sat, found);
+    if (lane == 0 && found.count > 0) emitManifold(d, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
   }
 }
 
