@@ -141,6 +141,9 @@ __device__ __forceinline__ uint32_t clipHull(const V *src, uint32_t n, V *dst, V
   }
   return count;
 }
+/** The clip and point selection of a face contact, given the incident face. */
+__device__ __noinline__ void hullFaceClip(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, const Poly &R,
+                                          const Poly &I, V rn, float pw, uint32_t inc, Found &found);
 __device__ __noinline__ void hullFaceContact(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, Found &found) {
   found.count = 0;
   const Poly &R = refIsA ? A : B;
@@ -164,6 +167,11 @@ __device__ __noinline__ void hullFaceContact(const uint4 *hulls, const Poly &A, 
     float d = dot(fn, rn);
     if (d < least) { least = d; inc = f; }
   }
+  hullFaceClip(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, found);
+}
+__device__ __noinline__ void hullFaceClip(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, const Poly &R,
+                                          const Poly &I, V rn, float pw, uint32_t inc, Found &found) {
+  found.count = 0;
   V poly[64], tmp[64];
   uint32_t n = faceVertCount(hulls, I, inc);
   if (n > MAX_CLIP) n = MAX_CLIP;
@@ -415,33 +423,153 @@ __device__ __forceinline__ EdgeQuery queryEdgesWave(const uint4 *hulls, const Po
   if (!out.valid) out.sep = -3.4e38f;
   return out;
 }
+/** Per-wavefront staging of one pair's hull data in LDS: local vertices of both shapes and, per
+ *  edge, the two face normals of its Gauss-map arc (A's rotated into B's frame, B's negated). */
+static const uint32_t STAGE_V = 64, STAGE_E = 128;
+struct HullStage { float4 vA[STAGE_V], vB[STAGE_V], nA[2 * STAGE_E], nB[2 * STAGE_E]; };
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ bool stageable(const Poly &A, const Poly &B) {
+  return A.nv <= STAGE_V && B.nv <= STAGE_V && A.ne <= STAGE_E && B.ne <= STAGE_E;
+}
+__device__ __forceinline__ void stagePair(const uint4 *hulls, const Poly &A, const Poly &B, uint32_t lane, HullStage &st) {
+  Q toB = qmul(qconj(B.q), A.q);
+  waveSync();  // the previous pair's readers are done
+  for (uint32_t v = lane; v < A.nv; v += 64) st.vA[v] = f4(vertLocal(hulls, A, v), 0.0f);
+  for (uint32_t v = lane; v < B.nv; v += 64) st.vB[v] = f4(vertLocal(hulls, B, v), 0.0f);
+  for (uint32_t e = lane; e < A.ne; e += 64) {
+    uint4 ea = edgeOf(hulls, A, e);
+    st.nA[2 * e] = f4(qrotate(toB, faceNormalLocal(hulls, A, ea.z)), 0.0f);
+    st.nA[2 * e + 1] = f4(qrotate(toB, faceNormalLocal(hulls, A, ea.w)), 0.0f);
+  }
+  for (uint32_t e = lane; e < B.ne; e += 64) {
+    uint4 eb = edgeOf(hulls, B, e);
+    st.nB[2 * e] = f4(neg(faceNormalLocal(hulls, B, eb.z)), 0.0f);
+    st.nB[2 * e + 1] = f4(neg(faceNormalLocal(hulls, B, eb.w)), 0.0f);
+  }
+  waveSync();
+}
+/** polySupport with the shape's local vertices from the stage. */
+__device__ __forceinline__ V polySupportStaged(const Poly &P, const float4 *verts, V dir) {
+  V l = qrotate(qconj(P.q), dir);
+  if (P.hull == NO_HULL) return add(P.c, qrotate(P.q, vv(l.x >= 0.0f ? P.h.x : -P.h.x, l.y >= 0.0f ? P.h.y : -P.h.y, l.z >= 0.0f ? P.h.z : -P.h.z)));
+  float best = -3.4e38f;
+  V at = vv(0.0f, 0.0f, 0.0f);
+  for (uint32_t v = 0; v < P.nv; v++) {
+    V p = xyz(verts[v]);
+    float s = dot(p, l);
+    if (s > best) { best = s; at = p; }
+  }
+  return add(P.c, qrotate(P.q, at));
+}
+__device__ __forceinline__ void queryFacesStaged(const uint4 *hulls, const Poly &A, const Poly &B, const float4 *vertsB, uint32_t lane, float &sep,
+                                                 uint32_t &face) {
+  ArgBest best{-3.4e38f, 0xffffffffu};
+  for (uint32_t f = lane; f < A.nf; f += 64) {
+    V n;
+    float d;
+    facePlane(hulls, A, f, n, d);
+    float s = dot(n, polySupportStaged(B, vertsB, neg(n))) - d;
+    if (s > best.v) { best.v = s; best.i = f; }
+  }
+  best = waveArgMax(best);
+  sep = best.v;
+  face = best.i == 0xffffffffu ? 0u : best.i;
+}
+__device__ __forceinline__ EdgeQuery queryEdgesStaged(const uint4 *hulls, const Poly &A, const Poly &B, const HullStage &st, uint32_t lane) {
+  ArgBest best{-3.4e38f, 0xffffffffu};
+  V bestN = vv(0.0f, 0.0f, 0.0f);
+  uint32_t total = A.ne * B.ne;
+  for (uint32_t k = lane; k < total; k += 64) {
+    uint32_t i = k / B.ne, j = k - i * B.ne;
+    if (!minkowskiFace(xyz(st.nA[2 * i]), xyz(st.nA[2 * i + 1]), xyz(st.nB[2 * j]), xyz(st.nB[2 * j + 1]))) continue;
+    uint4 ea = edgeOf(hulls, A, i), eb = edgeOf(hulls, B, j);
+    V pa = vert(hulls, A, ea.x), qa = vert(hulls, A, ea.y), pb = vert(hulls, B, eb.x), qb = vert(hulls, B, eb.y);
+    V n = cross(sub(qa, pa), sub(qb, pb));
+    float l = length(n);
+    if (l < 1e-5f * length(sub(qa, pa)) * length(sub(qb, pb))) continue;
+    n = divs(n, l);
+    if (dot(n, sub(pa, A.c)) < 0.0f) n = neg(n);
+    float s = dot(n, sub(pb, pa));
+    if (s > best.v) { best.v = s; best.i = k; bestN = n; }
+  }
+  ArgBest w = waveArgMax(best);
+  EdgeQuery out;
+  out.valid = w.i != 0xffffffffu;
+  out.sep = out.valid ? w.v : -3.4e38f;
+  out.ea = out.valid ? w.i / B.ne : 0u;
+  out.eb = out.valid ? w.i - out.ea * B.ne : 0u;
+  out.n = out.valid ? shflV(bestN, (int)(w.i & 63u)) : vv(0.0f, 0.0f, 0.0f);
+  return out;
+}
+/** hullFaceContact with the deepest-vertex and incident-face searches spread over the lanes
+ *  (first minimum, as the serial loops); lane 0 clips and selects. */
+__device__ __forceinline__ void hullFaceContactWave(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, uint32_t lane,
+                                                    Found &found) {
+  found.count = 0;
+  const Poly &R = refIsA ? A : B;
+  const Poly &I = refIsA ? B : A;
+  V rn;
+  float pw;
+  facePlane(hulls, R, refFace, rn, pw);
+  ArgBest low{3.4e38f, 0xffffffffu};
+  for (uint32_t v = lane; v < I.nv; v += 64) { float dd = dot(vert(hulls, I, v), rn); if (dd < low.v) { low.v = dd; low.i = v; } }
+  low = waveArgMin(low);
+  uint32_t deepest = low.i == 0xffffffffu ? 0u : low.i;
+  ArgBest least{3.4e38f, 0xffffffffu};
+  for (uint32_t f = lane; f < I.nf; f += 64) {
+    bool touches = false;
+    uint32_t cnt = faceVertCount(hulls, I, f);
+    for (uint32_t j = 0; j < cnt; j++) if (faceVert(hulls, I, f, j) == deepest) { touches = true; break; }
+    if (!touches) continue;
+    V fn;
+    float fd;
+    facePlane(hulls, I, f, fn, fd);
+    float dd = dot(fn, rn);
+    if (dd < least.v) { least.v = dd; least.i = f; }
+  }
+  least = waveArgMin(least);
+  uint32_t inc = least.i == 0xffffffffu ? 0u : least.i;
+  if (lane != 0) return;
+  hullFaceClip(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, found);
+}
 /** collidePoly with the SAT queries spread over the wavefront; lane 0 builds the contact. */
-__device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &A, const Poly &B, bool faceBias, uint32_t lane, Sat &sat, Found &found) {
+__device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &A, const Poly &B, bool faceBias, uint32_t lane, HullStage *st,
+                                                Sat &sat, Found &found) {
   found.count = 0;
   float faSep, fbSep;
   uint32_t faFace, fbFace;
-  queryFacesWave(hulls, A, B, lane, faSep, faFace);
+  bool staged = st != nullptr && stageable(A, B);
+  if (staged) stagePair(hulls, A, B, lane, *st);
+  if (staged) queryFacesStaged(hulls, A, B, st->vB, lane, faSep, faFace);
+  else queryFacesWave(hulls, A, B, lane, faSep, faFace);
   if (faSep > 0.0f) return;
-  queryFacesWave(hulls, B, A, lane, fbSep, fbFace);
+  if (staged) queryFacesStaged(hulls, B, A, st->vA, lane, fbSep, fbFace);
+  else queryFacesWave(hulls, B, A, lane, fbSep, fbFace);
   if (fbSep > 0.0f) return;
-  EdgeQuery eq = queryEdgesWave(hulls, A, B, lane);
+  EdgeQuery eq = staged ? queryEdgesStaged(hulls, A, B, *st, lane) : queryEdgesWave(hulls, A, B, lane);
   if (eq.valid && eq.sep > 0.0f) return;
-  if (lane != 0) return;
   float faceSep = mx(faSep, fbSep);
   bool edgeWins = eq.valid && eq.sep > 0.95f * faceSep + 0.01f;
   if (!faceBias) edgeWins = eq.valid && 0.95f * eq.sep > faceSep + 0.01f;
-  if (edgeWins) { sat.n = eq.n; hullEdgeContact(hulls, A, B, eq, found); return; }
+  if (edgeWins) {
+    if (lane == 0) { sat.n = eq.n; hullEdgeContact(hulls, A, B, eq, found); }
+    return;
+  }
   V n;
   float d;
   if (fbSep > faSep) {
     facePlane(hulls, B, fbFace, n, d);
     sat.n = neg(n);
-    hullFaceContact(hulls, A, B, false, fbFace, found);
+    hullFaceContactWave(hulls, A, B, false, fbFace, lane, found);
     return;
   }
   facePlane(hulls, A, faFace, n, d);
   sat.n = n;
-  hullFaceContact(hulls, A, B, true, faFace, found);
+  hullFaceContactWave(hulls, A, B, true, faFace, lane, found);
 }
 /** sphereHull with the faces spread over the wavefront (the result is valid in every lane). */
 __device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, float r, const Poly &X, uint32_t lane) {
@@ -514,7 +642,7 @@ __device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, flo
 }
 /** collideHullPair for one wavefront; the contact is valid in lane 0. */
 __device__ __forceinline__ void collideHullPairWave(const uint4 *hulls, const Box &A, Q qA, float shA, const Box &B, Q qB, float shB,
-                                                    bool sphereA, bool sphereB, bool faceBias, uint32_t lane, Sat &sat, Found &found) {
+                                                    bool sphereA, bool sphereB, bool faceBias, uint32_t lane, HullStage *st, Sat &sat, Found &found) {
   found.count = 0;
   if (sphereA || sphereB) {
     if (sphereA) {
@@ -528,7 +656,7 @@ __device__ __forceinline__ void collideHullPairWave(const uint4 *hulls, const Bo
     }
     return;
   }
-  collidePolyWave(hulls, polyOf(hulls, A.c, qA, A.h, shA), polyOf(hulls, B.c, qB, B.h, shB), faceBias, lane, sat, found);
+  collidePolyWave(hulls, polyOf(hulls, A.c, qA, A.h, shA), polyOf(hulls, B.c, qB, B.h, shB), faceBias, lane, st, sat, found);
 }
 
 }  // namespace avbd

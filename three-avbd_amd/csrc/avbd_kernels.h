@@ -460,6 +460,8 @@ __global__ void __launch_bounds__(256) k_narrow_hull(Dev d) {
   if (count > d.pairCap) count = d.pairCap;
   uint32_t lane = threadIdx.x & 63u;
   uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+  __shared__ HullStage stage[4];  // one per wavefront of the 256-thread workgroup
+  HullStage *st = &stage[threadIdx.x >> 6];
   for (uint32_t p = wave; p < count; p += waves) {
     uint2 pr = d.hullPairs[p];
     uint32_t a = pr.x, b = pr.y;
@@ -472,7 +474,7 @@ __global__ void __launch_bounds__(256) k_narrow_hull(Dev d) {
     sat.n = vv(0.0f, 0.0f, 0.0f);
     Found found;
     found.count = 0;
-    collideHullPairWave(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, lane, sat, found);
+    collideHullPairWave(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, lane, st, sat, found);
     if (lane == 0 && found.count > 0) emitManifold(d, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
   }
 }
