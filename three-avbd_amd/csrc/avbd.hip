@@ -229,6 +229,7 @@ struct avbd_solver {
     Dev d;
     d.n = n; d.nj = nj; d.b = b; d.refPos = refPos; d.refRot = refRot; d.filters = filters; d.joints = joints; d.jinfo = jinfo;
     d.statics = statics; d.largeCount = (uint32_t)large.size(); d.noCollideCount = noCollideCount; d.hulls = hulls;
+    d.ncFirst = statics + large.size() + 3ull * noCollideCount;
     d.worldOf = worldOf; d.worldStart = worldStart; d.largeStart = largeStart; d.largeRadius = largeRadiusW;
     d.worlds = (uint32_t)worldStartHost.size() - 1;
     d.cellSize = cellSize; d.tableMask = tableSize - 1;
@@ -304,6 +305,11 @@ struct avbd_solver {
     noCollideCount = (uint32_t)nc.size();
     std::vector<uint32_t> words(large.begin(), large.end());
     for (auto &e : nc) { words.push_back(e.hi); words.push_back(e.lo); words.push_back(e.slot); }
+    // per body, the first no-collide entry whose higher body it is (entries are sorted by hi)
+    for (uint32_t body = 0, k = 0; body <= n; body++) {
+      while (k < nc.size() && nc[k].hi < body) k++;
+      words.push_back(k);
+    }
     if (words.size() > staticsCap) {
       staticsCap = (uint32_t)words.size() * 2 + 16;
       statics = alloc<uint32_t>(staticsCap);

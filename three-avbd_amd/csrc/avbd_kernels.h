@@ -19,6 +19,7 @@ struct Dev {
   int4 *jinfo;     // type, a (-1 world), b, 0
   uint32_t *statics;  // large bodies [largeCount] | no-collide (hi, lo, slot) triples sorted by (hi, lo)
   uint32_t largeCount, noCollideCount;
+  const uint32_t *ncFirst;  // n + 1: body b's no-collide entries (as the higher body) are [ncFirst[b], ncFirst[b + 1])
   // worlds: bodies [worldStart[w], worldStart[w+1]) never touch another world's; large bodies are
   // listed by world (statics[largeStart[w] .. largeStart[w+1]))
   uint32_t *worldOf, *worldStart, *largeStart;
@@ -190,18 +191,15 @@ __device__ __forceinline__ bool faceSeparated(const Shape &A, const Shape &B, V 
   }
   return false;
 }
-/** Binary search of the (hi, lo) no-collide list; an entry holds while its joint does. */
+/** The (hi, lo) entries of the no-collide list (sorted by (hi, lo); body hi's range indexed by
+ *  ncFirst); an entry holds while its joint does. */
 __device__ __forceinline__ bool ignored(const Dev &d, uint32_t hi, uint32_t lo) {
   const uint32_t *nc = d.statics + d.largeCount;
-  uint32_t first = 0, count = d.noCollideCount;
-  while (count > 0) {
-    uint32_t step = count / 2, mid = first + step;
-    uint32_t mh = nc[3 * mid], ml = nc[3 * mid + 1];
-    if (mh < hi || (mh == hi && ml < lo)) { first = mid + 1; count -= step + 1; }
-    else count = step;
-  }
-  for (uint32_t k = first; k < d.noCollideCount; k++) {
-    if (nc[3 * k] != hi || nc[3 * k + 1] != lo) break;
+  uint32_t end = d.ncFirst[hi + 1];
+  for (uint32_t k = d.ncFirst[hi]; k < end; k++) {
+    uint32_t ml = nc[3 * k + 1];
+    if (ml < lo) continue;
+    if (ml > lo) break;
     uint32_t j = nc[3 * k + 2];
     if (j == IGNORE_SLOT) return true;
     if (d.joints[10 * j].w != 0.0f || d.joints[10 * j + 1].w != 0.0f) return true;
@@ -790,12 +788,23 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t la
     V dq = qsub(q4(d.b.rot[i]), q4(d.b.inrot[i]));
     acc.ra[0] = I.x * dq.x; acc.ra[1] = I.y * dq.y; acc.ra[2] = I.z * dq.z;
   }
+  // Software-pipelined over this lane's entries: the next entry's id and manifold record are
+  // loaded while the current one is evaluated (the arithmetic and its order are unchanged).
   uint32_t end = d.adjStart[i + 1];
-  for (uint32_t e = d.adjStart[i] + lane; e < end; e += lanes) {
-    uint32_t id = d.adjList[e];
-    if (id < d.nj) { addJoint(d, acc, id, alpha, i); continue; }
-    uint4 ids = d.mIds[id - d.nj];
-    float4 geo = d.mGeo[id - d.nj];
+  uint32_t e = d.adjStart[i] + lane;
+  uint32_t id = e < end ? d.adjList[e] : 0u;
+  uint4 nIds = make_uint4(0u, 0u, 0u, 0u);
+  float4 nGeo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (e < end && id >= d.nj) { nIds = d.mIds[id - d.nj]; nGeo = d.mGeo[id - d.nj]; }
+  for (; e < end; e += lanes) {
+    uint4 ids = nIds;
+    float4 geo = nGeo;
+    uint32_t cur = id, en = e + lanes;
+    if (en < end) {
+      id = d.adjList[en];
+      if (id >= d.nj) { nIds = d.mIds[id - d.nj]; nGeo = d.mGeo[id - d.nj]; }
+    }
+    if (cur < d.nj) { addJoint(d, acc, cur, alpha, i); continue; }
     bool isA = i == ids.x;
     PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
     V basis[3];
