@@ -125,6 +125,8 @@ struct avbd_solver {
   Contact *con[2] = {nullptr, nullptr};
   uint32_t *table = nullptr;
   uint32_t *adjCount = nullptr, *adjStart = nullptr, *adjFill = nullptr, *adjList = nullptr;
+  uint4 *adjIds = nullptr, *colorItems = nullptr;
+  float4 *adjGeo = nullptr;
   uint32_t *colA = nullptr, *colB = nullptr, *colorStart = nullptr, *colorBodies = nullptr, *colorRank = nullptr, *hist = nullptr;
   uint32_t *fixedDev = nullptr;
   uint32_t *counters = nullptr;
@@ -239,6 +241,7 @@ struct avbd_solver {
     d.con = con[parity]; d.pcon = con[1 - parity]; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
     d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
+    d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
     d.src = rec ? src : nullptr;
@@ -349,6 +352,7 @@ struct avbd_solver {
     scan(adjCount, adjStart, n + 1);
     k_fill_adj<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d);
     k_sort_adj<<<blocksFor(n, 64), 64, 0, stream>>>(d);
+    k_adj_records<<<blocksFor(adjCap, 256), 256, 0, stream>>>(d);
     if (fixedColors) {
       hipMemcpyAsync(colA, fixedDev, n * 4, hipMemcpyDeviceToDevice, stream);
     } else {
@@ -420,9 +424,9 @@ struct avbd_solver {
   /** The body state as (pointer, float4s per body): the interleaved pose block, the other six
    *  streams, the reference poses. */
   int streamPtrs(float4 *out[9], uint32_t width[9]) {
-    float4 *list[9] = {b.pos.p, b.size, b.moment, b.inpos, b.inrot, b.vel, b.angvel, refPos, refRot};
-    for (int k = 0; k < 9; k++) { out[k] = list[k]; width[k] = k == 0 ? 4 : 1; }
-    return 9;
+    float4 *list[6] = {b.pos.p, b.size.p, b.vel, b.angvel, refPos, refRot};
+    for (int k = 0; k < 6; k++) { out[k] = list[k]; width[k] = k < 2 ? 4 : 1; }
+    return 6;
   }
   void allocCheckpoint(Checkpoint &c) {
     c.streams = adjAlloc<float4>(12ull * bodyCap);  // 4 (pose block) + 8 streams
@@ -647,7 +651,10 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     uint32_t N = s->bodyCap;
     float4 *pose = s->alloc<float4>(4ull * N);  // pos, rot, x-, q- interleaved: 64 bytes per body
     s->b.pos = Strided4{pose, 4}; s->b.rot = Strided4{pose + 1, 4}; s->b.ipos = Strided4{pose + 2, 4}; s->b.irot = Strided4{pose + 3, 4};
-    float4 **streams[6] = {&s->b.size, &s->b.moment, &s->b.inpos, &s->b.inrot, &s->b.vel, &s->b.angvel};
+    float4 *solveBlock = s->alloc<float4>(4ull * N);  // size, moment, inertial pos, inertial rot
+    s->b.size = Strided4{solveBlock, 4}; s->b.moment = Strided4{solveBlock + 1, 4};
+    s->b.inpos = Strided4{solveBlock + 2, 4}; s->b.inrot = Strided4{solveBlock + 3, 4};
+    float4 **streams[2] = {&s->b.vel, &s->b.angvel};
     for (auto p : streams) *p = s->alloc<float4>(N);
     s->refPos = s->alloc<float4>(N);
     s->refRot = s->alloc<float4>(N);
@@ -674,10 +681,13 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->adjStart = s->alloc<uint32_t>(N + 1);
     s->adjFill = s->alloc<uint32_t>(N);
     s->adjList = s->alloc<uint32_t>(s->adjCap);
+    s->adjIds = s->alloc<uint4>(s->adjCap);
+    s->adjGeo = s->alloc<float4>(s->adjCap);
     s->colA = s->alloc<uint32_t>(N);
     s->colB = s->alloc<uint32_t>(N);
     s->colorStart = s->alloc<uint32_t>(65);
     s->colorBodies = s->alloc<uint32_t>(N);
+    s->colorItems = s->alloc<uint4>(N);
     s->colorRank = s->alloc<uint32_t>(N);
     s->hist = s->alloc<uint32_t>(64ull * s->colorGroups + 1);
     s->fixedDev = s->alloc<uint32_t>(N);

@@ -109,10 +109,10 @@ struct Bodies {
   Strided4 rot;    // quaternion xyzw
   Strided4 ipos;   // x- xyz, lifecycle flags (1 = rewritten slot: no contact history)
   Strided4 irot;
-  float4 *size;    // full widths, mass (0: static)
-  float4 *moment;  // principal moments, bounding radius
-  float4 *inpos;   // inertial target, step it last moved from its reference pose (u32 bits)
-  float4 *inrot;
+  Strided4 size;    // full widths, mass (0: static)           } a second 64-byte block per body:
+  Strided4 moment;  // principal moments, bounding radius      } what a body's own update reads
+  Strided4 inpos;   // inertial target, step it last moved from its reference pose (u32 bits)
+  Strided4 inrot;
   float4 *vel;     // xyz, previous step's velocity along up
   float4 *angvel;  // xyz, shape code (0 box, 1 sphere, 3 + hull header offset)
 };

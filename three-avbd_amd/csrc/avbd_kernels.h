@@ -48,6 +48,11 @@ struct Dev {
   // adjacency
   uint32_t *adjCount, *adjStart, *adjFill, *adjList;
   uint32_t adjCap;
+  // the solve's flattened views: per adjacency entry its manifold record (ids.x = 0xffffffff marks
+  // a joint, id in ids.y), per colour-bucket slot its body and adjacency range
+  uint4 *adjIds;
+  float4 *adjGeo;
+  uint4 *colorItems;
   // colouring
   uint32_t *colA, *colB, *colorStart, *colorBodies, *colorRank, *hist;
   uint32_t colorGroups;
@@ -519,6 +524,16 @@ __device__ __forceinline__ uint2 adjKey(const Dev &d, uint32_t b, uint32_t id) {
   return make_uint2(other + 2, sec);
 }
 __device__ __forceinline__ bool keyLess(uint2 a, uint2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); }
+/** The manifold record of every (sorted) adjacency entry, for the solve's flattened walk. */
+__global__ void k_adj_records(Dev d) {
+  uint32_t total = d.adjStart[d.n];
+  GRID_STRIDE(e, total) {
+    uint32_t id = d.adjList[e];
+    if (id < d.nj) { d.adjIds[e] = make_uint4(0xffffffffu, id, 0u, 0u); continue; }
+    d.adjIds[e] = d.mIds[id - d.nj];
+    d.adjGeo[e] = d.mGeo[id - d.nj];
+  }
+}
 __global__ void k_sort_adj(Dev d) {
   GRID_STRIDE(b, d.n) {
     uint32_t lo = d.adjStart[b], hi = d.adjStart[b + 1];
@@ -644,6 +659,7 @@ __global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
   uint32_t col = d.colA[b] & 0xffu;
   uint32_t slot = d.hist[col * d.colorGroups + blockIdx.x] + d.colorRank[b];
   d.colorBodies[slot] = b;
+  d.colorItems[slot] = make_uint4(b, d.adjStart[b], d.adjStart[b + 1], 0u);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -774,7 +790,7 @@ __device__ __forceinline__ Eval evalContact(const Dev &d, const Contact &k, cons
   return e;
 }
 /** accumulate (wgsl-solve.ts:436-501): body i's system from adjacency entries lane, lane+lanes, ... */
-__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t lane, uint32_t lanes, float alpha, Acc &acc) {
+__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0, uint32_t end, uint32_t lane, uint32_t lanes, float alpha, Acc &acc) {
   accZero(acc);
   float dt = d.p.dt, dt2 = dt * dt;
   if (lane == 0) {
@@ -788,23 +804,12 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t la
     V dq = qsub(q4(d.b.rot[i]), q4(d.b.inrot[i]));
     acc.ra[0] = I.x * dq.x; acc.ra[1] = I.y * dq.y; acc.ra[2] = I.z * dq.z;
   }
-  // Software-pipelined over this lane's entries: the next entry's id and manifold record are
-  // loaded while the current one is evaluated (the arithmetic and its order are unchanged).
-  uint32_t end = d.adjStart[i + 1];
-  uint32_t e = d.adjStart[i] + lane;
-  uint32_t id = e < end ? d.adjList[e] : 0u;
-  uint4 nIds = make_uint4(0u, 0u, 0u, 0u);
-  float4 nGeo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (e < end && id >= d.nj) { nIds = d.mIds[id - d.nj]; nGeo = d.mGeo[id - d.nj]; }
-  for (; e < end; e += lanes) {
-    uint4 ids = nIds;
-    float4 geo = nGeo;
-    uint32_t cur = id, en = e + lanes;
-    if (en < end) {
-      id = d.adjList[en];
-      if (id >= d.nj) { nIds = d.mIds[id - d.nj]; nGeo = d.mGeo[id - d.nj]; }
-    }
-    if (cur < d.nj) { addJoint(d, acc, cur, alpha, i); continue; }
+  // Entries [e0, end) of body i, this lane's every lanes-th: each carries its manifold record
+  // (k_adj_records), so a contact's poses and points load right after the entry.
+  for (uint32_t e = e0 + lane; e < end; e += lanes) {
+    uint4 ids = d.adjIds[e];
+    if (ids.x == 0xffffffffu) { addJoint(d, acc, ids.y, alpha, i); continue; }
+    float4 geo = d.adjGeo[e];
     bool isA = i == ids.x;
     PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
     V basis[3];
@@ -887,9 +892,10 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     for (uint32_t base = 0; base < count; base += per) {
       uint32_t k = base + slot;
       bool live = threadIdx.x < 64 && k < count;
-      uint32_t i = live ? d.colorBodies[start + k] : 0;
+      uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
+      uint32_t i = item.x;
       Acc acc;
-      if (live) accumulate(d, i, lane, lanes, d.p.alpha, acc);
+      if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc);
       else accZero(acc);
       reduceLanes(acc, lanes);
       __syncthreads();
@@ -910,9 +916,10 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
   for (uint32_t base = block * per; base < count; base += nblocks * per) {
     uint32_t k = base + slot;
     bool live = k < count;
-    uint32_t i = live ? d.colorBodies[start + k] : 0;
+    uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t i = item.x;
     Acc acc;
-    if (live) accumulate(d, i, lane, lanes, d.p.alpha, acc);
+    if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc);
     else accZero(acc);
     reduceLanes(acc, lanes);
     if (live && lane == 0) finishBody(d, i, acc);
