@@ -148,13 +148,16 @@ struct avbd_solver {
   };
   std::vector<Checkpoint> ckpts;   // segment starts, then the rollout's end
   std::vector<void *> adjBufs;     // every adjoint allocation (freed together)
+  std::vector<void *> adjHostBufs; // pinned host buffers of the adjoint
   int *src = nullptr;              // per contact warm-start source while recording
-  struct TapeBuf { Tape t{}; uint32_t slots = 0; std::vector<uint32_t> cap; };  // cap: colours each recorded step swept
+  // cap: colours each recorded step swept; hostStarts/hostCounts: pinned copies of the tape's
+  // colour buckets and counts, filled on the solver stream after a segment is recorded
+  struct TapeBuf { Tape t{}; uint32_t slots = 0; std::vector<uint32_t> cap; uint32_t *hostStarts = nullptr, *hostCounts = nullptr; };
   TapeBuf tapes[2];                // a segment recorded while the previous one is reversed
   TapeBuf *rec = nullptr;          // when set, steps record into slot recSlot
   uint32_t recSlot = 0;
   hipStream_t revStream = nullptr; // the reverse sweep's stream (the re-runs use the solver's)
-  hipEvent_t recorded[2] = {nullptr, nullptr}, reversed = nullptr;
+  hipEvent_t recorded[2] = {nullptr, nullptr}, reversed[2] = {nullptr, nullptr};  // per tape buffer
   uint32_t *remapTable = nullptr;
   AdjBuf adj{};
   uint32_t adjConCap = 0;
@@ -174,7 +177,7 @@ struct avbd_solver {
     freeAdjoint();
     if (revStream) hipStreamDestroy(revStream);
     for (auto e : recorded) if (e) hipEventDestroy(e);
-    if (reversed) hipEventDestroy(reversed);
+    for (auto e : reversed) if (e) hipEventDestroy(e);
     for (auto &d : bufs) hipFree(d.p);
     if (staging) hipFree(staging);
     for (auto &m : timer.marks) { hipEventDestroy(m.second.first); hipEventDestroy(m.second.second); }
@@ -393,7 +396,10 @@ struct avbd_solver {
     if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, 0);
     for (uint32_t it = 0; it < hp.iterations; it++) {
       if (rec) d.bits = rec->t.bits + ((size_t)recSlot * rec->t.iters + it) * rec->t.conCap;
-      for (uint32_t c = 0; c < cap; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
+      for (uint32_t c = 0; c < cap; c++) {
+        if (rec) k_tape_system<<<blocksFor(8ull * n, 64), 64, 0, stream>>>(d, rec->t, recSlot, it, c, 8);
+        timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
+      }
       timed(K_DUAL, [&] { k_dual<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d); });
       if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, it + 1);
     }
@@ -410,7 +416,9 @@ struct avbd_solver {
     if (stream) hipStreamSynchronize(stream);
     if (revStream) hipStreamSynchronize(revStream);
     for (void *p : adjBufs) hipFree(p);
+    for (void *p : adjHostBufs) hipHostFree(p);
     adjBufs.clear();
+    adjHostBufs.clear();
     ckpts.clear();
     src = nullptr; adj = AdjBuf{}; adjConCap = 0; rec = nullptr; remapTable = nullptr;
     for (auto &t : tapes) t = TapeBuf{};
@@ -419,7 +427,7 @@ struct avbd_solver {
     if (revStream) return;
     if (hipStreamCreateWithFlags(&revStream, hipStreamNonBlocking) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
     for (auto &e : recorded) hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    hipEventCreateWithFlags(&reversed, hipEventDisableTiming);
+    for (auto &e : reversed) hipEventCreateWithFlags(&e, hipEventDisableTiming);
   }
   /** The body state as (pointer, float4s per body): the interleaved pose block, the other six
    *  streams, the reference poses. */
@@ -491,9 +499,16 @@ struct avbd_solver {
     t.counts = adjAlloc<uint32_t>(S * 4);
     t.cman = adjAlloc<uint32_t>(S * t.conCap);
     t.bits = adjAlloc<uchar4>(S * hp.iterations * t.conCap);
+    t.sys = adjAlloc<double>(S * hp.iterations * (size_t)n * SYS_WORDS);
     tb.t = t;
     tb.slots = slots;
     tb.cap.assign(slots, 0);
+    void *h[2] = {nullptr, nullptr};
+    if (hipHostMalloc(&h[0], 4 * 65 * S) != hipSuccess || (adjHostBufs.push_back(h[0]), hipHostMalloc(&h[1], 4 * 4 * S)) != hipSuccess)
+      throw std::runtime_error("hipHostMalloc failed (adjoint)");
+    adjHostBufs.push_back(h[1]);
+    tb.hostStarts = (uint32_t *)h[0];
+    tb.hostCounts = (uint32_t *)h[1];
     if (!src) src = adjAlloc<int>(conCap);
   }
   void allocAdjoint(uint32_t cCap) {
@@ -506,29 +521,34 @@ struct avbd_solver {
     a.lambPrev = adjAlloc<double>(4ull * cCap);
     a.c0b = adjAlloc<double>(4ull * cCap);
     a.mub = adjAlloc<double>(cCap);
+    a.dpart = adjAlloc<double>(32ull * cCap);
     adj = a;
     adjConCap = cCap;
     remapTable = adjAlloc<uint32_t>(hashSize);
   }
   void zeroF64(double *p, size_t count, hipStream_t st) { k_fill_f64<<<blocksFor(count, 256), 256, 0, st>>>(p, count, 0.0); }
-  /** Reverse of the recorded slots [0, len) (global steps first + slot). Reads the recorded
-   *  colour buckets and counts back first (one synchronisation) to size each launch. */
+  /** Recorded colour buckets and counts of slots [0, len) into the pinned host copies, on the
+   *  stream after the recording (they size the reverse's launches). */
+  void copySizes(TapeBuf &tb, uint32_t len, hipStream_t st) {
+    hipMemcpyAsync(tb.hostStarts, tb.t.colorStart, 4ull * 65 * len, hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(tb.hostCounts, tb.t.counts, 4ull * 4 * len, hipMemcpyDeviceToHost, st);
+  }
+  /** Reverse of the recorded slots [0, len) (global steps first + slot). The host copies of the
+   *  colour buckets and counts (copySizes) must have arrived: they size each launch. */
   void reverseSlots(const TapeBuf &tb, uint32_t len, uint32_t first, hipStream_t st) {
     const Tape &tape = tb.t;
-    std::vector<uint32_t> starts(65ull * len), counts(4ull * len);
-    if (hipMemcpy(starts.data(), tape.colorStart, 4 * starts.size(), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(counts.data(), tape.counts, 4 * counts.size(), hipMemcpyDeviceToHost) != hipSuccess)
-      throw std::runtime_error("reading the tape's colour buckets failed");
     uint32_t manBlocks = blocksFor(tape.manCap, 64), nb = blocksFor(n, 256);
     for (uint32_t t = len; t-- > 0;) {
       Dev d = dev();
       d.b.irot = Strided4{tape.q0 + (size_t)t * n, 1};  // x- of step t: the sphere contacts' rotations in evalContact
-      uint32_t conBlocks = blocksFor(std::max(counts[4 * t + 1], 1u), 64);
+      const uint32_t *starts = tb.hostStarts + 65 * t;
+      uint32_t conBlocks = blocksFor(std::max(tb.hostCounts[4 * t + 1], 1u), 64);
       k_adj_velocities<<<nb, 256, 0, st>>>(d, tape, adj, t);
       for (uint32_t it = hp.iterations; it-- > 0;) {
         k_adj_dual<<<conBlocks, 64, 0, st>>>(d, tape, adj, t, it);
+        k_adj_dual_gather<<<blocksFor(4ull * n, 256), 256, 0, st>>>(d, tape, adj, t);
         for (uint32_t c = tb.cap[t]; c-- > 0;) {
-          uint32_t size = starts[65 * t + c + 1] - starts[65 * t + c];
+          uint32_t size = starts[c + 1] - starts[c];
           if (size == 0) continue;
           uint32_t lanes = 16;  // about 64k threads per launch: one wave per SIMD of the chip
           while (lanes > 1 && (uint64_t)size * lanes > 65536) lanes >>= 1;
@@ -1134,24 +1154,27 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
     for (int k = 0; k < 4; k++) s->zeroF64(zero[k], sizes[k], R);
     s->zeroF64(a.mu, n, R);
     if (K == 0) {
+      s->copySizes(s->tapes[0], steps, S);
+      if (hipStreamSynchronize(S) != hipSuccess) throw std::runtime_error("reading the tape's colour buckets failed");
       s->reverseSlots(s->tapes[0], steps, 0, R);
     } else {
       uint32_t segments = (steps + K - 1) / K;
       auto runSegment = [&](uint32_t g) {
+        uint32_t len = std::min(K, steps - g * K);
         s->restoreCheckpoint(s->ckpts[g]);
-        s->recordSteps(s->tapes[g % 2], std::min(K, steps - g * K));
+        s->recordSteps(s->tapes[g % 2], len);
+        s->copySizes(s->tapes[g % 2], len, S);
         hipEventRecord(s->recorded[g % 2], S);
       };
+      // Segment g's reverse is enqueued on the reverse stream before segment g-1's re-run is
+      // enqueued on the solver stream (whose host side may block: adaptive colouring reads the
+      // counters), so the two run side by side; the re-run into a tape buffer waits for the
+      // reverse that read it last.
       runSegment(segments - 1);
       for (uint32_t g = segments; g-- > 0;) {
         uint32_t first = g * K, len = std::min(K, steps - first);
         avbd_solver::TapeBuf &tb = s->tapes[g % 2];
         if (hipEventSynchronize(s->recorded[g % 2]) != hipSuccess) throw std::runtime_error("re-running a segment failed");
-        if (g > 0) {
-          // the next segment re-runs into the other buffer once the reverse that read it is done
-          hipStreamWaitEvent(S, s->reversed, 0);
-          runSegment(g - 1);
-        }
         if (g + 1 < segments) {
           // λ adjoint of this segment's last step arrives in the checkpoint's contact order
           const avbd_solver::Checkpoint &c = s->ckpts[g + 1];
@@ -1165,9 +1188,14 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
           s->zeroF64(a.lambPrev, 4ull * s->adjConCap, R);
         }
         s->reverseSlots(tb, len, first, R);
-        hipEventRecord(s->reversed, R);
+        hipEventRecord(s->reversed[g % 2], R);
+        if (g > 0) {
+          // the other buffer was last read by the reverse of segment g+1
+          if (g + 1 < segments) hipStreamWaitEvent(S, s->reversed[(g + 1) % 2], 0);
+          runSegment(g - 1);
+        }
       }
-      hipStreamWaitEvent(S, s->reversed, 0);
+      hipStreamWaitEvent(S, s->reversed[0], 0);
       s->restoreCheckpoint(s->ckpts.back());
     }
     hipMemcpyAsync(d_initial, a.state, 128ull * n, hipMemcpyDeviceToHost, R);

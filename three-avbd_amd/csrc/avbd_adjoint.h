@@ -107,6 +107,8 @@ __device__ __forceinline__ void qrotateRev(DQ q, DV v, DV g, DQ &qb) {
 // ---- tape, adjoint buffers ------------------------------------------------------------------
 /** A segment of recorded steps (slot s = step within the segment). Snapshot k of a step: k = 0
  *  the warm-started state, k = it + 1 after iteration it. */
+/** Per body update: L (15), 1/D (6), the step x (6), one spare. */
+static const uint32_t SYS_WORDS = 28;
 struct Tape {
   uint32_t n, iters, conCap, manCap, adjCap;
   float4 *p0, *q0, *inpos, *inrot, *v0, *w0;  // [s*n + i]: x-, inertial target, velocities at the step's start
@@ -123,6 +125,7 @@ struct Tape {
   uint32_t *counts;                           // [s*4]: manifolds, contacts, adjacency entries
   uint32_t *cman;                             // [s*conCap + c]: the manifold contact c belongs to
   uchar4 *bits;                               // [(s*iters + it)*conCap + c]: the forward's branch bits
+  double *sys;                                // [((s*iters + it)*n + i)*SYS_WORDS]: f64 LDLᵀ factors and step of each update
   __device__ __forceinline__ size_t snap(uint32_t s, uint32_t k) const { return (size_t)s * (iters + 1) + k; }
 };
 /** f64 adjoints. state: 16 per body (pos xyz, prev-up, rot xyzw, vel xyz, 0, angvel xyz, 0): the
@@ -133,6 +136,8 @@ struct AdjBuf {
   double *mu;                      // per body friction
   double *lamb, *lambPrev;         // 4 per contact: this step's λ, the previous step's
   double *c0b, *mub;               // 4 / 1 per contact of this step
+  double *dpart;                   // 32 per contact: the dual's partials for body A, then body B
+                                   // (pose p, pad, q, x- p, pad, x- q), summed per body by the gather
 };
 
 // f64 adds into the adjoint buffers (device memory from hipMalloc, coarse-grained): the hardware
@@ -144,6 +149,9 @@ __device__ __forceinline__ DV ldV(const double *p) { return dv(p[0], p[1], p[2])
 __device__ __forceinline__ DQ ldQ(const double *p) { return dq(p[0], p[1], p[2], p[3]); }
 __device__ __forceinline__ void stV(double *p, DV v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 __device__ __forceinline__ void stQ(double *p, DQ v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w; }
+__device__ __forceinline__ double shx(double v, int m) { return __shfl_xor(v, m); }
+__device__ __forceinline__ DV shxV(DV v, int m) { return dv(shx(v.x, m), shx(v.y, m), shx(v.z, m)); }
+__device__ __forceinline__ DQ shxQ(DQ v, int m) { return dq(shx(v.x, m), shx(v.y, m), shx(v.z, m), shx(v.w, m)); }
 
 // ---- recording ------------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t contactCount(const Dev &d) { uint32_t k = d.counters[C_CONTACTS]; return k < d.conCap ? k : d.conCap; }
@@ -308,7 +316,17 @@ __global__ void k_adj_velocities(Dev d, Tape T, AdjBuf A, uint32_t s) {
     st[3] = 0; for (int k = 8; k < 16; k++) st[k] = 0;
   }
 }
-/** The dual of iteration `it` of step s in reverse, one contact per thread. */
+/** One side's partials of a contact: pose p, q and x- p, q (16 doubles, two pads). */
+__device__ __forceinline__ void stPart(double *o, DV p, DQ q, DV ip, DQ ir) {
+  double2 *v = reinterpret_cast<double2 *>(o);
+  v[0] = make_double2(p.x, p.y); v[1] = make_double2(p.z, 0.0);
+  v[2] = make_double2(q.x, q.y); v[3] = make_double2(q.z, q.w);
+  v[4] = make_double2(ip.x, ip.y); v[5] = make_double2(ip.z, 0.0);
+  v[6] = make_double2(ir.x, ir.y); v[7] = make_double2(ir.z, ir.w);
+}
+/** The dual of iteration `it` of step s in reverse, one contact per thread. Each contact's
+ *  adjoints for its two bodies go to A.dpart; k_adj_dual_gather adds them to the bodies (a gather
+ *  over each body's recorded adjacency instead of float atomics on shared bodies). */
 __global__ void __launch_bounds__(64) k_adj_dual(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it) {
   uint32_t C = T.counts[s * 4 + 1], n = d.n;
   const float4 *P = T.P + T.snap(s, it + 1) * n, *R = T.R + T.snap(s, it + 1) * n;
@@ -340,8 +358,42 @@ __global__ void __launch_bounds__(64) k_adj_dual(Dev d, Tape T, AdjBuf A, uint32
     double *cb = A.c0b + 4 * (size_t)c;
     stV(cb, ldV(cb) + o.c0);
     A.mub[c] += o.mu;
-    scatterBody(d, A, a, o.pA, o.qA, o.ipA, o.irA);
-    scatterBody(d, A, b, o.pB, o.qB, o.ipB, o.irB);
+    stPart(A.dpart + 32 * (size_t)c, o.pA, o.qA, o.ipA, o.irA);
+    stPart(A.dpart + 32 * (size_t)c + 16, o.pB, o.qB, o.ipB, o.irB);
+  }
+}
+/** The dual's partials of step s into the dynamic bodies: 4 lanes per body, lane k taking
+ *  points k, k+4, ... of each of the body's manifolds (the slot's recorded adjacency). */
+__global__ void __launch_bounds__(256) k_adj_dual_gather(Dev d, Tape T, AdjBuf A, uint32_t s) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x, i = g >> 2, lane = g & 3u, n = d.n;
+  if (i >= n || !dynamicBody(d, i)) return;  // the body's 4 lanes leave together
+  const uint32_t *adjS = T.adjStart + (size_t)s * (n + 1), *adjL = T.adjList + (size_t)s * T.adjCap;
+  const uint4 *mIds = T.mIds + (size_t)s * T.manCap;
+  DV p = dv(0, 0, 0), ip = p;
+  DQ q = dq(0, 0, 0, 0), ir = q;
+  for (uint32_t e = adjS[i], e1 = adjS[i + 1]; e < e1; e++) {
+    if (adjL[e] < d.nj) continue;
+    uint4 ids = mIds[adjL[e] - d.nj];
+    uint32_t side = ids.x == i ? 0u : 16u;
+    for (uint32_t k = lane; k < (ids.w & 15u); k += 4) {
+      const double2 *v = reinterpret_cast<const double2 *>(A.dpart + 32 * (size_t)(ids.z + k) + side);
+      double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3], v4 = v[4], v5 = v[5], v6 = v[6], v7 = v[7];
+      p = p + dv(v0.x, v0.y, v1.x);
+      q = q + dq(v2.x, v2.y, v3.x, v3.y);
+      ip = ip + dv(v4.x, v4.y, v5.x);
+      ir = ir + dq(v6.x, v6.y, v7.x, v7.y);
+    }
+  }
+  for (int w = 1; w < 4; w <<= 1) {
+    p = p + shxV(p, w); q = q + shxQ(q, w);
+    ip = ip + shxV(ip, w); ir = ir + shxQ(ir, w);
+  }
+  if (lane == 0) {
+    double *st = A.state + 16 * (size_t)i, *ipp = A.ip + 4 * (size_t)i, *irp = A.ir + 4 * (size_t)i;
+    stV(st, ldV(st) + p);
+    stQ(st + 4, ldQ(st + 4) + q);
+    stV(ipp, ldV(ipp) + ip);
+    stQ(irp, ldQ(irp) + ir);
   }
 }
 
@@ -402,9 +454,85 @@ __device__ __forceinline__ void solveF(const DFac &F, const double *b, double *x
     for (int j = i + 1; j < 6; j++) x[i] -= F.L[lidx(j, i)] * x[j];
   }
 }
-__device__ __forceinline__ double shx(double v, int m) { return __shfl_xor(v, m); }
-__device__ __forceinline__ DV shxV(DV v, int m) { return dv(shx(v.x, m), shx(v.y, m), shx(v.z, m)); }
-__device__ __forceinline__ DQ shxQ(DQ v, int m) { return dq(shx(v.x, m), shx(v.y, m), shx(v.z, m), shx(v.w, m)); }
+/** Recording: the f64 system of every body of colour `col` at the state its f32 update is about
+ *  to read (run just before k_primal of that colour), factored and solved; the reverse sweep then
+ *  only solves the adjoint system with these factors. `lanes` threads per body share its contact
+ *  points; the system is reduced across them with shuffles. */
+__global__ void __launch_bounds__(64) k_tape_system(Dev d, Tape T, uint32_t s, uint32_t it, uint32_t col, uint32_t lanes) {
+  uint32_t n = d.n;
+  uint32_t start = d.colorStart[col], count = d.colorStart[col + 1] - start;
+  double dt = d.p.dt, dt2 = dt * dt, alpha = d.p.alpha;
+  uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes, per = blockDim.x / lanes;
+  for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
+    uint32_t k = base + slot;
+    bool live = k < count;
+    uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t i = item.x;
+    DSys S;
+#pragma unroll
+    for (int e = 0; e < 6; e++) { S.lin[e] = 0; S.ang[e] = 0; S.r[e] = 0; }
+#pragma unroll
+    for (int e = 0; e < 9; e++) S.cr[e] = 0;
+    if (live) {
+      if (lane == 0) {
+        float4 mo = d.b.moment[i];
+        double m = (double)d.b.size[i].w / dt2, I0 = mo.x / dt2, I1 = mo.y / dt2, I2 = mo.z / dt2;
+        S.lin[0] = m; S.lin[2] = m; S.lin[5] = m;
+        S.ang[0] = I0; S.ang[2] = I1; S.ang[5] = I2;
+        DV dx = dv3(d.b.pos[i]) - dv3(d.b.inpos[i]);
+        DV dqv = dqsub(dq4(d.b.rot[i]), dq4(d.b.inrot[i]));
+        S.r[0] = m * dx.x; S.r[1] = m * dx.y; S.r[2] = m * dx.z;
+        S.r[3] = I0 * dqv.x; S.r[4] = I1 * dqv.y; S.r[5] = I2 * dqv.z;
+      }
+      uint32_t run = 0;
+      for (uint32_t e = item.y; e < item.z; e++) {
+        uint4 ids = d.adjIds[e];
+        if (ids.x == 0xffffffffu) continue;  // joints: refused by avbd_forward
+        uint32_t cnt = ids.w & 15u, k0 = (lane + lanes - run % lanes) % lanes;
+        run += cnt;
+        if (k0 >= cnt) continue;
+        float4 geo = d.adjGeo[e];
+        bool isA = ids.x == i;
+        V basis[3];
+        orthonormal(xyz(geo), basis);
+        DV l[3];
+        dorthonormal(dv3(geo), l);
+        double mu = sqrt((double)d.b.pos[ids.x].w * (double)d.b.pos[ids.y].w);
+        PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
+        PoseView PA{dv3(d.b.pos[ids.x]), dq4(d.b.rot[ids.x]), dv3(d.b.ipos[ids.x]), dq4(d.b.irot[ids.x])};
+        PoseView PB{dv3(d.b.pos[ids.y]), dq4(d.b.rot[ids.y]), dv3(d.b.ipos[ids.y]), dq4(d.b.irot[ids.y])};
+        double sg = isA ? 1.0 : -1.0;
+        for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
+          Contact kc = d.con[c];
+          uint32_t bits = evalBits(evalContact(d, kc, basis, geo.w, A, ids.x, B, ids.y, d.p.alpha));
+          DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), l, dv3(kc.pen), dv3(kc.lam), bits, mu, alpha, PA, PB);
+          DV r = isA ? ev.rAW : ev.rBW;
+          for (int row = 0; row < 3; row++) {
+            DV jl = l[row] * sg;
+            addRowD(S, jl, dcross(r, jl), row == 0 ? kc.pen.x : (row == 1 ? kc.pen.y : kc.pen.z), dcomp(ev.F, row));
+          }
+        }
+      }
+    }
+    for (uint32_t w = 1; w < lanes; w <<= 1) {
+#pragma unroll
+      for (int e = 0; e < 6; e++) { S.lin[e] += __shfl_xor(S.lin[e], w); S.ang[e] += __shfl_xor(S.ang[e], w); S.r[e] += __shfl_xor(S.r[e], w); }
+#pragma unroll
+      for (int e = 0; e < 9; e++) S.cr[e] += __shfl_xor(S.cr[e], w);
+    }
+    if (live && lane == 0) {
+      DFac F;
+      factor(S, F);
+      double x[6];
+      solveF(F, S.r, x);
+      double *out = T.sys + (((size_t)s * T.iters + it) * n + i) * SYS_WORDS;
+#pragma unroll
+      for (int q = 0; q < 15; q++) out[q] = F.L[q];
+#pragma unroll
+      for (int q = 0; q < 6; q++) { out[15 + q] = F.iD[q]; out[21 + q] = x[q]; }
+    }
+  }
+}
 /** Inputs of one adjacency entry (a manifold) of body i at the versions its update read. */
 struct EntryView {
   uint4 ids; float4 geo; bool isA; uint32_t j; DV l[3]; double mu; PoseView PA, PB;
@@ -444,14 +572,13 @@ __global__ void __launch_bounds__(64) k_adj_primal(Dev d, Tape T, AdjBuf A, uint
     bool live = k < count;
     uint32_t i = live ? T.colorBodies[(size_t)s * n + start + k] : 0;
     double m = 0, I[3] = {0, 0, 0};
-    DSys S;
-#pragma unroll
-    for (int e = 0; e < 6; e++) { S.lin[e] = 0; S.ang[e] = 0; S.r[e] = 0; }
-#pragma unroll
-    for (int e = 0; e < 9; e++) S.cr[e] = 0;
     DV pi = dv(0, 0, 0);
     DQ qi = dq(0, 0, 0, 1);
     uint32_t e0 = 0, e1 = 0;
+    double x[6] = {0, 0, 0, 0, 0, 0}, rb[6] = {0, 0, 0, 0, 0, 0};
+    DV pbar = dv(0, 0, 0), ipb = pbar, npb = pbar;
+    DQ qbar = dq(0, 0, 0, 0), irb = qbar, nrb = qbar;
+    double *st = A.state + 16 * (size_t)i;
     if (live) {
       float4 mo = d.b.moment[i];
       m = (double)d.b.size[i].w / dt2;
@@ -459,54 +586,13 @@ __global__ void __launch_bounds__(64) k_adj_primal(Dev d, Tape T, AdjBuf A, uint
       pi = dv3(Po[i]);
       qi = dq4(Ro[i]);
       e0 = adjS[i]; e1 = adjS[i + 1];
-      if (lane == 0) {
-        S.lin[0] = m; S.lin[2] = m; S.lin[5] = m;
-        S.ang[0] = I[0]; S.ang[2] = I[1]; S.ang[5] = I[2];
-        DV dx = pi - dv3(T.inpos[(size_t)s * n + i]);
-        DV dqv = dqsub(qi, dq4(T.inrot[(size_t)s * n + i]));
-        S.r[0] = m * dx.x; S.r[1] = m * dx.y; S.r[2] = m * dx.z;
-        S.r[3] = I[0] * dqv.x; S.r[4] = I[1] * dqv.y; S.r[5] = I[2] * dqv.z;
-      }
-      // pass 1: this lane's share of the system at the pre-update state
-      uint32_t run = 0;
-      for (uint32_t e = e0; e < e1; e++) {
-        uint32_t mid = adjL[e] - d.nj;
-        uint4 ids = T.mIds[(size_t)s * T.manCap + mid];
-        uint32_t cnt = ids.w & 15u, k0 = (lane + lanes - run % lanes) % lanes;
-        run += cnt;
-        if (k0 >= cnt) continue;
-        EntryView v;
-        entryView(d, T, s, i, ids, mid, col, Pn, Rn, Po, Ro, p0, q0, v);
-        double sg = v.isA ? 1.0 : -1.0;
-        for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
-          Contact kc = T.con[(size_t)s * T.conCap + c];
-          kc.pen = pen[c];
-          kc.lam = lam[c];
-          uchar4 bb = bitsIt[c];
-          uint32_t bits = v.isA ? bb.x : bb.y;
-          DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), v.l, dv3(kc.pen), dv3(kc.lam), bits, v.mu, alpha, v.PA, v.PB);
-          DV r = v.isA ? ev.rAW : ev.rBW;
-          for (int row = 0; row < 3; row++) {
-            DV jl = v.l[row] * sg;
-            addRowD(S, jl, dcross(r, jl), row == 0 ? kc.pen.x : (row == 1 ? kc.pen.y : kc.pen.z), dcomp(ev.F, row));
-          }
-        }
-      }
-    }
-    for (uint32_t w = 1; w < lanes; w <<= 1) {
-#pragma unroll
-      for (int e = 0; e < 6; e++) { S.lin[e] += shx(S.lin[e], w); S.ang[e] += shx(S.ang[e], w); S.r[e] += shx(S.r[e], w); }
-#pragma unroll
-      for (int e = 0; e < 9; e++) S.cr[e] += shx(S.cr[e], w);
-    }
-    double x[6] = {0, 0, 0, 0, 0, 0}, rb[6] = {0, 0, 0, 0, 0, 0};
-    DV pbar = dv(0, 0, 0), ipb = pbar, npb = pbar;
-    DQ qbar = dq(0, 0, 0, 0), irb = qbar, nrb = qbar;
-    double *st = A.state + 16 * (size_t)i;
-    if (live) {
+      // the system's factors and the step, recorded by the forward (k_tape_system)
+      const double *sys = T.sys + (((size_t)s * T.iters + it) * n + i) * SYS_WORDS;
       DFac F;
-      factor(S, F);
-      solveF(F, S.r, x);
+#pragma unroll
+      for (int q = 0; q < 15; q++) F.L[q] = sys[q];
+#pragma unroll
+      for (int q = 0; q < 6; q++) { F.iD[q] = sys[15 + q]; x[q] = sys[21 + q]; F.D[q] = 0; }
       // reverse of p+ = p - Δlin, q+ = qadd(q, -Δang)
       DV P = ldV(st);
       DQ qb0 = dq(0, 0, 0, 0);
