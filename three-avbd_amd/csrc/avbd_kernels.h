@@ -228,56 +228,104 @@ __device__ __forceinline__ void emitPair(const Dev &d, uint32_t a, uint32_t b) {
   if (slot >= d.pairCap) { atomicOr(&d.counters[C_OVERFLOW], 1u); return; }
   d.pairs[slot] = make_uint2(a, b);
 }
-/** testPair (wgsl-collision.ts:208-234), given body j's position and radius already loaded. */
-__device__ __forceinline__ void testPair(const Dev &d, const Probe &P, uint32_t j, float4 pj) {
-  if (!P.dynamic && d.b.size[j].w <= 0.0f) return;
+/** testPair (wgsl-collision.ts:208-234), given body j's position and radius already loaded:
+ *  whether the pair is a candidate, and the pair (higher index first). */
+__device__ __forceinline__ bool testPair(const Dev &d, const Probe &P, uint32_t j, float4 pj, uint2 &pair) {
+  if (!P.dynamic && d.b.size[j].w <= 0.0f) return false;
   uint2 fj = d.filters[j];
-  if ((P.filter.x & fj.y) == 0u || (fj.x & P.filter.y) == 0u) return;
+  if ((P.filter.x & fj.y) == 0u || (fj.x & P.filter.y) == 0u) return false;
   V dd = sub(P.pos, xyz(pj));
   float r = P.radius + pj.w;
-  if (dot(dd, dd) > r * r) return;
+  if (dot(dd, dd) > r * r) return false;
   Shape B = shapeOf(d, j);
   V hb = aabbHalf(B);
   V lim = add(add(P.aabb, hb), vv(1e-4f, 1e-4f, 1e-4f));
   V ad = vabs(dd);
-  if (ad.x > lim.x || ad.y > lim.y || ad.z > lim.z) return;
-  if (faceSeparated(P.shape, B, dd)) return;
+  if (ad.x > lim.x || ad.y > lim.y || ad.z > lim.z) return false;
+  if (faceSeparated(P.shape, B, dd)) return false;
   uint32_t a = P.index > j ? P.index : j, b = P.index > j ? j : P.index;
-  if (ignored(d, a, b)) return;
-  emitPair(d, a, b);
+  if (ignored(d, a, b)) return false;
+  pair = make_uint2(a, b);
+  return true;
 }
-/** Dynamic bodies probe: small ones the 27 cells around them (a same-cell dynamic pair from
- *  its higher index), then everyone the large list (a dynamic large-large pair once). */
-__global__ void k_find_pairs(Dev d) {
-  GRID_STRIDE(i, d.n) {
-    if (d.b.size[i].w <= 0.0f) continue;
-    bool small = !isLarge(d, i);
-    Probe P = probeOf(d, i);
-    uint32_t w = d.worldOf[i];
-    if (small) {
-      int4 c = d.cellOf[i];
-      for (int oz = -1; oz <= 1; oz++)
-        for (int oy = -1; oy <= 1; oy++)
-          for (int ox = -1; ox <= 1; ox++) {
-            int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
-            uint32_t h = cellHash(d, cell);
-            uint32_t end = d.cellStart[h + 1];
-            for (uint32_t k = d.cellStart[h]; k < end; k++) {
-              int4 key = d.sortedKey[k];  // the cell check and the ordering rule from the sorted stream
-              if (key.x != cell.x || key.y != cell.y || key.z != cell.z || (key.w & 0x7fffffff) != cell.w) continue;
-              uint32_t j = d.cellSorted[k];
-              if (j == i) continue;
-              if (j > i && key.w < 0) continue;
-              testPair(d, P, j, d.sortedPos[k]);
-            }
+/** A workgroup's pairs gather in LDS (an LDS atomic per pair) and go out together, one global
+ *  atomic per flush: a returning global atomic per pair would stall the whole wavefront each time
+ *  one of its lanes finds a pair. A full queue falls back to the global append. */
+struct PairQueue {
+  static const uint32_t CAP = 1024;
+  uint2 item[CAP];
+  uint32_t count, base;
+};
+__device__ __forceinline__ void queuePair(const Dev &d, PairQueue &q, uint2 pair) {
+  uint32_t s = atomicAdd(&q.count, 1u);
+  if (s < PairQueue::CAP) q.item[s] = pair;
+  else emitPair(d, pair.x, pair.y);
+}
+/** Every thread of the workgroup calls this (it synchronises). */
+__device__ __forceinline__ void flushPairs(const Dev &d, PairQueue &q) {
+  __syncthreads();
+  uint32_t cnt = q.count < PairQueue::CAP ? q.count : PairQueue::CAP;
+  if (threadIdx.x == 0 && cnt > 0) q.base = atomicAdd(&d.counters[C_PAIRS], cnt);
+  __syncthreads();
+  if (cnt > 0) {
+    uint32_t base = q.base;
+    for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+      if (base + t < d.pairCap) d.pairs[base + t] = q.item[t];
+      else atomicOr(&d.counters[C_OVERFLOW], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) q.count = 0;
+  __syncthreads();
+}
+/** Body at position k of the grid's cell order (the small bodies as scattered, then the large
+ *  list), when dynamic, probes: a small one the 27 cells around it (a same-cell dynamic pair from
+ *  its higher index), every one the large list (a dynamic large-large pair once). Neighbouring
+ *  threads take neighbouring bodies, so a wavefront's lanes walk the same and adjacent cells. */
+__device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t k, uint32_t smallCount) {
+  uint32_t i;
+  if (k < smallCount) i = d.cellSorted[k];
+  else if (k < d.n && k - smallCount < d.largeCount) i = d.statics[k - smallCount];
+  else return;
+  if (d.b.size[i].w <= 0.0f) return;
+  bool small = !isLarge(d, i);
+  Probe P = probeOf(d, i);
+  uint32_t w = d.worldOf[i];
+  uint2 pair;
+  if (small) {
+    int4 c = d.cellOf[i];
+    for (int oz = -1; oz <= 1; oz++)
+      for (int oy = -1; oy <= 1; oy++)
+        for (int ox = -1; ox <= 1; ox++) {
+          int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
+          uint32_t h = cellHash(d, cell);
+          uint32_t end = d.cellStart[h + 1];
+          for (uint32_t e = d.cellStart[h]; e < end; e++) {
+            int4 key = d.sortedKey[e];  // the cell check and the ordering rule from the sorted stream
+            if (key.x != cell.x || key.y != cell.y || key.z != cell.z || (key.w & 0x7fffffff) != cell.w) continue;
+            uint32_t j = d.cellSorted[e];
+            if (j == i) continue;
+            if (j > i && key.w < 0) continue;
+            if (testPair(d, P, j, d.sortedPos[e], pair)) queuePair(d, q, pair);
           }
-    }
-    for (uint32_t k = d.largeStart[w]; k < d.largeStart[w + 1]; k++) {
-      uint32_t l = d.statics[k];
-      if (l == i || (!small && l < i && d.b.size[l].w > 0.0f)) continue;
-      float4 pl = d.b.pos[l];
-      testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w));
-    }
+        }
+  }
+  for (uint32_t e = d.largeStart[w]; e < d.largeStart[w + 1]; e++) {
+    uint32_t l = d.statics[e];
+    if (l == i || (!small && l < i && d.b.size[l].w > 0.0f)) continue;
+    float4 pl = d.b.pos[l];
+    if (testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w), pair)) queuePair(d, q, pair);
+  }
+}
+__global__ void k_find_pairs(Dev d) {
+  __shared__ PairQueue q;
+  if (threadIdx.x == 0) q.count = 0;
+  __syncthreads();
+  uint32_t smallCount = d.cellStart[d.tableMask + 1];
+  // the workgroup walks its bodies together so that it can flush between passes
+  for (uint32_t base = blockIdx.x * blockDim.x; base < d.n; base += gridDim.x * blockDim.x) {
+    findPairsOf(d, q, base + threadIdx.x, smallCount);
+    flushPairs(d, q);
   }
 }
 /** Small fixed bodies against the large dynamic ones (findFixedLarge, wgsl-collision.ts:285). */
@@ -292,7 +340,8 @@ __global__ void k_find_fixed_large(Dev d) {
       if (d.b.size[l].w <= 0.0f) continue;
       if (!loaded) { P = probeOf(d, i); loaded = true; }
       float4 pl = d.b.pos[l];
-      testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w));
+      uint2 pair;
+      if (testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w), pair)) emitPair(d, pair.x, pair.y);
     }
   }
 }
