@@ -146,13 +146,15 @@ struct avbd_solver {
     bool manifoldsValid = false;
     avbd_params hp{};
   };
-  std::vector<Checkpoint> ckpts;   // segment starts, then the rollout's end
-  std::vector<void *> adjBufs;     // every adjoint allocation (freed together)
-  std::vector<void *> adjHostBufs; // pinned host buffers of the adjoint
+  /** Device and pinned host allocations of one part of the adjoint's state (freed together). */
+  struct Mem { std::vector<void *> dev, host; };
+  std::vector<Checkpoint> ckpts;   // segment starts, then the rollout's end (the first ckptCount)
+  uint32_t ckptCount = 0;
+  Mem ckptMem, adjMem;             // checkpoints + per-contact sources + remap table; AdjBuf
   int *src = nullptr;              // per contact warm-start source while recording
   // cap: colours each recorded step swept; hostStarts/hostCounts: pinned copies of the tape's
   // colour buckets and counts, filled on the solver stream after a segment is recorded
-  struct TapeBuf { Tape t{}; uint32_t slots = 0; std::vector<uint32_t> cap; uint32_t *hostStarts = nullptr, *hostCounts = nullptr; };
+  struct TapeBuf { Tape t{}; uint32_t slots = 0; std::vector<uint32_t> cap; uint32_t *hostStarts = nullptr, *hostCounts = nullptr; Mem mem; };
   TapeBuf tapes[2];                // a segment recorded while the previous one is reversed
   TapeBuf *rec = nullptr;          // when set, steps record into slot recSlot
   uint32_t recSlot = 0;
@@ -163,6 +165,7 @@ struct avbd_solver {
   uint32_t adjConCap = 0;
   uint32_t fwdSteps = 0, fwdEvery = 0, fwdEndStep = 0, fwdMaxContacts = 0, fwdMaxManifolds = 0;
   bool fwdValid = false;
+  bool fwdTail = false;            // the rollout's last segment is already on tape (recorded by the forward)
   std::string fwdWhy = "no forward rollout recorded";
 
   template <class T> T *alloc(size_t count) {
@@ -406,22 +409,34 @@ struct avbd_solver {
     k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
   }
   // ---- adjoint (avbd_forward / avbd_backward) ----
-  template <class T> T *adjAlloc(size_t count) {
+  template <class T> T *adjAlloc(Mem &m, size_t count) {
     void *p = nullptr;
     if (hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256)) != hipSuccess) throw std::runtime_error("hipMalloc failed (adjoint)");
-    adjBufs.push_back(p);
+    m.dev.push_back(p);
     return (T *)p;
   }
+  template <class T> T *hostAlloc(Mem &m, size_t count) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, std::max<size_t>(count * sizeof(T), 64)) != hipSuccess) throw std::runtime_error("hipHostMalloc failed (adjoint)");
+    m.host.push_back(p);
+    return (T *)p;
+  }
+  void freeMem(Mem &m) {
+    for (void *p : m.dev) hipFree(p);
+    for (void *p : m.host) hipHostFree(p);
+    m = Mem{};
+  }
+  /** Every adjoint buffer (the solver's end, or an error). Between rollouts the buffers stay and
+   *  are reused: a part is reallocated only when a rollout needs more than it holds. */
   void freeAdjoint() {
     if (stream) hipStreamSynchronize(stream);
     if (revStream) hipStreamSynchronize(revStream);
-    for (void *p : adjBufs) hipFree(p);
-    for (void *p : adjHostBufs) hipHostFree(p);
-    adjBufs.clear();
-    adjHostBufs.clear();
+    freeMem(ckptMem);
+    freeMem(adjMem);
+    for (auto &t : tapes) { freeMem(t.mem); t = TapeBuf{}; }
     ckpts.clear();
+    ckptCount = 0;
     src = nullptr; adj = AdjBuf{}; adjConCap = 0; rec = nullptr; remapTable = nullptr;
-    for (auto &t : tapes) t = TapeBuf{};
   }
   void ensureReverseStream() {
     if (revStream) return;
@@ -437,12 +452,22 @@ struct avbd_solver {
     return 6;
   }
   void allocCheckpoint(Checkpoint &c) {
-    c.streams = adjAlloc<float4>(12ull * bodyCap);  // 4 (pose block) + 8 streams
-    c.colA = adjAlloc<uint32_t>(bodyCap);
-    c.counters = adjAlloc<uint32_t>(C_WORDS);
-    c.mIds = adjAlloc<uint4>(manCap);
-    c.mGeo = adjAlloc<float4>(manCap);
-    c.con = adjAlloc<Contact>(conCap);
+    c.streams = adjAlloc<float4>(ckptMem, 12ull * bodyCap);  // 4 (pose block) + 8 streams
+    c.colA = adjAlloc<uint32_t>(ckptMem, bodyCap);
+    c.counters = adjAlloc<uint32_t>(ckptMem, C_WORDS);
+    c.mIds = adjAlloc<uint4>(ckptMem, manCap);
+    c.mGeo = adjAlloc<float4>(ckptMem, manCap);
+    c.con = adjAlloc<Contact>(ckptMem, conCap);
+  }
+  /** count checkpoints (their sizes are the solver's capacities: kept across rollouts). */
+  void ensureCheckpoints(uint32_t count) {
+    while (ckpts.size() < count) {
+      ckpts.emplace_back();
+      allocCheckpoint(ckpts.back());
+    }
+    ckptCount = count;
+    if (!src) src = adjAlloc<int>(ckptMem, conCap);
+    if (!remapTable) remapTable = adjAlloc<uint32_t>(ckptMem, hashSize);
   }
   /** Everything the next step reads from earlier steps, on the stream. */
   void saveCheckpoint(Checkpoint &c) {
@@ -477,54 +502,58 @@ struct avbd_solver {
     k_copy_manifolds<<<blocksFor(manCap, 256), 256, 0, stream>>>(c.mIds, c.mGeo, mIds[last], mGeo[last], c.counters, C_MANIFOLDS, manCap);
     k_copy_contacts<<<blocksFor(conCap, 256), 256, 0, stream>>>(c.con, con[last], c.counters, C_CONTACTS, conCap);
   }
-  void allocTape(TapeBuf &tb, uint32_t slots, uint32_t cCap, uint32_t mCap) {
+  /** A tape of at least `slots` steps with room for cCap contacts and mCap manifolds per step
+   *  (the one held is kept when it is large enough). */
+  void ensureTape(TapeBuf &tb, uint32_t slots, uint32_t cCap, uint32_t mCap) {
+    cCap = std::max(cCap, 1u); mCap = std::max(mCap, 1u);
+    if (tb.slots >= slots && tb.t.conCap >= cCap && tb.t.manCap >= mCap && tb.t.n == n && tb.t.iters == hp.iterations) return;
+    freeMem(tb.mem);
+    Mem &m = tb.mem;
     Tape t{};
-    t.n = n; t.iters = hp.iterations; t.conCap = std::max(cCap, 1u); t.manCap = std::max(mCap, 1u); t.adjCap = 2 * t.manCap + 2 * nj;
+    t.n = n; t.iters = hp.iterations; t.conCap = cCap; t.manCap = mCap; t.adjCap = 2 * t.manCap + 2 * nj;
     size_t S = slots, K = hp.iterations + 1;
     float4 **bodyArrays[6] = {&t.p0, &t.q0, &t.inpos, &t.inrot, &t.v0, &t.w0};
-    for (auto a : bodyArrays) *a = adjAlloc<float4>(S * n);
-    t.P = adjAlloc<float4>(S * K * n);
-    t.R = adjAlloc<float4>(S * K * n);
-    t.con = adjAlloc<Contact>(S * t.conCap);
-    t.src = adjAlloc<int>(S * t.conCap);
-    t.lam = adjAlloc<float4>(S * K * t.conCap);
-    t.pen = adjAlloc<float4>(S * K * t.conCap);
-    t.mIds = adjAlloc<uint4>(S * t.manCap);
-    t.mGeo = adjAlloc<float4>(S * t.manCap);
-    t.adjStart = adjAlloc<uint32_t>(S * (n + 1));
-    t.adjList = adjAlloc<uint32_t>(S * t.adjCap);
-    t.colorStart = adjAlloc<uint32_t>(S * 65);
-    t.colorBodies = adjAlloc<uint32_t>(S * n);
-    t.col = adjAlloc<uint32_t>(S * n);
-    t.counts = adjAlloc<uint32_t>(S * 4);
-    t.cman = adjAlloc<uint32_t>(S * t.conCap);
-    t.bits = adjAlloc<uchar4>(S * hp.iterations * t.conCap);
-    t.sys = adjAlloc<double>(S * hp.iterations * (size_t)n * SYS_WORDS);
+    for (auto a : bodyArrays) *a = adjAlloc<float4>(m, S * n);
+    t.P = adjAlloc<float4>(m, S * K * n);
+    t.R = adjAlloc<float4>(m, S * K * n);
+    t.con = adjAlloc<Contact>(m, S * t.conCap);
+    t.src = adjAlloc<int>(m, S * t.conCap);
+    t.lam = adjAlloc<float4>(m, S * K * t.conCap);
+    t.pen = adjAlloc<float4>(m, S * K * t.conCap);
+    t.mIds = adjAlloc<uint4>(m, S * t.manCap);
+    t.mGeo = adjAlloc<float4>(m, S * t.manCap);
+    t.adjStart = adjAlloc<uint32_t>(m, S * (n + 1));
+    t.adjList = adjAlloc<uint32_t>(m, S * t.adjCap);
+    t.colorStart = adjAlloc<uint32_t>(m, S * 65);
+    t.colorBodies = adjAlloc<uint32_t>(m, S * n);
+    t.col = adjAlloc<uint32_t>(m, S * n);
+    t.counts = adjAlloc<uint32_t>(m, S * 4);
+    t.cman = adjAlloc<uint32_t>(m, S * t.conCap);
+    t.bits = adjAlloc<uchar4>(m, S * hp.iterations * t.conCap);
+    t.sys = adjAlloc<double>(m, S * hp.iterations * (size_t)n * SYS_WORDS);
     tb.t = t;
     tb.slots = slots;
     tb.cap.assign(slots, 0);
-    void *h[2] = {nullptr, nullptr};
-    if (hipHostMalloc(&h[0], 4 * 65 * S) != hipSuccess || (adjHostBufs.push_back(h[0]), hipHostMalloc(&h[1], 4 * 4 * S)) != hipSuccess)
-      throw std::runtime_error("hipHostMalloc failed (adjoint)");
-    adjHostBufs.push_back(h[1]);
-    tb.hostStarts = (uint32_t *)h[0];
-    tb.hostCounts = (uint32_t *)h[1];
-    if (!src) src = adjAlloc<int>(conCap);
+    tb.hostStarts = hostAlloc<uint32_t>(m, 65 * S);
+    tb.hostCounts = hostAlloc<uint32_t>(m, 4 * S);
   }
-  void allocAdjoint(uint32_t cCap) {
+  /** The reverse sweep's accumulators for up to cCap contacts per step (kept when large enough). */
+  void ensureAdjoint(uint32_t cCap) {
+    cCap = std::max(cCap, 1u);
+    if (adj.state && adjConCap >= cCap) return;
+    freeMem(adjMem);
     AdjBuf a{};
-    a.state = adjAlloc<double>(16ull * n);
+    a.state = adjAlloc<double>(adjMem, 16ull * n);
     double **bodies4[5] = {&a.ip, &a.ir, &a.np, &a.nr, &a.vs};
-    for (auto p : bodies4) *p = adjAlloc<double>(4ull * n);
-    a.mu = adjAlloc<double>(n);
-    a.lamb = adjAlloc<double>(4ull * cCap);
-    a.lambPrev = adjAlloc<double>(4ull * cCap);
-    a.c0b = adjAlloc<double>(4ull * cCap);
-    a.mub = adjAlloc<double>(cCap);
-    a.dpart = adjAlloc<double>(32ull * cCap);
+    for (auto p : bodies4) *p = adjAlloc<double>(adjMem, 4ull * n);
+    a.mu = adjAlloc<double>(adjMem, n);
+    a.lamb = adjAlloc<double>(adjMem, 4ull * cCap);
+    a.lambPrev = adjAlloc<double>(adjMem, 4ull * cCap);
+    a.c0b = adjAlloc<double>(adjMem, 4ull * cCap);
+    a.mub = adjAlloc<double>(adjMem, cCap);
+    a.dpart = adjAlloc<double>(adjMem, 32ull * cCap);
     adj = a;
     adjConCap = cCap;
-    remapTable = adjAlloc<uint32_t>(hashSize);
   }
   void zeroF64(double *p, size_t count, hipStream_t st) { k_fill_f64<<<blocksFor(count, 256), 256, 0, st>>>(p, count, 0.0); }
   /** Recorded colour buckets and counts of slots [0, len) into the pinned host copies, on the
@@ -1095,23 +1124,31 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
   if (steps == 0) return fail(s, AVBD_ERR_INVALID, "steps must be > 0");
   try {
     s->fwdValid = false;
+    s->fwdTail = false;
     s->fwdWhy = "forward rollout incomplete";
-    s->freeAdjoint();
     uint32_t K = checkpoint_every;
     uint32_t segments = K ? (steps + K - 1) / K : 0;
-    s->ckpts.resize(segments + 1);
-    for (auto &c : s->ckpts) s->allocCheckpoint(c);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error before the forward");
+    s->ensureCheckpoints(segments + 1);
     k_fill<<<1, 64, 0, s->stream>>>(s->counters + C_STICKY, 3, 0u);
     if (K == 0) {
-      s->allocTape(s->tapes[0], steps, s->conCap, s->manCap);
+      s->ensureTape(s->tapes[0], steps, s->conCap, s->manCap);
       s->recordSteps(s->tapes[0], steps);
     } else {
-      for (uint32_t t = 0; t < steps; t++) {
+      // the last segment records as it runs: re-running it in the backward would overlap nothing
+      uint32_t tail = (segments - 1) * K;
+      avbd_solver::TapeBuf &tb = s->tapes[(segments - 1) % 2];
+      s->ensureTape(tb, K, s->conCap, s->manCap);
+      for (uint32_t t = 0; t < tail; t++) {
         if (t % K == 0) s->saveCheckpoint(s->ckpts[t / K]);
         s->step(false);
       }
+      s->saveCheckpoint(s->ckpts[segments - 1]);
+      s->recordSteps(tb, steps - tail);
+      s->copySizes(tb, steps - tail, s->stream);
+      s->fwdTail = true;
     }
-    s->saveCheckpoint(s->ckpts.back());
+    s->saveCheckpoint(s->ckpts[s->ckptCount - 1]);
     uint32_t c[C_WORDS];
     hipMemcpyAsync(c, s->counters, sizeof(c), hipMemcpyDeviceToHost, s->stream);
     if (int rc = avbd_sync(s)) return rc;
@@ -1143,9 +1180,10 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
     uint32_t K = s->fwdEvery, steps = s->fwdSteps, n = s->n;
     s->ensureReverseStream();
     hipStream_t R = s->revStream, S = s->stream;
-    if (!s->adj.state) s->allocAdjoint(K ? std::max(s->fwdMaxContacts, 1u) : s->conCap);
-    for (int k = 0; K && k < 2; k++)
-      if (s->tapes[k].slots < K) s->allocTape(s->tapes[k], K, std::max(s->fwdMaxContacts, 1u), std::max(s->fwdMaxManifolds, 1u));
+    s->ensureAdjoint(K ? s->fwdMaxContacts : s->conCap);
+    uint32_t tailBuf = K ? ((steps + K - 1) / K - 1) % 2 : 0;
+    for (uint32_t k = 0; K && k < 2; k++)
+      if (!(s->fwdTail && k == tailBuf)) s->ensureTape(s->tapes[k], K, s->fwdMaxContacts, s->fwdMaxManifolds);
     AdjBuf &a = s->adj;
     if (hipStreamSynchronize(S) != hipSuccess) throw std::runtime_error("device error before the backward");
     hipMemcpyAsync(a.state, d_final, 128ull * n, hipMemcpyHostToDevice, R);
@@ -1170,7 +1208,9 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
       // enqueued on the solver stream (whose host side may block: adaptive colouring reads the
       // counters), so the two run side by side; the re-run into a tape buffer waits for the
       // reverse that read it last.
-      runSegment(segments - 1);
+      if (s->fwdTail) hipEventRecord(s->recorded[(segments - 1) % 2], S);  // recorded by the forward
+      else runSegment(segments - 1);
+      s->fwdTail = false;  // the sweep below overwrites that buffer: a second backward re-runs the segment
       for (uint32_t g = segments; g-- > 0;) {
         uint32_t first = g * K, len = std::min(K, steps - first);
         avbd_solver::TapeBuf &tb = s->tapes[g % 2];
@@ -1196,7 +1236,7 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
         }
       }
       hipStreamWaitEvent(S, s->reversed[0], 0);
-      s->restoreCheckpoint(s->ckpts.back());
+      s->restoreCheckpoint(s->ckpts[s->ckptCount - 1]);
     }
     hipMemcpyAsync(d_initial, a.state, 128ull * n, hipMemcpyDeviceToHost, R);
     if (d_friction) hipMemcpyAsync(d_friction, a.mu, 8ull * n, hipMemcpyDeviceToHost, R);
