@@ -200,6 +200,20 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
         solver.set_schedule(**schedule)
         return solver
 
+    if args.adjoint_pair_only:
+        # for kernel-trace runs: two identical pairs (the first loads the kernels and allocates)
+        for rep in range(2):
+            s = fresh()
+            s.sync()
+            t0 = time.perf_counter()
+            s.forward(steps, every)
+            t1 = time.perf_counter()
+            s.backward(d_final)
+            t2 = time.perf_counter()
+            s.destroy()
+            print(f"# pair {rep}: forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t2 - t1):.1f} ms ({(t2 - t1) / (t1 - t0):.2f}x)",
+                  file=sys.stderr, flush=True)
+        return
     s = fresh()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -372,6 +386,8 @@ def main():
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
     ap.add_argument("--adjoint-steps", type=int, default=120, help="adjoint-256 workload: steps")
+    ap.add_argument("--adjoint-pair-only", action="store_true",
+                    help="adjoint-256 workload: only two forward/backward pairs, no checks and no JSON line (for kernel traces)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration kernel)")
     args = ap.parse_args()
