@@ -8,10 +8,11 @@ exactly `--steps` steps (each `substeps` substeps: the first detects contacts, t
 manifolds) are timed between device synchronisations, bodies resident in HBM, no readback.
 
 Also printed on the JSON line:
-- roofline: the dominant kernel (the cooperative solve kernel: warm starts, iterations x
-  (per-colour primal, dual), velocities) -- algorithmic bytes per launch (the unique-touch model
-  below, from the counters read back after the timed run) / its average launch duration, timed
-  with hipEvents on the solver's stream, against the MI355X HBM peak of 8 TB/s;
+- roofline: the dominant kernel (k_primal, one colour of the primal sweep, under the per-colour
+  schedule the autotuner picks; the cooperative solve kernel when it picks that one) --
+  algorithmic bytes per launch (the unique-touch model below, from the counters read back after
+  the timed run) / its average launch duration, timed with hipEvents on the solver's stream,
+  against the MI355X HBM peak of 8 TB/s; traffic from the committed rocprofv3 PMC summary;
 - cpu_baseline: the oracle's f32 restatement of the same step (single thread), seeded with the
   GPU's state after the timed run and stepped for a bounded sample of substeps on the host.
 """
