@@ -342,7 +342,7 @@ struct avbd_solver {
       k_grid_count<<<blocksFor(n, 256), 256, 0, stream>>>(d);
       scan(cellCount, cellStart, tableSize + 1);
       k_grid_scatter<<<blocksFor(n, 256), 256, 0, stream>>>(d);
-      k_find_pairs<<<blocksFor(n, 64), 64, 0, stream>>>(d);
+      k_find_pairs<<<blocksFor((uint64_t)n * PROBE_LANES, 64), 64, 0, stream>>>(d);
       if (anyDynamicLarge) k_find_fixed_large<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
     timed(K_NARROW, [&] {

@@ -278,11 +278,16 @@ __device__ __forceinline__ void flushPairs(const Dev &d, PairQueue &q) {
   if (threadIdx.x == 0) q.count = 0;
   __syncthreads();
 }
+/** Threads per body of the broadphase probe: they split the 27 neighbour cells and the large list. */
+#ifndef PROBE_LANES
+#define PROBE_LANES 4
+#endif
 /** Body at position k of the grid's cell order (the small bodies as scattered, then the large
- *  list), when dynamic, probes: a small one the 27 cells around it (a same-cell dynamic pair from
- *  its higher index), every one the large list (a dynamic large-large pair once). Neighbouring
- *  threads take neighbouring bodies, so a wavefront's lanes walk the same and adjacent cells. */
-__device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t k, uint32_t smallCount) {
+ *  list), when dynamic, probes with PROBE_LANES threads: a small one the 27 cells around it (a
+ *  same-cell dynamic pair from its higher index), every one the large list (a dynamic large-large
+ *  pair once); lane l takes cells l, l + PROBE_LANES, ... and large-list entries alike.
+ *  Neighbouring bodies take neighbouring threads, so a wavefront walks the same and adjacent cells. */
+__device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t k, uint32_t lane, uint32_t smallCount) {
   uint32_t i;
   if (k < smallCount) i = d.cellSorted[k];
   else if (k < d.n && k - smallCount < d.largeCount) i = d.statics[k - smallCount];
@@ -294,37 +299,39 @@ __device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t
   uint2 pair;
   if (small) {
     int4 c = d.cellOf[i];
-    for (int oz = -1; oz <= 1; oz++)
-      for (int oy = -1; oy <= 1; oy++)
-        for (int ox = -1; ox <= 1; ox++) {
-          int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
-          uint32_t h = cellHash(d, cell);
-          uint32_t end = d.cellStart[h + 1];
-          for (uint32_t e = d.cellStart[h]; e < end; e++) {
-            int4 key = d.sortedKey[e];  // the cell check and the ordering rule from the sorted stream
-            if (key.x != cell.x || key.y != cell.y || key.z != cell.z || (key.w & 0x7fffffff) != cell.w) continue;
-            uint32_t j = d.cellSorted[e];
-            if (j == i) continue;
-            if (j > i && key.w < 0) continue;
-            if (testPair(d, P, j, d.sortedPos[e], pair)) queuePair(d, q, pair);
-          }
-        }
+    for (uint32_t q27 = lane; q27 < 27; q27 += PROBE_LANES) {
+      int ox = (int)(q27 % 3) - 1, oy = (int)(q27 / 3 % 3) - 1, oz = (int)(q27 / 9) - 1;
+      int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
+      uint32_t h = cellHash(d, cell);
+      uint32_t end = d.cellStart[h + 1];
+      for (uint32_t e = d.cellStart[h]; e < end; e++) {
+        int4 key = d.sortedKey[e];  // the cell check and the ordering rule from the sorted stream
+        if (key.x != cell.x || key.y != cell.y || key.z != cell.z || (key.w & 0x7fffffff) != cell.w) continue;
+        uint32_t j = d.cellSorted[e];
+        if (j == i) continue;
+        if (j > i && key.w < 0) continue;
+        if (testPair(d, P, j, d.sortedPos[e], pair)) queuePair(d, q, pair);
+      }
+    }
   }
-  for (uint32_t e = d.largeStart[w]; e < d.largeStart[w + 1]; e++) {
+  for (uint32_t e = d.largeStart[w] + lane; e < d.largeStart[w + 1]; e += PROBE_LANES) {
     uint32_t l = d.statics[e];
     if (l == i || (!small && l < i && d.b.size[l].w > 0.0f)) continue;
     float4 pl = d.b.pos[l];
     if (testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w), pair)) queuePair(d, q, pair);
   }
 }
-__global__ void k_find_pairs(Dev d) {
+/** Launched with 64-thread workgroups over PROBE_LANES threads per body. */
+__global__ void __launch_bounds__(64) k_find_pairs(Dev d) {
   __shared__ PairQueue q;
   if (threadIdx.x == 0) q.count = 0;
   __syncthreads();
   uint32_t smallCount = d.cellStart[d.tableMask + 1];
+  const uint32_t per = 64 / PROBE_LANES;
+  uint32_t slot = threadIdx.x / PROBE_LANES, lane = threadIdx.x % PROBE_LANES;
   // the workgroup walks its bodies together so that it can flush between passes
-  for (uint32_t base = blockIdx.x * blockDim.x; base < d.n; base += gridDim.x * blockDim.x) {
-    findPairsOf(d, q, base + threadIdx.x, smallCount);
+  for (uint32_t base = blockIdx.x * per; base < d.n; base += gridDim.x * per) {
+    findPairsOf(d, q, base + slot, lane, smallCount);
     flushPairs(d, q);
   }
 }
