@@ -165,6 +165,15 @@ typedef struct avbd_schedule {
 } avbd_schedule;
 int avbd_set_schedule(avbd_solver *s, const avbd_schedule *schedule);
 int avbd_get_schedule(const avbd_solver *s, avbd_schedule *schedule);
+/* fuseSparseColors / primalSparse (solver.ts:574, :2133-2143, :2248-2255; wgsl-solve.ts:378-394):
+ * with per-colour launches, colours from `first_colour` up to the cap run in one workgroup, in
+ * order, instead of a launch each. The reference starts that tail at colour 16 for scenes of at
+ * most ~1k bodies. Here the start is chosen: a colour index, 64 for none, or AVBD_TAIL_AUTO
+ * (the default), which picks the first of the trailing colours that held <= 32 bodies at each
+ * adaptive-colouring readback (AVBD_FLAG_ADAPT_COLORS; without that flag, no tail). Results are
+ * the same whichever runs. */
+#define AVBD_TAIL_AUTO 0xFFFFFFFFu
+int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour);
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
  * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`. */
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);

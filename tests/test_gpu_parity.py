@@ -270,6 +270,28 @@ def test_per_colour_launch_path_matches(gpu, monkeypatch):
     s.destroy()
 
 
+@pytest.mark.parametrize("first,cap,scene", [(0, 24, "pile"), (3, 24, "pile"), (1, 2, "pyramid")])
+def test_sparse_tail_matches(gpu, monkeypatch, first, cap, scene):
+    """Colours from `first` up fused in one workgroup (k_primal_tail) give the oracle's step bit
+    for bit, including clashing colours serialised inside the tail (the pyramid at cap 2)."""
+    say(f"case: sparse tail from colour {first}, cap {cap}, {scene} x 10 steps")
+    monkeypatch.setenv("AVBD_SOLVE", "colors")
+    bodies = scenes.random_pile(400, seed=6, bin_size=10.0)[0] if scene == "pile" else scenes.pyramid(10)[0]
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=cap)
+    s = gpu_for(bodies, kw)
+    s.set_sparse_tail(first)
+    m = model_for(bodies, kw)
+    for _ in range(10):
+        s.step()
+        m.step(1)
+    if scene == "pyramid":
+        assert s.read_counters()["clashes"] > 0
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    assert_same_contacts(s, m)
+    s.destroy()
+
+
 def test_substeps_reuse_manifolds_match(gpu):
     say("case: 4 substeps (reuse manifolds) hull/box/sphere scene, 5 steps")
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)

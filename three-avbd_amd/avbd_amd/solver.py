@@ -89,8 +89,9 @@ EXPORTS = [
     "avbd_read_contacts", "avbd_read_pairs", "avbd_read_colors", "avbd_seed", "avbd_set_fixed_colors",
     "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
-    "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward",
+    "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
 ]
+TAIL_AUTO = 0xFFFFFFFF  # avbd_set_sparse_tail: chosen at each adaptive-colouring readback
 ADJ_FLOATS = 16  # adjoint state slots per body: pos xyz, previous up-velocity, rot xyzw, vel xyz, 0, angvel xyz, 0
 
 
@@ -279,6 +280,11 @@ class GpuSolver3D:
 
     def set_schedule(self, cooperative=1, threads=512, blocks_per_cu=1):
         self._check(self.L.avbd_set_schedule(self.h, ctypes.byref(Schedule(cooperative, threads, blocks_per_cu))))
+
+    def set_sparse_tail(self, first_colour=TAIL_AUTO):
+        """fuseSparseColors (solver.ts:574): colours from first_colour up run in one workgroup (64: none)."""
+        self._live()
+        self._check(self.L.avbd_set_sparse_tail(self.h, ctypes.c_uint32(int(first_colour))))
 
     def autotune(self, steps=4, substeps=1):
         """Times the candidate solve schedules (the simulation advances) and keeps the fastest."""

@@ -982,6 +982,17 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
   }
 }
 __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
+/** The sweep's tail, colours [c0, c1), in one workgroup (primalSparse, wgsl-solve.ts:378-394, with
+ *  a chosen first colour): colours stay ordered, each one's bodies are solved chunk by chunk by the
+ *  workgroup, and every pose write is visible before the next colour reads it. For small trailing
+ *  colours and the cap's empty spares one launch replaces a launch each. */
+__global__ void __launch_bounds__(1024) k_primal_tail(Dev d, uint32_t c0, uint32_t c1) {
+  for (uint32_t c = c0; c < c1; c++) {
+    primalColor(d, c, 0, 1);
+    __threadfence_block();
+    __syncthreads();
+  }
+}
 /** dualJoint (wgsl-solve.ts:562-635) without the plastic (yield) extension. */
 __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha) {
   int4 in = d.jinfo[j];
