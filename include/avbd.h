@@ -174,6 +174,36 @@ int avbd_get_schedule(const avbd_solver *s, avbd_schedule *schedule);
  * the same whichever runs. */
 #define AVBD_TAIL_AUTO 0xFFFFFFFFu
 int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour);
+
+/* Contact events, ContactWatch (src/lib/contacts.ts:134-270; World drives it at world.ts:1736-1737):
+ * after a step, its manifolds whose bodies both carry a lifetime ID and one of which reports
+ * contacts go into this step's hash set of ID pairs. A pair missing from the last run's set
+ * began touching; one of the last run's set missing from this one ended. */
+typedef struct avbd_contact_event {
+  uint32_t kind;      /* AVBD_EVENT_BEGIN or AVBD_EVENT_END */
+  uint32_t a, b;      /* lifetime IDs; a belongs to the pair's higher solver slot */
+  uint32_t step;      /* the step passed to avbd_watch_run */
+  float point[3];     /* begin: mean of the contact points on body a, world frame */
+  float impulse;      /* begin: |sum of the points' normal lambdas| x dt */
+  float normal[3];    /* begin: the manifold normal (from b to a) */
+  float pad;
+} avbd_contact_event;
+#define AVBD_EVENT_BEGIN 1u
+#define AVBD_EVENT_END 2u
+/* The constructor (contacts.ts:158-177): two sets of 2^ceil(log2(max(64, 2 pairs))) slots and room
+ * for `capacity` events between reads. A later call replaces the sets and events (the next run
+ * reports every touching pair as begun, as the reference's growContactWatch, world.ts:1779-1790)
+ * and keeps the watched bodies. */
+int avbd_watch_create(avbd_solver *s, uint32_t pairs, uint32_t capacity);
+/* setWatched (contacts.ts:193-196): body indices[k] gets lifetime ID ids[k] (0: none) and reports
+ * its contacts when on[k] is nonzero. */
+int avbd_watch_set(avbd_solver *s, const uint32_t *indices, const uint32_t *ids, const uint8_t *on, uint32_t count);
+/* run (contacts.ts:199-217): the last step's first `pairs` manifolds against the last run's set. */
+int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt);
+/* read (contacts.ts:224-261): the events since the last read in the order written (at most
+ * `capacity` copied to out), how many did not fit (the watch's capacity, then out's), and whether
+ * a run since the last read had more touching pairs than a set holds. Clears them. */
+int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, uint32_t *count, uint32_t *dropped, uint32_t *full);
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
  * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`. */
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);

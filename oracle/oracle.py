@@ -7,6 +7,9 @@ checker. Two restatements live behind it:
   trajectories in tests/golden/oracle3d;
 * ``Model`` -- the WebGPU step's semantics (src/avbd3d/gpu/wgsl-*.ts), f32 or f64.
 
+``contact_events`` (plain Python, below) restates ContactWatch's two passes (src/lib/contacts.ts)
+on read-back records; it is pinned by known answers in tests/test_contact_events_oracle.py.
+
 Record layouts are the reference's (src/avbd3d/gpu/layout.ts): bodies 40 floats, joints 40
 floats + info (type, a, b, 0), manifolds 8 words, contacts 16 words. Here they travel as float64
 arrays with integer words (keys, ids, stamps) stored as plain numbers.
@@ -374,3 +377,60 @@ def seed_model_from_ref(model, ref, params):
             model.add_ignore(int(a), int(b))
     man, con = ref.manifold_records()
     model.set_prev(man, con)
+
+
+# --- contact events (ContactWatch, src/lib/contacts.ts) ------------------------------------------
+# A restatement of the touching/ended passes with set semantics (exact whenever a set is not full;
+# a full set's contents depend on the GPU's insertion order). f32 arithmetic in the passes' order.
+
+def _cross32(a, b):
+    return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+
+def turn32(q, v):
+    """contacts.ts:38-41: v rotated by the unit quaternion q (x, y, z, w), float32 throughout."""
+    f = np.float32
+    q = [f(x) for x in q]
+    v = [f(x) for x in v]
+    u = q[:3]
+    t = [c * f(2.0) for c in _cross32(u, v)]
+    w = [v[i] + t[i] * q[3] for i in range(3)]
+    c2 = _cross32(u, t)
+    return [w[i] + c2[i] for i in range(3)]
+
+
+def contact_events(prev, manifolds, contacts, bodies, watch, pairs, step, dt):
+    """One ContactWatch.run (contacts.ts:86-120) on a step's read-back records: manifolds (M, 8)
+    uint32 words, contacts (K, 16) uint32 words, bodies (N, 40) float32, watch (N,) uint32 words
+    (lifetime ID | 0x80000000 when reporting). prev: the last run's set of (idA, idB). Returns this
+    run's set and its events (dicts like GpuSolver3D.read_contact_events) in pass order: the
+    begins by manifold, then the ends."""
+    f = np.float32
+    now, events = set(), []
+    man = np.asarray(manifolds, np.uint32)
+    cw = np.asarray(contacts, np.uint32)
+    cf = cw.view(np.float32)
+    for m in man[:pairs]:
+        a, b, first, w = int(m[0]), int(m[1]), int(m[2]), int(m[3])
+        cnt = w & 15
+        wa, wb = int(watch[a]), int(watch[b])
+        ida, idb = wa & 0x7FFFFFFF, wb & 0x7FFFFFFF
+        if cnt == 0 or ((wa | wb) & 0x80000000) == 0 or ida == 0 or idb == 0:
+            continue
+        now.add((ida, idb))
+        if (ida, idb) in prev:
+            continue
+        pa = [f(x) for x in bodies[a, 0:3]]
+        qa = [f(x) for x in bodies[a, 4:8]]
+        point, force = [f(0.0)] * 3, f(0.0)
+        for c in range(first, first + cnt):
+            r = turn32(qa, cf[c, 0:3])
+            point = [point[i] + (r[i] + pa[i]) for i in range(3)]
+            force = force + cf[c, 12]
+        normal = m[4:7].view(np.float32)
+        events.append({"kind": 1, "a": ida, "b": idb, "step": step, "point": [float(x / f(cnt)) for x in point],
+                       "impulse": float(abs(force) * f(dt)), "normal": [float(x) for x in normal]})
+    for ida, idb in sorted(prev - now):
+        events.append({"kind": 2, "a": ida, "b": idb, "step": step, "point": [0.0, 0.0, 0.0], "impulse": 0.0,
+                       "normal": [0.0, 0.0, 0.0]})
+    return now, events

@@ -90,7 +90,14 @@ EXPORTS = [
     "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
+    "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read",
 ]
+EVENT_BEGIN, EVENT_END = 1, 2  # contacts.ts BEGIN / END
+
+
+class ContactEvent(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32), ("step", ctypes.c_uint32),
+                ("point", ctypes.c_float * 3), ("impulse", ctypes.c_float), ("normal", ctypes.c_float * 3), ("pad", ctypes.c_float)]
 TAIL_AUTO = 0xFFFFFFFF  # avbd_set_sparse_tail: chosen at each adaptive-colouring readback
 ADJ_FLOATS = 16  # adjoint state slots per body: pos xyz, previous up-velocity, rot xyzw, vel xyz, 0, angvel xyz, 0
 
@@ -280,6 +287,40 @@ class GpuSolver3D:
 
     def set_schedule(self, cooperative=1, threads=512, blocks_per_cu=1):
         self._check(self.L.avbd_set_schedule(self.h, ctypes.byref(Schedule(cooperative, threads, blocks_per_cu))))
+
+    # --- contact events (ContactWatch, src/lib/contacts.ts) ---------------------------------
+    def watch_create(self, pairs, capacity):
+        """new ContactWatch(device, bodies, maxBodies, pairs, capacity) (contacts.ts:158-177)."""
+        self._live()
+        self._check(self.L.avbd_watch_create(self.h, ctypes.c_uint32(int(pairs)), ctypes.c_uint32(int(capacity))))
+        self._watch_capacity = int(capacity)
+
+    def set_watched(self, indices, ids, on):
+        """setWatched(indices, ids, on) (contacts.ts:193-196)."""
+        self._live()
+        idx = np.ascontiguousarray(indices, np.uint32)
+        ii = np.ascontiguousarray(ids, np.uint32)
+        oo = np.ascontiguousarray(on, np.uint8)
+        if not (len(idx) == len(ii) == len(oo)):
+            raise ValueError("setWatched: indices, ids and on differ in length")
+        self._check(self.L.avbd_watch_set(self.h, _p(idx), _p(ii), _p(oo), ctypes.c_uint32(len(idx))))
+
+    def watch_run(self, pairs, step, dt):
+        """run(storage, pairs, step, dt) (contacts.ts:199-217), after a step."""
+        self._live()
+        self._check(self.L.avbd_watch_run(self.h, ctypes.c_uint32(int(pairs)), ctypes.c_uint32(int(step)), ctypes.c_float(dt)))
+
+    def read_contact_events(self):
+        """read() (contacts.ts:224-261): (events in step order, dropped, full); clears them."""
+        self._live()
+        cap = getattr(self, "_watch_capacity", 0)
+        out = (ContactEvent * max(cap, 1))()
+        n, dropped, full = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self.L.avbd_watch_read(self.h, out, ctypes.c_uint32(cap), ctypes.byref(n), ctypes.byref(dropped), ctypes.byref(full)))
+        events = [{"kind": e.kind, "a": e.a, "b": e.b, "step": e.step, "point": list(e.point), "impulse": e.impulse,
+                   "normal": list(e.normal)} for e in out[:n.value]]
+        events.sort(key=lambda e: e["step"])  # in step order; within a step, as the GPU wrote them
+        return events, dropped.value, bool(full.value)
 
     def set_sparse_tail(self, first_colour=TAIL_AUTO):
         """fuseSparseColors (solver.ts:574): colours from first_colour up run in one workgroup (64: none)."""

@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include "../../include/avbd.h"
 #include "avbd_adjoint.h"
+#include "avbd_events.h"
 
 using namespace avbd;
 
@@ -155,6 +156,12 @@ struct avbd_solver {
   std::vector<Checkpoint> ckpts;   // segment starts, then the rollout's end (the first ckptCount)
   uint32_t ckptCount = 0;
   Mem ckptMem, adjMem;             // checkpoints + per-contact sources + remap table; AdjBuf
+  // contact events (avbd_watch_*): the per-body watch words are kept, the sets and events are
+  // replaced by each avbd_watch_create
+  WatchBuf watchBuf{};
+  Mem watchMem;
+  uint32_t *watchWords = nullptr;
+  uint32_t watchFlip = 0;
   int *src = nullptr;              // per contact warm-start source while recording
   // cap: colours each recorded step swept; hostStarts/hostCounts: pinned copies of the tape's
   // colour buckets and counts, filled on the solver stream after a segment is recorded
@@ -182,6 +189,7 @@ struct avbd_solver {
   }
   ~avbd_solver() {
     freeAdjoint();
+    freeMem(watchMem);
     if (revStream) hipStreamDestroy(revStream);
     for (auto e : recorded) if (e) hipEventDestroy(e);
     for (auto e : reversed) if (e) hipEventDestroy(e);
@@ -1044,6 +1052,79 @@ int avbd_get_schedule(const avbd_solver *s, avbd_schedule *sch) {
   if (!s || !sch) return AVBD_ERR_INVALID;
   *sch = s->schedule();
   return AVBD_OK;
+}
+int avbd_watch_create(avbd_solver *s, uint32_t pairs, uint32_t capacity) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
+  if (capacity == 0) return fail(s, AVBD_ERR_INVALID, "contact events: capacity must be > 0");
+  try {
+    if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error before the contact watch");
+    if (!s->watchWords) {
+      s->watchWords = s->alloc<uint32_t>(s->bodyCap);  // zeroed: no body watched
+    }
+    s->freeMem(s->watchMem);
+    uint64_t want = std::max<uint64_t>(64, 2ull * pairs), slots = 1;
+    while (slots < want) slots <<= 1;
+    if (slots > (1u << 28)) return fail(s, AVBD_ERR_INVALID, "contact events: too many pairs");
+    WatchBuf w{};
+    w.watch = s->watchWords;
+    w.setSlots = (uint32_t)slots;
+    w.capacity = capacity;
+    w.table = s->adjAlloc<uint32_t>(s->watchMem, WATCH_HEAD + 4 * slots);
+    w.events = s->adjAlloc<float4>(s->watchMem, 3ull * capacity);
+    k_fill<<<blocksFor(WATCH_HEAD + 4 * slots, 256), 256, 0, s->stream>>>(w.table, (uint32_t)(WATCH_HEAD + 4 * slots), 0u);
+    s->watchBuf = w;
+    s->watchFlip = 0;
+    return avbd_sync(s);
+  } catch (const std::exception &e) {
+    return fail(s, AVBD_ERR_DEVICE, e.what());
+  }
+}
+int avbd_watch_set(avbd_solver *s, const uint32_t *indices, const uint32_t *ids, const uint8_t *on, uint32_t count) {
+  if (!s || (count && (!indices || !ids || !on))) return AVBD_ERR_INVALID;
+  if (!s->watchBuf.table) return fail(s, AVBD_ERR_INVALID, "contact events: no watch (avbd_watch_create first)");
+  std::vector<uint32_t> words(2ull * count);
+  for (uint32_t k = 0; k < count; k++) {
+    if (indices[k] >= s->n) return fail(s, AVBD_ERR_INVALID, "contact events: body index out of range");
+    words[k] = indices[k];
+    words[count + k] = (ids[k] & 0x7fffffffu) | (on[k] ? 0x80000000u : 0u);
+  }
+  if (!count) return AVBD_OK;
+  if (int rc = ensureStaging(s, 8ull * count)) return rc;
+  uint32_t *st = (uint32_t *)s->staging;
+  hipMemcpyAsync(st, words.data(), 8ull * count, hipMemcpyHostToDevice, s->stream);
+  k_watch_scatter<<<blocksFor(count, 256), 256, 0, s->stream>>>(s->watchWords, st, st + count, count, s->n);
+  return avbd_sync(s);
+}
+int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (!s->watchBuf.table) return fail(s, AVBD_ERR_INVALID, "contact events: no watch (avbd_watch_create first)");
+  WatchBuf &w = s->watchBuf;
+  s->watchFlip = 1 - s->watchFlip;
+  uint32_t prev = s->watchFlip ? 0u : w.setSlots, now = s->watchFlip ? w.setSlots : 0u;
+  k_fill<<<blocksFor(2ull * w.setSlots, 256), 256, 0, s->stream>>>(w.table + WATCH_HEAD + 2ull * now, 2 * w.setSlots, 0u);
+  int last = 1 - s->parity;  // the buffers the last step wrote
+  uint32_t scan = std::min(pairs, s->manCap);
+  Dev d = s->dev();
+  k_watch_touching<<<blocksFor(std::max(scan, 1u), 64), 64, 0, s->stream>>>(d, w, s->mIds[last], s->mGeo[last], s->con[last], prev, now,
+                                                                           scan, step, dt);
+  k_watch_ended<<<blocksFor(w.setSlots, 64), 64, 0, s->stream>>>(w, prev, now, step);
+  return avbd_sync(s);
+}
+int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, uint32_t *count, uint32_t *dropped, uint32_t *full) {
+  if (!s || !count || !dropped || !full || (capacity && !out)) return AVBD_ERR_INVALID;
+  if (!s->watchBuf.table) return fail(s, AVBD_ERR_INVALID, "contact events: no watch (avbd_watch_create first)");
+  WatchBuf &w = s->watchBuf;
+  uint32_t head[WATCH_HEAD];
+  hipMemcpyAsync(head, w.table, sizeof(head), hipMemcpyDeviceToHost, s->stream);
+  if (int rc = avbd_sync(s)) return rc;
+  uint32_t n = std::min(head[0], w.capacity), copied = std::min(n, capacity);
+  if (copied) hipMemcpyAsync(out, w.events, 48ull * copied, hipMemcpyDeviceToHost, s->stream);
+  k_fill<<<1, 64, 0, s->stream>>>(w.table, WATCH_HEAD, 0u);
+  *count = copied;
+  *dropped = head[0] - copied;
+  *full = head[1] != 0 ? 1u : 0u;
+  return avbd_sync(s);
 }
 int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour) {
   if (!s) return AVBD_ERR_INVALID;
