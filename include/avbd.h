@@ -175,6 +175,22 @@ int avbd_get_schedule(const avbd_solver *s, avbd_schedule *schedule);
 #define AVBD_TAIL_AUTO 0xFFFFFFFFu
 int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour);
 
+/* World's command passes on body records between steps; one record per body per call (a body
+ * commanded twice is refused, as the reference's host sums a body's pushes into one record).
+ * Pusher (src/lib/push.ts:28-45, apply :90-140): 12 floats per pushed body: the linear impulse xyz
+ * and the body index (u32 bits), the part applied off-centre xyz and 0, its moment about the origin
+ * plus angular impulses xyz and 0. velocity += lin / m, angular velocity += I^-1 (moment - x x
+ * offLin) in the principal frame; static and disabled bodies are left alone. */
+int avbd_push(avbd_solver *s, const float *pushes, uint32_t count);
+/* BodyMover (src/avbd3d/gpu/body-mover.ts:10-31, apply :56-74): 12 floats per moved body: target
+ * position xyz and the body index (u32 bits), target rotation xyzw, steps left, dt, 1 when the
+ * rotation is given (else 0), 0. Sets the velocity that reaches the target in `left` steps and the
+ * angular velocity whose step covers 1/left of the shortest arc (zero without a rotation). */
+int avbd_move(avbd_solver *s, const float *moves, uint32_t count);
+/* BodyReset (src/avbd3d/gpu/body-reset.ts:9-17, apply :41-52): x- and the inertial pose to the
+ * body's current pose, velocity and angular velocity to zero. */
+int avbd_reset(avbd_solver *s, const uint32_t *indices, uint32_t count);
+
 /* Contact events, ContactWatch (src/lib/contacts.ts:134-270; World drives it at world.ts:1736-1737):
  * after a step, its manifolds whose bodies both carry a lifetime ID and one of which reports
  * contacts go into this step's hash set of ID pairs. A pair missing from the last run's set

@@ -92,6 +92,10 @@ template <class T> struct Body {
   V3<T> pos; T friction; Q4<T> rot; V3<T> ipos; uint32_t lifecycle; Q4<T> irot;
   V3<T> size; T mass; V3<T> moment; T radius; V3<T> inpos; uint32_t stamp; Q4<T> inrot;
   V3<T> vel; T prevUp; V3<T> angvel; int shape; uint32_t hull;  // hull: header offset (vec4 units) when shape == HULL
+  /** lifecycleBits / bodyEnabled / bodyCold (layout.ts:26-32): 1 rewritten slot, 2 disabled, 4 cold. */
+  uint32_t bits() const { return lifecycle <= 7u ? lifecycle : 0u; }
+  bool enabled() const { return (bits() & 2u) == 0u; }
+  bool cold() const { return (bits() & 4u) != 0u; }
 };
 template <class T> struct Joint {  // layout.ts:232-243 (+ info)
   V3<T> penLin; T stiffLin; V3<T> penAng; T stiffAng; V3<T> lamLin; T fracture; V3<T> lamAng; T torqueArm;
@@ -195,6 +199,7 @@ template <class T> struct Model {
   }
   /** testPair(P, j) with P the probe of body i. */
   bool testPair(uint32_t i, uint32_t j) const {
+    if (!bodies[i].enabled() || !bodies[j].enabled()) return false;  // findPairs / testPair skip disabled bodies
     if (!dynamic(i) && !dynamic(j)) return false;
     if ((filterGroups[i] & filterMask[j]) == 0 || (filterGroups[j] & filterMask[i]) == 0) return false;
     V d = O::sub(bodies[i].pos, bodies[j].pos);
@@ -574,6 +579,7 @@ template <class T> struct Model {
     if (p.flags & FLAG_REUSE_CONTACTS) {  // updateRefs (wgsl-collision.ts:34-48)
       for (size_t i = 0; i < bodies.size(); i++) {
         Body<T> &b = bodies[i];
+        if (!b.enabled()) continue;
         bool moved = !refSet[i] || O::length(O::sub(b.pos, refPos[i])) > T(0.002) || O::length(O::qsub(b.rot, refRot[i])) > T(0.002);
         if (moved) { refPos[i] = b.pos; refRot[i] = b.rot; refSet[i] = true; b.stamp = p.step; }
       }
@@ -905,22 +911,27 @@ template <class T> struct Model {
     if (count == 0) return;
     uint32_t lanes = lanesFor(count), per = 64 / lanes;
     std::vector<Acc> accs(per);
+    std::vector<char> live(per);
     for (uint32_t chunk = 0; chunk * per < count; chunk++) {
       uint32_t k0 = chunk * per, k1 = std::min(count, k0 + per);
       for (uint32_t k = k0; k < k1; k++) {
         uint32_t i = bucketBodies[start + k];
+        live[k - k0] = bodies[i].enabled();  // a disabled body keeps its pose
+        if (!live[k - k0]) continue;
         Acc part[8];
         for (uint32_t l = 0; l < lanes; l++) part[l] = accumulate(i, l, lanes, alpha);
         for (uint32_t s = 4; s > 0; s >>= 1)
           if (s < lanes) for (uint32_t l = 0; l < s; l++) accAdd(part[l], part[l + s]);
         accs[k - k0] = part[0];
       }
-      for (uint32_t k = k0; k < k1; k++) finishBody(bucketBodies[start + k], accs[k - k0]);
+      for (uint32_t k = k0; k < k1; k++)
+        if (live[k - k0]) finishBody(bucketBodies[start + k], accs[k - k0]);
     }
   }
   void dualJoint(uint32_t j, T alpha) {
     Joint<T> &k = joints[j];
     if (k.type != (int)T_JOINT) return;
+    if ((k.a >= 0 && !bodies[k.a].enabled()) || !bodies[k.b].enabled()) return;
     Joint<T> k0 = k;
     T BIG = T(BIGD);
     V penLin = k0.penLin, penAng = k0.penAng, lamLin = k0.lamLin, lamAng = k0.lamAng;
@@ -974,8 +985,17 @@ template <class T> struct Model {
     T alpha = T(p.alpha), gamma = T(p.gamma), decay = alpha * gamma;
     for (Joint<T> &k : joints) {
       if (k.type != (int)T_JOINT) continue;
+      if ((k.a >= 0 && !bodies[k.a].enabled()) || !bodies[k.b].enabled()) continue;
       k.c0Lin = jointLinC(k);
       k.c0Ang = jointAngC(k);
+      if ((k.a >= 0 && bodies[k.a].cold()) || bodies[k.b].cold()) {  // a cold body's joints restart
+        k.lamLin = O::v(T(0), T(0), T(0));
+        k.lamAng = O::v(T(0), T(0), T(0));
+        T fl = O::mn(T(1.0), k.stiffLin), fa = O::mn(T(1.0), k.stiffAng);
+        k.penLin = O::v(fl, fl, fl);
+        k.penAng = O::v(fa, fa, fa);
+        continue;
+      }
       k.lamLin = O::mul(k.lamLin, decay);
       k.lamAng = O::mul(k.lamAng, decay);
       for (int r = 0; r < 3; r++) {
@@ -988,6 +1008,10 @@ template <class T> struct Model {
     T dt = dtT(), g = T(p.gravity);
     V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
     for (Body<T> &b : bodies) {
+      if (!b.enabled()) {  // untouched, its flag kept
+        if (tape) { tape->steps.back().atRest.push_back(0); tape->steps.back().wMode.push_back(1); }
+        continue;
+      }
       bool dyn = b.mass > T(0);
       V inpos = O::add(b.pos, O::mul(b.vel, dt));
       if (dyn) inpos = O::add(inpos, O::mul(up, g * (dt * dt)));
@@ -1020,6 +1044,7 @@ template <class T> struct Model {
     T dt = dtT();
     V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
     for (Body<T> &b : bodies) {
+      if (!b.enabled()) continue;
       T prevUp = O::dot(b.vel, up);
       if (b.mass > T(0)) {
         b.vel = O::divs(O::sub(b.pos, b.ipos), dt);

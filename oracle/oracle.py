@@ -434,3 +434,85 @@ def contact_events(prev, manifolds, contacts, bodies, watch, pairs, step, dt):
         events.append({"kind": 2, "a": ida, "b": idb, "step": step, "point": [0.0, 0.0, 0.0], "impulse": 0.0,
                        "normal": [0.0, 0.0, 0.0]})
     return now, events
+
+
+# --- World's command passes (Pusher, BodyMover, BodyReset) on 40-float body records ---------------
+# Record offsets (layout.ts): pos 0, rot 4, x- 8 (w: lifecycle flags), q- 12, size 16 (w: mass),
+# moment 20, inertial pos 24, inertial rot 28, vel 32, angvel 36 (w: shape).
+
+def _lifecycle(w):
+    w = np.float32(w)
+    return int(w) if 0.0 <= w <= 7.0 else 0
+
+
+def push_pass(records, pushes):
+    """Pusher's pass (src/lib/push.ts:28-45), float32 in its order: pushes (P, 12) as avbd_push."""
+    f = np.float32
+    r = np.array(records, np.float32, copy=True)
+    pw = np.asarray(pushes, np.float32)
+    for row in pw:
+        i = int(row[3:4].view(np.uint32)[0])
+        if _lifecycle(r[i, 11]) & 2:
+            continue
+        mass = r[i, 19]
+        if mass <= 0:
+            continue
+        for a in range(3):
+            r[i, 32 + a] = r[i, 32 + a] + row[a] / mass
+        pos, off, mom = [f(x) for x in r[i, 0:3]], [f(x) for x in row[4:7]], [f(x) for x in row[8:11]]
+        c = _cross32(pos, off)
+        L = [mom[a] - c[a] for a in range(3)]
+        q = [f(x) for x in r[i, 4:8]]
+        local = turn32([-q[0], -q[1], -q[2], q[3]], L)
+        local = [local[a] / max(f(r[i, 20 + a]), f(1e-12)) for a in range(3)]
+        dw = turn32(q, local)
+        for a in range(3):
+            r[i, 36 + a] = r[i, 36 + a] + dw[a]
+    return r
+
+
+def move_pass(records, moves):
+    """BodyMover's pass (src/avbd3d/gpu/body-mover.ts:10-31): moves (M, 12) as avbd_move."""
+    f = np.float32
+    r = np.array(records, np.float32, copy=True)
+    for row in np.asarray(moves, np.float32):
+        i = int(row[3:4].view(np.uint32)[0])
+        if _lifecycle(r[i, 11]) & 2:
+            continue
+        left, dt, has_rot = row[8], row[9], row[10]
+        time = left * dt
+        for a in range(3):
+            r[i, 32 + a] = (row[a] - r[i, a]) / time
+        omega = [f(0.0)] * 3
+        if has_rot != 0:
+            tq = [f(x) for x in row[4:8]]
+            q = [f(x) for x in r[i, 4:8]]
+            a_, b_ = tq, [-q[0], -q[1], -q[2], q[3]]
+            d = [a_[3] * b_[0] + a_[0] * b_[3] + a_[1] * b_[2] - a_[2] * b_[1],
+                 a_[3] * b_[1] - a_[0] * b_[2] + a_[1] * b_[3] + a_[2] * b_[0],
+                 a_[3] * b_[2] + a_[0] * b_[1] - a_[1] * b_[0] + a_[2] * b_[3],
+                 a_[3] * b_[3] - a_[0] * b_[0] - a_[1] * b_[1] - a_[2] * b_[2]]
+            ln = np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3], dtype=np.float32)
+            d = [x / ln for x in d]
+            if d[3] < 0:
+                d = [-x for x in d]
+            sine = np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], dtype=np.float32)
+            if sine > f(1e-7):
+                half = np.arctan2(sine, max(d[3], f(0.0)), dtype=np.float32) / left
+                s = f(2.0) * np.tan(min(half, f(1.5706963)), dtype=np.float32) / dt
+                omega = [d[a] / sine * s for a in range(3)]
+        r[i, 36:39] = omega
+    return r
+
+
+def reset_pass(records, indices):
+    """BodyReset's pass (src/avbd3d/gpu/body-reset.ts:9-17)."""
+    r = np.array(records, np.float32, copy=True)
+    for i in np.asarray(indices, np.uint32):
+        r[i, 8:11] = r[i, 0:3]
+        r[i, 24:27] = r[i, 0:3]
+        r[i, 12:16] = r[i, 4:8]
+        r[i, 28:32] = r[i, 4:8]
+        r[i, 32:36] = 0.0
+        r[i, 36:39] = 0.0
+    return r

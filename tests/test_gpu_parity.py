@@ -292,6 +292,37 @@ def test_sparse_tail_matches(gpu, monkeypatch, first, cap, scene):
     s.destroy()
 
 
+def test_lifecycle_flags_match(gpu):
+    """Disabled bodies (flag 2 in x-.w) stay out of the grid, the pairs, the warm starts, the solve
+    and the velocities, their joints idle; a cold body's (flag 4) joints restart; the warm start
+    consumes the other flags and keeps a disabled body's. Bit for bit with the oracle."""
+    say("case: lifecycle flags (every 7th body disabled, every 5th cold), hulls + chains, 15 steps")
+    bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
+    bodies = bodies.copy()
+    idx = np.arange(len(bodies))
+    dynamic = bodies[:, 19] > 0
+    disabled = dynamic & (idx % 7 == 3)
+    cold = dynamic & (idx % 5 == 1) & ~disabled
+    bodies[disabled, 11] = 2.0
+    bodies[cold, 11] = 4.0
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32)
+    s = gpu_for(bodies, kw, joints, info, hulls=hw)
+    m = model_for(bodies, kw, joints, info)
+    m.set_hulls(hw)
+    for _ in range(15):
+        s.step()
+        m.step(1)
+    got = s.read_bodies()
+    assert np.array_equal(records.bodies_to_oracle(got), m.bodies())
+    assert_same_contacts(s, m)
+    assert np.array_equal(got[disabled, 0:8], bodies[disabled, 0:8])  # never moved
+    assert (got[disabled, 11] == 2.0).all() and (got[cold, 11] == 0.0).all()
+    man, _ = s.read_contacts_raw()
+    touched = set(man[:, 0].tolist()) | set(man[:, 1].tolist())
+    assert not touched & set(np.nonzero(disabled)[0].tolist())
+    s.destroy()
+
+
 def test_substeps_reuse_manifolds_match(gpu):
     say("case: 4 substeps (reuse manifolds) hull/box/sphere scene, 5 steps")
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)

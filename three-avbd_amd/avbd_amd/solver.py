@@ -90,7 +90,7 @@ EXPORTS = [
     "avbd_set_color_state", "avbd_add_ignore_pairs", "avbd_set_filters", "avbd_set_hulls", "avbd_profile_step",
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
-    "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read",
+    "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read", "avbd_push", "avbd_move", "avbd_reset",
 ]
 EVENT_BEGIN, EVENT_END = 1, 2  # contacts.ts BEGIN / END
 
@@ -287,6 +287,43 @@ class GpuSolver3D:
 
     def set_schedule(self, cooperative=1, threads=512, blocks_per_cu=1):
         self._check(self.L.avbd_set_schedule(self.h, ctypes.byref(Schedule(cooperative, threads, blocks_per_cu))))
+
+    # --- command passes (Pusher, BodyMover, BodyReset) --------------------------------------
+    def push(self, bodies, lin, off_lin=None, moment=None):
+        """Pusher.apply (push.ts:90-140): per body (each once) its summed linear impulse, the part
+        of it applied off-centre and that part's moment about the origin plus angular impulses, as
+        addImpulse (push.ts:55-64) accumulates them."""
+        self._live()
+        n = len(bodies)
+        r = np.zeros((n, 12), np.float32)
+        r[:, 0:3] = np.asarray(lin, np.float32).reshape(n, 3)
+        r[:, 3] = np.asarray(bodies, np.uint32).view(np.float32)
+        if off_lin is not None:
+            r[:, 4:7] = np.asarray(off_lin, np.float32).reshape(n, 3)
+        if moment is not None:
+            r[:, 8:11] = np.asarray(moment, np.float32).reshape(n, 3)
+        self._check(self.L.avbd_push(self.h, _p(r), ctypes.c_uint32(n)))
+
+    def move(self, bodies, positions, left, dt, rotations=None):
+        """BodyMover.apply (body-mover.ts:56-74): kinematic bodies toward target poses."""
+        self._live()
+        n = len(bodies)
+        r = np.zeros((n, 12), np.float32)
+        r[:, 0:3] = np.asarray(positions, np.float32).reshape(n, 3)
+        r[:, 3] = np.asarray(bodies, np.uint32).view(np.float32)
+        r[:, 7] = 1.0
+        if rotations is not None:
+            r[:, 4:8] = np.asarray(rotations, np.float32).reshape(n, 4)
+            r[:, 10] = 1.0
+        r[:, 8] = np.broadcast_to(np.asarray(left, np.float32), (n,))
+        r[:, 9] = np.float32(dt)
+        self._check(self.L.avbd_move(self.h, _p(r), ctypes.c_uint32(n)))
+
+    def reset_bodies(self, indices):
+        """BodyReset.apply (body-reset.ts:41-52)."""
+        self._live()
+        idx = np.ascontiguousarray(indices, np.uint32)
+        self._check(self.L.avbd_reset(self.h, _p(idx), ctypes.c_uint32(len(idx))))
 
     # --- contact events (ContactWatch, src/lib/contacts.ts) ---------------------------------
     def watch_create(self, pairs, capacity):

@@ -14,6 +14,7 @@
 #include "../../include/avbd.h"
 #include "avbd_adjoint.h"
 #include "avbd_events.h"
+#include "avbd_commands.h"
 
 using namespace avbd;
 
@@ -1052,6 +1053,41 @@ int avbd_get_schedule(const avbd_solver *s, avbd_schedule *sch) {
   if (!s || !sch) return AVBD_ERR_INVALID;
   *sch = s->schedule();
   return AVBD_OK;
+}
+/** Upload `count` records of `words` 32-bit words and check their body words (one per record at
+ *  offset `bodyWord`): in range and, when `unique`, no body twice. */
+static int stageCommands(avbd_solver *s, const void *rec, uint32_t count, uint32_t words, uint32_t bodyWord, bool unique, const char *what) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
+  if (count && !rec) return AVBD_ERR_INVALID;
+  const uint32_t *w = (const uint32_t *)rec;
+  std::vector<uint32_t> seen;
+  for (uint32_t k = 0; k < count; k++) {
+    uint32_t i = w[(size_t)k * words + bodyWord];
+    if (i >= s->n) return fail(s, AVBD_ERR_INVALID, std::string(what) + ": no body slot " + std::to_string(i));
+    if (unique) seen.push_back(i);
+  }
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) return fail(s, AVBD_ERR_INVALID, std::string(what) + ": a body commanded twice");
+  if (!count) return AVBD_OK;
+  if (int rc = ensureStaging(s, 4ull * words * count)) return rc;
+  hipMemcpyAsync(s->staging, rec, 4ull * words * count, hipMemcpyHostToDevice, s->stream);
+  return AVBD_OK;
+}
+int avbd_push(avbd_solver *s, const float *pushes, uint32_t count) {
+  if (int rc = stageCommands(s, pushes, count, 12, 3, true, "push")) return rc;
+  if (count) k_push<<<blocksFor(count, 64), 64, 0, s->stream>>>(s->dev(), (const float4 *)s->staging, count);
+  return avbd_sync(s);
+}
+int avbd_move(avbd_solver *s, const float *moves, uint32_t count) {
+  if (int rc = stageCommands(s, moves, count, 12, 3, true, "move")) return rc;
+  if (count) k_move<<<blocksFor(count, 64), 64, 0, s->stream>>>(s->dev(), (const float4 *)s->staging, count);
+  return avbd_sync(s);
+}
+int avbd_reset(avbd_solver *s, const uint32_t *indices, uint32_t count) {
+  if (int rc = stageCommands(s, indices, count, 1, 0, false, "reset")) return rc;
+  if (count) k_reset<<<blocksFor(count, 64), 64, 0, s->stream>>>(s->dev(), (const uint32_t *)s->staging, count);
+  return avbd_sync(s);
 }
 int avbd_watch_create(avbd_solver *s, uint32_t pairs, uint32_t capacity) {
   if (!s) return AVBD_ERR_INVALID;

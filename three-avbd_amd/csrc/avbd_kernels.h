@@ -89,9 +89,17 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   c[C_NUM_COLORS] = 0;
 }
 
+/** lifecycleBits (layout.ts:26-31): the flags in x-.w when it holds a whole number 0..7 (1 rewritten
+ *  slot, 2 disabled, 4 cold); bodyEnabled / bodyCold (layout.ts:29-32). A disabled body is left
+ *  out of the grid, the pairs, the warm starts, the solve and the velocities; its flag stays
+ *  until the host clears it. A step's warm start consumes the others. */
+__device__ __forceinline__ uint32_t lifecycleBits(float w) { return (w >= 0.0f && w <= 7.0f) ? (uint32_t)w : 0u; }
+__device__ __forceinline__ bool bodyEnabled(const Dev &d, uint32_t i) { return (lifecycleBits(d.b.ipos[i].w) & 2u) == 0u; }
+__device__ __forceinline__ bool bodyCold(const Dev &d, uint32_t i) { return (lifecycleBits(d.b.ipos[i].w) & 4u) != 0u; }
 /** updateRefs (wgsl-collision.ts:34-48): reference poses for contact reuse. */
 __global__ void k_update_refs(Dev d) {
   GRID_STRIDE(i, d.n) {
+    if (!bodyEnabled(d, i)) continue;
     V pos = xyz(d.b.pos[i]);
     Q rot = q4(d.b.rot[i]);
     float4 lp = d.refPos[i], lr = d.refRot[i];
@@ -149,7 +157,7 @@ __device__ __forceinline__ uint32_t cellHash(const Dev &d, int4 c) {
 }
 __global__ void k_grid_count(Dev d) {
   GRID_STRIDE(i, d.n) {
-    if (isLarge(d, i)) continue;
+    if (!bodyEnabled(d, i) || isLarge(d, i)) continue;
     int4 c = cellAt(d, xyz(d.b.pos[i]));
     c.w = (int)d.worldOf[i];
     d.cellOf[i] = c;
@@ -158,7 +166,7 @@ __global__ void k_grid_count(Dev d) {
 }
 __global__ void k_grid_scatter(Dev d) {
   GRID_STRIDE(i, d.n) {
-    if (isLarge(d, i)) continue;
+    if (!bodyEnabled(d, i) || isLarge(d, i)) continue;
     uint32_t h = cellHash(d, d.cellOf[i]);
     uint32_t slot = d.cellStart[h] + atomicAdd(&d.cellCursor[h], 1u);
     d.cellSorted[slot] = i;
@@ -237,6 +245,7 @@ __device__ __forceinline__ bool testPair(const Dev &d, const Probe &P, uint32_t 
   V dd = sub(P.pos, xyz(pj));
   float r = P.radius + pj.w;
   if (dot(dd, dd) > r * r) return false;
+  if (!bodyEnabled(d, j)) return false;  // (the predicate is pure: checked where few candidates remain)
   Shape B = shapeOf(d, j);
   V hb = aabbHalf(B);
   V lim = add(add(P.aabb, hb), vv(1e-4f, 1e-4f, 1e-4f));
@@ -292,7 +301,7 @@ __device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t
   if (k < smallCount) i = d.cellSorted[k];
   else if (k < d.n && k - smallCount < d.largeCount) i = d.statics[k - smallCount];
   else return;
-  if (d.b.size[i].w <= 0.0f) return;
+  if (d.b.size[i].w <= 0.0f || !bodyEnabled(d, i)) return;
   bool small = !isLarge(d, i);
   Probe P = probeOf(d, i);
   uint32_t w = d.worldOf[i];
@@ -338,13 +347,13 @@ __global__ void __launch_bounds__(64) k_find_pairs(Dev d) {
 /** Small fixed bodies against the large dynamic ones (findFixedLarge, wgsl-collision.ts:285). */
 __global__ void k_find_fixed_large(Dev d) {
   GRID_STRIDE(i, d.n) {
-    if (d.b.size[i].w > 0.0f || isLarge(d, i)) continue;
+    if (d.b.size[i].w > 0.0f || !bodyEnabled(d, i) || isLarge(d, i)) continue;
     bool loaded = false;
     Probe P;
     uint32_t w = d.worldOf[i];
     for (uint32_t k = d.largeStart[w]; k < d.largeStart[w + 1]; k++) {
       uint32_t l = d.statics[k];
-      if (d.b.size[l].w <= 0.0f) continue;
+      if (!bodyEnabled(d, l) || d.b.size[l].w <= 0.0f) continue;
       if (!loaded) { P = probeOf(d, i); loaded = true; }
       float4 pl = d.b.pos[l];
       uint2 pair;
@@ -950,12 +959,14 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
       bool live = threadIdx.x < 64 && k < count;
       uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
       uint32_t i = item.x;
+      bool enabled = live && bodyEnabled(d, i);  // a disabled body has no contacts: only its update is skipped
+      if (d.bits) live = enabled;                // (while recording, nothing of it is evaluated)
       Acc acc;
       if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc);
       else accZero(acc);
       reduceLanes(acc, lanes);
       __syncthreads();
-      if (live && lane == 0) finishBody(d, i, acc);
+      if (enabled && lane == 0) finishBody(d, i, acc);
       __threadfence_block();
       __syncthreads();
     }
@@ -974,11 +985,13 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     bool live = k < count;
     uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
     uint32_t i = item.x;
+    bool enabled = live && bodyEnabled(d, i);  // loaded beside the walk; only the update waits on it
+    if (d.bits) live = enabled;                // (while recording, nothing of it is evaluated)
     Acc acc;
     if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc);
     else accZero(acc);
     reduceLanes(acc, lanes);
-    if (live && lane == 0) finishBody(d, i, acc);
+    if (enabled && lane == 0) finishBody(d, i, acc);
   }
 }
 __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
@@ -998,6 +1011,7 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
   int4 in = d.jinfo[j];
   if (in.x != (int)T_JOINT) return;
   int a = in.y, b = in.z;
+  if ((a >= 0 && !bodyEnabled(d, (uint32_t)a)) || !bodyEnabled(d, (uint32_t)b)) return;
   float4 *k = d.joints + 10 * j;
   float4 penLin = k[0], penAng = k[1], lamLin = k[2], lamAng = k[3];
   if (dot(xyz(penLin), xyz(penLin)) > 0.0f) {
@@ -1064,9 +1078,20 @@ __device__ __forceinline__ void warmJoints(const Dev &d, uint32_t t0, uint32_t n
   for (uint32_t j = t0; j < d.nj; j += nt) {
     int4 in = d.jinfo[j];
     if (in.x != (int)T_JOINT) continue;
+    if ((in.y >= 0 && !bodyEnabled(d, (uint32_t)in.y)) || !bodyEnabled(d, (uint32_t)in.z)) continue;
     float4 *k = d.joints + 10 * j;
     k[4] = f4(jointLinC(d, k, in.y, in.z), 0.0f);
     k[5] = f4(jointAngC(d, k, in.y, in.z), k[5].w);
+    if ((in.y >= 0 && bodyCold(d, (uint32_t)in.y)) || bodyCold(d, (uint32_t)in.z)) {
+      // a cold body's joints restart: no multipliers, penalties at their floor
+      float4 pl = k[0], pa = k[1], ll = k[2], la = k[3];
+      k[2] = make_float4(0.0f, 0.0f, 0.0f, ll.w);
+      k[3] = make_float4(0.0f, 0.0f, 0.0f, la.w);
+      float fl = mn(PENALTY_MIN, pl.w), fa = mn(PENALTY_MIN, pa.w);
+      k[0] = make_float4(fl, fl, fl, pl.w);
+      k[1] = make_float4(fa, fa, fa, pa.w);
+      continue;
+    }
     float decay = d.p.alpha * d.p.gamma, g = d.p.gamma;
     float4 pl = k[0], pa = k[1], ll = k[2], la = k[3];
     k[2] = f4(mul(xyz(ll), decay), ll.w);
@@ -1079,6 +1104,7 @@ __global__ void k_warm_joints(Dev d) { warmJoints(d, blockIdx.x * blockDim.x + t
 /** warmStartBodies (wgsl-solve.ts:293-335): inertial target, adaptive warm start, x-. */
 __device__ __forceinline__ void warmBodies(const Dev &d, uint32_t t0, uint32_t nt) {
   for (uint32_t i = t0; i < d.n; i += nt) {
+    if (!bodyEnabled(d, i)) continue;
     float4 pos = d.b.pos[i], vel4 = d.b.vel[i], av4 = d.b.angvel[i];
     Q rot = q4(d.b.rot[i]);
     V vel = xyz(vel4), angVel = xyz(av4);
@@ -1113,6 +1139,7 @@ __global__ void k_warm_bodies(Dev d) { warmBodies(d, blockIdx.x * blockDim.x + t
 /** updateVelocities (wgsl-solve.ts:672-684): BDF1. */
 __device__ __forceinline__ void velocities(const Dev &d, uint32_t t0, uint32_t nt) {
   for (uint32_t i = t0; i < d.n; i += nt) {
+    if (!bodyEnabled(d, i)) continue;
     V up = vv(d.p.upx, d.p.upy, d.p.upz);
     float4 vel4 = d.b.vel[i];
     float prevUp = dot(xyz(vel4), up);
