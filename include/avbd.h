@@ -46,6 +46,10 @@ extern "C" {
  * (GpuSolver3D.adaptColors, solver.ts:2379-2407); the read is synchronous, so runs stay
  * deterministic. Not in the reference's flag word (World.step drives its adapt, world.ts:1740). */
 #define AVBD_FLAG_ADAPT_COLORS 128u
+/* Host-side: fixed storage (the reference's fixed plan, src/lib/storage-budget.ts): capacities
+ * stay as created and work past them is dropped and counted. Without it, avbd_adapt (and the
+ * readback of AVBD_FLAG_ADAPT_COLORS) grows pairs, manifolds and contacts before they overflow. */
+#define AVBD_FLAG_FIXED_STORAGE 256u
 
 #define AVBD_BODY_FLOATS 40
 #define AVBD_JOINT_FLOATS 40
@@ -153,9 +157,15 @@ int avbd_set_filters(avbd_solver *s, const uint32_t *groups, const uint32_t *col
 /* Hull buffer (wgsl-hull.ts header format, vec4u words); bodies reference it by shape code. */
 int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t word_count);
 
-/* adapt's colouring half (solver.ts:2379-2407): colour cap = colours in use + 3 spare, grown at
- * once on clashes or a spent spare, shrunk after three quiet reads; rounds 4..32. */
+/* adapt (solver.ts:2362-2407). Colouring half: colour cap = colours in use + 3 spare, grown at
+ * once on clashes or a spent spare, shrunk after three quiet reads; rounds 4..32. Capacity half
+ * (not with AVBD_FLAG_FIXED_STORAGE or fixed colours, nor inside avbd_forward/avbd_backward): a
+ * buffer whose count overflowed or passed 0.8 of its capacity grows to max(2 x count, 1.25 x
+ * capacity), keeping the warm-start contacts; the adjoint's buffers and recorded rollout are
+ * dropped. */
 int avbd_adapt(avbd_solver *s, const avbd_counters *counters);
+/* contactCapacity / pairCapacity / manifoldCapacity (solver.ts) and the fixed ones. */
+int avbd_get_capacity(const avbd_solver *s, avbd_capacity *out);
 
 /* How the solve pass is scheduled (results are identical whichever runs). */
 typedef struct avbd_schedule {

@@ -25,6 +25,7 @@ FLAG_START_AT_REST = 8
 FLAG_MASS_PENALTY = 16
 FLAG_DEFAULTS = 31
 FLAG_ADAPT_COLORS = 128
+FLAG_FIXED_STORAGE = 256  # capacities stay as created (overflow counted, never grown)
 NO_COLOR = 255
 STICK_BIT = 0x80000000
 BODY_FLOATS = 40
@@ -91,6 +92,7 @@ EXPORTS = [
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
     "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read", "avbd_push", "avbd_move", "avbd_reset",
+    "avbd_get_capacity",
 ]
 EVENT_BEGIN, EVENT_END = 1, 2  # contacts.ts BEGIN / END
 
@@ -269,8 +271,16 @@ class GpuSolver3D:
         self._check(self.L.avbd_backward(self.h, _p(d), _p(d_init), _p(d_mu)))
         return d_init, d_mu
 
+    def capacity(self):
+        """contactCapacity / pairCapacity / manifoldCapacity and the fixed capacities."""
+        self._live()
+        c = Capacity()
+        self._check(self.L.avbd_get_capacity(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in Capacity._fields_}
+
     def adapt(self, counters=None):
-        """adaptColors (solver.ts:2379): the colour cap and rounds from a counters readback."""
+        """adapt (solver.ts:2362-2407) from a counters readback: the colour cap and rounds, and
+        (without FLAG_FIXED_STORAGE) pair/manifold/contact growth before they overflow."""
         c = Counters()
         for k, v in (counters or self.read_counters()).items():
             setattr(c, k, int(v))

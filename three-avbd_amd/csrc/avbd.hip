@@ -177,6 +177,7 @@ struct avbd_solver {
   uint32_t adjConCap = 0;
   uint32_t fwdSteps = 0, fwdEvery = 0, fwdEndStep = 0, fwdMaxContacts = 0, fwdMaxManifolds = 0;
   bool fwdValid = false;
+  bool noGrow = false;             // set while an adjoint rollout records or re-runs: storage stays
   bool fwdTail = false;            // the rollout's last segment is already on tape (recorded by the forward)
   std::string fwdWhy = "no forward rollout recorded";
 
@@ -187,6 +188,18 @@ struct avbd_solver {
     hipMemsetAsync(d.p, 0, d.bytes, stream);
     bufs.push_back(d);
     return (T *)d.p;
+  }
+  /** Frees one buffer made by alloc(). */
+  void release(void *p) {
+    for (size_t k = 0; k < bufs.size(); k++)
+      if (bufs[k].p == p) { hipFree(p); bufs.erase(bufs.begin() + (long)k); return; }
+  }
+  /** Replaces buffer `p` (of `oldCount` T) by one of `count`, keeping the first min of them. */
+  template <class T> T *regrow(T *p, size_t oldCount, size_t count, bool keep) {
+    T *q = alloc<T>(count);
+    if (keep && p) hipMemcpyAsync(q, p, sizeof(T) * std::min(oldCount, count), hipMemcpyDeviceToDevice, stream);
+    if (p) { hipStreamSynchronize(stream); release(p); }
+    return q;
   }
   ~avbd_solver() {
     freeAdjoint();
@@ -251,6 +264,46 @@ struct avbd_solver {
     }
   }
 
+  /** adapt's capacity half (solver.ts:2366-2374, GROW_AT 0.8, GROW_TO 2): a buffer whose demand
+   *  overflowed or passed 0.8 of its capacity grows to max(2 demand, 1.25 capacity); manifolds
+   *  never past pairs. Not with fixed storage, fixed colourings, or inside an adjoint rollout. */
+  void adaptStorage(const avbd_counters &c) {
+    if ((hp.flags & AVBD_FLAG_FIXED_STORAGE) || fixedColors || noGrow) return;
+    const uint64_t LIMIT = 1ull << 30;
+    auto want = [](uint32_t demand, uint32_t cap, uint32_t bit, uint32_t overflow) -> uint64_t {
+      if ((overflow & bit) || (double)demand > 0.8 * cap) return std::max<uint64_t>(2ull * demand, (uint64_t)std::ceil(1.25 * cap));
+      return cap;
+    };
+    uint64_t con = std::min(want(c.contacts, conCap, 2, c.overflow), LIMIT);
+    uint64_t pr = std::min(want(c.pairs, pairCap, 1, c.overflow), LIMIT);
+    uint64_t man = std::min(std::min(want(c.manifolds, manCap, 4, c.overflow), pr), LIMIT);
+    if (con > conCap || pr > pairCap || man > manCap) growStorage((uint32_t)std::max<uint64_t>(con, conCap), (uint32_t)std::max<uint64_t>(pr, pairCap),
+                                                                 (uint32_t)std::max<uint64_t>(man, manCap));
+  }
+  /** allocateContacts (solver.ts:1006-1040) when growing: pairs, manifolds, contacts and what is
+   *  sized by them. Both ping-pong manifold and contact buffers are copied over and the counters
+   *  left alone, so warm starts survive; the adjoint's buffers and any recorded rollout are dropped. */
+  void growStorage(uint32_t nCon, uint32_t nPairs, uint32_t nMan) {
+    hipStreamSynchronize(stream);
+    freeAdjoint();
+    fwdValid = false;
+    fwdWhy = "the solver's storage grew after avbd_forward";
+    pairs = regrow(pairs, pairCap, nPairs, false);
+    if (hullPairs) hullPairs = regrow(hullPairs, pairCap, nPairs, false);
+    for (int k = 0; k < 2; k++) {
+      mIds[k] = regrow(mIds[k], manCap, nMan, true);
+      mGeo[k] = regrow(mGeo[k], manCap, nMan, true);
+      con[k] = regrow(con[k], conCap, nCon, true);
+    }
+    uint32_t hs = pow2At(2ull * nMan), ac = 2u * (jointCap + nMan);
+    table = regrow(table, hashSize, hs, false);
+    adjList = regrow(adjList, adjCap, ac, false);
+    adjIds = regrow(adjIds, adjCap, ac, false);
+    adjGeo = regrow(adjGeo, adjCap, ac, false);
+    pairCap = nPairs; manCap = nMan; conCap = nCon; hashSize = hs; adjCap = ac;
+    manifoldsValid = false;
+    hipStreamSynchronize(stream);
+  }
   Params params() const {
     Params p;
     p.dt = hp.dt; p.gravity = hp.gravity; p.betaLin = hp.beta_lin; p.betaAng = hp.beta_ang; p.gamma = hp.gamma; p.alpha = hp.alpha;
@@ -643,7 +696,9 @@ struct avbd_solver {
         avbd_counters k{};
         k.colors = c[C_NUM_COLORS];
         k.clashes = c[C_CLASHES];
+        k.pairs = c[C_PAIRS]; k.contacts = c[C_CONTACTS]; k.manifolds = c[C_MANIFOLDS]; k.overflow = c[C_OVERFLOW];
         adapt(k);
+        adaptStorage(k);
         if (tailAuto) chooseTail(starts);
       }
     }
@@ -653,6 +708,12 @@ struct avbd_solver {
 // Fix the size of a scan's temporary: blocks of 1024 words, at most 4096 of them.
 static const uint32_t MAX_SCAN = 4096u * 1024u;
 
+/** Storage stays put while an adjoint rollout records or re-runs (its tapes index the buffers). */
+struct NoGrowScope {
+  avbd_solver *s;
+  explicit NoGrowScope(avbd_solver *x) : s(x) { s->noGrow = true; }
+  ~NoGrowScope() { s->noGrow = false; }
+};
 static int fail(avbd_solver *s, int code, const std::string &msg) {
   if (s) s->err = msg;
   return code;
@@ -1041,7 +1102,19 @@ int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t count) {
 
 int avbd_adapt(avbd_solver *s, const avbd_counters *c) {
   if (!s || !c) return AVBD_ERR_INVALID;
-  s->adapt(*c);
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
+  try {
+    s->adapt(*c);
+    s->adaptStorage(*c);
+  } catch (const std::exception &e) {
+    return fail(s, AVBD_ERR_DEVICE, e.what());
+  }
+  return AVBD_OK;
+}
+int avbd_get_capacity(const avbd_solver *s, avbd_capacity *out) {
+  if (!s || !out) return AVBD_ERR_INVALID;
+  out->bodies = s->bodyCap; out->joints = s->jointCap; out->pairs = s->pairCap; out->manifolds = s->manCap;
+  out->contacts = s->conCap; out->hull_words = s->hullCap;
   return AVBD_OK;
 }
 int avbd_set_schedule(avbd_solver *s, const avbd_schedule *sch) {
@@ -1272,6 +1345,7 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
   for (uint32_t j = 0; j < s->nj; j++)
     if (s->jinfoHost[4 * j] != (int)T_NONE) return fail(s, AVBD_ERR_INVALID, "the adjoint covers contact-only scenes (joint " + std::to_string(j) + ")");
   if (steps == 0) return fail(s, AVBD_ERR_INVALID, "steps must be > 0");
+  NoGrowScope scope(s);
   try {
     s->fwdValid = false;
     s->fwdTail = false;
@@ -1326,6 +1400,7 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
   if (!s->fwdValid) return fail(s, AVBD_ERR_INVALID, s->fwdWhy);
   if (s->stepCount != s->fwdEndStep) return fail(s, AVBD_ERR_INVALID, "the solver stepped after avbd_forward");
+  NoGrowScope scope(s);
   try {
     uint32_t K = s->fwdEvery, steps = s->fwdSteps, n = s->n;
     s->ensureReverseStream();
