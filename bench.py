@@ -130,6 +130,7 @@ def bench_batched(args, worlds=4096, loop_sample=256):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     c = batch.read_counters()
+    batch_lanes = batch.params.primal_lanes
     batched_rate = worlds * args.steps / wall
     batch.destroy()
     # the plain step looped over a sample of the same worlds (one solver each), same lead-in
@@ -158,7 +159,7 @@ def bench_batched(args, worlds=4096, loop_sample=256):
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batched-4096", "worlds": worlds, "bodies": n, "substeps": 1, "iterations": 10,
                    "contacts": c["contacts"], "colors": c["colors"], "clashes": c["clashes"], "overflow": c["overflow"],
-                   "schedule": schedule, "color_cap": c["color_cap"]},
+                   "schedule": schedule, "color_cap": c["color_cap"], "primal_lanes": list(batch_lanes)},
         "looped_plain_step": {"value": loop_rate, "unit": "worlds*steps/s", "sample_worlds": len(singles),
                               "steps": loop_steps, "speedup": batched_rate / loop_rate},
     }
@@ -426,7 +427,7 @@ def main():
                                   capacity={"pairs": 12 * n, "manifolds": 12 * n, "contacts": 40 * n})
     t = time.perf_counter()
     tune_steps = 2
-    lead = max(0, args.warmup - 12 * tune_steps)
+    lead = max(0, args.warmup - 14 * tune_steps)
     for done in range(0, lead, 40):  # in chunks, with a progress line each (profiled runs are slow)
         solver.steps(min(40, lead - done), substeps)
         solver.sync()
@@ -435,7 +436,7 @@ def main():
         schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
     else:  # counter-collection runs distort the autotuner's timings: fix the schedule instead
         solver.set_schedule(cooperative=int(args.schedule == "coop"), threads=512, blocks_per_cu=1)
-        schedule, tuned_ms = solver.schedule, solver.time_steps(12 * tune_steps, substeps) / (12 * tune_steps)
+        schedule, tuned_ms = solver.schedule, solver.time_steps(14 * tune_steps, substeps) / (14 * tune_steps)
     solver.sync()
     print(f"# lead-in {args.warmup} steps x {substeps} substeps (autotune: {schedule}, {tuned_ms:.2f} ms/step) "
           f"in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
@@ -482,6 +483,7 @@ def main():
             "contacts": c["contacts"], "manifolds": c["manifolds"], "pairs": c["pairs"], "colors": c["colors"],
             "clashes": c["clashes"], "overflow": c["overflow"], "parallelism": "single-gpu",
             "event_ms_per_step": ev_ms / 2, "schedule": schedule, "color_cap": c["color_cap"],
+            "primal_lanes": list(solver.params.primal_lanes),
         },
         "roofline": {
             "bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -85,6 +85,7 @@ typedef struct avbd_capacity {
 typedef struct avbd_counters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, color_cap;
   uint32_t step; /* steps since creation (the contact stamps' clock) */
+  uint32_t clashing_colors[2]; /* the last step's colours run serially (bit c of word c / 32) */
 } avbd_counters;
 
 /* StepProfile (avbd2d/gpu/layout.ts:289-293) in ms, from hipEvents on the solver's stream. */
@@ -231,7 +232,9 @@ int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt);
  * a run since the last read had more touching pairs than a set holds. Clears them. */
 int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, uint32_t *count, uint32_t *dropped, uint32_t *full);
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
- * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`. */
+ * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`.
+ * When per-colour launches win, it also times eight threads per body for every colour against
+ * the params' primal_lanes rule and keeps the faster in the params (avbd_get_params reads it). */
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);
 
 /* profileNextStep (solver.ts:1952): phase times of the next step, synchronously. */

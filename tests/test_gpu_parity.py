@@ -323,6 +323,33 @@ def test_lifecycle_flags_match(gpu):
     s.destroy()
 
 
+def test_clashing_colours_are_marked_per_step(gpu):
+    """A colour that clashes runs serially for that step only: once the cap is raised and the
+    colouring is proper again, no colour stays marked (colorStarts clears the marks each step,
+    wgsl-topology.ts:313-315). Bit for bit with the oracle through the change."""
+    say("case: pyramid at colour cap 2 (clashing), then cap 16")
+    bodies = scenes.pyramid(10)[0]
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=2)
+    s = gpu_for(bodies, kw)
+    m = model_for(bodies, kw)
+    for _ in range(3):
+        s.step()
+        m.step(1)
+    c = s.read_counters()
+    assert c["clashes"] > 0 and c["clashing_colors"] != [0, 0]
+    p = s.params
+    p.color_cap = 16
+    s.set_params(p)
+    m.set_params(oracle.make_params(gravity=-9.81, color_cap=16, lanes=(1 << 20, 1 << 19, 1 << 18)))
+    for _ in range(3):
+        s.step()
+        m.step(1)
+    c = s.read_counters()
+    assert c["clashes"] == 0 and c["clashing_colors"] == [0, 0]
+    assert np.array_equal(records.bodies_to_oracle(s.read_bodies()), m.bodies())
+    s.destroy()
+
+
 def test_substeps_reuse_manifolds_match(gpu):
     say("case: 4 substeps (reuse manifolds) hull/box/sphere scene, 5 steps")
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)

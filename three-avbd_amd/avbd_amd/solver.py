@@ -57,7 +57,8 @@ class Capacity(ctypes.Structure):
 
 
 class Counters(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_uint32) for k in ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "color_cap", "step")]
+    _fields_ = [(k, ctypes.c_uint32) for k in ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "color_cap", "step")] + [
+        ("clashing_colors", ctypes.c_uint32 * 2)]  # the last step's serial colours (bit c of word c // 32)
 
 
 class Profile(ctypes.Structure):
@@ -283,7 +284,10 @@ class GpuSolver3D:
         (without FLAG_FIXED_STORAGE) pair/manifold/contact growth before they overflow."""
         c = Counters()
         for k, v in (counters or self.read_counters()).items():
-            setattr(c, k, int(v))
+            if k == "clashing_colors":
+                c.clashing_colors[0], c.clashing_colors[1] = (int(x) for x in v)
+            else:
+                setattr(c, k, int(v))
         self._check(self.L.avbd_adapt(self.h, ctypes.byref(c)))
         p = Params()
         self._check(self.L.avbd_get_params(self.h, ctypes.byref(p)))
@@ -375,11 +379,15 @@ class GpuSolver3D:
         self._check(self.L.avbd_set_sparse_tail(self.h, ctypes.c_uint32(int(first_colour))))
 
     def autotune(self, steps=4, substeps=1):
-        """Times the candidate solve schedules (the simulation advances) and keeps the fastest."""
+        """Times the candidate solve schedules (the simulation advances) and keeps the fastest; under
+        per-colour launches also the primal lane rule (read back into params)."""
         best = Schedule()
         ms = ctypes.c_float()
         self._check(self.L.avbd_autotune(self.h, steps, substeps, ctypes.byref(best), ctypes.byref(ms)))
-        self.step_count += 12 * steps * substeps
+        self.step_count = self.read_counters()["step"]
+        p = Params()
+        self._check(self.L.avbd_get_params(self.h, ctypes.byref(p)))
+        self._params = p
         return {k: getattr(best, k) for k, _ in Schedule._fields_}, ms.value
 
     # --- reads -------------------------------------------------------------------------------
@@ -407,7 +415,9 @@ class GpuSolver3D:
         self._live()
         c = Counters()
         self._check(self.L.avbd_read_counters(self.h, ctypes.byref(c)))
-        return {k: getattr(c, k) for k, _ in Counters._fields_}
+        out = {k: getattr(c, k) for k, _ in Counters._fields_}
+        out["clashing_colors"] = list(c.clashing_colors)
+        return out
 
     def read_contacts_raw(self):
         """Manifold records (M, 8) u32 and contact records (K, 16) u32 of the last step."""

@@ -975,6 +975,8 @@ int avbd_read_counters(avbd_solver *s, avbd_counters *out) {
   out->colors = c[C_NUM_COLORS];
   out->color_cap = s->fixedColors ? s->fixedCap : s->hp.color_cap;
   out->step = s->stepCount;
+  out->clashing_colors[0] = c[C_CLASHING];
+  out->clashing_colors[1] = c[C_CLASHING + 1];
   return AVBD_OK;
 }
 int avbd_read_contacts(avbd_solver *s, uint32_t *man, uint32_t mcap, uint32_t *con, uint32_t ccap) {
@@ -1260,6 +1262,22 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
   int pick = -1;
   for (int k = 0; k < nc; k++) if (bestT[k] < INFINITY && (pick < 0 || bestT[k] < bestT[pick])) pick = k;
   if (pick < 0) { s->applySchedule(keep); return fail(s, AVBD_ERR_DEVICE, "autotune: no schedule ran"); }
+  // Threads per body: the given lane rule against eight threads for every colour, under the
+  // per-colour schedule when it won (results differ only in the order the lanes' sums are added)
+  const uint32_t ALL8[3] = {1u << 31, 1u << 31, 1u << 31};
+  if (!cands[pick].cooperative && !std::equal(ALL8, ALL8 + 3, s->hp.primal_lanes)) {
+    uint32_t given[3] = {s->hp.primal_lanes[0], s->hp.primal_lanes[1], s->hp.primal_lanes[2]};
+    s->applySchedule(cands[pick]);
+    std::copy(ALL8, ALL8 + 3, s->hp.primal_lanes);
+    float t8 = INFINITY;
+    for (int pass = 0; pass < 2; pass++) {
+      float ms = 0;
+      if (int rc = avbd_time_steps(s, steps, substeps, &ms)) return rc;
+      t8 = std::min(t8, ms / steps);
+    }
+    if (t8 < bestT[pick]) bestT[pick] = t8;
+    else std::copy(given, given + 3, s->hp.primal_lanes);
+  }
   s->applySchedule(cands[pick]);
   if (best) *best = cands[pick];
   if (best_ms) *best_ms = bestT[pick];

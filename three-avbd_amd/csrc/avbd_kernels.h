@@ -87,6 +87,10 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   c[C_HULL_PAIRS] = 0;
   c[C_CLASHES] = 0;
   c[C_NUM_COLORS] = 0;
+  // this step's colouring marks its clashing colours afresh (colorStarts clears them,
+  // wgsl-topology.ts:313-315): a colour that clashed once must not stay serial
+  c[C_CLASHING] = 0;
+  c[C_CLASHING + 1] = 0;
 }
 
 /** lifecycleBits (layout.ts:26-31): the flags in x-.w when it holds a whole number 0..7 (1 rewritten
