@@ -458,10 +458,12 @@ def main():
         kernel = "k_solve (cooperative: warm starts, iterations x (primal per colour, dual), velocities)"
         nbytes = solve_bytes(n, c2["manifolds"], c2["contacts"], len(joints), iters)
     else:
-        # one launch per colour below the cap per iteration: a sweep's bytes over its launches
+        # one launch per colour per iteration (trailing small colours fused into k_primal_tail,
+        # timed apart): a sweep's bytes over its k_primal launches
         avg_ms = ks["primal_ms"] / max(1, ks["primal_launches"])
-        kernel = "k_primal (one colour of the sweep; the cap's spare colours are empty launches)"
-        nbytes = primal_sweep_bytes(n, c2["manifolds"], c2["contacts"], len(joints)) / max(1, c2["color_cap"])
+        kernel = "k_primal (one colour of the sweep; trailing small colours run fused in k_primal_tail)"
+        per_sweep = ks["primal_launches"] / (2 * substeps * iters)
+        nbytes = primal_sweep_bytes(n, c2["manifolds"], c2["contacts"], len(joints)) / max(1.0, per_sweep)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
 
     value = n * args.steps * substeps / wall
