@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(64) k_tape_system(Dev d, Tape T, uint32_t s, u
         double sg = isA ? 1.0 : -1.0;
         for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
           Contact kc = d.con[c];
-          uint32_t bits = evalBits(evalContact(d, kc, basis, geo.w, A, ids.x, B, ids.y, d.p.alpha));
+          uint32_t bits = evalBits(evalContact(kc, basis, geo.w, A, B, d.p.alpha));
           DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), l, dv3(kc.pen), dv3(kc.lam), bits, mu, alpha, PA, PB);
           DV r = isA ? ev.rAW : ev.rBW;
           for (int row = 0; row < 3; row++) {

@@ -49,7 +49,7 @@ struct Dev {
   uint32_t *adjCount, *adjStart, *adjFill, *adjList;
   uint32_t adjCap;
   // the solve's flattened views: per adjacency entry its manifold record (ids.x = 0xffffffff marks
-  // a joint, id in ids.y), per colour-bucket slot its body and adjacency range
+  // a joint, id in ids.y), per colour-bucket slot its body, adjacency range and whether it is enabled
   uint4 *adjIds;
   float4 *adjGeo;
   uint4 *colorItems;
@@ -728,7 +728,7 @@ __global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
   uint32_t col = d.colA[b] & 0xffu;
   uint32_t slot = d.hist[col * d.colorGroups + blockIdx.x] + d.colorRank[b];
   d.colorBodies[slot] = b;
-  d.colorItems[slot] = make_uint4(b, d.adjStart[b], d.adjStart[b + 1], 0u);
+  d.colorItems[slot] = make_uint4(b, d.adjStart[b], d.adjStart[b + 1], bodyEnabled(d, b) ? 1u : 0u);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -817,12 +817,15 @@ __device__ __forceinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, flo
     addRow(acc, z, vv(0.0f, 0.0f, s), penAng.z, F.z);
   }
 }
-struct PairBody { Q rot; V dLin, dAng; };
+/** A contact partner's pose, its rotation at the substep's start (x-, a sphere contact's frame) and
+ *  its motion since then. */
+struct PairBody { Q rot, irot; V dLin, dAng; };
 __device__ __forceinline__ PairBody pairBody(const Dev &d, uint32_t i) {
   PairBody P;
   P.rot = q4(d.b.rot[i]);
+  P.irot = q4(d.b.irot[i]);
   P.dLin = sub(xyz(d.b.pos[i]), xyz(d.b.ipos[i]));
-  P.dAng = qsub(P.rot, q4(d.b.irot[i]));
+  P.dAng = qsub(P.rot, P.irot);
   return P;
 }
 struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; };
@@ -831,10 +834,10 @@ __device__ __forceinline__ unsigned char evalBits(const Eval &e) {
   return (unsigned char)((e.F.x < 0.0f ? 0u : 1u) | ((e.frictionScale > e.bounds && e.frictionScale > 0.0f) ? 2u : 0u));
 }
 /** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. */
-__device__ __forceinline__ Eval evalContact(const Dev &d, const Contact &k, const V *basis, float friction, const PairBody &A, uint32_t ia,
-                                            const PairBody &B, uint32_t ib, float alpha) {
+__device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, float friction, const PairBody &A, const PairBody &B,
+                                            float alpha) {
   Q rotA = A.rot, rotB = B.rot;
-  if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE) { rotA = q4(d.b.irot[ia]); rotB = q4(d.b.irot[ib]); }
+  if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE) { rotA = A.irot; rotB = B.irot; }
   Eval e;
   e.rAW = qrotate(rotA, xyz(k.rA));
   e.rBW = qrotate(rotB, xyz(k.rB));
@@ -858,26 +861,50 @@ __device__ __forceinline__ Eval evalContact(const Dev &d, const Contact &k, cons
   e.F = F;
   return e;
 }
-/** accumulate (wgsl-solve.ts:436-501): body i's system from adjacency entries lane, lane+lanes, ... */
-__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0, uint32_t end, uint32_t lane, uint32_t lanes, float alpha, Acc &acc) {
+/** Contacts of a manifold entry fetched per load batch in the primal walk. */
+#ifndef CONTACT_BATCH
+#define CONTACT_BATCH 4u
+#endif
+/** accumulate (wgsl-solve.ts:436-501): body i's system from adjacency entries lane, lane+lanes, ...
+ *  Lane 0 also returns the body's pose (pos, rot) for the update: only the body's own update
+ *  writes it, so the value read here is the one the update starts from. */
+__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0, uint32_t end, uint32_t lane, uint32_t lanes, float alpha, Acc &acc,
+                                           float4 &pos, float4 &rot) {
   accZero(acc);
+  // this lane's first entry record loads beside lane 0's body reads
+  uint32_t e = e0 + lane;
+  uint4 ids = make_uint4(0u, 0u, 0u, 0u), next;
+  if (e < end) ids = d.adjIds[e];
   float dt = d.p.dt, dt2 = dt * dt;
   if (lane == 0) {
     float4 sz = d.b.size[i], mo = d.b.moment[i];
+    pos = d.b.pos[i];
+    rot = d.b.rot[i];
     float m = sz.w / dt2;
     V I = divs(xyz(mo), dt2);
     acc.lin[0] = m; acc.lin[2] = m; acc.lin[5] = m;
     acc.ang[0] = I.x; acc.ang[2] = I.y; acc.ang[5] = I.z;
-    V dx = sub(xyz(d.b.pos[i]), xyz(d.b.inpos[i]));
+    V dx = sub(xyz(pos), xyz(d.b.inpos[i]));
     acc.rl[0] = m * dx.x; acc.rl[1] = m * dx.y; acc.rl[2] = m * dx.z;
-    V dq = qsub(q4(d.b.rot[i]), q4(d.b.inrot[i]));
+    V dq = qsub(q4(rot), q4(d.b.inrot[i]));
     acc.ra[0] = I.x * dq.x; acc.ra[1] = I.y * dq.y; acc.ra[2] = I.z * dq.z;
   }
   // Entries [e0, end) of body i, this lane's every lanes-th: each carries its manifold record
   // (k_adj_records), so a contact's poses and points load right after the entry.
-  for (uint32_t e = e0 + lane; e < end; e += lanes) {
-    uint4 ids = d.adjIds[e];
+  // The walk is a chain of dependent loads (entry -> poses and contacts), so each entry's loads go
+  // out together: the whole record, then geometry, both bodies and its first CONTACT_BATCH contacts
+  // (indexes clamped into the manifold's range); later contacts (a manifold holds up to 8)
+  // are fetched a batch at a time; the next entry's record loads beside them (its index clamped
+  // into the range). The order of the sums is the walk's own.
+  for (; e < end; e += lanes, ids = next) {
+    next = d.adjIds[e + lanes < end ? e + lanes : end - 1];
+    asm volatile("" : "+v"(ids.x), "+v"(ids.y), "+v"(ids.z), "+v"(ids.w));  // the record whole at the branch
     if (ids.x == 0xffffffffu) { addJoint(d, acc, ids.y, alpha, i); continue; }
+    uint32_t nc = ids.w & 15u;
+    if (nc == 0) continue;
+    Contact kc[CONTACT_BATCH];
+#pragma unroll
+    for (uint32_t j = 0; j < CONTACT_BATCH; j++) kc[j] = d.con[ids.z + (j < nc ? j : nc - 1)];
     float4 geo = d.adjGeo[e];
     bool isA = i == ids.x;
     PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
@@ -885,20 +912,30 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
     orthonormal(xyz(geo), basis);
     float sg = isA ? 1.0f : -1.0f;
     V l0 = mul(basis[0], sg), l1 = mul(basis[1], sg), l2 = mul(basis[2], sg);
-    uint32_t cEnd = ids.z + (ids.w & 15u);
-    for (uint32_t c = ids.z; c < cEnd; c++) {
-      Contact k = d.con[c];
-      Eval ev = evalContact(d, k, basis, geo.w, A, ids.x, B, ids.y, alpha);
-      if (d.bits) { if (isA) d.bits[c].x = evalBits(ev); else d.bits[c].y = evalBits(ev); }
-      V r = isA ? ev.rAW : ev.rBW;
-      addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-      addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-      addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+    auto batch = [&](uint32_t b0) {
+#pragma unroll
+      for (uint32_t j = 0; j < CONTACT_BATCH; j++) {
+        if (b0 + j >= nc) break;
+        const Contact &k = kc[j];
+        Eval ev = evalContact(k, basis, geo.w, A, B, alpha);
+        if (d.bits) { uint32_t c = ids.z + b0 + j; if (isA) d.bits[c].x = evalBits(ev); else d.bits[c].y = evalBits(ev); }
+        V r = isA ? ev.rAW : ev.rBW;
+        addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+        addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+        addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+      }
+    };
+    batch(0);
+    for (uint32_t b0 = CONTACT_BATCH; b0 < nc; b0 += CONTACT_BATCH) {
+#pragma unroll
+      for (uint32_t j = 0; j < CONTACT_BATCH; j++) kc[j] = d.con[ids.z + (b0 + j < nc ? b0 + j : nc - 1)];
+      batch(b0);
     }
   }
 }
-/** finishBody (wgsl-solve.ts:504-558): LDLᵀ in registers, pivots floored at 1e-5 of the diagonal. */
-__device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &acc) {
+/** finishBody (wgsl-solve.ts:504-558): LDLᵀ in registers, pivots floored at 1e-5 of the diagonal;
+ *  (pos, rot) is the body's pose as accumulate read it. */
+__device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &acc, float4 pos, float4 rot) {
   const float PF = 1e-5f;
   float A11 = acc.lin[0], A21 = acc.lin[1], A22 = acc.lin[2], A31 = acc.lin[3], A32 = acc.lin[4], A33 = acc.lin[5];
   float A41 = acc.cr[0], A42 = acc.cr[1], A43 = acc.cr[2], A44 = acc.ang[0];
@@ -922,9 +959,8 @@ __device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &
   float v3 = y3 / D3 - L43 * w1 - L53 * w2 - L63 * w3;
   float v2 = y2 / D2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
   float v1 = y1 / D1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
-  float4 pos = d.b.pos[i];
   d.b.pos[i] = make_float4(pos.x - v1, pos.y - v2, pos.z - v3, pos.w);
-  d.b.rot[i] = f4q(qadd(q4(d.b.rot[i]), neg(vv(w1, w2, w3))));
+  d.b.rot[i] = f4q(qadd(q4(rot), neg(vv(w1, w2, w3))));
 }
 __device__ __forceinline__ uint32_t lanesFor(const Params &p, uint32_t count) {
   if (count >= p.lanes1) return 1;
@@ -949,11 +985,12 @@ __device__ __forceinline__ void reduceLanes(Acc &acc, uint32_t lanes) {
  *  colour (bodies that share a constraint) runs in block 0 alone, chunk after chunk, every read
  *  of a chunk before any of its writes (the reference's serial workgroup, wgsl-solve.ts:363-374). */
 __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t block, uint32_t nblocks) {
+  // No early exit for an empty colour (the loops below run no chunk): a branch here would sink
+  // every argument load behind the colour's bounds, one more dependent load at each launch's start.
   uint32_t start = d.colorStart[c], count = d.colorStart[c + 1] - start;
-  if (count == 0) return;
+  bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
   uint32_t lanes = lanesFor(d.p, count);
   uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes;
-  bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
   if (serial) {
     // chunks of one 64-wide workgroup's bodies, in order, on block 0's first wavefront
     if (block != 0) return;
@@ -963,14 +1000,15 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
       bool live = threadIdx.x < 64 && k < count;
       uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
       uint32_t i = item.x;
-      bool enabled = live && bodyEnabled(d, i);  // a disabled body has no contacts: only its update is skipped
+      bool enabled = live && item.w != 0u;  // a disabled body has no contacts: only its update is skipped
       if (d.bits) live = enabled;                // (while recording, nothing of it is evaluated)
       Acc acc;
-      if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc);
+      float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rot = pos;
+      if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc, pos, rot);
       else accZero(acc);
       reduceLanes(acc, lanes);
       __syncthreads();
-      if (enabled && lane == 0) finishBody(d, i, acc);
+      if (enabled && lane == 0) finishBody(d, i, acc, pos, rot);
       __threadfence_block();
       __syncthreads();
     }
@@ -989,13 +1027,14 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     bool live = k < count;
     uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
     uint32_t i = item.x;
-    bool enabled = live && bodyEnabled(d, i);  // loaded beside the walk; only the update waits on it
+    bool enabled = live && item.w != 0u;  // bodyEnabled at the colouring
     if (d.bits) live = enabled;                // (while recording, nothing of it is evaluated)
     Acc acc;
-    if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc);
+    float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rot = pos;
+    if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc, pos, rot);
     else accZero(acc);
     reduceLanes(acc, lanes);
-    if (enabled && lane == 0) finishBody(d, i, acc);
+    if (enabled && lane == 0) finishBody(d, i, acc, pos, rot);
   }
 }
 __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
@@ -1052,7 +1091,7 @@ __device__ __forceinline__ void dualManifold(const Dev &d, uint32_t m, float alp
   uint32_t end = ids.z + (ids.w & 15u);
   for (uint32_t c = ids.z; c < end; c++) {
     Contact k = d.con[c];
-    Eval e = evalContact(d, k, basis, geo.w, A, ids.x, B, ids.y, alpha);
+    Eval e = evalContact(k, basis, geo.w, A, B, alpha);
     if (d.bits) d.bits[c].z = evalBits(e);
     V pen = xyz(k.pen);
     float BL = d.p.betaLin;
