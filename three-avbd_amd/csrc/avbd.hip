@@ -102,6 +102,7 @@ struct avbd_solver {
   uint32_t coopPerCU = 1;       // its workgroups per CU
   uint32_t coopMaxBlocks = 0;   // co-resident limit for coopThreads (0: no cooperative launch here)
   uint32_t numCUs = 0;
+  uint32_t hullBlocks = 4;  // k_narrow_hull's grid: its co-resident workgroups (a wavefront per pair, grid-strided)
   bool coopSupported = false;
   uint32_t shrinkVotes = 0;
   uint32_t stepsSinceAdapt = 0;
@@ -427,7 +428,7 @@ struct avbd_solver {
     });
     timed(K_NARROW, [&] {
       k_narrowphase<<<blocksFor(pairCap, 64), 64, 0, stream>>>(d);
-      if (hullPairs) k_narrow_hull<<<blocksFor(64ull * pairCap, 256, 4 * std::max(numCUs, 1u)), 256, 0, stream>>>(d);
+      if (hullPairs) k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
     });
   }
   void topology() {
@@ -773,6 +774,11 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
       s->numCUs = (uint32_t)(cus > 0 ? cus : 0);
       s->coopSupported = coop != 0 && cus > 0;
+      // as many hull workgroups as fit at once (registers and the LDS stage decide): every pair
+      // of the grid-strided loop then runs in the first wave of workgroups
+      int hullPerCU = 0;
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&hullPerCU, (const void *)k_narrow_hull, 256, 0);
+      s->hullBlocks = (uint32_t)std::max(hullPerCU, 1) * std::max(s->numCUs, 1u);
       avbd_schedule sch{1u, 512u, 1u};
       if (const char *t = getenv("AVBD_SOLVE_THREADS")) sch.threads = (uint32_t)atoi(t);
       if (const char *b = getenv("AVBD_SOLVE_BLOCKS_PER_CU")) sch.blocks_per_cu = (uint32_t)atoi(b);

@@ -423,10 +423,18 @@ __device__ __forceinline__ EdgeQuery queryEdgesWave(const uint4 *hulls, const Po
   if (!out.valid) out.sep = -3.4e38f;
   return out;
 }
-/** Per-wavefront staging of one pair's hull data in LDS: local vertices of both shapes and, per
- *  edge, the two face normals of its Gauss-map arc (A's rotated into B's frame, B's negated). */
-static const uint32_t STAGE_V = 64, STAGE_E = 128;
-struct HullStage { float4 vA[STAGE_V], vB[STAGE_V], nA[2 * STAGE_E], nB[2 * STAGE_E]; };
+/** Per-wavefront staging of one pair's hull data in LDS: both shapes' vertices in local and world
+ *  frames and their edges' vertex indices; per edge of B the two negated face normals of its
+ *  Gauss-map arc and their cross product (d x c); a list of edge pairs whose arcs cross. Sized for
+ *  hulls of up to 32 vertices (at most 90 edges), the hull builder's limit; a larger shape would
+ *  take the unstaged path. */
+static const uint32_t STAGE_V = 32, STAGE_E = 96, STAGE_LIST = 256;
+struct HullStage {
+  float4 vA[STAGE_V], vB[STAGE_V], wA[STAGE_V], wB[STAGE_V];
+  float4 nB[2 * STAGE_E], xB[STAGE_E];
+  uint2 eA[STAGE_E], eB[STAGE_E];
+  uint32_t list[STAGE_LIST];
+};
 __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -436,19 +444,28 @@ __device__ __forceinline__ bool stageable(const Poly &A, const Poly &B) {
   return A.nv <= STAGE_V && B.nv <= STAGE_V && A.ne <= STAGE_E && B.ne <= STAGE_E;
 }
 __device__ __forceinline__ void stagePair(const uint4 *hulls, const Poly &A, const Poly &B, uint32_t lane, HullStage &st) {
-  Q toB = qmul(qconj(B.q), A.q);
   waveSync();  // the previous pair's readers are done
-  for (uint32_t v = lane; v < A.nv; v += 64) st.vA[v] = f4(vertLocal(hulls, A, v), 0.0f);
-  for (uint32_t v = lane; v < B.nv; v += 64) st.vB[v] = f4(vertLocal(hulls, B, v), 0.0f);
+  for (uint32_t v = lane; v < A.nv; v += 64) {
+    V l = vertLocal(hulls, A, v);
+    st.vA[v] = f4(l, 0.0f);
+    st.wA[v] = f4(add(A.c, qrotate(A.q, l)), 0.0f);  // vert()
+  }
+  for (uint32_t v = lane; v < B.nv; v += 64) {
+    V l = vertLocal(hulls, B, v);
+    st.vB[v] = f4(l, 0.0f);
+    st.wB[v] = f4(add(B.c, qrotate(B.q, l)), 0.0f);
+  }
   for (uint32_t e = lane; e < A.ne; e += 64) {
     uint4 ea = edgeOf(hulls, A, e);
-    st.nA[2 * e] = f4(qrotate(toB, faceNormalLocal(hulls, A, ea.z)), 0.0f);
-    st.nA[2 * e + 1] = f4(qrotate(toB, faceNormalLocal(hulls, A, ea.w)), 0.0f);
+    st.eA[e] = make_uint2(ea.x, ea.y);
   }
   for (uint32_t e = lane; e < B.ne; e += 64) {
     uint4 eb = edgeOf(hulls, B, e);
-    st.nB[2 * e] = f4(neg(faceNormalLocal(hulls, B, eb.z)), 0.0f);
-    st.nB[2 * e + 1] = f4(neg(faceNormalLocal(hulls, B, eb.w)), 0.0f);
+    V c = neg(faceNormalLocal(hulls, B, eb.z)), d = neg(faceNormalLocal(hulls, B, eb.w));
+    st.nB[2 * e] = f4(c, 0.0f);
+    st.nB[2 * e + 1] = f4(d, 0.0f);
+    st.xB[e] = f4(cross(d, c), 0.0f);  // minkowskiFace's dxc
+    st.eB[e] = make_uint2(eb.x, eb.y);
   }
   waveSync();
 }
@@ -479,36 +496,157 @@ __device__ __forceinline__ void queryFacesStaged(const uint4 *hulls, const Poly 
   sep = best.v;
   face = best.i == 0xffffffffu ? 0u : best.i;
 }
-__device__ __forceinline__ EdgeQuery queryEdgesStaged(const uint4 *hulls, const Poly &A, const Poly &B, const HullStage &st, uint32_t lane) {
+/** Edge pair k = i·B.ne + j's candidate separation (queryEdges' body past the Gauss-map test),
+ *  from the stage; keeps (s, k) in `best` when s is larger, or equal with a lower k. */
+__device__ __forceinline__ void edgePairStaged(const Poly &A, const Poly &B, const HullStage &st, uint32_t i, uint32_t j, ArgBest &best,
+                                               V &bestN) {
+  uint2 ea = st.eA[i], eb = st.eB[j];
+  V pa = xyz(st.wA[ea.x]), qa = xyz(st.wA[ea.y]), pb = xyz(st.wB[eb.x]), qb = xyz(st.wB[eb.y]);
+  V n = cross(sub(qa, pa), sub(qb, pb));
+  float l = length(n);
+  if (l < 1e-5f * length(sub(qa, pa)) * length(sub(qb, pb))) return;
+  n = divs(n, l);
+  if (dot(n, sub(pa, A.c)) < 0.0f) n = neg(n);
+  float s = dot(n, sub(pb, pa));
+  uint32_t k = i * B.ne + j;
+  if (s > best.v || (s == best.v && k < best.i)) { best.v = s; best.i = k; bestN = n; }
+}
+/** queryEdges over the wavefront, bit for bit the serial loop: lane l holds A's edges l, l + 64, ...
+ *  (arc and arc cross product in registers) and walks B's edges from the stage; the pairs whose
+ *  arcs cross (the Gauss-map test, the serial one's arithmetic) are listed in LDS and evaluated a
+ *  lane each. The winner is the largest separation, lowest pair index on ties: the serial loop's
+ *  first maximum. */
+__device__ __forceinline__ EdgeQuery queryEdgesStaged(const uint4 *hulls, const Poly &A, const Poly &B, HullStage &st, uint32_t lane) {
+  Q toB = qmul(qconj(B.q), A.q);
   ArgBest best{-3.4e38f, 0xffffffffu};
-  V bestN = vv(0.0f, 0.0f, 0.0f);
-  uint32_t total = A.ne * B.ne;
-  for (uint32_t k = lane; k < total; k += 64) {
-    uint32_t i = k / B.ne, j = k - i * B.ne;
-    if (!minkowskiFace(xyz(st.nA[2 * i]), xyz(st.nA[2 * i + 1]), xyz(st.nB[2 * j]), xyz(st.nB[2 * j + 1]))) continue;
-    uint4 ea = edgeOf(hulls, A, i), eb = edgeOf(hulls, B, j);
-    V pa = vert(hulls, A, ea.x), qa = vert(hulls, A, ea.y), pb = vert(hulls, B, eb.x), qb = vert(hulls, B, eb.y);
-    V n = cross(sub(qa, pa), sub(qb, pb));
-    float l = length(n);
-    if (l < 1e-5f * length(sub(qa, pa)) * length(sub(qb, pb))) continue;
-    n = divs(n, l);
-    if (dot(n, sub(pa, A.c)) < 0.0f) n = neg(n);
-    float s = dot(n, sub(pb, pa));
-    if (s > best.v) { best.v = s; best.i = k; bestN = n; }
+  V bestN = vv(0.0f, 0.0f, 0.0f), zero = bestN;
+  uint64_t below = (1ull << lane) - 1ull;
+  uint32_t listed = 0;  // wavefront-uniform
+  for (uint32_t i0 = 0; i0 < A.ne; i0 += 64) {
+    uint32_t i = i0 + lane;
+    bool has = i < A.ne;
+    V a = zero, b = zero, bxa = zero;
+    if (has) {
+      uint4 ea = edgeOf(hulls, A, i);
+      a = qrotate(toB, faceNormalLocal(hulls, A, ea.z));
+      b = qrotate(toB, faceNormalLocal(hulls, A, ea.w));
+      bxa = cross(b, a);
+    }
+    for (uint32_t j = 0; j < B.ne; j++) {
+      V c = xyz(st.nB[2 * j]), d = xyz(st.nB[2 * j + 1]), dxc = xyz(st.xB[j]);
+      float cba = dot(c, bxa), dba = dot(d, bxa), adc = dot(a, dxc), bdc = dot(b, dxc);
+      bool crosses = has && cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f;
+      uint64_t m = __ballot(crosses);
+      if (m == 0) continue;
+      if (crosses) st.list[listed + (uint32_t)__popcll(m & below)] = (i << 16) | j;
+      listed += (uint32_t)__popcll(m);
+      if (listed > STAGE_LIST - 64) {  // room for one more round of 64
+        waveSync();
+        for (uint32_t t = lane; t < listed; t += 64) { uint32_t ij = st.list[t]; edgePairStaged(A, B, st, ij >> 16, ij & 0xffffu, best, bestN); }
+        waveSync();
+        listed = 0;
+      }
+    }
   }
+  waveSync();
+  for (uint32_t t = lane; t < listed; t += 64) { uint32_t ij = st.list[t]; edgePairStaged(A, B, st, ij >> 16, ij & 0xffffu, best, bestN); }
   ArgBest w = waveArgMax(best);
   EdgeQuery out;
   out.valid = w.i != 0xffffffffu;
   out.sep = out.valid ? w.v : -3.4e38f;
   out.ea = out.valid ? w.i / B.ne : 0u;
   out.eb = out.valid ? w.i - out.ea * B.ne : 0u;
-  out.n = out.valid ? shflV(bestN, (int)(w.i & 63u)) : vv(0.0f, 0.0f, 0.0f);
+  uint64_t owner = __ballot(out.valid && best.i == w.i);
+  out.n = owner ? shflV(bestN, (int)__ffsll((unsigned long long)owner) - 1) : vv(0.0f, 0.0f, 0.0f);
   return out;
 }
+/** Exclusive prefix and total over the wavefront of per-lane counts in {0, 1, 2}. */
+__device__ __forceinline__ uint32_t wavePrefix2(uint32_t lane, uint32_t count, uint32_t &total) {
+  uint64_t one = __ballot(count >= 1u), two = __ballot(count == 2u), below = (1ull << lane) - 1ull;
+  total = (uint32_t)(__popcll(one) + __popcll(two));
+  return (uint32_t)(__popcll(one & below) + __popcll(two & below));
+}
+/** hullFaceClip over the wavefront, bit for bit the serial one: lane m holds polygon vertex m; a
+ *  clip against a side plane has lane i emit edge (i-1, i)'s crossing and then vertex i if inside,
+ *  placed by an exclusive prefix through `buf` (64 entries of LDS, this wavefront's) in the
+ *  serial order, the first MAX_CLIP kept. The kept points stay on their lanes (in polygon order,
+ *  as the serial arrays), so each pick is a (score, lane) argmax, lowest lane on ties. The contact
+ *  is valid in lane 0. Every lane of the wavefront must take part. */
+__device__ __noinline__ void hullFaceClipWave(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, const Poly &R,
+                                                 const Poly &I, V rn, float pw, uint32_t inc, uint32_t lane, float4 *buf, Found &found) {
+  found.count = 0;
+  V zero = vv(0.0f, 0.0f, 0.0f);
+  uint32_t n = faceVertCount(hulls, I, inc);
+  if (n > MAX_CLIP) n = MAX_CLIP;
+  V p = lane < n ? vert(hulls, I, faceVert(hulls, I, inc, lane)) : zero;
+  uint32_t k = faceVertCount(hulls, R, refFace);
+  V centre = zero;
+  for (uint32_t j = 0; j < k; j++) centre = add(centre, vert(hulls, R, faceVert(hulls, R, refFace, j)));
+  centre = divs(centre, (float)k);
+  for (uint32_t j = 0; j < k && n > 0; j++) {
+    V v0 = vert(hulls, R, faceVert(hulls, R, refFace, j)), v1 = vert(hulls, R, faceVert(hulls, R, refFace, (j + 1) % k));
+    V sn = cross(sub(v1, v0), rn);
+    float l = length(sn);
+    if (l < 1.0e-6f) continue;
+    sn = divs(sn, l);
+    if (dot(sn, sub(centre, v0)) > 0.0f) sn = neg(sn);
+    float off = dot(sn, v0);
+    // clipHull: lane i takes edge (src[i-1], src[i]) (src[n-1] before src[0])
+    V a = shflV(p, (int)((lane + n - 1) % n));
+    float da = dot(sn, a) - off, db = dot(sn, p) - off;
+    bool act = lane < n;
+    bool cross_ = act && ((da <= 1.0e-5f) != (db <= 1.0e-5f)), inside = act && db <= 1.0e-5f;
+    V x = zero;
+    if (cross_) {
+      float t = 0.0f;
+      if (fabsf(da - db) > 1.0e-6f) t = clampf(da / (da - db), 0.0f, 1.0f);
+      x = add(a, mul(sub(p, a), t));
+    }
+    uint32_t total, at = wavePrefix2(lane, (cross_ ? 1u : 0u) + (inside ? 1u : 0u), total);
+    waveSync();  // the last round's reads of buf are done
+    if (cross_ && at < MAX_CLIP) buf[at] = f4(x, 0.0f);
+    if (inside && at + (cross_ ? 1u : 0u) < MAX_CLIP) buf[at + (cross_ ? 1u : 0u)] = f4(p, 0.0f);
+    waveSync();
+    n = total < MAX_CLIP ? total : MAX_CLIP;
+    p = lane < n ? xyz(buf[lane]) : zero;
+  }
+  float dist = dot(rn, p) - pw;
+  bool kept = lane < n && dist <= 1.0e-5f;
+  uint64_t keptMask = __ballot(kept);
+  uint32_t keep = (uint32_t)__popcll(keptMask);
+  uint32_t prefix = ((refIsA ? 0u : 1u) << 24) | ((refFace < 255u ? refFace : 255u) << 16) | ((inc < 255u ? inc : 255u) << 8);
+  uint64_t chosen = 0;  // over lanes
+  uint32_t picks = keep < 8u ? keep : 8u;
+  for (uint32_t c = 0; c < picks; c++) {
+    float score = -dist;
+    if (c > 0) {
+      score = 3.4e38f;
+      for (uint64_t left = chosen; left; left &= left - 1ull) {  // chosen points in increasing order
+        V dd = sub(p, shflV(p, (int)__ffsll((unsigned long long)left) - 1));
+        score = mn(score, dot(dd, dd));
+      }
+    }
+    bool cand = kept && !((chosen >> lane) & 1ull);
+    ArgBest best{-3.4e38f, 0xffffffffu};
+    if (cand && score > best.v) { best.v = score; best.i = lane; }
+    best = waveArgMax(best);
+    uint32_t pick = best.i == 0xffffffffu ? (uint32_t)__ffsll((unsigned long long)keptMask) - 1u : best.i;  // the serial loop's pick 0
+    chosen |= 1ull << pick;
+    V pt = shflV(p, (int)pick);
+    V onRef = sub(pt, mul(rn, __shfl(dist, (int)pick)));
+    uint32_t idx = pick < 255u ? pick : 255u;
+    if (lane == 0) addFound(found, refIsA ? onRef : pt, refIsA ? pt : onRef, prefix | idx);
+  }
+  if (lane == 0 && found.count == 0) {
+    V nAB = refIsA ? rn : neg(rn);
+    addFound(found, polySupport(hulls, A, nAB), polySupport(hulls, B, neg(nAB)), prefix);
+  }
+}
 /** hullFaceContact with the deepest-vertex and incident-face searches spread over the lanes
- *  (first minimum, as the serial loops); lane 0 clips and selects. */
+ *  (first minimum, as the serial loops); the clip and selection too when the wavefront has a
+ *  stage (`buf`), else lane 0 clips and selects. */
 __device__ __forceinline__ void hullFaceContactWave(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, uint32_t lane,
-                                                    Found &found) {
+                                                    float4 *buf, Found &found) {
   found.count = 0;
   const Poly &R = refIsA ? A : B;
   const Poly &I = refIsA ? B : A;
@@ -533,6 +671,10 @@ __device__ __forceinline__ void hullFaceContactWave(const uint4 *hulls, const Po
   }
   least = waveArgMin(least);
   uint32_t inc = least.i == 0xffffffffu ? 0u : least.i;
+  if (buf) {
+    hullFaceClipWave(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, lane, buf, found);
+    return;
+  }
   if (lane != 0) return;
   hullFaceClip(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, found);
 }
@@ -564,12 +706,12 @@ __device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &
   if (fbSep > faSep) {
     facePlane(hulls, B, fbFace, n, d);
     sat.n = neg(n);
-    hullFaceContactWave(hulls, A, B, false, fbFace, lane, found);
+    hullFaceContactWave(hulls, A, B, false, fbFace, lane, staged ? st->nB : nullptr, found);
     return;
   }
   facePlane(hulls, A, faFace, n, d);
   sat.n = n;
-  hullFaceContactWave(hulls, A, B, true, faFace, lane, found);
+  hullFaceContactWave(hulls, A, B, true, faFace, lane, staged ? st->nB : nullptr, found);
 }
 /** sphereHull with the faces spread over the wavefront (the result is valid in every lane). */
 __device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, float r, const Poly &X, uint32_t lane) {

@@ -968,17 +968,29 @@ __device__ __forceinline__ uint32_t lanesFor(const Params &p, uint32_t count) {
   if (count >= p.lanes4) return 4;
   return 8;
 }
-/** Reduce the partial systems of a body's lanes onto lane 0 (lanes s..2s-1 onto 0..s-1). */
+/** v of the lane S places up in the same 16-lane row (DPP row_shl:S); lanes whose source leaves
+ *  the row read 0. A body's lanes are an aligned group of at most 8, so lane j < S of a group reads
+ *  lane j + S of the same group. */
+template <int S>
+__device__ __forceinline__ float fromUp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + S, 0xf, 0xf, true));
+}
+template <int S>
+__device__ __forceinline__ void reduceStep(Acc &acc) {
+#pragma unroll
+  for (int e = 0; e < 6; e++) { acc.lin[e] += fromUp<S>(acc.lin[e]); acc.ang[e] += fromUp<S>(acc.ang[e]); }
+#pragma unroll
+  for (int e = 0; e < 9; e++) acc.cr[e] += fromUp<S>(acc.cr[e]);
+#pragma unroll
+  for (int e = 0; e < 3; e++) { acc.rl[e] += fromUp<S>(acc.rl[e]); acc.ra[e] += fromUp<S>(acc.ra[e]); }
+}
+/** Reduce the partial systems of a body's lanes onto lane 0: lanes s..2s-1 onto 0..s-1 for
+ *  s = 4, 2, 1 (those below the body's lane count), the sums lane 0 ends with being those of a
+ *  butterfly over the group. Every lane of the wavefront must be active. */
 __device__ __forceinline__ void reduceLanes(Acc &acc, uint32_t lanes) {
-  for (uint32_t s = 4; s > 0; s >>= 1) {
-    if (s >= lanes) continue;
-#pragma unroll
-    for (int e = 0; e < 6; e++) { acc.lin[e] += __shfl_xor(acc.lin[e], (int)s); acc.ang[e] += __shfl_xor(acc.ang[e], (int)s); }
-#pragma unroll
-    for (int e = 0; e < 9; e++) acc.cr[e] += __shfl_xor(acc.cr[e], (int)s);
-#pragma unroll
-    for (int e = 0; e < 3; e++) { acc.rl[e] += __shfl_xor(acc.rl[e], (int)s); acc.ra[e] += __shfl_xor(acc.ra[e], (int)s); }
-  }
+  if (lanes > 4) reduceStep<4>(acc);
+  if (lanes > 2) reduceStep<2>(acc);
+  if (lanes > 1) reduceStep<1>(acc);
 }
 /** One colour's primal over blocks [block, block + nblocks·k): lanesFor(size) threads per body in
  *  64-wide wavefronts, lane partials reduced by cross-lane shuffles, lane 0 solves. A clashing
