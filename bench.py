@@ -457,6 +457,11 @@ def main():
         avg_ms = ks["solve_ms"] / ks["solve_launches"]
         kernel = "k_solve (cooperative: warm starts, iterations x (primal per colour, dual), velocities)"
         nbytes = solve_bytes(n, c2["manifolds"], c2["contacts"], len(joints), iters)
+    elif ks["primal_launches"] == 0 and ks["tail_launches"]:
+        # every colour small enough for the fused tail: one workgroup launch per sweep
+        avg_ms = ks["tail_ms"] / ks["tail_launches"]
+        kernel = "k_primal_tail (every colour of the sweep in one workgroup)"
+        nbytes = primal_sweep_bytes(n, c2["manifolds"], c2["contacts"], len(joints))
     else:
         # one launch per colour per iteration (trailing small colours fused into k_primal_tail,
         # timed apart): a sweep's bytes over its k_primal launches
@@ -492,7 +497,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
             "traffic_source": f"profiles/{args.workload}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)",
             "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
-            "launches": ks["solve_launches"] or ks["primal_launches"], "solve_blocks": ks["solve_blocks"],
+            "launches": ks["solve_launches"] or ks["primal_launches"] or ks["tail_launches"], "solve_blocks": ks["solve_blocks"],
         },
     }
     if not args.no_cpu_baseline:

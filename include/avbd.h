@@ -98,6 +98,8 @@ typedef struct avbd_kernel_stats {
   float primal_ms, dual_ms, narrowphase_ms, broadphase_ms, solve_ms, other_ms;
   uint32_t primal_launches, dual_launches, narrowphase_launches, broadphase_launches, solve_launches;
   uint32_t solve_blocks; /* grid of the cooperative solve kernel (0: per-colour launches) */
+  float tail_ms;          /* the fused trailing colours (one workgroup per sweep) */
+  uint32_t tail_launches;
 } avbd_kernel_stats;
 
 typedef struct avbd_step_opts {

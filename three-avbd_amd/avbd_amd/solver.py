@@ -72,6 +72,7 @@ class KernelStats(ctypes.Structure):
         ("primal_launches", ctypes.c_uint32), ("dual_launches", ctypes.c_uint32),
         ("narrowphase_launches", ctypes.c_uint32), ("broadphase_launches", ctypes.c_uint32),
         ("solve_launches", ctypes.c_uint32), ("solve_blocks", ctypes.c_uint32),
+        ("tail_ms", ctypes.c_float), ("tail_launches", ctypes.c_uint32),
     ]
 
 

@@ -32,7 +32,7 @@ struct KernelTimer {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;  // kind, (start, stop)
 };
 
-enum Kind { K_PRIMAL = 0, K_DUAL = 1, K_NARROW = 2, K_BROAD = 3, K_SOLVE = 4, K_OTHER = 5 };
+enum Kind { K_PRIMAL = 0, K_DUAL = 1, K_NARROW = 2, K_BROAD = 3, K_SOLVE = 4, K_OTHER = 5, K_TAIL = 6 };
 
 uint32_t pow2At(uint64_t x) {
   uint64_t p = 1;
@@ -215,12 +215,23 @@ struct avbd_solver {
   }
 
   /** Sets the solve schedule; a cooperative one needs its grid co-resident on this device. */
+  /** The cooperative solve kernel for a workgroup size (nullptr: not one of 64 ... 1024). */
+  static const void *solveKernel(uint32_t threads) {
+    switch (threads) {
+      case 64: return (const void *)k_solve<64>;
+      case 128: return (const void *)k_solve<128>;
+      case 256: return (const void *)k_solve<256>;
+      case 512: return (const void *)k_solve<512>;
+      case 1024: return (const void *)k_solve<1024>;
+      default: return nullptr;
+    }
+  }
   int applySchedule(const avbd_schedule &sch) {
     uint32_t t = sch.threads;
     if (sch.cooperative) {
-      if (!(t == 64 || t == 128 || t == 256 || t == 512 || t == 1024) || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
+      if (!solveKernel(t) || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
       int perCU = 0;
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_solve, (int)t, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, solveKernel(t), (int)t, 0);
       if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
       coopThreads = t;
       coopPerCU = sch.blocks_per_cu;
@@ -349,6 +360,12 @@ struct avbd_solver {
     launch();
     hipEventRecord(e1, stream);
   }
+  /** Blocks for a grid-strided kernel over up to `count` items (a capacity; the live count is on
+   *  the device): enough to cover them, at most what the chip holds at once (2048 threads per
+   *  CU), so a launch over a capacity dispatches no waves that find nothing to do. */
+  uint32_t strideBlocks(uint64_t count, uint32_t threads) const {
+    return blocksFor(count, threads, std::max(numCUs, 1u) * (2048u / threads));
+  }
   void fill(uint32_t *p, uint32_t count, uint32_t v) { k_fill<<<blocksFor(count, 256), 256, 0, stream>>>(p, count, v); }
   /** Exclusive scan of count words (in may equal out). */
   void scan(const uint32_t *in, uint32_t *out, uint32_t count) {
@@ -412,11 +429,11 @@ struct avbd_solver {
     k_begin<<<1, 1, 0, stream>>>(d, reuseManifolds ? 1 : 0);
     if (hp.flags & FLAG_REUSE_CONTACTS) k_update_refs<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     if (reuseManifolds) {
-      timed(K_NARROW, [&] { k_reuse_manifolds<<<blocksFor(manCap, 256), 256, 0, stream>>>(d); });
+      timed(K_NARROW, [&] { k_reuse_manifolds<<<strideBlocks(manCap, 256), 256, 0, stream>>>(d); });
       return;
     }
     fill(table, hashSize, 0);
-    k_hash_insert<<<blocksFor(manCap, 256), 256, 0, stream>>>(d);
+    k_hash_insert<<<strideBlocks(manCap, 256), 256, 0, stream>>>(d);
     timed(K_BROAD, [&] {
       fill(cellCount, tableSize + 1, 0);
       fill(cellCursor, tableSize, 0);
@@ -427,7 +444,7 @@ struct avbd_solver {
       if (anyDynamicLarge) k_find_fixed_large<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
     timed(K_NARROW, [&] {
-      k_narrowphase<<<blocksFor(pairCap, 64), 64, 0, stream>>>(d);
+      k_narrowphase<<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
       if (hullPairs) k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
     });
   }
@@ -435,11 +452,11 @@ struct avbd_solver {
     Dev d = dev();
     fill(adjCount, n + 1, 0);
     fill(adjFill, n, 0);
-    k_degree<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d);
+    k_degree<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
     scan(adjCount, adjStart, n + 1);
-    k_fill_adj<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d);
+    k_fill_adj<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
     k_sort_adj<<<blocksFor(n, 64), 64, 0, stream>>>(d);
-    k_adj_records<<<blocksFor(adjCap, 256), 256, 0, stream>>>(d);
+    k_adj_records<<<strideBlocks(adjCap, 256), 256, 0, stream>>>(d);
     if (fixedColors) {
       hipMemcpyAsync(colA, fixedDev, n * 4, hipMemcpyDeviceToDevice, stream);
     } else {
@@ -468,7 +485,7 @@ struct avbd_solver {
     if (coopBlocks && !rec) {
       void *args[] = {&d, &barrier};
       hipError_t e;
-      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_solve, dim3(coopBlocks), dim3(coopThreads), args, 0, stream); });
+      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel(solveKernel(coopThreads), dim3(coopBlocks), dim3(coopThreads), args, 0, stream); });
       if (e == hipSuccess) return;
       (void)hipGetLastError();
       coopBlocks = 0;  // not co-resident here: per-colour launches from now on
@@ -483,12 +500,12 @@ struct avbd_solver {
       for (uint32_t c = 0; c < cap; c++) {
         if (rec) k_tape_system<<<blocksFor(8ull * n, 64), 64, 0, stream>>>(d, rec->t, recSlot, it, c, 8);
         if (!rec && c >= tailStart) {
-          timed(K_OTHER, [&] { k_primal_tail<<<1, 1024, 0, stream>>>(d, c, cap); });  // apart from k_primal's launches
+          timed(K_TAIL, [&] { k_primal_tail<<<1, 512, 0, stream>>>(d, c, cap); });  // apart from k_primal's launches
           break;
         }
         timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
       }
-      timed(K_DUAL, [&] { k_dual<<<blocksFor(nj + manCap, 256), 256, 0, stream>>>(d); });
+      timed(K_DUAL, [&] { k_dual<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d); });
       if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, it + 1);
     }
     k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
@@ -1352,6 +1369,7 @@ int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd
       case K_NARROW: out->narrowphase_ms += t; out->narrowphase_launches++; break;
       case K_BROAD: out->broadphase_ms += t; out->broadphase_launches++; break;
       case K_SOLVE: out->solve_ms += t; out->solve_launches++; break;
+      case K_TAIL: out->tail_ms += t; out->tail_launches++; break;
       default: out->other_ms += t; break;
     }
     hipEventDestroy(m.second.first);

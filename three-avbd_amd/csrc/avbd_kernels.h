@@ -833,9 +833,12 @@ struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; };
 __device__ __forceinline__ unsigned char evalBits(const Eval &e) {
   return (unsigned char)((e.F.x < 0.0f ? 0u : 1u) | ((e.frictionScale > e.bounds && e.frictionScale > 0.0f) ? 2u : 0u));
 }
-/** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. */
+/** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. With
+ *  J, also each row's angular Jacobian for the body on side A (isA) or B: r_A x l, or r_B x (-l),
+ *  the cross products C takes (a basis row times sg = +-1 is the row or its exact negation). */
+template <bool WANT_J = false>
 __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, float friction, const PairBody &A, const PairBody &B,
-                                            float alpha) {
+                                            float alpha, bool isA = true, V *J = nullptr) {
   Q rotA = A.rot, rotB = B.rot;
   if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE) { rotA = A.irot; rotB = B.irot; }
   Eval e;
@@ -846,8 +849,9 @@ __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, fl
   float C[3];
 #pragma unroll
   for (int r = 0; r < 3; r++) {
-    V l = basis[r];
-    C[r] = c0[r] * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cross(e.rAW, l), A.dAng) + dot(cross(e.rBW, neg(l)), B.dAng);
+    V l = basis[r], cA = cross(e.rAW, l), cB = cross(e.rBW, neg(l));
+    C[r] = c0[r] * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cA, A.dAng) + dot(cB, B.dAng);
+    if (WANT_J) J[r] = isA ? cA : cB;
   }
   e.C = vv(C[0], C[1], C[2]);
   V F = add(mulv(xyz(k.pen), e.C), xyz(k.lam));
@@ -892,10 +896,12 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
   // Entries [e0, end) of body i, this lane's every lanes-th: each carries its manifold record
   // (k_adj_records), so a contact's poses and points load right after the entry.
   // The walk is a chain of dependent loads (entry -> poses and contacts), so each entry's loads go
-  // out together: the whole record, then geometry, both bodies and its first CONTACT_BATCH contacts
-  // (indexes clamped into the manifold's range); later contacts (a manifold holds up to 8)
-  // are fetched a batch at a time; the next entry's record loads beside them (its index clamped
-  // into the range). The order of the sums is the walk's own.
+  // out together: the whole record, then geometry, the partner body and its first CONTACT_BATCH
+  // contacts (indexes clamped into the manifold's range); later contacts (a manifold holds up to
+  // 8) are fetched a batch at a time; the next entry's record loads beside them (its index clamped
+  // into the range). The body's own pose terms are the same for every entry: read once. The order
+  // of the sums is the walk's own.
+  PairBody self = pairBody(d, i);
   for (; e < end; e += lanes, ids = next) {
     next = d.adjIds[e + lanes < end ? e + lanes : end - 1];
     asm volatile("" : "+v"(ids.x), "+v"(ids.y), "+v"(ids.z), "+v"(ids.w));  // the record whole at the branch
@@ -907,7 +913,8 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
     for (uint32_t j = 0; j < CONTACT_BATCH; j++) kc[j] = d.con[ids.z + (j < nc ? j : nc - 1)];
     float4 geo = d.adjGeo[e];
     bool isA = i == ids.x;
-    PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
+    PairBody other = pairBody(d, isA ? ids.y : ids.x);
+    const PairBody &A = isA ? self : other, &B = isA ? other : self;
     V basis[3];
     orthonormal(xyz(geo), basis);
     float sg = isA ? 1.0f : -1.0f;
@@ -917,12 +924,12 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
       for (uint32_t j = 0; j < CONTACT_BATCH; j++) {
         if (b0 + j >= nc) break;
         const Contact &k = kc[j];
-        Eval ev = evalContact(k, basis, geo.w, A, B, alpha);
+        V J[3];
+        Eval ev = evalContact<true>(k, basis, geo.w, A, B, alpha, isA, J);  // J[r] = r x l_r·sg
         if (d.bits) { uint32_t c = ids.z + b0 + j; if (isA) d.bits[c].x = evalBits(ev); else d.bits[c].y = evalBits(ev); }
-        V r = isA ? ev.rAW : ev.rBW;
-        addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-        addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-        addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+        addRow(acc, l0, J[0], k.pen.x, ev.F.x);
+        addRow(acc, l1, J[1], k.pen.y, ev.F.y);
+        addRow(acc, l2, J[2], k.pen.z, ev.F.z);
       }
     };
     batch(0);
@@ -1013,7 +1020,7 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
       uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
       uint32_t i = item.x;
       bool enabled = live && item.w != 0u;  // a disabled body has no contacts: only its update is skipped
-      if (d.bits) live = enabled;                // (while recording, nothing of it is evaluated)
+      if (d.bits) live = enabled;           // (while recording, nothing of it is evaluated)
       Acc acc;
       float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rot = pos;
       if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc, pos, rot);
@@ -1040,7 +1047,7 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
     uint32_t i = item.x;
     bool enabled = live && item.w != 0u;  // bodyEnabled at the colouring
-    if (d.bits) live = enabled;                // (while recording, nothing of it is evaluated)
+    if (d.bits) live = enabled;           // (while recording, nothing of it is evaluated)
     Acc acc;
     float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rot = pos;
     if (live) accumulate(d, i, item.y, item.z, lane, lanes, d.p.alpha, acc, pos, rot);
@@ -1054,7 +1061,7 @@ __global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(
  *  a chosen first colour): colours stay ordered, each one's bodies are solved chunk by chunk by the
  *  workgroup, and every pose write is visible before the next colour reads it. For small trailing
  *  colours and the cap's empty spares one launch replaces a launch each. */
-__global__ void __launch_bounds__(1024) k_primal_tail(Dev d, uint32_t c0, uint32_t c1) {
+__global__ void __launch_bounds__(512) k_primal_tail(Dev d, uint32_t c0, uint32_t c1) {
   for (uint32_t c = c0; c < c1; c++) {
     primalColor(d, c, 0, 1);
     __threadfence_block();
@@ -1114,10 +1121,14 @@ __device__ __forceinline__ void dualManifold(const Dev &d, uint32_t m, float alp
       pen.z = mn(pen.z + BL * fabsf(e.C.z), PENALTY_MAX);
       bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < 0.00001f;
       key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
-      d.con[c].rA.w = __uint_as_float(key);
     }
-    d.con[c].pen = f4(pen, k.pen.w);
-    d.con[c].lam = f4(e.F, k.lam.w);
+    // the whole 64-byte record (anchors as read): a full-line write, no partial-line merge
+    Contact out;
+    out.rA = f4(xyz(k.rA), __uint_as_float(key));
+    out.rB = k.rB;
+    out.pen = f4(pen, k.pen.w);
+    out.lam = f4(e.F, k.lam.w);
+    d.con[c] = out;
   }
 }
 __device__ __forceinline__ void dualAll(const Dev &d, uint32_t t0, uint32_t nt) {
@@ -1240,8 +1251,11 @@ __device__ __forceinline__ void gridBarrier(uint32_t *bar, uint32_t *counters, u
 /** The whole solve of one substep in one cooperative launch (GpuSolver3D.step's solve pass,
  *  solver.ts:2226-2261): warm starts, iterations x (primal of each colour in use, dual), BDF1
  *  velocities, with a grid-wide barrier wherever the reference ends a dispatch. Sweeps only the
- *  colours the colouring used (the device's colour count), not the whole cap. */
-__global__ void __launch_bounds__(1024) k_solve(Dev d, uint32_t *bar) {
+ *  colours the colouring used (the device's colour count), not the whole cap. One instance per
+ *  workgroup size: the body walk holds its contact batch in registers, which a 1024-thread bound
+ *  (128 per lane) would spill. */
+template <uint32_t THREADS>
+__global__ void __launch_bounds__(THREADS) k_solve(Dev d, uint32_t *bar) {
   __shared__ uint32_t genShared;
   if (threadIdx.x == 0) genShared = __hip_atomic_load(&bar[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
