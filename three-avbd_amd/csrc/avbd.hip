@@ -132,6 +132,8 @@ struct avbd_solver {
   Contact *con[2] = {nullptr, nullptr};
   uint32_t *table = nullptr;
   uint32_t *adjCount = nullptr, *adjStart = nullptr, *adjFill = nullptr, *adjList = nullptr;
+  uint32_t *adjTmp = nullptr, *adjOwner = nullptr;  // the fill's unsorted entries and their bodies
+  uint2 *adjKeys = nullptr;                          // and their sort keys
   uint4 *adjIds = nullptr, *colorItems = nullptr;
   float4 *adjGeo = nullptr;
   uint32_t *colA = nullptr, *colB = nullptr, *colorStart = nullptr, *colorBodies = nullptr, *colorRank = nullptr, *hist = nullptr;
@@ -310,6 +312,9 @@ struct avbd_solver {
     uint32_t hs = pow2At(2ull * nMan), ac = 2u * (jointCap + nMan);
     table = regrow(table, hashSize, hs, false);
     adjList = regrow(adjList, adjCap, ac, false);
+    adjTmp = regrow(adjTmp, adjCap, ac, false);
+    adjOwner = regrow(adjOwner, adjCap, ac, false);
+    adjKeys = regrow(adjKeys, adjCap, ac, false);
     adjIds = regrow(adjIds, adjCap, ac, false);
     adjGeo = regrow(adjGeo, adjCap, ac, false);
     pairCap = nPairs; manCap = nMan; conCap = nCon; hashSize = hs; adjCap = ac;
@@ -339,6 +344,7 @@ struct avbd_solver {
     d.con = con[parity]; d.pcon = con[1 - parity]; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
     d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
+    d.adjTmp = adjTmp; d.adjOwner = adjOwner; d.adjKeys = adjKeys;
     d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
@@ -455,8 +461,7 @@ struct avbd_solver {
     k_degree<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
     scan(adjCount, adjStart, n + 1);
     k_fill_adj<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
-    k_sort_adj<<<blocksFor(n, 64), 64, 0, stream>>>(d);
-    k_adj_records<<<strideBlocks(adjCap, 256), 256, 0, stream>>>(d);
+    k_sort_adj<<<strideBlocks(adjCap, 256), 256, 0, stream>>>(d);
     if (fixedColors) {
       hipMemcpyAsync(colA, fixedDev, n * 4, hipMemcpyDeviceToDevice, stream);
     } else {
@@ -848,6 +853,9 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->adjStart = s->alloc<uint32_t>(N + 1);
     s->adjFill = s->alloc<uint32_t>(N);
     s->adjList = s->alloc<uint32_t>(s->adjCap);
+    s->adjTmp = s->alloc<uint32_t>(s->adjCap);
+    s->adjOwner = s->alloc<uint32_t>(s->adjCap);
+    s->adjKeys = s->alloc<uint2>(s->adjCap);
     s->adjIds = s->alloc<uint4>(s->adjCap);
     s->adjGeo = s->alloc<float4>(s->adjCap);
     s->colA = s->alloc<uint32_t>(N);

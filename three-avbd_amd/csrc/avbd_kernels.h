@@ -47,6 +47,8 @@ struct Dev {
   uint32_t hashMask;
   // adjacency
   uint32_t *adjCount, *adjStart, *adjFill, *adjList;
+  uint32_t *adjTmp, *adjOwner;  // the fill's unsorted entries and their bodies
+  uint2 *adjKeys;               // and their sort keys
   uint32_t adjCap;
   // the solve's flattened views: per adjacency entry its manifold record (ids.x = 0xffffffff marks
   // a joint, id in ids.y), per colour-bucket slot its body, adjacency range and whether it is enabled
@@ -374,11 +376,29 @@ __device__ __forceinline__ float contactPenaltyMin(const Dev &d, uint32_t a, uin
   float mA = ma <= 0.0f ? 3.4e38f : ma, mB = mb <= 0.0f ? 3.4e38f : mb;
   return clampf(mn(mA, mB) / (d.p.dt * d.p.dt), PENALTY_MIN, PENALTY_MAX);
 }
-__device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pgeo) {
+/** Exclusive prefix sum over the wavefront of v, and the wavefront's total. Every lane must take part. */
+__device__ __forceinline__ uint32_t waveExclusiveSum(uint32_t v, uint32_t &total) {
+  uint32_t lane = threadIdx.x & 63u, x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += t;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+/** Bump allocation of `v` slots of *counter per lane with one atomic per wavefront (lanes take
+ *  consecutive ranges in lane order); returns the lane's first slot. Every lane must take part. */
+__device__ __forceinline__ uint32_t waveAlloc(uint32_t *counter, uint32_t v) {
+  uint32_t total, pre = waveExclusiveSum(v, total), base = 0;
+  if ((threadIdx.x & 63u) == 0 && total) base = atomicAdd(counter, total);
+  return __shfl(base, 0, 64) + pre;
+}
+/** copyContacts (wgsl-collision.ts:721-755) with the manifold slot m and contact range base already
+ *  allocated. */
+__device__ __forceinline__ void copyContactsAt(const Dev &d, uint4 pids, float4 pgeo, uint32_t m, uint32_t base) {
   uint32_t a = pids.x, b = pids.y, count = pids.w & 15u;
-  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
   if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
-  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], count);
   if (base + count > d.conCap) {
     atomicOr(&d.counters[C_OVERFLOW], 2u);
     d.mIds[m] = make_uint4(a, b, base, pids.w & ~15u);
@@ -409,11 +429,28 @@ __device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pg
   }
 }
 /** reuseManifolds (wgsl-collision.ts:758): substeps after the first keep last substep's points. */
-__global__ void k_reuse_manifolds(Dev d) {
+/** copyContacts (wgsl-collision.ts:721-755): a manifold slot, then its contact range, one atomic
+ *  each (for callers whose lanes diverge). */
+__device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pgeo) {
+  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
+  if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
+  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], pids.w & 15u);
+  copyContactsAt(d, pids, pgeo, m, base);
+}
+/** reuseManifolds (wgsl-collision.ts:757-764). Each kept manifold's slot and contact range come
+ *  from one atomic per wavefront (waveAlloc):
+ *  which slot a manifold lands in is arbitrary either way (one atomic each lands them in
+ *  arrival order), and no later pass depends on it. */
+__global__ void __launch_bounds__(256) k_reuse_manifolds(Dev d) {
   uint32_t count = prevManifoldCount(d);
-  GRID_STRIDE(m, count) {
-    uint4 ids = d.pmIds[m];
-    if ((ids.w & 15u) > 0) copyContacts(d, ids, d.pmGeo[m]);
+  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
+  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
+    uint32_t m = w + (threadIdx.x & 63u);
+    uint4 ids = m < count ? d.pmIds[m] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t cnt = ids.w & 15u;
+    uint32_t slot = waveAlloc(&d.counters[C_MANIFOLDS], cnt > 0 ? 1u : 0u);
+    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], cnt > 0 && slot < d.manCap ? cnt : 0u);
+    if (cnt > 0) copyContactsAt(d, ids, d.pmGeo[m], slot, base);
   }
 }
 __device__ __forceinline__ bool reuseContacts(const Dev &d, uint32_t a, uint32_t b) {
@@ -570,6 +607,16 @@ __global__ void k_degree(Dev d) {
     if (isDynamic(d, e.y)) atomicAdd(&d.adjCount[e.y], 1u);
   }
 }
+/** An adjacency entry's sort key: joints first (by id), then manifolds by partner and first
+ *  contact's feature. */
+__device__ __forceinline__ uint2 adjKey(const Dev &d, uint32_t b, uint32_t id) {
+  if (id < d.nj) return make_uint2(0u, id);
+  uint4 ids = d.mIds[id - d.nj];
+  uint32_t other = ids.x == b ? ids.y : ids.x;
+  uint32_t sec = (ids.w & 15u) == 0 ? 0u : (__float_as_uint(d.con[ids.z].rA.w) & ~STICK_BIT);
+  return make_uint2(other + 2, sec);
+}
+__device__ __forceinline__ bool keyLess(uint2 a, uint2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); }
 __global__ void k_fill_adj(Dev d) {
   uint32_t total = d.nj + manifoldCount(d);
   GRID_STRIDE(id, total) {
@@ -580,43 +627,36 @@ __global__ void k_fill_adj(Dev d) {
       int x = ends[s];
       if (!isDynamic(d, x)) continue;
       uint32_t slot = d.adjStart[x] + atomicAdd(&d.adjFill[x], 1u);
-      if (slot < d.adjCap) d.adjList[slot] = id;
+      if (slot < d.adjCap) {
+        d.adjTmp[slot] = id;
+        d.adjOwner[slot] = (uint32_t)x;
+        d.adjKeys[slot] = adjKey(d, (uint32_t)x, id);
+      }
     }
   }
 }
 /** Where id goes in body b's list: joints by id, then manifolds by (other + 2, first key). */
-__device__ __forceinline__ uint2 adjKey(const Dev &d, uint32_t b, uint32_t id) {
-  if (id < d.nj) return make_uint2(0u, id);
-  uint4 ids = d.mIds[id - d.nj];
-  uint32_t other = ids.x == b ? ids.y : ids.x;
-  uint32_t sec = (ids.w & 15u) == 0 ? 0u : (__float_as_uint(d.con[ids.z].rA.w) & ~STICK_BIT);
-  return make_uint2(other + 2, sec);
-}
-__device__ __forceinline__ bool keyLess(uint2 a, uint2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); }
-/** The manifold record of every (sorted) adjacency entry, for the solve's flattened walk. */
-__global__ void k_adj_records(Dev d) {
-  uint32_t total = d.adjStart[d.n];
-  GRID_STRIDE(e, total) {
-    uint32_t id = d.adjList[e];
-    if (id < d.nj) { d.adjIds[e] = make_uint4(0xffffffffu, id, 0u, 0u); continue; }
-    d.adjIds[e] = d.mIds[id - d.nj];
-    d.adjGeo[e] = d.mGeo[id - d.nj];
-  }
-}
+/** Each body's entries in key order (joints first by id, then manifolds by partner and first
+ *  contact's feature; keys are distinct): entry e of the fill goes to its body's range at the
+ *  number of that body's keys below its own, with its manifold record beside it for the solve's
+ *  flattened walk (a joint's record: ids.x = 0xffffffff, id in ids.y). */
 __global__ void k_sort_adj(Dev d) {
-  GRID_STRIDE(b, d.n) {
+  uint32_t total = d.adjStart[d.n];
+  if (total > d.adjCap) total = d.adjCap;
+  GRID_STRIDE(e, total) {
+    uint32_t b = d.adjOwner[e];
     uint32_t lo = d.adjStart[b], hi = d.adjStart[b + 1];
-    for (uint32_t i = lo + 1; i < hi; i++) {
-      uint32_t id = d.adjList[i];
-      uint2 key = adjKey(d, b, id);
-      uint32_t j = i;
-      while (j > lo) {
-        uint32_t prev = d.adjList[j - 1];
-        if (!keyLess(key, adjKey(d, b, prev))) break;
-        d.adjList[j] = prev;
-        j--;
-      }
-      d.adjList[j] = id;
+    if (hi > d.adjCap) hi = d.adjCap;
+    uint2 key = d.adjKeys[e];
+    uint32_t rank = 0;
+    for (uint32_t k = lo; k < hi; k++) rank += keyLess(d.adjKeys[k], key) ? 1u : 0u;
+    uint32_t id = d.adjTmp[e], at = lo + rank;
+    d.adjList[at] = id;
+    if (id < d.nj) {
+      d.adjIds[at] = make_uint4(0xffffffffu, id, 0u, 0u);
+    } else {
+      d.adjIds[at] = d.mIds[id - d.nj];
+      d.adjGeo[at] = d.mGeo[id - d.nj];
     }
   }
 }
@@ -894,7 +934,7 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
     acc.ra[0] = I.x * dq.x; acc.ra[1] = I.y * dq.y; acc.ra[2] = I.z * dq.z;
   }
   // Entries [e0, end) of body i, this lane's every lanes-th: each carries its manifold record
-  // (k_adj_records), so a contact's poses and points load right after the entry.
+  // (k_sort_adj), so a contact's poses and points load right after the entry.
   // The walk is a chain of dependent loads (entry -> poses and contacts), so each entry's loads go
   // out together: the whole record, then geometry, the partner body and its first CONTACT_BATCH
   // contacts (indexes clamped into the manifold's range); later contacts (a manifold holds up to
