@@ -245,12 +245,13 @@ __device__ __forceinline__ void emitPair(const Dev &d, uint32_t a, uint32_t b) {
 /** testPair (wgsl-collision.ts:208-234), given body j's position and radius already loaded:
  *  whether the pair is a candidate, and the pair (higher index first). */
 __device__ __forceinline__ bool testPair(const Dev &d, const Probe &P, uint32_t j, float4 pj, uint2 &pair) {
-  if (!P.dynamic && d.b.size[j].w <= 0.0f) return false;
-  uint2 fj = d.filters[j];
-  if ((P.filter.x & fj.y) == 0u || (fj.x & P.filter.y) == 0u) return false;
+  // a conjunction of pure tests: the bounding-sphere one first, from what is loaded already
   V dd = sub(P.pos, xyz(pj));
   float r = P.radius + pj.w;
   if (dot(dd, dd) > r * r) return false;
+  if (!P.dynamic && d.b.size[j].w <= 0.0f) return false;
+  uint2 fj = d.filters[j];
+  if ((P.filter.x & fj.y) == 0u || (fj.x & P.filter.y) == 0u) return false;
   if (!bodyEnabled(d, j)) return false;  // (the predicate is pure: checked where few candidates remain)
   Shape B = shapeOf(d, j);
   V hb = aabbHalf(B);
@@ -868,17 +869,18 @@ __device__ __forceinline__ PairBody pairBody(const Dev &d, uint32_t i) {
   P.dAng = qsub(P.rot, P.irot);
   return P;
 }
-struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; };
+struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; V J[3]; };
 /** Branch bits of an evaluation: bit0 normal force clamped to 0, bit1 friction on the cone. */
 __device__ __forceinline__ unsigned char evalBits(const Eval &e) {
   return (unsigned char)((e.F.x < 0.0f ? 0u : 1u) | ((e.frictionScale > e.bounds && e.frictionScale > 0.0f) ? 2u : 0u));
 }
 /** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. With
- *  J, also each row's angular Jacobian for the body on side A (isA) or B: r_A x l, or r_B x (-l),
- *  the cross products C takes (a basis row times sg = +-1 is the row or its exact negation). */
+ *  WANT_J, also each row's angular Jacobian (e.J) for the body on side A (isA) or B: r_A x l, or
+ *  r_B x (-l), the cross products C takes (a basis row times sg = +-1 is the row or its exact
+ *  negation). */
 template <bool WANT_J = false>
 __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, float friction, const PairBody &A, const PairBody &B,
-                                            float alpha, bool isA = true, V *J = nullptr) {
+                                            float alpha, bool isA = true) {
   Q rotA = A.rot, rotB = B.rot;
   if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE) { rotA = A.irot; rotB = B.irot; }
   Eval e;
@@ -891,7 +893,7 @@ __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, fl
   for (int r = 0; r < 3; r++) {
     V l = basis[r], cA = cross(e.rAW, l), cB = cross(e.rBW, neg(l));
     C[r] = c0[r] * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cA, A.dAng) + dot(cB, B.dAng);
-    if (WANT_J) J[r] = isA ? cA : cB;
+    if (WANT_J) e.J[r] = isA ? cA : cB;
   }
   e.C = vv(C[0], C[1], C[2]);
   V F = add(mulv(xyz(k.pen), e.C), xyz(k.lam));
@@ -954,7 +956,8 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
     float4 geo = d.adjGeo[e];
     bool isA = i == ids.x;
     PairBody other = pairBody(d, isA ? ids.y : ids.x);
-    const PairBody &A = isA ? self : other, &B = isA ? other : self;
+    PairBody A = self, B = other;  // by value: a reference picked between two locals would put them in memory
+    if (!isA) { A = other; B = self; }
     V basis[3];
     orthonormal(xyz(geo), basis);
     float sg = isA ? 1.0f : -1.0f;
@@ -964,12 +967,11 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
       for (uint32_t j = 0; j < CONTACT_BATCH; j++) {
         if (b0 + j >= nc) break;
         const Contact &k = kc[j];
-        V J[3];
-        Eval ev = evalContact<true>(k, basis, geo.w, A, B, alpha, isA, J);  // J[r] = r x l_r·sg
+        Eval ev = evalContact<true>(k, basis, geo.w, A, B, alpha, isA);  // ev.J[r] = r x l_r·sg
         if (d.bits) { uint32_t c = ids.z + b0 + j; if (isA) d.bits[c].x = evalBits(ev); else d.bits[c].y = evalBits(ev); }
-        addRow(acc, l0, J[0], k.pen.x, ev.F.x);
-        addRow(acc, l1, J[1], k.pen.y, ev.F.y);
-        addRow(acc, l2, J[2], k.pen.z, ev.F.z);
+        addRow(acc, l0, ev.J[0], k.pen.x, ev.F.x);
+        addRow(acc, l1, ev.J[1], k.pen.y, ev.F.y);
+        addRow(acc, l2, ev.J[2], k.pen.z, ev.F.z);
       }
     };
     batch(0);
