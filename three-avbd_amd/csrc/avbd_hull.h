@@ -573,18 +573,21 @@ __device__ __forceinline__ uint32_t wavePrefix2(uint32_t lane, uint32_t count, u
  *  as the serial arrays), so each pick is a (score, lane) argmax, lowest lane on ties. The contact
  *  is valid in lane 0. Every lane of the wavefront must take part. */
 __device__ __noinline__ void hullFaceClipWave(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, const Poly &R,
-                                                 const Poly &I, V rn, float pw, uint32_t inc, uint32_t lane, float4 *buf, Found &found) {
+                                                 const Poly &I, V rn, float pw, uint32_t inc, uint32_t lane, float4 *buf, const float4 *worldR,
+                                                 const float4 *worldI, Found &found) {
   found.count = 0;
   V zero = vv(0.0f, 0.0f, 0.0f);
   uint32_t n = faceVertCount(hulls, I, inc);
   if (n > MAX_CLIP) n = MAX_CLIP;
-  V p = lane < n ? vert(hulls, I, faceVert(hulls, I, inc, lane)) : zero;
+  // world vertices from the stage (vert()'s own arithmetic), one face corner per lane
+  V p = lane < n ? xyz(worldI[faceVert(hulls, I, inc, lane)]) : zero;
   uint32_t k = faceVertCount(hulls, R, refFace);
+  V corner = lane < k ? xyz(worldR[faceVert(hulls, R, refFace, lane)]) : zero;
   V centre = zero;
-  for (uint32_t j = 0; j < k; j++) centre = add(centre, vert(hulls, R, faceVert(hulls, R, refFace, j)));
+  for (uint32_t j = 0; j < k; j++) centre = add(centre, shflV(corner, (int)j));  // the serial sum's order
   centre = divs(centre, (float)k);
   for (uint32_t j = 0; j < k && n > 0; j++) {
-    V v0 = vert(hulls, R, faceVert(hulls, R, refFace, j)), v1 = vert(hulls, R, faceVert(hulls, R, refFace, (j + 1) % k));
+    V v0 = shflV(corner, (int)j), v1 = shflV(corner, (int)((j + 1) % k));
     V sn = cross(sub(v1, v0), rn);
     float l = length(sn);
     if (l < 1.0e-6f) continue;
@@ -644,17 +647,21 @@ __device__ __noinline__ void hullFaceClipWave(const uint4 *hulls, const Poly &A,
 }
 /** hullFaceContact with the deepest-vertex and incident-face searches spread over the lanes
  *  (first minimum, as the serial loops); the clip and selection too when the wavefront has a
- *  stage (`buf`), else lane 0 clips and selects. */
+ *  stage (its world vertices, and nB as the clip's buffer), else lane 0 clips and selects. */
 __device__ __forceinline__ void hullFaceContactWave(const uint4 *hulls, const Poly &A, const Poly &B, bool refIsA, uint32_t refFace, uint32_t lane,
-                                                    float4 *buf, Found &found) {
+                                                    HullStage *st, Found &found) {
   found.count = 0;
   const Poly &R = refIsA ? A : B;
   const Poly &I = refIsA ? B : A;
+  const float4 *worldR = st ? (refIsA ? st->wA : st->wB) : nullptr, *worldI = st ? (refIsA ? st->wB : st->wA) : nullptr;
   V rn;
   float pw;
   facePlane(hulls, R, refFace, rn, pw);
   ArgBest low{3.4e38f, 0xffffffffu};
-  for (uint32_t v = lane; v < I.nv; v += 64) { float dd = dot(vert(hulls, I, v), rn); if (dd < low.v) { low.v = dd; low.i = v; } }
+  for (uint32_t v = lane; v < I.nv; v += 64) {
+    float dd = dot(worldI ? xyz(worldI[v]) : vert(hulls, I, v), rn);
+    if (dd < low.v) { low.v = dd; low.i = v; }
+  }
   low = waveArgMin(low);
   uint32_t deepest = low.i == 0xffffffffu ? 0u : low.i;
   ArgBest least{3.4e38f, 0xffffffffu};
@@ -671,8 +678,8 @@ __device__ __forceinline__ void hullFaceContactWave(const uint4 *hulls, const Po
   }
   least = waveArgMin(least);
   uint32_t inc = least.i == 0xffffffffu ? 0u : least.i;
-  if (buf) {
-    hullFaceClipWave(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, lane, buf, found);
+  if (st) {
+    hullFaceClipWave(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, lane, st->nB, worldR, worldI, found);
     return;
   }
   if (lane != 0) return;
@@ -706,15 +713,17 @@ __device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &
   if (fbSep > faSep) {
     facePlane(hulls, B, fbFace, n, d);
     sat.n = neg(n);
-    hullFaceContactWave(hulls, A, B, false, fbFace, lane, staged ? st->nB : nullptr, found);
+    hullFaceContactWave(hulls, A, B, false, fbFace, lane, staged ? st : nullptr, found);
     return;
   }
   facePlane(hulls, A, faFace, n, d);
   sat.n = n;
-  hullFaceContactWave(hulls, A, B, true, faFace, lane, staged ? st->nB : nullptr, found);
+  hullFaceContactWave(hulls, A, B, true, faFace, lane, staged ? st : nullptr, found);
 }
 /** sphereHull with the faces spread over the wavefront (the result is valid in every lane). */
-__device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, float r, const Poly &X, uint32_t lane) {
+__device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, float r, const Poly &X, uint32_t lane, const float4 *world = nullptr) {
+  // a corner of face f: from X's staged world vertices when there are (vert()'s own arithmetic)
+  auto corner = [&](uint32_t f, uint32_t j) { uint32_t v = faceVert(hulls, X, f, j); return world ? xyz(world[v]) : vert(hulls, X, v); };
   SphereHit out;
   out.hit = false;
   ArgBest deep{-3.4e38f, 0xffffffffu};
@@ -750,13 +759,13 @@ __device__ __forceinline__ SphereHit sphereHullWave(const uint4 *hulls, V c, flo
     V p = sub(c, mul(pn, s));
     uint32_t k = faceVertCount(hulls, X, f);
     V centre = vv(0.0f, 0.0f, 0.0f);
-    for (uint32_t j = 0; j < k; j++) centre = add(centre, vert(hulls, X, faceVert(hulls, X, f, j)));
+    for (uint32_t j = 0; j < k; j++) centre = add(centre, corner(f, j));
     centre = divs(centre, (float)k);
     bool inside = true;
     V edgeBest = vv(0.0f, 0.0f, 0.0f);
     float edgeDist = 3.4e38f;
     for (uint32_t j = 0; j < k; j++) {
-      V v0 = vert(hulls, X, faceVert(hulls, X, f, j)), v1 = vert(hulls, X, faceVert(hulls, X, f, (j + 1) % k));
+      V v0 = corner(f, j), v1 = corner(f, (j + 1) % k);
       V side = cross(sub(v1, v0), pn);
       if (dot(side, sub(centre, v0)) > 0.0f) side = neg(side);
       if (dot(side, sub(p, v0)) > 0.0f) inside = false;
@@ -787,12 +796,20 @@ __device__ __forceinline__ void collideHullPairWave(const uint4 *hulls, const Bo
                                                     bool sphereA, bool sphereB, bool faceBias, uint32_t lane, HullStage *st, Sat &sat, Found &found) {
   found.count = 0;
   if (sphereA || sphereB) {
+    Poly X = sphereA ? polyOf(hulls, B.c, qB, B.h, shB) : polyOf(hulls, A.c, qA, A.h, shA);
+    const float4 *world = nullptr;
+    if (st && X.nv <= STAGE_V) {  // the hull's world vertices into the stage, once for all its faces
+      waveSync();                  // the previous pair's readers are done
+      for (uint32_t v = lane; v < X.nv; v += 64) st->wA[v] = f4(vert(hulls, X, v), 0.0f);
+      waveSync();
+      world = st->wA;
+    }
     if (sphereA) {
-      SphereHit h = sphereHullWave(hulls, A.c, A.h.x, polyOf(hulls, B.c, qB, B.h, shB), lane);
+      SphereHit h = sphereHullWave(hulls, A.c, A.h.x, X, lane, world);
       sat.n = h.n;
       if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
     } else {
-      SphereHit h = sphereHullWave(hulls, B.c, B.h.x, polyOf(hulls, A.c, qA, A.h, shA), lane);
+      SphereHit h = sphereHullWave(hulls, B.c, B.h.x, X, lane, world);
       sat.n = neg(h.n);
       if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
     }
