@@ -130,6 +130,7 @@ struct avbd_solver {
   uint4 *mIds[2] = {nullptr, nullptr};
   float4 *mGeo[2] = {nullptr, nullptr};
   Contact *con[2] = {nullptr, nullptr};
+  uint32_t *cman = nullptr;  // per contact of the step, its manifold
   uint32_t *table = nullptr;
   uint32_t *adjCount = nullptr, *adjStart = nullptr, *adjFill = nullptr, *adjList = nullptr;
   uint32_t *adjTmp = nullptr, *adjOwner = nullptr;  // the fill's unsorted entries and their bodies
@@ -309,6 +310,7 @@ struct avbd_solver {
       mGeo[k] = regrow(mGeo[k], manCap, nMan, true);
       con[k] = regrow(con[k], conCap, nCon, true);
     }
+    cman = regrow(cman, conCap, nCon, false);
     uint32_t hs = pow2At(2ull * nMan), ac = 2u * (jointCap + nMan);
     table = regrow(table, hashSize, hs, false);
     adjList = regrow(adjList, adjCap, ac, false);
@@ -341,7 +343,7 @@ struct avbd_solver {
     d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
     d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
-    d.con = con[parity]; d.pcon = con[1 - parity]; d.manCap = manCap; d.conCap = conCap;
+    d.con = con[parity]; d.pcon = con[1 - parity]; d.cman = cman; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
     d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
     d.adjTmp = adjTmp; d.adjOwner = adjOwner; d.adjKeys = adjKeys;
@@ -510,7 +512,7 @@ struct avbd_solver {
         }
         timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
       }
-      timed(K_DUAL, [&] { k_dual<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d); });
+      timed(K_DUAL, [&] { k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d); });
       if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, it + 1);
     }
     k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
@@ -848,6 +850,7 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
       s->mGeo[k] = s->alloc<float4>(s->manCap);
       s->con[k] = s->alloc<Contact>(s->conCap);
     }
+    s->cman = s->alloc<uint32_t>(s->conCap);
     s->table = s->alloc<uint32_t>(s->hashSize);
     s->adjCount = s->alloc<uint32_t>(N + 1);
     s->adjStart = s->alloc<uint32_t>(N + 1);

@@ -39,6 +39,7 @@ struct Dev {
   uint4 *mIds, *pmIds;
   float4 *mGeo, *pmGeo;
   Contact *con, *pcon;
+  uint32_t *cman;  // per contact of this step, its manifold (the dual's thread-per-contact map)
   int *src;  // when recording: per contact, the previous step's contact it warm-started from (-1: none)
   uint2 *hullPairs;  // pairs with a hull on either side, for the wavefront-per-pair narrowphase (null: inline)
   uchar4 *bits;  // when recording: this iteration's branch bits per contact (x: body a's update, y: b's, z: dual)
@@ -426,6 +427,7 @@ __device__ __forceinline__ void copyContactsAt(const Dev &d, uint4 pids, float4 
     k.lam = f4(lam, k.lam.w);
     k.pen = f4(pen, k.pen.w);
     d.con[base + i] = k;
+    d.cman[base + i] = m;
     if (d.src) d.src[base + i] = (int)(pids.z + i);
   }
 }
@@ -518,6 +520,7 @@ __device__ __forceinline__ void emitManifold(const Dev &d, uint32_t a, uint32_t 
     k.pen = f4(pen, dot(basis[1], dd));
     k.lam = f4(lam, dot(basis[2], dd));
     d.con[base + i] = k;
+    d.cman[base + i] = m;
     if (d.src) d.src[base + i] = j;
   }
 }
@@ -1142,42 +1145,53 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
   }
   k[0] = penLin; k[1] = penAng; k[2] = lamLin; k[3] = lamAng;
 }
-/** dualManifold (wgsl-solve.ts:637-662): λ, penalty ramp, stick bit; writes only what changes. */
-__device__ __forceinline__ void dualManifold(const Dev &d, uint32_t m, float alpha) {
+/** dualManifold's update of one contact point c (wgsl-solve.ts:637-662), given its manifold's basis,
+ *  friction and both bodies' terms: penalty ramp, stick flag, λ; the whole 64-byte record written
+ *  back (anchors as read: a full-line write, no partial-line merge). */
+__device__ __forceinline__ void dualContact(const Dev &d, uint32_t c, const V *basis, float friction, const PairBody &A, const PairBody &B,
+                                            float alpha) {
+  Contact k = d.con[c];
+  Eval e = evalContact(k, basis, friction, A, B, alpha);
+  if (d.bits) d.bits[c].z = evalBits(e);
+  V pen = xyz(k.pen);
+  float BL = d.p.betaLin;
+  if (e.F.x < 0.0f) pen.x = mn(pen.x + BL * fabsf(e.C.x), PENALTY_MAX);
+  uint32_t key = __float_as_uint(k.rA.w);
+  if (e.frictionScale <= e.bounds) {
+    pen.y = mn(pen.y + BL * fabsf(e.C.y), PENALTY_MAX);
+    pen.z = mn(pen.z + BL * fabsf(e.C.z), PENALTY_MAX);
+    bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < 0.00001f;
+    key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+  }
+  Contact out;
+  out.rA = f4(xyz(k.rA), __uint_as_float(key));
+  out.rB = k.rB;
+  out.pen = f4(pen, k.pen.w);
+  out.lam = f4(e.F, k.lam.w);
+  d.con[c] = out;
+}
+/** dualManifold (wgsl-solve.ts:637-662) for contact point c alone: every point's update is its own,
+ *  so a thread per point gives the manifold loop's results while no wavefront waits on its widest
+ *  manifold. The point's manifold comes from the map the contact's writer left (cman); a slot no
+ *  manifold of this step covers (dropped on overflow) is skipped. */
+__device__ __forceinline__ void dualPoint(const Dev &d, uint32_t c, float alpha) {
+  uint32_t m = d.cman[c];
+  if (m >= manifoldCount(d)) return;
   uint4 ids = d.mIds[m];
+  if (c < ids.z || c >= ids.z + (ids.w & 15u)) return;
   float4 geo = d.mGeo[m];
   PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
   V basis[3];
   orthonormal(xyz(geo), basis);
-  uint32_t end = ids.z + (ids.w & 15u);
-  for (uint32_t c = ids.z; c < end; c++) {
-    Contact k = d.con[c];
-    Eval e = evalContact(k, basis, geo.w, A, B, alpha);
-    if (d.bits) d.bits[c].z = evalBits(e);
-    V pen = xyz(k.pen);
-    float BL = d.p.betaLin;
-    if (e.F.x < 0.0f) pen.x = mn(pen.x + BL * fabsf(e.C.x), PENALTY_MAX);
-    uint32_t key = __float_as_uint(k.rA.w);
-    if (e.frictionScale <= e.bounds) {
-      pen.y = mn(pen.y + BL * fabsf(e.C.y), PENALTY_MAX);
-      pen.z = mn(pen.z + BL * fabsf(e.C.z), PENALTY_MAX);
-      bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < 0.00001f;
-      key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
-    }
-    // the whole 64-byte record (anchors as read): a full-line write, no partial-line merge
-    Contact out;
-    out.rA = f4(xyz(k.rA), __uint_as_float(key));
-    out.rB = k.rB;
-    out.pen = f4(pen, k.pen.w);
-    out.lam = f4(e.F, k.lam.w);
-    d.con[c] = out;
-  }
+  dualContact(d, c, basis, geo.w, A, B, alpha);
 }
 __device__ __forceinline__ void dualAll(const Dev &d, uint32_t t0, uint32_t nt) {
-  uint32_t total = d.nj + manifoldCount(d);
+  uint32_t points = d.counters[C_CONTACTS];
+  if (points > d.conCap) points = d.conCap;
+  uint32_t total = d.nj + points;
   for (uint32_t id = t0; id < total; id += nt) {
     if (id < d.nj) dualJoint(d, id, d.p.alpha);
-    else dualManifold(d, id - d.nj, d.p.alpha);
+    else dualPoint(d, id - d.nj, d.p.alpha);
   }
 }
 __global__ void __launch_bounds__(256) k_dual(Dev d) { dualAll(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
