@@ -82,7 +82,9 @@ static const float BIG = 3.0e38f, HARD = 1e30f, PENALTY_MIN = 1.0f, PENALTY_MAX 
 enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASHES = 4, C_NUM_COLORS = 5,
        C_MANIFOLDS = 6, C_PREV_MANIFOLDS = 7, C_HULL_PAIRS = 8, C_CLASHING = 9,
        // since the host last cleared them: overflow bits | 0x10000 on clashes, high-water counts
-       C_STICKY = 11, C_MAX_CONTACTS = 12, C_MAX_MANIFOLDS = 13, C_WORDS = 16 };
+       C_STICKY = 11, C_MAX_CONTACTS = 12, C_MAX_MANIFOLDS = 13,
+       C_PREV_OVERFLOW = 14,  // the last step's overflow bits (k_begin)
+       C_WORDS = 16 };
 
 /** One contact point: 4 float4, the w's carrying the key and C(x-) (layout.ts:123-127). */
 struct __align__(16) Contact { float4 rA; float4 rB; float4 pen; float4 lam; };

@@ -86,6 +86,7 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   c[C_MANIFOLDS] = 0;
   c[C_CONTACTS] = 0;
   if (!reuseManifolds) c[C_PAIRS] = 0;
+  c[C_PREV_OVERFLOW] = c[C_OVERFLOW];
   c[C_OVERFLOW] = 0;
   c[C_HULL_PAIRS] = 0;
   c[C_CLASHES] = 0;
@@ -440,19 +441,29 @@ __device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pg
   uint32_t base = atomicAdd(&d.counters[C_CONTACTS], pids.w & 15u);
   copyContactsAt(d, pids, pgeo, m, base);
 }
-/** reuseManifolds (wgsl-collision.ts:757-764). Each kept manifold's slot and contact range come
- *  from one atomic per wavefront (waveAlloc):
+/** reuseManifolds (wgsl-collision.ts:757-764). When the last step left no manifold empty (no contact
+ *  overflow: a manifold is only emitted or copied with points), every previous manifold is kept, so
+ *  manifold m keeps slot m and its contact range, and the counts are the last step's: no atomics
+ *  (same-address atomics from every wavefront cost ~90 us a launch at 100k). Otherwise each kept
+ *  manifold's slot and contact range come from one atomic per wavefront (waveAlloc):
  *  which slot a manifold lands in is arbitrary either way (one atomic each lands them in
  *  arrival order), and no later pass depends on it. */
 __global__ void __launch_bounds__(256) k_reuse_manifolds(Dev d) {
   uint32_t count = prevManifoldCount(d);
+  bool keepAll = (d.counters[C_PREV_OVERFLOW] & 2u) == 0u;
   uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
+  if (keepAll && first == 0 && threadIdx.x == 0) {
+    d.counters[C_MANIFOLDS] = count;
+    d.counters[C_CONTACTS] = d.counters[C_PREV_CONTACTS];
+  }
   for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
     uint32_t m = w + (threadIdx.x & 63u);
     uint4 ids = m < count ? d.pmIds[m] : make_uint4(0u, 0u, 0u, 0u);
-    uint32_t cnt = ids.w & 15u;
-    uint32_t slot = waveAlloc(&d.counters[C_MANIFOLDS], cnt > 0 ? 1u : 0u);
-    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], cnt > 0 && slot < d.manCap ? cnt : 0u);
+    uint32_t cnt = ids.w & 15u, slot = m, base = ids.z;
+    if (!keepAll) {
+      slot = waveAlloc(&d.counters[C_MANIFOLDS], cnt > 0 ? 1u : 0u);
+      base = waveAlloc(&d.counters[C_CONTACTS], cnt > 0 && slot < d.manCap ? cnt : 0u);
+    }
     if (cnt > 0) copyContactsAt(d, ids, d.pmGeo[m], slot, base);
   }
 }
@@ -746,9 +757,13 @@ __global__ void __launch_bounds__(256) k_color_count(Dev d) {
     if (s & PENDING) atomicAdd(&d.counters[C_CLASHES], 1u);
     if (col == NO_COLOR) col = d.p.colorCap - 1;
     d.colA[b] = col | (s & PENDING);
-    atomicMax(&d.counters[C_NUM_COLORS], col + 1);
     if (s & PENDING) atomicOr(&d.counters[C_CLASHING + (col >> 5)], 1u << (col & 31u));
   }
+  // colours in use: a wavefront's maximum, one atomic per wavefront (not one per body)
+  uint32_t top = col == NO_COLOR ? 0u : col + 1;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) { uint32_t other = (uint32_t)__shfl_xor((int)top, o); top = other > top ? other : top; }
+  if ((tid & 63u) == 0 && top) atomicMax(&d.counters[C_NUM_COLORS], top);
   chunk[tid] = col;
   __syncthreads();
   if (col != NO_COLOR) {
