@@ -433,14 +433,6 @@ __device__ __forceinline__ void copyContactsAt(const Dev &d, uint4 pids, float4 
   }
 }
 /** reuseManifolds (wgsl-collision.ts:758): substeps after the first keep last substep's points. */
-/** copyContacts (wgsl-collision.ts:721-755): a manifold slot, then its contact range, one atomic
- *  each (for callers whose lanes diverge). */
-__device__ __forceinline__ void copyContacts(const Dev &d, uint4 pids, float4 pgeo) {
-  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
-  if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
-  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], pids.w & 15u);
-  copyContactsAt(d, pids, pgeo, m, base);
-}
 /** reuseManifolds (wgsl-collision.ts:757-764). When the last step left no manifold empty (no contact
  *  overflow: a manifold is only emitted or copied with points), every previous manifold is kept, so
  *  manifold m keeps slot m and its contact range, and the counts are the last step's: no atomics
@@ -467,26 +459,27 @@ __global__ void __launch_bounds__(256) k_reuse_manifolds(Dev d) {
     if (cnt > 0) copyContactsAt(d, ids, d.pmGeo[m], slot, base);
   }
 }
-__device__ __forceinline__ bool reuseContacts(const Dev &d, uint32_t a, uint32_t b) {
+/** reuseContacts' test (wgsl-collision.ts:706-719): the previous step's manifold of pair (a, b)
+ *  when its points were computed no earlier than both bodies last moved beyond the reuse
+ *  tolerance, else -1. */
+__device__ __forceinline__ int reusableManifold(const Dev &d, uint32_t a, uint32_t b) {
   int pm = hashFind(d, a, b);
-  if (pm < 0) return false;
+  if (pm < 0) return -1;
   uint4 ids = d.pmIds[pm];
   uint32_t count = ids.w & 15u, generated = ids.w >> 4;
   uint32_t sa = __float_as_uint(d.b.inpos[a].w), sb = __float_as_uint(d.b.inpos[b].w);
   uint32_t moved = sa > sb ? sa : sb;
-  if (count == 0 || generated < moved) return false;
-  copyContacts(d, ids, d.pmGeo[pm]);
-  return true;
+  if (count == 0 || generated < moved) return -1;
+  return pm;
 }
 
 /** The manifold of pair (a, b) from its contact points (wgsl-collision.ts:685-757): slot, normal
  *  basis, per point the body-local anchors, warm start from last step's contact with the same
  *  feature (or the nearest anchor), C0 and the decayed λ/penalty. */
-__device__ __forceinline__ void emitManifold(const Dev &d, uint32_t a, uint32_t b, const Box &A, const Box &B, Q qA, Q qB, float4 posA,
-                                             float4 posB, bool anySphere, const Sat &sat, const Found &found) {
-  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
+/** emitManifold with the manifold slot m and contact range base already allocated. */
+__device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_t base, uint32_t a, uint32_t b, const Box &A, const Box &B, Q qA,
+                                               Q qB, float4 posA, float4 posB, bool anySphere, const Sat &sat, const Found &found) {
   if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
-  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], found.count);
   uint32_t cnt = found.count;
   if (base + cnt > d.conCap) { atomicOr(&d.counters[C_OVERFLOW], 2u); cnt = 0; }
   V n = neg(sat.n);
@@ -535,45 +528,75 @@ __device__ __forceinline__ void emitManifold(const Dev &d, uint32_t a, uint32_t 
     if (d.src) d.src[base + i] = j;
   }
 }
+/** emitManifold (wgsl-collision.ts:685-757): a manifold slot, then its contact range, one atomic
+ *  each (the hull kernel's one emitting lane per pair). */
+__device__ __forceinline__ void emitManifold(const Dev &d, uint32_t a, uint32_t b, const Box &A, const Box &B, Q qA, Q qB, float4 posA,
+                                             float4 posB, bool anySphere, const Sat &sat, const Found &found) {
+  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
+  if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
+  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], found.count);
+  emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, anySphere, sat, found);
+}
 
 /** Narrowphase of every broadphase pair, one thread per pair; pairs with a hull on either side
  *  go to the hull list for k_narrow_hull (a wavefront per pair) when it is given. */
+/** narrowphase (wgsl-collision.ts:767): per candidate pair, kept points (contact reuse), a
+ *  deferral to the hull kernel, or a fresh manifold. The slots of all three come from one atomic
+ *  per wavefront each (waveAlloc) at a point every lane reaches: same-address atomics per lane
+ *  serialise across the chip. Which slot a pair lands in is arbitrary either way. */
 __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
   uint32_t count = d.counters[C_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
-  GRID_STRIDE(p, count) {
-    uint2 pr = d.pairs[p];
+  bool reuseOn = (d.p.flags & FLAG_REUSE_CONTACTS) != 0;
+  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
+  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
+    uint32_t p = w + (threadIdx.x & 63u);
+    bool live = p < count;
+    uint2 pr = live ? d.pairs[p] : make_uint2(0u, 0u);
     uint32_t a = pr.x, b = pr.y;
-    if ((d.p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(d, a, b)) continue;
-    float4 posA = d.b.pos[a], posB = d.b.pos[b];
-    float shA = d.b.angvel[a].w, shB = d.b.angvel[b].w;
-    bool sphereA = shA == SHAPE_SPHERE, sphereB = shB == SHAPE_SPHERE;
-    bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
-    if ((hullA || hullB) && d.hullPairs) {
-      d.hullPairs[atomicAdd(&d.counters[C_HULL_PAIRS], 1u)] = pr;
-      continue;
-    }
-    Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
-    Box A = makeBox(xyz(posA), qA, xyz(d.b.size[a])), B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
+    int pm = live && reuseOn ? reusableManifold(d, a, b) : -1;
+    bool defer = false, sphereA = false, sphereB = false;
+    float4 posA = make_float4(0.0f, 0.0f, 0.0f, 0.0f), posB = posA;
+    Q qA = qq(0.0f, 0.0f, 0.0f, 1.0f), qB = qA;
+    Box A, B;
     Sat sat;
     sat.n = vv(0.0f, 0.0f, 0.0f);
     Found found;
     found.count = 0;
-    if (sphereA && sphereB) sphereSphere(A, B, sat, found);
-    else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
-    else if (sphereA || sphereB) {
-      if (sphereA) {
-        SphereHit h = sphereBox(A, B, qB);
-        sat.n = h.n;
-        if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+    if (live && pm < 0) {
+      posA = d.b.pos[a]; posB = d.b.pos[b];
+      float shA = d.b.angvel[a].w, shB = d.b.angvel[b].w;
+      sphereA = shA == SHAPE_SPHERE; sphereB = shB == SHAPE_SPHERE;
+      bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
+      if ((hullA || hullB) && d.hullPairs) {
+        defer = true;
       } else {
-        SphereHit h = sphereBox(B, A, qA);
-        sat.n = neg(h.n);
-        if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+        qA = q4(d.b.rot[a]); qB = q4(d.b.rot[b]);
+        A = makeBox(xyz(posA), qA, xyz(d.b.size[a])); B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
+        if (sphereA && sphereB) sphereSphere(A, B, sat, found);
+        else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
+        else if (sphereA || sphereB) {
+          if (sphereA) {
+            SphereHit h = sphereBox(A, B, qB);
+            sat.n = h.n;
+            if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+          } else {
+            SphereHit h = sphereBox(B, A, qA);
+            sat.n = neg(h.n);
+            if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+          }
+        } else collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
       }
-    } else collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
-    if (found.count == 0) continue;
-    emitManifold(d, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
+    }
+    uint4 pids = pm >= 0 ? d.pmIds[pm] : make_uint4(0u, 0u, 0u, 0u);
+    bool wants = pm >= 0 || found.count > 0;
+    uint32_t points = pm >= 0 ? (pids.w & 15u) : found.count;
+    uint32_t hslot = waveAlloc(&d.counters[C_HULL_PAIRS], defer ? 1u : 0u);
+    uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], wants ? 1u : 0u);
+    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], wants && m < d.manCap ? points : 0u);
+    if (defer) d.hullPairs[hslot] = pr;
+    if (pm >= 0) copyContactsAt(d, pids, d.pmGeo[pm], m, base);
+    else if (found.count > 0) emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
   }
 }
 
