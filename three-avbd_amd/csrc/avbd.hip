@@ -127,6 +127,7 @@ struct avbd_solver {
   int4 *sortedKey = nullptr;
   uint2 *pairs = nullptr;
   uint2 *hullPairs = nullptr;  // the narrowphase's deferred hull pairs
+  float4 *hullOut = nullptr;   // and their contacts for the emit pass
   uint4 *mIds[2] = {nullptr, nullptr};
   float4 *mGeo[2] = {nullptr, nullptr};
   Contact *con[2] = {nullptr, nullptr};
@@ -305,6 +306,7 @@ struct avbd_solver {
     fwdWhy = "the solver's storage grew after avbd_forward";
     pairs = regrow(pairs, pairCap, nPairs, false);
     if (hullPairs) hullPairs = regrow(hullPairs, pairCap, nPairs, false);
+    if (hullOut) hullOut = regrow(hullOut, (size_t)HULL_OUT * pairCap, (size_t)HULL_OUT * nPairs, false);
     for (int k = 0; k < 2; k++) {
       mIds[k] = regrow(mIds[k], manCap, nMan, true);
       mGeo[k] = regrow(mGeo[k], manCap, nMan, true);
@@ -341,7 +343,7 @@ struct avbd_solver {
     d.worlds = (uint32_t)worldStartHost.size() - 1;
     d.cellSize = cellSize; d.tableMask = tableSize - 1;
     d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
-    d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs;
+    d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs; d.hullOut = hullOut;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
     d.con = con[parity]; d.pcon = con[1 - parity]; d.cman = cman; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
@@ -453,7 +455,10 @@ struct avbd_solver {
     });
     timed(K_NARROW, [&] {
       k_narrowphase<<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
-      if (hullPairs) k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
+      if (hullPairs) {
+        k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
+        k_hull_emit<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);
+      }
     });
   }
   void topology() {
@@ -844,7 +849,10 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->sortedPos = s->alloc<float4>(N);
     s->sortedKey = s->alloc<int4>(N);
     s->pairs = s->alloc<uint2>(s->pairCap);
-    if (getenv("AVBD_HULL_INLINE") == nullptr) s->hullPairs = s->alloc<uint2>(s->pairCap);
+    if (getenv("AVBD_HULL_INLINE") == nullptr) {
+      s->hullPairs = s->alloc<uint2>(s->pairCap);
+      s->hullOut = s->alloc<float4>((size_t)HULL_OUT * s->pairCap);
+    }
     for (int k = 0; k < 2; k++) {
       s->mIds[k] = s->alloc<uint4>(s->manCap);
       s->mGeo[k] = s->alloc<float4>(s->manCap);

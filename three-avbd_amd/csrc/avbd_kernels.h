@@ -42,6 +42,7 @@ struct Dev {
   uint32_t *cman;  // per contact of this step, its manifold (the dual's thread-per-contact map)
   int *src;  // when recording: per contact, the previous step's contact it warm-started from (-1: none)
   uint2 *hullPairs;  // pairs with a hull on either side, for the wavefront-per-pair narrowphase (null: inline)
+  float4 *hullOut;   // per hull pair, its contact for the emit pass (HULL_OUT float4)
   uchar4 *bits;  // when recording: this iteration's branch bits per contact (x: body a's update, y: b's, z: dual)
   uint32_t manCap, conCap;
   uint32_t *table;
@@ -602,6 +603,9 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
 
 /** The pairs the narrowphase deferred (a hull on either side), one 64-wide wavefront per pair:
  *  the SAT's face and edge-pair queries are spread over the lanes, lane 0 clips and emits. */
+/** float4 per hull pair in hullOut: sat.n and the point count, then per point xA with the feature
+ *  bits, then per point xB. */
+static const uint32_t HULL_OUT = 17;
 __global__ void __launch_bounds__(256) k_narrow_hull(Dev d) {
   uint32_t count = d.counters[C_HULL_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
@@ -622,7 +626,49 @@ __global__ void __launch_bounds__(256) k_narrow_hull(Dev d) {
     Found found;
     found.count = 0;
     collideHullPairWave(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, lane, st, sat, found);
-    if (lane == 0 && found.count > 0) emitManifold(d, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
+    if (lane == 0) {  // the contact, for k_hull_emit (no allocation here: see there)
+      float4 *o = d.hullOut + (size_t)p * HULL_OUT;
+      o[0] = f4(sat.n, __uint_as_float(found.count));
+      for (uint32_t i = 0; i < found.count; i++) {
+        o[1 + i] = f4(found.xA[i], __uint_as_float(found.feature[i]));
+        o[9 + i] = f4(found.xB[i], 0.0f);
+      }
+    }
+  }
+}
+/** The hull pairs' manifolds (emitManifold, wgsl-collision.ts:685-757), a thread per pair: the boxes
+ *  rebuilt from the same body data as in k_narrow_hull, the contact as it left it. Slots come
+ *  from one atomic per wavefront each (waveAlloc): an atomic per pair from the pair's wavefront
+ *  cost about a quarter of the hull kernel. */
+__global__ void __launch_bounds__(256) k_hull_emit(Dev d) {
+  uint32_t count = d.counters[C_HULL_PAIRS];
+  if (count > d.pairCap) count = d.pairCap;
+  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
+  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
+    uint32_t p = w + (threadIdx.x & 63u);
+    const float4 *o = d.hullOut + (size_t)p * HULL_OUT;
+    float4 h0 = p < count ? o[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t points = p < count ? __float_as_uint(h0.w) : 0u;
+    uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], points > 0 ? 1u : 0u);
+    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], points > 0 && m < d.manCap ? points : 0u);
+    if (points == 0) continue;
+    uint2 pr = d.hullPairs[p];
+    uint32_t a = pr.x, b = pr.y;
+    float4 posA = d.b.pos[a], posB = d.b.pos[b];
+    Q qA = q4(d.b.rot[a]), qB = q4(d.b.rot[b]);
+    bool anySphere = d.b.angvel[a].w == SHAPE_SPHERE || d.b.angvel[b].w == SHAPE_SPHERE;
+    Box A = makeBox(xyz(posA), qA, xyz(d.b.size[a])), B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
+    Sat sat;
+    sat.n = xyz(h0);
+    Found found;
+    found.count = points;
+    for (uint32_t i = 0; i < points; i++) {
+      float4 x = o[1 + i];
+      found.xA[i] = xyz(x);
+      found.feature[i] = __float_as_uint(x.w);
+      found.xB[i] = xyz(o[9 + i]);
+    }
+    emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, anySphere, sat, found);
   }
 }
 
