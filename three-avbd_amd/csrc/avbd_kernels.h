@@ -474,10 +474,10 @@ __device__ __forceinline__ int reusableManifold(const Dev &d, uint32_t a, uint32
   return pm;
 }
 
-/** The manifold of pair (a, b) from its contact points (wgsl-collision.ts:685-757): slot, normal
- *  basis, per point the body-local anchors, warm start from last step's contact with the same
- *  feature (or the nearest anchor), C0 and the decayed λ/penalty. */
-/** emitManifold with the manifold slot m and contact range base already allocated. */
+/** emitManifold (wgsl-collision.ts:685-757), the manifold of pair (a, b) from its contact points,
+ *  given its slot m and contact range base (allocated by the caller, one atomic per wavefront):
+ *  normal basis, per point the body-local anchors, warm start from last step's contact with the
+ *  same feature (or the nearest anchor), C0 and the decayed λ/penalty. */
 __device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_t base, uint32_t a, uint32_t b, const Box &A, const Box &B, Q qA,
                                                Q qB, float4 posA, float4 posB, bool anySphere, const Sat &sat, const Found &found) {
   if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
@@ -528,15 +528,6 @@ __device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_
     d.cman[base + i] = m;
     if (d.src) d.src[base + i] = j;
   }
-}
-/** emitManifold (wgsl-collision.ts:685-757): a manifold slot, then its contact range, one atomic
- *  each (the hull kernel's one emitting lane per pair). */
-__device__ __forceinline__ void emitManifold(const Dev &d, uint32_t a, uint32_t b, const Box &A, const Box &B, Q qA, Q qB, float4 posA,
-                                             float4 posB, bool anySphere, const Sat &sat, const Found &found) {
-  uint32_t m = atomicAdd(&d.counters[C_MANIFOLDS], 1u);
-  if (m >= d.manCap) { atomicOr(&d.counters[C_OVERFLOW], 4u); return; }
-  uint32_t base = atomicAdd(&d.counters[C_CONTACTS], found.count);
-  emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, anySphere, sat, found);
 }
 
 /** Narrowphase of every broadphase pair, one thread per pair; pairs with a hull on either side
