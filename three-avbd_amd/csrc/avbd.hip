@@ -136,6 +136,8 @@ struct avbd_solver {
   uint32_t *adjCount = nullptr, *adjStart = nullptr, *adjFill = nullptr, *adjList = nullptr;
   uint32_t *adjTmp = nullptr, *adjOwner = nullptr;  // the fill's unsorted entries and their bodies
   uint2 *adjKeys = nullptr;                          // and their sort keys
+  int *adjOther = nullptr;                           // per sorted entry, the dynamic neighbour or -1
+  uint32_t *prio = nullptr;                          // per body, its colouring priority
   uint4 *adjIds = nullptr, *colorItems = nullptr;
   float4 *adjGeo = nullptr;
   uint32_t *colA = nullptr, *colB = nullptr, *colorStart = nullptr, *colorBodies = nullptr, *colorRank = nullptr, *hist = nullptr;
@@ -319,6 +321,7 @@ struct avbd_solver {
     adjTmp = regrow(adjTmp, adjCap, ac, false);
     adjOwner = regrow(adjOwner, adjCap, ac, false);
     adjKeys = regrow(adjKeys, adjCap, ac, false);
+    adjOther = regrow(adjOther, adjCap, ac, false);
     adjIds = regrow(adjIds, adjCap, ac, false);
     adjGeo = regrow(adjGeo, adjCap, ac, false);
     pairCap = nPairs; manCap = nMan; conCap = nCon; hashSize = hs; adjCap = ac;
@@ -348,7 +351,7 @@ struct avbd_solver {
     d.con = con[parity]; d.pcon = con[1 - parity]; d.cman = cman; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
     d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
-    d.adjTmp = adjTmp; d.adjOwner = adjOwner; d.adjKeys = adjKeys;
+    d.adjTmp = adjTmp; d.adjOwner = adjOwner; d.adjKeys = adjKeys; d.adjOther = adjOther; d.prio = prio;
     d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
@@ -867,6 +870,7 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->adjTmp = s->alloc<uint32_t>(s->adjCap);
     s->adjOwner = s->alloc<uint32_t>(s->adjCap);
     s->adjKeys = s->alloc<uint2>(s->adjCap);
+    s->adjOther = s->alloc<int>(s->adjCap);
     s->adjIds = s->alloc<uint4>(s->adjCap);
     s->adjGeo = s->alloc<float4>(s->adjCap);
     s->colA = s->alloc<uint32_t>(N);
@@ -890,6 +894,7 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
       hipMemcpyAsync(s->worldStart, world_start, (worlds + 1) * 4ull, hipMemcpyHostToDevice, s->stream);
       if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error during create");
     }
+    s->prio = s->alloc<uint32_t>(N);
     s->scanSums = s->alloc<uint32_t>(4096);
     s->barrier = s->alloc<uint32_t>(64);
     s->fill(s->colA, N, NO_COLOR);
@@ -912,6 +917,10 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
       hipMemcpyAsync(s->jinfo, s->jinfoHost.data(), s->jinfoHost.size() * 4, hipMemcpyHostToDevice, s->stream);
     }
     s->uploadStatics(radius, mass);
+    {
+      Dev d = s->dev();
+      k_priorities<<<blocksFor(s->n, 256), 256, 0, s->stream>>>(d);  // the colouring's fixed priorities
+    }
     if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error during create");
   } catch (const std::exception &e) {
     g_create_error = e.what();

@@ -51,6 +51,8 @@ struct Dev {
   uint32_t *adjCount, *adjStart, *adjFill, *adjList;
   uint32_t *adjTmp, *adjOwner;  // the fill's unsorted entries and their bodies
   uint2 *adjKeys;               // and their sort keys
+  int *adjOther;                // per sorted entry, its other body when dynamic, else -1 (the colouring's neighbour)
+  uint32_t *prio;               // per body, its colouring priority (fixed at creation)
   uint32_t adjCap;
   // the solve's flattened views: per adjacency entry its manifold record (ids.x = 0xffffffff marks
   // a joint, id in ids.y), per colour-bucket slot its body, adjacency range and whether it is enabled
@@ -727,12 +729,18 @@ __global__ void k_sort_adj(Dev d) {
     for (uint32_t k = lo; k < hi; k++) rank += keyLess(d.adjKeys[k], key) ? 1u : 0u;
     uint32_t id = d.adjTmp[e], at = lo + rank;
     d.adjList[at] = id;
+    int other;
     if (id < d.nj) {
       d.adjIds[at] = make_uint4(0xffffffffu, id, 0u, 0u);
+      int4 in = d.jinfo[id];
+      other = in.y == (int)b ? in.z : in.y;
     } else {
-      d.adjIds[at] = d.mIds[id - d.nj];
+      uint4 ids = d.mIds[id - d.nj];
+      d.adjIds[at] = ids;
       d.adjGeo[at] = d.mGeo[id - d.nj];
+      other = ids.x == b ? (int)ids.y : (int)ids.x;
     }
+    d.adjOther[at] = isDynamic(d, other) ? other : -1;
   }
 }
 
@@ -740,16 +748,15 @@ __global__ void k_sort_adj(Dev d) {
 // Colouring: incremental Jones-Plassmann with hashed priorities (wgsl-topology.ts:155-262).
 /** lowbias32 priority of a body's index within its world (its global index in a single world). */
 __device__ __forceinline__ uint32_t priority(const Dev &d, uint32_t i) { return hash32(i - d.worldStart[d.worldOf[i]] + 0x9e3779b9u); }
+/** Every body's priority, once at creation (bodies and worlds are fixed). */
+__global__ void k_priorities(Dev d) { GRID_STRIDE(i, d.n) d.prio[i] = priority(d, i); }
 /** i beats j (always the same world: ties by index there, which the global index orders alike). */
 __device__ __forceinline__ bool beats(const Dev &d, uint32_t i, uint32_t j) {
-  uint32_t pi = priority(d, i), pj = priority(d, j);
+  uint32_t pi = d.prio[i], pj = d.prio[j];
   return pi > pj || (pi == pj && i > j);
 }
-__device__ __forceinline__ int neighbour(const Dev &d, uint32_t b, uint32_t e) {
-  int2 ends = endpoints(d, d.adjList[e]);
-  int other = ends.x == (int)b ? ends.y : ends.x;
-  return isDynamic(d, other) ? other : -1;
-}
+/** Entry e's other body when dynamic, else -1 (written by k_sort_adj: one load, not the record chain). */
+__device__ __forceinline__ int neighbour(const Dev &d, uint32_t b, uint32_t e) { return d.adjOther[e]; }
 __device__ __forceinline__ uint32_t smallestFree(const Dev &d, uint32_t b, const uint32_t *src, bool skipPending) {
   uint32_t lo = 0, hi = 0;
   for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1]; e++) {
