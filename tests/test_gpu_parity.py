@@ -367,6 +367,32 @@ def test_substeps_reuse_manifolds_match(gpu):
     s.destroy()
 
 
+def test_joint_breaking_in_a_reuse_substep_rebuilds_the_adjacency(gpu):
+    """A substep that keeps every manifold keeps the last adjacency unless a joint broke since it was
+    built (layout.ts:66-72, 86: a broken joint has its stiffnesses zeroed and leaves the adjacency
+    and the colouring). Fracture limit -4 (on the linear force) on each chain's second joint, between
+    two dynamic links, breaks both in the second substep of step 2 in the oracle, so substeps 2 and
+    3 reuse manifolds with a changed joint set and a colouring free of that edge."""
+    say("case: 4 substeps, breakable chains (fracture -4 on the second joints), 4 steps")
+    bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
+    joints = joints.copy()
+    joints[1::4, 11] = -4.0
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32, dt=1 / 240)
+    s = gpu_for(bodies, kw, joints, info, hulls=hw)
+    m = model_for(bodies, kw, joints, info)
+    m.set_hulls(hw)
+    for _ in range(4):
+        s.steps(1, 4)
+        for u in range(4):
+            m.step(1, reuse_manifolds=u > 0)
+    assert (m.joints()[:, 3] != 0).sum() == len(joints) - 2  # both second joints broke
+    got = records.bodies_to_oracle(s.read_bodies())
+    assert np.array_equal(got, m.bodies())
+    assert np.array_equal(s.read_joints().astype(np.float64), m.joints().astype(np.float32).astype(np.float64))
+    assert_same_contacts(s, m)
+    s.destroy()
+
+
 def test_batched_worlds_step_like_their_own_solvers(gpu):
     say("case: 24 batched worlds vs each world in its own solver, 15 steps")
     bodies, starts = scenes.batched_worlds(24, lo=8, hi=160)

@@ -379,14 +379,15 @@ struct avbd_solver {
   uint32_t strideBlocks(uint64_t count, uint32_t threads) const {
     return blocksFor(count, threads, std::max(numCUs, 1u) * (2048u / threads));
   }
-  void fill(uint32_t *p, uint32_t count, uint32_t v) { k_fill<<<blocksFor(count, 256), 256, 0, stream>>>(p, count, v); }
-  /** Exclusive scan of count words (in may equal out). */
-  void scan(const uint32_t *in, uint32_t *out, uint32_t count) {
+  void fill(uint32_t *p, uint32_t count, uint32_t v, const uint32_t *skip = nullptr) { k_fill<<<blocksFor(count, 256), 256, 0, stream>>>(p, count, v, skip); }
+  /** Exclusive scan of in[0, count) into out (in may equal out); nothing when skip is given and its
+   *  bit 0 is set (read on the device). */
+  void scan(const uint32_t *in, uint32_t *out, uint32_t count, const uint32_t *skip = nullptr) {
     uint32_t nb = (count + 1023) / 1024;
     if (nb == 0) return;
-    k_scan_local<<<nb, 256, 0, stream>>>(in, out, scanSums, count);
-    k_scan_sums<<<1, 1024, 0, stream>>>(scanSums, nb);
-    k_scan_add<<<nb, 256, 0, stream>>>(out, scanSums, count);
+    k_scan_local<<<nb, 256, 0, stream>>>(in, out, scanSums, count, skip);
+    k_scan_sums<<<1, 1024, 0, stream>>>(scanSums, nb, skip);
+    k_scan_add<<<nb, 256, 0, stream>>>(out, scanSums, count, skip);
   }
 
   /** uploadStatics (solver.ts:1263-1305): cell size, large list and sorted no-collide list. Each
@@ -466,10 +467,13 @@ struct avbd_solver {
   }
   void topology() {
     Dev d = dev();
-    fill(adjCount, n + 1, 0);
-    fill(adjFill, n, 0);
+    // a substep whose reuse kept every manifold keeps the adjacency too (C_TOPO bit 0, set on the
+    // device): these launches then return at once
+    const uint32_t *keep = counters + C_TOPO;
+    fill(adjCount, n + 1, 0, keep);
+    fill(adjFill, n, 0, keep);
     k_degree<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
-    scan(adjCount, adjStart, n + 1);
+    scan(adjCount, adjStart, n + 1, keep);
     k_fill_adj<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
     k_sort_adj<<<strideBlocks(adjCap, 256), 256, 0, stream>>>(d);
     if (fixedColors) {
@@ -1234,7 +1238,7 @@ int avbd_watch_create(avbd_solver *s, uint32_t pairs, uint32_t capacity) {
     w.capacity = capacity;
     w.table = s->adjAlloc<uint32_t>(s->watchMem, WATCH_HEAD + 4 * slots);
     w.events = s->adjAlloc<float4>(s->watchMem, 3ull * capacity);
-    k_fill<<<blocksFor(WATCH_HEAD + 4 * slots, 256), 256, 0, s->stream>>>(w.table, (uint32_t)(WATCH_HEAD + 4 * slots), 0u);
+    k_fill<<<blocksFor(WATCH_HEAD + 4 * slots, 256), 256, 0, s->stream>>>(w.table, (uint32_t)(WATCH_HEAD + 4 * slots), 0u, nullptr);
     s->watchBuf = w;
     s->watchFlip = 0;
     return avbd_sync(s);
@@ -1264,7 +1268,7 @@ int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt) {
   WatchBuf &w = s->watchBuf;
   s->watchFlip = 1 - s->watchFlip;
   uint32_t prev = s->watchFlip ? 0u : w.setSlots, now = s->watchFlip ? w.setSlots : 0u;
-  k_fill<<<blocksFor(2ull * w.setSlots, 256), 256, 0, s->stream>>>(w.table + WATCH_HEAD + 2ull * now, 2 * w.setSlots, 0u);
+  k_fill<<<blocksFor(2ull * w.setSlots, 256), 256, 0, s->stream>>>(w.table + WATCH_HEAD + 2ull * now, 2 * w.setSlots, 0u, nullptr);
   int last = 1 - s->parity;  // the buffers the last step wrote
   uint32_t scan = std::min(pairs, s->manCap);
   Dev d = s->dev();
@@ -1282,7 +1286,7 @@ int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, 
   if (int rc = avbd_sync(s)) return rc;
   uint32_t n = std::min(head[0], w.capacity), copied = std::min(n, capacity);
   if (copied) hipMemcpyAsync(out, w.events, 48ull * copied, hipMemcpyDeviceToHost, s->stream);
-  k_fill<<<1, 64, 0, s->stream>>>(w.table, WATCH_HEAD, 0u);
+  k_fill<<<1, 64, 0, s->stream>>>(w.table, WATCH_HEAD, 0u, nullptr);
   *count = copied;
   *dropped = head[0] - copied;
   *full = head[1] != 0 ? 1u : 0u;
@@ -1424,7 +1428,7 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
     uint32_t segments = K ? (steps + K - 1) / K : 0;
     if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error before the forward");
     s->ensureCheckpoints(segments + 1);
-    k_fill<<<1, 64, 0, s->stream>>>(s->counters + C_STICKY, 3, 0u);
+    k_fill<<<1, 64, 0, s->stream>>>(s->counters + C_STICKY, 3, 0u, nullptr);
     if (K == 0) {
       s->ensureTape(s->tapes[0], steps, s->conCap, s->manCap);
       s->recordSteps(s->tapes[0], steps);
@@ -1513,7 +1517,7 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
         if (g + 1 < segments) {
           // λ adjoint of this segment's last step arrives in the checkpoint's contact order
           const avbd_solver::Checkpoint &c = s->ckpts[g + 1];
-          k_fill<<<blocksFor(s->hashSize, 256), 256, 0, R>>>(s->remapTable, s->hashSize, 0u);
+          k_fill<<<blocksFor(s->hashSize, 256), 256, 0, R>>>(s->remapTable, s->hashSize, 0u, nullptr);
           uint32_t mb = blocksFor(s->manCap, 256);
           k_adj_remap_insert_counted<<<mb, 256, 0, R>>>(c.mIds, c.counters + C_MANIFOLDS, s->manCap, s->remapTable, s->hashSize - 1);
           s->zeroF64(a.lambPrev, 4ull * s->adjConCap, R);

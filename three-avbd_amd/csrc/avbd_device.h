@@ -84,6 +84,9 @@ enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASH
        // since the host last cleared them: overflow bits | 0x10000 on clashes, high-water counts
        C_STICKY = 11, C_MAX_CONTACTS = 12, C_MAX_MANIFOLDS = 13,
        C_PREV_OVERFLOW = 14,  // the last step's overflow bits (k_begin)
+       // bit 0: this substep keeps the last adjacency (a reuse that kept every manifold, no joint
+       // broken since the adjacency was built); bit 1: a joint broke since then
+       C_TOPO = 15,
        C_WORDS = 16 };
 
 /** One contact point: 4 float4, the w's carrying the key and C(x-) (layout.ts:123-127). */

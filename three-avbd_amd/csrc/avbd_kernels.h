@@ -74,7 +74,11 @@ __device__ __forceinline__ uint32_t prevManifoldCount(const Dev &d) { return d.c
 
 // ---------------------------------------------------------------------------------------------
 // Utilities
-__global__ void k_fill(uint32_t *p, uint32_t n, uint32_t v) { GRID_STRIDE(i, n) p[i] = v; }
+/** Fills p[0, n) with v; nothing when skip is given and its bit 0 is set (a kept adjacency). */
+__global__ void k_fill(uint32_t *p, uint32_t n, uint32_t v, const uint32_t *skip) {
+  if (skip && (*skip & 1u)) return;
+  GRID_STRIDE(i, n) p[i] = v;
+}
 
 /** beginFrame (wgsl-collision.ts:97-107): last step's counts become "previous". */
 __global__ void k_begin(Dev d, int reuseManifolds) {
@@ -91,6 +95,7 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   if (!reuseManifolds) c[C_PAIRS] = 0;
   c[C_PREV_OVERFLOW] = c[C_OVERFLOW];
   c[C_OVERFLOW] = 0;
+  c[C_TOPO] &= ~1u;  // the adjacency is rebuilt unless this substep's reuse keeps it
   c[C_HULL_PAIRS] = 0;
   c[C_CLASHES] = 0;
   c[C_NUM_COLORS] = 0;
@@ -450,6 +455,9 @@ __global__ void __launch_bounds__(256) k_reuse_manifolds(Dev d) {
   if (keepAll && first == 0 && threadIdx.x == 0) {
     d.counters[C_MANIFOLDS] = count;
     d.counters[C_CONTACTS] = d.counters[C_PREV_CONTACTS];
+    // same manifolds in the same slots, same records: the adjacency built from the last list still
+    // holds unless a joint broke since (its entry leaves the lists)
+    if (!(d.counters[C_TOPO] & 2u)) d.counters[C_TOPO] |= 1u;
   }
   for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
     uint32_t m = w + (threadIdx.x & 63u);
@@ -676,6 +684,8 @@ __device__ __forceinline__ int2 endpoints(const Dev &d, uint32_t id) {
   return make_int2((int)ids.x, (int)ids.y);
 }
 __global__ void k_degree(Dev d) {
+  if (d.counters[C_TOPO] & 1u) return;  // a kept adjacency
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&d.counters[C_TOPO], ~2u);  // rebuilt from here on
   uint32_t total = d.nj + manifoldCount(d);
   GRID_STRIDE(id, total) {
     if (id < d.nj && !jointActive(d, id)) continue;
@@ -695,6 +705,7 @@ __device__ __forceinline__ uint2 adjKey(const Dev &d, uint32_t b, uint32_t id) {
 }
 __device__ __forceinline__ bool keyLess(uint2 a, uint2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); }
 __global__ void k_fill_adj(Dev d) {
+  if (d.counters[C_TOPO] & 1u) return;  // a kept adjacency
   uint32_t total = d.nj + manifoldCount(d);
   GRID_STRIDE(id, total) {
     if (id < d.nj && !jointActive(d, id)) continue;
@@ -718,6 +729,7 @@ __global__ void k_fill_adj(Dev d) {
  *  number of that body's keys below its own, with its manifold record beside it for the solve's
  *  flattened walk (a joint's record: ids.x = 0xffffffff, id in ids.y). */
 __global__ void k_sort_adj(Dev d) {
+  if (d.counters[C_TOPO] & 1u) return;  // a kept adjacency
   uint32_t total = d.adjStart[d.n];
   if (total > d.adjCap) total = d.adjCap;
   GRID_STRIDE(e, total) {
@@ -1220,6 +1232,7 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
   }
   bool linear = frac < 0.0f && dot(xyz(lamLin), xyz(lamLin)) > limit * limit;
   if (limit < BIG && (dot(xyz(lamAng), xyz(lamAng)) > limit * limit || linear)) {
+    atomicOr(&d.counters[C_TOPO], 2u);  // broken: the next adjacency drops it
     penLin = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     penAng = penLin;
     lamLin = make_float4(0.0f, 0.0f, 0.0f, frac);
@@ -1445,7 +1458,8 @@ __device__ __forceinline__ uint32_t blockExclusive(uint32_t v, uint32_t *total) 
   __syncthreads();
   return r;
 }
-__global__ void __launch_bounds__(256) k_scan_local(const uint32_t *in, uint32_t *out, uint32_t *sums, uint32_t n) {
+__global__ void __launch_bounds__(256) k_scan_local(const uint32_t *in, uint32_t *out, uint32_t *sums, uint32_t n, const uint32_t *skip) {
+  if (skip && (*skip & 1u)) return;
   uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
   uint32_t v[4];
 #pragma unroll
@@ -1456,7 +1470,8 @@ __global__ void __launch_bounds__(256) k_scan_local(const uint32_t *in, uint32_t
   for (int k = 0; k < 4; k++) { if (base + k < n) out[base + k] = ex; ex += v[k]; }
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
-__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t *sums, uint32_t nb) {
+__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t *sums, uint32_t nb, const uint32_t *skip) {
+  if (skip && (*skip & 1u)) return;
   uint32_t base = threadIdx.x * 4;
   uint32_t v[4];
 #pragma unroll
@@ -1466,7 +1481,8 @@ __global__ void __launch_bounds__(1024) k_scan_sums(uint32_t *sums, uint32_t nb)
 #pragma unroll
   for (int k = 0; k < 4; k++) { if (base + k < nb) sums[base + k] = ex; ex += v[k]; }
 }
-__global__ void __launch_bounds__(256) k_scan_add(uint32_t *out, const uint32_t *sums, uint32_t n) {
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t *out, const uint32_t *sums, uint32_t n, const uint32_t *skip) {
+  if (skip && (*skip & 1u)) return;
   uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
   uint32_t s = sums[blockIdx.x];
 #pragma unroll
