@@ -382,8 +382,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--color-cap", type=int, default=32)
     ap.add_argument("--color-cap-adjoint", type=int, default=16)
-    ap.add_argument("--primal-lanes", type=int, nargs=3, default=[2 ** 31, 2 ** 15, 2 ** 12],
-                    help="colour sizes from which 1, 2, 4 threads per body are used in the primal (else 8)")
+    ap.add_argument("--primal-lanes", type=int, nargs=4, default=[2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11],
+                    help="colour sizes from which 1, 2, 4, 8 threads per body are used in the primal (else 16)")
     ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop"],
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")

@@ -73,7 +73,7 @@ typedef struct avbd_params {
   uint32_t iterations;  /* 10 */
   uint32_t color_cap;   /* colours the colouring may use and the solve sweeps, <= 64 */
   uint32_t color_rounds;/* Jones-Plassmann rounds per step, even */
-  uint32_t primal_lanes[3]; /* colour sizes from which 1, 2, 4 threads per body are used (else 8) */
+  uint32_t primal_lanes[4]; /* colour sizes from which 1, 2, 4, 8 threads per body are used (else 16) */
 } avbd_params;
 
 /* Replaces GpuSolverOptions.bodyCapacity/capacity (solver.ts:233-274). 0 = default sizing. */
@@ -235,8 +235,9 @@ int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt);
 int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, uint32_t *count, uint32_t *dropped, uint32_t *full);
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
  * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`.
- * When per-colour launches win, it also times eight threads per body for every colour against
- * the params' primal_lanes rule and keeps the faster in the params (avbd_get_params reads it). */
+ * When per-colour launches win, it also times eight threads per body for every colour (sixteen
+ * below the rule's last size) against the params' primal_lanes rule and keeps the faster in the
+ * params (avbd_get_params reads it). */
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);
 
 /* profileNextStep (solver.ts:1952): phase times of the next step, synchronously. */

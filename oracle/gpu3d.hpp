@@ -109,7 +109,7 @@ template <class T> struct Contact { V3<T> rA; uint32_t key; V3<T> rB; T c0x; V3<
 struct Params {  // the subset of Params (layout.ts:265-305) the semantics depend on
   double dt, gravity, betaLin, betaAng, gamma, alpha, contactMargin, up[3];
   uint32_t flags, iterations, colorCap, colorRounds, step;
-  uint32_t lanesRule[3];  // colour sizes from which 1, 2, 4 lanes per body are used (else 8)
+  uint32_t lanesRule[4];  // colour sizes from which 1, 2, 4, 8 lanes per body are used (else 16)
 };
 
 /** One step's frozen decisions, recorded in f64 whatever the model's float type (the adjoint's
@@ -828,7 +828,8 @@ template <class T> struct Model {
     if (count >= p.lanesRule[0]) return 1;
     if (count >= p.lanesRule[1]) return 2;
     if (count >= p.lanesRule[2]) return 4;
-    return 8;
+    if (count >= p.lanesRule[3]) return 8;
+    return 16;
   }
   /** accumulate(i, lane, lanes) (wgsl-solve.ts:436-501). */
   Acc accumulate(uint32_t i, uint32_t lane, uint32_t lanes, T alpha) const {
@@ -918,9 +919,9 @@ template <class T> struct Model {
         uint32_t i = bucketBodies[start + k];
         live[k - k0] = bodies[i].enabled();  // a disabled body keeps its pose
         if (!live[k - k0]) continue;
-        Acc part[8];
+        Acc part[16];
         for (uint32_t l = 0; l < lanes; l++) part[l] = accumulate(i, l, lanes, alpha);
-        for (uint32_t s = 4; s > 0; s >>= 1)
+        for (uint32_t s = 8; s > 0; s >>= 1)
           if (s < lanes) for (uint32_t l = 0; l < s; l++) accAdd(part[l], part[l + s]);
         accs[k - k0] = part[0];
       }

@@ -59,7 +59,7 @@ class Params(ctypes.Structure):
         ("betaAng", ctypes.c_double), ("gamma", ctypes.c_double), ("alpha", ctypes.c_double),
         ("contactMargin", ctypes.c_double), ("up", ctypes.c_double * 3),
         ("flags", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("colorCap", ctypes.c_uint32),
-        ("colorRounds", ctypes.c_uint32), ("step", ctypes.c_uint32), ("lanesRule", ctypes.c_uint32 * 3),
+        ("colorRounds", ctypes.c_uint32), ("step", ctypes.c_uint32), ("lanesRule", ctypes.c_uint32 * 4),
     ]
 
 
@@ -73,12 +73,12 @@ NO_COLOR = 255
 
 
 def make_params(dt=1 / 60, gravity=-10.0, iterations=10, alpha=0.99, beta_lin=1e4, beta_ang=100.0, gamma=0.999,
-                flags=FLAG_DEFAULTS, color_cap=64, color_rounds=16, margin=0.01, up=(0, 0, 1), lanes=(0, 0, 0)):
+                flags=FLAG_DEFAULTS, color_cap=64, color_rounds=16, margin=0.01, up=(0, 0, 1), lanes=(0, 0, 0, 0)):
     p = Params()
     p.dt, p.gravity, p.betaLin, p.betaAng, p.gamma, p.alpha, p.contactMargin = dt, gravity, beta_lin, beta_ang, gamma, alpha, margin
     p.up[0], p.up[1], p.up[2] = up
     p.flags, p.iterations, p.colorCap, p.colorRounds, p.step = flags, iterations, color_cap, color_rounds, 0
-    p.lanesRule[0], p.lanesRule[1], p.lanesRule[2] = lanes
+    p.lanesRule[0], p.lanesRule[1], p.lanesRule[2], p.lanesRule[3] = lanes
     return p
 
 

@@ -24,7 +24,7 @@ def say(*a):
 
 
 def gpu_params():
-    return avbd_amd.default_params(dt=1 / 60, gravity=-9.81, primal_lanes=(0, 0, 0))
+    return avbd_amd.default_params(dt=1 / 60, gravity=-9.81, primal_lanes=(0, 0, 0, 0))
 
 
 def oracle_params():

@@ -15,7 +15,7 @@ from helpers import PARITY_SCENES, brute_force_pairs, contact_match, random_box_
 
 pytestmark = pytest.mark.gpu
 
-LANES1 = (0, 0, 0)
+LANES1 = (0, 0, 0, 0)
 
 
 def say(*a):
@@ -186,10 +186,15 @@ def test_pile_steps_bit_exact_one_lane(gpu):
     s.destroy()
 
 
-def test_pile_lane_split_matches_restatement(gpu):
-    say("case: pile 400 bodies 10 steps, 2/4/8 lanes per body")
-    bodies, _, _ = scenes.random_pile(400, seed=4, bin_size=10.0)
-    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=64)
+@pytest.mark.parametrize("rule,pitch", [((1 << 20, 1 << 19, 1 << 18, 0), 2.5), ((1 << 20, 1 << 19, 1 << 18, 1 << 17), 2.5),
+                                        ((2 ** 31, 200, 50, 10), 1.6)], ids=["8", "16", "mixed"])
+def test_pile_lane_split_matches_restatement(gpu, rule, pitch):
+    """8 or 16 threads per body in every colour, and a rule giving a denser pile's colours (about
+    290, 70, 30 and 3 bodies by step 10) 2, 4, 8 and 16 threads: the lanes' sums added as the
+    oracle adds them."""
+    say(f"case: pile 400 bodies (pitch {pitch}) 10 steps, lane rule {rule}")
+    bodies, _, _ = scenes.random_pile(400, seed=4, bin_size=10.0 if pitch == 2.5 else 8.0, pitch=pitch)
+    kw = dict(lanes=rule, color_cap=64)
     s = gpu_for(bodies, kw)
     m = model_for(bodies, kw)
     for _ in range(10):
@@ -205,7 +210,7 @@ def test_determinism_twice(gpu):
     bodies, _, _ = scenes.random_pile(1000, seed=9, bin_size=14.0)
     outs = []
     for _ in range(2):
-        s = gpu_for(bodies, dict(color_cap=16, lanes=(2 ** 31, 2 ** 15, 2 ** 12)))
+        s = gpu_for(bodies, dict(color_cap=16, lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11)))
         for _ in range(30):
             s.step()
         outs.append(s.read_bodies().copy())
@@ -235,7 +240,7 @@ def test_hulls_spheres_boxes_chains_bit_exact(gpu):
 def test_hull_wavefront_narrowphase_matches_inline(gpu, monkeypatch):
     say("case: hull pairs one wavefront each vs one thread each, 1500-body mixed pile, 15 steps")
     bodies, joints, info, hw = scenes.large_scene(1500, seed=12, chains=2, links=4, workers=1)
-    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32)
     outs = []
     for inline in (False, True):
         if inline:
@@ -257,7 +262,7 @@ def test_per_colour_launch_path_matches(gpu, monkeypatch):
     say("case: per-colour launches (no cooperative solve) vs oracle, pile 400 x 10 steps")
     monkeypatch.setenv("AVBD_SOLVE", "colors")
     bodies, _, _ = scenes.random_pile(400, seed=6, bin_size=10.0)
-    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=24)
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18, 0), color_cap=24)
     s = gpu_for(bodies, kw)
     m = model_for(bodies, kw)
     for _ in range(10):
@@ -270,14 +275,16 @@ def test_per_colour_launch_path_matches(gpu, monkeypatch):
     s.destroy()
 
 
-@pytest.mark.parametrize("first,cap,scene", [(0, 24, "pile"), (3, 24, "pile"), (1, 2, "pyramid")])
-def test_sparse_tail_matches(gpu, monkeypatch, first, cap, scene):
+@pytest.mark.parametrize("first,cap,scene,lanes", [(0, 24, "pile", 8), (3, 24, "pile", 8), (1, 2, "pyramid", 8),
+                                                   (3, 24, "pile", 16), (1, 2, "pyramid", 16)])
+def test_sparse_tail_matches(gpu, monkeypatch, first, cap, scene, lanes):
     """Colours from `first` up fused in one workgroup (k_primal_tail) give the oracle's step bit
-    for bit, including clashing colours serialised inside the tail (the pyramid at cap 2)."""
-    say(f"case: sparse tail from colour {first}, cap {cap}, {scene} x 10 steps")
+    for bit, including clashing colours serialised inside the tail (the pyramid at cap 2), with 8
+    or 16 threads per body."""
+    say(f"case: sparse tail from colour {first}, cap {cap}, {scene}, {lanes} lanes x 10 steps")
     monkeypatch.setenv("AVBD_SOLVE", "colors")
     bodies = scenes.random_pile(400, seed=6, bin_size=10.0)[0] if scene == "pile" else scenes.pyramid(10)[0]
-    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=cap)
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18, 0 if lanes == 8 else 1 << 17), color_cap=cap)
     s = gpu_for(bodies, kw)
     s.set_sparse_tail(first)
     m = model_for(bodies, kw)
@@ -305,7 +312,7 @@ def test_lifecycle_flags_match(gpu):
     cold = dynamic & (idx % 5 == 1) & ~disabled
     bodies[disabled, 11] = 2.0
     bodies[cold, 11] = 4.0
-    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32)
     s = gpu_for(bodies, kw, joints, info, hulls=hw)
     m = model_for(bodies, kw, joints, info)
     m.set_hulls(hw)
@@ -329,7 +336,7 @@ def test_clashing_colours_are_marked_per_step(gpu):
     wgsl-topology.ts:313-315). Bit for bit with the oracle through the change."""
     say("case: pyramid at colour cap 2 (clashing), then cap 16")
     bodies = scenes.pyramid(10)[0]
-    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18), color_cap=2)
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18, 0), color_cap=2)
     s = gpu_for(bodies, kw)
     m = model_for(bodies, kw)
     for _ in range(3):
@@ -340,7 +347,7 @@ def test_clashing_colours_are_marked_per_step(gpu):
     p = s.params
     p.color_cap = 16
     s.set_params(p)
-    m.set_params(oracle.make_params(gravity=-9.81, color_cap=16, lanes=(1 << 20, 1 << 19, 1 << 18)))
+    m.set_params(oracle.make_params(gravity=-9.81, color_cap=16, lanes=(1 << 20, 1 << 19, 1 << 18, 0)))
     for _ in range(3):
         s.step()
         m.step(1)
@@ -353,7 +360,7 @@ def test_clashing_colours_are_marked_per_step(gpu):
 def test_substeps_reuse_manifolds_match(gpu):
     say("case: 4 substeps (reuse manifolds) hull/box/sphere scene, 5 steps")
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
-    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32, dt=1 / 240)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32, dt=1 / 240)
     s = gpu_for(bodies, kw, joints, info, hulls=hw)
     m = model_for(bodies, kw, joints, info)
     m.set_hulls(hw)
@@ -377,7 +384,7 @@ def test_joint_breaking_in_a_reuse_substep_rebuilds_the_adjacency(gpu):
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
     joints = joints.copy()
     joints[1::4, 11] = -4.0
-    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12), color_cap=32, dt=1 / 240)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32, dt=1 / 240)
     s = gpu_for(bodies, kw, joints, info, hulls=hw)
     m = model_for(bodies, kw, joints, info)
     m.set_hulls(hw)

@@ -48,7 +48,7 @@ class Params(ctypes.Structure):
         ("dt", ctypes.c_float), ("gravity", ctypes.c_float), ("beta_lin", ctypes.c_float), ("beta_ang", ctypes.c_float),
         ("gamma", ctypes.c_float), ("alpha", ctypes.c_float), ("contact_margin", ctypes.c_float), ("up", ctypes.c_float * 3),
         ("flags", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("color_cap", ctypes.c_uint32),
-        ("color_rounds", ctypes.c_uint32), ("primal_lanes", ctypes.c_uint32 * 3),
+        ("color_rounds", ctypes.c_uint32), ("primal_lanes", ctypes.c_uint32 * 4),
     ]
 
 
@@ -133,13 +133,13 @@ def _p(a):
 
 def default_params(dt=1 / 60, gravity=-9.81, iterations=10, alpha=0.99, beta_lin=1e4, beta_ang=100.0, gamma=0.999,
                    flags=FLAG_DEFAULTS, color_cap=64, color_rounds=16, margin=0.01, up=(0.0, 0.0, 1.0),
-                   primal_lanes=(2 ** 31, 2 ** 15, 2 ** 12)):
+                   primal_lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11)):
     """gpuParams3D (solver.ts:213-226) with z up, as a solver built from the reference's scenes."""
     p = Params()
     p.dt, p.gravity, p.beta_lin, p.beta_ang, p.gamma, p.alpha, p.contact_margin = dt, gravity, beta_lin, beta_ang, gamma, alpha, margin
     p.up[0], p.up[1], p.up[2] = up
     p.flags, p.iterations, p.color_cap, p.color_rounds = flags, iterations, color_cap, color_rounds
-    p.primal_lanes[0], p.primal_lanes[1], p.primal_lanes[2] = primal_lanes
+    p.primal_lanes[0], p.primal_lanes[1], p.primal_lanes[2], p.primal_lanes[3] = primal_lanes
     return p
 
 

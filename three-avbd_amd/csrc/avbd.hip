@@ -335,6 +335,7 @@ struct avbd_solver {
     p.flags = hp.flags; p.iterations = hp.iterations; p.colorCap = fixedColors ? fixedCap : hp.color_cap; p.rounds = hp.color_rounds;
     p.step = stepCount;
     p.lanes1 = hp.primal_lanes[0]; p.lanes2 = hp.primal_lanes[1]; p.lanes4 = hp.primal_lanes[2];
+    p.lanes8 = hp.primal_lanes[3];
     return p;
   }
   Dev dev() const {
@@ -1317,13 +1318,14 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
   int pick = -1;
   for (int k = 0; k < nc; k++) if (bestT[k] < INFINITY && (pick < 0 || bestT[k] < bestT[pick])) pick = k;
   if (pick < 0) { s->applySchedule(keep); return fail(s, AVBD_ERR_DEVICE, "autotune: no schedule ran"); }
-  // Threads per body: the given lane rule against eight threads for every colour, under the
-  // per-colour schedule when it won (results differ only in the order the lanes' sums are added)
-  const uint32_t ALL8[3] = {1u << 31, 1u << 31, 1u << 31};
-  if (!cands[pick].cooperative && !std::equal(ALL8, ALL8 + 3, s->hp.primal_lanes)) {
-    uint32_t given[3] = {s->hp.primal_lanes[0], s->hp.primal_lanes[1], s->hp.primal_lanes[2]};
+  // Threads per body: the given lane rule against eight threads for every colour (sixteen below
+  // the rule's last size), under the per-colour schedule when it won (results differ only in the
+  // order the lanes' sums are added)
+  const uint32_t ALL8[4] = {1u << 31, 1u << 31, 1u << 31, s->hp.primal_lanes[3]};
+  if (!cands[pick].cooperative && !std::equal(ALL8, ALL8 + 4, s->hp.primal_lanes)) {
+    uint32_t given[4] = {s->hp.primal_lanes[0], s->hp.primal_lanes[1], s->hp.primal_lanes[2], s->hp.primal_lanes[3]};
     s->applySchedule(cands[pick]);
-    std::copy(ALL8, ALL8 + 3, s->hp.primal_lanes);
+    std::copy(ALL8, ALL8 + 4, s->hp.primal_lanes);
     float t8 = INFINITY;
     for (int pass = 0; pass < 2; pass++) {
       float ms = 0;
@@ -1331,7 +1333,7 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
       t8 = std::min(t8, ms / steps);
     }
     if (t8 < bestT[pick]) bestT[pick] = t8;
-    else std::copy(given, given + 3, s->hp.primal_lanes);
+    else std::copy(given, given + 4, s->hp.primal_lanes);
   }
   s->applySchedule(cands[pick]);
   if (best) *best = cands[pick];

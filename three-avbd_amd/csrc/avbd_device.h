@@ -97,7 +97,7 @@ struct Params {
   float dt, gravity, betaLin, betaAng, gamma, alpha, margin;
   float upx, upy, upz;
   uint32_t flags, iterations, colorCap, rounds, step;
-  uint32_t lanes1, lanes2, lanes4;
+  uint32_t lanes1, lanes2, lanes4, lanes8;
 };
 
 /** A body stream whose elements are `stride` float4 apart: the four pose streams share one

@@ -1112,11 +1112,12 @@ __device__ __forceinline__ uint32_t lanesFor(const Params &p, uint32_t count) {
   if (count >= p.lanes1) return 1;
   if (count >= p.lanes2) return 2;
   if (count >= p.lanes4) return 4;
-  return 8;
+  if (count >= p.lanes8) return 8;
+  return 16;  // small colours: their time is the longest adjacency walk, so split it further
 }
 /** v of the lane S places up in the same 16-lane row (DPP row_shl:S); lanes whose source leaves
- *  the row read 0. A body's lanes are an aligned group of at most 8, so lane j < S of a group reads
- *  lane j + S of the same group. */
+ *  the row read 0. A body's lanes are an aligned group of at most 16 (one row), so lane j < S of a
+ *  group reads lane j + S of the same group. */
 template <int S>
 __device__ __forceinline__ float fromUp(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + S, 0xf, 0xf, true));
@@ -1131,9 +1132,10 @@ __device__ __forceinline__ void reduceStep(Acc &acc) {
   for (int e = 0; e < 3; e++) { acc.rl[e] += fromUp<S>(acc.rl[e]); acc.ra[e] += fromUp<S>(acc.ra[e]); }
 }
 /** Reduce the partial systems of a body's lanes onto lane 0: lanes s..2s-1 onto 0..s-1 for
- *  s = 4, 2, 1 (those below the body's lane count), the sums lane 0 ends with being those of a
+ *  s = 8, 4, 2, 1 (those below the body's lane count), the sums lane 0 ends with being those of a
  *  butterfly over the group. Every lane of the wavefront must be active. */
 __device__ __forceinline__ void reduceLanes(Acc &acc, uint32_t lanes) {
+  if (lanes > 8) reduceStep<8>(acc);
   if (lanes > 4) reduceStep<4>(acc);
   if (lanes > 2) reduceStep<2>(acc);
   if (lanes > 1) reduceStep<1>(acc);
