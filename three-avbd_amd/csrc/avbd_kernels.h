@@ -784,26 +784,67 @@ __device__ __forceinline__ uint32_t smallestFree(const Dev &d, uint32_t b, const
   if (~hi != 0u) return 32u + (uint32_t)__builtin_ctz(~hi);
   return MAX_COLORS;
 }
+/** Neighbour entries a colouring pass fetches at a time: the entries' loads, then the loads they
+ *  index, go out together instead of one dependent pair per entry. */
+#define COLOR_FETCH 4u
+/** colorCompact (avbd2d/gpu/wgsl-topology.ts:201-216): a coloured body whose priority beats every
+ *  neighbour's takes the smallest colour no neighbour holds, if lower. The beats test and the
+ *  neighbours' colours come from one pass over the entries. */
 __global__ void k_color_compact(Dev d) {
   GRID_STRIDE(b, d.n) {
     if (!isDynamic(d, b)) { d.colB[b] = NO_COLOR; continue; }
     uint32_t col = d.colA[b] & 0xffu;
     if (col != NO_COLOR) {
       bool isMax = true;
-      for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && isMax; e++) { int j = neighbour(d, b, e); if (j >= 0 && !beats(d, b, j)) isMax = false; }
-      if (isMax) { uint32_t f = smallestFree(d, b, d.colA, false); col = col < f ? col : f; }
+      uint32_t lo = 0, hi = 0, pb = d.prio[b], e1 = d.adjStart[b + 1];
+      for (uint32_t e = d.adjStart[b]; e < e1; e += COLOR_FETCH) {
+        int j[COLOR_FETCH];
+        uint32_t sj[COLOR_FETCH], pj[COLOR_FETCH];
+#pragma unroll
+        for (uint32_t k = 0; k < COLOR_FETCH; k++) j[k] = e + k < e1 ? d.adjOther[e + k] : -1;
+#pragma unroll
+        for (uint32_t k = 0; k < COLOR_FETCH; k++) {
+          sj[k] = j[k] >= 0 ? d.colA[j[k]] : NO_COLOR;
+          pj[k] = j[k] >= 0 ? d.prio[j[k]] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < COLOR_FETCH; k++) {
+          if (j[k] < 0) continue;
+          if (!(pb > pj[k] || (pb == pj[k] && b > (uint32_t)j[k]))) isMax = false;  // !beats(b, j)
+          uint32_t c = sj[k] & 0xffu;
+          if (c == NO_COLOR) continue;
+          if (c < 32) lo |= 1u << c; else hi |= 1u << (c - 32);
+        }
+      }
+      if (isMax) {
+        uint32_t f = ~lo != 0u ? (uint32_t)__builtin_ctz(~lo) : ~hi != 0u ? 32u + (uint32_t)__builtin_ctz(~hi) : MAX_COLORS;
+        col = col < f ? col : f;
+      }
     }
     d.colB[b] = col;
   }
 }
+/** colorMark (avbd2d/gpu/wgsl-topology.ts:220-232): a body is pending when uncoloured or when a
+ *  neighbour of the same colour beats it. */
 __global__ void k_color_mark(Dev d) {
   GRID_STRIDE(b, d.n) {
     uint32_t col = d.colB[b] & 0xffu;
     if (!isDynamic(d, b)) { d.colA[b] = NO_COLOR; continue; }
     bool pending = col == NO_COLOR;
-    for (uint32_t e = d.adjStart[b]; e < d.adjStart[b + 1] && !pending; e++) {
-      int j = neighbour(d, b, e);
-      if (j >= 0 && (d.colB[j] & 0xffu) == col && beats(d, j, b)) pending = true;
+    uint32_t pb = d.prio[b], e1 = d.adjStart[b + 1];
+    for (uint32_t e = d.adjStart[b]; e < e1 && !pending; e += COLOR_FETCH) {
+      int j[COLOR_FETCH];
+      uint32_t cj[COLOR_FETCH], pj[COLOR_FETCH];
+#pragma unroll
+      for (uint32_t k = 0; k < COLOR_FETCH; k++) j[k] = e + k < e1 ? d.adjOther[e + k] : -1;
+#pragma unroll
+      for (uint32_t k = 0; k < COLOR_FETCH; k++) {
+        cj[k] = j[k] >= 0 ? d.colB[j[k]] & 0xffu : NO_COLOR;
+        pj[k] = j[k] >= 0 ? d.prio[j[k]] : 0u;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < COLOR_FETCH; k++)
+        if (j[k] >= 0 && cj[k] == col && (pj[k] > pb || (pj[k] == pb && (uint32_t)j[k] > b))) pending = true;  // beats(j, b)
     }
     d.colA[b] = col | (pending ? PENDING : 0u);
   }
@@ -823,12 +864,13 @@ __global__ void k_color_round(Dev d, const uint32_t *src, uint32_t *dst) {
     } else dst[b] = s;
   }
 }
-/** colorCount (wgsl-topology.ts:276-304): per 256-body chunk, colour histogram and stable ranks. */
+/** colorCount (wgsl-topology.ts:276-304): per 256-body chunk, colour histogram and stable ranks
+ *  (a body's rank: the chunk's bodies of its colour at lower indexes). Within a wavefront the rank
+ *  comes from one ballot per distinct colour; the earlier wavefronts' counts of it are added. */
 __global__ void __launch_bounds__(256) k_color_count(Dev d) {
-  __shared__ uint32_t hist[64];
-  __shared__ uint32_t chunk[256];
-  uint32_t tid = threadIdx.x, b = blockIdx.x * 256 + tid;
-  if (tid < 64) hist[tid] = 0;
+  __shared__ uint32_t waveHist[4][64];
+  uint32_t tid = threadIdx.x, b = blockIdx.x * 256 + tid, w = tid >> 6, lane = tid & 63u;
+  waveHist[w][lane] = 0;
   uint32_t col = NO_COLOR;
   if (b < d.n && isDynamic(d, b)) {
     uint32_t s = d.colA[b];
@@ -838,27 +880,37 @@ __global__ void __launch_bounds__(256) k_color_count(Dev d) {
     d.colA[b] = col | (s & PENDING);
     if (s & PENDING) atomicOr(&d.counters[C_CLASHING + (col >> 5)], 1u << (col & 31u));
   }
-  // colours in use: a wavefront's maximum, one atomic per wavefront (not one per body)
-  uint32_t top = col == NO_COLOR ? 0u : col + 1;
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) { uint32_t other = (uint32_t)__shfl_xor((int)top, o); top = other > top ? other : top; }
-  if ((tid & 63u) == 0 && top) atomicMax(&d.counters[C_NUM_COLORS], top);
-  chunk[tid] = col;
-  __syncthreads();
-  if (col != NO_COLOR) {
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < tid; j++) rank += chunk[j] == col ? 1u : 0u;
-    d.colorRank[b] = rank;
-    atomicAdd(&hist[col], 1u);
+  // wavefront-uniform loop: each round takes the colour of the lowest lane not yet ranked
+  uint64_t below = (1ull << lane) - 1ull;
+  uint32_t rank = 0;
+  bool done = col == NO_COLOR;
+  for (;;) {
+    uint64_t left = __ballot(!done);
+    if (left == 0) break;
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)left) - 1u;
+    uint32_t c = (uint32_t)__shfl((int)col, (int)leader);
+    bool mine = !done && col == c;
+    uint64_t same = __ballot(mine);
+    if (mine) { rank = (uint32_t)__popcll(same & below); done = true; }
+    if (lane == leader) waveHist[w][c] = (uint32_t)__popcll(same);
   }
   __syncthreads();
-  if (tid < 64) d.hist[tid * d.colorGroups + blockIdx.x] = hist[tid];
+  if (col != NO_COLOR) {
+    for (uint32_t v = 0; v < w; v++) rank += waveHist[v][col];
+    d.colorRank[b] = rank;
+  }
+  if (tid < 64) d.hist[tid * d.colorGroups + blockIdx.x] = waveHist[0][tid] + waveHist[1][tid] + waveHist[2][tid] + waveHist[3][tid];
   if (b == 0) d.hist[64 * d.colorGroups] = 0;
 }
+/** Colour starts from the scanned histogram, and the colours in use (one past the highest colour
+ *  holding a body) from the same counts: no atomic from the counting pass. One wavefront. */
 __global__ void k_color_starts(Dev d) {
   uint32_t c = threadIdx.x;
-  if (c < 64) d.colorStart[c] = d.hist[c * d.colorGroups];
+  uint32_t at = c < 64 ? d.hist[c * d.colorGroups] : 0u, next = c < 64 ? d.hist[(c + 1) * d.colorGroups] : 0u;
+  if (c < 64) d.colorStart[c] = at;
   if (c == 0) d.colorStart[64] = d.hist[64 * d.colorGroups];
+  uint64_t used = __ballot(c < 64 && next > at);
+  if (c == 0) d.counters[C_NUM_COLORS] = used ? 64u - (uint32_t)__clzll((long long)used) : 0u;
 }
 __global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
   uint32_t b = blockIdx.x * 256 + threadIdx.x;
