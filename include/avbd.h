@@ -235,9 +235,9 @@ int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt);
 int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, uint32_t *count, uint32_t *dropped, uint32_t *full);
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
  * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`.
- * When per-colour launches win, it also times eight threads per body for every colour (sixteen
- * below the rule's last size) against the params' primal_lanes rule and keeps the faster in the
- * params (avbd_get_params reads it). */
+ * When per-colour launches win, it also times, against the params' primal_lanes rule, eight
+ * threads per body for every colour above the rule's last size, and each of the two without
+ * sixteen-thread colours, and keeps the fastest in the params (avbd_get_params reads it). */
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);
 
 /* profileNextStep (solver.ts:1952): phase times of the next step, synchronously. */

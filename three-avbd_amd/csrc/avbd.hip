@@ -1318,22 +1318,31 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
   int pick = -1;
   for (int k = 0; k < nc; k++) if (bestT[k] < INFINITY && (pick < 0 || bestT[k] < bestT[pick])) pick = k;
   if (pick < 0) { s->applySchedule(keep); return fail(s, AVBD_ERR_DEVICE, "autotune: no schedule ran"); }
-  // Threads per body: the given lane rule against eight threads for every colour (sixteen below
-  // the rule's last size), under the per-colour schedule when it won (results differ only in the
-  // order the lanes' sums are added)
-  const uint32_t ALL8[4] = {1u << 31, 1u << 31, 1u << 31, s->hp.primal_lanes[3]};
-  if (!cands[pick].cooperative && !std::equal(ALL8, ALL8 + 4, s->hp.primal_lanes)) {
-    uint32_t given[4] = {s->hp.primal_lanes[0], s->hp.primal_lanes[1], s->hp.primal_lanes[2], s->hp.primal_lanes[3]};
+  // Threads per body, under the per-colour schedule when it won: the given lane rule, eight threads
+  // for every colour above the rule's last size, and each of the two without sixteen-thread
+  // colours; the fastest is kept (results differ only in the order the lanes' sums are added).
+  // Sixteen threads pay where small colours hold bodies with long adjacency walks, and cost an
+  // extra reduction step where they do not.
+  if (!cands[pick].cooperative) {
+    const uint32_t *g = s->hp.primal_lanes;
+    const uint32_t rules[4][4] = {{g[0], g[1], g[2], g[3]}, {1u << 31, 1u << 31, 1u << 31, g[3]}, {g[0], g[1], g[2], 0u}, {1u << 31, 1u << 31, 1u << 31, 0u}};
+    uint32_t given[4] = {g[0], g[1], g[2], g[3]}, bestRule = 0;
+    float t[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
     s->applySchedule(cands[pick]);
-    std::copy(ALL8, ALL8 + 4, s->hp.primal_lanes);
-    float t8 = INFINITY;
-    for (int pass = 0; pass < 2; pass++) {
-      float ms = 0;
-      if (int rc = avbd_time_steps(s, steps, substeps, &ms)) return rc;
-      t8 = std::min(t8, ms / steps);
-    }
-    if (t8 < bestT[pick]) bestT[pick] = t8;
-    else std::copy(given, given + 4, s->hp.primal_lanes);
+    // every rule timed again, passes interleaved: the scene keeps changing while it is timed
+    for (int pass = 0; pass < 2; pass++)
+      for (int r = 0; r < 4; r++) {
+        bool seen = false;
+        for (int q = 0; q < r; q++) seen = seen || std::equal(rules[q], rules[q] + 4, rules[r]);
+        if (seen) continue;
+        std::copy(rules[r], rules[r] + 4, s->hp.primal_lanes);
+        float ms = 0;
+        if (int rc = avbd_time_steps(s, steps, substeps, &ms)) { std::copy(given, given + 4, s->hp.primal_lanes); return rc; }
+        t[r] = std::min(t[r], ms / steps);
+      }
+    for (int r = 1; r < 4; r++) if (t[r] < t[bestRule]) bestRule = (uint32_t)r;
+    std::copy(rules[bestRule], rules[bestRule] + 4, s->hp.primal_lanes);
+    bestT[pick] = t[bestRule];
   }
   s->applySchedule(cands[pick]);
   if (best) *best = cands[pick];
