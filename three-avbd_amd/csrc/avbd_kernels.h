@@ -440,7 +440,6 @@ __device__ __forceinline__ void copyContactsAt(const Dev &d, uint4 pids, float4 
     if (d.src) d.src[base + i] = (int)(pids.z + i);
   }
 }
-/** reuseManifolds (wgsl-collision.ts:758): substeps after the first keep last substep's points. */
 /** reuseManifolds (wgsl-collision.ts:757-764). When the last step left no manifold empty (no contact
  *  overflow: a manifold is only emitted or copied with points), every previous manifold is kept, so
  *  manifold m keeps slot m and its contact range, and the counts are the last step's: no atomics
@@ -1312,12 +1311,12 @@ __device__ __forceinline__ void dualContact(const Dev &d, uint32_t c, const V *b
     bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < 0.00001f;
     key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
   }
-  Contact out;
-  out.rA = f4(xyz(k.rA), __uint_as_float(key));
-  out.rB = k.rB;
-  out.pen = f4(pen, k.pen.w);
-  out.lam = f4(e.F, k.lam.w);
-  d.con[c] = out;
+  // only what changes goes out: penalties and multipliers (the record's upper 32 bytes), and the
+  // key word when its stick bit flipped (anchors and C0 stay)
+  float4 *rec = reinterpret_cast<float4 *>(d.con + c);
+  if (key != __float_as_uint(k.rA.w)) reinterpret_cast<float *>(rec)[3] = __uint_as_float(key);
+  rec[2] = f4(pen, k.pen.w);
+  rec[3] = f4(e.F, k.lam.w);
 }
 /** dualManifold (wgsl-solve.ts:637-662) for contact point c alone: every point's update is its own,
  *  so a thread per point gives the manifold loop's results while no wavefront waits on its widest
