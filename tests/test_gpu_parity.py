@@ -400,6 +400,28 @@ def test_joint_breaking_in_a_reuse_substep_rebuilds_the_adjacency(gpu):
     s.destroy()
 
 
+def test_lone_reuse_steps_rebuild_the_adjacency(gpu):
+    """Substeps driven one call at a time (avbd_step with reuse_manifolds) rebuild the adjacency
+    every time (the host may have changed anything between calls): the same results as the oracle,
+    with the chains breaking as above."""
+    say("case: lone substeps with reuse, breakable chains, 4 steps")
+    bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
+    joints = joints.copy()
+    joints[1::4, 11] = -4.0
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32, dt=1 / 240)
+    s = gpu_for(bodies, kw, joints, info, hulls=hw)
+    m = model_for(bodies, kw, joints, info)
+    m.set_hulls(hw)
+    for _ in range(4):
+        for u in range(4):
+            s.step(reuse_manifolds=u > 0)
+            m.step(1, reuse_manifolds=u > 0)
+    assert (m.joints()[:, 3] != 0).sum() == len(joints) - 2
+    assert np.array_equal(records.bodies_to_oracle(s.read_bodies()), m.bodies())
+    assert_same_contacts(s, m)
+    s.destroy()
+
+
 def test_batched_worlds_step_like_their_own_solvers(gpu):
     say("case: 24 batched worlds vs each world in its own solver, 15 steps")
     bodies, starts = scenes.batched_worlds(24, lo=8, hi=160)

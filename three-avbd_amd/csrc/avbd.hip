@@ -80,6 +80,7 @@ struct avbd_solver {
   avbd_params hp{};
   uint32_t n = 0, nj = 0, bodyCap = 0, jointCap = 0, pairCap = 0, manCap = 0, conCap = 0, hullCap = 0;
   uint32_t tableSize = 0, hashSize = 0, colorGroups = 0, adjCap = 0;
+  bool adjacencyHeld = false;  // the adjacency buffers hold the last substep's adjacency
   float cellSize = 1;
   std::vector<uint32_t> large;          // large bodies, grouped by world
   std::vector<uint32_t> worldStartHost;  // W + 1 body offsets
@@ -439,9 +440,9 @@ struct avbd_solver {
     if (!words.empty()) hipMemcpyAsync(statics, words.data(), words.size() * 4, hipMemcpyHostToDevice, stream);
   }
 
-  void collision(bool reuseManifolds) {
+  void collision(bool reuseManifolds, bool mayKeepAdjacency) {
     Dev d = dev();
-    k_begin<<<1, 1, 0, stream>>>(d, reuseManifolds ? 1 : 0);
+    k_begin<<<1, 1, 0, stream>>>(d, reuseManifolds ? 1 : 0, mayKeepAdjacency ? 1 : 0);
     if (hp.flags & FLAG_REUSE_CONTACTS) k_update_refs<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     if (reuseManifolds) {
       timed(K_NARROW, [&] { k_reuse_manifolds<<<strideBlocks(manCap, 256), 256, 0, stream>>>(d); });
@@ -717,16 +718,21 @@ struct avbd_solver {
     rec = nullptr;
   }
 
-  void step(bool reuseManifolds) {
+  /** One substep. keepAdjacency: the caller ran the previous substep right before this one with
+   *  nothing of the host's in between (the substeps of one avbd_steps call), so a reuse that keeps
+   *  every manifold may keep the adjacency too; any other call rebuilds it. */
+  void step(bool reuseManifolds, bool keepAdjacency = false) {
     stepCount++;
     reuseManifolds = reuseManifolds && manifoldsValid;
-    collision(reuseManifolds);
+    collision(reuseManifolds, reuseManifolds && keepAdjacency && adjacencyHeld);
     topology();
     solve();
     parity = 1 - parity;
     manifoldsValid = true;
+    adjacencyHeld = true;
     if ((hp.flags & AVBD_FLAG_ADAPT_COLORS) && ++stepsSinceAdapt >= 10) {
       stepsSinceAdapt = 0;
+      adjacencyHeld = false;  // storage may be regrown here
       uint32_t c[C_WORDS], starts[MAX_COLORS + 1];
       if (hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, stream) == hipSuccess &&
           hipMemcpyAsync(starts, colorStart, sizeof(starts), hipMemcpyDeviceToHost, stream) == hipSuccess &&
@@ -967,7 +973,7 @@ int avbd_steps(avbd_solver *s, uint32_t steps, uint32_t substeps) {
   if (!s) return AVBD_ERR_INVALID;
   if (substeps == 0) return fail(s, AVBD_ERR_INVALID, "substeps must be >= 1");
   for (uint32_t k = 0; k < steps; k++)
-    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
+    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0, u > 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e));
   return AVBD_OK;
@@ -1357,7 +1363,7 @@ int avbd_profile_step(avbd_solver *s, const avbd_step_opts *opts, avbd_profile *
   s->stepCount++;
   bool reuse = opts && opts->reuse_manifolds && s->manifoldsValid;
   hipEventRecord(e[0], s->stream);
-  s->collision(reuse);
+  s->collision(reuse, false);  // a lone step: the adjacency is rebuilt
   hipEventRecord(e[1], s->stream);
   // adjacency and colouring share topology(); split them at the colouring launch
   s->topology();
@@ -1386,7 +1392,7 @@ int avbd_time_steps(avbd_solver *s, uint32_t steps, uint32_t substeps, float *ms
   hipEventCreate(&e1);
   hipEventRecord(e0, s->stream);
   for (uint32_t k = 0; k < steps; k++)
-    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
+    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0, u > 0);
   hipEventRecord(e1, s->stream);
   int rc = avbd_sync(s);
   hipEventElapsedTime(ms, e0, e1);
@@ -1399,7 +1405,7 @@ int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd
   memset(out, 0, sizeof(*out));
   s->timer.on = true;
   for (uint32_t k = 0; k < steps; k++)
-    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0);
+    for (uint32_t u = 0; u < substeps; u++) s->step(u > 0, u > 0);
   s->timer.on = false;
   int rc = avbd_sync(s);
   out->solve_blocks = s->coopBlocks;

@@ -85,7 +85,8 @@ enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASH
        C_STICKY = 11, C_MAX_CONTACTS = 12, C_MAX_MANIFOLDS = 13,
        C_PREV_OVERFLOW = 14,  // the last step's overflow bits (k_begin)
        // bit 0: this substep keeps the last adjacency (a reuse that kept every manifold, no joint
-       // broken since the adjacency was built); bit 1: a joint broke since then
+       // broken since the adjacency was built); bit 1: the next substep rebuilds it (a joint broke
+       // since it was built, or the host asked: a lone step, or storage regrown)
        C_TOPO = 15,
        C_WORDS = 16 };
 

@@ -81,7 +81,7 @@ __global__ void k_fill(uint32_t *p, uint32_t n, uint32_t v, const uint32_t *skip
 }
 
 /** beginFrame (wgsl-collision.ts:97-107): last step's counts become "previous". */
-__global__ void k_begin(Dev d, int reuseManifolds) {
+__global__ void k_begin(Dev d, int reuseManifolds, int mayKeepAdjacency) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t *c = d.counters;
   uint32_t m = c[C_MANIFOLDS], k = c[C_CONTACTS];
@@ -96,6 +96,7 @@ __global__ void k_begin(Dev d, int reuseManifolds) {
   c[C_PREV_OVERFLOW] = c[C_OVERFLOW];
   c[C_OVERFLOW] = 0;
   c[C_TOPO] &= ~1u;  // the adjacency is rebuilt unless this substep's reuse keeps it
+  if (!mayKeepAdjacency) c[C_TOPO] |= 2u;  // the host's word: rebuild (buffers regrown, or a lone step)
   c[C_HULL_PAIRS] = 0;
   c[C_CLASHES] = 0;
   c[C_NUM_COLORS] = 0;
