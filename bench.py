@@ -1,20 +1,30 @@
 #!/usr/bin/env python3
-"""Throughput of the MI355X AVBD step on one GPU: body-substeps/s.
+"""Throughput of the MI355X AVBD step on one GPU.
 
-Default workload: the large scene, 100,000 bodies (boxes, spheres, convex hulls 1:1:1) in a
-walled bin plus 200 chains of 20 ball-jointed boxes hanging from static anchors, 4 substeps x
-10 iterations per step, f32. A lead-in of `--warmup` steps lets the pile fall and settle; then
-exactly `--steps` steps (each `substeps` substeps: the first detects contacts, the rest reuse
+Default line (the headline): the large scene, 100,000 bodies (boxes, spheres, convex hulls 1:1:1)
+in a walled bin plus 200 chains of 20 ball-jointed boxes hanging from static anchors, 4 substeps x
+10 iterations per step, f32, in body-substeps/s. The scene first runs the workload's 300 steps
+(whatever --warmup says; a larger --warmup runs longer), so the timed window sees the settled
+pile; then exactly `--steps` steps (each 4 substeps: the first detects contacts, the rest reuse
 manifolds) are timed between device synchronisations, bodies resident in HBM, no readback.
 
-Also printed on the JSON line:
+Also printed on the same JSON line:
 - roofline: the dominant kernel (k_primal, one colour of the primal sweep, under the per-colour
   schedule the autotuner picks; the cooperative solve kernel when it picks that one) --
   algorithmic bytes per launch (the unique-touch model below, from the counters read back after
   the timed run) / its average launch duration, timed with hipEvents on the solver's stream,
-  against the MI355X HBM peak of 8 TB/s; traffic from the committed rocprofv3 PMC summary;
-- cpu_baseline: the oracle's f32 restatement of the same step (single thread), seeded with the
-  GPU's state after the timed run and stepped for a bounded sample of substeps on the host.
+  against the MI355X HBM peak of 8 TB/s; traffic from the committed rocprofv3 PMC summary taken
+  at the same settled state;
+- cpu_baseline: the oracle's f32 restatement of the same step, seeded with the GPU's state after
+  the timed run and stepped for a bounded sample of substeps on the host: on every host thread
+  (OpenMP over the bodies of each colour, the contacts, the pairs and the colouring rounds; the
+  result equals the one-thread run bit for bit) and, for context, on one thread;
+- batched: 4096 independent worlds stepped by one solver, in worlds*steps/s, against looping the
+  plain step over a sample of the same worlds;
+- adjoint: 256 worlds x 120 steps, backward time / forward time, and the gradient against the
+  oracle's f64 reverse at the same f32 states.
+`--workload batched-4096` / `adjoint-256` print those alone (the adjoint with its
+finite-difference checks by horizon); `pile-10k` and `pyramid-55` time the smaller scenes.
 """
 import argparse
 import json
@@ -72,9 +82,30 @@ def build_workload(name, workers):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(solver, hulls, params, substeps, budget_s=15.0, max_substeps=8):
-    """The oracle's f32 step on one host thread from the GPU's current state; stops after the
-    first whole step (all `substeps`) that takes the elapsed time past budget_s."""
+SETTLE_STEPS = {"large-100k": 300, "pile-10k": 60, "pyramid-55": 600}
+
+
+def host_info():
+    """Host cores this process may use and the CPU model (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", usable) or usable)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable, "threads": max(1, min(threads, usable))}
+
+
+def cpu_baseline(solver, hulls, substeps, threads, budget_s=10.0, max_substeps=40):
+    """The oracle's f32 step from the GPU's current state on `threads` host threads; stops after
+    the first whole step (all `substeps`) that takes the elapsed time past budget_s."""
     import numpy as np
     import oracle
     from avbd_amd import records
@@ -93,6 +124,7 @@ def cpu_baseline(solver, hulls, params, substeps, budget_s=15.0, max_substeps=8)
     m.set_prev(*records.contacts_to_oracle(man, con))
     m.set_color_state(solver.read_colors()[0])
     m.set_step_count(solver.read_counters()["step"])
+    m.set_threads(threads)
     done = 0
     t0 = time.perf_counter()
     while done < max_substeps:
@@ -105,33 +137,40 @@ def cpu_baseline(solver, hulls, params, substeps, budget_s=15.0, max_substeps=8)
     return done, dt
 
 
-def bench_batched(args, worlds=4096, loop_sample=256):
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def bench_batched(args, worlds=4096, loop_sample=256, settle=60, steps=None):
     """worlds x steps per second of 4096 independent worlds stepped by one batched solver (every
     launch covers all worlds), against looping the plain step (one solver per world) over a
-    bounded sample of the same worlds, same parameters and iterations."""
+    bounded sample of the same worlds, same parameters and iterations. Both run `settle` steps
+    first (the autotuner's steps among them), whatever --warmup says."""
     import numpy as np
     import torch
     import avbd_amd
     from avbd_amd import scenes
+    steps = steps or args.steps
     t = time.perf_counter()
     bodies, starts = scenes.batched_worlds(worlds)
-    print(f"# {worlds} worlds, {len(bodies)} bodies, built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    log(f"{worlds} worlds, {len(bodies)} bodies, built in {time.perf_counter() - t:.1f} s")
     params = avbd_amd.default_params(color_cap=24, color_rounds=16, flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS)
     n = len(bodies)
     batch = avbd_amd.GpuSolver3D(bodies, params=params, world_start=starts,
                                  capacity={"pairs": 8 * n, "manifolds": 8 * n, "contacts": 24 * n})
-    lead = max(0, args.warmup - 24)
+    lead = max(settle, args.warmup) - 24
     batch.steps(lead)
     schedule, _ = batch.autotune(2)
     batch.sync()
+    log(f"batched: lead-in {lead + 24} steps (autotune {schedule})")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batch.steps(args.steps)
+    batch.steps(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     c = batch.read_counters()
-    batch_lanes = batch.params.primal_lanes
-    batched_rate = worlds * args.steps / wall
+    batch_lanes = list(batch.params.primal_lanes)
+    batched_rate = worlds * steps / wall
     batch.destroy()
     # the plain step looped over a sample of the same worlds (one solver each), same lead-in
     rng = np.random.default_rng(1)
@@ -140,9 +179,9 @@ def bench_batched(args, worlds=4096, loop_sample=256):
     for w in sample:
         s = avbd_amd.GpuSolver3D(bodies[starts[w]:starts[w + 1]], params=avbd_amd.default_params(
             color_cap=24, color_rounds=16, flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS))
-        s.steps(args.warmup)
+        s.steps(lead + 24)
         singles.append(s)
-    loop_steps = max(2, args.steps // 4)
+    loop_steps = max(2, steps // 4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(loop_steps):
@@ -153,27 +192,70 @@ def bench_batched(args, worlds=4096, loop_sample=256):
     loop_rate = len(singles) * loop_steps / loop_wall
     for s in singles:
         s.destroy()
-    out = {
+    log(f"batched: {batched_rate:.0f} worlds*steps/s, looped plain step {loop_rate:.0f}")
+    return {
         "metric": "batched worlds*steps/s (4096 worlds)", "value": batched_rate, "unit": "worlds*steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "n_gpus": 1, "steps": steps, "warmup": lead + 24, "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batched-4096", "worlds": worlds, "bodies": n, "substeps": 1, "iterations": 10,
-                   "contacts": c["contacts"], "colors": c["colors"], "clashes": c["clashes"], "overflow": c["overflow"],
-                   "schedule": schedule, "color_cap": c["color_cap"], "primal_lanes": list(batch_lanes)},
+                   "lead_in_steps": lead + 24, "contacts": c["contacts"], "colors": c["colors"], "clashes": c["clashes"],
+                   "overflow": c["overflow"], "schedule": schedule, "color_cap": c["color_cap"], "primal_lanes": batch_lanes},
         "looped_plain_step": {"value": loop_rate, "unit": "worlds*steps/s", "sample_worlds": len(singles),
                               "steps": loop_steps, "speedup": batched_rate / loop_rate},
     }
-    print(json.dumps(out), flush=True)
 
 
-def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
+def adjoint_oracle_check(bodies, ws, params, d_final, steps, every, worlds=(0, 1)):
+    """The GPU's gradient against the oracle's f64 reverse at the same f32 states, for sample
+    worlds each on a solver of its own (its colour sizes pick the lane split, as the oracle's).
+    Returns (max relative error, CPU seconds, per-world results)."""
+    import numpy as np
+    import avbd_amd
+    import oracle
+    from avbd_amd import records
+    F32 = lambda x: float(np.float32(x))
+    op = oracle.make_params(dt=F32(params.dt), gravity=F32(params.gravity), alpha=F32(params.alpha), gamma=F32(params.gamma),
+                            margin=F32(params.contact_margin), beta_lin=F32(params.beta_lin), beta_ang=F32(params.beta_ang),
+                            lanes=tuple(params.primal_lanes))
+    worst, t_cpu, out = 0.0, 0.0, {}
+    for w in worlds:
+        sl = slice(int(ws[w]), int(ws[w + 1]))
+        one = avbd_amd.GpuSolver3D(bodies[sl], params=params)
+        one.forward(steps, every)
+        final = one.read_bodies()
+        gi, gm = one.backward(d_final[sl])
+        one.destroy()
+        rec = records.bodies_to_oracle(bodies[sl])
+        tc = time.perf_counter()
+        m = oracle.Model(32)
+        m.set_params(op)
+        m.set_bodies(rec)
+        m.record_tape(True)
+        m.step(steps)
+        fz = oracle.Frozen(m, rec)
+        fz.load_recorded()
+        g, gmu = fz.backward(d_final[sl])
+        t_cpu += time.perf_counter() - tc
+        assert np.array_equal(m.bodies()[:, 0:3].astype(np.float32), final[:, 0:3]), "forward differs from the oracle"
+        dyn = bodies[sl, 19] > 0
+        err = 0.0
+        for cols in (slice(0, 3), slice(4, 8), slice(8, 11), slice(12, 15)):
+            a, b = gi[dyn][:, cols], g[dyn][:, cols]
+            err = max(err, float(np.abs(a - b).max() / np.abs(b).max()))
+        err = max(err, abs(gm.sum() - gmu.sum()) / abs(gmu.sum()))
+        out[w] = {"gi": gi, "gm": gm, "g": g, "gmu": gmu, "fz": fz, "rec": rec, "err": err}
+        worst = max(worst, err)
+    return worst, t_cpu, out
+
+
+def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5, light=False):
     """Differentiation: 256 worlds, each 64 boxes thrown at a 4x4x4 box stack, 120 steps at dt
     1/60. Forward: the plain step with a checkpoint every 10 steps (avbd_forward); backward: the
     reverse sweep with the active set frozen, re-running each 10-step segment from its checkpoint
     (avbd_backward), for the gradient of sum over worlds of (1,1,1) . (final mean centroid) with
     respect to every body's initial state and friction. The value is backward time / forward
-    time. Gradient checks on sample worlds: the oracle's f64 reverse at the same f32 states, and
-    f64 central differences of the oracle's frozen replay."""
+    time. Gradient checks on sample worlds: the oracle's f64 reverse at the same f32 states and
+    (unless `light`) f64 central differences of the oracle's frozen replay by horizon."""
     import numpy as np
     import torch
     import avbd_amd
@@ -183,7 +265,7 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
     t = time.perf_counter()
     bodies, ws, mus = scenes.thrown_at_stack(worlds, seed=seed)
     n = len(bodies)
-    print(f"# {worlds} worlds, {n} bodies, built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    log(f"adjoint: {worlds} worlds, {n} bodies, built in {time.perf_counter() - t:.1f} s")
     params = avbd_amd.default_params(dt=1 / 60, gravity=-9.81, color_cap=args.color_cap_adjoint)
     d_final = np.zeros((n, 16))
     for w in range(worlds):
@@ -213,9 +295,8 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
             s.backward(d_final)
             t2 = time.perf_counter()
             s.destroy()
-            print(f"# pair {rep}: forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t2 - t1):.1f} ms ({(t2 - t1) / (t1 - t0):.2f}x)",
-                  file=sys.stderr, flush=True)
-        return
+            log(f"pair {rep}: forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t2 - t1):.1f} ms ({(t2 - t1) / (t1 - t0):.2f}x)")
+        return None
     s = fresh()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -224,7 +305,7 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
     plain = time.perf_counter() - t0
     s.destroy()
     timings, reps = {}, 3
-    for mode in (0, every):
+    for mode in ((every,) if light else (0, every)):
         runs = []
         for _ in range(reps):  # independent solvers: each pair allocates its checkpoints, tapes and accumulators
             s = fresh()
@@ -232,112 +313,18 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
             t0 = time.perf_counter()
             s.forward(steps, mode)
             t1 = time.perf_counter()
-            d_init, d_mu = s.backward(d_final)
+            s.backward(d_final)
             t2 = time.perf_counter()
             runs.append((t1 - t0, t2 - t1))
             s.destroy()
         runs.sort(key=lambda r: r[1] / r[0])
         timings[mode] = runs[len(runs) // 2]  # the median backward/forward ratio
-        print(f"# checkpoint_every={mode}: " + ", ".join(f"forward {1e3 * f:.1f} ms backward {1e3 * b:.1f} ms ({b / f:.2f}x)" for f, b in runs),
-              file=sys.stderr, flush=True)
+        log(f"adjoint checkpoint_every={mode}: " + ", ".join(f"forward {1e3 * f:.1f} ms backward {1e3 * b:.1f} ms ({b / f:.2f}x)" for f, b in runs))
         timings[f"runs{mode}"] = runs
     fwd, bwd = timings[every]
 
-    # the oracle at the same f32 states (world 0 and 1): same linear map, f64
-    F32 = lambda x: float(np.float32(x))
-    op = oracle.make_params(dt=F32(params.dt), gravity=F32(params.gravity), alpha=F32(params.alpha), gamma=F32(params.gamma),
-                            margin=F32(params.contact_margin), beta_lin=F32(params.beta_lin), beta_ang=F32(params.beta_ang),
-                            lanes=tuple(params.primal_lanes))
-    vs_oracle, fd_rows = 0.0, []
-    t_cpu = 0.0
-    for w in (0, 1):
-        sl = slice(int(ws[w]), int(ws[w + 1]))
-        # the world on a solver of its own (its colour sizes pick the lane split, as the oracle's)
-        one = avbd_amd.GpuSolver3D(bodies[sl], params=params)
-        one.forward(steps, every)
-        final1 = one.read_bodies()
-        gi, gm = one.backward(d_final[sl])
-        one.destroy()
-        d_init = np.zeros((n, 16)); d_mu = np.zeros(n); final = np.zeros((n, 40), np.float32)
-        d_init[sl], d_mu[sl], final[sl] = gi, gm, final1
-        rec = records.bodies_to_oracle(bodies[sl])
-        tc = time.perf_counter()
-        m = oracle.Model(32)
-        m.set_params(op)
-        m.set_bodies(rec)
-        m.record_tape(True)
-        m.step(steps)
-        fz = oracle.Frozen(m, rec)
-        fz.load_recorded()
-        g, gmu = fz.backward(d_final[sl])
-        t_cpu += time.perf_counter() - tc
-        assert np.array_equal(m.bodies()[:, 0:3].astype(np.float32), final[sl, 0:3]), "forward differs from the oracle"
-        dyn = bodies[sl, 19] > 0
-        for cols in (slice(0, 3), slice(4, 8), slice(8, 11), slice(12, 15)):
-            a, b = d_init[sl][dyn][:, cols], g[dyn][:, cols]
-            vs_oracle = max(vs_oracle, float(np.abs(a - b).max() / np.abs(b).max()))
-        vs_oracle = max(vs_oracle, abs(d_mu[sl].sum() - gmu.sum()) / abs(gmu.sum()))
-        if w == 0:
-            # f64 central differences of the frozen replay of this world's tape
-            st0 = oracle.state_from_records(rec)
-            out = fz.run(st0)
-            gr, gmur = fz.backward(d_final[sl])  # the replay's own adjoint (f64 trajectory)
-            loss = lambda o: float((o * d_final[sl]).sum())
-            order = np.argsort(-np.abs(gr[:, 8:15]).ravel())
-            picks = [(int(k // 7), 8 + int(k % 7)) for k in order if (k % 7) != 3][:6]
-            for i, slot in picks + [(-1, -1)]:
-                row = {"input": "friction" if i < 0 else f"body {i} slot {slot}"}
-                for h in (1e-4, 1e-7):
-                    if i < 0:
-                        mu0 = rec[:, 3].copy()
-                        fd = (loss(fz.run(st0, mu0 + h)) - loss(fz.run(st0, mu0 - h))) / (2 * h)
-                    else:
-                        e = np.zeros_like(st0)
-                        e[i, slot] = h
-                        fd = (loss(fz.run(st0 + e)) - loss(fz.run(st0 - e))) / (2 * h)
-                    row[f"fd_h{h:.0e}"] = fd
-                an = gmur.sum() if i < 0 else gr[i, slot]
-                gp = d_mu[sl].sum() if i < 0 else d_init[sl][i, slot]
-                row.update({"oracle_replay": an, "gpu": gp,
-                            "oracle_vs_fd1e-4": abs(an - row["fd_h1e-04"]) / abs(an),
-                            "oracle_vs_fd1e-7": abs(an - row["fd_h1e-07"]) / abs(an),
-                            "gpu_vs_fd1e-4": abs(gp - row["fd_h1e-04"]) / abs(row["fd_h1e-04"])})
-                fd_rows.append(row)
-    # the gradient against central differences as the horizon grows (world 0 on its own solver;
-    # differences of the oracle's f64 frozen replay of that run's tape at h = 1e-6, where they
-    # have converged; the six largest velocity components)
-    horizon = []
-    sl = slice(int(ws[0]), int(ws[1]))
-    rec0 = records.bodies_to_oracle(bodies[sl])
-    for hs in (10, 20, 40, 60, steps):
-        one = avbd_amd.GpuSolver3D(bodies[sl], params=params)
-        one.forward(hs, every)
-        gi, _ = one.backward(d_final[sl])
-        one.destroy()
-        m = oracle.Model(32)
-        m.set_params(op)
-        m.set_bodies(rec0)
-        m.record_tape(True)
-        m.step(hs)
-        fz = oracle.Frozen(m, rec0)
-        st0 = oracle.state_from_records(rec0)
-        fz.run(st0)
-        gr, _ = fz.backward(d_final[sl])
-        loss = lambda o: float((o * d_final[sl]).sum())
-        order = np.argsort(-np.abs(gr[:, 8:15]).ravel())
-        picks = [(int(k // 7), 8 + int(k % 7)) for k in order if (k % 7) != 3][:6]
-        fd = []
-        for i, slot in picks:
-            e = np.zeros_like(st0)
-            e[i, slot] = 1e-6
-            fd.append((loss(fz.run(st0 + e)) - loss(fz.run(st0 - e))) / 2e-6)
-        fd = np.array(fd)
-        g_gpu = np.array([gi[i, slot] for i, slot in picks])
-        g_or = np.array([gr[i, slot] for i, slot in picks])
-        horizon.append({"steps": hs, "gpu_vs_fd": float(np.abs(g_gpu - fd).max() / np.abs(fd).max()),
-                        "oracle_f64_vs_fd": float(np.abs(g_or - fd).max() / np.abs(fd).max())})
-        print(f"# horizon {hs}: gpu vs fd {horizon[-1]['gpu_vs_fd']:.2e}, oracle f64 vs fd {horizon[-1]['oracle_f64_vs_fd']:.2e}",
-              file=sys.stderr, flush=True)
+    vs_oracle, t_cpu, per_world = adjoint_oracle_check(bodies, ws, params, d_final, steps, every)
+    log(f"adjoint: gradient vs the oracle's f64 reverse at the same states {vs_oracle:.2e}")
     out = {
         "metric": f"adjoint/forward time ({worlds} worlds, {steps} steps)",
         "value": bwd / fwd,
@@ -358,54 +345,99 @@ def bench_adjoint(args, worlds=256, steps=120, every=10, seed=5):
         "adjoint": {
             "forward_ms": 1e3 * fwd, "backward_ms": 1e3 * bwd, "plain_forward_ms": 1e3 * plain,
             "forward_plus_backward_over_plain": (fwd + bwd) / plain,
-            "record_all": {"forward_ms": 1e3 * timings[0][0], "backward_ms": 1e3 * timings[0][1],
-                           "backward_over_forward": timings[0][1] / timings[0][0]},
-            "pairs_ms": {f"checkpoint_every={m}": [[1e3 * f, 1e3 * b] for f, b in timings[f"runs{m}"]] for m in (0, every)},
+            "pairs_ms": {f"checkpoint_every={m}": [[1e3 * f, 1e3 * b] for f, b in timings[f"runs{m}"]]
+                         for m in ((every,) if light else (0, every))},
             "grad_rel_err_vs_oracle_same_states": vs_oracle,
-            "grad_rel_err_vs_fd_by_horizon": horizon,
-            "fd_world0": fd_rows,
+            "grad_target_vs_oracle_1e-3": "pass" if vs_oracle <= 1e-3 else "fail",
         },
         "cpu_baseline": {"value": 2 * steps / t_cpu, "unit": "worlds*steps/s (forward + backward)", "cores": 1, "kind": "port",
                          "sample": f"2 worlds: the oracle's f32 step recording its tape, then its f64 reverse at those states, {t_cpu:.1f} s"},
         "gpu_worlds_steps_per_s": worlds * steps / (fwd + bwd),
     }
-    print(json.dumps(out), flush=True)
+    if not light:
+        out["adjoint"]["record_all"] = {"forward_ms": 1e3 * timings[0][0], "backward_ms": 1e3 * timings[0][1],
+                                        "backward_over_forward": timings[0][1] / timings[0][0]}
+        out["adjoint"].update(adjoint_fd_checks(bodies, ws, params, d_final, steps, every, per_world[0]))
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=240)
-    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55", "batched-4096", "adjoint-256"])
-    ap.add_argument("--workers", type=int, default=8, help="host processes building the scene's hulls")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--color-cap", type=int, default=32)
-    ap.add_argument("--color-cap-adjoint", type=int, default=16)
-    ap.add_argument("--primal-lanes", type=int, nargs=4, default=[2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11],
-                    help="colour sizes from which 1, 2, 4, 8 threads per body are used in the primal (else 16)")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop"],
-                    help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
-    ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
-    ap.add_argument("--adjoint-steps", type=int, default=120, help="adjoint-256 workload: steps")
-    ap.add_argument("--adjoint-pair-only", action="store_true",
-                    help="adjoint-256 workload: only two forward/backward pairs, no checks and no JSON line (for kernel traces)")
-    ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="first run a plain device copy of this many MiB (the HBM counters' calibration kernel)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (worlds would shard across GPUs "
-              "with no per-step exchange; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def adjoint_fd_checks(bodies, ws, params, d_final, steps, every, w0):
+    """f64 central differences of the oracle's frozen replay (world 0 on its own solver): at the
+    configured h = 1e-4 and at h = 1e-7 for the six largest velocity components and friction, and
+    by horizon at h = 1e-6 (where the differences have converged), with pass/fail against the
+    1e-3 target per horizon."""
+    import numpy as np
+    import avbd_amd
+    import oracle
+    from avbd_amd import records
+    F32 = lambda x: float(np.float32(x))
+    op = oracle.make_params(dt=F32(params.dt), gravity=F32(params.gravity), alpha=F32(params.alpha), gamma=F32(params.gamma),
+                            margin=F32(params.contact_margin), beta_lin=F32(params.beta_lin), beta_ang=F32(params.beta_ang),
+                            lanes=tuple(params.primal_lanes))
+    sl = slice(int(ws[0]), int(ws[1]))
+    fz, rec, gi0, gm0 = w0["fz"], w0["rec"], w0["gi"], w0["gm"]
+    st0 = oracle.state_from_records(rec)
+    fz.run(st0)
+    gr, gmur = fz.backward(d_final[sl])  # the replay's own adjoint (f64 trajectory)
+    loss = lambda o: float((o * d_final[sl]).sum())
+    order = np.argsort(-np.abs(gr[:, 8:15]).ravel())
+    picks = [(int(k // 7), 8 + int(k % 7)) for k in order if (k % 7) != 3][:6]
+    fd_rows = []
+    for i, slot in picks + [(-1, -1)]:
+        row = {"input": "friction" if i < 0 else f"body {i} slot {slot}"}
+        for h in (1e-4, 1e-7):
+            if i < 0:
+                mu0 = rec[:, 3].copy()
+                fd = (loss(fz.run(st0, mu0 + h)) - loss(fz.run(st0, mu0 - h))) / (2 * h)
+            else:
+                e = np.zeros_like(st0)
+                e[i, slot] = h
+                fd = (loss(fz.run(st0 + e)) - loss(fz.run(st0 - e))) / (2 * h)
+            row[f"fd_h{h:.0e}"] = fd
+        an = gmur.sum() if i < 0 else gr[i, slot]
+        gp = gm0.sum() if i < 0 else gi0[i, slot]
+        row.update({"oracle_replay": an, "gpu": gp,
+                    "oracle_vs_fd1e-4": abs(an - row["fd_h1e-04"]) / abs(an),
+                    "oracle_vs_fd1e-7": abs(an - row["fd_h1e-07"]) / abs(an),
+                    "gpu_vs_fd1e-4": abs(gp - row["fd_h1e-04"]) / abs(row["fd_h1e-04"])})
+        fd_rows.append(row)
+    horizon = []
+    for hs in (10, 20, 40, 60, steps):
+        one = avbd_amd.GpuSolver3D(bodies[sl], params=params)
+        one.forward(hs, every)
+        gi, _ = one.backward(d_final[sl])
+        one.destroy()
+        m = oracle.Model(32)
+        m.set_params(op)
+        m.set_bodies(rec)
+        m.record_tape(True)
+        m.step(hs)
+        fzh = oracle.Frozen(m, rec)
+        fzh.run(st0)
+        grh, _ = fzh.backward(d_final[sl])
+        order = np.argsort(-np.abs(grh[:, 8:15]).ravel())
+        pk = [(int(k // 7), 8 + int(k % 7)) for k in order if (k % 7) != 3][:6]
+        fd = []
+        for i, slot in pk:
+            e = np.zeros_like(st0)
+            e[i, slot] = 1e-6
+            fd.append((loss(fzh.run(st0 + e)) - loss(fzh.run(st0 - e))) / 2e-6)
+        fd = np.array(fd)
+        g_gpu = np.array([gi[i, slot] for i, slot in pk])
+        g_or = np.array([grh[i, slot] for i, slot in pk])
+        row = {"steps": hs, "gpu_vs_fd": float(np.abs(g_gpu - fd).max() / np.abs(fd).max()),
+               "oracle_f64_vs_fd": float(np.abs(g_or - fd).max() / np.abs(fd).max())}
+        row["gpu_target_1e-3"] = "pass" if row["gpu_vs_fd"] <= 1e-3 else "fail"
+        horizon.append(row)
+        log(f"horizon {hs}: gpu vs fd {row['gpu_vs_fd']:.2e} ({row['gpu_target_1e-3']}), oracle f64 vs fd {row['oracle_f64_vs_fd']:.2e}")
+    return {"grad_rel_err_vs_fd_by_horizon": horizon, "fd_world0": fd_rows}
 
-    if args.workload == "batched-4096":
-        return bench_batched(args)
-    if args.workload == "adjoint-256":
-        return bench_adjoint(args, worlds=args.adjoint_worlds, steps=args.adjoint_steps)
+
+def bench_step(args, workload):
+    """The plain step on one scene: settle, autotune, time `--steps` steps, roofline, CPU baseline."""
     t = time.perf_counter()
-    bodies, joints, info, hulls, substeps, iters = build_workload(args.workload, args.workers)
-    print(f"# scene {args.workload}: {len(bodies)} bodies, {len(joints)} joints, built in {time.perf_counter() - t:.1f} s",
-          file=sys.stderr, flush=True)
+    bodies, joints, info, hulls, substeps, iters = build_workload(workload, args.workers)
+    log(f"scene {workload}: {len(bodies)} bodies, {len(joints)} joints, built in {time.perf_counter() - t:.1f} s")
 
     import torch
     import avbd_amd
@@ -416,8 +448,7 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(f"# copy calibration: 3 x {args.copy_calibration_mb} MiB read + {args.copy_calibration_mb} MiB written",
-              file=sys.stderr, flush=True)
+        log(f"copy calibration: 3 x {args.copy_calibration_mb} MiB read + {args.copy_calibration_mb} MiB written")
         del src, dst
     params = avbd_amd.default_params(dt=1.0 / (60.0 * substeps), iterations=iters, color_cap=args.color_cap, color_rounds=16,
                                      flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS,
@@ -427,19 +458,23 @@ def main():
                                   capacity={"pairs": 12 * n, "manifolds": 12 * n, "contacts": 40 * n})
     t = time.perf_counter()
     tune_steps = 2
-    lead = max(0, args.warmup - 14 * tune_steps)
-    for done in range(0, lead, 40):  # in chunks, with a progress line each (profiled runs are slow)
-        solver.steps(min(40, lead - done), substeps)
-        solver.sync()
-        print(f"# lead-in {min(lead, done + 40)}/{lead} steps", file=sys.stderr, flush=True)
-    if args.schedule == "auto":
+    total_lead = max(SETTLE_STEPS.get(workload, 0), args.warmup)
+    tune_total = 14 * tune_steps if total_lead >= 14 * tune_steps else 0
+    lead = total_lead - tune_total
+    for done in range(0, lead, 50):  # in chunks, with a progress line each (profiled runs are slow)
+        solver.steps(min(50, lead - done), substeps)
+        c = solver.read_counters()
+        log(f"lead-in {min(lead, done + 50)}/{total_lead} steps: {c['contacts']} contacts, {c['manifolds']} manifolds, "
+            f"{c['colors']} colours")
+    if tune_total and args.schedule == "auto":
         schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
     else:  # counter-collection runs distort the autotuner's timings: fix the schedule instead
         solver.set_schedule(cooperative=int(args.schedule == "coop"), threads=512, blocks_per_cu=1)
-        schedule, tuned_ms = solver.schedule, solver.time_steps(14 * tune_steps, substeps) / (14 * tune_steps)
+        schedule, tuned_ms = solver.schedule, solver.time_steps(tune_total, substeps) / max(1, tune_total)
     solver.sync()
-    print(f"# lead-in {args.warmup} steps x {substeps} substeps (autotune: {schedule}, {tuned_ms:.2f} ms/step) "
-          f"in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    c = solver.read_counters()
+    log(f"lead-in {total_lead} steps x {substeps} substeps (autotune: {schedule}, {tuned_ms:.2f} ms/step) "
+        f"in {time.perf_counter() - t:.1f} s: {c['contacts']} contacts")
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -449,6 +484,7 @@ def main():
     wall = t1 - t0
     c = solver.read_counters()
     ev_ms = solver.time_steps(2, substeps)  # hipEvent-bracketed cross-check of the wall clock
+    log(f"timed {args.steps} steps: {wall / args.steps * 1e3:.2f} ms/step")
 
     # roofline of the dominant kernel: HIP events around every launch of it, then bytes from counters
     ks = solver.profile_kernels(2, substeps)
@@ -473,12 +509,12 @@ def main():
 
     value = n * args.steps * substeps / wall
     out = {
-        "metric": "body-substeps/s @100k bodies",
+        "metric": "body-substeps/s @100k bodies" if workload == "large-100k" else f"body-substeps/s ({workload})",
         "value": value,
         "unit": "body-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": total_lead,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "strong",
@@ -486,28 +522,86 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": args.workload, "bodies": n, "joints": int(len(joints)), "substeps": substeps, "iterations": iters,
-            "contacts": c["contacts"], "manifolds": c["manifolds"], "pairs": c["pairs"], "colors": c["colors"],
-            "clashes": c["clashes"], "overflow": c["overflow"], "parallelism": "single-gpu",
+            "workload": workload, "bodies": n, "joints": int(len(joints)), "substeps": substeps, "iterations": iters,
+            "lead_in_steps": total_lead, "contacts": c["contacts"], "manifolds": c["manifolds"], "pairs": c["pairs"],
+            "colors": c["colors"], "clashes": c["clashes"], "overflow": c["overflow"], "parallelism": "single-gpu",
             "event_ms_per_step": ev_ms / 2, "schedule": schedule, "color_cap": c["color_cap"],
             "primal_lanes": list(solver.params.primal_lanes),
         },
         "roofline": {
-            "bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
-            "traffic_source": f"profiles/{args.workload}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)",
+            "bound": "hbm", "limiter": "latency: a colour launch is one wavefront's dependent walk over its bodies' contacts",
+            "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
+            "traffic_source": f"profiles/{workload}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)",
             "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
             "launches": ks["solve_launches"] or ks["primal_launches"] or ks["tail_launches"], "solve_blocks": ks["solve_blocks"],
         },
     }
     if not args.no_cpu_baseline:
-        done, dt = cpu_baseline(solver, hulls, params, substeps)
+        info_h = host_info()
+        done, dt = cpu_baseline(solver, hulls, substeps, info_h["threads"], budget_s=10.0, max_substeps=40)
+        done1, dt1 = cpu_baseline(solver, hulls, substeps, 1, budget_s=6.0, max_substeps=8)
+        allcore = n * done / dt
         out["cpu_baseline"] = {
-            "value": n * done / dt, "unit": "body-substeps/s", "cores": 1, "kind": "port",
+            "value": allcore, "unit": "body-substeps/s", "cores": info_h["threads"], "kind": "port",
             "sample": f"{done} substeps of the same {n}-body state (after the timed run) through the oracle's f32 "
-                      f"restatement of the step on one host thread, {dt:.1f} s",
+                      f"restatement of the step, OpenMP over {info_h['threads']} host threads, {dt:.1f} s",
+            "cpu_model": info_h["cpu_model"], "nproc": info_h["nproc"], "usable_cpus": info_h["usable_cpus"],
+            "gpu_over_cpu": value / allcore,
+            "one_thread": {"value": n * done1 / dt1, "cores": 1, "gpu_over_cpu": value * dt1 / (n * done1),
+                           "sample": f"{done1} substeps on one host thread, {dt1:.1f} s"},
+            "f64_sequential": "not run: the reference's f64 CPU solver (src/avbd3d/ref) has boxes only; "
+                              "this scene has spheres and hulls",
         }
+        log(f"cpu baseline: {allcore:.0f} body-substeps/s on {info_h['threads']} threads "
+            f"({info_h['cpu_model']}), {n * done1 / dt1:.0f} on one")
     solver.destroy()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="lead-in steps before the timed window; at least the workload's own (300 for large-100k)")
+    ap.add_argument("--workload", default="large-100k", choices=["large-100k", "pile-10k", "pyramid-55", "batched-4096", "adjoint-256"])
+    ap.add_argument("--workers", type=int, default=8, help="host processes building the scene's hulls")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="large-100k: skip the batched and adjoint sub-objects")
+    ap.add_argument("--color-cap", type=int, default=32)
+    ap.add_argument("--color-cap-adjoint", type=int, default=16)
+    ap.add_argument("--primal-lanes", type=int, nargs=4, default=[2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11],
+                    help="colour sizes from which 1, 2, 4, 8 threads per body are used in the primal (else 16)")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop"],
+                    help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
+    ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
+    ap.add_argument("--adjoint-steps", type=int, default=120, help="adjoint-256 workload: steps")
+    ap.add_argument("--adjoint-pair-only", action="store_true",
+                    help="adjoint-256 workload: only two forward/backward pairs, no checks and no JSON line (for kernel traces)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (the HBM counters' calibration kernel)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (worlds would shard across GPUs "
+              "with no per-step exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == "batched-4096":
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
+    if args.workload == "adjoint-256":
+        out = bench_adjoint(args, worlds=args.adjoint_worlds, steps=args.adjoint_steps)
+        if out is not None:
+            print(json.dumps(out), flush=True)
+        return
+    out = bench_step(args, args.workload)
+    if args.workload == "large-100k" and not args.no_extras:
+        b = bench_batched(args, steps=20)
+        out["batched"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config", "looped_plain_step")}
+        a = bench_adjoint(args, worlds=args.adjoint_worlds, steps=args.adjoint_steps, light=True)
+        out["adjoint"] = {k: a[k] for k in ("metric", "value", "unit", "config", "adjoint", "gpu_worlds_steps_per_s")}
+        out["adjoint"]["higher_is_better"] = False
     print(json.dumps(out), flush=True)
 
 

@@ -293,6 +293,7 @@ void omodel_set_fixed_colors(void *h, const uint32_t *colors) {
 }
 void omodel_set_color_state(void *h, const uint32_t *state) { DISPATCH(h, M->colorA.assign(state, state + M->bodies.size())); }
 void omodel_set_step_count(void *h, uint32_t s) { DISPATCH(h, M->stepCount = s); }
+void omodel_set_threads(void *h, int n) { DISPATCH(h, M->threads = n < 1 ? 1 : n); }
 void omodel_step(void *h, int n, int reuseManifolds) { DISPATCH(h, for (int i = 0; i < n; i++) M->step(reuseManifolds != 0)); }
 /** The collision phase of one step only (pairs, manifolds, contacts), then adjacency and colour. */
 void omodel_collide(void *h, int reuseManifolds) {

@@ -19,6 +19,7 @@
 #include <vector>
 #include <algorithm>
 #include <unordered_map>
+#include <omp.h>
 
 namespace gpuo {
 
@@ -156,6 +157,10 @@ template <class T> struct Model {
   bool fixedColors = false;
   std::vector<uint32_t> fixed;
   uint32_t stepCount = 0;
+  /** Host threads (OpenMP) for the CPU baseline: bodies of a proper colour, contacts, pairs and
+   *  colouring rounds are independent, so the result equals the one-thread result bit for bit.
+   *  Clashing colours and tape recording run serially. */
+  int threads = 1;
 
   T dtT() const { return T(p.dt); }
   bool dynamic(int i) const { return i >= 0 && bodies[i].mass > T(0); }
@@ -238,14 +243,18 @@ template <class T> struct Model {
       grid[key(cx[i], cy[i], cz[i])].push_back(i);
     }
     std::vector<std::pair<uint32_t, uint32_t>> out;
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> part(threads);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) if (threads > 1)
     for (uint32_t i = 0; i < n; i++) {
       if (rs[i] > cut) continue;
+      auto &mine = part[threads > 1 ? omp_get_thread_num() : 0];
       for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
         auto it = grid.find(key(cx[i] + dx, cy[i] + dy, cz[i] + dz));
         if (it == grid.end()) continue;
-        for (uint32_t j : it->second) if (j < i && testPair(i, j)) out.push_back({i, j});
+        for (uint32_t j : it->second) if (j < i && testPair(i, j)) mine.push_back({i, j});
       }
     }
+    for (auto &v : part) out.insert(out.end(), v.begin(), v.end());
     for (uint32_t l : large)
       for (uint32_t j = 0; j < n; j++) {
         if (j == l) continue;
@@ -466,10 +475,14 @@ template <class T> struct Model {
     auto it = prevIndex.find(pkey(a, b));
     return it == prevIndex.end() ? -1 : (int)it->second;
   }
-  void copyContacts(const ManifoldT<T> &prev) {
+  /** Where a pass appends manifolds and contacts: the model's own lists, or a worker's private
+   *  lists that are concatenated in pair order afterwards (so the result equals the serial one). */
+  struct Sink { std::vector<ManifoldT<T>> &man; std::vector<Contact<T>> &con; std::vector<int> &src; };
+  Sink own() { return Sink{manifolds, contacts, contactSrc}; }
+  void copyContacts(const ManifoldT<T> &prev, Sink o) const {
     uint32_t a = prev.a, b = prev.b, count = prev.countStep & 15u;
     ManifoldT<T> m = prev;
-    m.first = (uint32_t)contacts.size();
+    m.first = (uint32_t)o.con.size();
     V pA = bodies[a].pos, pB = bodies[b].pos;
     Q qA = bodies[a].rot, qB = bodies[b].rot;
     V basis[3];
@@ -484,23 +497,23 @@ template <class T> struct Model {
       k.lam = O::mul(O::mul(k.lam, alpha), gamma);
       for (int r = 0; r < 3; r++) k.pen[r] = O::clampt(k.pen[r] * gamma, T(1.0), T(1e10));
       k.pen.x = O::mx(k.pen.x, pmin);
-      contacts.push_back(k);
-      contactSrc.push_back((int)(prev.first + i));
+      o.con.push_back(k);
+      o.src.push_back((int)(prev.first + i));
     }
-    manifolds.push_back(m);
+    o.man.push_back(m);
   }
-  bool reuseContacts(uint32_t a, uint32_t b) {
+  bool reuseContacts(uint32_t a, uint32_t b, Sink o) const {
     int pm = findPrev(a, b);
     if (pm < 0) return false;
     const ManifoldT<T> &prev = prevManifolds[pm];
     uint32_t count = prev.countStep & 15u, generated = prev.countStep >> 4;
     uint32_t moved = std::max(bodies[a].stamp, bodies[b].stamp);
     if (count == 0 || generated < moved) return false;
-    copyContacts(prev);
+    copyContacts(prev, o);
     return true;
   }
-  void narrowphase(uint32_t a, uint32_t b) {
-    if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(a, b)) return;
+  void narrowphase(uint32_t a, uint32_t b, Sink o) const {
+    if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(a, b, o)) return;
     Box A = makeBox(a), B = makeBox(b);
     Sat sat; sat.n = O::v(T(0), T(0), T(0));
     Found found; found.count = 0;
@@ -521,7 +534,7 @@ template <class T> struct Model {
     } else found = collideBoxes(A, B, sat);
     if (found.count == 0) return;
     ManifoldT<T> m;
-    m.a = a; m.b = b; m.first = (uint32_t)contacts.size(); m.countStep = found.count | (p.step << 4);
+    m.a = a; m.b = b; m.first = (uint32_t)o.con.size(); m.countStep = found.count | (p.step << 4);
     V n = O::neg(sat.n);
     m.n = n;
     m.friction = std::sqrt(bodies[a].friction * bodies[b].friction);
@@ -566,10 +579,10 @@ template <class T> struct Model {
       k.lam = O::mul(O::mul(lam, alpha), gamma);
       for (int r = 0; r < 3; r++) k.pen[r] = O::clampt(pen[r] * gamma, T(1.0), T(1e10));
       k.pen.x = O::mx(k.pen.x, penaltyMin);
-      contacts.push_back(k);
-      contactSrc.push_back(j);
+      o.con.push_back(k);
+      o.src.push_back(j);
     }
-    manifolds.push_back(m);
+    o.man.push_back(m);
   }
 
   /** beginFrame + updateRefs + (hashInsert, broadphase, narrowphase) or reuseManifolds. */
@@ -577,6 +590,7 @@ template <class T> struct Model {
     prevManifolds.swap(manifolds); prevContacts.swap(contacts);
     manifolds.clear(); contacts.clear(); contactSrc.clear();
     if (p.flags & FLAG_REUSE_CONTACTS) {  // updateRefs (wgsl-collision.ts:34-48)
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
       for (size_t i = 0; i < bodies.size(); i++) {
         Body<T> &b = bodies[i];
         if (!b.enabled()) continue;
@@ -586,7 +600,7 @@ template <class T> struct Model {
     }
     prevIndex.clear();
     if (reuseManifolds) {
-      for (const ManifoldT<T> &m : prevManifolds) if ((m.countStep & 15u) > 0) copyContacts(m);
+      for (const ManifoldT<T> &m : prevManifolds) if ((m.countStep & 15u) > 0) copyContacts(m, own());
       return;
     }
     for (size_t m = 0; m < prevManifolds.size(); m++) {  // hashInsert skips rewritten slots
@@ -595,7 +609,28 @@ template <class T> struct Model {
       prevIndex[pkey(pm.a, pm.b)] = (uint32_t)m;
     }
     broadphase();
-    for (auto &pr : pairs) narrowphase(pr.first, pr.second);
+    if (threads <= 1) {
+      for (auto &pr : pairs) narrowphase(pr.first, pr.second, own());
+      return;
+    }
+    // contiguous pair ranges per worker, appended in range order: the serial order exactly
+    const int parts = threads * 8;
+    std::vector<std::vector<ManifoldT<T>>> pm(parts);
+    std::vector<std::vector<Contact<T>>> pc(parts);
+    std::vector<std::vector<int>> ps(parts);
+    const size_t np = pairs.size();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (int w = 0; w < parts; w++) {
+      size_t lo = np * (size_t)w / parts, hi = np * (size_t)(w + 1) / parts;
+      Sink o{pm[w], pc[w], ps[w]};
+      for (size_t k = lo; k < hi; k++) narrowphase(pairs[k].first, pairs[k].second, o);
+    }
+    for (int w = 0; w < parts; w++) {
+      uint32_t base = (uint32_t)contacts.size();
+      for (ManifoldT<T> m : pm[w]) { m.first += base; manifolds.push_back(m); }
+      contacts.insert(contacts.end(), pc[w].begin(), pc[w].end());
+      contactSrc.insert(contactSrc.end(), ps[w].begin(), ps[w].end());
+    }
   }
 
   // ---- adjacency (wgsl-topology.ts:63-153) ----
@@ -632,8 +667,10 @@ template <class T> struct Model {
     for (uint32_t m = 0; m < manifolds.size(); m++) addId(J + m);
     adjStart.assign(n + 1, 0);
     adjList.clear();
-    for (uint32_t b = 0; b < n; b++) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) if (threads > 1)
+    for (uint32_t b = 0; b < n; b++)
       std::stable_sort(lists[b].begin(), lists[b].end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    for (uint32_t b = 0; b < n; b++) {
       adjStart[b] = (uint32_t)adjList.size();
       for (auto &e : lists[b]) adjList.push_back(e.second);
     }
@@ -673,6 +710,7 @@ template <class T> struct Model {
       cap = 1;
       for (uint32_t b = 0; b < n; b++) if (fixed[b] != NO_COLOR) cap = std::max(cap, fixed[b] + 1);
     } else {
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (threads > 1)
       for (uint32_t b = 0; b < n; b++) {  // colorCompact: A -> B
         if (!dynamic(b)) { colorB[b] = NO_COLOR; continue; }
         uint32_t col = colorA[b] & 0xffu;
@@ -683,6 +721,7 @@ template <class T> struct Model {
         }
         colorB[b] = col;
       }
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (threads > 1)
       for (uint32_t b = 0; b < n; b++) {  // colorMark: B -> A
         uint32_t col = colorB[b] & 0xffu;
         if (!dynamic(b)) { colorA[b] = NO_COLOR; continue; }
@@ -696,6 +735,7 @@ template <class T> struct Model {
       for (uint32_t r = 0; r < p.colorRounds; r++) {  // colorRoundAB / BA
         std::vector<uint32_t> &src = (r % 2 == 0) ? colorA : colorB;
         std::vector<uint32_t> &dst = (r % 2 == 0) ? colorB : colorA;
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (threads > 1)
         for (uint32_t b = 0; b < n; b++) {
           uint32_t s = src[b];
           if (!(s & PENDING)) { dst[b] = s; continue; }
@@ -911,6 +951,27 @@ template <class T> struct Model {
     uint32_t start = bucketStart[c], count = bucketStart[c + 1] - start;
     if (count == 0) return;
     uint32_t lanes = lanesFor(count), per = 64 / lanes;
+    if (threads > 1 && !tape && !((clashMask[c >> 5] >> (c & 31)) & 1u)) {
+      // a proper colour: no body of it reads another's pose, so every system is formed first
+      // and every update applied after, in parallel
+      std::vector<Acc> all(count);
+      std::vector<char> on(count);
+#pragma omp parallel for schedule(dynamic, 32) num_threads(threads)
+      for (uint32_t k = 0; k < count; k++) {
+        uint32_t i = bucketBodies[start + k];
+        on[k] = bodies[i].enabled();
+        if (!on[k]) continue;
+        Acc part[16];
+        for (uint32_t l = 0; l < lanes; l++) part[l] = accumulate(i, l, lanes, alpha);
+        for (uint32_t s = 8; s > 0; s >>= 1)
+          if (s < lanes) for (uint32_t l = 0; l < s; l++) accAdd(part[l], part[l + s]);
+        all[k] = part[0];
+      }
+#pragma omp parallel for schedule(static) num_threads(threads)
+      for (uint32_t k = 0; k < count; k++)
+        if (on[k]) finishBody(bucketBodies[start + k], all[k]);
+      return;
+    }
     std::vector<Acc> accs(per);
     std::vector<char> live(per);
     for (uint32_t chunk = 0; chunk * per < count; chunk++) {
@@ -984,7 +1045,9 @@ template <class T> struct Model {
   }
   void warmStartJoints() {
     T alpha = T(p.alpha), gamma = T(p.gamma), decay = alpha * gamma;
-    for (Joint<T> &k : joints) {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+    for (size_t jj = 0; jj < joints.size(); jj++) {
+      Joint<T> &k = joints[jj];
       if (k.type != (int)T_JOINT) continue;
       if ((k.a >= 0 && !bodies[k.a].enabled()) || !bodies[k.b].enabled()) continue;
       k.c0Lin = jointLinC(k);
@@ -1008,7 +1071,9 @@ template <class T> struct Model {
   void warmStartBodies() {
     T dt = dtT(), g = T(p.gravity);
     V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
-    for (Body<T> &b : bodies) {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1 && !tape)
+    for (size_t bi = 0; bi < bodies.size(); bi++) {
+      Body<T> &b = bodies[bi];
       if (!b.enabled()) {  // untouched, its flag kept
         if (tape) { tape->steps.back().atRest.push_back(0); tape->steps.back().wMode.push_back(1); }
         continue;
@@ -1044,7 +1109,9 @@ template <class T> struct Model {
   void updateVelocities() {
     T dt = dtT();
     V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
-    for (Body<T> &b : bodies) {
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+    for (size_t bi = 0; bi < bodies.size(); bi++) {
+      Body<T> &b = bodies[bi];
       if (!b.enabled()) continue;
       T prevUp = O::dot(b.vel, up);
       if (b.mass > T(0)) {
@@ -1081,7 +1148,9 @@ template <class T> struct Model {
       }
       for (uint32_t c = 0; c < colorCapUsed; c++) primalColor(c, alpha);
       if (ts) recordPoses(*ts);
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
       for (uint32_t j = 0; j < joints.size(); j++) dualJoint(j, alpha);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) if (threads > 1)
       for (uint32_t m = 0; m < manifolds.size(); m++) dualManifold(m, alpha);
     }
     curIt = -1;

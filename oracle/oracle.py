@@ -236,6 +236,10 @@ class Model:
     def set_step_count(self, s):
         self.L.omodel_set_step_count(self.h, s)
 
+    def set_threads(self, n):
+        """Host threads for the step (OpenMP); results equal the one-thread step bit for bit."""
+        self.L.omodel_set_threads(self.h, int(n))
+
     def step(self, n=1, reuse_manifolds=False):
         self.L.omodel_step(self.h, n, int(reuse_manifolds))
 
