@@ -82,7 +82,7 @@ def build_workload(name, workers):
     raise SystemExit(f"unknown workload {name}")
 
 
-SETTLE_STEPS = {"large-100k": 300, "pile-10k": 60, "pyramid-55": 600}
+SETTLE_STEPS = {"large-100k": 300, "pile-10k": 300, "pyramid-55": 600}
 
 
 def host_info():

@@ -19,7 +19,8 @@
  * Status codes: 0 ok; negative on error (avbd_last_error gives the message). Invalid input is
  * refused before anything changes (solver.ts:858-875). Capacity overflow during a step is not
  * an error: it is counted in avbd_counters.overflow (bits 1 pairs, 2 contacts, 4 manifolds),
- * as the reference counts it (wgsl-collision.ts:230, :727, :816).
+ * as the reference counts it (wgsl-collision.ts:230, :727, :816). Bit AVBD_OVERFLOW_BARRIER is
+ * this library's own: see avbd_set_barrier_polls.
  */
 #ifndef AVBD_MI355X_H
 #define AVBD_MI355X_H
@@ -146,6 +147,10 @@ int avbd_read_pairs(avbd_solver *s, uint32_t *out, uint32_t cap);
 /* Colour state per body (colour | PENDING<<8) and the colour buckets' starts (65 words). */
 int avbd_read_colors(avbd_solver *s, uint32_t *state, uint32_t count, uint32_t *starts);
 
+/* The reference poses updateRefs keeps per body (wgsl-collision.ts:34-48; its `refs` buffer), 8
+ * floats each: pos xyz, 0, rot xyzw (all zero: none yet). A validation read: with the bodies,
+ * contacts, colour state and step count it is the whole state the next step depends on. */
+int avbd_read_refs(avbd_solver *s, float *out, uint32_t first, uint32_t count);
 /* seedFrom (solver.ts:1320): bodies, joints with warm-start state, and last step's contacts. */
 int avbd_seed(avbd_solver *s, const float *bodies, uint32_t body_count, const float *joints, uint32_t joint_count,
               const uint32_t *manifolds, uint32_t manifold_count, const uint32_t *contacts, uint32_t contact_count);
@@ -178,6 +183,16 @@ typedef struct avbd_schedule {
 } avbd_schedule;
 int avbd_set_schedule(avbd_solver *s, const avbd_schedule *schedule);
 int avbd_get_schedule(const avbd_solver *s, avbd_schedule *schedule);
+/* The cooperative solve's grid barrier waits a bounded number of polls (default 2^22, ~0.1 s),
+ * then gives up rather than hang the device (no counterpart in the reference, which has no grid
+ * barrier). A give-up means that launch's solve ran partly unordered, so its step's result is not
+ * the step's: the solver then switches to per-colour launches for good, the avbd_step/avbd_steps
+ * call whose readback saw it (with AVBD_FLAG_ADAPT_COLORS, every 10 steps) returns
+ * AVBD_ERR_DEVICE, and the next avbd_read_counters sets AVBD_OVERFLOW_BARRIER in overflow. An
+ * adjoint rollout that saw one refuses its backward. `polls` 0 restores the default; a tiny value
+ * forces the give-up path (tests). */
+#define AVBD_OVERFLOW_BARRIER 256u
+int avbd_set_barrier_polls(avbd_solver *s, uint32_t polls);
 /* fuseSparseColors / primalSparse (solver.ts:574, :2133-2143, :2248-2255; wgsl-solve.ts:378-394):
  * with per-colour launches, colours from `first_colour` up to the cap run in one workgroup, in
  * order, instead of a launch each. The reference starts that tail at colour 16 for scenes of at

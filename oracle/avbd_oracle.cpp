@@ -294,6 +294,15 @@ void omodel_set_fixed_colors(void *h, const uint32_t *colors) {
 void omodel_set_color_state(void *h, const uint32_t *state) { DISPATCH(h, M->colorA.assign(state, state + M->bodies.size())); }
 void omodel_set_step_count(void *h, uint32_t s) { DISPATCH(h, M->stepCount = s); }
 void omodel_set_threads(void *h, int n) { DISPATCH(h, M->threads = n < 1 ? 1 : n); }
+// updateRefs' reference poses (8 per body: pos xyz, 0, rot xyzw); a zero rotation is "none yet"
+void omodel_set_refs(void *h, const double *r) {
+  DISPATCH(h, for (size_t i = 0; i < M->bodies.size(); i++) {
+    const double *o = r + 8 * i;
+    M->refPos[i].x = o[0]; M->refPos[i].y = o[1]; M->refPos[i].z = o[2];
+    M->refRot[i].x = o[4]; M->refRot[i].y = o[5]; M->refRot[i].z = o[6]; M->refRot[i].w = o[7];
+    M->refSet[i] = (o[4] * o[4] + o[5] * o[5] + o[6] * o[6] + o[7] * o[7]) != 0.0;
+  });
+}
 void omodel_step(void *h, int n, int reuseManifolds) { DISPATCH(h, for (int i = 0; i < n; i++) M->step(reuseManifolds != 0)); }
 /** The collision phase of one step only (pairs, manifolds, contacts), then adjacency and colour. */
 void omodel_collide(void *h, int reuseManifolds) {

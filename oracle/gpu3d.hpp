@@ -144,7 +144,7 @@ template <class T> struct Model {
   std::vector<uint32_t> hulls;  // hull buffer words (vec4u units * 4), wgsl-hull.ts header format
   struct NC { uint32_t hi, lo, slot; };
   std::vector<NC> noCollide;  // sorted by (hi, lo)
-  std::vector<V> refPos; std::vector<Q> refRot; std::vector<bool> refSet;
+  std::vector<V> refPos; std::vector<Q> refRot; std::vector<uint8_t> refSet;  // bytes: written by parallel workers
   std::vector<std::pair<uint32_t, uint32_t>> pairs;
   std::vector<ManifoldT<T>> manifolds, prevManifolds;
   std::vector<Contact<T>> contacts, prevContacts;

@@ -236,6 +236,11 @@ class Model:
     def set_step_count(self, s):
         self.L.omodel_set_step_count(self.h, s)
 
+    def set_refs(self, refs):
+        """updateRefs' reference poses (n x 8: pos xyz, 0, rot xyzw; a zero rotation: none yet)."""
+        r = np.ascontiguousarray(refs, np.float64).reshape(self.n, 8)
+        self.L.omodel_set_refs(self.h, _p(r))
+
     def set_threads(self, n):
         """Host threads for the step (OpenMP); results equal the one-thread step bit for bit."""
         self.L.omodel_set_threads(self.h, int(n))

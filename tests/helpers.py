@@ -122,3 +122,51 @@ def brute_force_pairs(bodies, ignore=()):
                 continue
             out.add((i, j))
     return out
+
+
+def oracle_params_of(solver):
+    """The oracle's params equal to a GPU solver's (its f32 values, exactly)."""
+    p = solver.params
+    return oracle.make_params(dt=p.dt, gravity=p.gravity, iterations=p.iterations, alpha=p.alpha, beta_lin=p.beta_lin,
+                              beta_ang=p.beta_ang, gamma=p.gamma, flags=p.flags & 0x7F, color_cap=p.color_cap,
+                              color_rounds=p.color_rounds, margin=p.contact_margin, up=tuple(p.up),
+                              lanes=tuple(p.primal_lanes))
+
+
+def oracle_from_gpu(solver, hulls=None, threads=1):
+    """An f32 oracle model holding a GPU solver's whole state: params (colour cap and lane rule as
+    adapted), body and joint records, hulls, the last step's manifolds and contacts, the colour
+    state, the step count and updateRefs' reference poses, so its next step is the GPU's next."""
+    from avbd_amd import records
+    m = oracle.Model(32)
+    m.set_params(oracle_params_of(solver))
+    m.set_bodies(records.bodies_to_oracle(solver.read_bodies()))
+    if solver.joint_count:
+        m.set_joints(solver.read_joints().astype(np.float64), solver._joint_info)
+    if hulls is not None:
+        m.set_hulls(hulls)
+    man, con = solver.read_contacts_raw()
+    m.set_prev(*records.contacts_to_oracle(man, con))
+    m.set_color_state(solver.read_colors()[0])
+    m.set_step_count(solver.read_counters()["step"])
+    m.set_refs(solver.read_refs().astype(np.float64))
+    m.set_threads(threads)
+    return m
+
+
+def canonical_contacts(man, con):
+    """Manifolds sorted by (a, b), each one's `first` rewritten to its place in the sorted order,
+    and the contact rows gathered in that order: two contact lists that differ only by emission
+    order give equal arrays. Manifold word 3 keeps its count bits only (the step stamp aside)."""
+    man = np.asarray(man)
+    con = np.asarray(con)
+    if len(man) == 0:
+        return man.reshape(0, 8), con.reshape(0, 16)
+    order = np.lexsort((man[:, 1], man[:, 0]))
+    M = man[order].copy()
+    cnt = (M[:, 3].astype(np.int64) & 15)
+    offs = np.cumsum(cnt) - cnt
+    idx = np.repeat(M[:, 2].astype(np.int64) - offs, cnt) + np.arange(int(cnt.sum()))
+    M[:, 2] = offs
+    M[:, 3] = cnt
+    return M, con[idx]

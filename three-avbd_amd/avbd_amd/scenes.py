@@ -138,16 +138,17 @@ def _hull_or_none(pts):
     return convex_hull(pts)
 
 
-def build_hulls(point_sets, workers=8):
-    """convex_hull of each point set; a pool of forked workers for many (call it before the GPU is used)."""
+def build_hulls(point_sets, workers=8, context="fork"):
+    """convex_hull of each point set; a pool of workers for many. Forked workers start fastest but
+    need a process that has not initialised the GPU yet; "spawn" is safe after it."""
     if len(point_sets) < 2000 or workers <= 1:
         return [_hull_or_none(p) for p in point_sets]
     import multiprocessing as mp
-    with mp.get_context("fork").Pool(workers) as pool:
+    with mp.get_context(context).Pool(workers) as pool:
         return pool.map(_hull_or_none, point_sets, chunksize=256)
 
 
-def large_scene(n=100000, seed=2026, chains=200, links=20, pitch=2.5, friction=0.5, workers=8):
+def large_scene(n=100000, seed=2026, chains=200, links=20, pitch=2.5, friction=0.5, workers=8, context="fork"):
     """Large scene: n bodies, boxes (half-extents U[0.3, 1.2] m), spheres (radius U[0.3, 1.2] m)
     and convex hulls (16-32 points on ellipsoids with axes U[0.3, 1.2] m) 1:1:1, density 1, on a
     jittered grid over a square walled bin; plus `chains` chains of `links` unit boxes joined by
@@ -175,7 +176,7 @@ def large_scene(n=100000, seed=2026, chains=200, links=20, pitch=2.5, friction=0
             draws.append((1, pos, q, rng.uniform(0.3, 1.2)))
         else:
             draws.append((2, pos, q, ellipsoid_points(rng)))
-    shapes = iter(build_hulls([d[3] for d in draws if d[0] == 2], workers))
+    shapes = iter(build_hulls([d[3] for d in draws if d[0] == 2], workers, context))
     for kind, pos, q, arg in draws:
         if kind == 0:
             recs.append(box_record(pos, arg, 1.0, friction, rot=q))

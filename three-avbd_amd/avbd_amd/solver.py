@@ -26,6 +26,7 @@ FLAG_MASS_PENALTY = 16
 FLAG_DEFAULTS = 31
 FLAG_ADAPT_COLORS = 128
 FLAG_FIXED_STORAGE = 256  # capacities stay as created (overflow counted, never grown)
+OVERFLOW_BARRIER = 256  # counters' overflow bit: a cooperative solve's grid barrier gave up waiting
 NO_COLOR = 255
 STICK_BIT = 0x80000000
 BODY_FLOATS = 40
@@ -94,7 +95,7 @@ EXPORTS = [
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
     "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read", "avbd_push", "avbd_move", "avbd_reset",
-    "avbd_get_capacity",
+    "avbd_get_capacity", "avbd_set_barrier_polls", "avbd_read_refs",
 ]
 EVENT_BEGIN, EVENT_END = 1, 2  # contacts.ts BEGIN / END
 
@@ -303,6 +304,12 @@ class GpuSolver3D:
     def set_schedule(self, cooperative=1, threads=512, blocks_per_cu=1):
         self._check(self.L.avbd_set_schedule(self.h, ctypes.byref(Schedule(cooperative, threads, blocks_per_cu))))
 
+    def set_barrier_polls(self, polls=0):
+        """The cooperative solve's grid-barrier wait limit in polls (0: the default, 2^22). A wait
+        that runs out gives up: the solver drops to per-colour launches and reports it."""
+        self._live()
+        self._check(self.L.avbd_set_barrier_polls(self.h, ctypes.c_uint32(polls)))
+
     # --- command passes (Pusher, BodyMover, BodyReset) --------------------------------------
     def push(self, bodies, lin, off_lin=None, moment=None):
         """Pusher.apply (push.ts:90-140): per body (each once) its summed linear impulse, the part
@@ -457,6 +464,12 @@ class GpuSolver3D:
         return state, starts
 
     # --- validation hooks --------------------------------------------------------------------
+    def read_refs(self):
+        """updateRefs' reference poses, n x 8 (pos xyz, 0, rot xyzw; zeros: none yet)."""
+        out = np.zeros((self.body_count, 8), np.float32)
+        self._check(self.L.avbd_read_refs(self.h, _p(out), 0, self.body_count))
+        return out
+
     def seed_from(self, bodies, joints, manifolds, contacts):
         """seedFrom (solver.ts:1320-1359): bodies, joints with warm-start state, last contacts."""
         self._live()

@@ -98,7 +98,10 @@ struct avbd_solver {
   bool anyDynamicLarge = false;
   bool destroyed = false;
   uint32_t coopBlocks = 0;  // grid of the cooperative solve kernel; 0: per-colour launches
-  uint32_t *barrier = nullptr;  // the solve kernel's grid barrier: arrivals, generation
+  uint32_t *barrier = nullptr;  // the solve kernel's grid barrier: arrivals over one launch
+  uint32_t barrierPolls = 1u << 22;  // a barrier wait gives up after this many polls
+  uint32_t barrierGiveUps = 0;       // give-ups seen by the host since avbd_read_counters last reported them
+  bool barrierFailed = false;        // seen during the current avbd_step/avbd_steps call
   uint32_t coopThreads = 512;   // threads per workgroup of the solve kernel
   uint32_t coopPerCU = 1;       // its workgroups per CU
   uint32_t coopMaxBlocks = 0;   // co-resident limit for coopThreads (0: no cooperative launch here)
@@ -504,7 +507,8 @@ struct avbd_solver {
   void solve() {
     Dev d = dev();
     if (coopBlocks && !rec) {
-      void *args[] = {&d, &barrier};
+      fill(barrier, 1, 0u);  // the arrival count starts each launch at zero
+      void *args[] = {&d, &barrier, &barrierPolls};
       hipError_t e;
       timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel(solveKernel(coopThreads), dim3(coopBlocks), dim3(coopThreads), args, 0, stream); });
       if (e == hipSuccess) return;
@@ -718,6 +722,19 @@ struct avbd_solver {
     rec = nullptr;
   }
 
+  /** A cooperative launch whose grid barrier gave up waiting ran part of its solve unordered: that
+   *  step's result is not the step's. From then on the solver uses per-colour launches; the
+   *  give-ups are reported (bit AVBD_OVERFLOW_BARRIER of the counters' overflow word, and an error
+   *  from the avbd_step/avbd_steps call that saw them). */
+  void noteBarrier(const uint32_t *c) {
+    if (!c[C_BARRIER]) return;
+    barrierGiveUps += c[C_BARRIER];
+    barrierFailed = true;
+    coopBlocks = 0;
+    fill(counters + C_BARRIER, 1, 0u);
+    err = "the cooperative solve's grid barrier gave up waiting (" + std::to_string(c[C_BARRIER]) +
+          " waits): the solve of the steps since the last readback ran unordered; the solver now uses per-colour launches";
+  }
   /** One substep. keepAdjacency: the caller ran the previous substep right before this one with
    *  nothing of the host's in between (the substeps of one avbd_steps call), so a reuse that keeps
    *  every manifold may keep the adjacency too; any other call rebuilds it. */
@@ -741,6 +758,7 @@ struct avbd_solver {
         k.colors = c[C_NUM_COLORS];
         k.clashes = c[C_CLASHES];
         k.pairs = c[C_PAIRS]; k.contacts = c[C_CONTACTS]; k.manifolds = c[C_MANIFOLDS]; k.overflow = c[C_OVERFLOW];
+        noteBarrier(c);
         adapt(k);
         adaptStorage(k);
         if (tailAuto) chooseTail(starts);
@@ -964,19 +982,22 @@ int avbd_get_params(const avbd_solver *s, avbd_params *p) {
 int avbd_step(avbd_solver *s, const avbd_step_opts *opts) {
   if (!s) return AVBD_ERR_INVALID;
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  s->barrierFailed = false;
   s->step(opts && opts->reuse_manifolds);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e));
-  return AVBD_OK;
+  return s->barrierFailed ? AVBD_ERR_DEVICE : AVBD_OK;
 }
 int avbd_steps(avbd_solver *s, uint32_t steps, uint32_t substeps) {
   if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
   if (substeps == 0) return fail(s, AVBD_ERR_INVALID, "substeps must be >= 1");
+  s->barrierFailed = false;
   for (uint32_t k = 0; k < steps; k++)
     for (uint32_t u = 0; u < substeps; u++) s->step(u > 0, u > 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(s, AVBD_ERR_DEVICE, hipGetErrorString(e));
-  return AVBD_OK;
+  return s->barrierFailed ? AVBD_ERR_DEVICE : AVBD_OK;
 }
 int avbd_sync(avbd_solver *s) {
   if (!s) return AVBD_ERR_INVALID;
@@ -1032,7 +1053,10 @@ int avbd_read_counters(avbd_solver *s, avbd_counters *out) {
   out->pairs = c[C_PAIRS];
   out->contacts = c[C_CONTACTS];
   out->manifolds = c[C_MANIFOLDS];
-  out->overflow = c[C_OVERFLOW];
+  s->noteBarrier(c);
+  out->overflow = c[C_OVERFLOW] & ~AVBD_OVERFLOW_BARRIER;
+  if (s->barrierGiveUps) out->overflow |= AVBD_OVERFLOW_BARRIER;
+  s->barrierGiveUps = 0;
   out->clashes = c[C_CLASHES];
   out->colors = c[C_NUM_COLORS];
   out->color_cap = s->fixedColors ? s->fixedCap : s->hp.color_cap;
@@ -1079,6 +1103,14 @@ int avbd_read_colors(avbd_solver *s, uint32_t *state, uint32_t count, uint32_t *
   return avbd_sync(s);
 }
 
+int avbd_read_refs(avbd_solver *s, float *out, uint32_t first, uint32_t count) {
+  if (!s || (!out && count)) return AVBD_ERR_INVALID;
+  if ((uint64_t)first + count > s->n) return fail(s, AVBD_ERR_INVALID, "body range out of bounds");
+  if (!count) return AVBD_OK;
+  hipMemcpy2DAsync(out, 32, s->refPos + first, 16, 16, count, hipMemcpyDeviceToHost, s->stream);
+  hipMemcpy2DAsync(out + 4, 32, s->refRot + first, 16, 16, count, hipMemcpyDeviceToHost, s->stream);
+  return avbd_sync(s);
+}
 int avbd_seed(avbd_solver *s, const float *bodies, uint32_t body_count, const float *jrec, uint32_t joint_count, const uint32_t *man,
               uint32_t nm, const uint32_t *con, uint32_t nc) {
   if (!s) return AVBD_ERR_INVALID;
@@ -1189,6 +1221,11 @@ int avbd_set_schedule(avbd_solver *s, const avbd_schedule *sch) {
 int avbd_get_schedule(const avbd_solver *s, avbd_schedule *sch) {
   if (!s || !sch) return AVBD_ERR_INVALID;
   *sch = s->schedule();
+  return AVBD_OK;
+}
+int avbd_set_barrier_polls(avbd_solver *s, uint32_t polls) {
+  if (!s) return AVBD_ERR_INVALID;
+  s->barrierPolls = polls ? polls : (1u << 22);
   return AVBD_OK;
 }
 /** Upload `count` records of `words` 32-bit words and check their body words (one per record at
@@ -1441,6 +1478,7 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
     s->fwdValid = false;
     s->fwdTail = false;
     s->fwdWhy = "forward rollout incomplete";
+    s->barrierFailed = false;
     uint32_t K = checkpoint_every;
     uint32_t segments = K ? (steps + K - 1) / K : 0;
     if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error before the forward");
@@ -1468,6 +1506,11 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
     hipMemcpyAsync(c, s->counters, sizeof(c), hipMemcpyDeviceToHost, s->stream);
     if (int rc = avbd_sync(s)) return rc;
     uint32_t sticky = c[C_STICKY] | c[C_OVERFLOW] | (c[C_CLASHES] ? 0x10000u : 0u);
+    if (c[C_BARRIER] || s->barrierFailed) {
+      s->noteBarrier(c);
+      s->fwdWhy = "the cooperative solve's grid barrier gave up waiting during the rollout";
+      return AVBD_OK;
+    }
     s->fwdMaxContacts = std::max(c[C_MAX_CONTACTS], c[C_CONTACTS]);
     s->fwdMaxManifolds = std::max(c[C_MAX_MANIFOLDS], c[C_MANIFOLDS]);
     s->fwdSteps = steps; s->fwdEvery = K; s->fwdEndStep = s->stepCount;

@@ -80,7 +80,7 @@ static const uint32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, IGNORE_SLOT = 0xfff
 static const float BIG = 3.0e38f, HARD = 1e30f, PENALTY_MIN = 1.0f, PENALTY_MAX = 1e10f;
 // counter words (layout.ts:136-141 + avbd2d/gpu/layout.ts:45-51)
 enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASHES = 4, C_NUM_COLORS = 5,
-       C_MANIFOLDS = 6, C_PREV_MANIFOLDS = 7, C_HULL_PAIRS = 8, C_CLASHING = 9,
+       C_MANIFOLDS = 6, C_PREV_MANIFOLDS = 7, C_HULL_PAIRS = 8, C_CLASHING = 9,  // + 10
        // since the host last cleared them: overflow bits | 0x10000 on clashes, high-water counts
        C_STICKY = 11, C_MAX_CONTACTS = 12, C_MAX_MANIFOLDS = 13,
        C_PREV_OVERFLOW = 14,  // the last step's overflow bits (k_begin)
@@ -88,7 +88,8 @@ enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASH
        // broken since the adjacency was built); bit 1: the next substep rebuilds it (a joint broke
        // since it was built, or the host asked: a lone step, or storage regrown)
        C_TOPO = 15,
-       C_WORDS = 16 };
+       C_BARRIER = 16,  // cooperative solve: grid-barrier waits given up since the host last cleared it
+       C_WORDS = 20 };
 
 /** One contact point: 4 float4, the w's carrying the key and C(x-) (layout.ts:123-127). */
 struct __align__(16) Contact { float4 rA; float4 rB; float4 pen; float4 lam; };

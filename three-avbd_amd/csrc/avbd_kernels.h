@@ -1426,30 +1426,27 @@ __device__ __forceinline__ void velocities(const Dev &d, uint32_t t0, uint32_t n
 }
 __global__ void k_velocities(Dev d) { velocities(d, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x); }
 
-/** Grid-wide barrier for a cooperative (co-resident) grid: one atomic arrival per workgroup on
- *  bar[0]; the last to arrive resets it and bumps the generation bar[32] (its own 128-byte line),
- *  the others wait for the bump. The wait is bounded: past 2^22 polls it records bit 8 in the overflow counter and lets
- *  the workgroup go on, so a barrier can give a wrong result but never hang the device. */
-__device__ __forceinline__ void gridBarrier(uint32_t *bar, uint32_t *counters, uint32_t &gen) {
+/** Grid-wide barrier for a cooperative (co-resident) grid. bar[0] counts arrivals over the whole
+ *  launch (the host zeroes it before each launch): a workgroup arriving at its k-th barrier adds 1
+ *  and waits until the count reaches k x gridDim.x, so a count never needs a reset between
+ *  barriers, and a workgroup that stopped waiting still arrives exactly once at every later barrier.
+ *  The wait is bounded: past `polls` polls it adds 1 to counters[C_BARRIER] (read by the host,
+ *  which then drops to per-colour launches) and lets the workgroup go on, so a barrier can give a
+ *  wrong result but never hang the device, nor spoil the barriers after it. */
+__device__ __forceinline__ void gridBarrier(uint32_t *bar, uint32_t *counters, uint32_t &target, uint32_t polls) {
   __syncthreads();
+  target += gridDim.x;
   if (threadIdx.x == 0) {
     __threadfence();
-    uint32_t arrived = atomicAdd(&bar[0], 1u);
-    if (arrived == gridDim.x - 1) {
-      atomicExch(&bar[0], 0u);
-      __threadfence();
-      atomicAdd(&bar[32], 1u);
-    } else {
-      // relaxed polls (no cache invalidation per poll), one acquire fence once released
-      uint32_t polls = 0;
-      while (__hip_atomic_load(&bar[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++polls > (1u << 22)) { atomicOr(&counters[C_OVERFLOW], 256u); break; }
-      }
+    atomicAdd(&bar[0], 1u);
+    // relaxed polls (no cache invalidation per poll), one acquire fence once released
+    uint32_t spins = 0;
+    while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > polls) { atomicAdd(&counters[C_BARRIER], 1u); break; }
     }
     __threadfence();
   }
-  gen++;
   __syncthreads();
 }
 
@@ -1460,25 +1457,22 @@ __device__ __forceinline__ void gridBarrier(uint32_t *bar, uint32_t *counters, u
  *  workgroup size: the body walk holds its contact batch in registers, which a 1024-thread bound
  *  (128 per lane) would spill. */
 template <uint32_t THREADS>
-__global__ void __launch_bounds__(THREADS) k_solve(Dev d, uint32_t *bar) {
-  __shared__ uint32_t genShared;
-  if (threadIdx.x == 0) genShared = __hip_atomic_load(&bar[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  uint32_t gen = genShared;
+__global__ void __launch_bounds__(THREADS) k_solve(Dev d, uint32_t *bar, uint32_t polls) {
+  uint32_t target = 0;
   uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   warmJoints(d, t0, nt);
-  gridBarrier(bar, d.counters, gen);
+  gridBarrier(bar, d.counters, target, polls);
   warmBodies(d, t0, nt);
-  gridBarrier(bar, d.counters, gen);
+  gridBarrier(bar, d.counters, target, polls);
   uint32_t colors = d.counters[C_NUM_COLORS];
   if (colors > d.p.colorCap) colors = d.p.colorCap;
   for (uint32_t it = 0; it < d.p.iterations; it++) {
     for (uint32_t c = 0; c < colors; c++) {
       primalColor(d, c, blockIdx.x, gridDim.x);
-      gridBarrier(bar, d.counters, gen);
+      gridBarrier(bar, d.counters, target, polls);
     }
     dualAll(d, t0, nt);
-    gridBarrier(bar, d.counters, gen);
+    gridBarrier(bar, d.counters, target, polls);
   }
   velocities(d, t0, nt);
 }
