@@ -136,6 +136,15 @@ int avbd_get_params(const avbd_solver *s, avbd_params *params);
 /* readBodies / rewriteBodies (solver.ts:2297, :1774): records [first, first+count). */
 int avbd_read_bodies(avbd_solver *s, float *out, uint32_t first, uint32_t count);
 int avbd_write_bodies(avbd_solver *s, const float *in, uint32_t first, uint32_t count);
+/* readBodies without allocating (no counterpart in the reference): for callers that cannot call
+ * hipMalloc, such as tight serving or render loops and code that must not allocate. Same records
+ * as avbd_read_bodies, staged through a fixed array of AVBD_READ_NOALLOC_MAX body records that the
+ * library keeps on each GPU for the life of the process (instead of the solver's lazily grown
+ * staging buffer). At most AVBD_READ_NOALLOC_MAX bodies per call (more: AVBD_ERR_INVALID, nothing
+ * read); read larger ranges in several calls. Every solver on one GPU shares that array: calls
+ * on the same GPU that overlap (solvers driven from different host threads) race. */
+#define AVBD_READ_NOALLOC_MAX 16384u
+int avbd_read_bodies_noalloc(avbd_solver *s, float *out, uint32_t first, uint32_t count);
 /* readJoints (solver.ts:2300) */
 int avbd_read_joints(avbd_solver *s, float *out, uint32_t first, uint32_t count);
 /* readCounters (solver.ts:2305) */

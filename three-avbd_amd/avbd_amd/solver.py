@@ -95,8 +95,9 @@ EXPORTS = [
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
     "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read", "avbd_push", "avbd_move", "avbd_reset",
-    "avbd_get_capacity", "avbd_set_barrier_polls", "avbd_read_refs",
+    "avbd_get_capacity", "avbd_set_barrier_polls", "avbd_read_refs", "avbd_read_bodies_noalloc",
 ]
+READ_NOALLOC_MAX = 16384  # avbd_read_bodies_noalloc: bodies per call
 EVENT_BEGIN, EVENT_END = 1, 2  # contacts.ts BEGIN / END
 
 

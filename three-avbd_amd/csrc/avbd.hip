@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include <stdexcept>
@@ -18,9 +19,33 @@
 
 using namespace avbd;
 
+// Staging of avbd_read_bodies_noalloc: file-scope device memory, one copy on every GPU the library
+// runs on (held until the process ends), room for AVBD_READ_NOALLOC_MAX body records. Host code
+// never passes the array's name to a kernel: that name is a host-side stand-in. Each device's
+// address comes from hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) float4 scratch[10u * AVBD_READ_NOALLOC_MAX];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+const int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mutex;
+float4 *g_scratch_ptr[SCRATCH_DEVICES] = {};
+bool g_scratch_tried[SCRATCH_DEVICES] = {};
+
+/** The address of `scratch` on device `dev` (the current device), looked up on its first use. */
+float4 *scratchOn(int dev) {
+  if (dev < 0 || dev >= SCRATCH_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_tried[dev]) {
+    g_scratch_tried[dev] = true;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[dev] = (float4 *)ptr;
+    else (void)hipGetLastError();
+  }
+  return g_scratch_ptr[dev];
+}
 
 struct DevBuf {
   void *p = nullptr;
@@ -150,6 +175,8 @@ struct avbd_solver {
   uint32_t *scanSums = nullptr;
   float4 *staging = nullptr;
   size_t stagingBytes = 0;
+  int device = 0;                    // the GPU the solver was created on (current at avbd_create)
+  float4 *readScratch = nullptr;     // that GPU's `scratch` (avbd_read_bodies_noalloc)
   KernelTimer timer;
   // ---- adjoint: checkpoints of the forward rollout, the recorded segment, f64 adjoints ----
   struct Checkpoint {
@@ -831,6 +858,8 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     {
       int dev = 0, coop = 0, cus = 0;
       (void)hipGetDevice(&dev);
+      s->device = dev;
+      s->readScratch = scratchOn(dev);
       (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
       s->numCUs = (uint32_t)(cus > 0 ? cus : 0);
@@ -1024,6 +1053,19 @@ int avbd_read_bodies(avbd_solver *s, float *out, uint32_t first, uint32_t count)
   if (int rc = ensureStaging(s, bytes)) return rc;
   k_bodies_to_records<<<blocksFor(count, 256), 256, 0, s->stream>>>(s->dev(), s->staging, first, count);
   hipMemcpyAsync(out, s->staging, bytes, hipMemcpyDeviceToHost, s->stream);
+  return avbd_sync(s);
+}
+int avbd_read_bodies_noalloc(avbd_solver *s, float *out, uint32_t first, uint32_t count) {
+  if (!s || (!out && count)) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if ((uint64_t)first + count > s->n) return fail(s, AVBD_ERR_INVALID, "body range out of bounds");
+  if (count > AVBD_READ_NOALLOC_MAX)
+    return fail(s, AVBD_ERR_INVALID, "avbd_read_bodies_noalloc reads at most " + std::to_string(AVBD_READ_NOALLOC_MAX) +
+                                         " bodies per call (asked for " + std::to_string(count) + ")");
+  if (!count) return AVBD_OK;
+  if (!s->readScratch) return fail(s, AVBD_ERR_DEVICE, "the readback scratch has no address on this device");
+  k_bodies_to_records<<<blocksFor(count, 256), 256, 0, s->stream>>>(s->dev(), s->readScratch, first, count);
+  hipMemcpyAsync(out, s->readScratch, 160ull * count, hipMemcpyDeviceToHost, s->stream);
   return avbd_sync(s);
 }
 int avbd_write_bodies(avbd_solver *s, const float *in, uint32_t first, uint32_t count) {
