@@ -257,6 +257,81 @@ int avbd_watch_run(avbd_solver *s, uint32_t pairs, uint32_t step, float dt);
  * `capacity` copied to out), how many did not fit (the watch's capacity, then out's), and whether
  * a run since the last read had more touching pairs than a set holds. Clears them. */
 int avbd_watch_read(avbd_solver *s, avbd_contact_event *out, uint32_t capacity, uint32_t *count, uint32_t *dropped, uint32_t *full);
+/* ---- Scene edits: what World.flush (src/lib/world.ts:1861-1995) calls between steps ----------
+ * Each is refused before anything changes when its input is invalid (AVBD_ERR_INVALID) or, with
+ * AVBD_FLAG_FIXED_STORAGE, does not fit (AVBD_ERR_CAPACITY, the reference's StorageCapacityError).
+ * An edit makes the next step detect contacts afresh (the reference's manifoldsValid = false). */
+/* jointInfo (solver.ts:2356): per joint slot its type (1 joint, 2 spring, 0 none/released), body a
+ * (-1: the world), body b, 0. */
+int avbd_read_joint_info(const avbd_solver *s, int32_t *out, uint32_t first, uint32_t count);
+/* Body and joint counts now (bodyCount, jointCount: slots in use or released). */
+int avbd_get_sizes(const avbd_solver *s, uint32_t *bodies, uint32_t *joints);
+/* addBodies (solver.ts:1544): appends body records after the last body, as many as fit in the body
+ * capacity (avbd_capacity.bodies); returns how many were added and sets *first to the first one's
+ * index (-1 when none fit). New bodies start uncoloured; a record's x- .w word (1: a new occupant,
+ * no contact history) is its lifecycle word, as the reference's pack writes it. Not on a batched
+ * solver (its worlds are fixed). */
+int avbd_add_bodies(avbd_solver *s, const float *bodies, uint32_t count, int32_t *first);
+/* patchBody (solver.ts:1103): from `record` (a whole 40-float body record), the fields in `fields`
+ * (AVBD_FIELD_*): position writes pos, x- and the inertial position; rotation rot, q- and the
+ * inertial rotation; velocity and angular velocity their xyz. `fixed`: also mass and moments (a
+ * body made fixed or dynamic; it starts uncoloured). `marker`: the lifecycle word becomes
+ * "rewritten" with the body's current lifecycle bits. */
+#define AVBD_FIELD_POSITION 1u
+#define AVBD_FIELD_ROTATION 2u
+#define AVBD_FIELD_VELOCITY 4u
+#define AVBD_FIELD_ANGULAR_VELOCITY 8u
+int avbd_patch_body(avbd_solver *s, uint32_t index, uint32_t fields, const float *record, int fixed, int marker);
+/* rewriteFixed (solver.ts:1140): bodies indices[k] become fixed boxes of `size` (3 floats, full
+ * widths) and `friction` at positions[3k..] turned by rotations[4k..] (NULL: unturned), at rest. */
+int avbd_rewrite_fixed(avbd_solver *s, const uint32_t *indices, uint32_t count, const float *positions, const float *size,
+                       float friction, const float *rotations);
+/* JointSpec (solver.ts:60-124). NaN marks a field left out. */
+typedef struct avbd_joint_spec {
+  int32_t a, b;        /* body indices */
+  float rA[3], rB[3];  /* anchors, body-local */
+  float angular;       /* angle-lock stiffness (0: a ball joint); NaN: rigid */
+  float fracture;      /* NaN: the call's; INFINITY: never */
+  int32_t linear;      /* breaks on the linear force too: 0/1, -1: the call's */
+  float rest[4];       /* the turn the angle lock holds (x, y, z, w, normalised); all 0: none */
+  float yield;         /* plastic: angular force past which it bends and keeps the bend; NaN: none */
+  float break_bend;    /* plastic: bend (rad, 0..pi) past which it breaks; NaN/INFINITY: never */
+} avbd_joint_spec;
+/* appendJoints (solver.ts:1637): joints into released slots (lowest first) then new ones, written
+ * in one upload; slots[k] receives spec k's slot. The joint storage grows unless fixed. */
+int avbd_append_joints(avbd_solver *s, const avbd_joint_spec *specs, uint32_t count, float fracture, int linear, uint32_t *slots);
+/* appendJoint (solver.ts:1559): one joint with the given stiffnesses (INFINITY: rigid), a = -1 for a
+ * world point rA; appended after the last slot unless the storage is fixed. */
+int avbd_append_joint(avbd_solver *s, int32_t a, int32_t b, const float *rA, const float *rB, float stiff_lin, float stiff_ang,
+                      uint32_t *slot);
+/* appendWorldJoint (solver.ts:1592), its 'legacy' policy: a hard ball joint from a world point to a
+ * point of body b (the mouse drag). */
+int avbd_append_world_joint(avbd_solver *s, uint32_t b, const float *anchor_world, const float *anchor_body, uint32_t *slot);
+/* appendSprings (solver.ts:1685): springs pulling rA on a towards rB on b (a = -1: a world point)
+ * with `stiffness` (N/m) about `rest` metres apart. */
+typedef struct avbd_spring_spec {
+  int32_t a, b;
+  float rA[3], rB[3];
+  float stiffness, rest;
+} avbd_spring_spec;
+int avbd_append_springs(avbd_solver *s, const avbd_spring_spec *springs, uint32_t count, uint32_t *slots);
+/* releaseJoints (solver.ts:1745): slots switched off and zeroed, their bodies colliding again, the
+ * slots reused by later appends; released or unknown slots are ignored. */
+int avbd_release_joints(avbd_solver *s, const uint32_t *slots, uint32_t count);
+/* setWorldAnchor (solver.ts:1937): a world joint's world point (3 floats). */
+int avbd_set_world_anchor(avbd_solver *s, uint32_t slot, const float *p);
+/* disableConstraint (solver.ts:1942): penalties, stiffnesses and multipliers zeroed (fracture and
+ * torque arm kept): the joint stops acting. */
+int avbd_disable_constraint(avbd_solver *s, uint32_t slot);
+/* setBodyAux (solver.ts:1814): body indices[k] falls with gravity x scales[k] and, if dynamic, its
+ * velocities decay by exp(-rate dt) per step (linear[k], angular[k] in 1/s; both NULL: a run of
+ * more than one slot resets the rates to 0, a single slot keeps them), from the next step. */
+int avbd_set_body_aux(avbd_solver *s, const uint32_t *indices, uint32_t count, const float *scales, const float *linear,
+                      const float *angular);
+/* setLifecycle (solver.ts:2069): lifecycle bits 0, 2 (disabled), 4 (cold) or 6; the body starts
+ * uncoloured and its contact history goes. */
+int avbd_set_lifecycle(avbd_solver *s, uint32_t index, uint32_t bits);
+
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
  * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`.
  * When per-colour launches win, it also times, against the params' primal_lanes rule, eight

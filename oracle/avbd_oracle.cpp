@@ -294,6 +294,13 @@ void omodel_set_fixed_colors(void *h, const uint32_t *colors) {
 void omodel_set_color_state(void *h, const uint32_t *state) { DISPATCH(h, M->colorA.assign(state, state + M->bodies.size())); }
 void omodel_set_step_count(void *h, uint32_t s) { DISPATCH(h, M->stepCount = s); }
 void omodel_set_threads(void *h, int n) { DISPATCH(h, M->threads = n < 1 ? 1 : n); }
+// the aux records (4 per body: gravity scale, linear and angular decay factors, damped 0/1)
+void omodel_set_aux(void *h, const double *r) {
+  DISPATCH(h, for (size_t i = 0; i < M->bodies.size(); i++) {
+    const double *o = r + 4 * i;
+    M->bodies[i].auxScale = o[0]; M->bodies[i].auxLin = o[1]; M->bodies[i].auxAng = o[2]; M->bodies[i].damped = o[3] != 0.0;
+  });
+}
 // updateRefs' reference poses (8 per body: pos xyz, 0, rot xyzw); a zero rotation is "none yet"
 void omodel_set_refs(void *h, const double *r) {
   DISPATCH(h, for (size_t i = 0; i < M->bodies.size(); i++) {

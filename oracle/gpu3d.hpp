@@ -93,6 +93,9 @@ template <class T> struct Body {
   V3<T> pos; T friction; Q4<T> rot; V3<T> ipos; uint32_t lifecycle; Q4<T> irot;
   V3<T> size; T mass; V3<T> moment; T radius; V3<T> inpos; uint32_t stamp; Q4<T> inrot;
   V3<T> vel; T prevUp; V3<T> angvel; int shape; uint32_t hull;  // hull: header offset (vec4 units) when shape == HULL
+  // the aux record (GpuSolver3D.setBodyAux, solver.ts:1814): gravity scale, the per-step velocity
+  // decay factors exp(-rate dt) as f32, and whether either rate is nonzero
+  T auxScale = T(1), auxLin = T(1), auxAng = T(1); bool damped = false;
   /** lifecycleBits / bodyEnabled / bodyCold (layout.ts:26-32): 1 rewritten slot, 2 disabled, 4 cold. */
   uint32_t bits() const { return lifecycle <= 7u ? lifecycle : 0u; }
   bool enabled() const { return (bits() & 2u) == 0u; }
@@ -1006,19 +1009,46 @@ template <class T> struct Model {
       penLin = O::v(O::mn(np.x, cap), O::mn(np.y, cap), O::mn(np.z, cap));
     }
     T frac = k0.fracture, limit = std::fabs(frac);
+    bool yielded = false;
     if (O::dot(penAng, penAng) > T(0)) {
       V C = jointAngRows(k0, alpha);
-      if (hard(stAng)) lamAng = O::add(O::mulv(penAng, C), lamAng);
-      T cap = O::mn(stAng, T(1e10));
-      V np = O::add(penAng, O::mul(O::vabs(C), T(p.betaAng)));
-      penAng = O::v(O::mn(np.x, cap), O::mn(np.y, cap), O::mn(np.z, cap));
+      if (hard(stAng)) {
+        lamAng = O::add(O::mulv(penAng, C), lamAng);
+        // a plastic joint (wgsl-solve.ts:586-599): past yield, short of fracture, the angular
+        // force is cut back to yield and the bend becomes the new rest
+        T yieldForce = k0.yield;
+        if (yieldForce < BIG) {
+          T force = O::length(lamAng);
+          if (force > yieldForce && !(limit < BIG && force > limit)) {
+            lamAng = O::mul(lamAng, yieldForce / force);
+            yielded = true;
+          }
+        }
+      }
+      if (!yielded) {  // a yielding row is at its bound: no penalty ramp
+        T cap = O::mn(stAng, T(1e10));
+        V np = O::add(penAng, O::mul(O::vabs(C), T(p.betaAng)));
+        penAng = O::v(O::mn(np.x, cap), O::mn(np.y, cap), O::mn(np.z, cap));
+      }
+    }
+    // the rest moves to where the bodies are; bent past breakBend from the rest it started with,
+    // the joint breaks (wgsl-solve.ts:606-614)
+    Q rest = k0.rest;
+    bool bentOff = false;
+    if (yielded) {
+      rest = O::qnormalize(O::qmul(O::qconj(rotAOf(k0.a)), bodies[k0.b].rot));
+      if (k0.breakBend < BIG) {
+        Q dq = O::qmul(O::qconj(k0.restStart), rest);
+        bentOff = T(2.0) * std::atan2(O::length(O::v(dq.x, dq.y, dq.z)), std::fabs(dq.w)) > k0.breakBend;
+      }
     }
     bool linear = frac < T(0) && O::dot(lamLin, lamLin) > limit * limit;
-    if (limit < BIG && (O::dot(lamAng, lamAng) > limit * limit || linear)) {
+    if ((limit < BIG && (O::dot(lamAng, lamAng) > limit * limit || linear)) || bentOff) {
       penLin = O::v(T(0), T(0), T(0)); penAng = penLin; lamLin = penLin; lamAng = penLin;
       stLin = T(0); stAng = T(0);
     }
     k.penLin = penLin; k.penAng = penAng; k.lamLin = lamLin; k.lamAng = lamAng; k.stiffLin = stLin; k.stiffAng = stAng;
+    if (yielded) { k.rest = rest; k.c0Ang = O::v(T(0), T(0), T(0)); }
   }
   void dualManifold(uint32_t m, T alpha) {
     const ManifoldT<T> &mf = manifolds[m];
@@ -1069,7 +1099,7 @@ template <class T> struct Model {
     }
   }
   void warmStartBodies() {
-    T dt = dtT(), g = T(p.gravity);
+    T dt = dtT();
     V up = O::v(T(p.up[0]), T(p.up[1]), T(p.up[2]));
 #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1 && !tape)
     for (size_t bi = 0; bi < bodies.size(); bi++) {
@@ -1079,6 +1109,11 @@ template <class T> struct Model {
         continue;
       }
       bool dyn = b.mass > T(0);
+      T g = T(p.gravity) * b.auxScale;  // the aux variant's scaled gravity (wgsl-solve.ts:300)
+      if (dyn && b.damped) {  // DAMPING_WGSL (wgsl-solve.ts:14-20): decayed before use, and stored
+        b.vel = O::mul(b.vel, b.auxLin);
+        b.angvel = O::mul(b.angvel, b.auxAng);
+      }
       V inpos = O::add(b.pos, O::mul(b.vel, dt));
       if (dyn) inpos = O::add(inpos, O::mul(up, g * (dt * dt)));
       b.inpos = inpos;

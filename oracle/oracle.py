@@ -236,6 +236,12 @@ class Model:
     def set_step_count(self, s):
         self.L.omodel_set_step_count(self.h, s)
 
+    def set_aux(self, aux):
+        """setBodyAux's records (n x 4: gravity scale, the linear and angular per-step decay
+        factors exp(-rate dt) as f32, 1 when damped)."""
+        r = np.ascontiguousarray(aux, np.float64).reshape(self.n, 4)
+        self.L.omodel_set_aux(self.h, _p(r))
+
     def set_refs(self, refs):
         """updateRefs' reference poses (n x 8: pos xyz, 0, rot xyzw; a zero rotation: none yet)."""
         r = np.ascontiguousarray(refs, np.float64).reshape(self.n, 8)

@@ -1,4 +1,6 @@
 """Shared helpers for the parity tests (test code; uses the oracle as the checker)."""
+import math
+
 import numpy as np
 
 import oracle
@@ -170,3 +172,57 @@ def canonical_contacts(man, con):
     M[:, 2] = offs
     M[:, 3] = cnt
     return M, con[idx]
+
+
+# --- records as the reference's GpuSolver3D packs them (scene edits) ----------------------------
+BIG = np.float32(3.0e38)
+
+
+def fresh_joint():
+    """freshJoint (solver.ts:132-138)."""
+    o = np.zeros(40, np.float32)
+    o[11] = BIG
+    o[23] = BIG
+    o[31] = BIG
+    o[35] = 1.0
+    o[39] = 1.0
+    return o
+
+
+def torque_arm(sa, sb):
+    return np.float32(sum((float(x) + float(y)) ** 2 for x, y in zip(sa, sb)))
+
+
+def spec_record(spec, sizes, fracture=math.inf, linear=False):
+    """appendJoints' record of one spec (solver.ts:1661-1683)."""
+    fin = lambda x: BIG if x == math.inf else (-BIG if x == -math.inf else np.float32(x))
+    o = fresh_joint()
+    o[3] = BIG
+    o[7] = BIG if "angular" not in spec else fin(spec["angular"])
+    limit = fin(spec.get("fracture", fracture))
+    o[11] = -limit if spec.get("linear", linear) else limit
+    if "yield" in spec:
+        o[23] = fin(spec["yield"])
+    if "breakBend" in spec:
+        o[31] = fin(spec["breakBend"])
+    if "rest" in spec:
+        q = np.asarray(spec["rest"], np.float64)
+        q = (q / math.sqrt(float((q * q).sum()))).astype(np.float32)
+        o[32:36] = q
+        o[36:40] = q
+    o[24:27] = spec["rA"]
+    o[28:31] = spec["rB"]
+    o[15] = torque_arm(sizes[spec["a"]], sizes[spec["b"]])
+    return o
+
+
+def aux_factors(rate, dt):
+    x = -(np.float32(rate) * np.float32(dt))
+    return np.float32(math.exp(float(x)))
+
+
+def new_box(pos, size, density, friction=0.5, rot=(0, 0, 0, 1)):
+    from avbd_amd import records
+    r = records.box_record(pos, size, density, friction, rot=rot).astype(np.float32)
+    r[11] = 1.0  # a new occupant: no contact history (pack, solver.ts:1200)
+    return r

@@ -96,7 +96,24 @@ EXPORTS = [
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
     "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read", "avbd_push", "avbd_move", "avbd_reset",
     "avbd_get_capacity", "avbd_set_barrier_polls", "avbd_read_refs", "avbd_read_bodies_noalloc",
+    "avbd_read_joint_info", "avbd_get_sizes", "avbd_add_bodies", "avbd_patch_body", "avbd_rewrite_fixed",
+    "avbd_append_joints", "avbd_append_joint", "avbd_append_world_joint", "avbd_append_springs", "avbd_release_joints",
+    "avbd_set_world_anchor", "avbd_disable_constraint", "avbd_set_body_aux", "avbd_set_lifecycle",
 ]
+FIELD_POSITION, FIELD_ROTATION, FIELD_VELOCITY, FIELD_ANGULAR_VELOCITY = 1, 2, 4, 8
+_FIELDS = {"position": 1, "rotation": 2, "velocity": 4, "angularVelocity": 8}
+
+
+class JointSpec(ctypes.Structure):
+    """JointSpec (solver.ts:60-124); NaN marks a field left out."""
+    _fields_ = [("a", ctypes.c_int32), ("b", ctypes.c_int32), ("rA", ctypes.c_float * 3), ("rB", ctypes.c_float * 3),
+                ("angular", ctypes.c_float), ("fracture", ctypes.c_float), ("linear", ctypes.c_int32),
+                ("rest", ctypes.c_float * 4), ("yield_", ctypes.c_float), ("break_bend", ctypes.c_float)]
+
+
+class SpringSpec(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_int32), ("b", ctypes.c_int32), ("rA", ctypes.c_float * 3), ("rB", ctypes.c_float * 3),
+                ("stiffness", ctypes.c_float), ("rest", ctypes.c_float)]
 READ_NOALLOC_MAX = 16384  # avbd_read_bodies_noalloc: bodies per call
 EVENT_BEGIN, EVENT_END = 1, 2  # contacts.ts BEGIN / END
 
@@ -216,7 +233,15 @@ class GpuSolver3D:
     # --- params ------------------------------------------------------------------------------
     @property
     def params(self):
-        return self._params
+        """The params the next step runs with, from the library: a copy, so edits go back through
+        set_params (the adaptive colouring and the autotuner change the colour cap, the rounds
+        and the lane rule inside the library)."""
+        if self.destroyed or not getattr(self, "h", None):
+            return self._params
+        p = Params()
+        self._check(self.L.avbd_get_params(self.h, ctypes.byref(p)))
+        self._params = p
+        return Params.from_buffer_copy(p)
 
     def set_params(self, params):
         self._live()
@@ -412,6 +437,137 @@ class GpuSolver3D:
         self._live()
         rec = np.ascontiguousarray(records, np.float32).reshape(-1, BODY_FLOATS)
         self._check(self.L.avbd_write_bodies(self.h, _p(rec), first, len(rec)))
+
+    # --- scene edits (World.flush, world.ts:1861-1995) -----------------------------------------
+    def _sync_sizes(self):
+        nb, nj = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self.L.avbd_get_sizes(self.h, ctypes.byref(nb), ctypes.byref(nj)))
+        self.body_count, self.joint_count = nb.value, nj.value
+        self._joint_info = self.joint_info()
+
+    def joint_info(self):
+        """jointInfo (solver.ts:2356): (type, a, b, 0) per joint slot."""
+        out = np.zeros((self.joint_count, 4), np.int32)
+        self._check(self.L.avbd_read_joint_info(self.h, _p(out), 0, self.joint_count))
+        return out
+
+    def add_bodies(self, records):
+        """addBodies (solver.ts:1544): the first new index, or -1 when none fit."""
+        self._live()
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, BODY_FLOATS)
+        first = ctypes.c_int32(-1)
+        self._check(self.L.avbd_add_bodies(self.h, _p(rec), len(rec), ctypes.byref(first)))
+        self._sync_sizes()
+        return first.value
+
+    def patch_body(self, index, fields, record, fixed=False, marker=True):
+        """patchBody (solver.ts:1103): `fields` names from {position, rotation, velocity,
+        angularVelocity} taken from the 40-float `record`."""
+        self._live()
+        mask = 0
+        for f in fields:
+            mask |= _FIELDS[f]
+        rec = np.ascontiguousarray(record, np.float32).reshape(BODY_FLOATS)
+        self._check(self.L.avbd_patch_body(self.h, ctypes.c_uint32(index), ctypes.c_uint32(mask), _p(rec), int(fixed), int(marker)))
+
+    def rewrite_fixed(self, indices, positions, size, friction, rotations=None):
+        """rewriteFixed (solver.ts:1140)."""
+        self._live()
+        idx = np.ascontiguousarray(indices, np.uint32)
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        sz = np.ascontiguousarray(size, np.float32).reshape(3)
+        rot = None if rotations is None else np.ascontiguousarray(rotations, np.float32).reshape(-1, 4)
+        self._check(self.L.avbd_rewrite_fixed(self.h, _p(idx), len(idx), _p(pos), _p(sz), ctypes.c_float(friction),
+                                              None if rot is None else _p(rot)))
+
+    def append_joints(self, joints, fracture=float("inf"), linear=False):
+        """appendJoints (solver.ts:1637): dicts with a, b, rA, rB and optionally angular,
+        fracture, linear, rest, yield, breakBend; returns their slots."""
+        self._live()
+        nan = float("nan")
+        specs = (JointSpec * max(1, len(joints)))()
+        for k, j in enumerate(joints):
+            sp = specs[k]
+            sp.a, sp.b = int(j["a"]), int(j["b"])
+            sp.rA[:] = [float(x) for x in j["rA"]]
+            sp.rB[:] = [float(x) for x in j["rB"]]
+            sp.angular = j.get("angular", nan)
+            sp.fracture = j.get("fracture", nan)
+            sp.linear = -1 if j.get("linear") is None else int(bool(j["linear"]))
+            sp.rest[:] = [float(x) for x in j.get("rest", (0, 0, 0, 0))]
+            sp.yield_ = j.get("yield", nan)
+            sp.break_bend = j.get("breakBend", nan)
+        slots = np.zeros(max(1, len(joints)), np.uint32)
+        self._check(self.L.avbd_append_joints(self.h, specs, len(joints), ctypes.c_float(fracture), int(linear), _p(slots)))
+        self._sync_sizes()
+        return [int(x) for x in slots[:len(joints)]]
+
+    def append_joint(self, a, b, rA, rB, stiffness_lin, stiffness_ang):
+        """appendJoint (solver.ts:1559)."""
+        self._live()
+        ra = np.ascontiguousarray(rA, np.float32)
+        rb = np.ascontiguousarray(rB, np.float32)
+        slot = ctypes.c_uint32()
+        self._check(self.L.avbd_append_joint(self.h, int(a), int(b), _p(ra), _p(rb), ctypes.c_float(stiffness_lin),
+                                             ctypes.c_float(stiffness_ang), ctypes.byref(slot)))
+        self._sync_sizes()
+        return slot.value
+
+    def append_world_joint(self, b, anchor_world, anchor_body):
+        """appendWorldJoint (solver.ts:1592), legacy policy."""
+        self._live()
+        aw = np.ascontiguousarray(anchor_world, np.float32)
+        ab = np.ascontiguousarray(anchor_body, np.float32)
+        slot = ctypes.c_uint32()
+        self._check(self.L.avbd_append_world_joint(self.h, ctypes.c_uint32(b), _p(aw), _p(ab), ctypes.byref(slot)))
+        self._sync_sizes()
+        return slot.value
+
+    def append_springs(self, springs):
+        """appendSprings (solver.ts:1685): dicts with a, b, rA, rB, stiffness, rest."""
+        self._live()
+        specs = (SpringSpec * max(1, len(springs)))()
+        for k, sp in enumerate(springs):
+            specs[k].a, specs[k].b = int(sp["a"]), int(sp["b"])
+            specs[k].rA[:] = [float(x) for x in sp["rA"]]
+            specs[k].rB[:] = [float(x) for x in sp["rB"]]
+            specs[k].stiffness, specs[k].rest = sp["stiffness"], sp["rest"]
+        slots = np.zeros(max(1, len(springs)), np.uint32)
+        self._check(self.L.avbd_append_springs(self.h, specs, len(springs), _p(slots)))
+        self._sync_sizes()
+        return [int(x) for x in slots[:len(springs)]]
+
+    def release_joints(self, slots):
+        """releaseJoints (solver.ts:1745)."""
+        self._live()
+        s = np.ascontiguousarray(slots, np.uint32)
+        self._check(self.L.avbd_release_joints(self.h, _p(s), len(s)))
+        self._sync_sizes()
+
+    def set_world_anchor(self, slot, p):
+        """setWorldAnchor (solver.ts:1937)."""
+        self._live()
+        self._check(self.L.avbd_set_world_anchor(self.h, ctypes.c_uint32(slot), _p(np.ascontiguousarray(p, np.float32))))
+
+    def disable_constraint(self, slot):
+        """disableConstraint (solver.ts:1942)."""
+        self._live()
+        self._check(self.L.avbd_disable_constraint(self.h, ctypes.c_uint32(slot)))
+
+    def set_body_aux(self, indices, scales, linear=None, angular=None):
+        """setBodyAux (solver.ts:1814): gravity scales and, optionally, damping rates (1/s)."""
+        self._live()
+        idx = np.ascontiguousarray(indices, np.uint32)
+        sc = np.ascontiguousarray(scales, np.float32)
+        lin = None if linear is None else np.ascontiguousarray(linear, np.float32)
+        ang = None if angular is None else np.ascontiguousarray(angular, np.float32)
+        self._check(self.L.avbd_set_body_aux(self.h, _p(idx), len(idx), _p(sc), None if lin is None else _p(lin),
+                                             None if ang is None else _p(ang)))
+
+    def set_lifecycle(self, index, bits):
+        """setLifecycle (solver.ts:2069): 0, 2 (disabled), 4 (cold) or 6."""
+        self._live()
+        self._check(self.L.avbd_set_lifecycle(self.h, ctypes.c_uint32(index), ctypes.c_uint32(bits)))
 
     def read_joints(self, first=0, count=None):
         self._live()

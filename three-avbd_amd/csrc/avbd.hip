@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -175,6 +176,16 @@ struct avbd_solver {
   uint32_t *scanSums = nullptr;
   float4 *staging = nullptr;
   size_t stagingBytes = 0;
+  // host mirrors for the scene edits (World.flush, world.ts:1861-1995): per body its bounding
+  // radius and mass (uploadStatics reads them), its lifecycle bits (patchBody's marker), its
+  // gravity scale and damping rates (the aux factors follow dt); the released joint slots
+  std::vector<float> radiusHost, massHost, sizeHost;  // sizeHost: full widths xyz per body (torque arms)
+  std::vector<uint8_t> lifeHost;
+  std::vector<float> auxScale, auxLin, auxAng;
+  float4 *aux = nullptr;  // per body: scale, exp(-lin dt), exp(-ang dt), damped (null: no aux yet)
+  std::vector<uint32_t> freeSlots;
+  bool staticsDirty = false;  // radii, masses or joint pairs changed: uploadStatics before the next step
+  bool seeding = false;       // avbd_seed's body write (colours kept)
   int device = 0;                    // the GPU the solver was created on (current at avbd_create)
   float4 *readScratch = nullptr;     // that GPU's `scratch` (avbd_read_bodies_noalloc)
   KernelTimer timer;
@@ -387,6 +398,7 @@ struct avbd_solver {
     d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
+    d.aux = aux;
     d.src = rec ? src : nullptr;
     d.bits = nullptr;
     return d;
@@ -749,6 +761,96 @@ struct avbd_solver {
     rec = nullptr;
   }
 
+  bool fixedStorage() const { return (hp.flags & AVBD_FLAG_FIXED_STORAGE) != 0; }
+  /** uploadStatics when a scene edit changed radii, masses, the body count or joint pairs (the
+   *  reference's radiiDirty, solver.ts:2119): cell size, large lists, no-collide list and the
+   *  colouring priorities of every body. */
+  void refreshStatics() {
+    if (!staticsDirty) return;
+    std::vector<float> r(radiusHost.begin(), radiusHost.begin() + n), m(massHost.begin(), massHost.begin() + n);
+    uploadStatics(r, m);
+    Dev d = dev();
+    k_priorities<<<blocksFor(std::max(n, 1u), 256), 256, 0, stream>>>(d);
+    staticsDirty = false;
+  }
+  /** A body's aux record: gravity scale, the velocity decay factors per step exp(-rate dt) with
+   *  the exponent rounded to f32 as the reference's exp(-aux.y * dt) takes it, and 1 when damped. */
+  float4 auxRecord(uint32_t i) const {
+    float lin = auxLin[i], ang = auxAng[i], dt = hp.dt;
+    float xl = -(lin * dt), xa = -(ang * dt);
+    return make_float4(auxScale[i], (float)std::exp((double)xl), (float)std::exp((double)xa), (lin > 0.0f || ang > 0.0f) ? 1.0f : 0.0f);
+  }
+  void uploadAux(uint32_t first, uint32_t count) {
+    if (!aux || !count) return;
+    std::vector<float4> r(count);
+    for (uint32_t k = 0; k < count; k++) r[k] = auxRecord(first + k);
+    hipMemcpyAsync(aux + first, r.data(), 16ull * count, hipMemcpyHostToDevice, stream);
+    hipStreamSynchronize(stream);
+  }
+  /** Joint storage for `cap` joints (the reference's allocateJoints, doubling): records, info and
+   *  the adjacency sized by it. */
+  void growJoints(uint32_t cap) {
+    if (cap <= jointCap) return;
+    hipStreamSynchronize(stream);
+    joints = regrow(joints, 10ull * jointCap, 10ull * cap, true);
+    jinfo = regrow(jinfo, jointCap, cap, true);
+    uint32_t ac = 2u * (cap + manCap);
+    adjList = regrow(adjList, adjCap, ac, false);
+    adjTmp = regrow(adjTmp, adjCap, ac, false);
+    adjOwner = regrow(adjOwner, adjCap, ac, false);
+    adjKeys = regrow(adjKeys, adjCap, ac, false);
+    adjOther = regrow(adjOther, adjCap, ac, false);
+    adjIds = regrow(adjIds, adjCap, ac, false);
+    adjGeo = regrow(adjGeo, adjCap, ac, false);
+    adjCap = ac;
+    jointCap = cap;
+    adjacencyHeld = false;
+    hipStreamSynchronize(stream);
+  }
+  /** Body records [first, first + count) from host memory (rewriteBodies / addBodies' writeBodies,
+   *  solver.ts:1087): the host mirrors follow; uncoloured unless `keepColours` (seedFrom). */
+  void writeRecords(const float *in, uint32_t first, uint32_t count, bool keepColours) {
+    for (uint32_t k = 0; k < count; k++) {
+      const float *r = in + 40ull * k;
+      uint32_t i = first + k;
+      if (radiusHost[i] != r[23] || (massHost[i] > 0.0f) != (r[19] > 0.0f) || massHost[i] != r[19]) staticsDirty = true;
+      radiusHost[i] = r[23];
+      massHost[i] = r[19];
+      for (int c = 0; c < 3; c++) sizeHost[3ull * i + c] = r[16 + c];
+      float w = r[11];
+      lifeHost[i] = (w >= 0.0f && w <= 7.0f && w == std::floor(w)) ? (uint8_t)((uint32_t)w & 6u) : 0;
+    }
+    hipMemcpyAsync(staging, in, 160ull * count, hipMemcpyHostToDevice, stream);
+    k_records_to_bodies<<<blocksFor(count, 256), 256, 0, stream>>>(dev(), staging, first, count);
+    if (!keepColours) { fill(colA + first, count, NO_COLOR); fill(colB + first, count, NO_COLOR); }
+    manifoldsValid = false;
+  }
+  /** Joint or spring records into `slots` with their info, a copy per run of consecutive slots
+   *  (appendConstraints' uploads, solver.ts:1718-1725); the no-collide list follows. */
+  void putJoints(const std::vector<uint32_t> &slots, const std::vector<float> &recs, const std::vector<int> &ends) {
+    std::vector<uint32_t> order(slots.size());
+    for (size_t k = 0; k < slots.size(); k++) order[k] = (uint32_t)k;
+    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return slots[x] < slots[y]; });
+    std::vector<float> run;
+    for (size_t k = 0; k < order.size();) {
+      size_t e = k + 1;
+      while (e < order.size() && slots[order[e]] == slots[order[e - 1]] + 1) e++;
+      run.assign(40 * (e - k), 0.0f);
+      for (size_t i = k; i < e; i++) {
+        uint32_t q = order[i], slot = slots[q];
+        std::copy(recs.begin() + 40 * q, recs.begin() + 40 * q + 40, run.begin() + 40 * (i - k));
+        int32_t *in = &jinfoHost[4ull * slot];
+        in[0] = ends[3 * q]; in[1] = ends[3 * q + 1]; in[2] = ends[3 * q + 2]; in[3] = 0;
+      }
+      uint32_t first = slots[order[k]];
+      hipMemcpyAsync(joints + 10ull * first, run.data(), 160ull * (e - k), hipMemcpyHostToDevice, stream);
+      hipMemcpyAsync(jinfo + first, &jinfoHost[4ull * first], 16ull * (e - k), hipMemcpyHostToDevice, stream);
+      hipStreamSynchronize(stream);
+      k = e;
+    }
+    manifoldsValid = false;
+    staticsDirty = true;
+  }
   /** A cooperative launch whose grid barrier gave up waiting ran part of its solve unordered: that
    *  step's result is not the step's. From then on the solver uses per-colour launches; the
    *  give-ups are reported (bit AVBD_OVERFLOW_BARRIER of the counters' overflow word, and an error
@@ -766,6 +868,7 @@ struct avbd_solver {
    *  nothing of the host's in between (the substeps of one avbd_steps call), so a reuse that keeps
    *  every manifold may keep the adjacency too; any other call rebuilds it. */
   void step(bool reuseManifolds, bool keepAdjacency = false) {
+    refreshStatics();
     stepCount++;
     reuseManifolds = reuseManifolds && manifoldsValid;
     collision(reuseManifolds, reuseManifolds && keepAdjacency && adjacencyHeld);
@@ -960,21 +1063,27 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     std::vector<uint32_t> f(2ull * N, 0xffffffffu);
     hipMemcpyAsync(s->filters, f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream);
     // bodies: records -> SoA streams; new bodies have no contact history (DIRTY, layout.ts:17)
-    std::vector<float> radius(body_count), mass(body_count);
+    s->radiusHost.assign(s->bodyCap, 0.0f);
+    s->massHost.assign(s->bodyCap, 0.0f);
+    s->sizeHost.assign(3ull * s->bodyCap, 0.0f);
+    s->lifeHost.assign(s->bodyCap, 0);
+    s->jinfoHost.assign(4ull * s->jointCap, 0);
+    for (uint32_t j = 0; j < s->jointCap; j++) { s->jinfoHost[4ull * j + 1] = -1; s->jinfoHost[4ull * j + 2] = -1; }
     if (body_count) {
-      std::vector<float> rec(bodies, bodies + 40ull * body_count);
-      for (uint32_t i = 0; i < body_count; i++) { radius[i] = rec[40ull * i + 23]; mass[i] = rec[40ull * i + 19]; }
-      int rc = avbd_write_bodies(s, rec.data(), 0, body_count);
+      int rc = avbd_write_bodies(s, bodies, 0, body_count);
       if (rc) throw std::runtime_error(s->err);
     }
     if (joint_count) {
       std::vector<float> jr(jrec, jrec + 40ull * joint_count);
       for (auto &x : jr) x = finiteF(x);
       hipMemcpyAsync(s->joints, jr.data(), jr.size() * 4, hipMemcpyHostToDevice, s->stream);
-      s->jinfoHost.assign(jinfo, jinfo + 4ull * joint_count);
-      hipMemcpyAsync(s->jinfo, s->jinfoHost.data(), s->jinfoHost.size() * 4, hipMemcpyHostToDevice, s->stream);
+      std::copy(jinfo, jinfo + 4ull * joint_count, s->jinfoHost.begin());
+      hipMemcpyAsync(s->jinfo, s->jinfoHost.data(), 16ull * joint_count, hipMemcpyHostToDevice, s->stream);
     }
+    std::vector<float> radius(s->radiusHost.begin(), s->radiusHost.begin() + body_count);
+    std::vector<float> mass(s->massHost.begin(), s->massHost.begin() + body_count);
     s->uploadStatics(radius, mass);
+    s->staticsDirty = false;
     {
       Dev d = s->dev();
       k_priorities<<<blocksFor(s->n, 256), 256, 0, s->stream>>>(d);  // the colouring's fixed priorities
@@ -999,7 +1108,9 @@ int avbd_set_params(avbd_solver *s, const avbd_params *p) {
   if (!s) return AVBD_ERR_INVALID;
   std::string why;
   if (!checkParams(p, why)) return fail(s, AVBD_ERR_INVALID, why);
+  bool dtChanged = p->dt != s->hp.dt;
   s->hp = *p;
+  if (dtChanged && s->aux) s->uploadAux(0, s->bodyCap);  // the damping factors are per step of dt
   return AVBD_OK;
 }
 int avbd_get_params(const avbd_solver *s, avbd_params *p) {
@@ -1035,6 +1146,18 @@ int avbd_sync(avbd_solver *s) {
   return AVBD_OK;
 }
 
+/** Shape codes a record may carry: box 0, sphere 1, hull 3 + a header offset (vec4 units) inside
+ *  the hull buffer's capacity. Sails (2) have no wind model here and are refused. */
+static int checkShapes(avbd_solver *s, const float *in, uint32_t count) {
+  for (uint32_t k = 0; k < count; k++) {
+    float code = in[40ull * k + 39];
+    if (code == 0.0f || code == 1.0f) continue;
+    if (code == 2.0f) return fail(s, AVBD_ERR_INVALID, "body " + std::to_string(k) + ": sails (shape 2) are not supported");
+    if (!(code >= 3.0f) || code != std::floor(code) || ((double)code - 3.0) * 4.0 + 4.0 > (double)s->hullCap)
+      return fail(s, AVBD_ERR_INVALID, "body " + std::to_string(k) + ": shape code " + std::to_string(code) + " is no box, sphere or hull in the hull buffer");
+  }
+  return AVBD_OK;
+}
 static int ensureStaging(avbd_solver *s, size_t bytes) {
   if (s->stagingBytes >= bytes) return AVBD_OK;
   if (s->staging) hipFree(s->staging);
@@ -1074,10 +1197,9 @@ int avbd_write_bodies(avbd_solver *s, const float *in, uint32_t first, uint32_t 
   for (uint64_t k = 0; k < 40ull * count; k++)
     if (!std::isfinite(in[k]) && (k % 40) != 27) return fail(s, AVBD_ERR_INVALID, "body record holds a non-finite value");
   if (!count) return AVBD_OK;
-  size_t bytes = 160ull * count;
-  if (int rc = ensureStaging(s, bytes)) return rc;
-  hipMemcpyAsync(s->staging, in, bytes, hipMemcpyHostToDevice, s->stream);
-  k_records_to_bodies<<<blocksFor(count, 256), 256, 0, s->stream>>>(s->dev(), s->staging, first, count);
+  if (int rc = checkShapes(s, in, count)) return rc;
+  if (int rc = ensureStaging(s, 160ull * count)) return rc;
+  s->writeRecords(in, first, count, s->seeding);
   return avbd_sync(s);
 }
 int avbd_read_joints(avbd_solver *s, float *out, uint32_t first, uint32_t count) {
@@ -1163,7 +1285,10 @@ int avbd_seed(avbd_solver *s, const float *bodies, uint32_t body_count, const fl
     const uint32_t *o = man + 8ull * m;
     if (o[0] >= s->n || o[1] >= s->n || (uint64_t)o[2] + (o[3] & 15u) > nc) return fail(s, AVBD_ERR_INVALID, "seedFrom: manifold out of range");
   }
-  if (int rc = avbd_write_bodies(s, bodies, 0, body_count)) return rc;
+  s->seeding = true;  // seedFrom keeps the colour state (solver.ts:1324)
+  int wrc = avbd_write_bodies(s, bodies, 0, body_count);
+  s->seeding = false;
+  if (wrc) return wrc;
   if (joint_count) {
     std::vector<float> jr(jrec, jrec + 40ull * joint_count);
     for (auto &x : jr) x = finiteF(x);
@@ -1214,11 +1339,8 @@ int avbd_add_ignore_pairs(avbd_solver *s, const uint32_t *pairs, uint32_t count)
     uint32_t a = pairs[2 * k], b = pairs[2 * k + 1];
     s->ignorePairs.push_back({std::max(a, b), std::min(a, b), IGNORE_SLOT});
   }
-  std::vector<float> rec(40ull * s->n);
-  if (int rc = avbd_read_bodies(s, rec.data(), 0, s->n)) return rc;
-  std::vector<float> radius(s->n), mass(s->n);
-  for (uint32_t i = 0; i < s->n; i++) { radius[i] = rec[40ull * i + 23]; mass[i] = rec[40ull * i + 19]; }
-  s->uploadStatics(radius, mass);
+  s->staticsDirty = true;
+  s->refreshStatics();
   s->manifoldsValid = false;
   return avbd_sync(s);
 }
@@ -1647,6 +1769,386 @@ int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, doub
   } catch (const std::exception &e) {
     return fail(s, AVBD_ERR_DEVICE, e.what());
   }
+}
+
+}  // extern "C"
+
+// ---- Scene edits (World.flush, world.ts:1861-1995) -------------------------------------------
+
+namespace {
+const float JBIG = 3.0e38f;
+float finiteOr(float x) { return std::isinf(x) ? (x > 0 ? JBIG : -JBIG) : x; }
+bool finite32(float x) { return std::isfinite(x); }
+/** freshJoint (solver.ts:132-138): never breaks or yields, rest and starting rest no turn. */
+void freshJoint(float *o) {
+  std::fill(o, o + 40, 0.0f);
+  o[11] = JBIG;  // J_LAM_LIN + 3: fracture
+  o[23] = JBIG;  // J_YIELD
+  o[31] = JBIG;  // J_BREAK_BEND
+  o[35] = 1.0f;  // J_REST + 3
+  o[39] = 1.0f;  // J_REST_START + 3
+}
+/** The torque arm of a joint between sizes sa and sb (the reference Joint constructor). */
+float torqueArm(const float *sa, const float *sb) {
+  double x = (double)sa[0] + sb[0], y = (double)sa[1] + sb[1], z = (double)sa[2] + sb[2];
+  return (float)(x * x + y * y + z * z);
+}
+}  // namespace
+
+/** The solver's slots for `n` new constraints (appendConstraints, solver.ts:1697-1727): released
+ *  slots first, lowest first, then appended ones; grows the joint storage, or with fixed storage
+ *  refuses before anything changes. */
+static int takeSlots(avbd_solver *s, uint32_t n, std::vector<uint32_t> &slots) {
+  if (s->fixedStorage() && (uint64_t)n > s->freeSlots.size() + (uint64_t)s->jointCap - s->nj)
+    return fail(s, AVBD_ERR_CAPACITY, "StorageCapacityError: joints " + std::to_string(s->nj - s->freeSlots.size() + n) +
+                                          " > capacity " + std::to_string(s->jointCap));
+  std::sort(s->freeSlots.begin(), s->freeSlots.end(), [](uint32_t x, uint32_t y) { return x > y; });
+  while (slots.size() < n && !s->freeSlots.empty()) { slots.push_back(s->freeSlots.back()); s->freeSlots.pop_back(); }
+  uint32_t fresh = n - (uint32_t)slots.size();
+  uint32_t cap = s->jointCap;
+  while ((uint64_t)s->nj + fresh > cap) cap *= 2;
+  try {
+    if (cap > s->jointCap) {
+      s->growJoints(cap);
+      s->jinfoHost.resize(4ull * cap, 0);
+      for (uint32_t j = s->nj; j < cap; j++) { s->jinfoHost[4ull * j + 1] = -1; s->jinfoHost[4ull * j + 2] = -1; }
+    }
+  } catch (const std::exception &e) {
+    for (uint32_t x : slots) s->freeSlots.push_back(x);
+    return fail(s, AVBD_ERR_DEVICE, e.what());
+  }
+  for (uint32_t k = 0; k < fresh; k++) slots.push_back(s->nj++);
+  return AVBD_OK;
+}
+static bool isBody(const avbd_solver *s, int32_t i) { return i >= 0 && (uint32_t)i < s->n; }
+static const float ZERO3[3] = {0.0f, 0.0f, 0.0f};
+static const float *sizeOf(const avbd_solver *s, int32_t i) { return i < 0 ? ZERO3 : &s->sizeHost[3ull * i]; }
+
+extern "C" {
+
+int avbd_read_joint_info(const avbd_solver *s, int32_t *out, uint32_t first, uint32_t count) {
+  if (!s || (!out && count)) return AVBD_ERR_INVALID;
+  if ((uint64_t)first + count > s->nj) return AVBD_ERR_INVALID;
+  std::copy(s->jinfoHost.begin() + 4ull * first, s->jinfoHost.begin() + 4ull * (first + count), out);
+  return AVBD_OK;
+}
+int avbd_get_sizes(const avbd_solver *s, uint32_t *bodies, uint32_t *joints) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (bodies) *bodies = s->n;
+  if (joints) *joints = s->nj;
+  return AVBD_OK;
+}
+
+int avbd_add_bodies(avbd_solver *s, const float *bodies, uint32_t count, int32_t *first) {
+  if (!s || (!bodies && count)) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (first) *first = -1;
+  if (s->worldStartHost.size() != 2) return fail(s, AVBD_ERR_INVALID, "addBodies: a batched solver's worlds are fixed");
+  uint32_t k = std::min(count, s->bodyCap - s->n);
+  if (k == 0) return 0;
+  for (uint64_t i = 0; i < 40ull * k; i++)
+    if (!std::isfinite(bodies[i]) && (i % 40) != 27) return fail(s, AVBD_ERR_INVALID, "body record holds a non-finite value");
+  if (int rc = checkShapes(s, bodies, k)) return rc;
+  if (int rc = ensureStaging(s, 160ull * k)) return rc;
+  uint32_t at = s->n;
+  s->n += k;
+  s->writeRecords(bodies, at, k, false);
+  s->worldStartHost[1] = s->n;
+  hipMemcpyAsync(s->worldStart + 1, &s->worldStartHost[1], 4, hipMemcpyHostToDevice, s->stream);
+  s->staticsDirty = true;
+  s->adjacencyHeld = false;
+  if (int rc = avbd_sync(s)) return rc;
+  if (s->aux) s->uploadAux(at, k);
+  if (first) *first = (int32_t)at;
+  return (int)k;
+}
+
+int avbd_patch_body(avbd_solver *s, uint32_t index, uint32_t fields, const float *record, int fixed, int marker) {
+  if (!s || !record) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (index >= s->n) return fail(s, AVBD_ERR_INVALID, "patchBody: body index out of range");
+  if (fields & ~15u) return fail(s, AVBD_ERR_INVALID, "patchBody: unknown field bits");
+  const int lanes[4][2] = {{0, 3}, {4, 4}, {32, 3}, {36, 3}};
+  for (int f = 0; f < 4; f++)
+    if (fields & (1u << f))
+      for (int c = 0; c < lanes[f][1]; c++)
+        if (!std::isfinite(record[lanes[f][0] + c])) return fail(s, AVBD_ERR_INVALID, "patchBody: a patched field is not finite");
+  if (fixed && !(std::isfinite(record[19]) && std::isfinite(record[20]) && std::isfinite(record[21]) && std::isfinite(record[22])))
+    return fail(s, AVBD_ERR_INVALID, "patchBody: mass and moments must be finite");
+  PatchRecord r;
+  memcpy(r.r, record, 160);
+  uint32_t f = fields;
+  if (fixed) {
+    f |= 16u;
+    if (s->massHost[index] != record[19]) s->staticsDirty = true;
+    s->massHost[index] = record[19];
+    s->fill(s->colA + index, 1, NO_COLOR);
+    s->fill(s->colB + index, 1, NO_COLOR);
+  }
+  if (marker) {
+    f |= 32u;
+    r.r[2].w = (float)(1u | s->lifeHost[index]);  // BODY_DIRTY | the body's lifecycle bits
+  }
+  k_patch_body<<<1, 64, 0, s->stream>>>(s->dev(), index, f, r);
+  s->manifoldsValid = false;
+  return avbd_sync(s);
+}
+
+int avbd_rewrite_fixed(avbd_solver *s, const uint32_t *indices, uint32_t count, const float *positions, const float *size,
+                       float friction, const float *rotations) {
+  if (!s || (count && (!indices || !positions)) || !size) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  s->manifoldsValid = false;
+  if (!count) return AVBD_OK;
+  for (uint32_t k = 0; k < count; k++) {
+    if (indices[k] >= s->n) return fail(s, AVBD_ERR_INVALID, "rewriteFixed: body index out of range");
+    for (int c = 0; c < 3; c++) if (!std::isfinite(positions[3ull * k + c])) return fail(s, AVBD_ERR_INVALID, "rewriteFixed: positions must be finite");
+    if (rotations) for (int c = 0; c < 4; c++) if (!std::isfinite(rotations[4ull * k + c])) return fail(s, AVBD_ERR_INVALID, "rewriteFixed: rotations must be finite");
+  }
+  // what a Rigid of density 0 packs to, at rest (solver.ts:1145-1162)
+  float t[40] = {0};
+  t[3] = friction;
+  t[7] = 1.0f;
+  t[11] = 1.0f;  // a new occupant: no contact history
+  t[15] = 1.0f;
+  t[16] = size[0]; t[17] = size[1]; t[18] = size[2];
+  t[23] = (float)std::sqrt((double)size[0] * size[0] / 4 + (double)size[1] * size[1] / 4 + (double)size[2] * size[2] / 4);
+  t[31] = 1.0f;
+  if (int rc = ensureStaging(s, 160ull * count)) return rc;
+  for (uint32_t k = 0; k < count;) {
+    uint32_t e = k + 1;
+    while (e < count && indices[e] == indices[e - 1] + 1) e++;
+    std::vector<float> run(40ull * (e - k));
+    for (uint32_t i = k; i < e; i++) {
+      float *o = run.data() + 40ull * (i - k);
+      std::copy(t, t + 40, o);
+      for (int c = 0; c < 3; c++) o[c] = positions[3ull * i + c];
+      if (rotations) for (int c = 0; c < 4; c++) o[4 + c] = rotations[4ull * i + c];
+    }
+    s->writeRecords(run.data(), indices[k], e - k, false);
+    if (int rc = avbd_sync(s)) return rc;
+    k = e;
+  }
+  return AVBD_OK;
+}
+
+int avbd_append_joints(avbd_solver *s, const avbd_joint_spec *specs, uint32_t count, float fracture, int linear, uint32_t *slots) {
+  if (!s || (!specs && count)) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  // every spec checked before a slot is taken (solver.ts:1637-1660)
+  std::vector<std::array<float, 4>> rests(count);
+  for (uint32_t k = 0; k < count; k++) {
+    const avbd_joint_spec &j = specs[k];
+    std::string at = "appendJoints: spec " + std::to_string(k) + ": ";
+    if (!isBody(s, j.a)) return fail(s, AVBD_ERR_INVALID, at + "a must be the index of a body in the solver");
+    if (!isBody(s, j.b)) return fail(s, AVBD_ERR_INVALID, at + "b must be the index of a body in the solver");
+    bool hasRest = j.rest[0] != 0.0f || j.rest[1] != 0.0f || j.rest[2] != 0.0f || j.rest[3] != 0.0f;
+    if (hasRest) {
+      double l = std::sqrt((double)j.rest[0] * j.rest[0] + (double)j.rest[1] * j.rest[1] + (double)j.rest[2] * j.rest[2] + (double)j.rest[3] * j.rest[3]);
+      if (!(l > 0) || !std::isfinite(l)) return fail(s, AVBD_ERR_INVALID, at + "a rotation is four finite numbers (x, y, z, w), not all zero");
+      for (int c = 0; c < 4; c++) rests[k][c] = (float)(j.rest[c] / l);
+    }
+    float limit = std::isnan(j.fracture) ? fracture : j.fracture;
+    if (!(limit >= 0.0f)) return fail(s, AVBD_ERR_INVALID, at + "fracture is a force of at least 0");
+    if (hasRest && j.angular == 0.0f) return fail(s, AVBD_ERR_INVALID, at + "rest needs an angle lock to hold it: a ball joint (angular 0) has none");
+    bool hasYield = !std::isnan(j.yield);
+    if (!std::isnan(j.break_bend)) {
+      if (!std::isinf(j.break_bend) && !(j.break_bend >= 0.0f && (double)j.break_bend <= 3.14159265358979323846))
+        return fail(s, AVBD_ERR_INVALID, at + "breakBend is an angle from 0 to pi (rad)");
+      if (!std::isinf(j.break_bend) && !(hasYield && !std::isinf(j.yield)))
+        return fail(s, AVBD_ERR_INVALID, at + "breakBend limits how far a plastic joint bends: it needs a yield");
+    }
+    if (!hasYield) continue;
+    if (!(j.yield >= 0.0f)) return fail(s, AVBD_ERR_INVALID, at + "yield is a force of at least 0");
+    if (!std::isnan(j.angular)) return fail(s, AVBD_ERR_INVALID, at + "yield needs a rigid angle lock (no angular stiffness given)");
+  }
+  std::vector<uint32_t> got;
+  if (int rc = takeSlots(s, count, got)) return rc;
+  std::vector<float> recs(40ull * count);
+  std::vector<int> ends(3ull * count);
+  for (uint32_t k = 0; k < count; k++) {
+    const avbd_joint_spec &j = specs[k];
+    float *o = &recs[40ull * k];
+    freshJoint(o);
+    o[3] = JBIG;
+    o[7] = std::isnan(j.angular) ? JBIG : finiteOr(j.angular);
+    float limit = finiteOr(std::isnan(j.fracture) ? fracture : j.fracture);
+    bool lin = j.linear < 0 ? linear != 0 : j.linear != 0;
+    o[11] = lin ? -limit : limit;
+    if (!std::isnan(j.yield)) o[23] = finiteOr(j.yield);
+    if (!std::isnan(j.break_bend)) o[31] = finiteOr(j.break_bend);
+    bool hasRest = j.rest[0] != 0.0f || j.rest[1] != 0.0f || j.rest[2] != 0.0f || j.rest[3] != 0.0f;
+    if (hasRest) for (int c = 0; c < 4; c++) { o[32 + c] = rests[k][c]; o[36 + c] = rests[k][c]; }
+    for (int c = 0; c < 3; c++) { o[24 + c] = j.rA[c]; o[28 + c] = j.rB[c]; }
+    o[15] = torqueArm(sizeOf(s, j.a), sizeOf(s, j.b));
+    ends[3 * k] = (int)T_JOINT; ends[3 * k + 1] = j.a; ends[3 * k + 2] = j.b;
+    if (slots) slots[k] = got[k];
+  }
+  s->putJoints(got, recs, ends);
+  return avbd_sync(s);
+}
+
+int avbd_append_joint(avbd_solver *s, int32_t a, int32_t b, const float *rA, const float *rB, float stiff_lin, float stiff_ang,
+                      uint32_t *slot) {
+  if (!s || !rA || !rB) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (!(a == -1 || isBody(s, a)) || !isBody(s, b)) return fail(s, AVBD_ERR_INVALID, "appendJoint: a must be -1 (the world) or a body, b a body");
+  std::vector<uint32_t> got;
+  if (s->fixedStorage()) {
+    if (int rc = takeSlots(s, 1, got)) return rc;
+  } else {  // appendJoint appends (released slots are for appendJoints), growing the storage
+    std::vector<uint32_t> keep = s->freeSlots;
+    s->freeSlots.clear();
+    int rc = takeSlots(s, 1, got);
+    s->freeSlots = keep;
+    if (rc) return rc;
+  }
+  std::vector<float> o(40);
+  freshJoint(o.data());
+  o[3] = finiteOr(stiff_lin);
+  o[7] = finiteOr(stiff_ang);
+  for (int c = 0; c < 3; c++) { o[24 + c] = rA[c]; o[28 + c] = rB[c]; }
+  o[15] = torqueArm(sizeOf(s, a), sizeOf(s, b));
+  s->putJoints(got, o, {(int)T_JOINT, a, b});
+  if (slot) *slot = got[0];
+  return avbd_sync(s);
+}
+
+int avbd_append_world_joint(avbd_solver *s, uint32_t b, const float *anchor_world, const float *anchor_body, uint32_t *slot) {
+  if (!s || !anchor_world || !anchor_body) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (!isBody(s, (int32_t)b)) return fail(s, AVBD_ERR_INVALID, "appendWorldJoint: b must be a body in the solver");
+  for (int c = 0; c < 3; c++)
+    if (!finite32(anchor_world[c]) || !finite32(anchor_body[c])) return fail(s, AVBD_ERR_INVALID, "appendWorldJoint: anchors need three finite f32 numbers");
+  const float *sb = sizeOf(s, (int32_t)b);
+  double arm = (double)sb[0] * sb[0] + (double)sb[1] * sb[1] + (double)sb[2] * sb[2];
+  if (!std::isfinite((float)arm)) return fail(s, AVBD_ERR_INVALID, "appendWorldJoint: torque arm must be finite as f32");
+  std::vector<uint32_t> got;
+  if (int rc = takeSlots(s, 1, got)) return rc;
+  std::vector<float> o(40);
+  freshJoint(o.data());
+  o[3] = JBIG;
+  for (int c = 0; c < 3; c++) { o[24 + c] = anchor_world[c]; o[28 + c] = anchor_body[c]; }
+  o[15] = (float)arm;
+  s->putJoints(got, o, {(int)T_JOINT, -1, (int)b});
+  if (slot) *slot = got[0];
+  return avbd_sync(s);
+}
+
+int avbd_append_springs(avbd_solver *s, const avbd_spring_spec *springs, uint32_t count, uint32_t *slots) {
+  if (!s || (!springs && count)) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  for (uint32_t k = 0; k < count; k++)
+    if (!(springs[k].a == -1 || isBody(s, springs[k].a)) || !isBody(s, springs[k].b))
+      return fail(s, AVBD_ERR_INVALID, "appendSprings: bodies " + std::to_string(springs[k].a) + " and " + std::to_string(springs[k].b) +
+                                           " are not both bodies in the solver (a may be -1, the world)");
+  std::vector<uint32_t> got;
+  if (int rc = takeSlots(s, count, got)) return rc;
+  std::vector<float> recs(40ull * count, 0.0f);
+  std::vector<int> ends(3ull * count);
+  for (uint32_t k = 0; k < count; k++) {
+    const avbd_spring_spec &sp = springs[k];
+    float *o = &recs[40ull * k];
+    o[3] = finiteOr(sp.stiffness);
+    for (int c = 0; c < 3; c++) { o[24 + c] = sp.rA[c]; o[28 + c] = sp.rB[c]; }
+    o[27] = sp.rest;
+    ends[3 * k] = (int)T_SPRING; ends[3 * k + 1] = sp.a; ends[3 * k + 2] = sp.b;
+    if (slots) slots[k] = got[k];
+  }
+  s->putJoints(got, recs, ends);
+  return avbd_sync(s);
+}
+
+int avbd_release_joints(avbd_solver *s, const uint32_t *slots, uint32_t count) {
+  if (!s || (!slots && count)) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  s->manifoldsValid = false;
+  std::vector<uint32_t> order;
+  for (uint32_t k = 0; k < count; k++)
+    if (slots[k] < s->nj && s->jinfoHost[4ull * slots[k]] != (int)T_NONE) order.push_back(slots[k]);
+  std::sort(order.begin(), order.end());
+  order.erase(std::unique(order.begin(), order.end()), order.end());
+  if (order.empty()) return AVBD_OK;
+  std::vector<float> zeros(40 * order.size(), 0.0f);
+  std::vector<int> ends;
+  for (uint32_t slot : order) {
+    ends.insert(ends.end(), {(int)T_NONE, -1, -1});
+    s->freeSlots.push_back(slot);
+  }
+  s->putJoints(order, zeros, ends);
+  return avbd_sync(s);
+}
+
+int avbd_set_world_anchor(avbd_solver *s, uint32_t slot, const float *p) {
+  if (!s || !p) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (slot >= s->nj) return fail(s, AVBD_ERR_INVALID, "setWorldAnchor: joint slot out of range");
+  hipMemcpyAsync((float *)(s->joints + 10ull * slot) + 24, p, 12, hipMemcpyHostToDevice, s->stream);
+  return avbd_sync(s);
+}
+
+int avbd_disable_constraint(avbd_solver *s, uint32_t slot) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (slot >= s->nj) return fail(s, AVBD_ERR_INVALID, "disableConstraint: joint slot out of range");
+  s->manifoldsValid = false;
+  float zero[8] = {0};
+  float *k = (float *)(s->joints + 10ull * slot);
+  hipMemcpyAsync(k, zero, 32, hipMemcpyHostToDevice, s->stream);       // penalties and stiffnesses
+  hipMemcpyAsync(k + 8, zero, 12, hipMemcpyHostToDevice, s->stream);   // lambda lin xyz
+  hipMemcpyAsync(k + 12, zero, 12, hipMemcpyHostToDevice, s->stream);  // lambda ang xyz
+  return avbd_sync(s);
+}
+
+int avbd_set_body_aux(avbd_solver *s, const uint32_t *indices, uint32_t count, const float *scales, const float *linear,
+                      const float *angular) {
+  if (!s || (count && (!indices || !scales))) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if ((linear == nullptr) != (angular == nullptr)) return fail(s, AVBD_ERR_INVALID, "setBodyAux: pass both damping rate arrays or neither");
+  bool rates = linear != nullptr;
+  for (uint32_t k = 0; k < count; k++) {
+    if (indices[k] >= s->bodyCap) return fail(s, AVBD_ERR_INVALID, "setBodyAux: slot " + std::to_string(indices[k]) + " is outside the body capacity");
+    if (!std::isfinite(scales[k])) return fail(s, AVBD_ERR_INVALID, "setBodyAux: scales must be finite (as f32)");
+    if (rates && !(std::isfinite(linear[k]) && linear[k] >= 0.0f && std::isfinite(angular[k]) && angular[k] >= 0.0f))
+      return fail(s, AVBD_ERR_INVALID, "setBodyAux: damping rates must be finite (as f32) and at least 0");
+  }
+  if (!s->aux) {
+    try {
+      s->aux = s->alloc<float4>(s->bodyCap);
+    } catch (const std::exception &e) {
+      return fail(s, AVBD_ERR_DEVICE, e.what());
+    }
+    s->auxScale.assign(s->bodyCap, 1.0f);
+    s->auxLin.assign(s->bodyCap, 0.0f);
+    s->auxAng.assign(s->bodyCap, 0.0f);
+    s->uploadAux(0, s->bodyCap);
+  }
+  for (uint32_t k = 0; k < count;) {
+    uint32_t e = k + 1;
+    while (e < count && indices[e] == indices[e - 1] + 1) e++;
+    for (uint32_t i = k; i < e; i++) {
+      uint32_t b = indices[i];
+      s->auxScale[b] = scales[i];
+      if (rates) { s->auxLin[b] = linear[i]; s->auxAng[b] = angular[i]; }
+      else if (e - k > 1) { s->auxLin[b] = 0.0f; s->auxAng[b] = 0.0f; }  // a run's whole records hold rates 0
+    }
+    s->uploadAux(indices[k], e - k);
+    k = e;
+  }
+  return avbd_sync(s);
+}
+
+int avbd_set_lifecycle(avbd_solver *s, uint32_t index, uint32_t bits) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (index >= s->n || bits > 6 || (bits & ~6u) != 0) return fail(s, AVBD_ERR_INVALID, "Invalid body lifecycle flags/index");
+  s->lifeHost[index] = (uint8_t)bits;
+  float w = (float)(1u | bits);  // BODY_DIRTY | bits
+  hipMemcpyAsync((float *)&s->b.ipos[index] + 3, &w, 4, hipMemcpyHostToDevice, s->stream);
+  s->fill(s->colA + index, 1, NO_COLOR);
+  s->fill(s->colB + index, 1, NO_COLOR);
+  s->manifoldsValid = false;
+  return avbd_sync(s);
 }
 
 }  // extern "C"

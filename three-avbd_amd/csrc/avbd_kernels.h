@@ -16,6 +16,9 @@ struct Dev {
   float4 *refPos, *refRot;
   uint2 *filters;
   float4 *joints;  // 10 float4 per joint (layout.ts:232-243)
+  // per body (null until avbd_set_body_aux): gravity scale, the per-step velocity decay factors
+  // exp(-rate dt) (linear, angular), 1 when either rate is nonzero (the reference's aux record)
+  const float4 *aux;
   int4 *jinfo;     // type, a (-1 world), b, 0
   uint32_t *statics;  // large bodies [largeCount] | no-collide (hi, lo, slot) triples sorted by (hi, lo)
   uint32_t largeCount, noCollideCount;
@@ -1277,15 +1280,41 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
     penLin = make_float4(mn(np.x, cap), mn(np.y, cap), mn(np.z, cap), penLin.w);
   }
   float frac = lamLin.w, limit = fabsf(frac);
+  bool yielded = false;
   if (dot(xyz(penAng), xyz(penAng)) > 0.0f) {
     V C = jointAngRows(d, k, a, b, alpha);
-    if (penAng.w >= HARD) lamAng = f4(add(mulv(xyz(penAng), C), xyz(lamAng)), lamAng.w);
-    float cap = mn(penAng.w, PENALTY_MAX);
-    V np = add(xyz(penAng), mul(vabs(C), d.p.betaAng));
-    penAng = make_float4(mn(np.x, cap), mn(np.y, cap), mn(np.z, cap), penAng.w);
+    if (penAng.w >= HARD) {
+      lamAng = f4(add(mulv(xyz(penAng), C), xyz(lamAng)), lamAng.w);
+      // a plastic joint (yield in c0Ang.w, wgsl-solve.ts:586-599): past yield, short of fracture,
+      // the angular force is cut back to yield and the bend becomes the new rest
+      float yieldForce = k[5].w;
+      if (yieldForce < BIG) {
+        float force = length(xyz(lamAng));
+        if (force > yieldForce && !(limit < BIG && force > limit)) {
+          lamAng = f4(mul(xyz(lamAng), yieldForce / force), lamAng.w);
+          yielded = true;
+        }
+      }
+    }
+    if (!yielded) {  // a yielding row is at its bound: its penalty is not ramped
+      float cap = mn(penAng.w, PENALTY_MAX);
+      V np = add(xyz(penAng), mul(vabs(C), d.p.betaAng));
+      penAng = make_float4(mn(np.x, cap), mn(np.y, cap), mn(np.z, cap), penAng.w);
+    }
+  }
+  // the rest moves to where the bodies are; past the bend limit (rB.w) from the rest it started
+  // with, the joint breaks (wgsl-solve.ts:606-614)
+  Q rest = q4(k[8]);
+  bool bentOff = false;
+  if (yielded) {
+    rest = qnormalize(qmul(qconj(rotAOf(d, a)), q4(d.b.rot[b])));
+    if (k[7].w < BIG) {
+      Q dq = qmul(qconj(q4(k[9])), rest);
+      bentOff = 2.0f * atan2f(length(vv(dq.x, dq.y, dq.z)), fabsf(dq.w)) > k[7].w;
+    }
   }
   bool linear = frac < 0.0f && dot(xyz(lamLin), xyz(lamLin)) > limit * limit;
-  if (limit < BIG && (dot(xyz(lamAng), xyz(lamAng)) > limit * limit || linear)) {
+  if ((limit < BIG && (dot(xyz(lamAng), xyz(lamAng)) > limit * limit || linear)) || bentOff) {
     atomicOr(&d.counters[C_TOPO], 2u);  // broken: the next adjacency drops it
     penLin = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     penAng = penLin;
@@ -1293,6 +1322,10 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
     lamAng = make_float4(0.0f, 0.0f, 0.0f, lamAng.w);
   }
   k[0] = penLin; k[1] = penAng; k[2] = lamLin; k[3] = lamAng;
+  if (yielded) {
+    k[8] = f4q(rest);
+    k[5] = make_float4(0.0f, 0.0f, 0.0f, k[5].w);
+  }
 }
 /** dualManifold's update of one contact point c (wgsl-solve.ts:637-662), given its manifold's basis,
  *  friction and both bodies' terms: penalty ramp, stick flag, λ; the whole 64-byte record written
@@ -1381,6 +1414,18 @@ __device__ __forceinline__ void warmBodies(const Dev &d, uint32_t t0, uint32_t n
     V vel = xyz(vel4), angVel = xyz(av4);
     float dt = d.p.dt, g = d.p.gravity;
     bool dyn = d.b.size[i].w > 0.0f;
+    if (d.aux) {
+      // the aux variant (wgsl-solve.ts:293-305, DAMPING_WGSL :14-20): scaled gravity, and a damped
+      // body's velocities decayed before the inertial target and the warm start read them
+      float4 ax = d.aux[i];
+      g = g * ax.x;
+      if (dyn && ax.w != 0.0f) {
+        vel = mul(vel, ax.y);
+        angVel = mul(angVel, ax.z);
+        d.b.vel[i] = f4(vel, vel4.w);
+        d.b.angvel[i] = f4(angVel, av4.w);
+      }
+    }
     V up = vv(d.p.upx, d.p.upy, d.p.upz);
     V inpos = add(xyz(pos), mul(vel, dt));
     if (dyn) inpos = add(inpos, mul(up, g * (dt * dt)));
@@ -1555,6 +1600,25 @@ __global__ void k_records_to_bodies(Dev d, const float4 *rec, uint32_t first, ui
     d.b.moment[i] = o[5]; d.b.inpos[i] = o[6]; d.b.inrot[i] = o[7]; d.b.vel[i] = o[8]; d.b.angvel[i] = o[9];
     d.refRot[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
+}
+/** patchBody's masked writes (solver.ts:1103-1131) on one body: `fields` bit 1 position (pos, x-,
+ *  inertial pos xyz), 2 rotation (rot, q-, inertial rot), 4 velocity xyz, 8 angular velocity xyz,
+ *  16 mass and moments (a fixed/dynamic switch), 32 the lifecycle word of x- (rec[2].w). */
+struct PatchRecord { float4 r[10]; };
+__global__ void k_patch_body(Dev d, uint32_t i, uint32_t fields, PatchRecord rec) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float4 *r = rec.r;
+  auto xyzOf = [](float4 dst, float4 src) { return make_float4(src.x, src.y, src.z, dst.w); };
+  if (fields & 1u) { d.b.pos[i] = xyzOf(d.b.pos[i], r[0]); d.b.ipos[i] = xyzOf(d.b.ipos[i], r[0]); d.b.inpos[i] = xyzOf(d.b.inpos[i], r[0]); }
+  if (fields & 2u) { d.b.rot[i] = r[1]; d.b.irot[i] = r[1]; d.b.inrot[i] = r[1]; }
+  if (fields & 4u) d.b.vel[i] = xyzOf(d.b.vel[i], r[8]);
+  if (fields & 8u) d.b.angvel[i] = xyzOf(d.b.angvel[i], r[9]);
+  if (fields & 16u) {
+    float4 sz = d.b.size[i];
+    d.b.size[i] = make_float4(sz.x, sz.y, sz.z, r[4].w);
+    d.b.moment[i] = xyzOf(d.b.moment[i], r[5]);
+  }
+  if (fields & 32u) { float4 ip = d.b.ipos[i]; ip.w = r[2].w; d.b.ipos[i] = ip; }
 }
 
 }  // namespace avbd
