@@ -263,6 +263,32 @@ template <class T> void getContacts(const std::vector<gpuo::ManifoldT<T>> &ms, c
 }
 }  // namespace
 
+// The hull SAT queries of bodies a and b (hull3d.hpp collidePoly's three queries), for tests that
+// check them against a brute force: faSep, faFace, fbSep, fbFace, edgeSep, edgeValid, ea, eb.
+template <class T> void hullQuery(gpuo::Model<T> *M, int a, int b, double *out) {
+  gpuo::HullOps<T> H;
+  H.w = M->hulls.data();
+  auto A = H.polyOf(M->bodies[a]), B = H.polyOf(M->bodies[b]);
+  T fa, fb;
+  uint32_t faf, fbf;
+  H.queryFaces(A, B, fa, faf);
+  H.queryFaces(B, A, fb, fbf);
+  auto e = H.queryEdges(A, B);
+  out[0] = fa; out[1] = faf; out[2] = fb; out[3] = fbf; out[4] = e.sep; out[5] = e.valid ? 1 : 0; out[6] = e.ea; out[7] = e.eb;
+}
+// finishBody's floored LDLT on a given 6x6 system (row-major, symmetric) and right-hand side:
+// x = A^-1 rhs, the pivots D and each pivot S before its floor, in f32 (prec 32) or f64.
+template <class T> void solve6(const double *A, const double *r, double *x, double *D, double *S) {
+  typename gpuo::Model<T>::Acc acc;
+  const int I[6] = {0, 1, 1, 2, 2, 2}, J[6] = {0, 0, 1, 0, 1, 2};
+  for (int e = 0; e < 6; e++) { acc.lin[e] = (T)A[6 * I[e] + J[e]]; acc.ang[e] = (T)A[6 * (3 + I[e]) + 3 + J[e]]; }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) acc.cr[3 * i + j] = (T)A[6 * (3 + i) + j];
+  for (int i = 0; i < 3; i++) { acc.rLin[i] = (T)r[i]; acc.rAng[i] = (T)r[3 + i]; }
+  T xx[6], dd[6], ss[6];
+  gpuo::Model<T>::solve6Floor(acc, xx, dd, ss);
+  for (int k = 0; k < 6; k++) { x[k] = xx[k]; D[k] = dd[k]; S[k] = ss[k]; }
+}
 extern "C" {
 void *omodel_create(int prec) {
   ModelHandle *h = new ModelHandle{prec, nullptr, nullptr};
@@ -294,6 +320,10 @@ void omodel_set_fixed_colors(void *h, const uint32_t *colors) {
 void omodel_set_color_state(void *h, const uint32_t *state) { DISPATCH(h, M->colorA.assign(state, state + M->bodies.size())); }
 void omodel_set_step_count(void *h, uint32_t s) { DISPATCH(h, M->stepCount = s); }
 void omodel_set_threads(void *h, int n) { DISPATCH(h, M->threads = n < 1 ? 1 : n); }
+void omodel_hull_query(void *h, int a, int b, double *out) { DISPATCH(h, hullQuery(M, a, b, out)); }
+void omodel_solve6(int prec, const double *A, const double *r, double *x, double *D, double *S) {
+  if (prec == 32) solve6<float>(A, r, x, D, S); else solve6<double>(A, r, x, D, S);
+}
 // the aux records (4 per body: gravity scale, linear and angular decay factors, damped 0/1)
 void omodel_set_aux(void *h, const double *r) {
   DISPATCH(h, for (size_t i = 0; i < M->bodies.size(); i++) {

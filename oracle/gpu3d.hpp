@@ -913,8 +913,9 @@ template <class T> struct Model {
     }
     return acc;
   }
-  /** finishBody (wgsl-solve.ts:504-558): LDLᵀ with the 1e-5 pivot floor, then the update. */
-  void finishBody(uint32_t i, const Acc &acc) {
+  /** finishBody's LDLT of the body's 6x6 system (wgsl-solve.ts:504-539): unpivoted, each pivot
+   *  D_k floored at 1e-5 A_kk; S receives each pivot before its floor. x = -A^-1 rhs as (v, w). */
+  static void solve6Floor(const Acc &acc, T x[6], T Dout[6], T Sout[6]) {
     const T PF = T(1e-5);
     T A11 = acc.lin[0], A21 = acc.lin[1], A22 = acc.lin[2], A31 = acc.lin[3], A32 = acc.lin[4], A33 = acc.lin[5];
     T A41 = acc.cr[0], A42 = acc.cr[1], A43 = acc.cr[2], A44 = acc.ang[0];
@@ -931,11 +932,6 @@ template <class T> struct Model {
     T S5 = A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), D5 = O::mx(S5, A55 * PF);
     T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
     T S6 = A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), D6 = O::mx(S6, A66 * PF);
-    if (tape && curIt >= 0) {
-      double *dl = &tape->steps.back().delta[curIt][(size_t)i * 6];
-      dl[1] = (double)D2 - (double)S2; dl[2] = (double)D3 - (double)S3; dl[3] = (double)D4 - (double)S4;
-      dl[4] = (double)D5 - (double)S5; dl[5] = (double)D6 - (double)S6;
-    }
     T y1 = acc.rLin[0], y2 = acc.rLin[1] - L21 * y1, y3 = acc.rLin[2] - L31 * y1 - L32 * y2;
     T y4 = acc.rAng[0] - L41 * y1 - L42 * y2 - L43 * y3, y5 = acc.rAng[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
     T y6 = acc.rAng[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
@@ -943,6 +939,19 @@ template <class T> struct Model {
     T v3 = y3 / D3 - L43 * w1 - L53 * w2 - L63 * w3;
     T v2 = y2 / D2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
     T v1 = y1 / D1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
+    x[0] = v1; x[1] = v2; x[2] = v3; x[3] = w1; x[4] = w2; x[5] = w3;
+    Dout[0] = D1; Dout[1] = D2; Dout[2] = D3; Dout[3] = D4; Dout[4] = D5; Dout[5] = D6;
+    Sout[0] = D1; Sout[1] = S2; Sout[2] = S3; Sout[3] = S4; Sout[4] = S5; Sout[5] = S6;
+  }
+  /** finishBody (wgsl-solve.ts:504-558): LDLᵀ with the 1e-5 pivot floor, then the update. */
+  void finishBody(uint32_t i, const Acc &acc) {
+    T x[6], D[6], S[6];
+    solve6Floor(acc, x, D, S);
+    if (tape && curIt >= 0) {
+      double *dl = &tape->steps.back().delta[curIt][(size_t)i * 6];
+      for (int k = 1; k < 6; k++) dl[k] = (double)D[k] - (double)S[k];
+    }
+    T v1 = x[0], v2 = x[1], v3 = x[2], w1 = x[3], w2 = x[4], w3 = x[5];
     Body<T> &bd = bodies[i];
     bd.pos = O::sub(bd.pos, O::v(v1, v2, v3));
     bd.rot = O::qadd(bd.rot, O::neg(O::v(w1, w2, w3)));

@@ -164,6 +164,16 @@ class RefSolver:
         return basis.reshape(3, 3), out[:n]
 
 
+def solve6_floor(A, rhs, prec=32):
+    """finishBody's floored LDLT (the oracle's): x = A^-1 rhs, the pivots D and the pivots before
+    their 1e-5 A_kk floor S."""
+    A = np.ascontiguousarray(A, np.float64).reshape(6, 6)
+    r = np.ascontiguousarray(rhs, np.float64).reshape(6)
+    x, D, S = np.zeros(6), np.zeros(6), np.zeros(6)
+    lib().omodel_solve6(int(prec), _p(A), _p(r), _p(x), _p(D), _p(S))
+    return x, D, S
+
+
 def solve6(A, b):
     A = np.ascontiguousarray(A, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
@@ -235,6 +245,14 @@ class Model:
 
     def set_step_count(self, s):
         self.L.omodel_set_step_count(self.h, s)
+
+    def hull_query(self, a, b):
+        """The hull SAT's queries for bodies a, b (faces of a, faces of b, edge pairs): a dict of
+        their separations and the features that give them."""
+        out = np.zeros(8)
+        self.L.omodel_hull_query(self.h, int(a), int(b), _p(out))
+        return {"fa_sep": out[0], "fa_face": int(out[1]), "fb_sep": out[2], "fb_face": int(out[3]), "edge_sep": out[4],
+                "edge_valid": bool(out[5]), "ea": int(out[6]), "eb": int(out[7])}
 
     def set_aux(self, aux):
         """setBodyAux's records (n x 4: gravity scale, the linear and angular per-step decay
