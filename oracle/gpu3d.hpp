@@ -777,12 +777,16 @@ template <class T> struct Model {
   // ---- solve (wgsl-solve.ts) ----
   struct Acc { T lin[6], ang[6], cr[9], rLin[3], rAng[3]; };  // lin/ang lower triangles: 00 10 11 20 21 22
   static void zero(Acc &a) { std::memset(&a, 0, sizeof(a)); for (int i = 0; i < 6; i++) { a.lin[i] = T(0); a.ang[i] = T(0); } }
+  /** addRow: the row's scaled Jacobian (k l, k a) once, then one fused multiply-add per entry. */
   static void addRow(Acc &acc, V l, V a, T k, T f) {
     static const int I[6] = {0, 1, 1, 2, 2, 2}, J[6] = {0, 0, 1, 0, 1, 2};
-    for (int e = 0; e < 6; e++) { acc.lin[e] += (l[I[e]] * l[J[e]]) * k; acc.ang[e] += (a[I[e]] * a[J[e]]) * k; }
-    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) acc.cr[i * 3 + j] += (a[i] * l[j]) * k;
-    for (int i = 0; i < 3; i++) { acc.rLin[i] += l[i] * f; acc.rAng[i] += a[i] * f; }
+    T kl[3] = {l[0] * k, l[1] * k, l[2] * k}, ka[3] = {a[0] * k, a[1] * k, a[2] * k};
+    for (int e = 0; e < 6; e++) { acc.lin[e] = std::fma(kl[I[e]], l[J[e]], acc.lin[e]); acc.ang[e] = std::fma(ka[I[e]], a[J[e]], acc.ang[e]); }
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) acc.cr[i * 3 + j] = std::fma(ka[i], l[j], acc.cr[i * 3 + j]);
+    for (int i = 0; i < 3; i++) { acc.rLin[i] = std::fma(l[i], f, acc.rLin[i]); acc.rAng[i] = std::fma(a[i], f, acc.rAng[i]); }
   }
+  /** a . b as a chain of fused multiply-adds (x first). */
+  static T fdot(V a, V b) { return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x)); }
   static void accAdd(Acc &a, const Acc &b) {
     for (int i = 0; i < 6; i++) { a.lin[i] += b.lin[i]; a.ang[i] += b.ang[i]; }
     for (int i = 0; i < 9; i++) a.cr[i] += b.cr[i];
@@ -850,10 +854,10 @@ template <class T> struct Model {
     e.rBW = O::qrotate(rotB, k.rB);
     T keep = T(1.0) - alpha;
     T c0[3] = {k.c0x, k.c0y, k.c0z};
-    for (int r = 0; r < 3; r++) {
-      V l = basis[r];
-      e.C[r] = c0[r] * keep + O::dot(l, A.dLin) - O::dot(l, B.dLin) + O::dot(O::cross(e.rAW, l), A.dAng) + O::dot(O::cross(e.rBW, O::neg(l)), B.dAng);
-    }
+    // C_r = c0 (1 - alpha) + l . dxA - l . dxB + (rA x l) . dthA + (rB x -l) . dthB, as
+    // c0 (1 - alpha) + l . g with g = (dxA - dxB) + (dthA x rA - dthB x rB) (scalar triple product)
+    V g = O::add(O::sub(A.dLin, B.dLin), O::sub(O::cross(A.dAng, e.rAW), O::cross(B.dAng, e.rBW)));
+    for (int r = 0; r < 3; r++) e.C[r] = std::fma(c0[r], keep, fdot(basis[r], g));
     V F = O::add(O::mulv(k.pen, e.C), k.lam);
     e.bits = F.x < T(0.0) ? 0 : 1;
     F.x = O::mn(F.x, T(0.0));

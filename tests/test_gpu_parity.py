@@ -72,6 +72,17 @@ def test_stack_two_steps_bit_exact(gpu):
     s.destroy()
 
 
+# The reference allows multipliers within 1e-3 of the largest (tests-gpu/avbd3d-gpu.gpu.test.ts:77).
+# Two frames need a little more in f32: Static Friction's sticking boxes carry round-off through a
+# stiff friction row to 1.14e-3 at frame 120 and 1.05e-3 at frame 300 (the same step in f64 agrees
+# to 1e-15: test_seeded_single_step_f64_equals_reference).
+LAM_WIDER = {("Static Friction", 120): 1.2e-3, ("Static Friction", 300): 1.2e-3}
+
+
+def lam_tol(name, at):
+    return LAM_WIDER.get((name, at), 1e-3)
+
+
 def test_seeded_single_step_matches_reference(gpu):
     tie_frames = 0
     for name, sid in PARITY_SCENES.items():
@@ -107,7 +118,7 @@ def test_seeded_single_step_matches_reference(gpu):
             # the reference's tolerances against the f64 reference
             rm, rc = ref.manifold_records()
             gm, gc = records.contacts_to_oracle(*s.read_contacts_raw())
-            expected, found, keymis, lam_ok = contact_match(rm, rc, gm, gc, bodies64, lam_tol=2e-3)
+            expected, found, keymis, lam_ok = contact_match(rm, rc, gm, gc, bodies64, lam_tol=lam_tol(name, at))
             assert found == expected and len(gc) == expected, (name, at)
             s.destroy()
             if keymis:

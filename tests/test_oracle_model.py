@@ -33,13 +33,22 @@ def test_seeded_single_step_f64_equals_reference():
             assert d < 1e-9, (name, at, d)
 
 
+# The reference allows multipliers within 1e-3 of the largest (tests-gpu/avbd3d-gpu.gpu.test.ts:77).
+# Two frames need a little more in f32: Static Friction's sticking boxes carry round-off through a
+# stiff friction row to 1.14e-3 at frame 120 and 1.05e-3 at frame 300 (the same step in f64 agrees
+# to 1e-15: test_seeded_single_step_f64_equals_reference).
+LAM_WIDER = {("Static Friction", 120): 1.2e-3, ("Static Friction", 300): 1.2e-3}
+
+
+def lam_tol(name, at):
+    return LAM_WIDER.get((name, at), 1e-3)
+
+
 def test_seeded_single_step_matches_reference():
     """tests-gpu/avbd3d-gpu.gpu.test.ts:35-99 with the f32 restatement in place of the GPU: the
     reference's exact rules (heuristics off), seeded from the reference mid-run, one colour per
-    body in the reference's order; contacts found and poses within 1e-5 unless a feature tie.
-    Multipliers within 2e-3 of the largest (the reference allows 1e-3 on its device): Static
-    Friction's sticking boxes carry f32 round-off through a stiff friction row to 1.1e-3, while
-    the same step in f64 agrees to 1e-15 (test above)."""
+    body in the reference's order; contacts found and poses within 1e-5 unless a feature tie,
+    multipliers within the reference's 1e-3 (two frames named in LAM_WIDER)."""
     tie_frames = 0
     for name, sid in PARITY_SCENES.items():
         ref = oracle.RefSolver(sid)
@@ -56,7 +65,7 @@ def test_seeded_single_step_matches_reference():
             frame += 1
             rm, rc = ref.manifold_records()
             gm, gc = m.contacts()
-            expected, found, keymis, lam_ok = contact_match(rm, rc, gm, gc, bodies, lam_tol=2e-3)
+            expected, found, keymis, lam_ok = contact_match(rm, rc, gm, gc, bodies, lam_tol=lam_tol(name, at))
             where = f"{name} frame {at}"
             assert found == expected, where
             assert len(gc) == expected, where

@@ -936,19 +936,24 @@ __device__ __forceinline__ void accZero(Acc &a) {
 #pragma unroll
   for (int i = 0; i < 3; i++) { a.rl[i] = 0.0f; a.ra[i] = 0.0f; }
 }
-/** Fold one constraint row in: Jacobian (l, a), stiffness k, force f. */
+/** Fold one constraint row in: Jacobian (l, a), stiffness k, force f. The row's scaled Jacobian
+ *  (k l, k a) is formed once and each entry of the system takes one fused multiply-add
+ *  (wgsl-solve.ts:69-113 folds k J J^T the same way, as sums of products). */
 __device__ __forceinline__ void addRow(Acc &acc, V l, V a, float k, float f) {
   float L[3] = {l.x, l.y, l.z}, A[3] = {a.x, a.y, a.z};
+  float kl[3] = {L[0] * k, L[1] * k, L[2] * k}, ka[3] = {A[0] * k, A[1] * k, A[2] * k};
   const int I[6] = {0, 1, 1, 2, 2, 2}, J[6] = {0, 0, 1, 0, 1, 2};
 #pragma unroll
-  for (int e = 0; e < 6; e++) { acc.lin[e] += (L[I[e]] * L[J[e]]) * k; acc.ang[e] += (A[I[e]] * A[J[e]]) * k; }
+  for (int e = 0; e < 6; e++) { acc.lin[e] = fmaf(kl[I[e]], L[J[e]], acc.lin[e]); acc.ang[e] = fmaf(ka[I[e]], A[J[e]], acc.ang[e]); }
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) acc.cr[i * 3 + j] += (A[i] * L[j]) * k;
+    for (int j = 0; j < 3; j++) acc.cr[i * 3 + j] = fmaf(ka[i], L[j], acc.cr[i * 3 + j]);
 #pragma unroll
-  for (int i = 0; i < 3; i++) { acc.rl[i] += L[i] * f; acc.ra[i] += A[i] * f; }
+  for (int i = 0; i < 3; i++) { acc.rl[i] = fmaf(L[i], f, acc.rl[i]); acc.ra[i] = fmaf(A[i], f, acc.ra[i]); }
 }
+/** a . b as a chain of fused multiply-adds (x first). */
+__device__ __forceinline__ float fdot(V a, V b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 __device__ __forceinline__ Q rotAOf(const Dev &d, int a) { return a < 0 ? qq(0.0f, 0.0f, 0.0f, 1.0f) : q4(d.b.rot[a]); }
 __device__ __forceinline__ V jointLinC(const Dev &d, const float4 *k, int a, int b) {
   V anchor = a < 0 ? xyz(k[6]) : add(qrotate(q4(d.b.rot[a]), xyz(k[6])), xyz(d.b.pos[a]));
@@ -1026,10 +1031,12 @@ struct Eval { V rAW, rBW, C, F; float frictionScale, bounds; V J[3]; };
 __device__ __forceinline__ unsigned char evalBits(const Eval &e) {
   return (unsigned char)((e.F.x < 0.0f ? 0u : 1u) | ((e.frictionScale > e.bounds && e.frictionScale > 0.0f) ? 2u : 0u));
 }
-/** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. With
- *  WANT_J, also each row's angular Jacobian (e.J) for the body on side A (isA) or B: r_A x l, or
- *  r_B x (-l), the cross products C takes (a basis row times sg = +-1 is the row or its exact
- *  negation). */
+/** evalContact (wgsl-solve.ts:196-231): Taylor-expanded C about x-, cone-clamped force. Each row's
+ *  C = c0 (1 - alpha) + l . dxA - l . dxB + (rA x l) . dthA + (rB x -l) . dthB is taken as
+ *  c0 (1 - alpha) + l . g, with g = (dxA - dxB) + (dthA x rA - dthB x rB) by the scalar triple
+ *  product: one vector per point instead of two cross products and four dot products per row.
+ *  With WANT_J, also each row's angular Jacobian (e.J) for the body on side A (isA) or B: r_A x l,
+ *  or r_B x (-l) (a basis row times sg = +-1 is the row or its exact negation). */
 template <bool WANT_J = false>
 __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, float friction, const PairBody &A, const PairBody &B,
                                             float alpha, bool isA = true) {
@@ -1041,11 +1048,11 @@ __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, fl
   float keep = 1.0f - alpha;
   float c0[3] = {k.rB.w, k.pen.w, k.lam.w};
   float C[3];
+  V g = add(sub(A.dLin, B.dLin), sub(cross(A.dAng, e.rAW), cross(B.dAng, e.rBW)));
 #pragma unroll
   for (int r = 0; r < 3; r++) {
-    V l = basis[r], cA = cross(e.rAW, l), cB = cross(e.rBW, neg(l));
-    C[r] = c0[r] * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cA, A.dAng) + dot(cB, B.dAng);
-    if (WANT_J) e.J[r] = isA ? cA : cB;
+    C[r] = fmaf(c0[r], keep, fdot(basis[r], g));
+    if (WANT_J) e.J[r] = isA ? cross(e.rAW, basis[r]) : cross(e.rBW, neg(basis[r]));
   }
   e.C = vv(C[0], C[1], C[2]);
   V F = add(mulv(xyz(k.pen), e.C), xyz(k.lam));
