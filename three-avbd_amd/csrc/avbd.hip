@@ -540,6 +540,10 @@ struct avbd_solver {
       uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
       k_tape_topology<<<blocksFor(std::max({conCap, manCap, adjCap, n + 1}), 256), 256, 0, stream>>>(d, rec->t, recSlot);
       k_tape_colors<<<blocksFor(n, 256), 256, 0, stream>>>(d, rec->t, recSlot, cap);
+      uint32_t *off = rec->t.ptOff + (size_t)recSlot * (n + 1);
+      k_tape_point_count<<<blocksFor(n + 1, 256), 256, 0, stream>>>(d, rec->t, recSlot, cap);
+      scan(off, off, n + 1);
+      k_tape_point_fill<<<blocksFor(n, 256), 256, 0, stream>>>(d, rec->t, recSlot, cap);
       rec->cap[recSlot] = cap;
     }
   }
@@ -562,7 +566,6 @@ struct avbd_solver {
     for (uint32_t it = 0; it < hp.iterations; it++) {
       if (rec) d.bits = rec->t.bits + ((size_t)recSlot * rec->t.iters + it) * rec->t.conCap;
       for (uint32_t c = 0; c < cap; c++) {
-        if (rec) k_tape_system<<<blocksFor(8ull * n, 64), 64, 0, stream>>>(d, rec->t, recSlot, it, c, 8);
         if (!rec && c >= tailStart) {
           timed(K_TAIL, [&] { k_primal_tail<<<1, 512, 0, stream>>>(d, c, cap); });  // apart from k_primal's launches
           break;
@@ -572,6 +575,7 @@ struct avbd_solver {
       timed(K_DUAL, [&] { k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d); });
       if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, it + 1);
     }
+    if (rec) k_tape_systems<<<blocksFor(4ull * n * hp.iterations, 64), 64, 0, stream>>>(d, rec->t, recSlot, rec->cap[recSlot], 4);
     k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
   }
   // ---- adjoint (avbd_forward / avbd_backward) ----
@@ -697,6 +701,9 @@ struct avbd_solver {
     t.cman = adjAlloc<uint32_t>(m, S * t.conCap);
     t.bits = adjAlloc<uchar4>(m, S * hp.iterations * t.conCap);
     t.sys = adjAlloc<double>(m, S * hp.iterations * (size_t)n * SYS_WORDS);
+    t.ptCap = 2 * t.conCap;
+    t.ptOff = adjAlloc<uint32_t>(m, S * (n + 1));
+    t.pts = adjAlloc<uint4>(m, S * t.ptCap);
     tb.t = t;
     tb.slots = slots;
     tb.cap.assign(slots, 0);
@@ -718,6 +725,7 @@ struct avbd_solver {
     a.c0b = adjAlloc<double>(adjMem, 4ull * cCap);
     a.mub = adjAlloc<double>(adjMem, cCap);
     a.dpart = adjAlloc<double>(adjMem, 32ull * cCap);
+    a.nslot = adjAlloc<double>(adjMem, 32ull * cCap);
     adj = a;
     adjConCap = cCap;
   }
@@ -739,17 +747,15 @@ struct avbd_solver {
       const uint32_t *starts = tb.hostStarts + 65 * t;
       uint32_t conBlocks = blocksFor(std::max(tb.hostCounts[4 * t + 1], 1u), 64);
       k_adj_velocities<<<nb, 256, 0, st>>>(d, tape, adj, t);
+      zeroF64(adj.nslot, 32ull * std::max(tb.hostCounts[4 * t + 1], 1u), st);
       for (uint32_t it = hp.iterations; it-- > 0;) {
         k_adj_dual<<<conBlocks, 64, 0, st>>>(d, tape, adj, t, it);
-        k_adj_dual_gather<<<blocksFor(4ull * n, 256), 256, 0, st>>>(d, tape, adj, t);
         for (uint32_t c = tb.cap[t]; c-- > 0;) {
           uint32_t size = starts[c + 1] - starts[c];
-          if (size == 0) continue;
-          uint32_t lanes = 16;  // about 64k threads per launch: one wave per SIMD of the chip
-          while (lanes > 1 && (uint64_t)size * lanes > 65536) lanes >>= 1;
-          k_adj_primal<<<blocksFor((uint64_t)size * lanes, 64), 64, 0, st>>>(d, tape, adj, t, it, c, lanes);
+          if (size) k_adj_color<<<blocksFor((uint64_t)size * ADJ_LANES, 64), 64, 0, st>>>(d, tape, adj, t, it, c);
         }
       }
+      k_adj_slots_final<<<nb, 256, 0, st>>>(d, tape, adj, t, tb.cap[t]);
       k_adj_contacts<<<manBlocks, 64, 0, st>>>(d, tape, adj, t, first + t > 0 ? 1 : 0);
       k_adj_bodies<<<nb, 256, 0, st>>>(d, tape, adj, t);
       std::swap(adj.lamb, adj.lambPrev);

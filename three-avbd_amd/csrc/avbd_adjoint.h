@@ -13,13 +13,20 @@
 // evalContact on the recorded inputs, so they are the forward's decisions exactly.
 //
 // Per step the reverse is: velocities, then for each iteration from the last the dual (per
-// contact) and the colours from the last (per body: re-form the body's 6x6 system, solve the
-// adjoint system with the same LDLᵀ, scatter to the neighbours' live adjoints with f64 atomics),
-// then the λ warm start, the c0 and friction terms, and the bodies' warm start. Within one colour
-// no two bodies share a contact, so a contact's own adjoints are written by one thread.
+// contact) and the colours from the last (per body: solve the adjoint of the body's 6x6 system
+// with the LDLᵀ factors recorded by the forward, then each contact point's rows back through its
+// evaluation), then the λ warm start, the c0 and friction terms, and the bodies' warm start. A
+// contact's parts for one of its bodies wait in a per-contact slot until that body's colour comes
+// up, so the sweep needs no float atomics (except for a dynamic body no colour sweeps). Within one
+// colour no two bodies share a contact, so a contact's own adjoints are written by one thread.
+//
+// The reverse is checked against the oracle's f64 reverse to a relative tolerance, not bit for
+// bit: multiply-adds in this file may be fused (the f32 forward's arithmetic is elsewhere and
+// stays unfused). The pragma is undone at the end of the file.
 #pragma once
 #include "avbd_kernels.h"
 
+#pragma clang fp contract(fast)
 namespace avbd {
 
 // ---- f64 vectors and quaternions -----------------------------------------------------------
@@ -126,6 +133,10 @@ struct Tape {
   uint32_t *cman;                             // [s*conCap + c]: the manifold contact c belongs to
   uchar4 *bits;                               // [(s*iters + it)*conCap + c]: the forward's branch bits
   double *sys;                                // [((s*iters + it)*n + i)*SYS_WORDS]: f64 LDLᵀ factors and step of each update
+  uint32_t ptCap;
+  uint32_t *ptOff;                            // [s*(n+1) + k]: first point of the k-th swept body (colour-bucket order)
+  uint4 *pts;                                 // [s*ptCap + p]: a body's contact point (contact, manifold, body | side << 31
+                                              //   with side 0 = A / 1 = B, partner)
   __device__ __forceinline__ size_t snap(uint32_t s, uint32_t k) const { return (size_t)s * (iters + 1) + k; }
 };
 /** f64 adjoints. state: 16 per body (pos xyz, prev-up, rot xyzw, vel xyz, 0, angvel xyz, 0): the
@@ -136,8 +147,10 @@ struct AdjBuf {
   double *mu;                      // per body friction
   double *lamb, *lambPrev;         // 4 per contact: this step's λ, the previous step's
   double *c0b, *mub;               // 4 / 1 per contact of this step
-  double *dpart;                   // 32 per contact: the dual's partials for body A, then body B
-                                   // (pose p, pad, q, x- p, pad, x- q), summed per body by the gather
+  double *dpart;                   // 32 per contact: the dual's partials of the iteration for body A, then
+                                   // body B (pose p, pad, q, x- p, pad, x- q)
+  double *nslot;                   // 32 per contact, the same layout: a point's partials for the partner
+                                   // body, written by the owner's colour, read by the partner's
 };
 
 // f64 adds into the adjoint buffers (device memory from hipMalloc, coarse-grained): the hardware
@@ -183,6 +196,40 @@ __global__ void k_tape_colors(Dev d, Tape T, uint32_t s, uint32_t cap) {
     uint32_t lo = 0, hi = cap;  // cs[lo] <= k < cs[hi]
     while (hi - lo > 1) { uint32_t mid = (lo + hi) / 2; if (cs[mid] <= k) lo = mid; else hi = mid; }
     T.col[(size_t)s * d.n + d.colorBodies[k]] = lo;
+  }
+}
+/** The contact points of every swept body (k-th in colour-bucket order) over its adjacency,
+ *  counted (zero past the swept ones) for the exclusive scan into ptOff. */
+__global__ void k_tape_point_count(Dev d, Tape T, uint32_t s, uint32_t cap) {
+  uint32_t n = d.n, total = d.colorStart[cap];
+  uint32_t *off = T.ptOff + (size_t)s * (n + 1);
+  GRID_STRIDE(k, n + 1) {
+    uint32_t cnt = 0;
+    if (k < total) {
+      uint32_t i = d.colorBodies[k];
+      for (uint32_t e = d.adjStart[i], e1 = d.adjStart[i + 1]; e < e1; e++) {
+        uint32_t a = d.adjList[e];
+        if (a >= d.nj) cnt += d.mIds[a - d.nj].w & 15u;
+      }
+    }
+    off[k] = cnt;
+  }
+}
+/** The point list: each swept body's points in adjacency order at its scanned offset. */
+__global__ void k_tape_point_fill(Dev d, Tape T, uint32_t s, uint32_t cap) {
+  uint32_t n = d.n, total = d.colorStart[cap];
+  const uint32_t *off = T.ptOff + (size_t)s * (n + 1);
+  uint4 *pts = T.pts + (size_t)s * T.ptCap;
+  GRID_STRIDE(k, total) {
+    uint32_t i = d.colorBodies[k], o = off[k];
+    for (uint32_t e = d.adjStart[i], e1 = d.adjStart[i + 1]; e < e1; e++) {
+      uint32_t a = d.adjList[e];
+      if (a < d.nj) continue;
+      uint4 ids = d.mIds[a - d.nj];
+      bool isA = ids.x == i;
+      for (uint32_t q = 0; q < (ids.w & 15u); q++, o++)
+        if (o < T.ptCap) pts[o] = make_uint4(ids.z + q, a - d.nj, isA ? i : i | 0x80000000u, isA ? ids.y : ids.x);
+    }
   }
 }
 /** Snapshot k of step s: poses and the contacts' λ and penalties; k = 0 adds the step's start. */
@@ -316,7 +363,7 @@ __global__ void k_adj_velocities(Dev d, Tape T, AdjBuf A, uint32_t s) {
     st[3] = 0; for (int k = 8; k < 16; k++) st[k] = 0;
   }
 }
-/** One side's partials of a contact: pose p, q and x- p, q (16 doubles, two pads). */
+/** One side's partials of a contact (16 doubles, two pads): pose p, q and x- p, q. */
 __device__ __forceinline__ void stPart(double *o, DV p, DQ q, DV ip, DQ ir) {
   double2 *v = reinterpret_cast<double2 *>(o);
   v[0] = make_double2(p.x, p.y); v[1] = make_double2(p.z, 0.0);
@@ -324,14 +371,30 @@ __device__ __forceinline__ void stPart(double *o, DV p, DQ q, DV ip, DQ ir) {
   v[4] = make_double2(ip.x, ip.y); v[5] = make_double2(ip.z, 0.0);
   v[6] = make_double2(ir.x, ir.y); v[7] = make_double2(ir.z, ir.w);
 }
-/** The dual of iteration `it` of step s in reverse, one contact per thread. Each contact's
- *  adjoints for its two bodies go to A.dpart; k_adj_dual_gather adds them to the bodies (a gather
- *  over each body's recorded adjacency instead of float atomics on shared bodies). */
+__device__ __forceinline__ void addPart(const double *o, DV &p, DQ &q, DV &ip, DQ &ir) {
+  const double2 *v = reinterpret_cast<const double2 *>(o);
+  double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3], v4 = v[4], v5 = v[5], v6 = v[6], v7 = v[7];
+  p = p + dv(v0.x, v0.y, v1.x);
+  q = q + dq(v2.x, v2.y, v3.x, v3.y);
+  ip = ip + dv(v4.x, v4.y, v5.x);
+  ir = ir + dq(v6.x, v6.y, v7.x, v7.y);
+}
+/** A contact's partials for one of its bodies, left for that body's colour in the reverse
+ *  (k_adj_color reads them when it takes the body up); a dynamic body that no colour sweeps gets
+ *  them added at once with f64 atomics. */
+__device__ __forceinline__ void leavePart(const Dev &d, const AdjBuf &A, const uint32_t *colS, double *slot, uint32_t b, DV p, DQ q, DV ip,
+                                          DQ ir) {
+  if (colS[b] != NO_COLOR) stPart(slot, p, q, ip, ir);
+  else scatterBody(d, A, b, p, q, ip, ir);
+}
+/** The dual of iteration `it` of step s in reverse, one contact per thread: each body's partials
+ *  go to the contact's dpart slots. */
 __global__ void __launch_bounds__(64) k_adj_dual(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it) {
   uint32_t C = T.counts[s * 4 + 1], n = d.n;
   const float4 *P = T.P + T.snap(s, it + 1) * n, *R = T.R + T.snap(s, it + 1) * n;
   const float4 *p0 = T.p0 + (size_t)s * n, *q0 = T.q0 + (size_t)s * n;
   const float4 *lam = T.lam + T.snap(s, it) * T.conCap, *pen = T.pen + T.snap(s, it) * T.conCap;
+  const uint32_t *colS = T.col + (size_t)s * n;
   double alpha = d.p.alpha;
   GRID_STRIDE(c, C) {
     size_t cc = (size_t)s * T.conCap + c;
@@ -358,45 +421,10 @@ __global__ void __launch_bounds__(64) k_adj_dual(Dev d, Tape T, AdjBuf A, uint32
     double *cb = A.c0b + 4 * (size_t)c;
     stV(cb, ldV(cb) + o.c0);
     A.mub[c] += o.mu;
-    stPart(A.dpart + 32 * (size_t)c, o.pA, o.qA, o.ipA, o.irA);
-    stPart(A.dpart + 32 * (size_t)c + 16, o.pB, o.qB, o.ipB, o.irB);
+    leavePart(d, A, colS, A.dpart + 32 * (size_t)c, a, o.pA, o.qA, o.ipA, o.irA);
+    leavePart(d, A, colS, A.dpart + 32 * (size_t)c + 16, b, o.pB, o.qB, o.ipB, o.irB);
   }
 }
-/** The dual's partials of step s into the dynamic bodies: 4 lanes per body, lane k taking
- *  points k, k+4, ... of each of the body's manifolds (the slot's recorded adjacency). */
-__global__ void __launch_bounds__(256) k_adj_dual_gather(Dev d, Tape T, AdjBuf A, uint32_t s) {
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x, i = g >> 2, lane = g & 3u, n = d.n;
-  if (i >= n || !dynamicBody(d, i)) return;  // the body's 4 lanes leave together
-  const uint32_t *adjS = T.adjStart + (size_t)s * (n + 1), *adjL = T.adjList + (size_t)s * T.adjCap;
-  const uint4 *mIds = T.mIds + (size_t)s * T.manCap;
-  DV p = dv(0, 0, 0), ip = p;
-  DQ q = dq(0, 0, 0, 0), ir = q;
-  for (uint32_t e = adjS[i], e1 = adjS[i + 1]; e < e1; e++) {
-    if (adjL[e] < d.nj) continue;
-    uint4 ids = mIds[adjL[e] - d.nj];
-    uint32_t side = ids.x == i ? 0u : 16u;
-    for (uint32_t k = lane; k < (ids.w & 15u); k += 4) {
-      const double2 *v = reinterpret_cast<const double2 *>(A.dpart + 32 * (size_t)(ids.z + k) + side);
-      double2 v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3], v4 = v[4], v5 = v[5], v6 = v[6], v7 = v[7];
-      p = p + dv(v0.x, v0.y, v1.x);
-      q = q + dq(v2.x, v2.y, v3.x, v3.y);
-      ip = ip + dv(v4.x, v4.y, v5.x);
-      ir = ir + dq(v6.x, v6.y, v7.x, v7.y);
-    }
-  }
-  for (int w = 1; w < 4; w <<= 1) {
-    p = p + shxV(p, w); q = q + shxQ(q, w);
-    ip = ip + shxV(ip, w); ir = ir + shxQ(ir, w);
-  }
-  if (lane == 0) {
-    double *st = A.state + 16 * (size_t)i, *ipp = A.ip + 4 * (size_t)i, *irp = A.ir + 4 * (size_t)i;
-    stV(st, ldV(st) + p);
-    stQ(st + 4, ldQ(st + 4) + q);
-    stV(ipp, ldV(ipp) + ip);
-    stQ(irp, ldQ(irp) + ir);
-  }
-}
-
 /** A body's f64 system: packed lower triangle of the 6x6 (lin 00 10 11 20 21 22, ang alike,
  *  cross rows angular / columns linear) and the right-hand side. */
 struct DSys { double lin[6], ang[6], cr[9], r[6]; };
@@ -454,63 +482,79 @@ __device__ __forceinline__ void solveF(const DFac &F, const double *b, double *x
     for (int j = i + 1; j < 6; j++) x[i] -= F.L[lidx(j, i)] * x[j];
   }
 }
-/** Recording: the f64 system of every body of colour `col` at the state its f32 update is about
- *  to read (run just before k_primal of that colour), factored and solved; the reverse sweep then
- *  only solves the adjoint system with these factors. `lanes` threads per body share its contact
- *  points; the system is reduced across them with shuffles. */
-__global__ void __launch_bounds__(64) k_tape_system(Dev d, Tape T, uint32_t s, uint32_t it, uint32_t col, uint32_t lanes) {
-  uint32_t n = d.n;
-  uint32_t start = d.colorStart[col], count = d.colorStart[col + 1] - start;
+/** The poses a contact point's evaluation read in the forward: the owner's before its update,
+ *  the partner's after its update when the partner's colour came first (swept), before it
+ *  otherwise; x- of both; returns the pair's friction (the geometric mean). Both versions of the
+ *  partner are loaded so that no load waits on its colour. */
+__device__ __forceinline__ double pointPoses(const Dev &d, const float4 *Pn, const float4 *Rn, const float4 *Po, const float4 *Ro,
+                                             const float4 *p0, const float4 *q0, uint32_t i, uint32_t j, bool isA, bool swept,
+                                             PoseView &PA, PoseView &PB) {
+  float4 pjn = Pn[j], rjn = Rn[j], pjo = Po[j], rjo = Ro[j];
+  float4 pj = swept ? pjn : pjo, rj = swept ? rjn : rjo, pi = Po[i], ri = Ro[i];
+  uint32_t a = isA ? i : j, b = isA ? j : i;
+  PA = PoseView{dv3(isA ? pi : pj), dq4(isA ? ri : rj), dv3(p0[a]), dq4(q0[a])};
+  PB = PoseView{dv3(isA ? pj : pi), dq4(isA ? rj : ri), dv3(p0[b]), dq4(q0[b])};
+  return sqrt((double)d.b.pos[i].w * (double)d.b.pos[j].w);
+}
+/** Recording, after the step: the f64 system of every swept body's update in every iteration at
+ *  the state that update read (its own pose before the update, each partner's after its update
+ *  when the partner's colour came first, before it otherwise; the recorded snapshots, λ and
+ *  penalties and the forward's branch bits), factored and solved; the reverse sweep then only
+ *  solves the adjoint system with these factors. Work items are (iteration, swept body); `lanes`
+ *  threads per item share its recorded contact points and the system is reduced across them. */
+__global__ void __launch_bounds__(64) k_tape_systems(Dev d, Tape T, uint32_t s, uint32_t cap, uint32_t lanes) {
+  uint32_t n = d.n, total = T.colorStart[s * 65 + cap], items = total * T.iters;
   double dt = d.p.dt, dt2 = dt * dt, alpha = d.p.alpha;
+  const float4 *p0 = T.p0 + (size_t)s * n, *q0 = T.q0 + (size_t)s * n;
+  const uint32_t *off = T.ptOff + (size_t)s * (n + 1), *colS = T.col + (size_t)s * n;
+  const uint4 *pts = T.pts + (size_t)s * T.ptCap;
   uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes, per = blockDim.x / lanes;
-  for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
-    uint32_t k = base + slot;
-    bool live = k < count;
-    uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
-    uint32_t i = item.x;
+  for (uint32_t base = blockIdx.x * per; base < items; base += gridDim.x * per) {
+    uint32_t item = base + slot;
+    bool live = item < items;
+    uint32_t it = live ? item / total : 0, k = live ? item % total : 0;
+    uint32_t i = live ? T.colorBodies[(size_t)s * n + k] : 0;
     DSys S;
 #pragma unroll
     for (int e = 0; e < 6; e++) { S.lin[e] = 0; S.ang[e] = 0; S.r[e] = 0; }
 #pragma unroll
     for (int e = 0; e < 9; e++) S.cr[e] = 0;
+    const float4 *Pn = T.P + T.snap(s, it + 1) * n, *Rn = T.R + T.snap(s, it + 1) * n;
+    const float4 *Po = T.P + T.snap(s, it) * n, *Ro = T.R + T.snap(s, it) * n;
     if (live) {
+      uint32_t col = colS[i];
       if (lane == 0) {
         float4 mo = d.b.moment[i];
         double m = (double)d.b.size[i].w / dt2, I0 = mo.x / dt2, I1 = mo.y / dt2, I2 = mo.z / dt2;
         S.lin[0] = m; S.lin[2] = m; S.lin[5] = m;
         S.ang[0] = I0; S.ang[2] = I1; S.ang[5] = I2;
-        DV dx = dv3(d.b.pos[i]) - dv3(d.b.inpos[i]);
-        DV dqv = dqsub(dq4(d.b.rot[i]), dq4(d.b.inrot[i]));
+        DV dx = dv3(Po[i]) - dv3(T.inpos[(size_t)s * n + i]);
+        DV dqv = dqsub(dq4(Ro[i]), dq4(T.inrot[(size_t)s * n + i]));
         S.r[0] = m * dx.x; S.r[1] = m * dx.y; S.r[2] = m * dx.z;
         S.r[3] = I0 * dqv.x; S.r[4] = I1 * dqv.y; S.r[5] = I2 * dqv.z;
       }
-      uint32_t run = 0;
-      for (uint32_t e = item.y; e < item.z; e++) {
-        uint4 ids = d.adjIds[e];
-        if (ids.x == 0xffffffffu) continue;  // joints: refused by avbd_forward
-        uint32_t cnt = ids.w & 15u, k0 = (lane + lanes - run % lanes) % lanes;
-        run += cnt;
-        if (k0 >= cnt) continue;
-        float4 geo = d.adjGeo[e];
-        bool isA = ids.x == i;
-        V basis[3];
-        orthonormal(xyz(geo), basis);
+      const float4 *lam = T.lam + T.snap(s, it) * T.conCap, *pen = T.pen + T.snap(s, it) * T.conCap;
+      const uchar4 *bitsIt = T.bits + ((size_t)s * T.iters + it) * T.conCap;
+      for (uint32_t p = off[k] + lane, p1 = off[k + 1]; p < p1; p += lanes) {
+        uint4 pt = pts[p];
+        uint32_t c = pt.x, j = pt.w;
+        bool isA = (pt.z >> 31) == 0;
+        float4 geo = T.mGeo[(size_t)s * T.manCap + pt.y];
+        bool swept = colS[j] < col;
+        PoseView PA, PB;
+        double mu = pointPoses(d, Pn, Rn, Po, Ro, p0, q0, i, j, isA, swept, PA, PB);
         DV l[3];
         dorthonormal(dv3(geo), l);
-        double mu = sqrt((double)d.b.pos[ids.x].w * (double)d.b.pos[ids.y].w);
-        PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
-        PoseView PA{dv3(d.b.pos[ids.x]), dq4(d.b.rot[ids.x]), dv3(d.b.ipos[ids.x]), dq4(d.b.irot[ids.x])};
-        PoseView PB{dv3(d.b.pos[ids.y]), dq4(d.b.rot[ids.y]), dv3(d.b.ipos[ids.y]), dq4(d.b.irot[ids.y])};
+        Contact kc = T.con[(size_t)s * T.conCap + c];
+        kc.pen = pen[c];
+        kc.lam = lam[c];
+        uchar4 bb = bitsIt[c];
+        DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), l, dv3(kc.pen), dv3(kc.lam), isA ? bb.x : bb.y, mu, alpha, PA, PB);
+        DV r = isA ? ev.rAW : ev.rBW;
         double sg = isA ? 1.0 : -1.0;
-        for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
-          Contact kc = d.con[c];
-          uint32_t bits = evalBits(evalContact(kc, basis, geo.w, A, B, d.p.alpha));
-          DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), l, dv3(kc.pen), dv3(kc.lam), bits, mu, alpha, PA, PB);
-          DV r = isA ? ev.rAW : ev.rBW;
-          for (int row = 0; row < 3; row++) {
-            DV jl = l[row] * sg;
-            addRowD(S, jl, dcross(r, jl), row == 0 ? kc.pen.x : (row == 1 ? kc.pen.y : kc.pen.z), dcomp(ev.F, row));
-          }
+        for (int row = 0; row < 3; row++) {
+          DV jl = l[row] * sg;
+          addRowD(S, jl, dcross(r, jl), row == 0 ? kc.pen.x : (row == 1 ? kc.pen.y : kc.pen.z), dcomp(ev.F, row));
         }
       }
     }
@@ -533,134 +577,164 @@ __global__ void __launch_bounds__(64) k_tape_system(Dev d, Tape T, uint32_t s, u
     }
   }
 }
-/** Inputs of one adjacency entry (a manifold) of body i at the versions its update read. */
-struct EntryView {
-  uint4 ids; float4 geo; bool isA; uint32_t j; DV l[3]; double mu; PoseView PA, PB;
+static const uint32_t ADJ_LANES = 8;
+/** What the reverse of one contact point reads, loaded in one round. */
+struct PointIn {
+  uint32_t c, j;
+  bool isA, swept;
+  float4 geo;
+  Contact kc;
+  uint32_t bits;
+  PoseView PA, PB;
+  double mu;
+  DV lamb, c0b;
+  double mub;
 };
-__device__ __forceinline__ void entryView(const Dev &d, const Tape &T, uint32_t s, uint32_t i, uint4 ids, uint32_t mid, uint32_t col,
-                                          const float4 *Pn, const float4 *Rn, const float4 *Po, const float4 *Ro, const float4 *p0,
-                                          const float4 *q0, EntryView &v) {
-  v.ids = ids;
-  v.geo = T.mGeo[(size_t)s * T.manCap + mid];
-  v.isA = v.ids.x == i;
-  v.j = v.isA ? v.ids.y : v.ids.x;
-  bool swept = T.col[(size_t)s * d.n + v.j] < col;  // already updated in this iteration
-  float4 pj = swept ? Pn[v.j] : Po[v.j], rj = swept ? Rn[v.j] : Ro[v.j];
-  float4 pa = v.isA ? Po[i] : pj, ra = v.isA ? Ro[i] : rj, pb = v.isA ? pj : Po[i], rb = v.isA ? rj : Ro[i];
-  dorthonormal(dv3(v.geo), v.l);
-  v.mu = sqrt((double)d.b.pos[v.ids.x].w * (double)d.b.pos[v.ids.y].w);
-  v.PA = PoseView{dv3(pa), dq4(ra), dv3(p0[v.ids.x]), dq4(q0[v.ids.x])};
-  v.PB = PoseView{dv3(pb), dq4(rb), dv3(p0[v.ids.y]), dq4(q0[v.ids.y])};
+__device__ __forceinline__ void pointIn(const Dev &d, const Tape &T, const AdjBuf &A, uint32_t s, uint32_t col, uint4 pt, const float4 *Pn,
+                                        const float4 *Rn, const float4 *Po, const float4 *Ro, const float4 *p0, const float4 *q0,
+                                        const float4 *lam, const float4 *pen, const uchar4 *bitsIt, const uint32_t *colS, PointIn &v) {
+  v.c = pt.x; v.j = pt.w;
+  v.isA = (pt.z >> 31) == 0;
+  uint32_t i = pt.z & 0x7fffffffu;
+  v.geo = T.mGeo[(size_t)s * T.manCap + pt.y];
+  v.swept = colS[v.j] < col;  // already updated in this iteration
+  v.kc = T.con[(size_t)s * T.conCap + v.c];
+  v.kc.pen = pen[v.c];
+  v.kc.lam = lam[v.c];
+  uchar4 bb = bitsIt[v.c];
+  v.bits = v.isA ? bb.x : bb.y;
+  v.mu = pointPoses(d, Pn, Rn, Po, Ro, p0, q0, i, v.j, v.isA, v.swept, v.PA, v.PB);
+  v.lamb = ldV(A.lamb + 4 * (size_t)v.c);
+  v.c0b = ldV(A.c0b + 4 * (size_t)v.c);
+  v.mub = A.mub[v.c];
 }
-/** The colour `col` of iteration `it` of step s in reverse: `lanes` threads per body share its
- *  contact points (the k-th point over its adjacency entries goes to lane k mod lanes); the
- *  system and the body's own adjoints are reduced across them with shuffles (every lane then
- *  holds the sums and solves the same 6x6 systems). */
-__global__ void __launch_bounds__(64) k_adj_primal(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it, uint32_t col, uint32_t lanes) {
+/** The reverse of one contact point of the body: its three rows of the system and right-hand
+ *  side (adjoint solve rb, step x) back through the f64 evaluation with the forward's branches.
+ *  The body's own parts go to its accumulators, the partner's to the contact's slot. */
+__device__ __forceinline__ void pointRev(const Dev &d, const AdjBuf &A, const uint32_t *colS, const PointIn &v, const double *rb, const double *x,
+                                         double alpha, DV &pbar, DQ &qbar, DV &ipb, DQ &irb) {
+  DV l[3];
+  dorthonormal(dv3(v.geo), l);
+  DV penD = dv3(v.kc.pen);
+  DEv ev = evalD(v.kc, dv(v.kc.rB.w, v.kc.pen.w, v.kc.lam.w), l, penD, dv3(v.kc.lam), v.bits, v.mu, alpha, v.PA, v.PB);
+  DV r = v.isA ? ev.rAW : ev.rBW;
+  double sg = v.isA ? 1.0 : -1.0;
+  DV Fb = dv(0, 0, 0), rW = dv(0, 0, 0);
+  for (int row = 0; row < 3; row++) {
+    DV jl = l[row] * sg, ja = dcross(r, jl);
+    double J[6] = {jl.x, jl.y, jl.z, ja.x, ja.y, ja.z};
+    double kk = dcomp(penD, row), Jx = 0, rJ = 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) { Jx += J[q] * x[q]; rJ += rb[q] * J[q]; }
+    if (row == 0) Fb.x = rJ; else if (row == 1) Fb.y = rJ; else Fb.z = rJ;
+    double f = dcomp(ev.F, row);
+    DV jab = dv(f * rb[3] - kk * (Jx * rb[3] + rJ * x[3]), f * rb[4] - kk * (Jx * rb[4] + rJ * x[4]), f * rb[5] - kk * (Jx * rb[5] + rJ * x[5]));
+    rW = rW + dcross(jl, jab);
+  }
+  EvBar ob;
+  zeroBar(ob);
+  evalRev(ev, v.kc, l, penD, v.bits, v.mu, alpha, v.PA, v.PB, Fb, v.isA ? rW : dv(0, 0, 0), v.isA ? dv(0, 0, 0) : rW, ob);
+  stV(A.lamb + 4 * (size_t)v.c, v.lamb + ob.lam);
+  stV(A.c0b + 4 * (size_t)v.c, v.c0b + ob.c0);
+  A.mub[v.c] = v.mub + ob.mu;
+  double *ns = A.nslot + 32 * (size_t)v.c;
+  if (v.isA) {
+    pbar = pbar + ob.pA; qbar = qbar + ob.qA; ipb = ipb + ob.ipA; irb = irb + ob.irA;
+    if (dynamicBody(d, v.j)) leavePart(d, A, colS, ns + 16, v.j, ob.pB, ob.qB, ob.ipB, ob.irB);
+  } else {
+    pbar = pbar + ob.pB; qbar = qbar + ob.qB; ipb = ipb + ob.ipB; irb = irb + ob.irB;
+    if (dynamicBody(d, v.j)) leavePart(d, A, colS, ns, v.j, ob.pA, ob.qA, ob.ipA, ob.irA);
+  }
+}
+/** Colour `col` of iteration `it` of step s in reverse, eight lanes per body, each lane taking
+ *  the body's recorded contact points k, k+8, ...
+ *  1. The partials left for the body since its colour last came up: this iteration's dual
+ *     (dpart), each partner swept after it in the forward (its colour came up earlier in this
+ *     reverse iteration) and each partner swept before it (in the previous reverse iteration, or
+ *     nothing yet: the slots start each step at zero). They complete the adjoint of the body's
+ *     updated pose.
+ *  2. The adjoint of the body's step Δ from that of its updated pose (p+ = p - Δlin,
+ *     q+ = qadd(q, -Δang)) through the recorded LDLᵀ factors, and the inertial rows.
+ *  3. Each point's three rows of the system and right-hand side back through the f64 evaluation
+ *     with the forward's branches: the body's own parts summed across its lanes, the partner's
+ *     left in the contact's slot. A contact's λ, c0 and friction adjoints are written by its one
+ *     lane (within one colour no two bodies share a contact). */
+__global__ void __launch_bounds__(64) k_adj_color(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t it, uint32_t col) {
   uint32_t n = d.n;
   uint32_t start = T.colorStart[s * 65 + col], count = T.colorStart[s * 65 + col + 1] - start;
   const float4 *Pn = T.P + T.snap(s, it + 1) * n, *Rn = T.R + T.snap(s, it + 1) * n;
   const float4 *Po = T.P + T.snap(s, it) * n, *Ro = T.R + T.snap(s, it) * n;
   const float4 *p0 = T.p0 + (size_t)s * n, *q0 = T.q0 + (size_t)s * n;
   const float4 *lam = T.lam + T.snap(s, it) * T.conCap, *pen = T.pen + T.snap(s, it) * T.conCap;
-  const uint32_t *adjS = T.adjStart + (size_t)s * (n + 1), *adjL = T.adjList + (size_t)s * T.adjCap;
   const uchar4 *bitsIt = T.bits + ((size_t)s * T.iters + it) * T.conCap;
+  const uint32_t *off = T.ptOff + (size_t)s * (n + 1), *colS = T.col + (size_t)s * n;
+  const uint4 *pts = T.pts + (size_t)s * T.ptCap;
   double dt = d.p.dt, dt2 = dt * dt, alpha = d.p.alpha;
-  uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes, per = blockDim.x / lanes;
-  for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
+  uint32_t slot = threadIdx.x / ADJ_LANES, lane = threadIdx.x % ADJ_LANES, per = blockDim.x / ADJ_LANES;
+  for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {  // whole waves: sums across lanes
     uint32_t k = base + slot;
     bool live = k < count;
     uint32_t i = live ? T.colorBodies[(size_t)s * n + start + k] : 0;
-    double m = 0, I[3] = {0, 0, 0};
-    DV pi = dv(0, 0, 0);
-    DQ qi = dq(0, 0, 0, 1);
-    uint32_t e0 = 0, e1 = 0;
+    uint32_t pBegin = live ? off[start + k] : 0, pEnd = live ? off[start + k + 1] : 0;
+    // one round of loads: the body's recorded system and adjoint, this lane's first point and the
+    // slots of all its points
+    uint32_t p = pBegin + lane;
+    bool first = p < pEnd;
+    PointIn v;
+    if (first) pointIn(d, T, A, s, col, pts[p], Pn, Rn, Po, Ro, p0, q0, lam, pen, bitsIt, colS, v);
+    DV gp = dv(0, 0, 0), gip = gp;
+    DQ gq = dq(0, 0, 0, 0), gir = gq;
+    for (uint32_t e = p; e < pEnd; e += ADJ_LANES) {
+      uint4 pt = pts[e];
+      size_t side = pt.z >> 31;
+      addPart(A.dpart + 32 * (size_t)pt.x + 16 * side, gp, gq, gip, gir);
+      addPart(A.nslot + 32 * (size_t)pt.x + 16 * side, gp, gq, gip, gir);
+    }
     double x[6] = {0, 0, 0, 0, 0, 0}, rb[6] = {0, 0, 0, 0, 0, 0};
-    DV pbar = dv(0, 0, 0), ipb = pbar, npb = pbar;
-    DQ qbar = dq(0, 0, 0, 0), irb = qbar, nrb = qbar;
+    DFac F;
+    double m = 0, I0 = 0, I1 = 0, I2 = 0;
     double *st = A.state + 16 * (size_t)i;
+    DV P = dv(0, 0, 0);
+    DQ Q = dq(0, 0, 0, 0), qi = Q, qin = Q;
     if (live) {
       float4 mo = d.b.moment[i];
-      m = (double)d.b.size[i].w / dt2;
-      I[0] = mo.x / dt2; I[1] = mo.y / dt2; I[2] = mo.z / dt2;
-      pi = dv3(Po[i]);
-      qi = dq4(Ro[i]);
-      e0 = adjS[i]; e1 = adjS[i + 1];
-      // the system's factors and the step, recorded by the forward (k_tape_system)
+      m = (double)d.b.size[i].w / dt2; I0 = mo.x / dt2; I1 = mo.y / dt2; I2 = mo.z / dt2;
       const double *sys = T.sys + (((size_t)s * T.iters + it) * n + i) * SYS_WORDS;
-      DFac F;
 #pragma unroll
       for (int q = 0; q < 15; q++) F.L[q] = sys[q];
 #pragma unroll
       for (int q = 0; q < 6; q++) { F.iD[q] = sys[15 + q]; x[q] = sys[21 + q]; F.D[q] = 0; }
-      // reverse of p+ = p - Δlin, q+ = qadd(q, -Δang)
-      DV P = ldV(st);
+      P = ldV(st); Q = ldQ(st + 4);
+      qi = dq4(Ro[i]); qin = dq4(T.inrot[(size_t)s * n + i]);
+    }
+    for (uint32_t w = 1; w < ADJ_LANES; w <<= 1) {
+      gp = gp + shxV(gp, w); gip = gip + shxV(gip, w);
+      gq = gq + shxQ(gq, w); gir = gir + shxQ(gir, w);
+    }
+    DV pbar = dv(0, 0, 0), ipb = pbar, rl = pbar;
+    DQ qbar = dq(0, 0, 0, 0), irb = qbar, nrb = qbar;
+    if (live) {
+      P = P + gp;
       DQ qb0 = dq(0, 0, 0, 0);
       DV wb = dv(0, 0, 0);
-      qaddRev(qi, dv(-x[3], -x[4], -x[5]), ldQ(st + 4), qb0, wb);
+      qaddRev(qi, dv(-x[3], -x[4], -x[5]), Q + gq, qb0, wb);
       double xb[6] = {-P.x, -P.y, -P.z, -wb.x, -wb.y, -wb.z};
       solveF(F, xb, rb);
-      if (lane == 0) {
-        DV rl = dv(m * rb[0], m * rb[1], m * rb[2]);
+      if (lane == 0) {  // the inertial rows and the partials gathered above
+        rl = dv(m * rb[0], m * rb[1], m * rb[2]);
         pbar = P + rl;
-        npb = -rl;
         qbar = qb0;
-        qsubRev(qi, dq4(T.inrot[(size_t)s * n + i]), dv(I[0] * rb[3], I[1] * rb[4], I[2] * rb[5]), qbar, nrb);
-      }
-      // pass 2: the rows' adjoints, this lane's points
-      uint32_t run2 = 0;
-      for (uint32_t e = e0; e < e1; e++) {
-        uint32_t mid = adjL[e] - d.nj;
-        uint4 ids = T.mIds[(size_t)s * T.manCap + mid];
-        uint32_t cnt = ids.w & 15u, k0 = (lane + lanes - run2 % lanes) % lanes;
-        run2 += cnt;
-        if (k0 >= cnt) continue;
-        EntryView v;
-        entryView(d, T, s, i, ids, mid, col, Pn, Rn, Po, Ro, p0, q0, v);
-        double sg = v.isA ? 1.0 : -1.0;
-        DV nbP = dv(0, 0, 0), nbIp = nbP;
-        DQ nbQ = dq(0, 0, 0, 0), nbIr = nbQ;
-        for (uint32_t c = ids.z + k0; c < ids.z + cnt; c += lanes) {
-          Contact kc = T.con[(size_t)s * T.conCap + c];
-          kc.pen = pen[c];
-          kc.lam = lam[c];
-          uchar4 bb = bitsIt[c];
-          uint32_t bits = v.isA ? bb.x : bb.y;
-          DV penD = dv3(kc.pen);
-          DEv ev = evalD(kc, dv(kc.rB.w, kc.pen.w, kc.lam.w), v.l, penD, dv3(kc.lam), bits, v.mu, alpha, v.PA, v.PB);
-          DV r = v.isA ? ev.rAW : ev.rBW;
-          DV Fb = dv(0, 0, 0), rW = dv(0, 0, 0);
-          for (int row = 0; row < 3; row++) {
-            DV jl = v.l[row] * sg, ja = dcross(r, jl);
-            double J[6] = {jl.x, jl.y, jl.z, ja.x, ja.y, ja.z};
-            double kk = dcomp(penD, row), Jx = 0, rJ = 0;
-#pragma unroll
-            for (int q = 0; q < 6; q++) { Jx += J[q] * x[q]; rJ += rb[q] * J[q]; }
-            if (row == 0) Fb.x = rJ; else if (row == 1) Fb.y = rJ; else Fb.z = rJ;
-            double f = dcomp(ev.F, row);
-            DV jab = dv(f * rb[3] - kk * (Jx * rb[3] + rJ * x[3]), f * rb[4] - kk * (Jx * rb[4] + rJ * x[4]), f * rb[5] - kk * (Jx * rb[5] + rJ * x[5]));
-            rW = rW + dcross(jl, jab);
-          }
-          EvBar o;
-          zeroBar(o);
-          evalRev(ev, kc, v.l, penD, bits, v.mu, alpha, v.PA, v.PB, Fb, v.isA ? rW : dv(0, 0, 0), v.isA ? dv(0, 0, 0) : rW, o);
-          double *lb = A.lamb + 4 * (size_t)c;
-          stV(lb, ldV(lb) + o.lam);
-          double *cb = A.c0b + 4 * (size_t)c;
-          stV(cb, ldV(cb) + o.c0);
-          A.mub[c] += o.mu;
-          if (v.isA) {
-            pbar = pbar + o.pA; qbar = qbar + o.qA; ipb = ipb + o.ipA; irb = irb + o.irA;
-            nbP = nbP + o.pB; nbQ = nbQ + o.qB; nbIp = nbIp + o.ipB; nbIr = nbIr + o.irB;
-          } else {
-            pbar = pbar + o.pB; qbar = qbar + o.qB; ipb = ipb + o.ipB; irb = irb + o.irB;
-            nbP = nbP + o.pA; nbQ = nbQ + o.qA; nbIp = nbIp + o.ipA; nbIr = nbIr + o.irA;
-          }
-        }
-        scatterBody(d, A, v.j, nbP, nbQ, nbIp, nbIr);
+        qsubRev(qi, qin, dv(I0 * rb[3], I1 * rb[4], I2 * rb[5]), qbar, nrb);
+        ipb = gip;
+        irb = gir;
       }
     }
-    for (uint32_t w = 1; w < lanes; w <<= 1) {
+    if (first) pointRev(d, A, colS, v, rb, x, alpha, pbar, qbar, ipb, irb);
+    for (uint32_t e = p + ADJ_LANES; e < pEnd; e += ADJ_LANES) {  // bodies with more than eight points
+      pointIn(d, T, A, s, col, pts[e], Pn, Rn, Po, Ro, p0, q0, lam, pen, bitsIt, colS, v);
+      pointRev(d, A, colS, v, rb, x, alpha, pbar, qbar, ipb, irb);
+    }
+    for (uint32_t w = 1; w < ADJ_LANES; w <<= 1) {
       pbar = pbar + shxV(pbar, w); ipb = ipb + shxV(ipb, w);
       qbar = qbar + shxQ(qbar, w); irb = irb + shxQ(irb, w);
     }
@@ -670,9 +744,34 @@ __global__ void __launch_bounds__(64) k_adj_primal(Dev d, Tape T, AdjBuf A, uint
       double *ip = A.ip + 4 * (size_t)i, *ir = A.ir + 4 * (size_t)i, *np = A.np + 4 * (size_t)i, *nr = A.nr + 4 * (size_t)i;
       stV(ip, ldV(ip) + ipb);
       stQ(ir, ldQ(ir) + irb);
-      stV(np, ldV(np) + npb);
+      stV(np, ldV(np) - rl);
       stQ(nr, ldQ(nr) + nrb);
     }
+  }
+}
+/** After the last reverse iteration of step s: the slots each swept body's partners of lower colour
+ *  left in that iteration (its colour had already come up) into the body's adjoints. */
+__global__ void k_adj_slots_final(Dev d, Tape T, AdjBuf A, uint32_t s, uint32_t cap) {
+  uint32_t n = d.n, total = T.colorStart[s * 65 + cap];
+  const uint32_t *off = T.ptOff + (size_t)s * (n + 1), *colS = T.col + (size_t)s * n;
+  const uint4 *pts = T.pts + (size_t)s * T.ptCap;
+  GRID_STRIDE(k, total) {
+    uint32_t i = T.colorBodies[(size_t)s * n + k], ci = colS[i];
+    DV p = dv(0, 0, 0), ip = p;
+    DQ q = dq(0, 0, 0, 0), ir = q;
+    bool any = false;
+    for (uint32_t e = off[k], e1 = off[k + 1]; e < e1; e++) {
+      uint4 pt = pts[e];
+      if (colS[pt.w] >= ci) continue;
+      addPart(A.nslot + 32 * (size_t)pt.x + 16 * (size_t)(pt.z >> 31), p, q, ip, ir);
+      any = true;
+    }
+    if (!any) continue;
+    double *st = A.state + 16 * (size_t)i, *ipp = A.ip + 4 * (size_t)i, *irp = A.ir + 4 * (size_t)i;
+    stV(st, ldV(st) + p);
+    stQ(st + 4, ldQ(st + 4) + q);
+    stV(ipp, ldV(ipp) + ip);
+    stQ(irp, ldQ(irp) + ir);
   }
 }
 /** Warm start of λ (to the previous step's contacts), c0 and friction of step s in reverse; clears
@@ -801,3 +900,4 @@ __global__ void k_adj_remap_counted(const uint4 *from, const uint4 *to, const ui
 }
 
 }  // namespace avbd
+#pragma clang fp contract(off)
