@@ -363,6 +363,26 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every);
  * rollout ended. */
 int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, double *d_friction);
 
+/* Scene planning before a solver exists, as GpuSolver3D's constructor does it from the reference
+ * scene (host code: no GPU is touched). */
+/* mortonOrder (solver.ts:463-485): order[g] = the index into `bodies` of the body placed g-th
+ * along a Z-order curve of the positions (10 bits per axis over their bounding box), ties by
+ * index; the reference writes the scene in this order unless spatialSort is false (:800). */
+int avbd_morton_order(const float *bodies, uint32_t body_count, uint32_t *order);
+/* estimatePairs (solver.ts:350-460): the broadphase's pairs for these bodies (bounding spheres and
+ * world AABBs overlap, not both static) counted on a CPU hash grid, how many of them touch (the
+ * separating-axis test, 1e-3 tolerance), and each body's touching pairs added into `degree`
+ * (body_count ints, may be NULL). */
+int avbd_estimate_pairs(const float *bodies, uint32_t body_count, uint64_t *pairs, uint64_t *touching, int32_t *degree);
+/* The constructor's sizing from that estimate (solver.ts:745, :815-843): body capacity
+ * (least->bodies, else bodies + 1024), joints (+256 spare), pairs max(4096, 4 cap, 2 pairs),
+ * manifolds max(4096, cap, 1.6 touching) but at most the pairs, contacts max(8192, 4 cap,
+ * 8 touching), each at least least's (NULL: none); and the starting colour cap
+ * min(64, max(8, max degree of a dynamic body over touching pairs and joints + 4, least_colors)).
+ * joint_info: (type, a, b, 0) per joint, a = -1 for a world joint. */
+int avbd_plan_capacity(const float *bodies, uint32_t body_count, const int32_t *joint_info, uint32_t joint_count,
+                       const avbd_capacity *least, uint32_t least_colors, avbd_capacity *out, uint32_t *color_cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,8 @@ import pytest
 import avbd_amd
 import oracle
 from avbd_amd import records, scenes
-from helpers import PARITY_SCENES, brute_force_pairs, contact_match, random_box_pairs, random_broadphase_scene, sorted_contacts
+from helpers import (PARITY_SCENES, brute_force_pairs, contact_match, oracle_from_gpu, random_box_pairs, random_broadphase_scene,
+                     sorted_contacts)
 
 pytestmark = pytest.mark.gpu
 
@@ -430,6 +431,42 @@ def test_lone_reuse_steps_rebuild_the_adjacency(gpu):
     assert (m.joints()[:, 3] != 0).sum() == len(joints) - 2
     assert np.array_equal(records.bodies_to_oracle(s.read_bodies()), m.bodies())
     assert_same_contacts(s, m)
+    s.destroy()
+
+
+def test_lone_reuse_steps_after_host_edits(gpu):
+    """A host edit between lone reuse substeps: a body with contacts disabled, a body pushed
+    (rewriteBodies) and a body's pose patched. Each time the oracle is seeded from the GPU's state
+    after the edit, and the next reuse substep must match it, so an adjacency wrongly kept from
+    before the edit (the disabled body's contacts still in the sweep, or stale manifolds) fails."""
+    say("case: lone reuse substeps after host edits (disable, rewrite, patch), 300 bodies")
+    bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32, dt=1 / 240)
+    s = gpu_for(bodies, kw, joints, info, hulls=hw)
+    for _ in range(40):  # settle into contact
+        s.step()
+    man, _ = s.read_contacts_raw()
+    dynamic = s.read_bodies()[:, 19] > 0
+    touching = [int(b) for b in np.unique(man[:, :2]) if dynamic[int(b)]]
+    assert len(touching) >= 3
+    edits = ("disable", "rewrite", "patch")
+    for k, edit in enumerate(edits):
+        b = touching[(k * 37) % len(touching)]
+        rec = s.read_bodies(b, 1)
+        if edit == "disable":
+            s.set_lifecycle(b, 2)
+        elif edit == "rewrite":
+            rec[0, 0:3] += (0.05, -0.02, 0.03)
+            s.write_bodies(rec, first=b)
+        else:
+            rec[0, 4:8] = [0.0, 0.0, np.sin(0.05), np.cos(0.05)]
+            s.patch_body(b, ("rotation",), rec[0])
+        m = oracle_from_gpu(s, hw)
+        for u in range(2):
+            s.step(reuse_manifolds=True)
+            m.step(1, reuse_manifolds=True)
+            assert np.array_equal(records.bodies_to_oracle(s.read_bodies()), m.bodies()), (edit, u)
+            assert_same_contacts(s, m)
     s.destroy()
 
 

@@ -99,6 +99,7 @@ EXPORTS = [
     "avbd_read_joint_info", "avbd_get_sizes", "avbd_add_bodies", "avbd_patch_body", "avbd_rewrite_fixed",
     "avbd_append_joints", "avbd_append_joint", "avbd_append_world_joint", "avbd_append_springs", "avbd_release_joints",
     "avbd_set_world_anchor", "avbd_disable_constraint", "avbd_set_body_aux", "avbd_set_lifecycle",
+    "avbd_morton_order", "avbd_estimate_pairs", "avbd_plan_capacity",
 ]
 FIELD_POSITION, FIELD_ROTATION, FIELD_VELOCITY, FIELD_ANGULAR_VELOCITY = 1, 2, 4, 8
 _FIELDS = {"position": 1, "rotation": 2, "velocity": 4, "angularVelocity": 8}
@@ -160,6 +161,41 @@ def default_params(dt=1 / 60, gravity=-9.81, iterations=10, alpha=0.99, beta_lin
     p.flags, p.iterations, p.color_cap, p.color_rounds = flags, iterations, color_cap, color_rounds
     p.primal_lanes[0], p.primal_lanes[1], p.primal_lanes[2], p.primal_lanes[3] = primal_lanes
     return p
+
+
+# --- scene planning before a solver exists (host code in the library) -------------------------
+def morton_order(bodies):
+    """mortonOrder (solver.ts:463-485): order[g] = the index of the body placed g-th."""
+    rec = np.ascontiguousarray(bodies, np.float32).reshape(-1, BODY_FLOATS)
+    order = np.zeros(len(rec), np.uint32)
+    if lib().avbd_morton_order(_p(rec), len(rec), _p(order)) != 0:
+        raise ValueError("avbd_morton_order refused its arguments")
+    return order
+
+
+def estimate_pairs(bodies):
+    """estimatePairs (solver.ts:350-460): (pairs, touching, degree per body)."""
+    rec = np.ascontiguousarray(bodies, np.float32).reshape(-1, BODY_FLOATS)
+    pairs, touching = ctypes.c_uint64(), ctypes.c_uint64()
+    degree = np.zeros(len(rec), np.int32)
+    if lib().avbd_estimate_pairs(_p(rec), len(rec), ctypes.byref(pairs), ctypes.byref(touching), _p(degree)) != 0:
+        raise ValueError("avbd_estimate_pairs refused its arguments")
+    return pairs.value, touching.value, degree
+
+
+def plan_capacity(bodies, joint_info=None, least=None, colors=0):
+    """The constructor's sizing (solver.ts:745, :815-843): ({bodies, joints, pairs, manifolds,
+    contacts, hull_words}, starting colour cap); `least` as GpuSolverOptions.capacity."""
+    rec = np.ascontiguousarray(bodies, np.float32).reshape(-1, BODY_FLOATS)
+    info = np.zeros((0, 4), np.int32) if joint_info is None else np.ascontiguousarray(joint_info, np.int32).reshape(-1, 4)
+    lo = Capacity()
+    for k, v in (least or {}).items():
+        setattr(lo, k, int(v))
+    out, cap = Capacity(), ctypes.c_uint32()
+    if lib().avbd_plan_capacity(_p(rec), len(rec), _p(info), len(info), ctypes.byref(lo), int(colors), ctypes.byref(out),
+                                ctypes.byref(cap)) != 0:
+        raise ValueError("avbd_plan_capacity refused its arguments")
+    return {k: getattr(out, k) for k, _ in Capacity._fields_}, cap.value
 
 
 class GpuSolver3D:

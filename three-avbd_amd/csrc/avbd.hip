@@ -17,6 +17,7 @@
 #include "avbd_adjoint.h"
 #include "avbd_events.h"
 #include "avbd_commands.h"
+#include "avbd_plan.h"
 
 using namespace avbd;
 
@@ -2158,3 +2159,42 @@ int avbd_set_lifecycle(avbd_solver *s, uint32_t index, uint32_t bits) {
 }
 
 }  // extern "C"
+
+// ---- scene planning (host only) ----
+int avbd_morton_order(const float *bodies, uint32_t body_count, uint32_t *order) {
+  if ((!bodies || !order) && body_count) return AVBD_ERR_INVALID;
+  avbd_plan::mortonOrder(bodies, body_count, order);
+  return AVBD_OK;
+}
+int avbd_estimate_pairs(const float *bodies, uint32_t body_count, uint64_t *pairs, uint64_t *touching, int32_t *degree) {
+  if (!bodies && body_count) return AVBD_ERR_INVALID;
+  avbd_plan::Estimate e = avbd_plan::estimatePairs(bodies, body_count, degree, largestSmall);
+  if (pairs) *pairs = e.pairs;
+  if (touching) *touching = e.touching;
+  return AVBD_OK;
+}
+int avbd_plan_capacity(const float *bodies, uint32_t body_count, const int32_t *joint_info, uint32_t joint_count,
+                       const avbd_capacity *least, uint32_t least_colors, avbd_capacity *out, uint32_t *color_cap) {
+  if ((!bodies && body_count) || (!joint_info && joint_count) || !out) return AVBD_ERR_INVALID;
+  avbd_capacity l = least ? *least : avbd_capacity{};
+  std::vector<int32_t> degree(body_count, 0);
+  avbd_plan::Estimate e = avbd_plan::estimatePairs(bodies, body_count, degree.data(), largestSmall);
+  for (uint32_t k = 0; k < joint_count; k++) {
+    int32_t a = joint_info[4ull * k + 1], b = joint_info[4ull * k + 2];
+    if (a >= 0 && (uint32_t)a < body_count) degree[a]++;
+    if (b >= 0 && (uint32_t)b < body_count) degree[b]++;
+  }
+  int32_t maxDegree = 0;
+  for (uint32_t i = 0; i < body_count; i++)
+    if (bodies[40ull * i + 19] > 0.0f) maxDegree = std::max(maxDegree, degree[i]);
+  uint64_t cap = l.bodies ? std::max<uint64_t>({l.bodies, body_count, 1}) : std::max<uint64_t>(body_count + 1024ull, 1);
+  auto fit = [](double v) { return (uint32_t)std::min(std::ceil(v), 4294967295.0); };
+  out->bodies = (uint32_t)std::min<uint64_t>(cap, 0xffffffffull);
+  out->joints = std::max(joint_count + 256u, l.joints);
+  out->contacts = fit(std::max({8192.0, 4.0 * cap, 8.0 * (double)e.touching, (double)l.contacts}));
+  out->pairs = fit(std::max({4096.0, 4.0 * cap, 2.0 * (double)e.pairs, (double)l.pairs}));
+  out->manifolds = std::min(fit(std::max({4096.0, (double)cap, 1.6 * (double)e.touching, (double)l.manifolds})), out->pairs);
+  out->hull_words = l.hull_words;
+  if (color_cap) *color_cap = (uint32_t)std::min<int64_t>(64, std::max<int64_t>({8, (int64_t)maxDegree + 1 + 3, (int64_t)least_colors}));
+  return AVBD_OK;
+}

@@ -1335,8 +1335,8 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
   }
 }
 /** dualManifold's update of one contact point c (wgsl-solve.ts:637-662), given its manifold's basis,
- *  friction and both bodies' terms: penalty ramp, stick flag, λ; the whole 64-byte record written
- *  back (anchors as read: a full-line write, no partial-line merge). */
+ *  friction and both bodies' terms: penalty ramp, stick flag, λ. Written back: the record's upper
+ *  32 bytes (penalties and multipliers) and, when the stick bit flipped, the key word. */
 __device__ __forceinline__ void dualContact(const Dev &d, uint32_t c, const V *basis, float friction, const PairBody &A, const PairBody &B,
                                             float alpha) {
   Contact k = d.con[c];
