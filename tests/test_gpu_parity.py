@@ -435,10 +435,12 @@ def test_lone_reuse_steps_rebuild_the_adjacency(gpu):
 
 
 def test_lone_reuse_steps_after_host_edits(gpu):
-    """A host edit between lone reuse substeps: a body with contacts disabled, a body pushed
-    (rewriteBodies) and a body's pose patched. Each time the oracle is seeded from the GPU's state
-    after the edit, and the next reuse substep must match it, so an adjacency wrongly kept from
-    before the edit (the disabled body's contacts still in the sweep, or stale manifolds) fails."""
+    """A host edit between lone reuse substeps: a body with contacts disabled (setLifecycle), a
+    body moved (rewriteBodies) and a body's pose patched (patchBody). Each edit invalidates the
+    manifolds (the reference's manifoldsValid = false, solver.ts:1131, :1792, :2075), so the
+    next substep detects afresh although reuse was asked for, and the one after it reuses. The
+    oracle is seeded from the GPU's state after the edit and runs the same two substeps: stale
+    manifolds or an adjacency kept from before the edit fail the comparison."""
     say("case: lone reuse substeps after host edits (disable, rewrite, patch), 300 bodies")
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
     kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32, dt=1 / 240)
@@ -464,7 +466,7 @@ def test_lone_reuse_steps_after_host_edits(gpu):
         m = oracle_from_gpu(s, hw)
         for u in range(2):
             s.step(reuse_manifolds=True)
-            m.step(1, reuse_manifolds=True)
+            m.step(1, reuse_manifolds=u > 0)
             assert np.array_equal(records.bodies_to_oracle(s.read_bodies()), m.bodies()), (edit, u)
             assert_same_contacts(s, m)
     s.destroy()
