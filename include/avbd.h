@@ -363,6 +363,18 @@ int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every);
  * rollout ended. */
 int avbd_backward(avbd_solver *s, const double *d_final, double *d_initial, double *d_friction);
 
+/* Per-substep hipGraphs (no counterpart in the reference, whose WebGPU command buffers are encoded
+ * per step): with `on`, each substep of avbd_step/avbd_steps runs as one graph launch. A substep is
+ * captured the first time its launch sequence occurs (the host's every input to it: buffers and
+ * capacities, params and colour cap, schedule, sparse tail, reuse) and replayed whenever that
+ * sequence recurs; a few such graphs are kept per solver. Results equal the plain launches' bit
+ * for bit (the step clock the contact stamps use lives on the device). Rollouts recorded for the
+ * adjoint and profiled steps launch plainly, and so does a cooperative solve the runtime will not
+ * capture. `on` = 0 drops the graphs. */
+int avbd_set_graphs(avbd_solver *s, int on);
+/* Substeps captured into graphs and substeps replayed from them since creation. */
+int avbd_graph_stats(const avbd_solver *s, uint64_t *captures, uint64_t *replays);
+
 /* Scene planning before a solver exists, as GpuSolver3D's constructor does it from the reference
  * scene (host code: no GPU is touched). */
 /* mortonOrder (solver.ts:463-485): order[g] = the index into `bodies` of the body placed g-th

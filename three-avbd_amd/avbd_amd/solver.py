@@ -99,7 +99,7 @@ EXPORTS = [
     "avbd_read_joint_info", "avbd_get_sizes", "avbd_add_bodies", "avbd_patch_body", "avbd_rewrite_fixed",
     "avbd_append_joints", "avbd_append_joint", "avbd_append_world_joint", "avbd_append_springs", "avbd_release_joints",
     "avbd_set_world_anchor", "avbd_disable_constraint", "avbd_set_body_aux", "avbd_set_lifecycle",
-    "avbd_morton_order", "avbd_estimate_pairs", "avbd_plan_capacity",
+    "avbd_morton_order", "avbd_estimate_pairs", "avbd_plan_capacity", "avbd_set_graphs", "avbd_graph_stats",
 ]
 FIELD_POSITION, FIELD_ROTATION, FIELD_VELOCITY, FIELD_ANGULAR_VELOCITY = 1, 2, 4, 8
 _FIELDS = {"position": 1, "rotation": 2, "velocity": 4, "angularVelocity": 8}
@@ -342,6 +342,18 @@ class GpuSolver3D:
         c = Capacity()
         self._check(self.L.avbd_get_capacity(self.h, ctypes.byref(c)))
         return {k: getattr(c, k) for k, _ in Capacity._fields_}
+
+    def set_graphs(self, on=True):
+        """Per-substep hipGraphs: each substep's launch sequence captured once and replayed
+        (results equal the plain launches bit for bit)."""
+        self._live()
+        self._check(self.L.avbd_set_graphs(self.h, 1 if on else 0))
+
+    def graph_stats(self):
+        """(substeps captured, substeps replayed) since creation."""
+        c, r = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.avbd_graph_stats(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def adapt(self, counters=None):
         """adapt (solver.ts:2362-2407) from a counters readback: the colour cap and rounds, and

@@ -252,6 +252,7 @@ struct avbd_solver {
     return q;
   }
   ~avbd_solver() {
+    dropGraphs();
     freeAdjoint();
     freeMem(watchMem);
     if (revStream) hipStreamDestroy(revStream);
@@ -346,6 +347,7 @@ struct avbd_solver {
    *  left alone, so warm starts survive; the adjoint's buffers and any recorded rollout are dropped. */
   void growStorage(uint32_t nCon, uint32_t nPairs, uint32_t nMan) {
     hipStreamSynchronize(stream);
+    dropGraphs();
     freeAdjoint();
     fwdValid = false;
     fwdWhy = "the solver's storage grew after avbd_forward";
@@ -376,13 +378,13 @@ struct avbd_solver {
     p.dt = hp.dt; p.gravity = hp.gravity; p.betaLin = hp.beta_lin; p.betaAng = hp.beta_ang; p.gamma = hp.gamma; p.alpha = hp.alpha;
     p.margin = hp.contact_margin; p.upx = hp.up[0]; p.upy = hp.up[1]; p.upz = hp.up[2];
     p.flags = hp.flags; p.iterations = hp.iterations; p.colorCap = fixedColors ? fixedCap : hp.color_cap; p.rounds = hp.color_rounds;
-    p.step = stepCount;
     p.lanes1 = hp.primal_lanes[0]; p.lanes2 = hp.primal_lanes[1]; p.lanes4 = hp.primal_lanes[2];
     p.lanes8 = hp.primal_lanes[3];
     return p;
   }
   Dev dev() const {
     Dev d;
+    std::memset(&d, 0, sizeof d);  // every byte defined: a substep's graph is keyed by these bytes
     d.n = n; d.nj = nj; d.b = b; d.refPos = refPos; d.refRot = refRot; d.filters = filters; d.joints = joints; d.jinfo = jinfo;
     d.statics = statics; d.largeCount = (uint32_t)large.size(); d.noCollideCount = noCollideCount; d.hulls = hulls;
     d.ncFirst = statics + large.size() + 3ull * noCollideCount;
@@ -557,6 +559,7 @@ struct avbd_solver {
       timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel(solveKernel(coopThreads), dim3(coopBlocks), dim3(coopThreads), args, 0, stream); });
       if (e == hipSuccess) return;
       (void)hipGetLastError();
+      if (capturing) { captureFailed = true; return; }  // not capturable: the substep runs plainly
       coopBlocks = 0;  // not co-resident here: per-colour launches from now on
     }
     if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
@@ -871,6 +874,74 @@ struct avbd_solver {
     err = "the cooperative solve's grid barrier gave up waiting (" + std::to_string(c[C_BARRIER]) +
           " waits): the solve of the steps since the last readback ran unordered; the solver now uses per-colour launches";
   }
+  void substep(bool reuseManifolds, bool keepAdjacency) {
+    collision(reuseManifolds, keepAdjacency);
+    topology();
+    solve();
+  }
+  // ---- per-substep hipGraphs (avbd_set_graphs) ----
+  /** Everything the host decides a substep's launches by: the kernels' arguments (buffers,
+   *  capacities, params) and the choices taken outside them. Equal bytes, equal launches. */
+  struct GraphKey {
+    Dev d;
+    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge;
+    const uint32_t *fixedDev;
+  };
+  struct GraphEntry { GraphKey key; hipGraphExec_t exec; uint64_t used; };
+  static const size_t GRAPH_CACHE = 8;
+  bool graphs = false, capturing = false, captureFailed = false, coopUncapturable = false;
+  std::vector<GraphEntry> graphCache;
+  uint64_t graphClock = 0, graphCaptures = 0, graphReplays = 0;
+  GraphKey graphKey(bool reuse, bool keep) const {
+    GraphKey k;
+    std::memset(&k, 0, sizeof k);
+    k.d = dev();
+    k.reuse = reuse; k.keep = keep; k.coopBlocks = coopBlocks; k.coopThreads = coopThreads; k.tailStart = tailStart;
+    k.barrierPolls = barrierPolls; k.hullBlocks = hullBlocks; k.anyLarge = anyDynamicLarge; k.fixedDev = fixedColors ? fixedDev : nullptr;
+    return k;
+  }
+  void dropGraphs() {
+    for (auto &g : graphCache) hipGraphExecDestroy(g.exec);
+    graphCache.clear();
+  }
+  /** The substep as one graph launch: replayed when a graph of the same key was captured, else
+   *  captured now (the capture runs nothing) and launched. False: run the launches plainly (a
+   *  cooperative launch this runtime will not capture, or a failed capture). */
+  bool replayGraph(bool reuse, bool keep) {
+    if (coopBlocks && coopUncapturable) return false;
+    GraphKey key = graphKey(reuse, keep);
+    graphClock++;
+    for (auto &g : graphCache)
+      if (std::memcmp(&g.key, &key, sizeof key) == 0) {
+        if (hipGraphLaunch(g.exec, stream) != hipSuccess) return false;
+        g.used = graphClock;
+        graphReplays++;
+        return true;
+      }
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) { (void)hipGetLastError(); return false; }
+    capturing = true;
+    captureFailed = false;
+    substep(reuse, keep);
+    capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    hipGraphExec_t exec = nullptr;
+    bool ok = e == hipSuccess && !captureFailed && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    if (!ok) {
+      (void)hipGetLastError();
+      if (coopBlocks) coopUncapturable = true;
+      return false;
+    }
+    if (graphCache.size() >= GRAPH_CACHE) {
+      auto old = std::min_element(graphCache.begin(), graphCache.end(), [](const GraphEntry &a, const GraphEntry &b) { return a.used < b.used; });
+      hipGraphExecDestroy(old->exec);
+      graphCache.erase(old);
+    }
+    graphCache.push_back(GraphEntry{key, exec, graphClock});
+    graphCaptures++;
+    return hipGraphLaunch(exec, stream) == hipSuccess;
+  }
   /** One substep. keepAdjacency: the caller ran the previous substep right before this one with
    *  nothing of the host's in between (the substeps of one avbd_steps call), so a reuse that keeps
    *  every manifold may keep the adjacency too; any other call rebuilds it. */
@@ -878,9 +949,8 @@ struct avbd_solver {
     refreshStatics();
     stepCount++;
     reuseManifolds = reuseManifolds && manifoldsValid;
-    collision(reuseManifolds, reuseManifolds && keepAdjacency && adjacencyHeld);
-    topology();
-    solve();
+    bool keep = reuseManifolds && keepAdjacency && adjacencyHeld;
+    if (!(graphs && !rec && !timer.on && replayGraph(reuseManifolds, keep))) substep(reuseManifolds, keep);
     parity = 1 - parity;
     manifoldsValid = true;
     adjacencyHeld = true;
@@ -1313,6 +1383,7 @@ int avbd_seed(avbd_solver *s, const float *bodies, uint32_t body_count, const fl
   uint32_t c[C_WORDS] = {0};
   c[C_CONTACTS] = nc;
   c[C_MANIFOLDS] = nm;
+  c[C_STEP] = s->stepCount;
   hipMemcpyAsync(s->counters, c, sizeof(c), hipMemcpyHostToDevice, s->stream);
   s->manifoldsValid = false;
   return avbd_sync(s);
@@ -2196,5 +2267,19 @@ int avbd_plan_capacity(const float *bodies, uint32_t body_count, const int32_t *
   out->manifolds = std::min(fit(std::max({4096.0, (double)cap, 1.6 * (double)e.touching, (double)l.manifolds})), out->pairs);
   out->hull_words = l.hull_words;
   if (color_cap) *color_cap = (uint32_t)std::min<int64_t>(64, std::max<int64_t>({8, (int64_t)maxDegree + 1 + 3, (int64_t)least_colors}));
+  return AVBD_OK;
+}
+int avbd_set_graphs(avbd_solver *s, int on) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (avbd_sync(s)) return AVBD_ERR_DEVICE;
+  s->graphs = on != 0;
+  if (!s->graphs) s->dropGraphs();
+  return AVBD_OK;
+}
+int avbd_graph_stats(const avbd_solver *s, uint64_t *captures, uint64_t *replays) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (captures) *captures = s->graphCaptures;
+  if (replays) *replays = s->graphReplays;
   return AVBD_OK;
 }

@@ -89,6 +89,7 @@ enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASH
        // since it was built, or the host asked: a lone step, or storage regrown)
        C_TOPO = 15,
        C_BARRIER = 16,  // cooperative solve: grid-barrier waits given up since the host last cleared it
+       C_STEP = 17,     // substeps since creation (the contact stamps' clock), advanced by k_begin
        C_WORDS = 20 };
 
 /** One contact point: 4 float4, the w's carrying the key and C(x-) (layout.ts:123-127). */
@@ -98,7 +99,7 @@ struct __align__(16) Contact { float4 rA; float4 rB; float4 pen; float4 lam; };
 struct Params {
   float dt, gravity, betaLin, betaAng, gamma, alpha, margin;
   float upx, upy, upz;
-  uint32_t flags, iterations, colorCap, rounds, step;
+  uint32_t flags, iterations, colorCap, rounds;
   uint32_t lanes1, lanes2, lanes4, lanes8;
 };
 

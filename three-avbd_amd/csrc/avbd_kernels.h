@@ -98,6 +98,7 @@ __global__ void k_begin(Dev d, int reuseManifolds, int mayKeepAdjacency) {
   if (!reuseManifolds) c[C_PAIRS] = 0;
   c[C_PREV_OVERFLOW] = c[C_OVERFLOW];
   c[C_OVERFLOW] = 0;
+  c[C_STEP] += 1u;  // the step clock lives on the device, so a captured substep replays as it ran
   c[C_TOPO] &= ~1u;  // the adjacency is rebuilt unless this substep's reuse keeps it
   if (!mayKeepAdjacency) c[C_TOPO] |= 2u;  // the host's word: rebuild (buffers regrown, or a lone step)
   c[C_HULL_PAIRS] = 0;
@@ -129,7 +130,7 @@ __global__ void k_update_refs(Dev d) {
       d.refPos[i] = f4(pos, 0.0f);
       d.refRot[i] = f4q(rot);
       float4 ip = d.b.inpos[i];
-      ip.w = __uint_as_float(d.p.step);
+      ip.w = __uint_as_float(d.counters[C_STEP]);
       d.b.inpos[i] = ip;
     }
   }
@@ -497,7 +498,7 @@ __device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_
   uint32_t cnt = found.count;
   if (base + cnt > d.conCap) { atomicOr(&d.counters[C_OVERFLOW], 2u); cnt = 0; }
   V n = neg(sat.n);
-  d.mIds[m] = make_uint4(a, b, base, cnt | (d.p.step << 4));
+  d.mIds[m] = make_uint4(a, b, base, cnt | (d.counters[C_STEP] << 4));
   d.mGeo[m] = f4(n, sqrtf(posA.w * posB.w));
   if (cnt == 0) return;
   V basis[3];
