@@ -505,7 +505,8 @@ struct avbd_solver {
       if (anyDynamicLarge) k_find_fixed_large<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
     timed(K_NARROW, [&] {
-      k_narrowphase<<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
+      if (hullPairs) k_narrowphase<false><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
+      else k_narrowphase<true><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
       if (hullPairs) {
         k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
         k_hull_emit<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);

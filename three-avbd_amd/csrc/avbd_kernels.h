@@ -544,12 +544,14 @@ __device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_
   }
 }
 
-/** Narrowphase of every broadphase pair, one thread per pair; pairs with a hull on either side
- *  go to the hull list for k_narrow_hull (a wavefront per pair) when it is given. */
-/** narrowphase (wgsl-collision.ts:767): per candidate pair, kept points (contact reuse), a
- *  deferral to the hull kernel, or a fresh manifold. The slots of all three come from one atomic
- *  per wavefront each (waveAlloc) at a point every lane reaches: same-address atomics per lane
- *  serialise across the chip. Which slot a pair lands in is arbitrary either way. */
+/** narrowphase (wgsl-collision.ts:767) of every broadphase pair, one thread per pair: kept points
+ *  (contact reuse), a deferral to the hull kernel, or a fresh manifold. The slots of all three
+ *  come from one atomic per wavefront each (waveAlloc) at a point every lane reaches: same-address
+ *  atomics per lane serialise across the chip. Which slot a pair lands in is arbitrary either way.
+ *  INLINE_HULLS: hull pairs are collided here by one thread each (the solver built without the
+ *  hull list); otherwise they go to the list for k_narrow_hull (a wavefront per pair), and this
+ *  kernel carries only the box and sphere code and its private memory. */
+template <bool INLINE_HULLS>
 __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
   uint32_t count = d.counters[C_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
@@ -574,13 +576,14 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
       float shA = d.b.angvel[a].w, shB = d.b.angvel[b].w;
       sphereA = shA == SHAPE_SPHERE; sphereB = shB == SHAPE_SPHERE;
       bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
-      if ((hullA || hullB) && d.hullPairs) {
+      if ((hullA || hullB) && !INLINE_HULLS) {
         defer = true;
       } else {
         qA = q4(d.b.rot[a]); qB = q4(d.b.rot[b]);
         A = makeBox(xyz(posA), qA, xyz(d.b.size[a])); B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
         if (sphereA && sphereB) sphereSphere(A, B, sat, found);
-        else if (hullA || hullB) collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
+        else if (INLINE_HULLS && (hullA || hullB))
+          collideHullPair(d.hulls, A, qA, shA, B, qB, shB, sphereA, sphereB, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
         else if (sphereA || sphereB) {
           if (sphereA) {
             SphereHit h = sphereBox(A, B, qB);
