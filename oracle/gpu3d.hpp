@@ -878,7 +878,9 @@ template <class T> struct Model {
     if (count >= p.lanesRule[3]) return 8;
     return 16;
   }
-  /** accumulate(i, lane, lanes) (wgsl-solve.ts:436-501). */
+  /** accumulate(i, lane, lanes) (wgsl-solve.ts:436-501): the body's walk items (one per joint
+   *  entry and per contact point of each manifold entry, in adjacency order), lane taking items
+   *  lane, lane + lanes, ... in order. With one lane that is the reference's order. */
   Acc accumulate(uint32_t i, uint32_t lane, uint32_t lanes, T alpha) const {
     Acc acc; zero(acc);
     const Body<T> &bd = bodies[i];
@@ -894,15 +896,20 @@ template <class T> struct Model {
       acc.rAng[0] = I.x * dq.x; acc.rAng[1] = I.y * dq.y; acc.rAng[2] = I.z * dq.z;
     }
     const uint32_t J = (uint32_t)joints.size();
-    for (uint32_t e = adjStart[i] + lane; e < adjStart[i + 1]; e += lanes) {
+    uint32_t q = 0;  // the walk item's index
+    for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++) {
       uint32_t id = adjList[e];
-      if (id < J) { addJoint(acc, id, alpha, i); continue; }
+      if (id < J) {
+        if (q++ % lanes == lane) addJoint(acc, id, alpha, i);
+        continue;
+      }
       const ManifoldT<T> &mf = manifolds[id - J];
       bool isA = i == mf.a;
       PairBody A = pairBody(mf.a), B = pairBody(mf.b);
       V basis[3];
       uint32_t c = mf.first, cEnd = mf.first + (mf.countStep & 15u);
       for (; c < cEnd; c++) {
+        if (q++ % lanes != lane) continue;
         const Contact<T> &k = contacts[c];
         O::orthonormal(mf.n, basis);
         Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);

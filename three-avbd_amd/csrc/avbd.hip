@@ -169,7 +169,8 @@ struct avbd_solver {
   uint2 *adjKeys = nullptr;                          // and their sort keys
   int *adjOther = nullptr;                           // per sorted entry, the dynamic neighbour or -1
   uint32_t *prio = nullptr;                          // per body, its colouring priority
-  uint4 *adjIds = nullptr, *colorItems = nullptr;
+  uint4 *adjIds = nullptr, *colorItems = nullptr, *items = nullptr;
+  uint32_t *itemStart = nullptr, itemCap = 0;
   float4 *adjGeo = nullptr;
   uint32_t *colA = nullptr, *colB = nullptr, *colorStart = nullptr, *colorBodies = nullptr, *colorRank = nullptr, *hist = nullptr;
   uint32_t *fixedDev = nullptr;
@@ -369,6 +370,9 @@ struct avbd_solver {
     adjOther = regrow(adjOther, adjCap, ac, false);
     adjIds = regrow(adjIds, adjCap, ac, false);
     adjGeo = regrow(adjGeo, adjCap, ac, false);
+    uint32_t ic = itemsFor(nCon, jointCap);
+    items = regrow(items, itemCap, ic, false);
+    itemCap = ic;
     pairCap = nPairs; manCap = nMan; conCap = nCon; hashSize = hs; adjCap = ac;
     manifoldsValid = false;
     hipStreamSynchronize(stream);
@@ -398,7 +402,7 @@ struct avbd_solver {
     d.table = table; d.hashMask = hashSize - 1;
     d.adjCount = adjCount; d.adjStart = adjStart; d.adjFill = adjFill; d.adjList = adjList; d.adjCap = adjCap;
     d.adjTmp = adjTmp; d.adjOwner = adjOwner; d.adjKeys = adjKeys; d.adjOther = adjOther; d.prio = prio;
-    d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems;
+    d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems; d.itemStart = itemStart; d.items = items; d.itemCap = itemCap;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
     d.aux = aux;
@@ -524,6 +528,9 @@ struct avbd_solver {
     scan(adjCount, adjStart, n + 1, keep);
     k_fill_adj<<<strideBlocks(nj + manCap, 256), 256, 0, stream>>>(d);
     k_sort_adj<<<strideBlocks(adjCap, 256), 256, 0, stream>>>(d);
+    k_item_count<<<blocksFor(n + 1, 256), 256, 0, stream>>>(d);
+    scan(itemStart, itemStart, n + 1, keep);
+    k_item_fill<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     if (fixedColors) {
       hipMemcpyAsync(colA, fixedDev, n * 4, hipMemcpyDeviceToDevice, stream);
     } else {
@@ -798,6 +805,8 @@ struct avbd_solver {
     hipMemcpyAsync(aux + first, r.data(), 16ull * count, hipMemcpyHostToDevice, stream);
     hipStreamSynchronize(stream);
   }
+  /** The primal's walk items: one per contact point and joint on each side (both bodies). */
+  static uint32_t itemsFor(uint32_t contacts, uint32_t joints) { return (uint32_t)std::min<uint64_t>(2ull * contacts + 2ull * joints, 0xffffffffu); }
   /** Joint storage for `cap` joints (the reference's allocateJoints, doubling): records, info and
    *  the adjacency sized by it. */
   void growJoints(uint32_t cap) {
@@ -813,6 +822,9 @@ struct avbd_solver {
     adjOther = regrow(adjOther, adjCap, ac, false);
     adjIds = regrow(adjIds, adjCap, ac, false);
     adjGeo = regrow(adjGeo, adjCap, ac, false);
+    uint32_t ic = itemsFor(conCap, cap);
+    items = regrow(items, itemCap, ic, false);
+    itemCap = ic;
     adjCap = ac;
     jointCap = cap;
     adjacencyHeld = false;
@@ -1112,6 +1124,9 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->adjOther = s->alloc<int>(s->adjCap);
     s->adjIds = s->alloc<uint4>(s->adjCap);
     s->adjGeo = s->alloc<float4>(s->adjCap);
+    s->itemCap = avbd_solver::itemsFor(s->conCap, s->jointCap);
+    s->items = s->alloc<uint4>(s->itemCap);
+    s->itemStart = s->alloc<uint32_t>(N + 1);
     s->colA = s->alloc<uint32_t>(N);
     s->colB = s->alloc<uint32_t>(N);
     s->colorStart = s->alloc<uint32_t>(65);

@@ -58,9 +58,15 @@ struct Dev {
   uint32_t *prio;               // per body, its colouring priority (fixed at creation)
   uint32_t adjCap;
   // the solve's flattened views: per adjacency entry its manifold record (ids.x = 0xffffffff marks
-  // a joint, id in ids.y), per colour-bucket slot its body, adjacency range and whether it is enabled
+  // a joint, id in ids.y); per body its walk items (one per joint entry and per contact point of
+  // its manifold entries, adjacency order: contact or joint slot, entry, partner, bit 0 the body is
+  // A, bit 1 a joint) from itemStart; per colour-bucket slot its body, item range and whether it is
+  // enabled
   uint4 *adjIds;
   float4 *adjGeo;
+  uint32_t *itemStart;
+  uint4 *items;
+  uint32_t itemCap;
   uint4 *colorItems;
   // colouring
   uint32_t *colA, *colB, *colorStart, *colorBodies, *colorRank, *hist;
@@ -925,7 +931,39 @@ __global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
   uint32_t col = d.colA[b] & 0xffu;
   uint32_t slot = d.hist[col * d.colorGroups + blockIdx.x] + d.colorRank[b];
   d.colorBodies[slot] = b;
-  d.colorItems[slot] = make_uint4(b, d.adjStart[b], d.adjStart[b + 1], bodyEnabled(d, b) ? 1u : 0u);
+  d.colorItems[slot] = make_uint4(b, d.itemStart[b], d.itemStart[b + 1], bodyEnabled(d, b) ? 1u : 0u);
+}
+/** The primal's walk items of every body: counted (then scanned into itemStart) and filled in
+ *  adjacency order. Nothing when the substep keeps the last adjacency (C_TOPO bit 0). */
+__global__ void __launch_bounds__(256) k_item_count(Dev d) {
+  if (d.counters[C_TOPO] & 1u) return;
+  GRID_STRIDE(i, d.n + 1) {
+    uint32_t c = 0;
+    if (i < d.n)
+      for (uint32_t e = d.adjStart[i], e1 = d.adjStart[i + 1]; e < e1; e++) {
+        uint4 ids = d.adjIds[e];
+        c += ids.x == 0xffffffffu ? 1u : (ids.w & 15u);
+      }
+    d.itemStart[i] = c;
+  }
+}
+__global__ void __launch_bounds__(256) k_item_fill(Dev d) {
+  if (d.counters[C_TOPO] & 1u) return;
+  GRID_STRIDE(i, d.n) {
+    uint32_t o = d.itemStart[i];
+    for (uint32_t e = d.adjStart[i], e1 = d.adjStart[i + 1]; e < e1; e++) {
+      uint4 ids = d.adjIds[e];
+      if (ids.x == 0xffffffffu) {
+        if (o < d.itemCap) d.items[o] = make_uint4(ids.y, e, 0u, 2u);
+        o++;
+        continue;
+      }
+      bool isA = ids.x == i;
+      uint32_t partner = isA ? ids.y : ids.x, cnt = ids.w & 15u;
+      for (uint32_t k = 0; k < cnt; k++, o++)
+        if (o < d.itemCap) d.items[o] = make_uint4(ids.z + k, e, partner, isA ? 1u : 0u);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1070,20 +1108,20 @@ __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, fl
   e.F = F;
   return e;
 }
-/** Contacts of a manifold entry fetched per load batch in the primal walk. */
-#ifndef CONTACT_BATCH
-#define CONTACT_BATCH 4u
-#endif
-/** accumulate (wgsl-solve.ts:436-501): body i's system from adjacency entries lane, lane+lanes, ...
+/** accumulate (wgsl-solve.ts:436-501): body i's system from its walk items lane, lane+lanes, ...
+ *  (a joint entry, or one contact point of a manifold entry, in adjacency order). With one lane
+ *  this is the reference's order; with more, lane l sums its items in order and the lanes are
+ *  added by reduceLanes's tree, as the oracle does. A body's points spread over its lanes (not
+ *  its manifolds), so a lane evaluates about one point however the points group into manifolds.
  *  Lane 0 also returns the body's pose (pos, rot) for the update: only the body's own update
  *  writes it, so the value read here is the one the update starts from. */
-__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0, uint32_t end, uint32_t lane, uint32_t lanes, float alpha, Acc &acc,
+__device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t q0, uint32_t end, uint32_t lane, uint32_t lanes, float alpha, Acc &acc,
                                            float4 &pos, float4 &rot) {
   accZero(acc);
-  // this lane's first entry record loads beside lane 0's body reads
-  uint32_t e = e0 + lane;
-  uint4 ids = make_uint4(0u, 0u, 0u, 0u), next;
-  if (e < end) ids = d.adjIds[e];
+  // this lane's first item loads beside lane 0's body reads
+  uint32_t q = q0 + lane;
+  uint4 it = make_uint4(0u, 0u, 0u, 0u), next;
+  if (q < end) it = d.items[q];
   float dt = d.p.dt, dt2 = dt * dt;
   if (lane == 0) {
     float4 sz = d.b.size[i], mo = d.b.moment[i];
@@ -1098,51 +1136,28 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t e0
     V dq = qsub(q4(rot), q4(d.b.inrot[i]));
     acc.ra[0] = I.x * dq.x; acc.ra[1] = I.y * dq.y; acc.ra[2] = I.z * dq.z;
   }
-  // Entries [e0, end) of body i, this lane's every lanes-th: each carries its manifold record
-  // (k_sort_adj), so a contact's poses and points load right after the entry.
-  // The walk is a chain of dependent loads (entry -> poses and contacts), so each entry's loads go
-  // out together: the whole record, then geometry, the partner body and its first CONTACT_BATCH
-  // contacts (indexes clamped into the manifold's range); later contacts (a manifold holds up to
-  // 8) are fetched a batch at a time; the next entry's record loads beside them (its index clamped
-  // into the range). The body's own pose terms are the same for every entry: read once. The order
-  // of the sums is the walk's own.
+  // The walk is a chain of dependent loads (item -> contact, geometry, partner), so an item's loads
+  // go out together and the next item's record loads beside them (its index clamped into the
+  // range). The body's own pose terms are the same for every item: read once.
   PairBody self = pairBody(d, i);
-  for (; e < end; e += lanes, ids = next) {
-    next = d.adjIds[e + lanes < end ? e + lanes : end - 1];
-    asm volatile("" : "+v"(ids.x), "+v"(ids.y), "+v"(ids.z), "+v"(ids.w));  // the record whole at the branch
-    if (ids.x == 0xffffffffu) { addJoint(d, acc, ids.y, alpha, i); continue; }
-    uint32_t nc = ids.w & 15u;
-    if (nc == 0) continue;
-    Contact kc[CONTACT_BATCH];
-#pragma unroll
-    for (uint32_t j = 0; j < CONTACT_BATCH; j++) kc[j] = d.con[ids.z + (j < nc ? j : nc - 1)];
-    float4 geo = d.adjGeo[e];
-    bool isA = i == ids.x;
-    PairBody other = pairBody(d, isA ? ids.y : ids.x);
+  for (; q < end; q += lanes, it = next) {
+    next = d.items[q + lanes < end ? q + lanes : end - 1];
+    asm volatile("" : "+v"(it.x), "+v"(it.y), "+v"(it.z), "+v"(it.w));  // the record whole at the branch
+    if (it.w & 2u) { addJoint(d, acc, it.x, alpha, i); continue; }
+    Contact k = d.con[it.x];
+    float4 geo = d.adjGeo[it.y];
+    bool isA = (it.w & 1u) != 0u;
+    PairBody other = pairBody(d, it.z);
     PairBody A = self, B = other;  // by value: a reference picked between two locals would put them in memory
     if (!isA) { A = other; B = self; }
     V basis[3];
     orthonormal(xyz(geo), basis);
     float sg = isA ? 1.0f : -1.0f;
-    V l0 = mul(basis[0], sg), l1 = mul(basis[1], sg), l2 = mul(basis[2], sg);
-    auto batch = [&](uint32_t b0) {
-#pragma unroll
-      for (uint32_t j = 0; j < CONTACT_BATCH; j++) {
-        if (b0 + j >= nc) break;
-        const Contact &k = kc[j];
-        Eval ev = evalContact<true>(k, basis, geo.w, A, B, alpha, isA);  // ev.J[r] = r x l_r·sg
-        if (d.bits) { uint32_t c = ids.z + b0 + j; if (isA) d.bits[c].x = evalBits(ev); else d.bits[c].y = evalBits(ev); }
-        addRow(acc, l0, ev.J[0], k.pen.x, ev.F.x);
-        addRow(acc, l1, ev.J[1], k.pen.y, ev.F.y);
-        addRow(acc, l2, ev.J[2], k.pen.z, ev.F.z);
-      }
-    };
-    batch(0);
-    for (uint32_t b0 = CONTACT_BATCH; b0 < nc; b0 += CONTACT_BATCH) {
-#pragma unroll
-      for (uint32_t j = 0; j < CONTACT_BATCH; j++) kc[j] = d.con[ids.z + (b0 + j < nc ? b0 + j : nc - 1)];
-      batch(b0);
-    }
+    Eval ev = evalContact<true>(k, basis, geo.w, A, B, alpha, isA);  // ev.J[r] = r x l_r·sg
+    if (d.bits) { if (isA) d.bits[it.x].x = evalBits(ev); else d.bits[it.x].y = evalBits(ev); }
+    addRow(acc, mul(basis[0], sg), ev.J[0], k.pen.x, ev.F.x);
+    addRow(acc, mul(basis[1], sg), ev.J[1], k.pen.y, ev.F.y);
+    addRow(acc, mul(basis[2], sg), ev.J[2], k.pen.z, ev.F.z);
   }
 }
 /** finishBody (wgsl-solve.ts:504-558): LDLᵀ in registers, pivots floored at 1e-5 of the diagonal;
