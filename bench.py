@@ -616,7 +616,8 @@ def main():
     out = bench_step(args, args.workload)
     if args.workload == "large-100k" and not args.no_extras:
         b = bench_batched(args, steps=20)
-        out["batched"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config", "looped_plain_step")}
+        out["batched"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config", "looped_plain_step", "graphs",
+                                            "plain_launches")}
         a = bench_adjoint(args, worlds=args.adjoint_worlds, steps=args.adjoint_steps, light=True)
         out["adjoint"] = {k: a[k] for k in ("metric", "value", "unit", "config", "adjoint", "gpu_worlds_steps_per_s")}
         out["adjoint"]["higher_is_better"] = False
