@@ -395,6 +395,108 @@ int avbd_estimate_pairs(const float *bodies, uint32_t body_count, uint64_t *pair
 int avbd_plan_capacity(const float *bodies, uint32_t body_count, const int32_t *joint_info, uint32_t joint_count,
                        const avbd_capacity *least, uint32_t least_colors, avbd_capacity *out, uint32_t *color_cap);
 
+/* ---- SDF collision backend: SdfCollisionBridge (src/avbd3d/gpu/sdf-bridge.ts) ---------------
+ * The reference's other producer of the manifold/contact ABI. Installed with
+ * avbd_set_collision_backend (setCollisionBackend, solver.ts:2078), it replaces ALL collision
+ * detection: each step prepares every body's signed distance field at the body's pose, takes the
+ * candidate pairs (given by the caller, or generated on the GPU by a spatial grid), runs Algorithm
+ * 1 per pair (cached points advected and re-optimised, fresh samples from the field seeds, first-wins
+ * dedup within the tolerance, stable rank order) and exports one manifold per active point into the
+ * solver's contact storage, warm-started from the solved contact the point's previous self was
+ * exported to (its lambda rotated into the new basis, its penalties, and its stick anchors unless
+ * the surface rolls). Solver masses, inertia and poses stay authoritative: a field only gives
+ * collision geometry. Not with fixed storage, batched worlds or the adjoint. */
+typedef struct avbd_sdf avbd_sdf;
+/* Field kinds (packSdfAtlas kinds, sdf-layout.ts:29). */
+#define AVBD_SDF_BOX 0u         /* params: half extents xyz */
+#define AVBD_SDF_SPHERE 1u      /* radius */
+#define AVBD_SDF_CAPSULE 2u     /* radius, half length (along y) */
+#define AVBD_SDF_CYLINDER 3u    /* radius, half height (along y) */
+#define AVBD_SDF_CONE 4u        /* radius, half height */
+#define AVBD_SDF_TORUS 5u       /* major radius, minor radius (about y) */
+#define AVBD_SDF_WARP 6u        /* half extents xyz, k; params[4] = type (0 bend, 1 twist, 2 displace) */
+#define AVBD_SDF_VOXEL 7u       /* dims, origin, cell, grid (4 floats per node: phi, gradient xyz; x fastest) */
+#define AVBD_SDF_SPIKE_PLATE 8u /* half extents xyz, count; spike radius, min height, max height, 0 */
+#define AVBD_SDF_SPIKY_WHEEL 9u /* hub radius, hub half width, spike radius, spike half height; spikes */
+/* One field (the reference's Field: AnalyticField / VoxelField, ref/sdf/shapes.ts, voxel.ts), in
+ * its local frame: bounds, centre of mass (field-local; a body's position is its centre of mass),
+ * the kind's parameters (the atlas header words 20..27), the curvature seeds (finalizeSdf's,
+ * field-local xyz) and, for meshVertexContacts, the mesh's unique surface vertices. */
+typedef struct avbd_sdf_field {
+  uint32_t kind;
+  uint32_t convex;          /* 1: convex (box, sphere, capsule, cylinder, cone) */
+  float bounds_min[3], bounds_max[3];
+  float center_of_mass[3];
+  float params[8];
+  uint32_t dims[3];         /* voxel only */
+  float origin[3];
+  float cell;
+  const float *grid;        /* voxel: 4 * dims[0] * dims[1] * dims[2] floats */
+  const float *seeds;       /* 3 per seed */
+  uint32_t seed_count;
+  const float *supports;    /* 3 per vertex */
+  uint32_t support_count;
+} avbd_sdf_field;
+/* CollisionParams (ref/sdf/manifold.ts:13-28); defaults from avbd_sdf_default_params. */
+typedef struct avbd_sdf_params {
+  uint32_t optimizer;          /* 0 Nelder-Mead, 1 deep-cut ellipsoid, 2 stochastic gradient descent */
+  uint32_t cache_size;         /* cached points re-optimised per pair (4; <= points - 1) */
+  uint32_t random_samples;     /* fresh samples per pair and step (1; cache + samples <= points) */
+  uint32_t warm_start;         /* 1: keep points across steps */
+  uint32_t curvature_sampling; /* 1: fresh samples prefer the fields' curvature seeds */
+  uint32_t skip_convex_samples;/* 1: no fresh sample when a convex pair's cache is full and active */
+  float epsilon;               /* softmax smoothing (0.1, a length squared) */
+  float alpha;                 /* repulsion between points (0.01) */
+  float replacement_threshold; /* rank bonus of a cached point (0.01) */
+  float tolerance;             /* optimiser and dedup tolerance (0.01) */
+  uint32_t max_iterations;     /* 50, 1..512 */
+  float step_size;             /* gradient descent only (0.1) */
+  float margin;                /* activation distance (0.02) */
+  float allowed_penetration;   /* resting gap (0.01) */
+  uint32_t signed_separation;  /* signed projection against static boxes */
+  uint32_t mesh_vertex_contacts;    /* exact vertex witnesses (fields with supports) */
+  uint32_t refresh_deeper_contacts; /* drop a sticking anchor its relocated point is deeper than */
+  uint32_t sgd_gradient_stop_only;  /* gradient descent stops on the gradient or the limit only */
+} avbd_sdf_params;
+void avbd_sdf_default_params(avbd_sdf_params *out);
+/* SdfCollisionBridge.create / createAutomatic (sdf-bridge.ts:48-82) for the solver's bodies, which
+ * must all exist (one field per body slot: body_field[i] indexes `fields`; body_scale NULL: 1;
+ * body_rotation, 4 floats per body xyzw, NULL: none, turns the field in the body frame).
+ * automatic = 0: explicit candidates (avbd_sdf_set_pairs), at most max_pairs (0: 1). automatic = 1:
+ * the GPU spatial grid generates them each step, at most max_pairs (0: min(n(n-1)/2, max(64, 8n)))
+ * with cells of cell_size (0: the median field bounds diagonal); past the capacity they are dropped
+ * and counted (overflow bit 1). points: points per pair result, 2..20 (0: 5). The solver's contact
+ * and manifold storage is grown to max_pairs x points (automatic), or must hold it (explicit). */
+int avbd_sdf_create(avbd_solver *s, const avbd_sdf_field *fields, uint32_t field_count, const uint32_t *body_field,
+                    const float *body_scale, const float *body_rotation, const avbd_sdf_params *params, int automatic,
+                    uint32_t max_pairs, uint32_t points, float cell_size, avbd_sdf **out);
+/* setPairs (sdf-bridge.ts:133): explicit candidates, 3 words each (a, b, key) with a > b, unique
+ * pairs and keys; a changed key starts the pair's history afresh. */
+int avbd_sdf_set_pairs(avbd_sdf *g, const uint32_t *pairs, uint32_t count);
+/* bridge.params (mutable between steps; validated at the next step, as prepare does). */
+int avbd_sdf_set_params(avbd_sdf *g, const avbd_sdf_params *params);
+/* setCollisionBackend (solver.ts:2078-2091): installs the bridge (NULL: the built-in collision
+ * again). The solver then owns it: a replaced bridge and the solver's own end destroy it. The next
+ * step detects afresh; a solver with fixed storage, batched worlds or pending lifecycle bits
+ * refuses. Graph replays are off while a bridge is installed. */
+int avbd_set_collision_backend(avbd_solver *s, avbd_sdf *g);
+/* readResults (sdf-bridge.ts:216): the last step's pair results, (8 + 28 points) words each:
+ * a, b, key, point count, queries, iterations, id counter, rolling; then per point x xyz, depth,
+ * normal xyz, separation, anchor A xyz, softmax, anchor B xyz, rank, active, cached, id, exported
+ * contact + 1, ellipsoid e00 e01 e02 e11 e12 e22 0 0. Slots of no pair have a = 0xffffffff. *count
+ * receives the slots (explicit: the pairs; automatic: the capacity). */
+int avbd_sdf_read_results(avbd_sdf *g, uint32_t *out, uint32_t cap, uint32_t *count);
+/* readBroadphaseStats (sdf-bridge.ts:217): pairs found, candidates tested, large bodies, the pair
+ * capacity, cell size (automatic only). */
+int avbd_sdf_read_stats(avbd_sdf *g, uint32_t *pairs, uint32_t *tested, uint32_t *large, uint32_t *capacity, float *cell_size);
+/* grown (sdf-bridge.ts:183-201), in place: the automatic pair capacity raised to max(required,
+ * 2 x capacity), the pair results kept (history survives), the solver's contact storage grown. */
+int avbd_sdf_grow(avbd_sdf *g, uint32_t required);
+/* destroy (sdf-bridge.ts:219); an installed bridge is detached first. A bridge lives no longer
+ * than its solver: avbd_destroy frees every bridge made for it (do not destroy those again). */
+void avbd_sdf_destroy(avbd_sdf *g);
+const char *avbd_sdf_last_error(const avbd_sdf *g);
+
 #ifdef __cplusplus
 }
 #endif

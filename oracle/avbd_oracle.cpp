@@ -444,3 +444,245 @@ double ofrozen_check_helpers(const double *q, const double *b, const double *w, 
   return worst;
 }
 }  // extern "C"
+
+// ---- SDF collision backend (sdf3d.hpp) ----
+#include "sdf3d.hpp"
+namespace {
+/** A field as the tests describe it (the same field order as the library's avbd_sdf_field). */
+struct OField {
+  uint32_t kind, convex;
+  float lo[3], hi[3], com[3], prm[8];
+  uint32_t dims[3];
+  float origin[3], cell;
+  const float *grid, *seeds;
+  uint32_t seedCount;
+  const float *supports;
+  uint32_t supportCount;
+};
+/** CollisionParams (ref/sdf/manifold.ts:13-28), the library's avbd_sdf_params field order. */
+struct OPrm {
+  uint32_t optimizer, cacheSize, randomSamples, warmStart, curvature, skipConvex;
+  float epsilon, alpha, replacement, tol;
+  uint32_t maxIt;
+  float stepSize, margin, allowedPen;
+  uint32_t signedSep, meshVertex, refreshDeeper, sgdStopOnly;
+};
+sdfo::Field toField(const OField &o) {
+  sdfo::Field f;
+  f.kind = o.kind; f.convex = o.convex;
+  f.lo = sdfo::v3(o.lo[0], o.lo[1], o.lo[2]); f.hi = sdfo::v3(o.hi[0], o.hi[1], o.hi[2]);
+  f.com = sdfo::v3(o.com[0], o.com[1], o.com[2]);
+  for (int k = 0; k < 8; k++) f.prm[k] = o.prm[k];
+  f.warpType = (uint32_t)o.prm[4];
+  for (int k = 0; k < 3; k++) f.dims[k] = o.dims[k];
+  f.origin = sdfo::v3(o.origin[0], o.origin[1], o.origin[2]); f.cell = o.cell;
+  if (o.kind == sdfo::K_VOXEL && o.grid) f.grid.assign(o.grid, o.grid + 4ull * o.dims[0] * o.dims[1] * o.dims[2]);
+  for (uint32_t i = 0; i < o.seedCount; i++) f.seeds.push_back(sdfo::v3(o.seeds[3 * i], o.seeds[3 * i + 1], o.seeds[3 * i + 2]));
+  for (uint32_t i = 0; i < o.supportCount; i++) f.supports.push_back(sdfo::v3(o.supports[3 * i], o.supports[3 * i + 1], o.supports[3 * i + 2]));
+  return f;
+}
+sdfo::Prm toPrm(const OPrm &o, uint32_t maxPoints) {
+  sdfo::Prm p;
+  p.cacheSize = o.cacheSize; p.randomSamples = o.randomSamples; p.epsilon = o.epsilon; p.alpha = o.alpha; p.tol = o.tol;
+  p.margin = o.margin; p.replacement = o.replacement; p.signedSep = o.signedSep != 0; p.meshVertex = o.meshVertex != 0;
+  p.maxIt = o.maxIt; p.warmStart = o.warmStart != 0; p.curvature = o.curvature != 0; p.skipConvex = o.skipConvex != 0;
+  p.optimizer = o.optimizer; p.stepSize = o.optimizer == 2 ? o.stepSize : 0.0f; p.sgdStopOnly = o.optimizer == 2 && o.sgdStopOnly != 0;
+  p.maxPoints = maxPoints;
+  return p;
+}
+/** Result records in the library's avbd_sdf_read_results layout: identity (a, b, key, count),
+ *  stats (queries, iterations, id counter, rolling), then maxPoints points of 28 words. */
+void putResult(const sdfo::Result &r, uint32_t maxPoints, uint32_t *o) {
+  std::memset(o, 0, 4ull * (8 + 28 * maxPoints));
+  o[0] = r.a; o[1] = r.b; o[2] = r.key; o[3] = r.count; o[4] = r.queries; o[5] = r.iters; o[6] = r.idc; o[7] = r.rolling;
+  for (uint32_t i = 0; i < r.count; i++) {
+    const sdfo::Point &p = r.pts[i];
+    float *f = (float *)(o + 8 + 28 * i);
+    uint32_t *u = o + 8 + 28 * i;
+    f[0] = p.x.x; f[1] = p.x.y; f[2] = p.x.z; f[3] = p.depth;
+    f[4] = p.n.x; f[5] = p.n.y; f[6] = p.n.z; f[7] = p.sep;
+    f[8] = p.rA.x; f[9] = p.rA.y; f[10] = p.rA.z; f[11] = p.soft;
+    f[12] = p.rB.x; f[13] = p.rB.y; f[14] = p.rB.z; f[15] = p.rank;
+    u[16] = p.active; u[17] = p.cached; u[18] = p.id; u[19] = p.exported;
+    for (int k = 0; k < 4; k++) { f[20 + k] = p.eA[k]; f[24 + k] = p.eB[k]; }
+  }
+}
+void getPoint(const float *f, sdfo::Point &p) {
+  const uint32_t *u = (const uint32_t *)f;
+  p.x = sdfo::v3(f[0], f[1], f[2]); p.depth = f[3];
+  p.n = sdfo::v3(f[4], f[5], f[6]); p.sep = f[7];
+  p.rA = sdfo::v3(f[8], f[9], f[10]); p.soft = f[11];
+  p.rB = sdfo::v3(f[12], f[13], f[14]); p.rank = f[15];
+  p.active = u[16]; p.cached = u[17]; p.id = u[18]; p.exported = u[19];
+  for (int k = 0; k < 4; k++) { p.eA[k] = f[20 + k]; p.eB[k] = f[24 + k]; }
+}
+/** A standalone body for the pinning harness: the CPU reference's SdfBody (body.ts), its
+ *  orientation normalised, its bounds the field's corners through the pose. */
+sdfo::SBody cpuBody(const sdfo::Field *f, const float *pose) {
+  sdfo::SBody b;
+  b.f = f;
+  b.pos = sdfo::v3(pose[0], pose[1], pose[2]);
+  float qx = pose[3], qy = pose[4], qz = pose[5], qw = pose[6];
+  float norm = std::sqrt(((qx * qx + qy * qy) + qz * qz) + qw * qw);
+  b.rot = sdfo::q4(qx / norm, qy / norm, qz / norm, qw / norm);
+  b.scale = pose[7];
+  b.fixed = pose[8] != 0.0f;
+  b.vel = sdfo::v3(pose[9], pose[10], pose[11]);
+  b.omega = sdfo::v3(pose[12], pose[13], pose[14]);
+  b.anchor = sdfo::q4(0, 0, 0, 1);
+  sdfo::bodyBounds(b);
+  return b;
+}
+struct BridgeHandle { gpuo::Model<float> *model; sdfo::Bridge bridge; };
+}  // namespace
+extern "C" {
+/** Installs the SDF bridge on an f32 model (setCollisionBackend): every body gets a field
+ *  (body_field), a scale and a field rotation (4 per body, xyzw). */
+void *osdf_create(void *h, const OField *fields, uint32_t nf, const uint32_t *bodyField, const float *bodyScale, const float *bodyRot,
+                  const OPrm *prm, int automatic, uint32_t maxPairs, uint32_t maxPoints, float cellSize) {
+  ModelHandle *m = (ModelHandle *)h;
+  if (m->prec != 32) return nullptr;
+  BridgeHandle *b = new BridgeHandle();
+  b->model = m->f;
+  for (uint32_t i = 0; i < nf; i++) b->bridge.fields.push_back(toField(fields[i]));
+  size_t n = m->f->bodies.size();
+  for (size_t i = 0; i < n; i++) {
+    sdfo::BodyConfig c;
+    c.field = bodyField[i];
+    c.scale = bodyScale ? bodyScale[i] : 1.0f;
+    c.rot = bodyRot ? sdfo::q4(bodyRot[4 * i], bodyRot[4 * i + 1], bodyRot[4 * i + 2], bodyRot[4 * i + 3]) : sdfo::q4(0, 0, 0, 1);
+    b->bridge.cfg.push_back(c);
+  }
+  b->bridge.prm = toPrm(*prm, maxPoints);
+  b->bridge.allowedPen = prm->allowedPen;
+  b->bridge.refreshDeeper = prm->refreshDeeper != 0;
+  b->bridge.automatic = automatic != 0;
+  b->bridge.maxPairs = maxPairs;
+  b->bridge.cellSize = cellSize;
+  sdfo::Bridge *br = &b->bridge;
+  m->f->collisionHook = [br](gpuo::Model<float> &M) { br->encode(M); };
+  m->f->manifolds.clear();  // setCollisionBackend: no later step warm-starts from the last contacts
+  m->f->contacts.clear();
+  return b;
+}
+void osdf_destroy(void *b) {
+  BridgeHandle *x = (BridgeHandle *)b;
+  x->model->collisionHook = nullptr;
+  x->model->manifolds.clear();
+  x->model->contacts.clear();
+  delete x;
+}
+void osdf_set_pairs(void *b, const uint32_t *pairs, uint32_t n) {
+  BridgeHandle *x = (BridgeHandle *)b;
+  x->bridge.pairs.clear();
+  for (uint32_t i = 0; i < n; i++) x->bridge.pairs.push_back(sdfo::PairSpec{pairs[3 * i], pairs[3 * i + 1], pairs[3 * i + 2]});
+}
+void osdf_set_params(void *b, const OPrm *prm) {
+  BridgeHandle *x = (BridgeHandle *)b;
+  uint32_t mp = x->bridge.prm.maxPoints;
+  x->bridge.prm = toPrm(*prm, mp);
+  x->bridge.allowedPen = prm->allowedPen;
+  x->bridge.refreshDeeper = prm->refreshDeeper != 0;
+}
+/** The last step's results sorted by (a, b) into out (room for cap records); returns how many,
+ *  and the candidates past the capacity in *overflow. */
+uint32_t osdf_results(void *b, uint32_t *out, uint32_t cap, uint32_t *overflow) {
+  BridgeHandle *x = (BridgeHandle *)b;
+  uint32_t mp = x->bridge.prm.maxPoints, n = (uint32_t)x->bridge.last.size();
+  for (uint32_t i = 0; i < n && i < cap; i++) putResult(x->bridge.last[i], mp, out + (size_t)i * (8 + 28 * mp));
+  if (overflow) *overflow = x->bridge.overflowPairs;
+  return n;
+}
+/** The prepared bodies of the last step (28 floats each: the SdfBody record of wgsl-sdf-bridge.ts). */
+void osdf_bodies(void *b, float *out) {
+  BridgeHandle *x = (BridgeHandle *)b;
+  for (size_t i = 0; i < x->bridge.sb.size(); i++) {
+    const sdfo::SBody &s = x->bridge.sb[i];
+    float *o = out + 28 * i;
+    float v[28] = {s.pos.x, s.pos.y, s.pos.z, s.scale, s.rot.x, s.rot.y, s.rot.z, s.rot.w, s.vel.x, s.vel.y, s.vel.z, s.fixed ? 1.0f : 0.0f,
+                   s.omega.x, s.omega.y, s.omega.z, 0, s.lo.x, s.lo.y, s.lo.z, 0, s.hi.x, s.hi.y, s.hi.z, 0, s.anchor.x, s.anchor.y, s.anchor.z, s.anchor.w};
+    std::memcpy(o, v, sizeof v);
+  }
+}
+
+// pinning harness: the pieces of Algorithm 1 on their own
+/** phi, gradient xyz per point of a field. */
+void osdf_sample(const OField *fo, const float *p, uint32_t n, float *out) {
+  sdfo::Field f = toField(*fo);
+  for (uint32_t i = 0; i < n; i++) {
+    sdfo::Sample s = sdfo::localField(f, sdfo::v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+    out[4 * i] = s.phi; out[4 * i + 1] = s.g.x; out[4 * i + 2] = s.g.y; out[4 * i + 3] = s.g.z;
+  }
+}
+/** which: 0 sin, 1 cos, 2 atan2(x[2i], x[2i+1]). */
+void osdf_trig(int which, const float *x, uint32_t n, float *out) {
+  for (uint32_t i = 0; i < n; i++)
+    out[i] = which == 0 ? sdfo::ssin(x[i]) : (which == 1 ? sdfo::scos(x[i]) : sdfo::satan2(x[2 * i], x[2 * i + 1]));
+}
+/** One optimizer run between body A (pose 15 floats: pos, rot, scale, static, vel, omega) and body B
+ *  over the domain lo/hi, with repulsion points rep: kind 0 Nelder-Mead (from seed, initial step
+ *  `step`), 1 the ellipsoid method (E row-major 9 floats in, final out), 2 gradient descent (PCG state
+ *  in and out), 3 the objective and its gradient at the seed. out: x xyz, value/gradient,
+ *  iterations, queries. */
+void osdf_optimize(int kind, const OField *fa, const float *poseA, const OField *fb, const float *poseB, const float *domain,
+                   const float *rep, uint32_t nrep, const float *seed, const float *step, const OPrm *prm, float *E, uint64_t *pcg,
+                   float *out, uint32_t *counts) {
+  sdfo::Field FA = toField(*fa), FB = toField(*fb);
+  sdfo::SBody A = cpuBody(&FA, poseA), B = cpuBody(&FB, poseB);
+  sdfo::Prm P = toPrm(*prm, 20);
+  sdfo::PcgRng rng;
+  if (pcg) rng.state = *pcg;
+  sdfo::PairUpdate<sdfo::PcgRng, true> u(P, A, B, rng);
+  u.lo = sdfo::v3(domain[0], domain[1], domain[2]);
+  u.hi = sdfo::v3(domain[3], domain[4], domain[5]);
+  for (uint32_t i = 0; i < nrep; i++) { u.work.pts[i].x = sdfo::v3(rep[3 * i], rep[3 * i + 1], rep[3 * i + 2]); }
+  u.work.count = nrep;
+  sdfo::V s = sdfo::v3(seed[0], seed[1], seed[2]), x{};
+  if (kind == 0) x = u.nelderMead(s, sdfo::v3(step[0], step[1], step[2]));
+  else if (kind == 1) {
+    float M[3][3];
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) M[r][c] = E[3 * r + c];
+    x = u.ellipsoidMethod(s, M);
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) E[3 * r + c] = u.E[r][c];
+  } else if (kind == 2) x = u.gradientDescent(s);
+  else {
+    sdfo::V g; float value;
+    u.objectiveGradient(s, g, value);
+    out[0] = value; out[1] = g.x; out[2] = g.y; out[3] = g.z;
+    counts[0] = 0; counts[1] = u.work.queries;
+    return;
+  }
+  if (pcg) *pcg = rng.state;
+  out[0] = x.x; out[1] = x.y; out[2] = x.z;
+  out[3] = kind == 0 ? u.objective(x) : 0.0f;
+  counts[0] = u.work.iters;
+  counts[1] = u.work.queries - (kind == 0 ? 1u : 0u);
+}
+/** The CPU reference's manifold update (manifold.ts update) on one pair, with its PCG stream:
+ *  cache (ncache points of 28 floats, the result point layout) in, the updated points out (room
+ *  for 20); stats: pairs, contacts, queries, iterations; returns the point count. */
+uint32_t osdf_manifold_cpu(const OField *fa, const float *poseA, const OField *fb, const float *poseB, const OPrm *prm, float dt,
+                           const float *cache, uint32_t ncache, uint64_t *pcg, float *out, uint32_t *stats) {
+  sdfo::Field FA = toField(*fa), FB = toField(*fb);
+  sdfo::SBody A = cpuBody(&FA, poseA), B = cpuBody(&FB, poseB);
+  sdfo::Prm P = toPrm(*prm, 20);
+  P.dt = dt;
+  sdfo::PcgRng rng;
+  rng.state = *pcg;
+  sdfo::Result prev;
+  prev.pts.resize(20);
+  prev.count = ncache;
+  for (uint32_t i = 0; i < ncache; i++) getPoint(cache + 28 * i, prev.pts[i]);
+  sdfo::PairUpdate<sdfo::PcgRng, true> u(P, A, B, rng);
+  u.run(0, 0, 0, P.warmStart && ncache ? &prev : nullptr, 0);
+  bool valid = !(u.lo.x > u.hi.x || u.lo.y > u.hi.y || u.lo.z > u.hi.z);
+  *pcg = rng.state;
+  uint32_t active = 0;
+  uint32_t buf[8 + 28 * 20];
+  putResult(u.work, 20, buf);
+  std::memcpy(out, buf + 8, 4ull * 28 * u.work.count);
+  for (uint32_t i = 0; i < u.work.count; i++) active += u.work.pts[i].active;
+  stats[0] = valid ? 1u : 0u; stats[1] = active; stats[2] = u.work.queries; stats[3] = u.work.iters;
+  return u.work.count;
+}
+}  // extern "C"

@@ -19,12 +19,13 @@
 #include <vector>
 #include <algorithm>
 #include <unordered_map>
+#include <functional>
 #include <omp.h>
 
 namespace gpuo {
 
 // layout.ts / avbd2d layout.ts constants
-static const uint32_t STICK_BIT = 0x80000000u, SPHERE_FEATURE = 3u << 24;
+static const uint32_t STICK_BIT = 0x80000000u, SPHERE_FEATURE = 3u << 24, SDF_ROLLING_BIT = 0x20000000u;
 static const uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
                       FLAG_MASS_PENALTY = 16, FLAG_REUSE_MANIFOLDS = 64;
 static const int SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_HULL = 3;
@@ -588,10 +589,18 @@ template <class T> struct Model {
     o.man.push_back(m);
   }
 
+  /** An installed collision backend (GpuCollisionBackend3D, solver.ts:493-502): replaces the
+   *  refs, broadphase and narrowphase; it fills pairs, manifolds and contacts itself. */
+  std::function<void(Model &)> collisionHook;
   /** beginFrame + updateRefs + (hashInsert, broadphase, narrowphase) or reuseManifolds. */
   void collision(bool reuseManifolds) {
     prevManifolds.swap(manifolds); prevContacts.swap(contacts);
     manifolds.clear(); contacts.clear(); contactSrc.clear();
+    if (collisionHook) {
+      pairs.clear();
+      collisionHook(*this);
+      return;
+    }
     if (p.flags & FLAG_REUSE_CONTACTS) {  // updateRefs (wgsl-collision.ts:34-48)
 #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
       for (size_t i = 0; i < bodies.size(); i++) {
@@ -848,7 +857,7 @@ template <class T> struct Model {
   struct Eval { V rAW, rBW, C, F; T frictionScale, bounds; uint8_t bits; };
   Eval evalContact(const Contact<T> &k, const V basis[3], T friction, const PairBody &A, const PairBody &B, T alpha) const {
     Q rotA = A.rot, rotB = B.rot;
-    if ((k.key & ~STICK_BIT) == SPHERE_FEATURE) { rotA = bodies[A.index].irot; rotB = bodies[B.index].irot; }
+    if ((k.key & ~STICK_BIT) == SPHERE_FEATURE || (k.key & SDF_ROLLING_BIT) != 0u) { rotA = bodies[A.index].irot; rotB = bodies[B.index].irot; }
     Eval e;
     e.rAW = O::qrotate(rotA, k.rA);
     e.rBW = O::qrotate(rotB, k.rB);

@@ -549,3 +549,206 @@ def reset_pass(records, indices):
         r[i, 32:36] = 0.0
         r[i, 36:39] = 0.0
     return r
+
+
+# ---- SDF collision backend (oracle/sdf3d.hpp) -------------------------------------------------
+class OField(ctypes.Structure):
+    """A field as the oracle takes it (kind, convex, bounds, centre of mass, the kind's 8
+    parameters, voxel grid, seeds, mesh supports)."""
+    _fields_ = [("kind", ctypes.c_uint32), ("convex", ctypes.c_uint32), ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+                ("com", ctypes.c_float * 3), ("prm", ctypes.c_float * 8), ("dims", ctypes.c_uint32 * 3),
+                ("origin", ctypes.c_float * 3), ("cell", ctypes.c_float), ("grid", ctypes.c_void_p), ("seeds", ctypes.c_void_p),
+                ("seedCount", ctypes.c_uint32), ("supports", ctypes.c_void_p), ("supportCount", ctypes.c_uint32)]
+
+
+class OPrm(ctypes.Structure):
+    """CollisionParams (ref/sdf/manifold.ts:13-28)."""
+    _fields_ = [("optimizer", ctypes.c_uint32), ("cacheSize", ctypes.c_uint32), ("randomSamples", ctypes.c_uint32),
+                ("warmStart", ctypes.c_uint32), ("curvature", ctypes.c_uint32), ("skipConvex", ctypes.c_uint32),
+                ("epsilon", ctypes.c_float), ("alpha", ctypes.c_float), ("replacement", ctypes.c_float), ("tol", ctypes.c_float),
+                ("maxIt", ctypes.c_uint32), ("stepSize", ctypes.c_float), ("margin", ctypes.c_float),
+                ("allowedPen", ctypes.c_float), ("signedSep", ctypes.c_uint32), ("meshVertex", ctypes.c_uint32),
+                ("refreshDeeper", ctypes.c_uint32), ("sgdStopOnly", ctypes.c_uint32)]
+
+
+_OPRM = {"optimizer": "optimizer", "cacheSize": "cacheSize", "randomSamples": "randomSamples", "warmStart": "warmStart",
+         "curvatureSampling": "curvature", "skipConvexSamples": "skipConvex", "epsilon": "epsilon", "alpha": "alpha",
+         "replacementThreshold": "replacement", "tolerance": "tol", "maxIterations": "maxIt", "stepSize": "stepSize",
+         "margin": "margin", "allowedPenetration": "allowedPen", "signedSeparation": "signedSep",
+         "meshVertexContacts": "meshVertex", "refreshDeeperContacts": "refreshDeeper", "sgdGradientStopOnly": "sgdStopOnly"}
+
+
+def sdf_params(p):
+    """OPrm from a dict with the reference's CollisionParams names (missing ones: the defaults)."""
+    f = np.float32
+    d = {"optimizer": 0, "cacheSize": 4, "randomSamples": 1, "warmStart": True, "curvatureSampling": True,
+         "skipConvexSamples": True, "epsilon": f(0.1), "alpha": f(0.01), "replacementThreshold": f(0.01),
+         "tolerance": f(0.01), "maxIterations": 50, "stepSize": f(0.1), "margin": f(0.02), "allowedPenetration": f(0.01),
+         "signedSeparation": False, "meshVertexContacts": False, "refreshDeeperContacts": False, "sgdGradientStopOnly": False}
+    d.update(p or {})
+    o = OPrm()
+    for k, name in _OPRM.items():
+        cur = getattr(o, name)
+        setattr(o, name, int(d[k]) if isinstance(cur, int) else float(d[k]))
+    return o
+
+
+def sdf_field(d, keep):
+    """OField from a field description dict: kind, convex, lo, hi, com, params (8), and for a voxel
+    field dims, origin, cell, grid; seeds and supports (n x 3). `keep` holds the arrays."""
+    o = OField()
+    o.kind, o.convex = int(d["kind"]), int(d.get("convex", 0))
+    z3 = np.zeros(3, np.float32)
+    for i in range(3):
+        o.lo[i] = d.get("lo", z3)[i]; o.hi[i] = d.get("hi", z3)[i]; o.com[i] = d.get("com", z3)[i]
+        o.dims[i] = d.get("dims", (0, 0, 0))[i]; o.origin[i] = d.get("origin", z3)[i]
+    prm = np.zeros(8, np.float32)
+    prm[:len(d.get("params", []))] = d.get("params", [])
+    for i in range(8):
+        o.prm[i] = prm[i]
+    o.cell = d.get("cell", 0.0)
+    if d.get("grid") is not None:
+        g = np.ascontiguousarray(d["grid"], np.float32)
+        keep.append(g)
+        o.grid = g.ctypes.data
+    for name, cname, count in (("seeds", "seeds", "seedCount"), ("supports", "supports", "supportCount")):
+        a = np.ascontiguousarray(d.get(name, np.zeros((0, 3))), np.float32).reshape(-1, 3)
+        keep.append(a)
+        setattr(o, cname, a.ctypes.data if len(a) else None)
+        setattr(o, count, len(a))
+    return o
+
+
+def sdf_sample(field, points):
+    """phi and gradient (n x 4) of a field at field-local points."""
+    keep = []
+    f = sdf_field(field, keep)
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.zeros((len(p), 4), np.float32)
+    lib().osdf_sample(ctypes.byref(f), _p(p), len(p), _p(out))
+    return out
+
+
+def sdf_trig(which, x):
+    """The oracle's own sin (0), cos (1) or atan2 (2, x: n x 2 of (y, x)) in float32."""
+    x = np.ascontiguousarray(x, np.float32)
+    n = len(x) if which < 2 else len(x) // 2 if x.ndim == 1 else len(x)
+    out = np.zeros(n, np.float32)
+    lib().osdf_trig(int(which), _p(x), n, _p(out))
+    return out
+
+
+def _pose(body):
+    """pos xyz, rot xyzw, scale, static, vel xyz, omega xyz."""
+    b = body
+    return np.array(list(b.get("position", (0, 0, 0))) + list(b.get("orientation", (0, 0, 0, 1))) + [b.get("scale", 1.0),
+                    1.0 if b.get("static", False) else 0.0] + list(b.get("velocity", (0, 0, 0))) +
+                    list(b.get("angularVelocity", (0, 0, 0))), np.float32)
+
+
+def sdf_optimize(kind, fa, body_a, fb, body_b, domain, repulsion, seed, params, step=None, E=None, pcg=None):
+    """One optimiser run of the CPU Algorithm 1: kind 'nelder-mead', 'ellipsoid', 'descent' or
+    'gradient' (the objective and its gradient at the seed). Returns a dict."""
+    keep = []
+    A, B = sdf_field(fa, keep), sdf_field(fb, keep)
+    pa, pb = _pose(body_a), _pose(body_b)
+    dom = np.ascontiguousarray(list(domain[0]) + list(domain[1]), np.float32)
+    rep = np.ascontiguousarray(repulsion, np.float32).reshape(-1, 3)
+    sd = np.ascontiguousarray(seed, np.float32)
+    st = np.ascontiguousarray(step if step is not None else (0, 0, 0), np.float32)
+    Em = np.ascontiguousarray(E if E is not None else np.eye(3), np.float32).reshape(9)
+    state = ctypes.c_uint64(0x853c49e6748fea9b if pcg is None else pcg)
+    out = np.zeros(4, np.float32)
+    counts = np.zeros(2, np.uint32)
+    k = {"nelder-mead": 0, "ellipsoid": 1, "descent": 2, "gradient": 3}[kind]
+    lib().osdf_optimize(k, ctypes.byref(A), _p(pa), ctypes.byref(B), _p(pb), _p(dom), _p(rep), len(rep), _p(sd), _p(st),
+                        ctypes.byref(sdf_params(params)), _p(Em), ctypes.byref(state), _p(out), _p(counts))
+    if k == 3:
+        return {"value": float(out[0]), "gradient": out[1:4].copy(), "queries": int(counts[1])}
+    return {"x": out[:3].copy(), "objective": float(out[3]), "iterations": int(counts[0]), "queries": int(counts[1]),
+            "E": Em.reshape(3, 3).copy(), "pcg": state.value}
+
+
+POINT_WORDS = 28
+
+
+def sdf_points_to_words(points):
+    """Points as the oracle's 28-word records (x, depth, normal, separation, anchors, softmax, rank,
+    active, cached, id, exported, ellipsoid)."""
+    out = np.zeros((len(points), POINT_WORDS), np.float32)
+    u = out.view(np.uint32)
+    for i, p in enumerate(points):
+        out[i, 0:3] = p["x"]; out[i, 3] = p.get("depth", 0)
+        out[i, 4:7] = p.get("normal", (0, 1, 0)); out[i, 7] = p.get("separation", 0)
+        out[i, 11] = p.get("softmax", 0); out[i, 15] = p.get("rank", 0)
+        u[i, 16] = int(bool(p.get("active", False))); u[i, 17] = int(bool(p.get("cached", False)))
+        u[i, 18] = int(p.get("id", 0))
+        e = p.get("ellipsoid6", (1, 0, 0, 1, 0, 1))
+        out[i, 20:24] = e[0:4]; out[i, 24:26] = e[4:6]
+    return out
+
+
+def sdf_words_to_points(w):
+    u = w.view(np.uint32)
+    return [{"x": w[i, 0:3].copy(), "depth": float(w[i, 3]), "normal": w[i, 4:7].copy(), "separation": float(w[i, 7]),
+             "softmax": float(w[i, 11]), "rank": float(w[i, 15]), "active": bool(u[i, 16]), "cached": bool(u[i, 17]),
+             "id": int(u[i, 18]), "ellipsoid6": np.concatenate([w[i, 20:24], w[i, 24:26]])} for i in range(len(w))]
+
+
+def sdf_manifold_cpu(fa, body_a, fb, body_b, params, dt, cache, pcg):
+    """The CPU reference's manifold update (manifold.ts update) with its PCG stream: returns
+    (points, stats dict, pcg state after)."""
+    keep = []
+    A, B = sdf_field(fa, keep), sdf_field(fb, keep)
+    pa, pb = _pose(body_a), _pose(body_b)
+    cw = sdf_points_to_words(cache)
+    out = np.zeros((20, POINT_WORDS), np.float32)
+    stats = np.zeros(4, np.uint32)
+    state = ctypes.c_uint64(pcg)
+    n = lib().osdf_manifold_cpu(ctypes.byref(A), _p(pa), ctypes.byref(B), _p(pb), ctypes.byref(sdf_params(params)),
+                                ctypes.c_float(dt), _p(cw), len(cw), ctypes.byref(state), _p(out), _p(stats))
+    return sdf_words_to_points(out[:n]), dict(zip(("pairs", "contacts", "queries", "iterations"), (int(x) for x in stats))), state.value
+
+
+class SdfBridge:
+    """The SDF bridge installed on an f32 Model (setCollisionBackend): fields (description dicts),
+    per body its field index, scale and field rotation; explicit pairs or automatic."""
+
+    def __init__(self, model, fields, body_field, body_scale=None, body_rotation=None, params=None, automatic=False,
+                 max_pairs=1, max_points=5, cell_size=1.0):
+        self.L, self.model, self.points = lib(), model, max_points
+        self._keep = []
+        arr = (OField * len(fields))(*[sdf_field(f, self._keep) for f in fields])
+        bf = np.ascontiguousarray(body_field, np.uint32)
+        sc = None if body_scale is None else np.ascontiguousarray(body_scale, np.float32)
+        rot = None if body_rotation is None else np.ascontiguousarray(body_rotation, np.float32).reshape(-1)
+        self._keep += [bf, sc, rot]
+        self.L.osdf_create.restype = ctypes.c_void_p
+        self.h = ctypes.c_void_p(self.L.osdf_create(model.h, arr, len(fields), _p(bf), None if sc is None else _p(sc),
+                                                    None if rot is None else _p(rot), ctypes.byref(sdf_params(params)),
+                                                    int(automatic), int(max_pairs), int(max_points), ctypes.c_float(cell_size)))
+
+    def set_pairs(self, pairs):
+        w = np.ascontiguousarray([[p["a"], p["b"], p["key"]] for p in pairs], np.uint32).reshape(-1, 3)
+        self.L.osdf_set_pairs(self.h, _p(w), len(w))
+
+    def set_params(self, params):
+        self.L.osdf_set_params(self.h, ctypes.byref(sdf_params(params)))
+
+    def results_raw(self, cap=1 << 16):
+        stride = 8 + POINT_WORDS * self.points
+        out = np.zeros(cap * stride, np.uint32)
+        ov = ctypes.c_uint32()
+        self.L.osdf_results.restype = ctypes.c_uint32
+        n = self.L.osdf_results(self.h, _p(out), cap, ctypes.byref(ov))
+        return out[:n * stride].reshape(n, stride), ov.value
+
+    def bodies(self):
+        out = np.zeros((self.model.n, 28), np.float32)
+        self.L.osdf_bodies(self.h, _p(out))
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.L.osdf_destroy(self.h)
+            self.h = None

@@ -29,6 +29,7 @@ FLAG_FIXED_STORAGE = 256  # capacities stay as created (overflow counted, never 
 OVERFLOW_BARRIER = 256  # counters' overflow bit: a cooperative solve's grid barrier gave up waiting
 NO_COLOR = 255
 STICK_BIT = 0x80000000
+SDF_ROLLING_BIT = 0x20000000  # contact key bit of a rolling SDF surface (layout.ts:134)
 BODY_FLOATS = 40
 JOINT_FLOATS = 40
 
@@ -100,6 +101,8 @@ EXPORTS = [
     "avbd_append_joints", "avbd_append_joint", "avbd_append_world_joint", "avbd_append_springs", "avbd_release_joints",
     "avbd_set_world_anchor", "avbd_disable_constraint", "avbd_set_body_aux", "avbd_set_lifecycle",
     "avbd_morton_order", "avbd_estimate_pairs", "avbd_plan_capacity", "avbd_set_graphs", "avbd_graph_stats",
+    "avbd_sdf_default_params", "avbd_sdf_create", "avbd_sdf_set_pairs", "avbd_sdf_set_params", "avbd_set_collision_backend",
+    "avbd_sdf_read_results", "avbd_sdf_read_stats", "avbd_sdf_grow", "avbd_sdf_destroy", "avbd_sdf_last_error",
 ]
 FIELD_POSITION, FIELD_ROTATION, FIELD_VELOCITY, FIELD_ANGULAR_VELOCITY = 1, 2, 4, 8
 _FIELDS = {"position": 1, "rotation": 2, "velocity": 4, "angularVelocity": 8}
@@ -143,6 +146,15 @@ def lib():
             if name not in ("avbd_last_error", "avbd_create_error"):
                 fn.restype = ctypes.c_int
         L.avbd_destroy.restype = None
+        L.avbd_sdf_destroy.restype = None
+        L.avbd_sdf_default_params.restype = None
+        L.avbd_sdf_last_error.restype = ctypes.c_char_p
+        L.avbd_sdf_last_error.argtypes = [vp]
+        L.avbd_sdf_create.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_float, ctypes.POINTER(vp)]
+        for name in ("avbd_sdf_set_pairs", "avbd_sdf_set_params", "avbd_sdf_read_results", "avbd_sdf_read_stats",
+                     "avbd_sdf_grow", "avbd_sdf_destroy", "avbd_set_collision_backend"):
+            getattr(L, name).argtypes = None
         _lib = L
     return _lib
 
@@ -252,9 +264,31 @@ class GpuSolver3D:
 
     def destroy(self):
         if not self.destroyed and getattr(self, "h", None):
-            self.L.avbd_destroy(self.h)
+            self.L.avbd_destroy(self.h)  # frees every SDF bridge made for this solver too
+            for b in getattr(self, "_bridges", []):
+                b.disposed, b.installed, b.h = True, False, None
         self.destroyed = True
         self.h = None
+
+    def set_collision_backend(self, backend):
+        """setCollisionBackend (solver.ts:2078-2091): install an SdfCollisionBridge (None: the
+        built-in collision). The solver owns it from then on: a replaced bridge is destroyed."""
+        self._live()
+        if backend is not None and (backend.disposed or backend.solver is not self):
+            raise ValueError("Invalid SDF bridge owner/lifecycle")
+        old = getattr(self, "_backend", None)
+        if backend is old:
+            return
+        self._check(self.L.avbd_set_collision_backend(self.h, None if backend is None else backend.h))
+        if old is not None:
+            old.disposed, old.installed, old.h = True, False, None
+        self._backend = backend
+        if backend is not None:
+            backend.installed = True
+
+    @property
+    def active_collision_backend(self):
+        return getattr(self, "_backend", None)
 
     def __del__(self):
         try:
@@ -649,7 +683,8 @@ class GpuSolver3D:
             a, b, first, cs = (int(x) for x in m[:4])
             for c in range(first, first + (cs & 15)):
                 out.append({
-                    "a": a, "b": b, "feature": int(con[c, 3]) & ~STICK_BIT, "stick": bool(int(con[c, 3]) & STICK_BIT),
+                    "a": a, "b": b, "feature": int(con[c, 3]) & ~(STICK_BIT | SDF_ROLLING_BIT),
+                    "stick": bool(int(con[c, 3]) & STICK_BIT), "rolling": bool(int(con[c, 3]) & SDF_ROLLING_BIT),
                     "rA": conf[c, 0:3].copy(), "rB": conf[c, 4:7].copy(), "pen": conf[c, 8:11].copy(),
                     "lam": conf[c, 12:15].copy(), "c0": np.array([conf[c, 7], conf[c, 11], conf[c, 15]]),
                     "normal": m[4:7].view(np.float32).copy(),

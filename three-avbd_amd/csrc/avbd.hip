@@ -18,8 +18,17 @@
 #include "avbd_events.h"
 #include "avbd_commands.h"
 #include "avbd_plan.h"
+#include "avbd_sdf.h"
 
 using namespace avbd;
+
+struct avbd_sdf;
+/** The installed SDF bridge's part of a step's collision phase (defined with the bridge below). */
+static void sdfEncode(avbd_sdf *g, const Dev &d);
+/** Frees a bridge and its device memory (its solver's stream is synchronised first). */
+static void sdfRelease(avbd_sdf *g);
+/** The bridge's checks before a step (prepare, sdf-bridge.ts:154-160); false sets why. */
+static bool sdfPrepare(const avbd_sdf *g, std::string &why);
 
 // Staging of avbd_read_bodies_noalloc: file-scope device memory, one copy on every GPU the library
 // runs on (held until the process ends), room for AVBD_READ_NOALLOC_MAX body records. Host code
@@ -252,7 +261,12 @@ struct avbd_solver {
     if (p) { hipStreamSynchronize(stream); release(p); }
     return q;
   }
+  avbd_sdf *backend = nullptr;      // the installed collision backend (setCollisionBackend)
+  std::vector<avbd_sdf *> sdfs;     // every SDF bridge made for this solver (they end with it)
   ~avbd_solver() {
+    for (avbd_sdf *g : sdfs) sdfRelease(g);
+    sdfs.clear();
+    backend = nullptr;
     dropGraphs();
     freeAdjoint();
     freeMem(watchMem);
@@ -492,6 +506,7 @@ struct avbd_solver {
   void collision(bool reuseManifolds, bool mayKeepAdjacency) {
     Dev d = dev();
     k_begin<<<1, 1, 0, stream>>>(d, reuseManifolds ? 1 : 0, mayKeepAdjacency ? 1 : 0);
+    if (backend) { sdfEncode(backend, d); return; }  // the backend replaces refs, broadphase and narrowphase
     if (hp.flags & FLAG_REUSE_CONTACTS) k_update_refs<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     if (reuseManifolds) {
       timed(K_NARROW, [&] { k_reuse_manifolds<<<strideBlocks(manCap, 256), 256, 0, stream>>>(d); });
@@ -961,9 +976,11 @@ struct avbd_solver {
   void step(bool reuseManifolds, bool keepAdjacency = false) {
     refreshStatics();
     stepCount++;
-    reuseManifolds = reuseManifolds && manifoldsValid;
+    // a collision backend writes its own contacts each step: manifold reuse is the built-in
+    // narrowphase's (solver.ts:2121), and its host-side pair uploads keep it out of graphs
+    reuseManifolds = reuseManifolds && manifoldsValid && !backend;
     bool keep = reuseManifolds && keepAdjacency && adjacencyHeld;
-    if (!(graphs && !rec && !timer.on && replayGraph(reuseManifolds, keep))) substep(reuseManifolds, keep);
+    if (!(graphs && !rec && !timer.on && !backend && replayGraph(reuseManifolds, keep))) substep(reuseManifolds, keep);
     parity = 1 - parity;
     manifoldsValid = true;
     adjacencyHeld = true;
@@ -1215,6 +1232,7 @@ int avbd_get_params(const avbd_solver *s, avbd_params *p) {
 int avbd_step(avbd_solver *s, const avbd_step_opts *opts) {
   if (!s) return AVBD_ERR_INVALID;
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (s->backend) { std::string why; if (!sdfPrepare(s->backend, why)) return fail(s, AVBD_ERR_INVALID, why); }
   s->barrierFailed = false;
   s->step(opts && opts->reuse_manifolds);
   hipError_t e = hipGetLastError();
@@ -1225,6 +1243,7 @@ int avbd_steps(avbd_solver *s, uint32_t steps, uint32_t substeps) {
   if (!s) return AVBD_ERR_INVALID;
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
   if (substeps == 0) return fail(s, AVBD_ERR_INVALID, "substeps must be >= 1");
+  if (s->backend) { std::string why; if (!sdfPrepare(s->backend, why)) return fail(s, AVBD_ERR_INVALID, why); }
   s->barrierFailed = false;
   for (uint32_t k = 0; k < steps; k++)
     for (uint32_t u = 0; u < substeps; u++) s->step(u > 0, u > 0);
@@ -1728,6 +1747,7 @@ int avbd_profile_kernels(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd
 int avbd_forward(avbd_solver *s, uint32_t steps, uint32_t checkpoint_every) {
   if (!s) return AVBD_ERR_INVALID;
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "solver destroyed");
+  if (s->backend) return fail(s, AVBD_ERR_INVALID, "the adjoint covers the built-in collision only (an SDF bridge is installed)");
   for (uint32_t j = 0; j < s->nj; j++)
     if (s->jinfoHost[4 * j] != (int)T_NONE) return fail(s, AVBD_ERR_INVALID, "the adjoint covers contact-only scenes (joint " + std::to_string(j) + ")");
   if (steps == 0) return fail(s, AVBD_ERR_INVALID, "steps must be > 0");
@@ -2236,6 +2256,7 @@ int avbd_set_lifecycle(avbd_solver *s, uint32_t index, uint32_t bits) {
   if (!s) return AVBD_ERR_INVALID;
   if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
   if (index >= s->n || bits > 6 || (bits & ~6u) != 0) return fail(s, AVBD_ERR_INVALID, "Invalid body lifecycle flags/index");
+  if (s->backend) return fail(s, AVBD_ERR_INVALID, "Body lifecycle requires built-in standard shaders");
   s->lifeHost[index] = (uint8_t)bits;
   float w = (float)(1u | bits);  // BODY_DIRTY | bits
   hipMemcpyAsync((float *)&s->b.ipos[index] + 3, &w, 4, hipMemcpyHostToDevice, s->stream);
@@ -2299,3 +2320,439 @@ int avbd_graph_stats(const avbd_solver *s, uint64_t *captures, uint64_t *replays
   if (replays) *replays = s->graphReplays;
   return AVBD_OK;
 }
+
+// ---- SDF collision backend (SdfCollisionBridge, src/avbd3d/gpu/sdf-bridge.ts) ----------------
+using namespace avbd::sdfk;
+
+struct avbd_sdf {
+  avbd_solver *s = nullptr;
+  bool installed = false, automatic = false;
+  uint32_t n = 0, points = 5, cap = 1;
+  float cellSize = 1.0f;
+  avbd_sdf_params prm{};
+  std::string err;
+  std::vector<void *> mem;
+  uint4 *atlas = nullptr;
+  float4 *bodies = nullptr, *configRot = nullptr, *stage = nullptr;
+  uint4 *config = nullptr, *pairs = nullptr, *results[2] = {nullptr, nullptr};
+  uint32_t current = 0;    // results[current]: the last step's
+  uint32_t stepIndex = 0;  // the pass's step counter (seeds the per-pair streams)
+  uint32_t count = 0;      // explicit: candidates of the last step
+  std::vector<uint32_t> pairsHost, priorHost;  // explicit candidates (a, b, key), the last step's
+  // the spatial grid
+  uint32_t tableSize = 0, cacheSize = 0;
+  uint32_t *bucketCount = nullptr, *bucketStart = nullptr, *bucketCursor = nullptr, *sorted = nullptr, *largeList = nullptr;
+  uint32_t *largeCount = nullptr, *history = nullptr, *pairCount = nullptr, *pairOffset = nullptr, *live = nullptr;
+  int4 *cellKey = nullptr;
+
+  template <class T> T *alloc(size_t count) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256)) != hipSuccess) throw std::runtime_error("hipMalloc failed (SDF bridge)");
+    hipMemsetAsync(p, 0, std::max<size_t>(count * sizeof(T), 256), s->stream);
+    mem.push_back(p);
+    return (T *)p;
+  }
+  void release(void *p) {
+    for (size_t k = 0; k < mem.size(); k++) if (mem[k] == p) { hipFree(p); mem.erase(mem.begin() + (long)k); return; }
+  }
+  ~avbd_sdf() {
+    if (s) hipStreamSynchronize(s->stream);
+    for (void *p : mem) hipFree(p);
+  }
+  uint32_t stride() const { return RESULT_HEAD + POINT_F4 * points; }
+  /** The results, the staging of points being built and (automatic) the history table for `c` pairs. */
+  void allocPairs(uint32_t c) {
+    cap = c;
+    pairs = alloc<uint4>(cap);
+    results[0] = alloc<uint4>((size_t)stride() * cap);
+    results[1] = alloc<uint4>((size_t)stride() * cap);
+    stage = alloc<float4>((size_t)POINT_F4 * points * cap);
+    k_fill<<<blocksFor((uint64_t)cap * 4, 256), 256, 0, s->stream>>>((uint32_t *)pairs, cap * 4u, EMPTY, nullptr);
+    for (int k = 0; k < 2; k++) k_sdf_copy_history<<<blocksFor(cap, 256), 256, 0, s->stream>>>(nullptr, 0, results[k], cap, stride());
+    if (automatic) {
+      cacheSize = pow2At(2ull * cap);
+      history = alloc<uint32_t>(cacheSize);
+    }
+  }
+  SdfDev dev() const {
+    SdfDev g;
+    std::memset(&g, 0, sizeof g);
+    g.atlas = atlas; g.bodies = bodies; g.config = config; g.configRot = configRot; g.pairs = pairs;
+    g.prev = results[current]; g.cur = results[current ^ 1u]; g.stage = stage; g.cap = cap; g.live = live;
+    SdfParams &p = g.p;
+    p.pairCount = automatic ? cap : count; p.step = stepIndex; p.cacheSize = prm.cache_size; p.randomSamples = prm.random_samples;
+    p.dt = s->hp.dt; p.epsilon = prm.epsilon; p.alpha = prm.alpha; p.tol = prm.tolerance;
+    p.margin = prm.margin; p.replacement = prm.replacement_threshold; p.signedSep = prm.signed_separation ? 1u : 0u;
+    p.meshVertex = prm.mesh_vertex_contacts ? 1u : 0u; p.maxIt = prm.max_iterations; p.warmStart = prm.warm_start ? 1u : 0u;
+    p.curvature = prm.curvature_sampling ? 1u : 0u; p.skipConvex = prm.skip_convex_samples ? 1u : 0u; p.optimizer = prm.optimizer;
+    p.stepSize = prm.optimizer == 2 ? prm.step_size : 0.0f; p.sgdStopOnly = prm.optimizer == 2 && prm.sgd_gradient_stop_only ? 1u : 0u;
+    p.points = points; p.allowedPen = prm.allowed_penetration; p.refresh = prm.refresh_deeper_contacts ? prm.tolerance : 0.0f;
+    g.cellSize = cellSize; g.tableMask = tableSize ? tableSize - 1 : 0; g.cacheMask = cacheSize ? cacheSize - 1 : 0;
+    g.bucketCount = bucketCount; g.bucketStart = bucketStart; g.bucketCursor = bucketCursor; g.sorted = sorted;
+    g.largeList = largeList; g.largeCount = largeCount; g.cellKey = cellKey; g.history = history;
+    g.pairCount = pairCount; g.pairOffset = pairOffset;
+    return g;
+  }
+  /** prepare (sdf-bridge.ts:154-160): the checks before a step. */
+  bool prepare(std::string &why) const {
+    if (s->n != n) { why = "Recreate SDF bridge after changing body slots"; return false; }
+    uint64_t need = (uint64_t)(automatic ? cap : count) * points;
+    if (need > std::min(s->conCap, s->manCap)) { why = "Insufficient SDF solver contact capacity"; return false; }
+    if (!std::isfinite(prm.allowed_penetration) || prm.allowed_penetration < 0) { why = "Invalid SDF allowed penetration"; return false; }
+    return true;
+  }
+  /** encode (sdf-bridge.ts:162-181): the collision phase of one step. */
+  void encode(const Dev &d) {
+    hipStream_t st = s->stream;
+    if (!automatic) {
+      // previous slots: the last step's slot of the same key, if its bodies are the same
+      std::vector<uint32_t> up(4ull * count);
+      for (uint32_t i = 0; i < count; i++) {
+        uint32_t a = pairsHost[3 * i], b = pairsHost[3 * i + 1], key = pairsHost[3 * i + 2], prevSlot = EMPTY;
+        for (uint32_t j = 0; 3 * j < priorHost.size(); j++)
+          if (priorHost[3 * j + 2] == key) { if (priorHost[3 * j] == a && priorHost[3 * j + 1] == b) prevSlot = j; break; }
+        up[4 * i] = a; up[4 * i + 1] = b; up[4 * i + 2] = key; up[4 * i + 3] = prevSlot;
+      }
+      if (count) hipMemcpyAsync(pairs, up.data(), 16ull * count, hipMemcpyHostToDevice, st);
+      if (count) hipStreamSynchronize(st);  // `up` is a host temporary
+    }
+    SdfDev g = dev();
+    k_sdf_prepare<<<blocksFor(n, 256), 256, 0, st>>>(d, g);
+    if (automatic) {
+      s->fill(bucketCount, tableSize + 1, 0);
+      s->fill(bucketCursor, tableSize, 0);
+      s->fill(largeCount, 1, 0);
+      s->fill(live, 4, 0);
+      s->fill(history, cacheSize, EMPTY);
+      k_sdf_cells<<<s->strideBlocks(n, 256), 256, 0, st>>>(d, g);
+      s->scan(bucketCount, bucketStart, tableSize + 1);
+      k_sdf_scatter<<<blocksFor(n, 256), 256, 0, st>>>(d, g);
+      k_sdf_history<<<blocksFor(cap, 256), 256, 0, st>>>(d, g);
+      k_sdf_find<false><<<blocksFor(n, 64), 64, 0, st>>>(d, g);
+      s->scan(pairCount, pairOffset, n + 1);
+      k_sdf_find<true><<<blocksFor(n, 64), 64, 0, st>>>(d, g);
+      k_sdf_close<<<blocksFor(cap, 256), 256, 0, st>>>(d, g);
+    } else if (count) {
+      k_sdf_filter<<<blocksFor(count, 256), 256, 0, st>>>(d, g);
+    }
+    uint32_t slots = automatic ? cap : count;
+    if (slots) {
+      size_t lds = (size_t)points * 64 * (4 * sizeof(float) + 1);
+      k_sdf_narrow<<<std::min<uint32_t>(blocksFor(slots, 64, 1u << 20), std::max(s->numCUs, 1u) * 32u), 64, lds, st>>>(d, g);
+      k_sdf_export<<<s->strideBlocks(slots, 256), 256, 0, st>>>(d, g);
+    }
+    current ^= 1u;
+    stepIndex++;
+    if (!automatic) priorHost = pairsHost;
+  }
+};
+
+static void sdfEncode(avbd_sdf *g, const Dev &d) { g->encode(d); }
+static void sdfRelease(avbd_sdf *g) { delete g; }
+static bool sdfPrepare(const avbd_sdf *g, std::string &why) { return g->prepare(why); }
+
+namespace {
+bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+/** packSdfParams's checks (sdf-layout.ts:100-118) and the bridge's (sdf-bridge.ts:62-64). */
+bool checkSdfParams(const avbd_sdf_params &p, uint32_t points, std::string &why) {
+  if (p.optimizer > 2) { why = "Unknown SDF optimizer " + std::to_string(p.optimizer); return false; }
+  if (p.optimizer == 2 && !(p.step_size > 0 && std::isfinite(p.step_size))) { why = "SDF gradient descent needs a finite positive stepSize"; return false; }
+  if (p.cache_size > points - 1 || p.cache_size + p.random_samples > points || p.max_iterations < 1 || p.max_iterations > 512) {
+    why = "This GPU SDF pass supports at most " + std::to_string(points - 1) + " cached/" + std::to_string(points) + " total samples and 1-512 iterations";
+    return false;
+  }
+  if (!(p.epsilon == 0 || p.epsilon > 0) || !(p.tolerance > 0 && std::isfinite(p.tolerance)) || !std::isfinite(p.epsilon) ||
+      !(p.alpha >= 0 && std::isfinite(p.alpha)) || !(p.margin >= 0 && std::isfinite(p.margin)) ||
+      !(p.replacement_threshold >= 0 && std::isfinite(p.replacement_threshold))) {
+    why = "Invalid SDF collision distances";
+    return false;
+  }
+  if (!std::isfinite(p.allowed_penetration) || p.allowed_penetration < 0) { why = "Invalid SDF allowed penetration"; return false; }
+  return true;
+}
+/** packSdfAtlas (sdf-layout.ts:33-71): per field a 32-word header, its grid, seeds and supports. */
+bool packAtlas(const avbd_sdf_field *fields, uint32_t nf, std::vector<uint32_t> &words, std::vector<uint32_t> &offsets, std::string &why) {
+  words.clear();
+  offsets.clear();
+  auto putf = [&](size_t at, float v) { std::memcpy(&words[at], &v, 4); };
+  for (uint32_t f = 0; f < nf; f++) {
+    const avbd_sdf_field &F = fields[f];
+    if (F.kind > 9) { why = "Unsupported GPU SDF kind " + std::to_string(F.kind); return false; }
+    if (!finite3(F.bounds_min) || !finite3(F.bounds_max) || !finite3(F.center_of_mass)) { why = "SDF header exceeds finite float32 range"; return false; }
+    for (int k = 0; k < 3; k++) if (F.bounds_min[k] > F.bounds_max[k]) { why = "Invalid SDF bounds"; return false; }
+    uint64_t nodes = 0;
+    if (F.kind == AVBD_SDF_VOXEL) {
+      if (F.dims[0] < 2 || F.dims[1] < 2 || F.dims[2] < 2 || !(F.cell > 0) || !std::isfinite(F.cell) || !finite3(F.origin) || !F.grid) {
+        why = "Invalid SDF grid geometry";
+        return false;
+      }
+      nodes = (uint64_t)F.dims[0] * F.dims[1] * F.dims[2];
+      for (uint64_t i = 0; i < 4 * nodes; i++) if (!std::isfinite(F.grid[i])) { why = "SDF grid must be finite"; return false; }
+    } else {
+      for (int k = 0; k < (F.kind == AVBD_SDF_WARP ? 4 : 8); k++) if (!std::isfinite(F.params[k])) { why = "SDF secondary parameters exceed finite float32 range"; return false; }
+    }
+    if ((F.seed_count && !F.seeds) || (F.support_count && !F.supports)) { why = "SDF seeds or supports missing"; return false; }
+    size_t at = words.size();
+    offsets.push_back((uint32_t)(at / 4));
+    uint64_t total = at + 32 + 4 * nodes + 4ull * (F.seed_count + F.support_count);
+    if (total * 4 > (1ull << 31)) { why = "SDF atlas exceeds byte/index limit"; return false; }
+    words.resize((size_t)total, 0);
+    size_t gridAt = at + 32, seedAt = gridAt + 4 * nodes;
+    words[at] = F.kind; words[at + 1] = F.convex ? 1u : 0u; words[at + 2] = (uint32_t)(seedAt / 4); words[at + 3] = F.seed_count;
+    if (F.kind == AVBD_SDF_VOXEL) {
+      words[at + 4] = (uint32_t)(gridAt / 4); words[at + 5] = F.dims[0]; words[at + 6] = F.dims[1]; words[at + 7] = F.dims[2];
+      for (int k = 0; k < 3; k++) putf(at + 28 + k, F.origin[k]);
+      std::memcpy(&words[gridAt], F.grid, 16 * nodes);
+    }
+    for (int k = 0; k < 3; k++) { putf(at + 8 + k, F.bounds_min[k]); putf(at + 12 + k, F.bounds_max[k]); putf(at + 16 + k, F.center_of_mass[k]); }
+    putf(at + 11, F.kind == AVBD_SDF_VOXEL ? F.cell : 0.0f);
+    words[at + 15] = F.support_count;
+    for (int k = 0; k < 8; k++) putf(at + 20 + k, F.params[k]);
+    if (F.kind == AVBD_SDF_WARP) words[at + 24] = (uint32_t)F.params[4];
+    for (uint32_t i = 0; i < F.seed_count; i++) for (int k = 0; k < 3; k++) putf(seedAt + 4 * i + k, F.seeds[3 * i + k]);
+    size_t supAt = seedAt + 4ull * F.seed_count;
+    for (uint32_t i = 0; i < F.support_count; i++) for (int k = 0; k < 3; k++) putf(supAt + 4 * i + k, F.supports[3 * i + k]);
+  }
+  if (words.empty()) words.assign(4, 0);
+  return true;
+}
+int sdfFail(avbd_sdf *g, int code, const std::string &msg) {
+  if (g) g->err = msg;
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+void avbd_sdf_default_params(avbd_sdf_params *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->optimizer = 0; o->cache_size = 4; o->random_samples = 1; o->warm_start = 1; o->curvature_sampling = 1; o->skip_convex_samples = 1;
+  o->epsilon = 0.1f; o->alpha = 0.01f; o->replacement_threshold = 0.01f; o->tolerance = 0.01f; o->max_iterations = 50;
+  o->step_size = 0.1f; o->margin = 0.02f; o->allowed_penetration = 0.01f;
+}
+
+int avbd_sdf_create(avbd_solver *s, const avbd_sdf_field *fields, uint32_t nf, const uint32_t *bodyField, const float *bodyScale,
+                    const float *bodyRot, const avbd_sdf_params *params, int automatic, uint32_t maxPairs, uint32_t points, float cellSize,
+                    avbd_sdf **out) {
+  if (!out) return AVBD_ERR_INVALID;
+  *out = nullptr;
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "SDF solver was destroyed");
+  if (!s->n || !bodyField || !fields || !nf) return fail(s, AVBD_ERR_INVALID, "SDF bridge requires a field for every GPU body slot");
+  if (s->worldStartHost.size() > 2) return fail(s, AVBD_ERR_INVALID, "SDF bridge: one world per solver (not a batched solver)");
+  if (points == 0) points = 5;
+  if (points < 2 || points > 20) return fail(s, AVBD_ERR_INVALID, "SDF point capacity must be an integer 2-20");
+  avbd_sdf_params prm;
+  if (params) prm = *params; else avbd_sdf_default_params(&prm);
+  std::string why;
+  if (!checkSdfParams(prm, points, why)) return fail(s, AVBD_ERR_INVALID, why);
+  uint32_t n = s->n;
+  for (uint32_t i = 0; i < n; i++) {
+    if (bodyField[i] >= nf) return fail(s, AVBD_ERR_INVALID, "body " + std::to_string(i) + ": no such field");
+    float sc = bodyScale ? bodyScale[i] : 1.0f;
+    if (!(sc > 0) || !std::isfinite(sc)) return fail(s, AVBD_ERR_INVALID, "Invalid SDF bridge scale");
+    if (bodyRot) {
+      const float *q = bodyRot + 4 * i;
+      double norm = std::sqrt((double)q[0] * q[0] + (double)q[1] * q[1] + (double)q[2] * q[2] + (double)q[3] * q[3]);
+      if (!std::isfinite(norm) || std::fabs(norm - 1) > 1e-5) return fail(s, AVBD_ERR_INVALID, "SDF field rotation must be a unit quaternion");
+    }
+  }
+  std::vector<uint32_t> words, offsets;
+  if (!packAtlas(fields, nf, words, offsets, why)) return fail(s, AVBD_ERR_INVALID, why);
+  uint64_t maxAuto = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n * (n - 1) / 2, std::max<uint64_t>(64, 8ull * n)));
+  uint64_t capWanted = maxPairs ? maxPairs : (automatic ? maxAuto : 1);
+  if (capWanted > (1u << 24)) return fail(s, AVBD_ERR_INVALID, "Invalid SDF broadphase pair capacity");
+  if (automatic) {
+    if (!(cellSize >= 0) || !std::isfinite(cellSize)) return fail(s, AVBD_ERR_INVALID, "Invalid SDF broadphase cell size");
+    if (cellSize == 0) {  // the median field bounds diagonal x scale (sdfBroadphaseOptions)
+      std::vector<double> sizes(n);
+      for (uint32_t i = 0; i < n; i++) {
+        const avbd_sdf_field &F = fields[bodyField[i]];
+        double dx = F.bounds_max[0] - F.bounds_min[0], dy = F.bounds_max[1] - F.bounds_min[1], dz = F.bounds_max[2] - F.bounds_min[2];
+        sizes[i] = std::sqrt(dx * dx + dy * dy + dz * dz) * (bodyScale ? bodyScale[i] : 1.0f);
+      }
+      std::sort(sizes.begin(), sizes.end());
+      cellSize = (float)sizes[n / 2];
+    }
+    if (!(cellSize > 0)) return fail(s, AVBD_ERR_INVALID, "Invalid SDF broadphase cell size");
+  } else if ((uint64_t)capWanted * points > std::min(s->conCap, s->manCap)) {
+    return fail(s, AVBD_ERR_CAPACITY, "SDF bridge requires capacity for " + std::to_string(points) + " contacts/manifolds per candidate pair");
+  }
+  if (automatic && (s->hp.flags & AVBD_FLAG_FIXED_STORAGE) && capWanted * points > std::min(s->conCap, s->manCap))
+    return fail(s, AVBD_ERR_CAPACITY, "Fixed storage: the SDF pairs' contacts exceed the solver's contact capacity");
+  avbd_sdf *g = new avbd_sdf();
+  g->s = s; g->n = n; g->points = points; g->automatic = automatic != 0; g->cellSize = cellSize; g->prm = prm;
+  try {
+    hipStreamSynchronize(s->stream);
+    g->atlas = g->alloc<uint4>(words.size() / 4);
+    hipMemcpyAsync(g->atlas, words.data(), words.size() * 4, hipMemcpyHostToDevice, s->stream);
+    std::vector<uint32_t> cfg(4ull * n);
+    std::vector<float> rot(4ull * n);
+    for (uint32_t i = 0; i < n; i++) {
+      float sc = bodyScale ? bodyScale[i] : 1.0f;
+      cfg[4 * i] = offsets[bodyField[i]];
+      std::memcpy(&cfg[4 * i + 1], &sc, 4);
+      for (int k = 0; k < 4; k++) rot[4 * i + k] = bodyRot ? bodyRot[4 * i + k] : (k == 3 ? 1.0f : 0.0f);
+    }
+    g->config = g->alloc<uint4>(n);
+    g->configRot = g->alloc<float4>(n);
+    g->bodies = g->alloc<float4>((size_t)BODY_F4 * n);
+    hipMemcpyAsync(g->config, cfg.data(), cfg.size() * 4, hipMemcpyHostToDevice, s->stream);
+    hipMemcpyAsync(g->configRot, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, s->stream);
+    if (g->automatic) {
+      g->tableSize = pow2At(2ull * n);
+      g->bucketCount = g->alloc<uint32_t>(g->tableSize + 1);
+      g->bucketStart = g->alloc<uint32_t>(g->tableSize + 1);
+      g->bucketCursor = g->alloc<uint32_t>(g->tableSize);
+      g->sorted = g->alloc<uint32_t>(n);
+      g->largeList = g->alloc<uint32_t>(n);
+      g->largeCount = g->alloc<uint32_t>(1);
+      g->pairCount = g->alloc<uint32_t>(n + 1);
+      g->pairOffset = g->alloc<uint32_t>(n + 1);
+      g->live = g->alloc<uint32_t>(4);
+      g->cellKey = g->alloc<int4>(n);
+      if (g->tableSize + 1 > MAX_SCAN) throw std::runtime_error("SDF spatial grid exceeds the scan's size");
+    }
+    g->allocPairs((uint32_t)capWanted);
+    if (g->automatic) {  // reserveCollisionCapacity (solver.ts:2043-2055)
+      uint64_t need = (uint64_t)g->cap * points;
+      if (need > std::min(s->conCap, s->manCap)) s->growStorage(std::max<uint32_t>((uint32_t)need, s->conCap), std::max<uint32_t>((uint32_t)need, s->pairCap),
+                                                               std::max<uint32_t>((uint32_t)need, s->manCap));
+    }
+    if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error creating the SDF bridge");
+  } catch (const std::exception &e) {
+    delete g;
+    return fail(s, AVBD_ERR_DEVICE, e.what());
+  }
+  s->sdfs.push_back(g);
+  *out = g;
+  return AVBD_OK;
+}
+
+int avbd_sdf_set_pairs(avbd_sdf *g, const uint32_t *pairs, uint32_t count) {
+  if (!g) return AVBD_ERR_INVALID;
+  if (g->automatic) return sdfFail(g, AVBD_ERR_INVALID, "Automatic SDF candidates are GPU generated");
+  if (count > g->cap) return sdfFail(g, AVBD_ERR_CAPACITY, "SDF candidate capacity exceeded");
+  if (count && !pairs) return sdfFail(g, AVBD_ERR_INVALID, "pairs is null");
+  std::vector<std::pair<uint32_t, uint32_t>> ab;
+  std::vector<uint32_t> keys;
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t a = pairs[3 * i], b = pairs[3 * i + 1], key = pairs[3 * i + 2];
+    if (a >= g->n || b >= g->n || a == b) return sdfFail(g, AVBD_ERR_INVALID, "Invalid SDF pair body IDs");
+    if (key == EMPTY) return sdfFail(g, AVBD_ERR_INVALID, "SDF pair keys must be unique uint32 IDs");
+    if (a <= b) return sdfFail(g, AVBD_ERR_INVALID, "SDF candidates require unique body pairs with a > b");
+    ab.push_back({a, b});
+    keys.push_back(key);
+  }
+  std::sort(ab.begin(), ab.end());
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) return sdfFail(g, AVBD_ERR_INVALID, "SDF pair keys must be unique uint32 IDs");
+  if (std::adjacent_find(ab.begin(), ab.end()) != ab.end()) return sdfFail(g, AVBD_ERR_INVALID, "SDF candidates require unique body pairs with a > b");
+  g->pairsHost.assign(pairs, pairs + 3ull * count);
+  g->count = count;
+  return AVBD_OK;
+}
+
+int avbd_sdf_set_params(avbd_sdf *g, const avbd_sdf_params *params) {
+  if (!g || !params) return AVBD_ERR_INVALID;
+  std::string why;
+  if (!checkSdfParams(*params, g->points, why)) return sdfFail(g, AVBD_ERR_INVALID, why);
+  g->prm = *params;
+  return AVBD_OK;
+}
+
+int avbd_set_collision_backend(avbd_solver *s, avbd_sdf *g) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (s->destroyed) return fail(s, AVBD_ERR_DESTROYED, "Solver is destroyed");
+  if (g == s->backend) return AVBD_OK;
+  if (g && g->s != s) return fail(s, AVBD_ERR_INVALID, "Invalid SDF bridge owner/lifecycle");
+  if (s->hp.flags & AVBD_FLAG_FIXED_STORAGE)
+    return fail(s, AVBD_ERR_INVALID, "Fixed storage: collision backends are not available (a fixed solver keeps the standard narrowphase; use adaptive storage for SDF collisions)");
+  for (uint32_t i = 0; i < s->n; i++)
+    if (s->lifeHost[i] != 0) return fail(s, AVBD_ERR_INVALID, "Submit an enabled standard step before switching collision backends: pending body lifecycle state");
+  std::string why;
+  if (g && !g->prepare(why)) return fail(s, AVBD_ERR_INVALID, why);
+  hipStreamSynchronize(s->stream);
+  if (s->backend) {
+    s->sdfs.erase(std::remove(s->sdfs.begin(), s->sdfs.end(), s->backend), s->sdfs.end());
+    sdfRelease(s->backend);
+  }
+  s->backend = g;
+  if (g) g->installed = true;
+  s->manifoldsValid = false;
+  s->adjacencyHeld = false;
+  // the counts of the last step's contacts go, so no later step warm-starts from them
+  uint32_t zero[2] = {0, 0};
+  const int words[] = {C_PAIRS, C_CONTACTS, C_MANIFOLDS, C_OVERFLOW, C_PREV_CONTACTS, C_PREV_MANIFOLDS};
+  for (int w : words) hipMemcpyAsync(s->counters + w, zero, 4, hipMemcpyHostToDevice, s->stream);
+  return avbd_sync(s);
+}
+
+int avbd_sdf_read_results(avbd_sdf *g, uint32_t *out, uint32_t cap, uint32_t *count) {
+  if (!g) return AVBD_ERR_INVALID;
+  uint32_t slots = g->automatic ? g->cap : g->count;
+  if (count) *count = slots;
+  if (!out || !cap) return AVBD_OK;
+  uint32_t nr = std::min(cap, slots);
+  if (nr && hipMemcpyAsync(out, g->results[g->current], 16ull * g->stride() * nr, hipMemcpyDeviceToHost, g->s->stream) != hipSuccess)
+    return sdfFail(g, AVBD_ERR_DEVICE, "read failed");
+  if (hipStreamSynchronize(g->s->stream) != hipSuccess) return sdfFail(g, AVBD_ERR_DEVICE, "read failed");
+  return AVBD_OK;
+}
+
+int avbd_sdf_read_stats(avbd_sdf *g, uint32_t *pairs, uint32_t *tested, uint32_t *large, uint32_t *capacity, float *cell) {
+  if (!g) return AVBD_ERR_INVALID;
+  if (!g->automatic) return sdfFail(g, AVBD_ERR_INVALID, "Explicit candidates have no spatial grid");
+  uint32_t w[4] = {0, 0, 0, 0};
+  hipMemcpyAsync(w, g->live, 16, hipMemcpyDeviceToHost, g->s->stream);
+  if (hipStreamSynchronize(g->s->stream) != hipSuccess) return sdfFail(g, AVBD_ERR_DEVICE, "read failed");
+  if (pairs) *pairs = w[0];
+  if (tested) *tested = w[1];
+  if (large) *large = w[2];
+  if (capacity) *capacity = g->cap;
+  if (cell) *cell = g->cellSize;
+  return AVBD_OK;
+}
+
+int avbd_sdf_grow(avbd_sdf *g, uint32_t required) {
+  if (!g) return AVBD_ERR_INVALID;
+  if (!g->automatic) return sdfFail(g, AVBD_ERR_INVALID, "SDF growth requires an active spatial bridge");
+  if (required <= g->cap || required > (1u << 24)) return sdfFail(g, AVBD_ERR_INVALID, "Invalid SDF requested capacity");
+  avbd_solver *s = g->s;
+  uint32_t capacity = std::max(required, std::min(2u * g->cap, 1u << 24));
+  try {
+    hipStreamSynchronize(s->stream);
+    uint32_t oldCap = g->cap, st = g->stride();
+    uint4 *keep = g->results[g->current];
+    uint4 *oldOther = g->results[g->current ^ 1u], *oldPairs = g->pairs, *oldHistory = (uint4 *)g->history;
+    float4 *oldStage = g->stage;
+    g->allocPairs(capacity);
+    k_sdf_copy_history<<<blocksFor(capacity, 256), 256, 0, s->stream>>>(keep, oldCap, g->results[g->current], capacity, st);
+    hipStreamSynchronize(s->stream);
+    g->release(keep); g->release(oldOther); g->release(oldPairs); g->release(oldStage); g->release(oldHistory);
+    uint64_t need = (uint64_t)g->cap * g->points;
+    if (need > std::min(s->conCap, s->manCap))
+      s->growStorage(std::max<uint32_t>((uint32_t)need, s->conCap), std::max<uint32_t>((uint32_t)need, s->pairCap), std::max<uint32_t>((uint32_t)need, s->manCap));
+    if (hipStreamSynchronize(s->stream) != hipSuccess) throw std::runtime_error("device error growing the SDF bridge");
+  } catch (const std::exception &e) {
+    return sdfFail(g, AVBD_ERR_DEVICE, e.what());
+  }
+  return AVBD_OK;
+}
+
+void avbd_sdf_destroy(avbd_sdf *g) {
+  if (!g) return;
+  avbd_solver *s = g->s;
+  if (s && s->backend == g) {
+    s->backend = nullptr;
+    s->manifoldsValid = false;
+    s->adjacencyHeld = false;
+  }
+  if (s) s->sdfs.erase(std::remove(s->sdfs.begin(), s->sdfs.end(), g), s->sdfs.end());
+  sdfRelease(g);
+}
+
+const char *avbd_sdf_last_error(const avbd_sdf *g) { return g ? g->err.c_str() : "null SDF bridge"; }
+
+}  // extern "C"

@@ -73,6 +73,9 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {  // lowbias32
 
 // --- records ---------------------------------------------------------------------------------
 static const uint32_t STICK_BIT = 0x80000000u, SPHERE_FEATURE = 3u << 24;
+/** A contact of a rolling SDF surface (sphere, capsule, cylinder, cone, torus; layout.ts:129-134):
+ *  its lever arms use the step-start rotation, as a sphere's do (wgsl-solve.ts:203). */
+static const uint32_t SDF_ROLLING_BIT = 0x20000000u;
 static const uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8, FLAG_MASS_PENALTY = 16;
 static const float SHAPE_SPHERE = 1.0f, SHAPE_HULL = 3.0f;
 static const uint32_t NO_COLOR = 255u, PENDING = 256u, MAX_COLORS = 64u;

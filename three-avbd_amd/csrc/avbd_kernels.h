@@ -1083,7 +1083,7 @@ template <bool WANT_J = false>
 __device__ __forceinline__ Eval evalContact(const Contact &k, const V *basis, float friction, const PairBody &A, const PairBody &B,
                                             float alpha, bool isA = true) {
   Q rotA = A.rot, rotB = B.rot;
-  if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE) { rotA = A.irot; rotB = B.irot; }
+  if ((__float_as_uint(k.rA.w) & ~STICK_BIT) == SPHERE_FEATURE || (__float_as_uint(k.rA.w) & SDF_ROLLING_BIT) != 0u) { rotA = A.irot; rotB = B.irot; }
   Eval e;
   e.rAW = qrotate(rotA, xyz(k.rA));
   e.rBW = qrotate(rotB, xyz(k.rB));
