@@ -186,8 +186,10 @@ int avbd_get_capacity(const avbd_solver *s, avbd_capacity *out);
 
 /* How the solve pass is scheduled (results are identical whichever runs). */
 typedef struct avbd_schedule {
-  uint32_t cooperative;      /* 1: one cooperative launch per substep with grid barriers; 0: per-colour launches */
-  uint32_t threads;          /* workgroup size of the cooperative kernel: 64..1024 */
+  uint32_t cooperative;      /* 0: per-colour launches; 1: one cooperative launch per substep with grid
+                                barriers; 2: the dataflow sweep, one cooperative launch per substep whose
+                                bodies wait only for their neighbours' per-body completion counts */
+  uint32_t threads;          /* workgroup size of the cooperative kernel: 64..1024 (the dataflow sweep: 64) */
   uint32_t blocks_per_cu;    /* workgroups per CU of the cooperative kernel */
 } avbd_schedule;
 int avbd_set_schedule(avbd_solver *s, const avbd_schedule *schedule);

@@ -25,8 +25,10 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_world_fills_an_empty_solver_and_steps_like_the_oracle(gpu):
-    say("case: World-style edits on an empty solver (pyramid, plastic chain, spring, world joint, aux, lifecycle), 90 steps")
+@pytest.mark.parametrize("solve", ["coop", "flow"])
+def test_world_fills_an_empty_solver_and_steps_like_the_oracle(gpu, monkeypatch, solve):
+    say(f"case: World-style edits on an empty solver (pyramid, plastic chain, spring, world joint, aux, lifecycle), 90 steps, {solve} solve")
+    monkeypatch.setenv("AVBD_SOLVE", solve)
     dt = 1 / 60
     p = avbd_amd.default_params(dt=dt, gravity=-9.81, color_cap=32, primal_lanes=(0, 0, 0, 0))
     s = avbd_amd.GpuSolver3D(np.zeros((0, 40), np.float32), params=p, capacity={"bodies": 64, "joints": 2})

@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include "../../include/avbd.h"
 #include "avbd_adjoint.h"
+#include "avbd_flow.h"
 #include "avbd_events.h"
 #include "avbd_commands.h"
 #include "avbd_plan.h"
@@ -138,6 +139,7 @@ struct avbd_solver {
   uint32_t barrierPolls = 1u << 22;  // a barrier wait gives up after this many polls
   uint32_t barrierGiveUps = 0;       // give-ups seen by the host since avbd_read_counters last reported them
   bool barrierFailed = false;        // seen during the current avbd_step/avbd_steps call
+  bool coopFlow = false;        // the cooperative launch is the dataflow sweep (k_flow), not k_solve
   uint32_t coopThreads = 512;   // threads per workgroup of the solve kernel
   uint32_t coopPerCU = 1;       // its workgroups per CU
   uint32_t coopMaxBlocks = 0;   // co-resident limit for coopThreads (0: no cooperative launch here)
@@ -158,6 +160,8 @@ struct avbd_solver {
   uint2 *filters = nullptr;
   float4 *joints = nullptr;
   int4 *jinfo = nullptr;
+  uint32_t *flowOrphan = nullptr;  // per joint: its dual runs before the dataflow sweep (no body walks it)
+  uint32_t *bodyKey = nullptr, *flowDone = nullptr;  // the dataflow sweep's order keys and counts
   uint32_t *statics = nullptr;
   uint32_t staticsCap = 0;
   uint4 *hulls = nullptr;
@@ -293,20 +297,34 @@ struct avbd_solver {
   }
   int applySchedule(const avbd_schedule &sch) {
     uint32_t t = sch.threads;
-    if (sch.cooperative) {
+    if (sch.cooperative == 2) {
+      // the dataflow sweep: one wavefront per workgroup, all co-resident
+      if (t != 64 || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
+      int perCU = 0;
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_flow, 64, 0);
+      if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
+      coopFlow = true;
+      coopThreads = 64;
+      coopPerCU = sch.blocks_per_cu;
+      coopBlocks = coopPerCU * numCUs;
+    } else if (sch.cooperative == 1) {
       if (!solveKernel(t) || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
       int perCU = 0;
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, solveKernel(t), (int)t, 0);
       if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
+      coopFlow = false;
       coopThreads = t;
       coopPerCU = sch.blocks_per_cu;
       coopBlocks = coopPerCU * numCUs;
-    } else {
+    } else if (sch.cooperative == 0) {
       coopBlocks = 0;
+      coopFlow = false;
+    } else {
+      return AVBD_ERR_INVALID;
     }
     return AVBD_OK;
   }
-  avbd_schedule schedule() const { return avbd_schedule{coopBlocks ? 1u : 0u, coopThreads, coopPerCU}; }
+  avbd_schedule schedule() const { return avbd_schedule{coopBlocks ? (coopFlow ? 2u : 1u) : 0u, coopThreads, coopPerCU}; }
   /** The fused tail starts where every later colour held at most TAIL_BODIES bodies at this
    *  readback (the cap's empty spares included). One workgroup runs on one CU, so only small
    *  colours gain: a 55-box pyramid sweeps in one launch per iteration (1.12 -> 0.61 ms per
@@ -419,6 +437,7 @@ struct avbd_solver {
     d.adjIds = adjIds; d.adjGeo = adjGeo; d.colorItems = colorItems; d.itemStart = itemStart; d.items = items; d.itemCap = itemCap;
     d.colA = colA; d.colB = colB; d.colorStart = colorStart; d.colorBodies = colorBodies; d.colorRank = colorRank; d.hist = hist;
     d.colorGroups = colorGroups; d.counters = counters; d.p = params();
+    d.bodyKey = bodyKey; d.flowDone = flowDone; d.flowOrphan = flowOrphan;
     d.aux = aux;
     d.src = rec ? src : nullptr;
     d.bits = nullptr;
@@ -575,6 +594,27 @@ struct avbd_solver {
   }
   void solve() {
     Dev d = dev();
+    if (coopBlocks && coopFlow && !rec) {
+      // the dataflow sweep; prep writes only the walk items' flags and the orphan joints' marks, so
+      // a launch refused here leaves the per-colour sweep below everything it needs
+      if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
+      k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+      k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d);
+      fill(flowDone, n, 0u);
+      void *args[] = {&d, &barrierPolls};
+      hipError_t e;
+      timed(K_SOLVE, [&] { e = hipLaunchCooperativeKernel((const void *)k_flow, dim3(coopBlocks), dim3(64), args, 0, stream); });
+      if (e == hipSuccess) {
+        k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+        return;
+      }
+      (void)hipGetLastError();
+      if (capturing) { captureFailed = true; return; }
+      coopBlocks = 0;
+      coopFlow = false;
+      sweepColors(d);
+      return;
+    }
     if (coopBlocks && !rec) {
       fill(barrier, 1, 0u);  // the arrival count starts each launch at zero
       void *args[] = {&d, &barrier, &barrierPolls};
@@ -587,6 +627,11 @@ struct avbd_solver {
     }
     if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
     k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    sweepColors(d);
+  }
+  /** The iterations colour by colour (one launch per colour, the dual, per iteration), then the
+   *  velocities; the warm starts have run. */
+  void sweepColors(Dev &d) {
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
     uint32_t pb = blocksFor(n, 64), snapBlocks = blocksFor(std::max(n, conCap), 256);
     if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, 0);
@@ -829,6 +874,7 @@ struct avbd_solver {
     hipStreamSynchronize(stream);
     joints = regrow(joints, 10ull * jointCap, 10ull * cap, true);
     jinfo = regrow(jinfo, jointCap, cap, true);
+    flowOrphan = regrow(flowOrphan, jointCap, cap, false);
     uint32_t ac = 2u * (cap + manCap);
     adjList = regrow(adjList, adjCap, ac, false);
     adjTmp = regrow(adjTmp, adjCap, ac, false);
@@ -898,6 +944,7 @@ struct avbd_solver {
     barrierGiveUps += c[C_BARRIER];
     barrierFailed = true;
     coopBlocks = 0;
+    coopFlow = false;
     fill(counters + C_BARRIER, 1, 0u);
     err = "the cooperative solve's grid barrier gave up waiting (" + std::to_string(c[C_BARRIER]) +
           " waits): the solve of the steps since the last readback ran unordered; the solver now uses per-colour launches";
@@ -912,7 +959,7 @@ struct avbd_solver {
    *  capacities, params) and the choices taken outside them. Equal bytes, equal launches. */
   struct GraphKey {
     Dev d;
-    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge;
+    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge, coopFlow;
     const uint32_t *fixedDev;
   };
   struct GraphEntry { GraphKey key; hipGraphExec_t exec; uint64_t used; };
@@ -926,6 +973,7 @@ struct avbd_solver {
     k.d = dev();
     k.reuse = reuse; k.keep = keep; k.coopBlocks = coopBlocks; k.coopThreads = coopThreads; k.tailStart = tailStart;
     k.barrierPolls = barrierPolls; k.hullBlocks = hullBlocks; k.anyLarge = anyDynamicLarge; k.fixedDev = fixedColors ? fixedDev : nullptr;
+    k.coopFlow = coopFlow;
     return k;
   }
   void dropGraphs() {
@@ -1082,7 +1130,14 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
       avbd_schedule sch{1u, 512u, 1u};
       if (const char *t = getenv("AVBD_SOLVE_THREADS")) sch.threads = (uint32_t)atoi(t);
       if (const char *b = getenv("AVBD_SOLVE_BLOCKS_PER_CU")) sch.blocks_per_cu = (uint32_t)atoi(b);
-      if (const char *mode = getenv("AVBD_SOLVE")) sch.cooperative = std::string(mode) == "colors" ? 0u : 1u;
+      if (const char *mode = getenv("AVBD_SOLVE")) {
+        std::string m(mode);
+        sch.cooperative = m == "colors" ? 0u : (m == "flow" ? 2u : 1u);
+        if (m == "flow") {
+          sch.threads = 64;
+          if (!getenv("AVBD_SOLVE_BLOCKS_PER_CU")) sch.blocks_per_cu = 4;
+        }
+      }
       if (s->applySchedule(sch) != AVBD_OK) s->applySchedule(avbd_schedule{0u, 512u, 1u});
     }
     s->n = body_count;
@@ -1111,6 +1166,7 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->filters = s->alloc<uint2>(N);
     s->joints = s->alloc<float4>(10ull * s->jointCap);
     s->jinfo = s->alloc<int4>(s->jointCap);
+    s->flowOrphan = s->alloc<uint32_t>(s->jointCap);
     s->hulls = s->alloc<uint4>(s->hullCap / 4 + 1);
     s->cellCount = s->alloc<uint32_t>(s->tableSize + 1);
     s->cellStart = s->alloc<uint32_t>(s->tableSize + 1);
@@ -1150,6 +1206,8 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->colorBodies = s->alloc<uint32_t>(N);
     s->colorItems = s->alloc<uint4>(N);
     s->colorRank = s->alloc<uint32_t>(N);
+    s->bodyKey = s->alloc<uint32_t>(N);
+    s->flowDone = s->alloc<uint32_t>(N);
     s->hist = s->alloc<uint32_t>(64ull * s->colorGroups + 1);
     s->fixedDev = s->alloc<uint32_t>(N);
     s->counters = s->alloc<uint32_t>(C_WORDS);
@@ -1623,9 +1681,10 @@ int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour) {
 }
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms) {
   if (!s || substeps == 0 || steps == 0) return AVBD_ERR_INVALID;
-  const avbd_schedule cands[] = {{0, 512, 1}, {1, 256, 1}, {1, 512, 1}, {1, 1024, 1}, {1, 256, 2}, {1, 512, 2}};
+  const avbd_schedule cands[] = {{0, 512, 1}, {1, 256, 1}, {1, 512, 1}, {1, 1024, 1}, {1, 256, 2}, {1, 512, 2},
+                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}};
   const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
-  float bestT[8];
+  float bestT[sizeof(cands) / sizeof(cands[0])];
   for (int k = 0; k < nc; k++) bestT[k] = INFINITY;
   avbd_schedule keep = s->schedule();
   for (int pass = 0; pass < 2; pass++)
@@ -1638,12 +1697,12 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
   int pick = -1;
   for (int k = 0; k < nc; k++) if (bestT[k] < INFINITY && (pick < 0 || bestT[k] < bestT[pick])) pick = k;
   if (pick < 0) { s->applySchedule(keep); return fail(s, AVBD_ERR_DEVICE, "autotune: no schedule ran"); }
-  // Threads per body, under the per-colour schedule when it won: the given lane rule, eight threads
+  // Threads per body, under the per-colour or dataflow schedule when it won: the given lane rule, eight threads
   // for every colour above the rule's last size, and each of the two without sixteen-thread
   // colours; the fastest is kept (results differ only in the order the lanes' sums are added).
   // Sixteen threads pay where small colours hold bodies with long adjacency walks, and cost an
   // extra reduction step where they do not.
-  if (!cands[pick].cooperative) {
+  if (cands[pick].cooperative != 1) {
     const uint32_t *g = s->hp.primal_lanes;
     const uint32_t rules[4][4] = {{g[0], g[1], g[2], g[3]}, {1u << 31, 1u << 31, 1u << 31, g[3]}, {g[0], g[1], g[2], 0u}, {1u << 31, 1u << 31, 1u << 31, 0u}};
     uint32_t given[4] = {g[0], g[1], g[2], g[3]}, bestRule = 0;

@@ -71,6 +71,10 @@ struct Dev {
   // colouring
   uint32_t *colA, *colB, *colorStart, *colorBodies, *colorRank, *hist;
   uint32_t colorGroups;
+  // the dataflow sweep: per body its order key (colour << 26 | its chunk within a serial colour)
+  // and the iterations of this substep's sweep it has completed
+  uint32_t *bodyKey, *flowDone;
+  uint32_t *flowOrphan;  // per joint: 1 when no body walks it (its duals run apart from the sweep)
   uint32_t *counters;
   Params p;
 };
@@ -925,6 +929,14 @@ __global__ void k_color_starts(Dev d) {
   uint64_t used = __ballot(c < 64 && next > at);
   if (c == 0) d.counters[C_NUM_COLORS] = used ? 64u - (uint32_t)__clzll((long long)used) : 0u;
 }
+/** Threads per body of a colour's primal, from the colour's size (the reference's primal_lanes rule). */
+__device__ __forceinline__ uint32_t lanesFor(const Params &p, uint32_t count) {
+  if (count >= p.lanes1) return 1;
+  if (count >= p.lanes2) return 2;
+  if (count >= p.lanes4) return 4;
+  if (count >= p.lanes8) return 8;
+  return 16;  // small colours: their time is the longest adjacency walk, so split it further
+}
 __global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
   uint32_t b = blockIdx.x * 256 + threadIdx.x;
   if (b >= d.n || !isDynamic(d, b)) return;
@@ -932,6 +944,12 @@ __global__ void __launch_bounds__(256) k_color_scatter(Dev d) {
   uint32_t slot = d.hist[col * d.colorGroups + blockIdx.x] + d.colorRank[b];
   d.colorBodies[slot] = b;
   d.colorItems[slot] = make_uint4(b, d.itemStart[b], d.itemStart[b + 1], bodyEnabled(d, b) ? 1u : 0u);
+  if (d.bodyKey) {
+    // a serial (clashing) colour runs its bucket in chunks of one wavefront's bodies, in order
+    uint32_t start = d.colorStart[col], chunk = 0;
+    if ((d.counters[C_CLASHING + (col >> 5)] >> (col & 31u)) & 1u) chunk = (slot - start) / (64u / lanesFor(d.p, d.colorStart[col + 1] - start));
+    d.bodyKey[b] = (col << 26) | chunk;
+  }
 }
 /** The primal's walk items of every body: counted (then scanned into itemStart) and filled in
  *  adjacency order. Nothing when the substep keeps the last adjacency (C_TOPO bit 0). */
@@ -968,6 +986,41 @@ __global__ void __launch_bounds__(256) k_item_fill(Dev d) {
 
 // ---------------------------------------------------------------------------------------------
 // Solve (wgsl-solve.ts)
+/** How the solve's helpers touch state that a sweep updates (poses, contact multipliers and
+ *  penalties, joint records). Plain: ordinary loads and stores; a kernel boundary orders them.
+ *  Coherent: device-scope relaxed atomics (global_load/store ... sc1), which read and write past
+ *  the per-XCD caches, so that within one launch a workgroup on one XCD reads what a workgroup on
+ *  another wrote once that store completed (the dataflow sweep orders them by per-body counters). */
+struct Plain {
+  static __device__ __forceinline__ float4 ld(const float4 &r) { return r; }
+  static __device__ __forceinline__ void st(float4 &r, float4 v) { r = v; }
+  static __device__ __forceinline__ void stw(float &r, float v) { r = v; }
+};
+struct Coherent {
+  static __device__ __forceinline__ float4 ld(const float4 &r) {
+    uint64_t *q = reinterpret_cast<uint64_t *>(const_cast<float4 *>(&r));
+    uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)), __uint_as_float((uint32_t)b),
+                       __uint_as_float((uint32_t)(b >> 32)));
+  }
+  static __device__ __forceinline__ void st(float4 &r, float4 v) {
+    uint64_t *q = reinterpret_cast<uint64_t *>(&r);
+    __hip_atomic_store(q, (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, (uint64_t)__float_as_uint(v.z) | ((uint64_t)__float_as_uint(v.w) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ void stw(float &r, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t *>(&r), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+};
+template <class M>
+__device__ __forceinline__ Contact loadContact(const Contact &c) {
+  Contact k;
+  k.rA = M::ld(c.rA); k.rB = M::ld(c.rB); k.pen = M::ld(c.pen); k.lam = M::ld(c.lam);
+  return k;
+}
+template <class M>
+__device__ __forceinline__ bool enabledM(const Dev &d, uint32_t i) { return (lifecycleBits(M::ld(d.b.ipos[i]).w) & 2u) == 0u; }
 /** One body's Newton system: lin/ang lower triangles (00 10 11 20 21 22), cross (row = angular). */
 struct Acc { float lin[6], ang[6], cr[9], rl[3], ra[3]; };
 __device__ __forceinline__ void accZero(Acc &a) {
@@ -996,26 +1049,32 @@ __device__ __forceinline__ void addRow(Acc &acc, V l, V a, float k, float f) {
 }
 /** a . b as a chain of fused multiply-adds (x first). */
 __device__ __forceinline__ float fdot(V a, V b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
-__device__ __forceinline__ Q rotAOf(const Dev &d, int a) { return a < 0 ? qq(0.0f, 0.0f, 0.0f, 1.0f) : q4(d.b.rot[a]); }
+template <class M = Plain>
+__device__ __forceinline__ Q rotAOf(const Dev &d, int a) { return a < 0 ? qq(0.0f, 0.0f, 0.0f, 1.0f) : q4(M::ld(d.b.rot[a])); }
+template <class M = Plain>
 __device__ __forceinline__ V jointLinC(const Dev &d, const float4 *k, int a, int b) {
-  V anchor = a < 0 ? xyz(k[6]) : add(qrotate(q4(d.b.rot[a]), xyz(k[6])), xyz(d.b.pos[a]));
-  return sub(anchor, add(qrotate(q4(d.b.rot[b]), xyz(k[7])), xyz(d.b.pos[b])));
+  V anchor = a < 0 ? xyz(M::ld(k[6])) : add(qrotate(q4(M::ld(d.b.rot[a])), xyz(M::ld(k[6]))), xyz(M::ld(d.b.pos[a])));
+  return sub(anchor, add(qrotate(q4(M::ld(d.b.rot[b])), xyz(M::ld(k[7]))), xyz(M::ld(d.b.pos[b]))));
 }
+template <class M = Plain>
 __device__ __forceinline__ V jointAngC(const Dev &d, const float4 *k, int a, int b) {
-  Q qb = q4(d.b.rot[b]);
+  Q qb = q4(M::ld(d.b.rot[b]));
   float dd = dot4(qb, qb);
-  Q dq = qmul(qmul(rotAOf(d, a), q4(k[8])), qq(-qb.x / dd, -qb.y / dd, -qb.z / dd, qb.w / dd));
+  Q dq = qmul(qmul(rotAOf<M>(d, a), q4(M::ld(k[8]))), qq(-qb.x / dd, -qb.y / dd, -qb.z / dd, qb.w / dd));
   if (dq.w < 0.0f) dq = qq(-dq.x, -dq.y, -dq.z, -dq.w);
-  return mul(mul(vv(dq.x, dq.y, dq.z), 2.0f), k[3].w);
+  return mul(mul(vv(dq.x, dq.y, dq.z), 2.0f), M::ld(k[3]).w);
 }
+template <class M = Plain>
 __device__ __forceinline__ V jointLinRows(const Dev &d, const float4 *k, int a, int b, float alpha) {
-  return sub(jointLinC(d, k, a, b), k[0].w >= HARD ? mul(xyz(k[4]), alpha) : vv(0.0f, 0.0f, 0.0f));
+  return sub(jointLinC<M>(d, k, a, b), M::ld(k[0]).w >= HARD ? mul(xyz(M::ld(k[4])), alpha) : vv(0.0f, 0.0f, 0.0f));
 }
+template <class M = Plain>
 __device__ __forceinline__ V jointAngRows(const Dev &d, const float4 *k, int a, int b, float alpha) {
-  return sub(jointAngC(d, k, a, b), k[1].w >= HARD ? mul(xyz(k[5]), alpha) : vv(0.0f, 0.0f, 0.0f));
+  return sub(jointAngC<M>(d, k, a, b), M::ld(k[1]).w >= HARD ? mul(xyz(M::ld(k[5])), alpha) : vv(0.0f, 0.0f, 0.0f));
 }
 /** addJoint (wgsl-solve.ts:115-165): spring row, ball-socket rows with lumped geometric
  *  stiffness, angle-lock rows. */
+template <class M = Plain>
 __device__ __forceinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, float alpha, uint32_t i) {
   int4 in = d.jinfo[j];
   int a = in.y, b = in.z;
@@ -1023,20 +1082,20 @@ __device__ __forceinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, flo
   bool isA = (int)i == a;
   float sg = isA ? 1.0f : -1.0f;
   if (in.x == (int)T_SPRING) {
-    V rAW = qrotate(q4(d.b.rot[a]), xyz(k[6])), rBW = qrotate(q4(d.b.rot[b]), xyz(k[7]));
-    V dd = sub(add(rAW, xyz(d.b.pos[a])), add(rBW, xyz(d.b.pos[b])));
+    V rAW = qrotate(q4(M::ld(d.b.rot[a])), xyz(M::ld(k[6]))), rBW = qrotate(q4(M::ld(d.b.rot[b])), xyz(M::ld(k[7])));
+    V dd = sub(add(rAW, xyz(M::ld(d.b.pos[a]))), add(rBW, xyz(M::ld(d.b.pos[b]))));
     float len = length(dd);
     if (len <= 1.0e-6f) return;
     V n = divs(dd, len);
-    float st = k[0].w;
+    float st = M::ld(k[0]).w;
     V r = isA ? rAW : rBW;
-    addRow(acc, mul(n, sg), mul(cross(r, n), sg), st, st * (len - k[6].w));
+    addRow(acc, mul(n, sg), mul(cross(r, n), sg), st, st * (len - M::ld(k[6]).w));
     return;
   }
-  V penLin = xyz(k[0]);
+  V penLin = xyz(M::ld(k[0]));
   if (dot(penLin, penLin) > 0.0f) {
-    V F = add(mulv(penLin, jointLinRows(d, k, a, b, alpha)), xyz(k[2]));
-    V r = isA ? qrotate(q4(d.b.rot[a]), xyz(k[6])) : neg(qrotate(q4(d.b.rot[b]), xyz(k[7])));
+    V F = add(mulv(penLin, jointLinRows<M>(d, k, a, b, alpha)), xyz(M::ld(k[2])));
+    V r = isA ? qrotate(q4(M::ld(d.b.rot[a])), xyz(M::ld(k[6]))) : neg(qrotate(q4(M::ld(d.b.rot[b])), xyz(M::ld(k[7]))));
     V v = neg(r);
     addRow(acc, vv(sg, 0.0f, 0.0f), vv(0.0f, -v.z, v.y), penLin.x, F.x);
     addRow(acc, vv(0.0f, sg, 0.0f), vv(v.z, 0.0f, -v.x), penLin.y, F.y);
@@ -1047,10 +1106,11 @@ __device__ __forceinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, flo
     float g2 = length(sub(mul(r, F.z), vv(0.0f, 0.0f, rf)));
     acc.ang[0] += g0; acc.ang[1] += 0.0f; acc.ang[2] += g1; acc.ang[3] += 0.0f; acc.ang[4] += 0.0f; acc.ang[5] += g2;
   }
-  V penAng = xyz(k[1]);
+  V penAng = xyz(M::ld(k[1]));
   if (dot(penAng, penAng) > 0.0f) {
-    V F = add(mulv(penAng, jointAngRows(d, k, a, b, alpha)), xyz(k[3]));
-    float s = sg * k[3].w;
+    float4 lamAng = M::ld(k[3]);
+    V F = add(mulv(penAng, jointAngRows<M>(d, k, a, b, alpha)), xyz(lamAng));
+    float s = sg * lamAng.w;
     V z = vv(0.0f, 0.0f, 0.0f);
     addRow(acc, z, vv(s, 0.0f, 0.0f), penAng.x, F.x);
     addRow(acc, z, vv(0.0f, s, 0.0f), penAng.y, F.y);
@@ -1060,11 +1120,12 @@ __device__ __forceinline__ void addJoint(const Dev &d, Acc &acc, uint32_t j, flo
 /** A contact partner's pose, its rotation at the substep's start (x-, a sphere contact's frame) and
  *  its motion since then. */
 struct PairBody { Q rot, irot; V dLin, dAng; };
+template <class M = Plain>
 __device__ __forceinline__ PairBody pairBody(const Dev &d, uint32_t i) {
   PairBody P;
-  P.rot = q4(d.b.rot[i]);
-  P.irot = q4(d.b.irot[i]);
-  P.dLin = sub(xyz(d.b.pos[i]), xyz(d.b.ipos[i]));
+  P.rot = q4(M::ld(d.b.rot[i]));
+  P.irot = q4(M::ld(d.b.irot[i]));
+  P.dLin = sub(xyz(M::ld(d.b.pos[i])), xyz(M::ld(d.b.ipos[i])));
   P.dAng = qsub(P.rot, P.irot);
   return P;
 }
@@ -1161,8 +1222,9 @@ __device__ __forceinline__ void accumulate(const Dev &d, uint32_t i, uint32_t q0
   }
 }
 /** finishBody (wgsl-solve.ts:504-558): LDLᵀ in registers, pivots floored at 1e-5 of the diagonal;
- *  (pos, rot) is the body's pose as accumulate read it. */
-__device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &acc, float4 pos, float4 rot) {
+ *  (pos, rot) is the body's pose as accumulate read it, and returns the updated one. */
+template <class M = Plain>
+__device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &acc, float4 &pos, float4 &rot) {
   const float PF = 1e-5f;
   float A11 = acc.lin[0], A21 = acc.lin[1], A22 = acc.lin[2], A31 = acc.lin[3], A32 = acc.lin[4], A33 = acc.lin[5];
   float A41 = acc.cr[0], A42 = acc.cr[1], A43 = acc.cr[2], A44 = acc.ang[0];
@@ -1186,15 +1248,10 @@ __device__ __forceinline__ void finishBody(const Dev &d, uint32_t i, const Acc &
   float v3 = y3 / D3 - L43 * w1 - L53 * w2 - L63 * w3;
   float v2 = y2 / D2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
   float v1 = y1 / D1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
-  d.b.pos[i] = make_float4(pos.x - v1, pos.y - v2, pos.z - v3, pos.w);
-  d.b.rot[i] = f4q(qadd(q4(rot), neg(vv(w1, w2, w3))));
-}
-__device__ __forceinline__ uint32_t lanesFor(const Params &p, uint32_t count) {
-  if (count >= p.lanes1) return 1;
-  if (count >= p.lanes2) return 2;
-  if (count >= p.lanes4) return 4;
-  if (count >= p.lanes8) return 8;
-  return 16;  // small colours: their time is the longest adjacency walk, so split it further
+  pos = make_float4(pos.x - v1, pos.y - v2, pos.z - v3, pos.w);
+  rot = f4q(qadd(q4(rot), neg(vv(w1, w2, w3))));
+  M::st(d.b.pos[i], pos);
+  M::st(d.b.rot[i], rot);
 }
 /** v of the lane S places up in the same 16-lane row (DPP row_shl:S); lanes whose source leaves
  *  the row read 0. A body's lanes are an aligned group of at most 16 (one row), so lane j < S of a
@@ -1290,16 +1347,17 @@ __global__ void __launch_bounds__(512) k_primal_tail(Dev d, uint32_t c0, uint32_
     __syncthreads();
   }
 }
-/** dualJoint (wgsl-solve.ts:562-635) without the plastic (yield) extension. */
+/** dualJoint (wgsl-solve.ts:562-635): multipliers, penalty ramp, plastic yield and bend limit, fracture. */
+template <class M = Plain>
 __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha) {
   int4 in = d.jinfo[j];
   if (in.x != (int)T_JOINT) return;
   int a = in.y, b = in.z;
-  if ((a >= 0 && !bodyEnabled(d, (uint32_t)a)) || !bodyEnabled(d, (uint32_t)b)) return;
+  if ((a >= 0 && !enabledM<M>(d, (uint32_t)a)) || !enabledM<M>(d, (uint32_t)b)) return;
   float4 *k = d.joints + 10 * j;
-  float4 penLin = k[0], penAng = k[1], lamLin = k[2], lamAng = k[3];
+  float4 penLin = M::ld(k[0]), penAng = M::ld(k[1]), lamLin = M::ld(k[2]), lamAng = M::ld(k[3]);
   if (dot(xyz(penLin), xyz(penLin)) > 0.0f) {
-    V C = jointLinRows(d, k, a, b, alpha);
+    V C = jointLinRows<M>(d, k, a, b, alpha);
     if (penLin.w >= HARD) lamLin = f4(add(mulv(xyz(penLin), C), xyz(lamLin)), lamLin.w);
     float cap = mn(penLin.w, PENALTY_MAX);
     V np = add(xyz(penLin), mul(vabs(C), d.p.betaLin));
@@ -1308,12 +1366,12 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
   float frac = lamLin.w, limit = fabsf(frac);
   bool yielded = false;
   if (dot(xyz(penAng), xyz(penAng)) > 0.0f) {
-    V C = jointAngRows(d, k, a, b, alpha);
+    V C = jointAngRows<M>(d, k, a, b, alpha);
     if (penAng.w >= HARD) {
       lamAng = f4(add(mulv(xyz(penAng), C), xyz(lamAng)), lamAng.w);
       // a plastic joint (yield in c0Ang.w, wgsl-solve.ts:586-599): past yield, short of fracture,
       // the angular force is cut back to yield and the bend becomes the new rest
-      float yieldForce = k[5].w;
+      float yieldForce = M::ld(k[5]).w;
       if (yieldForce < BIG) {
         float force = length(xyz(lamAng));
         if (force > yieldForce && !(limit < BIG && force > limit)) {
@@ -1330,13 +1388,13 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
   }
   // the rest moves to where the bodies are; past the bend limit (rB.w) from the rest it started
   // with, the joint breaks (wgsl-solve.ts:606-614)
-  Q rest = q4(k[8]);
+  Q rest = q4(M::ld(k[8]));
   bool bentOff = false;
   if (yielded) {
-    rest = qnormalize(qmul(qconj(rotAOf(d, a)), q4(d.b.rot[b])));
-    if (k[7].w < BIG) {
-      Q dq = qmul(qconj(q4(k[9])), rest);
-      bentOff = 2.0f * atan2f(length(vv(dq.x, dq.y, dq.z)), fabsf(dq.w)) > k[7].w;
+    rest = qnormalize(qmul(qconj(rotAOf<M>(d, a)), q4(M::ld(d.b.rot[b]))));
+    if (M::ld(k[7]).w < BIG) {
+      Q dq = qmul(qconj(q4(M::ld(k[9]))), rest);
+      bentOff = 2.0f * atan2f(length(vv(dq.x, dq.y, dq.z)), fabsf(dq.w)) > M::ld(k[7]).w;
     }
   }
   bool linear = frac < 0.0f && dot(xyz(lamLin), xyz(lamLin)) > limit * limit;
@@ -1347,18 +1405,18 @@ __device__ __forceinline__ void dualJoint(const Dev &d, uint32_t j, float alpha)
     lamLin = make_float4(0.0f, 0.0f, 0.0f, frac);
     lamAng = make_float4(0.0f, 0.0f, 0.0f, lamAng.w);
   }
-  k[0] = penLin; k[1] = penAng; k[2] = lamLin; k[3] = lamAng;
+  M::st(k[0], penLin); M::st(k[1], penAng); M::st(k[2], lamLin); M::st(k[3], lamAng);
   if (yielded) {
-    k[8] = f4q(rest);
-    k[5] = make_float4(0.0f, 0.0f, 0.0f, k[5].w);
+    M::st(k[8], f4q(rest));
+    M::st(k[5], make_float4(0.0f, 0.0f, 0.0f, M::ld(k[5]).w));
   }
 }
 /** dualManifold's update of one contact point c (wgsl-solve.ts:637-662), given its manifold's basis,
  *  friction and both bodies' terms: penalty ramp, stick flag, λ. Written back: the record's upper
  *  32 bytes (penalties and multipliers) and, when the stick bit flipped, the key word. */
-__device__ __forceinline__ void dualContact(const Dev &d, uint32_t c, const V *basis, float friction, const PairBody &A, const PairBody &B,
-                                            float alpha) {
-  Contact k = d.con[c];
+template <class M = Plain>
+__device__ __forceinline__ void dualContactOf(const Dev &d, uint32_t c, const Contact &k, const V *basis, float friction, const PairBody &A,
+                                              const PairBody &B, float alpha) {
   Eval e = evalContact(k, basis, friction, A, B, alpha);
   if (d.bits) d.bits[c].z = evalBits(e);
   V pen = xyz(k.pen);
@@ -1374,9 +1432,14 @@ __device__ __forceinline__ void dualContact(const Dev &d, uint32_t c, const V *b
   // only what changes goes out: penalties and multipliers (the record's upper 32 bytes), and the
   // key word when its stick bit flipped (anchors and C0 stay)
   float4 *rec = reinterpret_cast<float4 *>(d.con + c);
-  if (key != __float_as_uint(k.rA.w)) reinterpret_cast<float *>(rec)[3] = __uint_as_float(key);
-  rec[2] = f4(pen, k.pen.w);
-  rec[3] = f4(e.F, k.lam.w);
+  if (key != __float_as_uint(k.rA.w)) M::stw(reinterpret_cast<float *>(rec)[3], __uint_as_float(key));
+  M::st(rec[2], f4(pen, k.pen.w));
+  M::st(rec[3], f4(e.F, k.lam.w));
+}
+template <class M = Plain>
+__device__ __forceinline__ void dualContact(const Dev &d, uint32_t c, const V *basis, float friction, const PairBody &A, const PairBody &B,
+                                            float alpha) {
+  dualContactOf<M>(d, c, loadContact<M>(d.con[c]), basis, friction, A, B, alpha);
 }
 /** dualManifold (wgsl-solve.ts:637-662) for contact point c alone: every point's update is its own,
  *  so a thread per point gives the manifold loop's results while no wavefront waits on its widest
