@@ -183,6 +183,11 @@ int avbd_set_hulls(avbd_solver *s, const uint32_t *words, uint32_t word_count);
 int avbd_adapt(avbd_solver *s, const avbd_counters *counters);
 /* contactCapacity / pairCapacity / manifoldCapacity (solver.ts) and the fixed ones. */
 int avbd_get_capacity(const avbd_solver *s, avbd_capacity *out);
+/* The broadphase configuration uploadStatics chose (solver.ts:1263-1293): cell size, the fine
+ * level's cell size and radius threshold (equal to the cell size and the large-body threshold when
+ * there is no fine level), whether the coarse level holds moving bodies, and the large bodies. */
+int avbd_broadphase_info(const avbd_solver *s, float *cell_size, float *fine_cell_size, float *fine_radius, uint32_t *coarse_movers,
+                         uint32_t *large_count);
 
 /* How the solve pass is scheduled (results are identical whichever runs). */
 typedef struct avbd_schedule {

@@ -96,7 +96,7 @@ EXPORTS = [
     "avbd_profile_kernels", "avbd_time_steps", "avbd_sync", "avbd_adapt", "avbd_set_schedule", "avbd_get_schedule",
     "avbd_autotune", "avbd_create_batch", "avbd_forward", "avbd_backward", "avbd_set_sparse_tail",
     "avbd_watch_create", "avbd_watch_set", "avbd_watch_run", "avbd_watch_read", "avbd_push", "avbd_move", "avbd_reset",
-    "avbd_get_capacity", "avbd_set_barrier_polls", "avbd_read_refs", "avbd_read_bodies_noalloc",
+    "avbd_get_capacity", "avbd_broadphase_info", "avbd_set_barrier_polls", "avbd_read_refs", "avbd_read_bodies_noalloc",
     "avbd_read_joint_info", "avbd_get_sizes", "avbd_add_bodies", "avbd_patch_body", "avbd_rewrite_fixed",
     "avbd_append_joints", "avbd_append_joint", "avbd_append_world_joint", "avbd_append_springs", "avbd_release_joints",
     "avbd_set_world_anchor", "avbd_disable_constraint", "avbd_set_body_aux", "avbd_set_lifecycle",
@@ -376,6 +376,16 @@ class GpuSolver3D:
         c = Capacity()
         self._check(self.L.avbd_get_capacity(self.h, ctypes.byref(c)))
         return {k: getattr(c, k) for k, _ in Capacity._fields_}
+
+    def broadphase_info(self):
+        """The grid uploadStatics chose: cell sizes of both levels, the fine radius threshold,
+        whether coarse cells hold movers, and the large bodies tested against every body."""
+        self._live()
+        f = [ctypes.c_float() for _ in range(3)]
+        u = [ctypes.c_uint32() for _ in range(2)]
+        self._check(self.L.avbd_broadphase_info(self.h, *[ctypes.byref(x) for x in f + u]))
+        return {"cell_size": f[0].value, "fine_cell_size": f[1].value, "fine_radius": f[2].value,
+                "coarse_movers": bool(u[0].value), "large": u[1].value}
 
     def set_graphs(self, on=True):
         """Per-substep hipGraphs: each substep's launch sequence captured once and replayed

@@ -119,6 +119,8 @@ struct avbd_solver {
   uint32_t tableSize = 0, hashSize = 0, colorGroups = 0, adjCap = 0;
   bool adjacencyHeld = false;  // the adjacency buffers hold the last substep's adjacency
   float cellSize = 1;
+  float fineCell = 1, fineRadius = 0;  // the grid's fine level (uploadStatics, solver.ts:1276-1290)
+  bool hasFine = false, coarseMovers = false, anyFixedSmall = false;
   std::vector<uint32_t> large;          // large bodies, grouped by world
   std::vector<uint32_t> worldStartHost;  // W + 1 body offsets
   uint32_t *worldOf = nullptr, *worldStart = nullptr, *largeStart = nullptr;
@@ -427,6 +429,7 @@ struct avbd_solver {
     d.worldOf = worldOf; d.worldStart = worldStart; d.largeStart = largeStart; d.largeRadius = largeRadiusW;
     d.worlds = (uint32_t)worldStartHost.size() - 1;
     d.cellSize = cellSize; d.tableMask = tableSize - 1;
+    d.fineCell = fineCell; d.fineRadius = fineRadius; d.hasFine = hasFine ? 1u : 0u; d.coarseMovers = coarseMovers ? 1u : 0u;
     d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
     d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs; d.hullOut = hullOut;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
@@ -499,6 +502,36 @@ struct avbd_solver {
     }
     lstart[W] = (uint32_t)large.size();
     cellSize = (float)std::max(maxCell, 1e-3);
+    // A fine level for small moving bodies much smaller than the small fixed ones (held slabs must
+    // not force them into slab-sized cells), as the reference's uploadStatics picks it
+    // (solver.ts:1274-1290); one world only (batched worlds share one cell size).
+    hasFine = coarseMovers = false;
+    anyFixedSmall = false;
+    for (uint32_t w = 0; w < W; w++)
+      for (uint32_t i = worldStartHost[w]; i < worldStartHost[w + 1]; i++)
+        if (!(mass[i] > 0) && radius[i] <= thr[w]) anyFixedSmall = true;
+    fineCell = cellSize;
+    fineRadius = W ? thr[0] : 0.0f;
+    if (W == 1 && n > 0) {
+      std::vector<double> r(radius.begin(), radius.begin() + n), moving;
+      for (uint32_t i = 0; i < n; i++) if (mass[i] > 0) moving.push_back(radius[i]);
+      double maxSmall = largestSmall(r);
+      double target = moving.empty() ? maxSmall : std::min(maxSmall, 2.0 * largestSmall(moving));
+      double fine = 0;
+      for (double x : r) if (x <= target) fine = std::max(fine, x);
+      double fc = cellSize, fr = thr[0];
+      if (fine > 0 && fine * 2 < maxSmall) {
+        double next = INFINITY;
+        for (double x : r) if (x > fine) next = std::min(next, x);
+        fc = std::max(2 * fine * (1 + 1e-4), 1e-3);
+        fr = (fine + next) / 2;
+      }
+      fineCell = (float)fc;
+      fineRadius = (float)fr;
+      hasFine = fineCell < cellSize;
+      for (uint32_t i = 0; i < n && hasFine; i++)
+        if (mass[i] > 0 && radius[i] > fineRadius && radius[i] <= thr[0]) { coarseMovers = true; break; }
+    }
     hipMemcpyAsync(largeRadiusW, thr.data(), W * 4ull, hipMemcpyHostToDevice, stream);
     hipMemcpyAsync(largeStart, lstart.data(), (W + 1) * 4ull, hipMemcpyHostToDevice, stream);
     std::vector<NC> nc = ignorePairs;
@@ -540,7 +573,7 @@ struct avbd_solver {
       scan(cellCount, cellStart, tableSize + 1);
       k_grid_scatter<<<blocksFor(n, 256), 256, 0, stream>>>(d);
       k_find_pairs<<<blocksFor((uint64_t)n * PROBE_LANES, 64), 64, 0, stream>>>(d);
-      if (anyDynamicLarge) k_find_fixed_large<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+      if (anyDynamicLarge || anyFixedSmall) k_find_fixed<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
     timed(K_NARROW, [&] {
       if (hullPairs) k_narrowphase<false><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
@@ -972,7 +1005,7 @@ struct avbd_solver {
     std::memset(&k, 0, sizeof k);
     k.d = dev();
     k.reuse = reuse; k.keep = keep; k.coopBlocks = coopBlocks; k.coopThreads = coopThreads; k.tailStart = tailStart;
-    k.barrierPolls = barrierPolls; k.hullBlocks = hullBlocks; k.anyLarge = anyDynamicLarge; k.fixedDev = fixedColors ? fixedDev : nullptr;
+    k.barrierPolls = barrierPolls; k.hullBlocks = hullBlocks; k.anyLarge = anyDynamicLarge || anyFixedSmall; k.fixedDev = fixedColors ? fixedDev : nullptr;
     k.coopFlow = coopFlow;
     return k;
   }
@@ -1546,6 +1579,16 @@ int avbd_get_capacity(const avbd_solver *s, avbd_capacity *out) {
   if (!s || !out) return AVBD_ERR_INVALID;
   out->bodies = s->bodyCap; out->joints = s->jointCap; out->pairs = s->pairCap; out->manifolds = s->manCap;
   out->contacts = s->conCap; out->hull_words = s->hullCap;
+  return AVBD_OK;
+}
+int avbd_broadphase_info(const avbd_solver *s, float *cell_size, float *fine_cell_size, float *fine_radius, uint32_t *coarse_movers,
+                         uint32_t *large_count) {
+  if (!s) return AVBD_ERR_INVALID;
+  if (cell_size) *cell_size = s->cellSize;
+  if (fine_cell_size) *fine_cell_size = s->hasFine ? s->fineCell : s->cellSize;
+  if (fine_radius) *fine_radius = s->fineRadius;
+  if (coarse_movers) *coarse_movers = s->coarseMovers ? 1u : 0u;
+  if (large_count) *large_count = (uint32_t)s->large.size();
   return AVBD_OK;
 }
 int avbd_set_schedule(avbd_solver *s, const avbd_schedule *sch) {

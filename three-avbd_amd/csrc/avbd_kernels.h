@@ -29,8 +29,11 @@ struct Dev {
   float *largeRadius;  // per world: bodies with a larger bounding radius are tested against all of theirs
   uint32_t worlds;
   const uint4 *hulls;
-  // uniform grid
-  float cellSize;
+  // uniform grid, two levels (uploadStatics, solver.ts:1263-1290): small bodies with a bounding
+  // radius above fineRadius use cellSize cells (level 1), the rest fineCell cells (level 0);
+  // hasFine 0: one level (level 0, cellSize cells). coarseMovers: level 1 holds dynamic bodies.
+  float cellSize, fineCell, fineRadius;
+  uint32_t hasFine, coarseMovers;
   uint32_t tableMask;
   uint32_t *cellCount, *cellStart, *cellCursor, *cellSorted;
   int4 *cellOf;
@@ -176,21 +179,31 @@ __device__ __forceinline__ int hashFind(const Dev &d, uint32_t a, uint32_t b) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Broadphase: uniform grid over bounding spheres (cells 2x the largest small radius), a list of
-// large bodies tested against every body, and the reference's exact pair predicate
-// (wgsl-collision.ts:150-234). The predicate decides; the grid only has to be conservative.
+// Broadphase: a hashed uniform grid over bounding spheres, in two levels when the scene's small
+// bodies separate by size (cells 2x the largest radius of each level), a list of large bodies
+// tested against every body, and the reference's exact pair predicate (wgsl-collision.ts:150-234).
+// The predicate decides; the grid only has to be conservative. A cell key is (x, y, z, slot) with
+// slot = world * 4 + level * 2 + fixed: moving and fixed bodies sit in separate cells of the table.
 __device__ __forceinline__ bool isLarge(const Dev &d, uint32_t i) { return d.b.moment[i].w > d.largeRadius[d.worldOf[i]]; }
-__device__ __forceinline__ int4 cellAt(const Dev &d, V pos) {
-  return make_int4((int)floorf(pos.x / d.cellSize), (int)floorf(pos.y / d.cellSize), (int)floorf(pos.z / d.cellSize), 0);
+__device__ __forceinline__ uint32_t levelOf(const Dev &d, uint32_t i) { return d.hasFine && d.b.moment[i].w > d.fineRadius ? 1u : 0u; }
+__device__ __forceinline__ int4 cellAt(const Dev &d, V pos, uint32_t level) {
+  float size = level || !d.hasFine ? d.cellSize : d.fineCell;
+  return make_int4((int)floorf(pos.x / size), (int)floorf(pos.y / size), (int)floorf(pos.z / size), 0);
 }
 __device__ __forceinline__ uint32_t cellHash(const Dev &d, int4 c) {
   return (((uint32_t)c.x * 73856093u) ^ ((uint32_t)c.y * 19349663u) ^ ((uint32_t)c.z * 83492791u) ^ ((uint32_t)c.w * 2654435769u)) & d.tableMask;
 }
+/** A small body's cell key at its own level. */
+__device__ __forceinline__ int4 ownCell(const Dev &d, uint32_t i) {
+  uint32_t level = levelOf(d, i);
+  int4 c = cellAt(d, xyz(d.b.pos[i]), level);
+  c.w = (int)(d.worldOf[i] * 4u + level * 2u + (d.b.size[i].w > 0.0f ? 0u : 1u));
+  return c;
+}
 __global__ void k_grid_count(Dev d) {
   GRID_STRIDE(i, d.n) {
     if (!bodyEnabled(d, i) || isLarge(d, i)) continue;
-    int4 c = cellAt(d, xyz(d.b.pos[i]));
-    c.w = (int)d.worldOf[i];
+    int4 c = ownCell(d, i);
     d.cellOf[i] = c;
     atomicAdd(&d.cellCount[cellHash(d, c)], 1u);
   }
@@ -198,13 +211,13 @@ __global__ void k_grid_count(Dev d) {
 __global__ void k_grid_scatter(Dev d) {
   GRID_STRIDE(i, d.n) {
     if (!bodyEnabled(d, i) || isLarge(d, i)) continue;
-    uint32_t h = cellHash(d, d.cellOf[i]);
+    int4 c = d.cellOf[i];
+    uint32_t h = cellHash(d, c);
     uint32_t slot = d.cellStart[h] + atomicAdd(&d.cellCursor[h], 1u);
     d.cellSorted[slot] = i;
     float4 p = d.b.pos[i];
     d.sortedPos[slot] = make_float4(p.x, p.y, p.z, d.b.moment[i].w);
-    int4 c = d.cellOf[i];
-    d.sortedKey[slot] = make_int4(c.x, c.y, c.z, (int)((uint32_t)c.w | (d.b.size[i].w > 0.0f ? 0x80000000u : 0u)));
+    d.sortedKey[slot] = c;
   }
 }
 
@@ -319,14 +332,50 @@ __device__ __forceinline__ void flushPairs(const Dev &d, PairQueue &q) {
   if (threadIdx.x == 0) q.count = 0;
   __syncthreads();
 }
-/** Threads per body of the broadphase probe: they split the 27 neighbour cells and the large list. */
+/** Threads per body of the broadphase probe: they split its cell probes and the large list. */
 #ifndef PROBE_LANES
 #define PROBE_LANES 4
 #endif
-/** Body at position k of the grid's cell order (the small bodies as scattered, then the large
- *  list), when dynamic, probes with PROBE_LANES threads: a small one the 27 cells around it (a
- *  same-cell dynamic pair from its higher index), every one the large list (a dynamic large-large
- *  pair once); lane l takes cells l, l + PROBE_LANES, ... and large-list entries alike.
+/** The cell offset of probe q of a half stencil: q = 0 the cell itself, 1..13 the neighbours after
+ *  it in (z, y, x) order. Of two neighbouring cells, exactly one has the other in its half. */
+__device__ __forceinline__ int4 halfOffset(uint32_t q) {
+  uint32_t l = q == 0 ? 13u : 13u + q;
+  return make_int4((int)(l % 3) - 1, (int)(l / 3 % 3) - 1, (int)(l / 9) - 1, 0);
+}
+__device__ __forceinline__ int4 fullOffset(uint32_t q) { return make_int4((int)(q % 3) - 1, (int)(q / 3 % 3) - 1, (int)(q / 9) - 1, 0); }
+/** Tests body P against the entries of cell `cell` (its hashed bucket holds other cells too, told
+ *  apart by key), four entries' key, position and body loaded together; `below`: only bodies of a
+ *  lower index (a pair within one cell is found once). */
+template <class Emit>
+__device__ __forceinline__ void probeCell(const Dev &d, const Probe &P, int4 cell, uint32_t from, uint32_t to, bool below, Emit emit) {
+  uint2 pair;
+  for (uint32_t e = from; e < to; e += 4) {
+    int4 key[4];
+    float4 pe[4];
+    uint32_t je[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      uint32_t x = e + k < to ? e + k : to - 1;
+      key[k] = d.sortedKey[x];
+      pe[k] = d.sortedPos[x];
+      je[k] = d.cellSorted[x];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      if (e + k >= to) break;
+      if (key[k].x != cell.x || key[k].y != cell.y || key[k].z != cell.z || key[k].w != cell.w) continue;
+      if (below && je[k] >= P.index) continue;
+      if (testPair(d, P, je[k], pe[k], pair)) emit(pair);
+    }
+  }
+}
+/** Body at position k of the grid's cell order (the small bodies as sorted, then the large list),
+ *  when dynamic, probes with PROBE_LANES threads (lane l takes probes l, l + PROBE_LANES, ...):
+ *  - a small one, at its own level, the moving bodies of its cell (lower indexes) and of the 13
+ *    cells of its half stencil; fixed small bodies find their movers themselves (k_find_fixed);
+ *  - a fine one, at the coarse level, every body (moving or fixed) of the 27 cells around it:
+ *    pairs across the levels belong to the finer body (wgsl-collision.ts:244-246);
+ *  - every one the large list (a dynamic large-large pair once).
  *  Neighbouring bodies take neighbouring threads, so a wavefront walks the same and adjacent cells. */
 __device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t k, uint32_t lane, uint32_t smallCount) {
   uint32_t i;
@@ -337,24 +386,39 @@ __device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t
   bool small = !isLarge(d, i);
   Probe P = probeOf(d, i);
   uint32_t w = d.worldOf[i];
-  uint2 pair;
+  auto emit = [&](uint2 pair) { queuePair(d, q, pair); };
   if (small) {
     int4 c = d.cellOf[i];
-    for (uint32_t q27 = lane; q27 < 27; q27 += PROBE_LANES) {
-      int ox = (int)(q27 % 3) - 1, oy = (int)(q27 / 3 % 3) - 1, oz = (int)(q27 / 9) - 1;
-      int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
-      uint32_t h = cellHash(d, cell);
-      uint32_t end = d.cellStart[h + 1];
-      for (uint32_t e = d.cellStart[h]; e < end; e++) {
-        int4 key = d.sortedKey[e];  // the cell check and the ordering rule from the sorted stream
-        if (key.x != cell.x || key.y != cell.y || key.z != cell.z || (key.w & 0x7fffffff) != cell.w) continue;
-        uint32_t j = d.cellSorted[e];
-        if (j == i) continue;
-        if (j > i && key.w < 0) continue;
-        if (testPair(d, P, j, d.sortedPos[e], pair)) queuePair(d, q, pair);
+    // own level: the half stencil's 14 cells, this lane's ranges loaded together first
+    const uint32_t NH = (14 + PROBE_LANES - 1) / PROBE_LANES;
+    uint32_t from[NH], to[NH];
+#pragma unroll
+    for (uint32_t u = 0; u < NH; u++) {
+      uint32_t p = lane + u * PROBE_LANES;
+      int4 o = halfOffset(p < 14 ? p : 0u);
+      uint32_t h = cellHash(d, make_int4(c.x + o.x, c.y + o.y, c.z + o.z, c.w));
+      from[u] = p < 14 ? d.cellStart[h] : 0u;
+      to[u] = p < 14 ? d.cellStart[h + 1] : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NH; u++) {
+      uint32_t p = lane + u * PROBE_LANES;
+      int4 o = halfOffset(p < 14 ? p : 0u);
+      if (from[u] < to[u]) probeCell(d, P, make_int4(c.x + o.x, c.y + o.y, c.z + o.z, c.w), from[u], to[u], p == 0, emit);
+    }
+    if (((c.w >> 1) & 1) == 0 && d.hasFine) {
+      // a fine body, at the coarse level: moving (r < 27) and fixed (r >= 27) bodies of 27 cells
+      int4 cc = cellAt(d, P.pos, 1);
+      for (uint32_t r = lane; r < 54; r += PROBE_LANES) {
+        int4 o = fullOffset(r % 27);
+        int4 cell = make_int4(cc.x + o.x, cc.y + o.y, cc.z + o.z, (int)(w * 4u + 2u + (r < 27 ? 0u : 1u)));
+        uint32_t h = cellHash(d, cell);
+        uint32_t f = d.cellStart[h], t = d.cellStart[h + 1];
+        if (f < t) probeCell(d, P, cell, f, t, false, emit);
       }
     }
   }
+  uint2 pair;
   for (uint32_t e = d.largeStart[w] + lane; e < d.largeStart[w + 1]; e += PROBE_LANES) {
     uint32_t l = d.statics[e];
     if (l == i || (!small && l < i && d.b.size[l].w > 0.0f)) continue;
@@ -376,19 +440,35 @@ __global__ void __launch_bounds__(64) k_find_pairs(Dev d) {
     flushPairs(d, q);
   }
 }
-/** Small fixed bodies against the large dynamic ones (findFixedLarge, wgsl-collision.ts:285). */
-__global__ void k_find_fixed_large(Dev d) {
+/** Fixed bodies find their moving partners: a small one the moving bodies of the 27 cells around
+ *  it at its own level and, when fine, at the coarse level (a coarse mover does not probe fine
+ *  cells: wgsl-collision.ts:288-307); every small one the large dynamic bodies (findFixedLarge,
+ *  wgsl-collision.ts:285-322). Fixed large bodies are every mover's large-list partners. */
+__global__ void k_find_fixed(Dev d) {
   GRID_STRIDE(i, d.n) {
     if (d.b.size[i].w > 0.0f || !bodyEnabled(d, i) || isLarge(d, i)) continue;
-    bool loaded = false;
-    Probe P;
+    Probe P = probeOf(d, i);
     uint32_t w = d.worldOf[i];
+    auto emit = [&](uint2 pair) { emitPair(d, pair.x, pair.y); };
+    int4 c = d.cellOf[i];
+    uint32_t level = ((uint32_t)c.w >> 1) & 1u;
+    for (uint32_t pass = 0; pass < 2; pass++) {
+      if (pass == 1 && !(level == 0 && d.hasFine && d.coarseMovers)) break;
+      int4 base = pass == 0 ? c : cellAt(d, P.pos, 1);
+      int slotW = (int)(w * 4u + (pass == 0 ? level : 1u) * 2u);
+      for (uint32_t q27 = 0; q27 < 27; q27++) {
+        int4 o = fullOffset(q27);
+        int4 cell = make_int4(base.x + o.x, base.y + o.y, base.z + o.z, slotW);
+        uint32_t h = cellHash(d, cell);
+        uint32_t from = d.cellStart[h], to = d.cellStart[h + 1];
+        if (from < to) probeCell(d, P, cell, from, to, false, emit);
+      }
+    }
+    uint2 pair;
     for (uint32_t k = d.largeStart[w]; k < d.largeStart[w + 1]; k++) {
       uint32_t l = d.statics[k];
       if (!bodyEnabled(d, l) || d.b.size[l].w <= 0.0f) continue;
-      if (!loaded) { P = probeOf(d, i); loaded = true; }
       float4 pl = d.b.pos[l];
-      uint2 pair;
       if (testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w), pair)) emitPair(d, pair.x, pair.y);
     }
   }
