@@ -167,7 +167,8 @@ struct avbd_solver {
   uint32_t *statics = nullptr;
   uint32_t staticsCap = 0;
   uint4 *hulls = nullptr;
-  uint32_t *cellCount = nullptr, *cellStart = nullptr, *cellCursor = nullptr, *cellSorted = nullptr;
+  uint32_t *cellStart = nullptr, *cellSorted = nullptr;
+  uint32_t *sortKey[2] = {nullptr, nullptr}, *sortIdx[2] = {nullptr, nullptr}, *sortHist = nullptr;
   int4 *cellOf = nullptr;
   float4 *sortedPos = nullptr;
   int4 *sortedKey = nullptr;
@@ -430,7 +431,8 @@ struct avbd_solver {
     d.worlds = (uint32_t)worldStartHost.size() - 1;
     d.cellSize = cellSize; d.tableMask = tableSize - 1;
     d.fineCell = fineCell; d.fineRadius = fineRadius; d.hasFine = hasFine ? 1u : 0u; d.coarseMovers = coarseMovers ? 1u : 0u;
-    d.cellCount = cellCount; d.cellStart = cellStart; d.cellCursor = cellCursor; d.cellSorted = cellSorted; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
+    d.cellStart = cellStart; d.cellSorted = cellSorted;
+    d.sortKey[0] = sortKey[0]; d.sortKey[1] = sortKey[1]; d.sortIdx[0] = sortIdx[0]; d.sortIdx[1] = sortIdx[1]; d.sortHist = sortHist; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
     d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs; d.hullOut = hullOut;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
     d.con = con[parity]; d.pcon = con[1 - parity]; d.cman = cman; d.manCap = manCap; d.conCap = conCap;
@@ -567,11 +569,7 @@ struct avbd_solver {
     fill(table, hashSize, 0);
     k_hash_insert<<<strideBlocks(manCap, 256), 256, 0, stream>>>(d);
     timed(K_BROAD, [&] {
-      fill(cellCount, tableSize + 1, 0);
-      fill(cellCursor, tableSize, 0);
-      k_grid_count<<<blocksFor(n, 256), 256, 0, stream>>>(d);
-      scan(cellCount, cellStart, tableSize + 1);
-      k_grid_scatter<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+      sortCells(d);
       k_find_pairs<<<blocksFor((uint64_t)n * PROBE_LANES, 64), 64, 0, stream>>>(d);
       if (anyDynamicLarge || anyFixedSmall) k_find_fixed<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
@@ -583,6 +581,28 @@ struct avbd_solver {
         k_hull_emit<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);
       }
     });
+  }
+  /** The grid's cell order by an LSD radix sort of the bodies' cell hashes (the hash bits plus one
+   *  for bodies outside the grid, in passes of at most 10 bits), then the buckets' starts and the
+   *  sorted streams. */
+  void sortCells(Dev &d) {
+    uint32_t keyBits = 1;
+    while ((1ull << keyBits) <= tableSize) keyBits++;  // keys 0 .. tableSize
+    uint32_t blocks = (n + 255) / 256, maxBits = 10;
+    // the histogram's scan takes at most 4096 x 1024 entries (k_scan_sums: one workgroup)
+    while (maxBits > 1 && ((uint64_t)blocks << maxBits) + 1 > 4096ull * 1024ull) maxBits--;
+    uint32_t passes = (keyBits + maxBits - 1) / maxBits, bits = (keyBits + passes - 1) / passes;
+    k_grid_keys<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    uint32_t src = 0;
+    for (uint32_t p = 0, shift = 0; p < passes && n; p++, shift += bits) {
+      uint32_t b = std::min(bits, keyBits - shift);
+      k_radix<false><<<blocks, 256, 0, stream>>>(sortKey[src], sortIdx[src], sortKey[1 - src], sortIdx[1 - src], sortHist, n, shift, b);
+      scan(sortHist, sortHist, (blocks << b) + 1);
+      k_radix<true><<<blocks, 256, 0, stream>>>(sortKey[src], sortIdx[src], sortKey[1 - src], sortIdx[1 - src], sortHist, n, shift, b);
+      src = 1 - src;
+    }
+    k_cell_starts<<<blocksFor(tableSize + 1ull, 256), 256, 0, stream>>>(d, sortKey[src]);
+    k_grid_place<<<blocksFor(n, 256), 256, 0, stream>>>(d, sortKey[src], sortIdx[src]);
   }
   void topology() {
     Dev d = dev();
@@ -1201,9 +1221,12 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     s->jinfo = s->alloc<int4>(s->jointCap);
     s->flowOrphan = s->alloc<uint32_t>(s->jointCap);
     s->hulls = s->alloc<uint4>(s->hullCap / 4 + 1);
-    s->cellCount = s->alloc<uint32_t>(s->tableSize + 1);
     s->cellStart = s->alloc<uint32_t>(s->tableSize + 1);
-    s->cellCursor = s->alloc<uint32_t>(s->tableSize);
+    for (int k = 0; k < 2; k++) {
+      s->sortKey[k] = s->alloc<uint32_t>(N);
+      s->sortIdx[k] = s->alloc<uint32_t>(N);
+    }
+    s->sortHist = s->alloc<uint32_t>(1024ull * ((N + 255) / 256) + 1);
     s->cellSorted = s->alloc<uint32_t>(N);
     s->cellOf = s->alloc<int4>(N);
     s->sortedPos = s->alloc<float4>(N);

@@ -35,7 +35,10 @@ struct Dev {
   float cellSize, fineCell, fineRadius;
   uint32_t hasFine, coarseMovers;
   uint32_t tableMask;
-  uint32_t *cellCount, *cellStart, *cellCursor, *cellSorted;
+  uint32_t *cellStart, *cellSorted;
+  // the cell sort: keys (cell hashes; tableMask + 1 for bodies not in the grid) and body indexes,
+  // ping-pong, and the per-block digit histograms
+  uint32_t *sortKey[2], *sortIdx[2], *sortHist;
   int4 *cellOf;
   float4 *sortedPos;  // pos.xyz, bounding radius of cellSorted[k]: the candidate scan's stream
   int4 *sortedKey;    // cell xyz, world | 0x80000000 when dynamic, of cellSorted[k]
@@ -200,24 +203,79 @@ __device__ __forceinline__ int4 ownCell(const Dev &d, uint32_t i) {
   c.w = (int)(d.worldOf[i] * 4u + level * 2u + (d.b.size[i].w > 0.0f ? 0u : 1u));
   return c;
 }
-__global__ void k_grid_count(Dev d) {
+/** The grid's sort keys: each small enabled body's cell (at its level) and the cell's hash, the
+ *  rest a key past every hash; body indexes as the payload. */
+__global__ void k_grid_keys(Dev d) {
   GRID_STRIDE(i, d.n) {
-    if (!bodyEnabled(d, i) || isLarge(d, i)) continue;
-    int4 c = ownCell(d, i);
-    d.cellOf[i] = c;
-    atomicAdd(&d.cellCount[cellHash(d, c)], 1u);
+    uint32_t key = d.tableMask + 1u;
+    if (bodyEnabled(d, i) && !isLarge(d, i)) {
+      int4 c = ownCell(d, i);
+      d.cellOf[i] = c;
+      key = cellHash(d, c);
+    }
+    d.sortKey[0][i] = key;
+    d.sortIdx[0][i] = i;
   }
 }
-__global__ void k_grid_scatter(Dev d) {
-  GRID_STRIDE(i, d.n) {
-    if (!bodyEnabled(d, i) || isLarge(d, i)) continue;
-    int4 c = d.cellOf[i];
-    uint32_t h = cellHash(d, c);
-    uint32_t slot = d.cellStart[h] + atomicAdd(&d.cellCursor[h], 1u);
-    d.cellSorted[slot] = i;
+/** One pass of an LSD radix sort of (key, index) over `bits`-bit digits at `shift` (at most 10
+ *  bits), 256 elements per workgroup staged through LDS. Each element's rank among the wavefront's
+ *  elements of its digit comes from one ballot per digit bit (the lanes agreeing on every bit);
+ *  the wavefronts' digit counts sit in LDS. Without SCATTER the workgroup writes its digit
+ *  histogram (digit-major, for an exclusive scan over digits and workgroups); with SCATTER each
+ *  element goes to the scanned offset of its digit and workgroup plus its rank in the workgroup.
+ *  Equal digits keep their order: the sort is stable, so after the last pass bodies are ordered
+ *  by key and, within a key, by index. */
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) k_radix(const uint32_t *kin, const uint32_t *iin, uint32_t *kout, uint32_t *iout, uint32_t *hist,
+                                               uint32_t n, uint32_t shift, uint32_t bits) {
+  __shared__ uint32_t waveCount[4][1024];
+  const uint32_t bins = 1u << bits, t = threadIdx.x, w = t >> 6, lane = t & 63u, e = blockIdx.x * 256u + t;
+  for (uint32_t k = t; k < 4u * bins; k += 256u) waveCount[k >> bits][k & (bins - 1u)] = 0u;
+  __syncthreads();
+  bool valid = e < n;
+  uint32_t key = valid ? kin[e] : 0u, digit = (key >> shift) & (bins - 1u);
+  uint64_t same = __ballot(valid);
+  for (uint32_t b = 0; b < bits; b++) {
+    uint64_t ones = __ballot((digit >> b) & 1u);
+    same &= ((digit >> b) & 1u) ? ones : ~ones;
+  }
+  uint32_t rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+  if (valid && rank == 0u) waveCount[w][digit] = (uint32_t)__popcll(same);
+  __syncthreads();
+  if (!SCATTER) {
+    for (uint32_t k = t; k < bins; k += 256u) hist[k * gridDim.x + blockIdx.x] = waveCount[0][k] + waveCount[1][k] + waveCount[2][k] + waveCount[3][k];
+    if (blockIdx.x == 0 && t == 0) hist[bins * gridDim.x] = 0u;
+    return;
+  }
+  if (valid) {
+    uint32_t at = hist[digit * gridDim.x + blockIdx.x] + rank;
+    for (uint32_t v = 0; v < w; v++) at += waveCount[v][digit];
+    kout[at] = key;
+    iout[at] = iin[e];
+  }
+}
+/** Every hash bucket's start in the sorted keys: the first position holding a key at least the
+ *  bucket (a binary search each), so bucket h holds [start[h], start[h + 1]) and start[tableMask + 1]
+ *  is the count of bodies in the grid. */
+__global__ void k_cell_starts(Dev d, const uint32_t *ks) {
+  GRID_STRIDE(h, d.tableMask + 2u) {
+    uint32_t lo = 0, hi = d.n;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (ks[mid] < h) lo = mid + 1u; else hi = mid;
+    }
+    d.cellStart[h] = lo;
+  }
+}
+/** The grid's streams in sorted order: body, position and radius, cell key. */
+__global__ void k_grid_place(Dev d, const uint32_t *ks, const uint32_t *is) {
+  GRID_STRIDE(k, d.n) {
+    if (ks[k] > d.tableMask) continue;
+    uint32_t i = is[k];
+    d.cellSorted[k] = i;
     float4 p = d.b.pos[i];
-    d.sortedPos[slot] = make_float4(p.x, p.y, p.z, d.b.moment[i].w);
-    d.sortedKey[slot] = c;
+    d.sortedPos[k] = make_float4(p.x, p.y, p.z, d.b.moment[i].w);
+    d.sortedKey[k] = d.cellOf[i];
   }
 }
 
