@@ -174,6 +174,7 @@ struct avbd_solver {
   int4 *sortedKey = nullptr;
   uint2 *pairs = nullptr;
   uint2 *hullPairs = nullptr;  // the narrowphase's deferred hull pairs
+  uint2 *boxPairs = nullptr;   // and box-box pairs
   float4 *hullOut = nullptr;   // and their contacts for the emit pass
   uint4 *mIds[2] = {nullptr, nullptr};
   float4 *mGeo[2] = {nullptr, nullptr};
@@ -389,6 +390,7 @@ struct avbd_solver {
     fwdWhy = "the solver's storage grew after avbd_forward";
     pairs = regrow(pairs, pairCap, nPairs, false);
     if (hullPairs) hullPairs = regrow(hullPairs, pairCap, nPairs, false);
+    if (boxPairs) boxPairs = regrow(boxPairs, pairCap, nPairs, false);
     if (hullOut) hullOut = regrow(hullOut, (size_t)HULL_OUT * pairCap, (size_t)HULL_OUT * nPairs, false);
     for (int k = 0; k < 2; k++) {
       mIds[k] = regrow(mIds[k], manCap, nMan, true);
@@ -433,7 +435,7 @@ struct avbd_solver {
     d.fineCell = fineCell; d.fineRadius = fineRadius; d.hasFine = hasFine ? 1u : 0u; d.coarseMovers = coarseMovers ? 1u : 0u;
     d.cellStart = cellStart; d.cellSorted = cellSorted;
     d.sortKey[0] = sortKey[0]; d.sortKey[1] = sortKey[1]; d.sortIdx[0] = sortIdx[0]; d.sortIdx[1] = sortIdx[1]; d.sortHist = sortHist; d.cellOf = cellOf; d.sortedPos = sortedPos; d.sortedKey = sortedKey;
-    d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs; d.hullOut = hullOut;
+    d.pairs = pairs; d.pairCap = pairCap; d.hullPairs = hullPairs; d.hullOut = hullOut; d.boxPairs = boxPairs;
     d.mIds = mIds[parity]; d.pmIds = mIds[1 - parity]; d.mGeo = mGeo[parity]; d.pmGeo = mGeo[1 - parity];
     d.con = con[parity]; d.pcon = con[1 - parity]; d.cman = cman; d.manCap = manCap; d.conCap = conCap;
     d.table = table; d.hashMask = hashSize - 1;
@@ -576,6 +578,7 @@ struct avbd_solver {
     timed(K_NARROW, [&] {
       if (hullPairs) k_narrowphase<false><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
       else k_narrowphase<true><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
+      if (boxPairs) k_narrow_box<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);
       if (hullPairs) {
         k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
         k_hull_emit<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);
@@ -1235,6 +1238,7 @@ int avbd_create_batch(const avbd_params *params, const float *bodies, uint32_t b
     if (getenv("AVBD_HULL_INLINE") == nullptr) {
       s->hullPairs = s->alloc<uint2>(s->pairCap);
       s->hullOut = s->alloc<float4>((size_t)HULL_OUT * s->pairCap);
+      s->boxPairs = s->alloc<uint2>(s->pairCap);
     }
     for (int k = 0; k < 2; k++) {
       s->mIds[k] = s->alloc<uint4>(s->manCap);

@@ -93,6 +93,7 @@ enum { C_PAIRS = 0, C_CONTACTS = 1, C_PREV_CONTACTS = 2, C_OVERFLOW = 3, C_CLASH
        C_TOPO = 15,
        C_BARRIER = 16,  // cooperative solve: grid-barrier waits given up since the host last cleared it
        C_STEP = 17,     // substeps since creation (the contact stamps' clock), advanced by k_begin
+       C_BOX_PAIRS = 18,  // box-box pairs the narrowphase deferred to k_narrow_box
        C_WORDS = 20 };
 
 /** One contact point: 4 float4, the w's carrying the key and C(x-) (layout.ts:123-127). */

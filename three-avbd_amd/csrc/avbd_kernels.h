@@ -51,6 +51,7 @@ struct Dev {
   uint32_t *cman;  // per contact of this step, its manifold (the dual's thread-per-contact map)
   int *src;  // when recording: per contact, the previous step's contact it warm-started from (-1: none)
   uint2 *hullPairs;  // pairs with a hull on either side, for the wavefront-per-pair narrowphase (null: inline)
+  uint2 *boxPairs;   // box-box pairs, collided in wavefronts of their own (null: inline)
   float4 *hullOut;   // per hull pair, its contact for the emit pass (HULL_OUT float4)
   uchar4 *bits;  // when recording: this iteration's branch bits per contact (x: body a's update, y: b's, z: dual)
   uint32_t manCap, conCap;
@@ -118,6 +119,7 @@ __global__ void k_begin(Dev d, int reuseManifolds, int mayKeepAdjacency) {
   c[C_TOPO] &= ~1u;  // the adjacency is rebuilt unless this substep's reuse keeps it
   if (!mayKeepAdjacency) c[C_TOPO] |= 2u;  // the host's word: rebuild (buffers regrown, or a lone step)
   c[C_HULL_PAIRS] = 0;
+  c[C_BOX_PAIRS] = 0;
   c[C_CLASHES] = 0;
   c[C_NUM_COLORS] = 0;
   // this step's colouring marks its clashing colours afresh (colorStarts clears them,
@@ -693,12 +695,13 @@ __device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_
 }
 
 /** narrowphase (wgsl-collision.ts:767) of every broadphase pair, one thread per pair: kept points
- *  (contact reuse), a deferral to the hull kernel, or a fresh manifold. The slots of all three
- *  come from one atomic per wavefront each (waveAlloc) at a point every lane reaches: same-address
- *  atomics per lane serialise across the chip. Which slot a pair lands in is arbitrary either way.
- *  INLINE_HULLS: hull pairs are collided here by one thread each (the solver built without the
- *  hull list); otherwise they go to the list for k_narrow_hull (a wavefront per pair), and this
- *  kernel carries only the box and sphere code and its private memory. */
+ *  (contact reuse), a deferral, or a fresh manifold. The slots of all of them come from one atomic
+ *  per wavefront each (waveAlloc) at a point every lane reaches: same-address atomics per lane
+ *  serialise across the chip. Which slot a pair lands in is arbitrary either way.
+ *  INLINE_HULLS: every pair is collided here by one thread (the solver built without the hull
+ *  list). Otherwise hull pairs go to the list for k_narrow_hull (a wavefront per pair) and box-box
+ *  pairs to the list for k_narrow_box, and this kernel carries the sphere pairs and the reuse
+ *  alone: no private memory, and no wavefront runs the box clip for a few of its lanes. */
 template <bool INLINE_HULLS>
 __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
   uint32_t count = d.counters[C_PAIRS];
@@ -711,7 +714,7 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
     uint2 pr = live ? d.pairs[p] : make_uint2(0u, 0u);
     uint32_t a = pr.x, b = pr.y;
     int pm = live && reuseOn ? reusableManifold(d, a, b) : -1;
-    bool defer = false, sphereA = false, sphereB = false;
+    bool defer = false, deferBox = false, sphereA = false, sphereB = false;
     float4 posA = make_float4(0.0f, 0.0f, 0.0f, 0.0f), posB = posA;
     Q qA = qq(0.0f, 0.0f, 0.0f, 1.0f), qB = qA;
     Box A, B;
@@ -726,6 +729,8 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
       bool hullA = shA >= SHAPE_HULL, hullB = shB >= SHAPE_HULL;
       if ((hullA || hullB) && !INLINE_HULLS) {
         defer = true;
+      } else if (!INLINE_HULLS && !sphereA && !sphereB) {
+        deferBox = true;  // the box SAT and clip, in wavefronts of box pairs alone (k_narrow_box)
       } else {
         qA = q4(d.b.rot[a]); qB = q4(d.b.rot[b]);
         A = makeBox(xyz(posA), qA, xyz(d.b.size[a])); B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
@@ -742,18 +747,52 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
             sat.n = neg(h.n);
             if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
           }
-        } else collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
+        } else if (INLINE_HULLS) collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
       }
     }
     uint4 pids = pm >= 0 ? d.pmIds[pm] : make_uint4(0u, 0u, 0u, 0u);
     bool wants = pm >= 0 || found.count > 0;
     uint32_t points = pm >= 0 ? (pids.w & 15u) : found.count;
     uint32_t hslot = waveAlloc(&d.counters[C_HULL_PAIRS], defer ? 1u : 0u);
+    uint32_t bslot = INLINE_HULLS ? 0u : waveAlloc(&d.counters[C_BOX_PAIRS], deferBox ? 1u : 0u);
     uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], wants ? 1u : 0u);
     uint32_t base = waveAlloc(&d.counters[C_CONTACTS], wants && m < d.manCap ? points : 0u);
     if (defer) d.hullPairs[hslot] = pr;
+    if (deferBox) d.boxPairs[bslot] = pr;
     if (pm >= 0) copyContactsAt(d, pids, d.pmGeo[pm], m, base);
     else if (found.count > 0) emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, sphereA || sphereB, sat, found);
+  }
+}
+
+/** The box-box pairs the narrowphase deferred: collideBoxes a thread per pair as it runs inline,
+ *  the wavefronts now all taking the SAT and clip path (mixed with sphere pairs, every wavefront
+ *  ran every shape's path). */
+__global__ void __launch_bounds__(256) k_narrow_box(Dev d) {
+  uint32_t count = d.counters[C_BOX_PAIRS];
+  if (count > d.pairCap) count = d.pairCap;
+  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
+  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
+    uint32_t p = w + (threadIdx.x & 63u);
+    bool live = p < count;
+    uint2 pr = live ? d.boxPairs[p] : make_uint2(0u, 0u);
+    uint32_t a = pr.x, b = pr.y;
+    float4 posA = make_float4(0.0f, 0.0f, 0.0f, 0.0f), posB = posA;
+    Q qA = qq(0.0f, 0.0f, 0.0f, 1.0f), qB = qA;
+    Box A, B;
+    Sat sat;
+    sat.n = vv(0.0f, 0.0f, 0.0f);
+    Found found;
+    found.count = 0;
+    if (live) {
+      posA = d.b.pos[a]; posB = d.b.pos[b];
+      qA = q4(d.b.rot[a]); qB = q4(d.b.rot[b]);
+      A = makeBox(xyz(posA), qA, xyz(d.b.size[a])); B = makeBox(xyz(posB), qB, xyz(d.b.size[b]));
+      collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
+    }
+    bool wants = found.count > 0;
+    uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], wants ? 1u : 0u);
+    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], wants && m < d.manCap ? found.count : 0u);
+    if (wants) emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, false, sat, found);
   }
 }
 
