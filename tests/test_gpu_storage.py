@@ -1,12 +1,13 @@
 """adapt's capacity half (solver.ts:2362-2374) and fixed storage on the GPU: pairs, manifolds and
 contacts grow before they overflow, keeping the warm-start contacts (a solver that grew steps
-exactly as one that was large from the start), and a fixed-storage solver keeps its capacities
-and counts what did not fit."""
+exactly as one that was large from the start, and as the oracle seeded from its state after the
+growth), and a fixed-storage solver keeps its capacities and counts what did not fit."""
 import numpy as np
 import pytest
 
 import avbd_amd
-from avbd_amd import scenes
+from avbd_amd import records, scenes
+from helpers import oracle_from_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -57,9 +58,12 @@ def test_growth_keeps_the_warm_starts(gpu):
     a.adapt(forced)
     b.adapt(forced)
     assert a.capacity()["contacts"] > before["contacts"] and b.capacity()["contacts"] == 120000
+    m = oracle_from_gpu(a)  # the grown buffers' contents are the warm starts the next step reads
     for _ in range(15):
         a.step()
         b.step()
+        m.step(1)
     assert np.array_equal(a.read_bodies().view(np.uint32), b.read_bodies().view(np.uint32))
+    assert np.array_equal(records.bodies_to_oracle(a.read_bodies()), m.bodies())
     a.destroy()
     b.destroy()
