@@ -137,6 +137,27 @@ def cpu_baseline(solver, hulls, substeps, threads, budget_s=10.0, max_substeps=4
     return done, dt
 
 
+def f64_reference_baseline(bodies, budget_s=6.0, max_steps=2000):
+    """The reference's f64 sequential CPU solver (src/avbd3d/ref/solver.ts, restated in the oracle)
+    on one host thread, for scenes of boxes only (it has no other shape), from the scene's initial
+    state: bodies by size, density, friction and pose, the reference's default parameters with
+    gravity -9.81. Returns (steps, seconds)."""
+    import numpy as np
+    import oracle
+    ref = oracle.RefSolver()
+    ref.set_params(dt=1 / 60, gravity=-9.81, iterations=10)
+    for b in np.asarray(bodies, np.float64):
+        size = b[16:19]
+        density = b[19] / float(np.prod(size)) if b[19] > 0 else 0.0
+        ref.add_body(size, density, b[3], b[0:3], rot=b[4:8], vel=b[32:35], angvel=b[36:39])
+    done = 0
+    t0 = time.perf_counter()
+    while done < max_steps and time.perf_counter() - t0 < budget_s:
+        ref.step(10)
+        done += 10
+    return done, time.perf_counter() - t0
+
+
 def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
@@ -571,9 +592,17 @@ def bench_step(args, workload):
             "gpu_over_cpu": value / allcore,
             "one_thread": {"value": n * done1 / dt1, "cores": 1, "gpu_over_cpu": value * dt1 / (n * done1),
                            "sample": f"{done1} substeps on one host thread, {dt1:.1f} s"},
-            "f64_sequential": "not run: the reference's f64 CPU solver (src/avbd3d/ref) has boxes only; "
-                              "this scene has spheres and hulls",
         }
+        if workload == "pyramid-55":
+            steps64, dt64 = f64_reference_baseline(bodies)
+            out["cpu_baseline"]["f64_sequential"] = {
+                "value": n * steps64 / dt64, "unit": "body-substeps/s", "cores": 1, "kind": "reference",
+                "sample": f"{steps64} steps of the pyramid from rest through the reference's f64 sequential solver "
+                          f"(src/avbd3d/ref, restated in the oracle), one host thread, {dt64:.1f} s",
+                "gpu_over_cpu": value * dt64 / (n * steps64)}
+        else:
+            out["cpu_baseline"]["f64_sequential"] = ("not run: the reference's f64 CPU solver (src/avbd3d/ref) has boxes "
+                                                     "only; this scene has spheres and hulls")
         log(f"cpu baseline: {allcore:.0f} body-substeps/s on {info_h['threads']} threads "
             f"({info_h['cpu_model']}), {n * done1 / dt1:.0f} on one")
     solver.destroy()

@@ -576,12 +576,12 @@ struct avbd_solver {
       if (anyDynamicLarge || anyFixedSmall) k_find_fixed<<<blocksFor(n, 256), 256, 0, stream>>>(d);
     });
     timed(K_NARROW, [&] {
-      if (hullPairs) k_narrowphase<false><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
-      else k_narrowphase<true><<<strideBlocks(pairCap, 64), 64, 0, stream>>>(d);
+      if (hullPairs) k_narrowphase<false><<<strideBlocks(pairCap, 1024), 1024, 0, stream>>>(d);
+      else k_narrowphase<true><<<strideBlocks(pairCap, 1024), 1024, 0, stream>>>(d);
       if (boxPairs) k_narrow_box<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);
       if (hullPairs) {
         k_narrow_hull<<<blocksFor(64ull * pairCap, 256, hullBlocks), 256, 0, stream>>>(d);
-        k_hull_emit<<<strideBlocks(pairCap, 256), 256, 0, stream>>>(d);
+        k_hull_emit<<<strideBlocks(pairCap, 1024), 1024, 0, stream>>>(d);
       }
     });
   }

@@ -486,7 +486,8 @@ __device__ __forceinline__ void findPairsOf(const Dev &d, PairQueue &q, uint32_t
     if (testPair(d, P, l, make_float4(pl.x, pl.y, pl.z, d.b.moment[l].w), pair)) queuePair(d, q, pair);
   }
 }
-/** Launched with 64-thread workgroups over PROBE_LANES threads per body. */
+/** Launched with 64-thread workgroups over PROBE_LANES threads per body (1024-thread workgroups,
+ *  with a sixteenth of the queue atomics, took longer: every pass waits for its slowest wavefront). */
 __global__ void __launch_bounds__(64) k_find_pairs(Dev d) {
   __shared__ PairQueue q;
   if (threadIdx.x == 0) q.count = 0;
@@ -559,6 +560,37 @@ __device__ __forceinline__ uint32_t waveAlloc(uint32_t *counter, uint32_t v) {
   uint32_t total, pre = waveExclusiveSum(v, total), base = 0;
   if ((threadIdx.x & 63u) == 0 && total) base = atomicAdd(counter, total);
   return __shfl(base, 0, 64) + pre;
+}
+/** Bump allocation for a whole workgroup (at most 16 wavefronts): v[k] slots of counter k per
+ *  thread, k < K, with one atomic per counter per workgroup (the wavefronts' totals prefixed in LDS);
+ *  each thread gets its first slots in out[k], threads in order. Every thread must take part. A
+ *  returning atomic on one word from every wavefront of a pass serialises across the chip: at 100k
+ *  the narrowphase's four counters cost a few hundred microseconds that way. */
+template <uint32_t K>
+__device__ __forceinline__ void blockAlloc(uint32_t *const *counter, const uint32_t *v, uint32_t *out) {
+  __shared__ uint32_t wsum[K][17];
+  const uint32_t nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint32_t pre[K];
+#pragma unroll
+  for (uint32_t k = 0; k < K; k++) {
+    uint32_t total;
+    pre[k] = waveExclusiveSum(v[k], total);
+    if (lane == 0) wsum[k][w] = total;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    uint32_t k = threadIdx.x, sum = 0;
+    for (uint32_t x = 0; x < nw; x++) {
+      uint32_t t = wsum[k][x];
+      wsum[k][x] = sum;
+      sum += t;
+    }
+    wsum[k][16] = sum ? atomicAdd(counter[k], sum) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < K; k++) out[k] = wsum[k][16] + wsum[k][w] + pre[k];
+  __syncthreads();  // the words are reused by the next call
 }
 /** copyContacts (wgsl-collision.ts:721-755) with the manifold slot m and contact range base already
  *  allocated. */
@@ -703,13 +735,12 @@ __device__ __forceinline__ void emitManifoldAt(const Dev &d, uint32_t m, uint32_
  *  pairs to the list for k_narrow_box, and this kernel carries the sphere pairs and the reuse
  *  alone: no private memory, and no wavefront runs the box clip for a few of its lanes. */
 template <bool INLINE_HULLS>
-__global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
+__global__ void __launch_bounds__(1024) k_narrowphase(Dev d) {
   uint32_t count = d.counters[C_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
   bool reuseOn = (d.p.flags & FLAG_REUSE_CONTACTS) != 0;
-  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
-  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
-    uint32_t p = w + (threadIdx.x & 63u);
+  for (uint32_t w = blockIdx.x * blockDim.x; w < count; w += gridDim.x * blockDim.x) {  // workgroup-uniform
+    uint32_t p = w + threadIdx.x;
     bool live = p < count;
     uint2 pr = live ? d.pairs[p] : make_uint2(0u, 0u);
     uint32_t a = pr.x, b = pr.y;
@@ -753,10 +784,15 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
     uint4 pids = pm >= 0 ? d.pmIds[pm] : make_uint4(0u, 0u, 0u, 0u);
     bool wants = pm >= 0 || found.count > 0;
     uint32_t points = pm >= 0 ? (pids.w & 15u) : found.count;
-    uint32_t hslot = waveAlloc(&d.counters[C_HULL_PAIRS], defer ? 1u : 0u);
-    uint32_t bslot = INLINE_HULLS ? 0u : waveAlloc(&d.counters[C_BOX_PAIRS], deferBox ? 1u : 0u);
-    uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], wants ? 1u : 0u);
-    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], wants && m < d.manCap ? points : 0u);
+    uint32_t *const ctr[3] = {&d.counters[C_HULL_PAIRS], &d.counters[C_BOX_PAIRS], &d.counters[C_MANIFOLDS]};
+    const uint32_t need[3] = {defer ? 1u : 0u, deferBox ? 1u : 0u, wants ? 1u : 0u};
+    uint32_t got[3];
+    blockAlloc<3>(ctr, need, got);
+    uint32_t hslot = got[0], bslot = got[1], m = got[2];
+    uint32_t *const cctr[1] = {&d.counters[C_CONTACTS]};
+    const uint32_t cneed[1] = {wants && m < d.manCap ? points : 0u};
+    uint32_t base;
+    blockAlloc<1>(cctr, cneed, &base);
     if (defer) d.hullPairs[hslot] = pr;
     if (deferBox) d.boxPairs[bslot] = pr;
     if (pm >= 0) copyContactsAt(d, pids, d.pmGeo[pm], m, base);
@@ -770,9 +806,8 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev d) {
 __global__ void __launch_bounds__(256) k_narrow_box(Dev d) {
   uint32_t count = d.counters[C_BOX_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
-  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
-  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
-    uint32_t p = w + (threadIdx.x & 63u);
+  for (uint32_t w = blockIdx.x * blockDim.x; w < count; w += gridDim.x * blockDim.x) {  // workgroup-uniform
+    uint32_t p = w + threadIdx.x;
     bool live = p < count;
     uint2 pr = live ? d.boxPairs[p] : make_uint2(0u, 0u);
     uint32_t a = pr.x, b = pr.y;
@@ -790,8 +825,12 @@ __global__ void __launch_bounds__(256) k_narrow_box(Dev d) {
       collideBoxes(A, B, (d.p.flags & FLAG_FACE_BIAS) != 0, sat, found);
     }
     bool wants = found.count > 0;
-    uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], wants ? 1u : 0u);
-    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], wants && m < d.manCap ? found.count : 0u);
+    uint32_t *const mctr[1] = {&d.counters[C_MANIFOLDS]}, *const cctr[1] = {&d.counters[C_CONTACTS]};
+    const uint32_t mneed[1] = {wants ? 1u : 0u};
+    uint32_t m, base;
+    blockAlloc<1>(mctr, mneed, &m);
+    const uint32_t cneed[1] = {wants && m < d.manCap ? found.count : 0u};
+    blockAlloc<1>(cctr, cneed, &base);
     if (wants) emitManifoldAt(d, m, base, a, b, A, B, qA, qB, posA, posB, false, sat, found);
   }
 }
@@ -835,17 +874,20 @@ __global__ void __launch_bounds__(256) k_narrow_hull(Dev d) {
  *  rebuilt from the same body data as in k_narrow_hull, the contact as it left it. Slots come
  *  from one atomic per wavefront each (waveAlloc): an atomic per pair from the pair's wavefront
  *  cost about a quarter of the hull kernel. */
-__global__ void __launch_bounds__(256) k_hull_emit(Dev d) {
+__global__ void __launch_bounds__(1024) k_hull_emit(Dev d) {
   uint32_t count = d.counters[C_HULL_PAIRS];
   if (count > d.pairCap) count = d.pairCap;
-  uint32_t first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u, stride = gridDim.x * blockDim.x;
-  for (uint32_t w = first; w < count; w += stride) {  // wavefront-uniform: every lane takes part
-    uint32_t p = w + (threadIdx.x & 63u);
+  for (uint32_t w = blockIdx.x * blockDim.x; w < count; w += gridDim.x * blockDim.x) {  // workgroup-uniform
+    uint32_t p = w + threadIdx.x;
     const float4 *o = d.hullOut + (size_t)p * HULL_OUT;
     float4 h0 = p < count ? o[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t points = p < count ? __float_as_uint(h0.w) : 0u;
-    uint32_t m = waveAlloc(&d.counters[C_MANIFOLDS], points > 0 ? 1u : 0u);
-    uint32_t base = waveAlloc(&d.counters[C_CONTACTS], points > 0 && m < d.manCap ? points : 0u);
+    uint32_t *const mctr[1] = {&d.counters[C_MANIFOLDS]}, *const cctr[1] = {&d.counters[C_CONTACTS]};
+    const uint32_t mneed[1] = {points > 0 ? 1u : 0u};
+    uint32_t m, base;
+    blockAlloc<1>(mctr, mneed, &m);
+    const uint32_t cneed[1] = {points > 0 && m < d.manCap ? points : 0u};
+    blockAlloc<1>(cctr, cneed, &base);
     if (points == 0) continue;
     uint2 pr = d.hullPairs[p];
     uint32_t a = pr.x, b = pr.y;
