@@ -508,8 +508,8 @@ def bench_step(args, workload):
     if tune_total and args.schedule == "auto":
         schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
     else:  # counter-collection runs distort the autotuner's timings: fix the schedule instead
-        if args.schedule == "flow":
-            solver.set_schedule(cooperative=2, threads=64, blocks_per_cu=args.flow_blocks_per_cu)
+        if args.schedule in ("flow", "tail"):
+            solver.set_schedule(cooperative=2 if args.schedule == "flow" else 3, threads=64, blocks_per_cu=args.flow_blocks_per_cu)
         else:
             solver.set_schedule(cooperative=int(args.schedule == "coop"), threads=512, blocks_per_cu=1)
         schedule, tuned_ms = solver.schedule, solver.time_steps(tune_total, substeps) / max(1, tune_total)
@@ -623,7 +623,7 @@ def main():
     ap.add_argument("--color-cap-adjoint", type=int, default=16)
     ap.add_argument("--primal-lanes", type=int, nargs=4, default=[2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11],
                     help="colour sizes from which 1, 2, 4, 8 threads per body are used in the primal (else 16)")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop", "flow"],
+    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop", "flow", "tail"],
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--flow-blocks-per-cu", type=int, default=8, help="--schedule flow: workgroups per CU of the dataflow sweep")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")

@@ -214,8 +214,10 @@ int avbd_set_barrier_polls(avbd_solver *s, uint32_t polls);
  * order, instead of a launch each. The reference starts that tail at colour 16 for scenes of at
  * most ~1k bodies. Here the start is chosen: a colour index, 64 for none, or AVBD_TAIL_AUTO
  * (the default), which picks the first of the trailing colours that held <= 32 bodies at each
- * adaptive-colouring readback (AVBD_FLAG_ADAPT_COLORS; without that flag, no tail). Results are
- * the same whichever runs. */
+ * adaptive-colouring readback (AVBD_FLAG_ADAPT_COLORS; without that flag, no tail). Under schedule
+ * 3 the same start opens the dataflow tail (the colours from it and the dual in one cooperative
+ * launch per iteration); its automatic start is the first of the trailing colours that held <= 2048
+ * bodies. Results are the same whichever runs. */
 #define AVBD_TAIL_AUTO 0xFFFFFFFFu
 int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour);
 

@@ -213,3 +213,89 @@ def test_flow_give_up_is_reported_and_the_solver_drops_to_per_colour_launches(gp
     same_bodies(s, m)
     no_give_up(s)
     s.destroy()
+
+
+TAIL = dict(cooperative=3, threads=64, blocks_per_cu=4)
+
+
+@pytest.mark.parametrize("first", [0, 1, 3, 64])
+def test_tail_pile_matches(gpu, first):
+    """Schedule 3: colours before `first` launched one by one, the rest and the dual in the
+    dataflow tail (0: the whole sweep; 64: only the dual)."""
+    say(f"case: dataflow tail from colour {first}, pile 400 x 12 steps, eight threads")
+    bodies = scenes.random_pile(400, seed=4, bin_size=10.0)[0]
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18, 0), color_cap=64)
+    pk = dict(kw, gravity=-9.81)
+    pk["primal_lanes"] = pk.pop("lanes")
+    s = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
+    s.set_schedule(**TAIL)
+    s.set_sparse_tail(first)
+    m = model_for(bodies, kw)
+    for _ in range(12):
+        s.step()
+        m.step(1)
+    same_bodies(s, m)
+    same_contacts(s, m)
+    assert s.schedule["cooperative"] == 3
+    assert s.read_counters()["overflow"] & avbd_amd.solver.OVERFLOW_BARRIER == 0
+    s.destroy()
+
+
+@pytest.mark.parametrize("first", [1, 2])
+def test_tail_chains_reuse_and_breaking_joints(gpu, first):
+    say(f"case: dataflow tail from colour {first}, 4 substeps with reuse, breakable chains, 4 steps")
+    bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
+    joints = joints.copy()
+    joints[1::4, 11] = -4.0
+    kw = dict(lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11), color_cap=32, dt=1 / 240)
+    pk = dict(kw, gravity=-9.81)
+    pk["primal_lanes"] = pk.pop("lanes")
+    s = avbd_amd.GpuSolver3D(bodies, joints, info, params=avbd_amd.default_params(**pk), hulls=hw)
+    s.set_schedule(**TAIL)
+    s.set_sparse_tail(first)
+    m = model_for(bodies, kw, joints, info)
+    m.set_hulls(hw)
+    for _ in range(4):
+        s.steps(1, 4)
+        for u in range(4):
+            m.step(1, reuse_manifolds=u > 0)
+    assert (m.joints()[:, 3] != 0).sum() == len(joints) - 2
+    same_bodies(s, m)
+    assert np.array_equal(s.read_joints().astype(np.float64), m.joints().astype(np.float32).astype(np.float64))
+    same_contacts(s, m)
+    s.destroy()
+
+
+def test_tail_clashing_colours_and_adaptive_start(gpu):
+    """A clashing last colour inside the tail (pyramid at cap 2, tail from colour 1), then the
+    automatic start under adaptive colouring (readbacks every 10 steps) on a pile."""
+    say("case: dataflow tail, pyramid at colour cap 2 from colour 1; then a pile with the automatic start")
+    bodies = scenes.pyramid(10)[0]
+    rule = (1 << 20, 1 << 19, 1 << 18, 0)
+    kw = dict(lanes=rule, color_cap=2)
+    pk = dict(kw, gravity=-9.81)
+    pk["primal_lanes"] = pk.pop("lanes")
+    s = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
+    s.set_schedule(**TAIL)
+    s.set_sparse_tail(1)
+    m = model_for(bodies, kw)
+    for _ in range(6):
+        s.step()
+        m.step(1)
+    assert s.read_counters()["clashes"] > 0
+    same_bodies(s, m)
+    s.destroy()
+    bodies = scenes.random_pile(1000, seed=9, bin_size=14.0)[0]
+    pk = dict(gravity=-9.81, color_cap=16, primal_lanes=(2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11),
+              flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS)
+    a = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
+    b = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
+    a.set_schedule(**TAIL)
+    b.set_schedule(cooperative=0, threads=512, blocks_per_cu=1)
+    for _ in range(3):
+        a.steps(10)
+        b.steps(10)
+    assert np.array_equal(a.read_bodies().view(np.uint32), b.read_bodies().view(np.uint32))
+    assert a.schedule["cooperative"] == 3
+    a.destroy()
+    b.destroy()

@@ -500,7 +500,8 @@ class GpuSolver3D:
         return events, dropped.value, bool(full.value)
 
     def set_sparse_tail(self, first_colour=TAIL_AUTO):
-        """fuseSparseColors (solver.ts:574): colours from first_colour up run in one workgroup (64: none)."""
+        """fuseSparseColors (solver.ts:574): colours from first_colour up run in one workgroup (64: none);
+        under schedule 3 they and the dual run in the dataflow tail."""
         self._live()
         self._check(self.L.avbd_set_sparse_tail(self.h, ctypes.c_uint32(int(first_colour))))
 

@@ -142,6 +142,8 @@ struct avbd_solver {
   uint32_t barrierGiveUps = 0;       // give-ups seen by the host since avbd_read_counters last reported them
   bool barrierFailed = false;        // seen during the current avbd_step/avbd_steps call
   bool coopFlow = false;        // the cooperative launch is the dataflow sweep (k_flow), not k_solve
+  bool coopTail = false;        // per-colour launches up to flowTailStart, then one dataflow tail (k_flow_tail) per iteration
+  uint32_t flowTailStart = MAX_COLORS;  // chosen at each adaptive-colouring readback
   uint32_t coopThreads = 512;   // threads per workgroup of the solve kernel
   uint32_t coopPerCU = 1;       // its workgroups per CU
   uint32_t coopMaxBlocks = 0;   // co-resident limit for coopThreads (0: no cooperative launch here)
@@ -301,13 +303,14 @@ struct avbd_solver {
   }
   int applySchedule(const avbd_schedule &sch) {
     uint32_t t = sch.threads;
-    if (sch.cooperative == 2) {
-      // the dataflow sweep: one wavefront per workgroup, all co-resident
+    if (sch.cooperative == 2 || sch.cooperative == 3) {
+      // the dataflow sweep (or tail): one wavefront per workgroup, all co-resident
       if (t != 64 || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
       int perCU = 0;
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_flow, 64, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, sch.cooperative == 2 ? (const void *)k_flow : (const void *)k_flow_tail, 64, 0);
       if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
-      coopFlow = true;
+      coopFlow = sch.cooperative == 2;
+      coopTail = sch.cooperative == 3;
       coopThreads = 64;
       coopPerCU = sch.blocks_per_cu;
       coopBlocks = coopPerCU * numCUs;
@@ -317,18 +320,20 @@ struct avbd_solver {
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, solveKernel(t), (int)t, 0);
       if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
       coopFlow = false;
+      coopTail = false;
       coopThreads = t;
       coopPerCU = sch.blocks_per_cu;
       coopBlocks = coopPerCU * numCUs;
     } else if (sch.cooperative == 0) {
       coopBlocks = 0;
       coopFlow = false;
+      coopTail = false;
     } else {
       return AVBD_ERR_INVALID;
     }
     return AVBD_OK;
   }
-  avbd_schedule schedule() const { return avbd_schedule{coopBlocks ? (coopFlow ? 2u : 1u) : 0u, coopThreads, coopPerCU}; }
+  avbd_schedule schedule() const { return avbd_schedule{coopBlocks ? (coopFlow ? 2u : (coopTail ? 3u : 1u)) : 0u, coopThreads, coopPerCU}; }
   /** The fused tail starts where every later colour held at most TAIL_BODIES bodies at this
    *  readback (the cap's empty spares included). One workgroup runs on one CU, so only small
    *  colours gain: a 55-box pyramid sweeps in one launch per iteration (1.12 -> 0.61 ms per
@@ -342,6 +347,15 @@ struct avbd_solver {
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap, c = std::min(cap, MAX_COLORS);
     while (c > 0 && starts[c] - starts[c - 1] <= TAIL_BODIES) c--;
     tailStart = c;
+  }
+  /** The dataflow tail starts where every later colour held at most FLOW_TAIL_BODIES bodies at this
+   *  readback: a colour that small costs a launch for little more than one body's dependent chain,
+   *  which the tail overlaps with the next colour's and with the dual. */
+  void chooseFlowTail(const uint32_t *starts) {
+    const uint32_t FLOW_TAIL_BODIES = 2048;
+    uint32_t cap = fixedColors ? fixedCap : hp.color_cap, c = std::min(cap, MAX_COLORS);
+    while (c > 0 && starts[c] - starts[c - 1] <= FLOW_TAIL_BODIES) c--;
+    flowTailStart = c;
   }
   /** adaptColors (solver.ts:2379-2407). */
   void adapt(const avbd_counters &c) {
@@ -655,7 +669,7 @@ struct avbd_solver {
       // a launch refused here leaves the per-colour sweep below everything it needs
       if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
       k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
-      k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d);
+      k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d, 0u);
       fill(flowDone, n, 0u);
       void *args[] = {&d, &barrierPolls};
       hipError_t e;
@@ -671,7 +685,7 @@ struct avbd_solver {
       sweepColors(d);
       return;
     }
-    if (coopBlocks && !rec) {
+    if (coopBlocks && !coopTail && !rec) {
       fill(barrier, 1, 0u);  // the arrival count starts each launch at zero
       void *args[] = {&d, &barrier, &barrierPolls};
       hipError_t e;
@@ -683,7 +697,48 @@ struct avbd_solver {
     }
     if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
     k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    if (coopBlocks && coopTail && !rec && sweepWithTail(d)) return;
     sweepColors(d);
+  }
+  /** The iterations as per-colour launches of the colours before flowTailStart, then one dataflow
+   *  tail per iteration (the later colours and the dual), then the velocities. False when the tail's
+   *  first launch was refused before anything of the sweep ran (the caller sweeps colour by colour
+   *  instead). */
+  bool sweepWithTail(Dev &d) {
+    uint32_t cap = fixedColors ? fixedCap : hp.color_cap, first = std::min(tailAuto ? flowTailStart : tailStart, cap);
+    uint32_t pb = blocksFor(n, 64);
+    k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d, flowFirstKey(first));
+    fill(flowDone, n, 0u);
+    for (uint32_t it = 0; it < hp.iterations; it++) {
+      for (uint32_t c = 0; c < first; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
+      // A plain launch: the grid is at most what the occupancy check found resident on every CU, and
+      // on the solver's stream the kernel starts once the one before it has ended, so every
+      // workgroup is resident together (a cooperative launch, which guarantees the same, cost ~12 us
+      // of idle device before each of these 40 launches a step); a wait that still found a
+      // workgroup missing would give up, be reported and drop the schedule, never hang.
+      hipError_t e;
+      timed(K_DUAL, [&] {
+        k_flow_tail<<<coopBlocks, 64, 0, stream>>>(d, first, it, barrierPolls);
+        e = hipGetLastError();
+      });
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (capturing) { captureFailed = true; return true; }
+        if (it == 0 && first == 0) return false;  // nothing of the sweep ran yet
+        // the colours before the tail of this iteration ran: finish the iteration, then sweep on
+        for (uint32_t c = first; c < cap; c++) k_primal<<<pb, 64, 0, stream>>>(d, c);
+        k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d);
+        coopBlocks = 0;
+        coopTail = false;
+        for (uint32_t rest = it + 1; rest < hp.iterations; rest++) {
+          for (uint32_t c = 0; c < cap; c++) k_primal<<<pb, 64, 0, stream>>>(d, c);
+          k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d);
+        }
+        break;
+      }
+    }
+    k_velocities<<<blocksFor(n, 256), 256, 0, stream>>>(d);
+    return true;
   }
   /** The iterations colour by colour (one launch per colour, the dual, per iteration), then the
    *  velocities; the warm starts have run. */
@@ -1001,6 +1056,7 @@ struct avbd_solver {
     barrierFailed = true;
     coopBlocks = 0;
     coopFlow = false;
+    coopTail = false;
     fill(counters + C_BARRIER, 1, 0u);
     err = "the cooperative solve's grid barrier gave up waiting (" + std::to_string(c[C_BARRIER]) +
           " waits): the solve of the steps since the last readback ran unordered; the solver now uses per-colour launches";
@@ -1015,7 +1071,7 @@ struct avbd_solver {
    *  capacities, params) and the choices taken outside them. Equal bytes, equal launches. */
   struct GraphKey {
     Dev d;
-    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge, coopFlow;
+    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge, coopFlow, coopTail, flowTailStart;
     const uint32_t *fixedDev;
   };
   struct GraphEntry { GraphKey key; hipGraphExec_t exec; uint64_t used; };
@@ -1029,7 +1085,7 @@ struct avbd_solver {
     k.d = dev();
     k.reuse = reuse; k.keep = keep; k.coopBlocks = coopBlocks; k.coopThreads = coopThreads; k.tailStart = tailStart;
     k.barrierPolls = barrierPolls; k.hullBlocks = hullBlocks; k.anyLarge = anyDynamicLarge || anyFixedSmall; k.fixedDev = fixedColors ? fixedDev : nullptr;
-    k.coopFlow = coopFlow;
+    k.coopFlow = coopFlow; k.coopTail = coopTail; k.flowTailStart = flowTailStart;
     return k;
   }
   void dropGraphs() {
@@ -1103,6 +1159,7 @@ struct avbd_solver {
         adapt(k);
         adaptStorage(k);
         if (tailAuto) chooseTail(starts);
+        chooseFlowTail(starts);
       }
     }
   }
@@ -1752,7 +1809,7 @@ int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour) {
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms) {
   if (!s || substeps == 0 || steps == 0) return AVBD_ERR_INVALID;
   const avbd_schedule cands[] = {{0, 512, 1}, {1, 256, 1}, {1, 512, 1}, {1, 1024, 1}, {1, 256, 2}, {1, 512, 2},
-                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}};
+                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}, {3, 64, 4}, {3, 64, 8}};
   const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
   float bestT[sizeof(cands) / sizeof(cands[0])];
   for (int k = 0; k < nc; k++) bestT[k] = INFINITY;

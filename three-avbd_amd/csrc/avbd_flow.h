@@ -63,16 +63,19 @@ __device__ __forceinline__ bool flowOrphan(const Dev &d, uint32_t j) {
 }
 
 /** Before the sweep: each walk item's flags from the bodies' order keys (and a joint item's other
- *  body in .z), and which joints no body walks. */
-__global__ void __launch_bounds__(256) k_flow_prep(Dev d) {
+ *  body in .z), and which joints no body walks. With firstKey, the sweep starts at the colour that
+ *  key opens (the dataflow tail): bodies ordered before it are final when it starts, so their
+ *  items need no flags and their partners no waits. */
+__global__ void __launch_bounds__(256) k_flow_prep(Dev d, uint32_t firstKey) {
   GRID_STRIDE(i, d.n) {
     if (!isDynamic(d, (int)i)) continue;
     uint32_t key = d.bodyKey[i];
+    if (key < firstKey) continue;
     for (uint32_t q = d.itemStart[i], q1 = d.itemStart[i + 1]; q < q1; q++) {
       uint4 it = d.items[q];
       int other = d.adjOther[it.y];
       uint32_t f = it.w & 3u;
-      if (other >= 0) {
+      if (other >= 0 && d.bodyKey[other] >= firstKey) {
         uint32_t ko = d.bodyKey[other];
         f |= FLOW_DEP;
         if (ko < key) f |= FLOW_LOWER | FLOW_OWN;
@@ -266,6 +269,59 @@ __global__ void __launch_bounds__(64) k_flow(Dev d, uint32_t polls) {
       for (uint32_t k = (w + W - g % W) % W; k < chunks; k += W) flowChunk(d, start, count, lanes, serial, k, it, polls);
       g += chunks;
     }
+  }
+}
+
+/** The order key a sweep starting at colour `first` starts from (past every key: no colour). */
+__host__ __device__ __forceinline__ uint32_t flowFirstKey(uint32_t first) { return first >= 64u ? 0xffffffffu : first << 26; }
+/** The later of a constraint's bodies in the sweep's order (a fixed body counts as first). */
+__device__ __forceinline__ uint32_t flowOwnerKey(const Dev &d, int a, int b) {
+  uint32_t ka = a >= 0 && isDynamic(d, a) ? d.bodyKey[a] : 0u, kb = b >= 0 && isDynamic(d, b) ? d.bodyKey[b] : 0u;
+  return ka > kb ? ka : kb;
+}
+/** The dataflow tail of one iteration, after the per-colour launches of colours [0, first): the
+ *  colours from `first` as the dataflow sweep runs them (each body waiting only for its neighbours
+ *  ordered before it in these colours, the duals it owns after its update), and beside them the
+ *  duals of the constraints whose bodies all come before `first` (final when the launch starts) and
+ *  of the joints no body walks. One launch replaces the trailing colours' launches and the dual's;
+ *  the reads are the colour-by-colour sweep's. Workgroups without a chunk of the trailing colours
+ *  take those duals (when every workgroup has a chunk, all take them after it): they read bodies
+ *  and records nothing in the launch writes, through the caches as k_dual does. Completion counts
+ *  accumulate over the substep's iterations (cleared before its first). */
+__global__ void __launch_bounds__(64) k_flow_tail(Dev d, uint32_t first, uint32_t iter, uint32_t polls) {
+  const uint32_t W = gridDim.x, w = blockIdx.x, firstKey = flowFirstKey(first);
+  uint32_t colors = d.counters[C_NUM_COLORS];
+  if (colors > d.p.colorCap) colors = d.p.colorCap;
+  uint32_t g = 0;
+  for (uint32_t c = first; c < colors; c++) {
+    uint32_t start = d.colorStart[c], count = d.colorStart[c + 1] - start;
+    uint32_t lanes = lanesFor(d.p, count), per = 64u / lanes;
+    uint32_t chunks = (count + per - 1u) / per;
+    bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
+    for (uint32_t k = (w + W - g % W) % W; k < chunks; k += W) flowChunk(d, start, count, lanes, serial, k, iter, polls);
+    g += chunks;
+  }
+  // the other duals: on the workgroups the chunks left free (chunk k went to workgroup k mod W)
+  uint32_t busy = g < W ? g : 0u;
+  if (w < busy) return;
+  uint32_t t0 = (w - busy) * 64u + threadIdx.x, nt = (W - busy) * 64u;
+  for (uint32_t j = t0; j < d.nj; j += nt) {
+    int4 in = d.jinfo[j];
+    if (d.flowOrphan[j] || flowOwnerKey(d, in.y, in.z) < firstKey) dualJoint(d, j, d.p.alpha);
+  }
+  uint32_t points = d.counters[C_CONTACTS];
+  if (points > d.conCap) points = d.conCap;
+  for (uint32_t c = t0; c < points; c += nt) {
+    uint32_t m = d.cman[c];
+    if (m >= manifoldCount(d)) continue;
+    uint4 ids = d.mIds[m];
+    if (c < ids.z || c >= ids.z + (ids.w & 15u)) continue;
+    if (flowOwnerKey(d, (int)ids.x, (int)ids.y) >= firstKey) continue;  // the later body's, in the sweep above
+    float4 geo = d.mGeo[m];
+    PairBody A = pairBody(d, ids.x), B = pairBody(d, ids.y);
+    V basis[3];
+    orthonormal(xyz(geo), basis);
+    dualContact(d, c, basis, geo.w, A, B, d.p.alpha);
   }
 }
 
