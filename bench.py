@@ -531,10 +531,20 @@ def bench_step(args, workload):
     # roofline of the dominant kernel: HIP events around every launch of it, then bytes from counters
     ks = solver.profile_kernels(2, substeps)
     c2 = solver.read_counters()
+    m2, k2, j2 = c2["manifolds"], c2["contacts"], len(joints)
     if ks["solve_launches"]:
         avg_ms = ks["solve_ms"] / ks["solve_launches"]
-        kernel = "k_solve (cooperative: warm starts, iterations x (primal per colour, dual), velocities)"
-        nbytes = solve_bytes(n, c2["manifolds"], c2["contacts"], len(joints), iters)
+        if schedule.get("cooperative") == 2:
+            kernel = "k_flow (cooperative dataflow: warm starts, iterations x (primal, owned duals), velocities)"
+        else:
+            kernel = "k_solve (cooperative: warm starts, iterations x (primal per colour, dual), velocities)"
+        nbytes = solve_bytes(n, m2, k2, j2, iters)
+    elif ks["primal_launches"] == 0 and ks["tail_launches"] == 0 and ks["dual_launches"]:
+        # dataflow tail from the first colour: one k_flow_tail launch per iteration runs the whole
+        # sweep and every dual (timed with the duals)
+        avg_ms = ks["dual_ms"] / ks["dual_launches"]
+        kernel = "k_flow_tail (dataflow sweep over every colour plus all duals, one launch per iteration)"
+        nbytes = solve_bytes(n, m2, k2, j2, 1) - (368 * n + 304 * j2)
     elif ks["primal_launches"] == 0 and ks["tail_launches"]:
         # every colour small enough for the fused tail: one workgroup launch per sweep
         avg_ms = ks["tail_ms"] / ks["tail_launches"]
@@ -547,6 +557,8 @@ def bench_step(args, workload):
         kernel = "k_primal (one colour of the sweep; trailing small colours run fused in k_primal_tail)"
         per_sweep = ks["primal_launches"] / (2 * substeps * iters)
         nbytes = primal_sweep_bytes(n, c2["manifolds"], c2["contacts"], len(joints)) / max(1.0, per_sweep)
+    if avg_ms <= 0.0:
+        raise SystemExit(f"bench: no timed launch of the solve kernels in the profiled steps ({ks})")
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
 
     value = n * args.steps * substeps / wall
@@ -576,7 +588,8 @@ def bench_step(args, workload):
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
             "traffic_source": f"profiles/{workload}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)",
             "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
-            "launches": ks["solve_launches"] or ks["primal_launches"] or ks["tail_launches"], "solve_blocks": ks["solve_blocks"],
+            "launches": ks["solve_launches"] or ks["primal_launches"] or ks["tail_launches"] or ks["dual_launches"],
+            "solve_blocks": ks["solve_blocks"],
         },
     }
     if not args.no_cpu_baseline:
@@ -596,7 +609,7 @@ def bench_step(args, workload):
         if workload == "pyramid-55":
             steps64, dt64 = f64_reference_baseline(bodies)
             out["cpu_baseline"]["f64_sequential"] = {
-                "value": n * steps64 / dt64, "unit": "body-substeps/s", "cores": 1, "kind": "reference",
+                "value": n * steps64 / dt64, "unit": "body-substeps/s", "cores": 1, "kind": "port",
                 "sample": f"{steps64} steps of the pyramid from rest through the reference's f64 sequential solver "
                           f"(src/avbd3d/ref, restated in the oracle), one host thread, {dt64:.1f} s",
                 "gpu_over_cpu": value * dt64 / (n * steps64)}

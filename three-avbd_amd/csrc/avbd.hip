@@ -1809,7 +1809,7 @@ int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour) {
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms) {
   if (!s || substeps == 0 || steps == 0) return AVBD_ERR_INVALID;
   const avbd_schedule cands[] = {{0, 512, 1}, {1, 256, 1}, {1, 512, 1}, {1, 1024, 1}, {1, 256, 2}, {1, 512, 2},
-                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}, {3, 64, 4}, {3, 64, 8}};
+                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}, {3, 64, 4}, {3, 64, 8}, {3, 64, 12}};
   const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
   float bestT[sizeof(cands) / sizeof(cands[0])];
   for (int k = 0; k < nc; k++) bestT[k] = INFINITY;
