@@ -505,6 +505,8 @@ def bench_step(args, workload):
         c = solver.read_counters()
         log(f"lead-in {min(lead, done + 50)}/{total_lead} steps: {c['contacts']} contacts, {c['manifolds']} manifolds, "
             f"{c['colors']} colours")
+    if args.sparse_tail is not None:
+        solver.set_sparse_tail(args.sparse_tail)
     if tune_total and args.schedule == "auto":
         schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
     else:  # counter-collection runs distort the autotuner's timings: fix the schedule instead
@@ -639,6 +641,8 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop", "flow", "tail"],
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--flow-blocks-per-cu", type=int, default=8, help="--schedule flow: workgroups per CU of the dataflow sweep")
+    ap.add_argument("--sparse-tail", type=int, default=None,
+                    help="first colour of the fused tail (k_primal_tail, or the dataflow tail under --schedule tail); default automatic")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
     ap.add_argument("--adjoint-steps", type=int, default=120, help="adjoint-256 workload: steps")
     ap.add_argument("--adjoint-pair-only", action="store_true",

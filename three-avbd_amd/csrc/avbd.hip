@@ -82,6 +82,22 @@ uint32_t blocksFor(uint64_t n, uint32_t threads, uint32_t cap = 16384) {
   if (b > cap) b = cap;
   return (uint32_t)b;
 }
+/** k_primal's grid: enough workgroups to take any colour of up to n bodies in one pass,
+ *  at lanesFor(size) threads a body. size x lanes is largest at the top of one of the lane rule's
+ *  ranges (lanes are constant between its thresholds), so only those sizes and n are tried. A
+ *  colour then spreads over as many wavefronts as it has bodies' threads: none walks two chunks
+ *  one after the other while other SIMDs idle. */
+uint32_t primalBlocks(const avbd_params &hp, uint32_t n) {
+  const uint64_t tops[5] = {n, (uint64_t)hp.primal_lanes[0] - 1, (uint64_t)hp.primal_lanes[1] - 1, (uint64_t)hp.primal_lanes[2] - 1,
+                            (uint64_t)hp.primal_lanes[3] - 1};
+  uint64_t most = 1;
+  for (uint64_t c : tops) {
+    if (c < 1 || c > n) continue;
+    uint64_t lanes = c >= hp.primal_lanes[0] ? 1 : c >= hp.primal_lanes[1] ? 2 : c >= hp.primal_lanes[2] ? 4 : c >= hp.primal_lanes[3] ? 8 : 16;
+    most = std::max(most, c * lanes);
+  }
+  return blocksFor(most, PRIMAL_THREADS);
+}
 float finiteF(float x) { return x > 3.0e38f ? 3.0e38f : (x < -3.0e38f ? -3.0e38f : x); }
 
 /** The k-th smallest (Wirth's selection, radii.ts:21-43), reordering a. */
@@ -706,11 +722,11 @@ struct avbd_solver {
    *  instead). */
   bool sweepWithTail(Dev &d) {
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap, first = std::min(tailAuto ? flowTailStart : tailStart, cap);
-    uint32_t pb = blocksFor(n, 64);
+    uint32_t pb = primalBlocks(hp, n);
     k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d, flowFirstKey(first));
     fill(flowDone, n, 0u);
     for (uint32_t it = 0; it < hp.iterations; it++) {
-      for (uint32_t c = 0; c < first; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
+      for (uint32_t c = 0; c < first; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c); });
       // A plain launch: the grid is at most what the occupancy check found resident on every CU, and
       // on the solver's stream the kernel starts once the one before it has ended, so every
       // workgroup is resident together (a cooperative launch, which guarantees the same, cost ~12 us
@@ -726,12 +742,12 @@ struct avbd_solver {
         if (capturing) { captureFailed = true; return true; }
         if (it == 0 && first == 0) return false;  // nothing of the sweep ran yet
         // the colours before the tail of this iteration ran: finish the iteration, then sweep on
-        for (uint32_t c = first; c < cap; c++) k_primal<<<pb, 64, 0, stream>>>(d, c);
+        for (uint32_t c = first; c < cap; c++) k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c);
         k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d);
         coopBlocks = 0;
         coopTail = false;
         for (uint32_t rest = it + 1; rest < hp.iterations; rest++) {
-          for (uint32_t c = 0; c < cap; c++) k_primal<<<pb, 64, 0, stream>>>(d, c);
+          for (uint32_t c = 0; c < cap; c++) k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c);
           k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d);
         }
         break;
@@ -744,7 +760,7 @@ struct avbd_solver {
    *  velocities; the warm starts have run. */
   void sweepColors(Dev &d) {
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap;
-    uint32_t pb = blocksFor(n, 64), snapBlocks = blocksFor(std::max(n, conCap), 256);
+    uint32_t pb = primalBlocks(hp, n), snapBlocks = blocksFor(std::max(n, conCap), 256);
     if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, 0);
     for (uint32_t it = 0; it < hp.iterations; it++) {
       if (rec) d.bits = rec->t.bits + ((size_t)recSlot * rec->t.iters + it) * rec->t.conCap;
@@ -753,7 +769,7 @@ struct avbd_solver {
           timed(K_TAIL, [&] { k_primal_tail<<<1, 512, 0, stream>>>(d, c, cap); });  // apart from k_primal's launches
           break;
         }
-        timed(K_PRIMAL, [&] { k_primal<<<pb, 64, 0, stream>>>(d, c); });
+        timed(K_PRIMAL, [&] { k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c); });
       }
       timed(K_DUAL, [&] { k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d); });
       if (rec) k_tape_snap<<<snapBlocks, 256, 0, stream>>>(d, rec->t, recSlot, it + 1);

@@ -16,13 +16,21 @@ __constant__ const uint32_t BOX_FACE_VERTS[24] = {1, 3, 7, 5, 0, 2, 6, 4, 2, 3, 
 __constant__ const uint4 BOX_EDGES[12] = {{0, 1, 3, 5}, {2, 3, 2, 5}, {4, 5, 3, 4}, {6, 7, 2, 4}, {0, 2, 1, 5}, {1, 3, 0, 5},
                                            {4, 6, 1, 4}, {5, 7, 0, 4}, {0, 4, 1, 3}, {1, 5, 0, 3}, {2, 6, 1, 2}, {3, 7, 0, 2}};
 
-/** A convex shape as the hull code sees it: a box (hull == NO_HULL, half extents h) or a hull. */
-struct Poly { V c; Q q; V h; uint32_t hull, vs, nv, fs, nf, es, ne, is; };
+/** A hull's faces staged in LDS for one wavefront: world planes (facePlane's arithmetic, done once),
+ *  each face's first index (high 16 bits) and vertex count, and the vertex-index list as bytes.
+ *  Every face lookup of a pair then reads LDS instead of a chain of dependent global loads. */
+static const uint32_t STAGE_F = 64, STAGE_I = 192;
+struct FaceStage { float4 plane[STAGE_F]; uint32_t info[STAGE_F]; uint8_t idx[STAGE_I]; };
+
+/** A convex shape as the hull code sees it: a box (hull == NO_HULL, half extents h) or a hull;
+ *  fst, when set, holds its faces staged (same values as the global reads). */
+struct Poly { V c; Q q; V h; uint32_t hull, vs, nv, fs, nf, es, ne, is; const FaceStage *fst; };
 
 __device__ __forceinline__ Poly polyOf(const uint4 *hulls, V c, Q q, V h, float shape) {
   Poly P;
   P.c = c; P.q = q; P.h = h;
   P.hull = NO_HULL; P.nv = 8; P.nf = 6; P.ne = 12; P.vs = P.fs = P.es = P.is = 0;
+  P.fst = nullptr;
   if (shape >= SHAPE_HULL) {
     uint32_t hh = (uint32_t)(shape - SHAPE_HULL + 0.5f);
     uint4 h0 = hulls[hh], h1 = hulls[hh + 1];
@@ -45,6 +53,12 @@ __device__ __forceinline__ V faceNormalLocal(const uint4 *hulls, const Poly &P, 
 }
 /** Face f's world plane: outward normal n and offset. */
 __device__ __forceinline__ void facePlane(const uint4 *hulls, const Poly &P, uint32_t f, V &n, float &off) {
+  if (P.fst) {
+    float4 w = P.fst->plane[f];
+    n = xyz(w);
+    off = w.w;
+    return;
+  }
   V nl;
   float d;
   if (P.hull == NO_HULL) {
@@ -61,9 +75,11 @@ __device__ __forceinline__ void facePlane(const uint4 *hulls, const Poly &P, uin
   off = d + dot(n, P.c);
 }
 __device__ __forceinline__ uint32_t faceVertCount(const uint4 *hulls, const Poly &P, uint32_t f) {
+  if (P.fst) return P.fst->info[f] & 0xffffu;
   return P.hull == NO_HULL ? 4u : hulls[P.fs + 2 * f + 1].y;
 }
 __device__ __forceinline__ uint32_t faceVert(const uint4 *hulls, const Poly &P, uint32_t f, uint32_t j) {
+  if (P.fst) return P.fst->idx[(P.fst->info[f] >> 16) + j];
   if (P.hull == NO_HULL) return BOX_FACE_VERTS[f * 4 + j];
   uint32_t k = hulls[P.fs + 2 * f + 1].x + j;
   uint4 w = hulls[P.is + k / 4];
@@ -429,11 +445,22 @@ __device__ __forceinline__ EdgeQuery queryEdgesWave(const uint4 *hulls, const Po
  *  hulls of up to 32 vertices (at most 90 edges), the hull builder's limit; a larger shape would
  *  take the unstaged path. */
 static const uint32_t STAGE_V = 32, STAGE_E = 96, STAGE_LIST = 256;
-struct HullStage {
-  float4 vA[STAGE_V], vB[STAGE_V], wA[STAGE_V], wB[STAGE_V];
+struct EdgeStage {
   float4 nB[2 * STAGE_E], xB[STAGE_E];
   uint2 eA[STAGE_E], eB[STAGE_E];
   uint32_t list[STAGE_LIST];
+};
+/** After the edge query: the face contact's clip buffer and both shapes' faces. */
+struct ContactStage {
+  float4 clip[64];
+  FaceStage f[2];
+};
+struct HullStage {
+  float4 vA[STAGE_V], vB[STAGE_V], wA[STAGE_V], wB[STAGE_V];
+  union {
+    EdgeStage e;     // SAT edge query
+    ContactStage c;  // face contact (the edge data is dead by then), or a sphere's hull faces
+  };
 };
 __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -442,6 +469,31 @@ __device__ __forceinline__ void waveSync() {
 }
 __device__ __forceinline__ bool stageable(const Poly &A, const Poly &B) {
   return A.nv <= STAGE_V && B.nv <= STAGE_V && A.ne <= STAGE_E && B.ne <= STAGE_E;
+}
+/** Stage hull P's faces into fs (a face a lane): false, with nothing to use, for a box or a hull
+ *  beyond the stage's sizes (wavefront-uniform). The caller syncs the wavefront before and after. */
+__device__ __forceinline__ bool stageFaces(const uint4 *hulls, const Poly &P, uint32_t lane, FaceStage &fs) {
+  if (P.hull == NO_HULL || P.nf > STAGE_F || P.nv > STAGE_V) return false;
+  uint32_t end = 0;
+  if (lane < P.nf) {
+    uint4 w = hulls[P.fs + 2 * lane + 1];
+    V n;
+    float off;
+    facePlane(hulls, P, lane, n, off);  // P.fst is unset: the global reads
+    fs.plane[lane] = f4(n, off);
+    fs.info[lane] = (w.x << 16) | (w.y & 0xffffu);
+    end = w.x + w.y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { uint32_t e = (uint32_t)__shfl_xor((int)end, o); end = e > end ? e : end; }
+  if (end > STAGE_I) return false;
+  // the index list: word t of the hull's list holds indices 4t..4t+3 (each below STAGE_V)
+  uint32_t *words = reinterpret_cast<uint32_t *>(fs.idx);
+  for (uint32_t t = lane; 4 * t < end; t += 64) {
+    uint4 w = hulls[P.is + t];
+    words[t] = w.x | (w.y << 8) | (w.z << 16) | (w.w << 24);
+  }
+  return true;
 }
 __device__ __forceinline__ void stagePair(const uint4 *hulls, const Poly &A, const Poly &B, uint32_t lane, HullStage &st) {
   waveSync();  // the previous pair's readers are done
@@ -457,15 +509,15 @@ __device__ __forceinline__ void stagePair(const uint4 *hulls, const Poly &A, con
   }
   for (uint32_t e = lane; e < A.ne; e += 64) {
     uint4 ea = edgeOf(hulls, A, e);
-    st.eA[e] = make_uint2(ea.x, ea.y);
+    st.e.eA[e] = make_uint2(ea.x, ea.y);
   }
   for (uint32_t e = lane; e < B.ne; e += 64) {
     uint4 eb = edgeOf(hulls, B, e);
     V c = neg(faceNormalLocal(hulls, B, eb.z)), d = neg(faceNormalLocal(hulls, B, eb.w));
-    st.nB[2 * e] = f4(c, 0.0f);
-    st.nB[2 * e + 1] = f4(d, 0.0f);
-    st.xB[e] = f4(cross(d, c), 0.0f);  // minkowskiFace's dxc
-    st.eB[e] = make_uint2(eb.x, eb.y);
+    st.e.nB[2 * e] = f4(c, 0.0f);
+    st.e.nB[2 * e + 1] = f4(d, 0.0f);
+    st.e.xB[e] = f4(cross(d, c), 0.0f);  // minkowskiFace's dxc
+    st.e.eB[e] = make_uint2(eb.x, eb.y);
   }
   waveSync();
 }
@@ -500,7 +552,7 @@ __device__ __forceinline__ void queryFacesStaged(const uint4 *hulls, const Poly 
  *  from the stage; keeps (s, k) in `best` when s is larger, or equal with a lower k. */
 __device__ __forceinline__ void edgePairStaged(const Poly &A, const Poly &B, const HullStage &st, uint32_t i, uint32_t j, ArgBest &best,
                                                V &bestN) {
-  uint2 ea = st.eA[i], eb = st.eB[j];
+  uint2 ea = st.e.eA[i], eb = st.e.eB[j];
   V pa = xyz(st.wA[ea.x]), qa = xyz(st.wA[ea.y]), pb = xyz(st.wB[eb.x]), qb = xyz(st.wB[eb.y]);
   V n = cross(sub(qa, pa), sub(qb, pb));
   float l = length(n);
@@ -533,23 +585,23 @@ __device__ __forceinline__ EdgeQuery queryEdgesStaged(const uint4 *hulls, const 
       bxa = cross(b, a);
     }
     for (uint32_t j = 0; j < B.ne; j++) {
-      V c = xyz(st.nB[2 * j]), d = xyz(st.nB[2 * j + 1]), dxc = xyz(st.xB[j]);
+      V c = xyz(st.e.nB[2 * j]), d = xyz(st.e.nB[2 * j + 1]), dxc = xyz(st.e.xB[j]);
       float cba = dot(c, bxa), dba = dot(d, bxa), adc = dot(a, dxc), bdc = dot(b, dxc);
       bool crosses = has && cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f;
       uint64_t m = __ballot(crosses);
       if (m == 0) continue;
-      if (crosses) st.list[listed + (uint32_t)__popcll(m & below)] = (i << 16) | j;
+      if (crosses) st.e.list[listed + (uint32_t)__popcll(m & below)] = (i << 16) | j;
       listed += (uint32_t)__popcll(m);
       if (listed > STAGE_LIST - 64) {  // room for one more round of 64
         waveSync();
-        for (uint32_t t = lane; t < listed; t += 64) { uint32_t ij = st.list[t]; edgePairStaged(A, B, st, ij >> 16, ij & 0xffffu, best, bestN); }
+        for (uint32_t t = lane; t < listed; t += 64) { uint32_t ij = st.e.list[t]; edgePairStaged(A, B, st, ij >> 16, ij & 0xffffu, best, bestN); }
         waveSync();
         listed = 0;
       }
     }
   }
   waveSync();
-  for (uint32_t t = lane; t < listed; t += 64) { uint32_t ij = st.list[t]; edgePairStaged(A, B, st, ij >> 16, ij & 0xffffu, best, bestN); }
+  for (uint32_t t = lane; t < listed; t += 64) { uint32_t ij = st.e.list[t]; edgePairStaged(A, B, st, ij >> 16, ij & 0xffffu, best, bestN); }
   ArgBest w = waveArgMax(best);
   EdgeQuery out;
   out.valid = w.i != 0xffffffffu;
@@ -679,18 +731,19 @@ __device__ __forceinline__ void hullFaceContactWave(const uint4 *hulls, const Po
   least = waveArgMin(least);
   uint32_t inc = least.i == 0xffffffffu ? 0u : least.i;
   if (st) {
-    hullFaceClipWave(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, lane, st->nB, worldR, worldI, found);
+    hullFaceClipWave(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, lane, st->c.clip, worldR, worldI, found);
     return;
   }
   if (lane != 0) return;
   hullFaceClip(hulls, A, B, refIsA, refFace, R, I, rn, pw, inc, found);
 }
 /** collidePoly with the SAT queries spread over the wavefront; lane 0 builds the contact. */
-__device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &A, const Poly &B, bool faceBias, uint32_t lane, HullStage *st,
+__device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &A0, const Poly &B0, bool faceBias, uint32_t lane, HullStage *st,
                                                 Sat &sat, Found &found) {
   found.count = 0;
   float faSep, fbSep;
   uint32_t faFace, fbFace;
+  Poly A = A0, B = B0;
   bool staged = st != nullptr && stageable(A, B);
   if (staged) stagePair(hulls, A, B, lane, *st);
   if (staged) queryFacesStaged(hulls, A, B, st->vB, lane, faSep, faFace);
@@ -707,6 +760,12 @@ __device__ __forceinline__ void collidePolyWave(const uint4 *hulls, const Poly &
   if (edgeWins) {
     if (lane == 0) { sat.n = eq.n; hullEdgeContact(hulls, A, B, eq, found); }
     return;
+  }
+  if (staged) {  // both shapes' faces over the dead edge data, for the face contact
+    waveSync();
+    if (stageFaces(hulls, A, lane, st->c.f[0])) A.fst = &st->c.f[0];
+    if (stageFaces(hulls, B, lane, st->c.f[1])) B.fst = &st->c.f[1];
+    waveSync();
   }
   V n;
   float d;
@@ -798,11 +857,13 @@ __device__ __forceinline__ void collideHullPairWave(const uint4 *hulls, const Bo
   if (sphereA || sphereB) {
     Poly X = sphereA ? polyOf(hulls, B.c, qB, B.h, shB) : polyOf(hulls, A.c, qA, A.h, shA);
     const float4 *world = nullptr;
-    if (st && X.nv <= STAGE_V) {  // the hull's world vertices into the stage, once for all its faces
+    if (st && X.nv <= STAGE_V) {  // the hull's world vertices and faces into the stage, once for all its faces
       waveSync();                  // the previous pair's readers are done
       for (uint32_t v = lane; v < X.nv; v += 64) st->wA[v] = f4(vert(hulls, X, v), 0.0f);
+      bool faces = stageFaces(hulls, X, lane, st->c.f[0]);
       waveSync();
       world = st->wA;
+      if (faces) X.fst = &st->c.f[0];
     }
     if (sphereA) {
       SphereHit h = sphereHullWave(hulls, A.c, A.h.x, X, lane, world);

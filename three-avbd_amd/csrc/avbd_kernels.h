@@ -1554,7 +1554,10 @@ __device__ __forceinline__ void primalColor(const Dev &d, uint32_t c, uint32_t b
     if (enabled && lane == 0) finishBody(d, i, acc, pos, rot);
   }
 }
-__global__ void __launch_bounds__(64) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
+#ifndef PRIMAL_THREADS
+#define PRIMAL_THREADS 64  // k_primal's workgroup size
+#endif
+__global__ void __launch_bounds__(PRIMAL_THREADS) k_primal(Dev d, uint32_t c) { primalColor(d, c, blockIdx.x, gridDim.x); }
 /** The sweep's tail, colours [c0, c1), in one workgroup (primalSparse, wgsl-solve.ts:378-394, with
  *  a chosen first colour): colours stay ordered, each one's bodies are solved chunk by chunk by the
  *  workgroup, and every pose write is visible before the next colour reads it. For small trailing
