@@ -224,6 +224,7 @@ __device__ __forceinline__ void flowDuals(const Dev &d, uint32_t i, uint32_t q0,
 
 /** Chunk `chunk` of a colour's bucket [start, start + count) at iteration iter: one wavefront,
  *  lanes threads per body, as the colour-by-colour sweep splits it. */
+template <bool EARLY = false>
 __device__ __forceinline__ void flowChunk(const Dev &d, uint32_t start, uint32_t count, uint32_t lanes, bool serial, uint32_t chunk,
                                           uint32_t iter, uint32_t polls) {
   uint32_t per = 64u / lanes, slot = threadIdx.x / lanes, lane = threadIdx.x % lanes;
@@ -242,6 +243,15 @@ __device__ __forceinline__ void flowChunk(const Dev &d, uint32_t start, uint32_t
     uint32_t first = threadIdx.x - lane;
     pos = flowShfl(pos, first);
     rot = flowShfl(rot, first);
+  }
+  if (EARLY) {
+    // One iteration per launch: the only readers this launch releases are later bodies of the same
+    // iteration, which read this body's pose and none of the constraints it owns (a constraint's
+    // owner is its later body, so theirs are their own). The owned duals then run off the chain.
+    flowStoresDone();
+    if (live && lane == 0) __hip_atomic_store(&d.flowDone[i], iter + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (live) flowDuals(d, i, item.y, item.z, lane, lanes, d.p.alpha, pos, rot, serial);
+    return;
   }
   if (live) flowDuals(d, i, item.y, item.z, lane, lanes, d.p.alpha, pos, rot, serial);
   flowStoresDone();  // everything out before the counts that release the readers
@@ -298,7 +308,7 @@ __global__ void __launch_bounds__(64) k_flow_tail(Dev d, uint32_t first, uint32_
     uint32_t lanes = lanesFor(d.p, count), per = 64u / lanes;
     uint32_t chunks = (count + per - 1u) / per;
     bool serial = (d.counters[C_CLASHING + (c >> 5)] >> (c & 31u)) & 1u;
-    for (uint32_t k = (w + W - g % W) % W; k < chunks; k += W) flowChunk(d, start, count, lanes, serial, k, iter, polls);
+    for (uint32_t k = (w + W - g % W) % W; k < chunks; k += W) flowChunk<true>(d, start, count, lanes, serial, k, iter, polls);
     g += chunks;
   }
   // the other duals: on the workgroups the chunks left free (chunk k went to workgroup k mod W)
