@@ -556,7 +556,10 @@ def bench_step(args, workload):
         # one launch per colour per iteration (trailing small colours fused into k_primal_tail,
         # timed apart): a sweep's bytes over its k_primal launches
         avg_ms = ks["primal_ms"] / max(1, ks["primal_launches"])
-        kernel = "k_primal (one colour of the sweep; trailing small colours run fused in k_primal_tail)"
+        if schedule.get("cooperative") == 3:
+            kernel = "k_primal (one leading colour of the sweep; the trailing colours and the duals run in k_flow_tail)"
+        else:
+            kernel = "k_primal (one colour of the sweep; trailing small colours run fused in k_primal_tail)"
         per_sweep = ks["primal_launches"] / (2 * substeps * iters)
         nbytes = primal_sweep_bytes(n, c2["manifolds"], c2["contacts"], len(joints)) / max(1.0, per_sweep)
     if avg_ms <= 0.0:
