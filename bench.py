@@ -515,6 +515,9 @@ def bench_step(args, workload):
         else:
             solver.set_schedule(cooperative=int(args.schedule == "coop"), threads=512, blocks_per_cu=1)
         schedule, tuned_ms = solver.schedule, solver.time_steps(tune_total, substeps) / max(1, tune_total)
+    if args.graphs:
+        solver.set_graphs(True)
+        solver.steps(4, substeps)  # capture the substeps' launch sequences before the timed steps
     solver.sync()
     c = solver.read_counters()
     log(f"lead-in {total_lead} steps x {substeps} substeps (autotune: {schedule}, {tuned_ms:.2f} ms/step) "
@@ -644,6 +647,7 @@ def main():
     ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop", "flow", "tail"],
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--flow-blocks-per-cu", type=int, default=8, help="--schedule flow: workgroups per CU of the dataflow sweep")
+    ap.add_argument("--graphs", action="store_true", help="replay each substep as a captured hipGraph (avbd_set_graphs)")
     ap.add_argument("--sparse-tail", type=int, default=None,
                     help="first colour of the fused tail (k_primal_tail, or the dataflow tail under --schedule tail); default automatic")
     ap.add_argument("--adjoint-worlds", type=int, default=256, help="adjoint-256 workload: worlds (smaller for profiling runs)")
