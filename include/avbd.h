@@ -343,9 +343,10 @@ int avbd_set_lifecycle(avbd_solver *s, uint32_t index, uint32_t bits);
 
 /* Project autotuner: times each candidate schedule over `steps` x `substeps` (the simulation
  * advances), twice interleaved, and keeps the fastest; writes it and its ms per step to `best`.
- * When per-colour launches win, it also times, against the params' primal_lanes rule, eight
- * threads per body for every colour above the rule's last size, and each of the two without
- * sixteen-thread colours, and keeps the fastest in the params (avbd_get_params reads it). */
+ * When a schedule with per-colour launches wins (all but the cooperative sweep), it also times,
+ * against the params' primal_lanes rule, eight threads per body for every colour above the rule's
+ * last size, each of the two without sixteen-thread colours, and eight threads with sixteen up to
+ * four times the last size, and keeps the fastest in the params (avbd_get_params reads it). */
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms);
 
 /* profileNextStep (solver.ts:1952): phase times of the next step, synchronously. */

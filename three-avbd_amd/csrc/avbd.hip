@@ -1841,19 +1841,22 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
   for (int k = 0; k < nc; k++) if (bestT[k] < INFINITY && (pick < 0 || bestT[k] < bestT[pick])) pick = k;
   if (pick < 0) { s->applySchedule(keep); return fail(s, AVBD_ERR_DEVICE, "autotune: no schedule ran"); }
   // Threads per body, under the per-colour or dataflow schedule when it won: the given lane rule, eight threads
-  // for every colour above the rule's last size, and each of the two without sixteen-thread
-  // colours; the fastest is kept (results differ only in the order the lanes' sums are added).
-  // Sixteen threads pay where small colours hold bodies with long adjacency walks, and cost an
-  // extra reduction step where they do not.
+  // for every colour above the rule's last size, each of the two without sixteen-thread colours,
+  // and eight threads with sixteen up to four times the last size; the fastest is kept (results
+  // differ only in the order the lanes' sums are added). Sixteen threads pay where small colours
+  // hold bodies with long adjacency walks, and cost an extra reduction step where they do not.
   if (cands[pick].cooperative != 1) {
     const uint32_t *g = s->hp.primal_lanes;
-    const uint32_t rules[4][4] = {{g[0], g[1], g[2], g[3]}, {1u << 31, 1u << 31, 1u << 31, g[3]}, {g[0], g[1], g[2], 0u}, {1u << 31, 1u << 31, 1u << 31, 0u}};
+    const uint32_t wide = g[3] <= (1u << 29) ? 4u * g[3] : g[3];
+    const uint32_t rules[5][4] = {{g[0], g[1], g[2], g[3]}, {1u << 31, 1u << 31, 1u << 31, g[3]}, {g[0], g[1], g[2], 0u}, {1u << 31, 1u << 31, 1u << 31, 0u},
+                                  {1u << 31, 1u << 31, 1u << 31, wide}};
+    const int nr = 5;
     uint32_t given[4] = {g[0], g[1], g[2], g[3]}, bestRule = 0;
-    float t[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    float t[5] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
     s->applySchedule(cands[pick]);
     // every rule timed again, passes interleaved: the scene keeps changing while it is timed
     for (int pass = 0; pass < 2; pass++)
-      for (int r = 0; r < 4; r++) {
+      for (int r = 0; r < nr; r++) {
         bool seen = false;
         for (int q = 0; q < r; q++) seen = seen || std::equal(rules[q], rules[q] + 4, rules[r]);
         if (seen) continue;
@@ -1862,7 +1865,7 @@ int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedu
         if (int rc = avbd_time_steps(s, steps, substeps, &ms)) { std::copy(given, given + 4, s->hp.primal_lanes); return rc; }
         t[r] = std::min(t[r], ms / steps);
       }
-    for (int r = 1; r < 4; r++) if (t[r] < t[bestRule]) bestRule = (uint32_t)r;
+    for (int r = 1; r < nr; r++) if (t[r] < t[bestRule]) bestRule = (uint32_t)r;
     std::copy(rules[bestRule], rules[bestRule] + 4, s->hp.primal_lanes);
     bestT[pick] = t[bestRule];
   }
