@@ -507,8 +507,7 @@ struct avbd_solver {
     uint32_t nb = (count + 1023) / 1024;
     if (nb == 0) return;
     k_scan_local<<<nb, 256, 0, stream>>>(in, out, scanSums, count, skip);
-    k_scan_sums<<<1, 1024, 0, stream>>>(scanSums, nb, skip);
-    k_scan_add<<<nb, 256, 0, stream>>>(out, scanSums, count, skip);
+    if (nb > 1) k_scan_add<<<nb, 256, 0, stream>>>(out, scanSums, count, skip);
   }
 
   /** uploadStatics (solver.ts:1263-1305): cell size, large list and sorted no-collide list. Each
@@ -622,7 +621,7 @@ struct avbd_solver {
     uint32_t keyBits = 1;
     while ((1ull << keyBits) <= tableSize) keyBits++;  // keys 0 .. tableSize
     uint32_t blocks = (n + 255) / 256, maxBits = 10;
-    // the histogram's scan takes at most 4096 x 1024 entries (k_scan_sums: one workgroup)
+    // the histogram's scan takes at most 4096 x 1024 entries (4096 block totals)
     while (maxBits > 1 && ((uint64_t)blocks << maxBits) + 1 > 4096ull * 1024ull) maxBits--;
     uint32_t passes = (keyBits + maxBits - 1) / maxBits, bits = (keyBits + passes - 1) / passes;
     k_grid_keys<<<blocksFor(n, 256), 256, 0, stream>>>(d);

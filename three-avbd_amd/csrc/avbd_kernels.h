@@ -1874,21 +1874,15 @@ __global__ void __launch_bounds__(256) k_scan_local(const uint32_t *in, uint32_t
   for (int k = 0; k < 4; k++) { if (base + k < n) out[base + k] = ex; ex += v[k]; }
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
-__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t *sums, uint32_t nb, const uint32_t *skip) {
-  if (skip && (*skip & 1u)) return;
-  uint32_t base = threadIdx.x * 4;
-  uint32_t v[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) v[k] = base + k < nb ? sums[base + k] : 0u;
-  uint32_t t = v[0] + v[1] + v[2] + v[3], total;
-  uint32_t ex = blockExclusive(t, &total);
-#pragma unroll
-  for (int k = 0; k < 4; k++) { if (base + k < nb) sums[base + k] = ex; ex += v[k]; }
-}
+/** Adds to block b's outputs the totals of blocks 0..b-1, summed here (integer sums: any order gives
+ *  the same result), so the block totals need no scan launch of their own. */
 __global__ void __launch_bounds__(256) k_scan_add(uint32_t *out, const uint32_t *sums, uint32_t n, const uint32_t *skip) {
   if (skip && (*skip & 1u)) return;
+  if (blockIdx.x == 0) return;  // nothing before it
+  uint32_t part = 0, s;
+  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += blockDim.x) part += sums[k];
+  (void)blockExclusive(part, &s);
   uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
-  uint32_t s = sums[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < 4; k++) if (base + k < n) out[base + k] += s;
 }
