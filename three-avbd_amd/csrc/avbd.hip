@@ -684,7 +684,7 @@ struct avbd_solver {
       // a launch refused here leaves the per-colour sweep below everything it needs
       if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
       k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
-      k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d, 0u);
+      k_flow_prep<<<strideBlocks(std::max<uint64_t>(16ull * n, nj), 256), 256, 0, stream>>>(d, 0u);
       fill(flowDone, n, 0u);
       void *args[] = {&d, &barrierPolls};
       hipError_t e;
@@ -722,7 +722,7 @@ struct avbd_solver {
   bool sweepWithTail(Dev &d) {
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap, first = std::min(tailAuto ? flowTailStart : tailStart, cap);
     uint32_t pb = primalBlocks(hp, n);
-    k_flow_prep<<<blocksFor(std::max(n, nj), 256), 256, 0, stream>>>(d, flowFirstKey(first));
+    k_flow_prep<<<strideBlocks(std::max<uint64_t>(16ull * n, nj), 256), 256, 0, stream>>>(d, first);
     fill(flowDone, n, 0u);
     for (uint32_t it = 0; it < hp.iterations; it++) {
       for (uint32_t c = 0; c < first; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c); });

@@ -62,16 +62,23 @@ __device__ __forceinline__ bool flowOrphan(const Dev &d, uint32_t j) {
   return !(jointActive(d, j) && (isDynamic(d, in.y) || isDynamic(d, in.z)));
 }
 
+/** The order key a sweep starting at colour `first` starts from (past every key: no colour). */
+__host__ __device__ __forceinline__ uint32_t flowFirstKey(uint32_t first) { return first >= 64u ? 0xffffffffu : first << 26; }
 /** Before the sweep: each walk item's flags from the bodies' order keys (and a joint item's other
- *  body in .z), and which joints no body walks. With firstKey, the sweep starts at the colour that
- *  key opens (the dataflow tail): bodies ordered before it are final when it starts, so their
- *  items need no flags and their partners no waits. */
-__global__ void __launch_bounds__(256) k_flow_prep(Dev d, uint32_t firstKey) {
-  GRID_STRIDE(i, d.n) {
-    if (!isDynamic(d, (int)i)) continue;
+ *  body in .z), and which joints no body walks. The sweep starts at colour `first` (0, or where the
+ *  dataflow tail starts): bodies ordered before it are final when it starts, so their items need no
+ *  flags and their partners no waits. */
+__global__ void __launch_bounds__(256) k_flow_prep(Dev d, uint32_t first) {
+  // the swept bodies are the buckets of colours [first, colours), contiguous in colorItems; sixteen
+  // threads a body split its items (each item's flags are its own)
+  const uint32_t firstKey = flowFirstKey(first), L = 16;
+  uint32_t colors = d.counters[C_NUM_COLORS];
+  if (colors > d.p.colorCap) colors = d.p.colorCap;
+  uint32_t b0 = first < colors ? d.colorStart[first] : 0u, b1 = first < colors ? d.colorStart[colors] : 0u;
+  GRID_STRIDE(t, (b1 - b0) * L) {
+    uint32_t i = d.colorItems[b0 + t / L].x;
     uint32_t key = d.bodyKey[i];
-    if (key < firstKey) continue;
-    for (uint32_t q = d.itemStart[i], q1 = d.itemStart[i + 1]; q < q1; q++) {
+    for (uint32_t q = d.itemStart[i] + t % L, q1 = d.itemStart[i + 1]; q < q1; q += L) {
       uint4 it = d.items[q];
       int other = d.adjOther[it.y];
       uint32_t f = it.w & 3u;
@@ -282,8 +289,6 @@ __global__ void __launch_bounds__(64) k_flow(Dev d, uint32_t polls) {
   }
 }
 
-/** The order key a sweep starting at colour `first` starts from (past every key: no colour). */
-__host__ __device__ __forceinline__ uint32_t flowFirstKey(uint32_t first) { return first >= 64u ? 0xffffffffu : first << 26; }
 /** The later of a constraint's bodies in the sweep's order (a fixed body counts as first). */
 __device__ __forceinline__ uint32_t flowOwnerKey(const Dev &d, int a, int b) {
   uint32_t ka = a >= 0 && isDynamic(d, a) ? d.bodyKey[a] : 0u, kb = b >= 0 && isDynamic(d, b) ? d.bodyKey[b] : 0u;
