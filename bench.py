@@ -510,8 +510,9 @@ def bench_step(args, workload):
     if tune_total and args.schedule == "auto":
         schedule, tuned_ms = solver.autotune(tune_steps, substeps)  # part of the lead-in: the simulation advances
     else:  # counter-collection runs distort the autotuner's timings: fix the schedule instead
-        if args.schedule in ("flow", "tail"):
-            solver.set_schedule(cooperative=2 if args.schedule == "flow" else 3, threads=64, blocks_per_cu=args.flow_blocks_per_cu)
+        if args.schedule in ("flow", "tail", "lead"):
+            coop = {"flow": 2, "tail": 3, "lead": 4}[args.schedule]
+            solver.set_schedule(cooperative=coop, threads=64, blocks_per_cu=args.flow_blocks_per_cu)
         else:
             solver.set_schedule(cooperative=int(args.schedule == "coop"), threads=512, blocks_per_cu=1)
         schedule, tuned_ms = solver.schedule, solver.time_steps(tune_total, substeps) / max(1, tune_total)
@@ -559,7 +560,9 @@ def bench_step(args, workload):
         # one launch per colour per iteration (trailing small colours fused into k_primal_tail,
         # timed apart): a sweep's bytes over its k_primal launches
         avg_ms = ks["primal_ms"] / max(1, ks["primal_launches"])
-        if schedule.get("cooperative") == 3:
+        if schedule.get("cooperative") == 4:
+            kernel = "k_flow_lead (the leading colours of one iteration as one dataflow launch; the trailing colours and the duals run in k_flow_tail)"
+        elif schedule.get("cooperative") == 3:
             kernel = "k_primal (one leading colour of the sweep; the trailing colours and the duals run in k_flow_tail)"
         else:
             kernel = "k_primal (one colour of the sweep; trailing small colours run fused in k_primal_tail)"
@@ -644,7 +647,7 @@ def main():
     ap.add_argument("--color-cap-adjoint", type=int, default=16)
     ap.add_argument("--primal-lanes", type=int, nargs=4, default=[2 ** 31, 2 ** 15, 2 ** 12, 2 ** 11],
                     help="colour sizes from which 1, 2, 4, 8 threads per body are used in the primal (else 16)")
-    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop", "flow", "tail"],
+    ap.add_argument("--schedule", default="auto", choices=["auto", "colors", "coop", "flow", "tail", "lead"],
                     help="solve schedule of the large/pile workloads: autotuned, or fixed (for counter-collection runs)")
     ap.add_argument("--flow-blocks-per-cu", type=int, default=8, help="--schedule flow: workgroups per CU of the dataflow sweep")
     ap.add_argument("--graphs", action="store_true", help="replay each substep as a captured hipGraph (avbd_set_graphs)")

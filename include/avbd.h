@@ -193,7 +193,10 @@ int avbd_broadphase_info(const avbd_solver *s, float *cell_size, float *fine_cel
 typedef struct avbd_schedule {
   uint32_t cooperative;      /* 0: per-colour launches; 1: one cooperative launch per substep with grid
                                 barriers; 2: the dataflow sweep, one cooperative launch per substep whose
-                                bodies wait only for their neighbours' per-body completion counts */
+                                bodies wait only for their neighbours' per-body completion counts;
+                                3: per-colour launches, then the dataflow tail (see avbd_set_sparse_tail);
+                                4: as 3 with the colours before the tail as one dataflow launch per
+                                iteration (a workgroup a chunk, in colour order) */
   uint32_t threads;          /* workgroup size of the cooperative kernel: 64..1024 (the dataflow sweep: 64) */
   uint32_t blocks_per_cu;    /* workgroups per CU of the cooperative kernel */
 } avbd_schedule;
@@ -215,8 +218,8 @@ int avbd_set_barrier_polls(avbd_solver *s, uint32_t polls);
  * most ~1k bodies. Here the start is chosen: a colour index, 64 for none, or AVBD_TAIL_AUTO
  * (the default), which picks the first of the trailing colours that held <= 32 bodies at each
  * adaptive-colouring readback (AVBD_FLAG_ADAPT_COLORS; without that flag, no tail). Under schedule
- * 3 the same start opens the dataflow tail (the colours from it and the dual in one cooperative
- * launch per iteration); its automatic start is the first of the trailing colours that held <= 2048
+ * 3 and 4 the same start opens the dataflow tail (the colours from it and the dual in one launch
+ * per iteration); its automatic start is the first of the trailing colours that held <= 2048
  * bodies. Results are the same whichever runs. */
 #define AVBD_TAIL_AUTO 0xFFFFFFFFu
 int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour);

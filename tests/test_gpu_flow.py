@@ -216,19 +216,22 @@ def test_flow_give_up_is_reported_and_the_solver_drops_to_per_colour_launches(gp
 
 
 TAIL = dict(cooperative=3, threads=64, blocks_per_cu=4)
+LEAD = dict(cooperative=4, threads=64, blocks_per_cu=4)
 
 
+@pytest.mark.parametrize("sched", [3, 4])
 @pytest.mark.parametrize("first", [0, 1, 3, 64])
-def test_tail_pile_matches(gpu, first):
+def test_tail_pile_matches(gpu, first, sched):
     """Schedule 3: colours before `first` launched one by one, the rest and the dual in the
-    dataflow tail (0: the whole sweep; 64: only the dual)."""
-    say(f"case: dataflow tail from colour {first}, pile 400 x 12 steps, eight threads")
+    dataflow tail (0: the whole sweep; 64: only the dual). Schedule 4: the colours before `first`
+    as one dataflow launch (k_flow_lead) instead."""
+    say(f"case: schedule {sched}, dataflow tail from colour {first}, pile 400 x 12 steps, eight threads")
     bodies = scenes.random_pile(400, seed=4, bin_size=10.0)[0]
     kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18, 0), color_cap=64)
     pk = dict(kw, gravity=-9.81)
     pk["primal_lanes"] = pk.pop("lanes")
     s = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
-    s.set_schedule(**TAIL)
+    s.set_schedule(**(TAIL if sched == 3 else LEAD))
     s.set_sparse_tail(first)
     m = model_for(bodies, kw)
     for _ in range(12):
@@ -236,14 +239,15 @@ def test_tail_pile_matches(gpu, first):
         m.step(1)
     same_bodies(s, m)
     same_contacts(s, m)
-    assert s.schedule["cooperative"] == 3
+    assert s.schedule["cooperative"] == sched
     assert s.read_counters()["overflow"] & avbd_amd.solver.OVERFLOW_BARRIER == 0
     s.destroy()
 
 
+@pytest.mark.parametrize("sched", [3, 4])
 @pytest.mark.parametrize("first", [1, 2])
-def test_tail_chains_reuse_and_breaking_joints(gpu, first):
-    say(f"case: dataflow tail from colour {first}, 4 substeps with reuse, breakable chains, 4 steps")
+def test_tail_chains_reuse_and_breaking_joints(gpu, first, sched):
+    say(f"case: schedule {sched}, dataflow tail from colour {first}, 4 substeps with reuse, breakable chains, 4 steps")
     bodies, joints, info, hw = scenes.large_scene(300, seed=9, chains=2, links=4, workers=1)
     joints = joints.copy()
     joints[1::4, 11] = -4.0
@@ -251,7 +255,7 @@ def test_tail_chains_reuse_and_breaking_joints(gpu, first):
     pk = dict(kw, gravity=-9.81)
     pk["primal_lanes"] = pk.pop("lanes")
     s = avbd_amd.GpuSolver3D(bodies, joints, info, params=avbd_amd.default_params(**pk), hulls=hw)
-    s.set_schedule(**TAIL)
+    s.set_schedule(**(TAIL if sched == 3 else LEAD))
     s.set_sparse_tail(first)
     m = model_for(bodies, kw, joints, info)
     m.set_hulls(hw)
@@ -266,17 +270,18 @@ def test_tail_chains_reuse_and_breaking_joints(gpu, first):
     s.destroy()
 
 
-def test_tail_clashing_colours_and_adaptive_start(gpu):
+@pytest.mark.parametrize("sched", [3, 4])
+def test_tail_clashing_colours_and_adaptive_start(gpu, sched):
     """A clashing last colour inside the tail (pyramid at cap 2, tail from colour 1), then the
     automatic start under adaptive colouring (readbacks every 10 steps) on a pile."""
-    say("case: dataflow tail, pyramid at colour cap 2 from colour 1; then a pile with the automatic start")
+    say(f"case: schedule {sched}, dataflow tail, pyramid at colour cap 2 from colour 1; then a pile with the automatic start")
     bodies = scenes.pyramid(10)[0]
     rule = (1 << 20, 1 << 19, 1 << 18, 0)
     kw = dict(lanes=rule, color_cap=2)
     pk = dict(kw, gravity=-9.81)
     pk["primal_lanes"] = pk.pop("lanes")
     s = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
-    s.set_schedule(**TAIL)
+    s.set_schedule(**(TAIL if sched == 3 else LEAD))
     s.set_sparse_tail(1)
     m = model_for(bodies, kw)
     for _ in range(6):
@@ -290,12 +295,12 @@ def test_tail_clashing_colours_and_adaptive_start(gpu):
               flags=avbd_amd.FLAG_DEFAULTS | avbd_amd.solver.FLAG_ADAPT_COLORS)
     a = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
     b = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
-    a.set_schedule(**TAIL)
+    a.set_schedule(**(TAIL if sched == 3 else LEAD))
     b.set_schedule(cooperative=0, threads=512, blocks_per_cu=1)
     for _ in range(3):
         a.steps(10)
         b.steps(10)
     assert np.array_equal(a.read_bodies().view(np.uint32), b.read_bodies().view(np.uint32))
-    assert a.schedule["cooperative"] == 3
+    assert a.schedule["cooperative"] == sched
     a.destroy()
     b.destroy()

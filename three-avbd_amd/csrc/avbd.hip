@@ -159,6 +159,7 @@ struct avbd_solver {
   bool barrierFailed = false;        // seen during the current avbd_step/avbd_steps call
   bool coopFlow = false;        // the cooperative launch is the dataflow sweep (k_flow), not k_solve
   bool coopTail = false;        // per-colour launches up to flowTailStart, then one dataflow tail (k_flow_tail) per iteration
+  bool leadFlow = false;        // with coopTail: the colours before the tail as one dataflow launch (k_flow_lead) per iteration
   uint32_t flowTailStart = MAX_COLORS;  // chosen at each adaptive-colouring readback
   uint32_t coopThreads = 512;   // threads per workgroup of the solve kernel
   uint32_t coopPerCU = 1;       // its workgroups per CU
@@ -319,14 +320,15 @@ struct avbd_solver {
   }
   int applySchedule(const avbd_schedule &sch) {
     uint32_t t = sch.threads;
-    if (sch.cooperative == 2 || sch.cooperative == 3) {
+    if (sch.cooperative == 2 || sch.cooperative == 3 || sch.cooperative == 4) {
       // the dataflow sweep (or tail): one wavefront per workgroup, all co-resident
       if (t != 64 || sch.blocks_per_cu < 1) return AVBD_ERR_INVALID;
       int perCU = 0;
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, sch.cooperative == 2 ? (const void *)k_flow : (const void *)k_flow_tail, 64, 0);
       if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
       coopFlow = sch.cooperative == 2;
-      coopTail = sch.cooperative == 3;
+      coopTail = sch.cooperative >= 3;
+      leadFlow = sch.cooperative == 4;
       coopThreads = 64;
       coopPerCU = sch.blocks_per_cu;
       coopBlocks = coopPerCU * numCUs;
@@ -337,6 +339,7 @@ struct avbd_solver {
       if (!coopSupported || perCU < (int)sch.blocks_per_cu) return AVBD_ERR_INVALID;
       coopFlow = false;
       coopTail = false;
+      leadFlow = false;
       coopThreads = t;
       coopPerCU = sch.blocks_per_cu;
       coopBlocks = coopPerCU * numCUs;
@@ -344,12 +347,15 @@ struct avbd_solver {
       coopBlocks = 0;
       coopFlow = false;
       coopTail = false;
+      leadFlow = false;
     } else {
       return AVBD_ERR_INVALID;
     }
     return AVBD_OK;
   }
-  avbd_schedule schedule() const { return avbd_schedule{coopBlocks ? (coopFlow ? 2u : (coopTail ? 3u : 1u)) : 0u, coopThreads, coopPerCU}; }
+  avbd_schedule schedule() const {
+    return avbd_schedule{coopBlocks ? (coopFlow ? 2u : (coopTail ? (leadFlow ? 4u : 3u) : 1u)) : 0u, coopThreads, coopPerCU};
+  }
   /** The fused tail starts where every later colour held at most TAIL_BODIES bodies at this
    *  readback (the cap's empty spares included). One workgroup runs on one CU, so only small
    *  colours gain: a 55-box pyramid sweeps in one launch per iteration (1.12 -> 0.61 ms per
@@ -684,7 +690,7 @@ struct avbd_solver {
       // a launch refused here leaves the per-colour sweep below everything it needs
       if (nj) k_warm_joints<<<blocksFor(nj, 256), 256, 0, stream>>>(d);
       k_warm_bodies<<<blocksFor(n, 256), 256, 0, stream>>>(d);
-      k_flow_prep<<<strideBlocks(std::max<uint64_t>(16ull * n, nj), 256), 256, 0, stream>>>(d, 0u);
+      k_flow_prep<<<strideBlocks(std::max<uint64_t>(16ull * n, nj), 256), 256, 0, stream>>>(d, 0u, 0u);
       fill(flowDone, n, 0u);
       void *args[] = {&d, &barrierPolls};
       hipError_t e;
@@ -722,10 +728,16 @@ struct avbd_solver {
   bool sweepWithTail(Dev &d) {
     uint32_t cap = fixedColors ? fixedCap : hp.color_cap, first = std::min(tailAuto ? flowTailStart : tailStart, cap);
     uint32_t pb = primalBlocks(hp, n);
-    k_flow_prep<<<strideBlocks(std::max<uint64_t>(16ull * n, nj), 256), 256, 0, stream>>>(d, first);
+    bool lead = leadFlow && first > 0;
+    // one workgroup a chunk: at most a quarter of the bodies (sixteen threads a body) plus a partial
+    // chunk per colour
+    uint32_t leadBlocks = (uint32_t)std::min<uint64_t>((uint64_t)n / 4u + first + 1u, 0x7fffffffull);
+    k_flow_prep<<<strideBlocks(std::max<uint64_t>(16ull * n, nj), 256), 256, 0, stream>>>(d, first, lead ? 1u : 0u);
     fill(flowDone, n, 0u);
     for (uint32_t it = 0; it < hp.iterations; it++) {
-      for (uint32_t c = 0; c < first; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c); });
+      if (lead) timed(K_PRIMAL, [&] { k_flow_lead<<<leadBlocks, 64, 0, stream>>>(d, first, it, barrierPolls); });
+      else
+        for (uint32_t c = 0; c < first; c++) timed(K_PRIMAL, [&] { k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c); });
       // A plain launch: the grid is at most what the occupancy check found resident on every CU, and
       // on the solver's stream the kernel starts once the one before it has ended, so every
       // workgroup is resident together (a cooperative launch, which guarantees the same, cost ~12 us
@@ -745,6 +757,7 @@ struct avbd_solver {
         k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d);
         coopBlocks = 0;
         coopTail = false;
+        leadFlow = false;
         for (uint32_t rest = it + 1; rest < hp.iterations; rest++) {
           for (uint32_t c = 0; c < cap; c++) k_primal<<<pb, PRIMAL_THREADS, 0, stream>>>(d, c);
           k_dual<<<strideBlocks(nj + conCap, 256), 256, 0, stream>>>(d);
@@ -1072,6 +1085,7 @@ struct avbd_solver {
     coopBlocks = 0;
     coopFlow = false;
     coopTail = false;
+    leadFlow = false;
     fill(counters + C_BARRIER, 1, 0u);
     err = "the cooperative solve's grid barrier gave up waiting (" + std::to_string(c[C_BARRIER]) +
           " waits): the solve of the steps since the last readback ran unordered; the solver now uses per-colour launches";
@@ -1086,7 +1100,7 @@ struct avbd_solver {
    *  capacities, params) and the choices taken outside them. Equal bytes, equal launches. */
   struct GraphKey {
     Dev d;
-    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge, coopFlow, coopTail, flowTailStart;
+    uint32_t reuse, keep, coopBlocks, coopThreads, tailStart, barrierPolls, hullBlocks, anyLarge, coopFlow, coopTail, flowTailStart, leadFlow;
     const uint32_t *fixedDev;
   };
   struct GraphEntry { GraphKey key; hipGraphExec_t exec; uint64_t used; };
@@ -1100,7 +1114,7 @@ struct avbd_solver {
     k.d = dev();
     k.reuse = reuse; k.keep = keep; k.coopBlocks = coopBlocks; k.coopThreads = coopThreads; k.tailStart = tailStart;
     k.barrierPolls = barrierPolls; k.hullBlocks = hullBlocks; k.anyLarge = anyDynamicLarge || anyFixedSmall; k.fixedDev = fixedColors ? fixedDev : nullptr;
-    k.coopFlow = coopFlow; k.coopTail = coopTail; k.flowTailStart = flowTailStart;
+    k.coopFlow = coopFlow; k.coopTail = coopTail; k.flowTailStart = flowTailStart; k.leadFlow = leadFlow;
     return k;
   }
   void dropGraphs() {
@@ -1824,7 +1838,8 @@ int avbd_set_sparse_tail(avbd_solver *s, uint32_t first_colour) {
 int avbd_autotune(avbd_solver *s, uint32_t steps, uint32_t substeps, avbd_schedule *best, float *best_ms) {
   if (!s || substeps == 0 || steps == 0) return AVBD_ERR_INVALID;
   const avbd_schedule cands[] = {{0, 512, 1}, {1, 256, 1}, {1, 512, 1}, {1, 1024, 1}, {1, 256, 2}, {1, 512, 2},
-                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}, {3, 64, 4}, {3, 64, 8}, {3, 64, 12}};
+                                 {2, 64, 4}, {2, 64, 8}, {2, 64, 12}, {3, 64, 4}, {3, 64, 8}, {3, 64, 12},
+                                 {4, 64, 8}, {4, 64, 12}};
   const int nc = (int)(sizeof(cands) / sizeof(cands[0]));
   float bestT[sizeof(cands) / sizeof(cands[0])];
   for (int k = 0; k < nc; k++) bestT[k] = INFINITY;

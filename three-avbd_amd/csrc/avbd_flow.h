@@ -68,13 +68,15 @@ __host__ __device__ __forceinline__ uint32_t flowFirstKey(uint32_t first) { retu
  *  body in .z), and which joints no body walks. The sweep starts at colour `first` (0, or where the
  *  dataflow tail starts): bodies ordered before it are final when it starts, so their items need no
  *  flags and their partners no waits. */
-__global__ void __launch_bounds__(256) k_flow_prep(Dev d, uint32_t first) {
-  // the swept bodies are the buckets of colours [first, colours), contiguous in colorItems; sixteen
-  // threads a body split its items (each item's flags are its own)
+__global__ void __launch_bounds__(256) k_flow_prep(Dev d, uint32_t first, uint32_t lead) {
+  // the swept bodies are the buckets of colours [first, colours) (from colour 0 with `lead`: the
+  // colours before the tail run as one launch too), contiguous in colorItems; sixteen threads a
+  // body split its items (each item's flags are its own)
   const uint32_t firstKey = flowFirstKey(first), L = 16;
   uint32_t colors = d.counters[C_NUM_COLORS];
   if (colors > d.p.colorCap) colors = d.p.colorCap;
-  uint32_t b0 = first < colors ? d.colorStart[first] : 0u, b1 = first < colors ? d.colorStart[colors] : 0u;
+  uint32_t from = lead ? 0u : first;
+  uint32_t b0 = from < colors ? d.colorStart[from] : 0u, b1 = from < colors ? d.colorStart[colors] : 0u;
   GRID_STRIDE(t, (b1 - b0) * L) {
     uint32_t i = d.colorItems[b0 + t / L].x;
     uint32_t key = d.bodyKey[i];
@@ -82,6 +84,17 @@ __global__ void __launch_bounds__(256) k_flow_prep(Dev d, uint32_t first) {
       uint4 it = d.items[q];
       int other = d.adjOther[it.y];
       uint32_t f = it.w & 3u;
+      if (key < firstKey) {
+        // a body of the leading launch: one iteration per launch, so it waits only for partners
+        // ordered before it (their update of this iteration); its duals run in the tail launch
+        if (other >= 0 && d.bodyKey[other] < key) {
+          f |= FLOW_DEP | FLOW_LOWER;
+          if (it.w & 2u) it.z = (uint32_t)other;
+        }
+        it.w = f;
+        d.items[q] = it;
+        continue;
+      }
       if (other >= 0 && d.bodyKey[other] >= firstKey) {
         uint32_t ko = d.bodyKey[other];
         f |= FLOW_DEP;
@@ -143,6 +156,7 @@ __device__ __forceinline__ float4 flowShfl(float4 v, uint32_t src) {
  *  anchors and C0 are fixed during the sweep and come through the caches (the key's stick bit,
  *  which the dual flips, is masked by the evaluation), its penalties and multipliers coherently.
  *  The body's own last update needs no wait: the same workgroup ran it, stores completed. */
+template <bool CACHED = false>
 __device__ __forceinline__ void flowAccumulate(const Dev &d, uint32_t i, uint32_t q0, uint32_t end, uint32_t lane, uint32_t lanes, float alpha,
                                                uint32_t iter, uint32_t polls, Acc &acc, float4 &pos, float4 &rot) {
   accZero(acc);
@@ -151,8 +165,8 @@ __device__ __forceinline__ void flowAccumulate(const Dev &d, uint32_t i, uint32_
   if (q < end) it = d.items[q];
   float dt = d.p.dt, dt2 = dt * dt;
   if (lane == 0) {
-    pos = Coherent::ld(d.b.pos[i]);
-    rot = Coherent::ld(d.b.rot[i]);
+    pos = CACHED ? d.b.pos[i] : Coherent::ld(d.b.pos[i]);
+    rot = CACHED ? d.b.rot[i] : Coherent::ld(d.b.rot[i]);
     float4 sz = d.b.size[i], mo = d.b.moment[i], inp = d.b.inpos[i], inr = d.b.inrot[i];
     float m = sz.w / dt2;
     V I = divs(xyz(mo), dt2);
@@ -176,10 +190,16 @@ __device__ __forceinline__ void flowAccumulate(const Dev &d, uint32_t i, uint32_
     if (it.w & FLOW_DEP) flowWait(d, it.z, iter + ((it.w & FLOW_LOWER) ? 1u : 0u), polls);
     flowOrder();
     if (it.w & 2u) { addJoint<Coherent>(d, acc, it.x, alpha, i); continue; }
-    k.pen = Coherent::ld(d.con[it.x].pen);
-    k.lam = Coherent::ld(d.con[it.x].lam);
+    if (CACHED) {
+      k.pen = d.con[it.x].pen;
+      k.lam = d.con[it.x].lam;
+    } else {
+      k.pen = Coherent::ld(d.con[it.x].pen);
+      k.lam = Coherent::ld(d.con[it.x].lam);
+    }
     bool isA = (it.w & 1u) != 0u;
-    PairBody other = flowPartner(d, it.z);
+    bool fresh = !CACHED || (it.w & (FLOW_DEP | FLOW_LOWER)) == (FLOW_DEP | FLOW_LOWER);
+    PairBody other = fresh ? flowPartner(d, it.z) : pairBody(d, it.z);
     PairBody A = self, B = other;
     if (!isA) { A = other; B = self; }
     V basis[3];
@@ -337,6 +357,45 @@ __global__ void __launch_bounds__(64) k_flow_tail(Dev d, uint32_t first, uint32_
     V basis[3];
     orthonormal(xyz(geo), basis);
     dualContact(d, c, basis, geo.w, A, B, d.p.alpha);
+  }
+}
+
+/** The leading colours [0, first) of one iteration in one launch, workgroup w taking chunk w in
+ *  (colour, chunk) order (one chunk a workgroup, the grid at least the chunks). A body waits only
+ *  for its partners ordered before it, whose chunks have lower indices: workgroups of a launch
+ *  start in index order on each XCD, so every chunk a waiting one needs has started and waits on
+ *  none after it, and the lowest unfinished chunk can always run. What it reads and nothing in the
+ *  launch writes comes through the caches: its own pose, its constraints' penalties and
+ *  multipliers (the duals run in the tail launch after this one), partners ordered after it (they
+ *  wait for it); the poses of partners ordered before it coherently, after their counts. Waits are
+ *  bounded as the tail's. */
+__global__ void __launch_bounds__(64) k_flow_lead(Dev d, uint32_t first, uint32_t iter, uint32_t polls) {
+  uint32_t colors = d.counters[C_NUM_COLORS];
+  if (colors > d.p.colorCap) colors = d.p.colorCap;
+  uint32_t last = first < colors ? first : colors, w = blockIdx.x, g = 0;
+  for (uint32_t c = 0; c < last; c++) {
+    uint32_t start = d.colorStart[c], count = d.colorStart[c + 1] - start;
+    uint32_t lanes = lanesFor(d.p, count), per = 64u / lanes;
+    uint32_t chunks = (count + per - 1u) / per;
+    if (w >= g + chunks) {
+      g += chunks;
+      continue;
+    }
+    uint32_t slot = threadIdx.x / lanes, lane = threadIdx.x % lanes;
+    uint32_t k = (w - g) * per + slot;
+    bool live = k < count;
+    uint4 item = live ? d.colorItems[start + k] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t i = item.x;
+    bool enabled = live && item.w != 0u;
+    Acc acc;
+    float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rot = pos;
+    if (live) flowAccumulate<true>(d, i, item.y, item.z, lane, lanes, d.p.alpha, iter, polls, acc, pos, rot);
+    else accZero(acc);
+    reduceLanes(acc, lanes);
+    if (enabled && lane == 0) finishBody<Coherent>(d, i, acc, pos, rot);
+    flowStoresDone();
+    if (live && lane == 0) __hip_atomic_store(&d.flowDone[i], iter + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
   }
 }
 
