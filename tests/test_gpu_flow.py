@@ -304,3 +304,39 @@ def test_tail_clashing_colours_and_adaptive_start(gpu, sched):
     assert a.schedule["cooperative"] == sched
     a.destroy()
     b.destroy()
+
+
+def test_lead_give_up_is_reported_and_the_solver_drops_to_per_colour_launches(gpu):
+    """Schedule 4 with a wait limit of one poll: a body of the leading launch (or the tail) gives
+    up, the launch still ends, the step is reported and the solver sweeps colour by colour from the
+    state that step left, exactly; schedule 4 is exact again once set."""
+    say("case: schedule 4 from colour 3, 400-body pile, wait poll limit 1 for one step")
+    bodies = scenes.random_pile(400, seed=6, bin_size=10.0)[0]
+    kw = dict(lanes=(1 << 20, 1 << 19, 1 << 18, 0), color_cap=24)
+    pk = dict(kw, gravity=-9.81)
+    pk["primal_lanes"] = pk.pop("lanes")
+    s = avbd_amd.GpuSolver3D(bodies, params=avbd_amd.default_params(**pk))
+    s.set_schedule(**LEAD)
+    s.set_sparse_tail(3)
+    s.steps(20)
+    assert s.read_counters()["overflow"] & avbd_amd.solver.OVERFLOW_BARRIER == 0
+    assert s.schedule["cooperative"] == 4
+    s.set_barrier_polls(1)
+    s.step()
+    c = s.read_counters()
+    assert c["overflow"] & avbd_amd.solver.OVERFLOW_BARRIER
+    assert s.schedule["cooperative"] == 0
+    s.set_barrier_polls(0)
+    m = oracle_from_gpu(s)
+    for _ in range(3):
+        s.step()
+        m.step(1)
+    same_bodies(s, m)
+    s.set_schedule(**LEAD)
+    for _ in range(3):
+        s.step()
+        m.step(1)
+    same_bodies(s, m)
+    assert s.schedule["cooperative"] == 4
+    assert s.read_counters()["overflow"] & avbd_amd.solver.OVERFLOW_BARRIER == 0
+    s.destroy()
